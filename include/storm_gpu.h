@@ -1,0 +1,182 @@
+/*
+ * storm_gpu.h -- C ABI of the MI355X (gfx950) Implicit Monte Carlo engine.
+ *
+ * Plain C: opaque handle, POD structs, pointers and sizes, int status codes
+ * (0 = success) with the message of the last failure on this thread in
+ * storm_gpu_last_error().  No torch, HIP or C++ types cross this boundary.
+ *
+ * The entry points replace the serial manager and its IMC physics plugin of
+ * the reference for one GPU:
+ *   MonteCarloManagerSerial<T,Grid>      manager/MonteCarloManagerSerial.hpp:25-117
+ *   RadiationIMC<...>                    radiation/RadiationIMC.hpp:332-556
+ *   CombPopulationControl                population/CombPopulationControl.hpp:20-32
+ *   SideTemperature / Vacuum / Rigid     boundary/SideTemperature.hpp:156-191,
+ *                                        boundary/Vacuum.hpp:10-45, boundary/RigidBoundary.hpp:9-24
+ * The C++ drop-in (include/storm_amd/MonteCarloManagerGPU.hpp) binds them.
+ */
+#ifndef STORM_GPU_H
+#define STORM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "storm_problem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STORM_GPU_ABI_VERSION 1
+
+typedef struct storm_gpu_engine storm_gpu_engine;
+
+/* Arithmetic of the transport. */
+enum {
+  STORM_GPU_MIXED = 0, /* FP32 cell-local tracking, own deterministic math, 92-bit fixed-point tallies */
+  STORM_GPU_FP64 = 1   /* plain FP64 tracking, device libm, FP64 atomic tallies                      */
+};
+
+/* Where the per-cell opacities, heat capacities and temperatures come from. */
+enum {
+  STORM_GPU_MATERIAL_DEVICE = 0, /* storm_material_desc evaluated on the GPU every step      */
+  STORM_GPU_MATERIAL_HOST = 1    /* host-tabulated plugins via storm_gpu_set_cell_state()   */
+};
+
+/* A census packet as the reference's Particle carries it
+ * (particle/Particle.hpp:67-85): global position, velocity = c * direction,
+ * cell, weight (erg), frequency. */
+typedef struct storm_packet_record {
+  double x, y, z;
+  double vx, vy, vz;
+  double weight;
+  double frequency;
+  uint64_t cell;
+  uint64_t id;
+} storm_packet_record;
+
+/* Counters of the last step (MonteCarloManagerSerial.hpp:74-83 getters and
+ * RadiationIMC counters, radiation/RadiationIMC.hpp:444-446). */
+typedef struct storm_gpu_counters {
+  uint64_t initial_particles;  /* census entering the step (GetInitialParticleCount)  */
+  uint64_t prestep_particles;  /* emission + boundary sources (GetPreStepParticleCount) */
+  uint64_t start_particles;    /* GetStartParticleCount                                 */
+  uint64_t end_particles;      /* census after population control (GetEndParticleCount) */
+  uint64_t emitted_particles;
+  uint64_t boundary_particles;
+  uint64_t census_before_comb;
+  uint64_t leaving_particles;  /* escapes through REMOVE faces                          */
+  uint64_t segments;           /* sum of GetCellsStepsCounters()                        */
+  uint64_t transport_passes;
+  uint64_t transport_launches;
+  uint64_t random_walk_steps;  /* getRandomWalkStepCount */
+  uint64_t ddmc_steps;         /* getDDMCStepCount       */
+  int32_t energy_quantum_exp;  /* Q = 2^eq erg of the last step */
+  int32_t reserved;
+  double seconds_step;         /* host wall time of the last storm_gpu_step          */
+  double seconds_transport;    /* GPU time of the transport kernels (HIP events)     */
+  double seconds_transport_max_launch; /* longest single transport launch              */
+} storm_gpu_counters;
+
+/* Exact energy ledger of the last step, in units of its quantum 2^eq erg
+ * (mixed engine; the FP64 engine reports the same terms as rounded doubles).
+ * Each term is hi * 2^64 + lo. */
+typedef struct storm_gpu_u128 {
+  uint64_t lo, hi;
+} storm_gpu_u128;
+
+typedef struct storm_gpu_ledger {
+  storm_gpu_u128 census_in;   /* census weight entering transport          */
+  storm_gpu_u128 emitted;     /* material emission packets                 */
+  storm_gpu_u128 source;      /* boundary source packets                   */
+  storm_gpu_u128 deposited;   /* absorbed in the material during transport */
+  storm_gpu_u128 escaped;     /* left through REMOVE faces                 */
+  storm_gpu_u128 census_pre;  /* census weight before population control   */
+  storm_gpu_u128 census_out;  /* census weight after population control    */
+  storm_gpu_u128 comb_rem;    /* comb rounding remainder (to the material) */
+  int32_t eq;                 /* quantum exponent                          */
+  int32_t eq_prev;
+  double requant_rem;         /* census requantisation remainder, erg      */
+  double clamp;               /* negative-energy guard, erg (RadiationIMC.hpp:771-817) */
+  double material_before, material_after; /* sum of cell internal energies, erg */
+} storm_gpu_ledger;
+
+const char *storm_gpu_last_error(void);
+int storm_gpu_abi_version(void);
+
+/* Device info string (name, CUs, arch) for logs. */
+int storm_gpu_device_info(char *buf, size_t len);
+
+/* Build the engine for `prob` (copied).  Replaces the manager constructor
+ * (MonteCarloManagerSerial.hpp:119-129) together with the RadiationIMC
+ * constructor (radiation/RadiationIMC.hpp:562-640). */
+int storm_gpu_create(const storm_problem *prob, int32_t precision, int32_t material_mode,
+                     storm_gpu_engine **out);
+
+/* ~MonteCarloManagerSerial (MonteCarloManagerSerial.hpp:185-194). */
+int storm_gpu_destroy(storm_gpu_engine *eng);
+
+/* One IMC step of length dt with the census kept in HBM:
+ * MonteCarloManagerSerial::step (MonteCarloManagerSerial.hpp:385-469) with
+ * RadiationIMC::preStep/step/postStep (radiation/RadiationIMC.hpp:1618-2029),
+ * the boundary sources and population control. */
+int storm_gpu_step(storm_gpu_engine *eng, double dt);
+
+/* SideTemperature::SetTemperature (boundary/SideTemperature.hpp:183) and the
+ * MarshakBoundary equivalent. */
+int storm_gpu_set_source_temperature(storm_gpu_engine *eng, double T);
+
+/* Host-tabulated material (STORM_GPU_MATERIAL_HOST): per-cell temperature,
+ * Planck/absorption opacity, scattering opacity, dT2cv and internal energy
+ * for the next step; replaces the per-cell virtual calls of preStep
+ * (radiation/RadiationIMC.hpp:1645-1648) and step (:1753).  Any pointer may
+ * be NULL to keep the current values. */
+int storm_gpu_set_cell_state(storm_gpu_engine *eng, const double *T, const double *sigma_a,
+                             const double *sigma_s, const double *cv,
+                             const double *internal_energy);
+
+/* Per-cell results after the last step (any pointer may be NULL):
+ * temperature (cells[i].temperature, RadiationIMC.hpp:848), internal energy
+ * (extensives[i].internal_energy), time-averaged radiation energy density
+ * (getEradTimeAvg, RadiationIMC.hpp:431, :1895), census radiation energy
+ * (extensives[i].Erad, :1997), Fleck factor (getFactorFleck, :429) and
+ * Planck opacity (getPlanckOpacities, :430). */
+int storm_gpu_get_cells(storm_gpu_engine *eng, double *T, double *internal_energy,
+                        double *erad_time_avg, double *census_erad, double *fleck,
+                        double *planck_opacity);
+
+/* GetCellsStepsCounters (MonteCarloManagerSerial.hpp:74-75). */
+int storm_gpu_get_segments(storm_gpu_engine *eng, uint64_t *per_cell);
+
+/* GetBeginningParticleCount (MonteCarloManagerSerial.hpp:81-82). */
+int storm_gpu_get_beginning_counts(storm_gpu_engine *eng, uint64_t *per_cell);
+
+/* Counters of the last step. */
+int storm_gpu_get_counters(storm_gpu_engine *eng, storm_gpu_counters *out);
+
+/* Exact energy ledger of the last step. */
+int storm_gpu_get_ledger(storm_gpu_engine *eng, storm_gpu_ledger *out);
+
+/* Census exchange with the host -- the std::vector<Particle> moved into and
+ * returned from MonteCarloManagerSerial::step (:44, :197-248, :464-468). */
+int storm_gpu_census_size(storm_gpu_engine *eng, uint64_t *n);
+int storm_gpu_export_census(storm_gpu_engine *eng, storm_packet_record *out, uint64_t capacity,
+                            uint64_t *n_written);
+int storm_gpu_import_census(storm_gpu_engine *eng, const storm_packet_record *in, uint64_t n);
+
+/* Launch parameters of the transport (block size, segments per pass). */
+int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segments_per_pass);
+
+/* Keep a copy of the next step's transport inputs (initial packets, cell
+ * opacities, step constants) and its integer tallies, for an exact CPU replay
+ * of the packet random streams.  Sizes first with NULL buffers. */
+int storm_gpu_debug_keep_transport_inputs(storm_gpu_engine *eng, int32_t enable);
+int storm_gpu_debug_transport_inputs(storm_gpu_engine *eng, uint64_t *n_packets, void *packets,
+                                     void *cell_xs, int64_t *consts4);
+int storm_gpu_debug_tallies(storm_gpu_engine *eng, uint64_t *dep, uint64_t *pint, uint64_t *esc,
+                            uint32_t *segs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STORM_GPU_H */

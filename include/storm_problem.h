@@ -1,0 +1,109 @@
+/*
+ * storm_problem.h -- plain-C description of one gray Implicit Monte Carlo
+ * problem on a Cartesian mesh: mesh, material models, boundary and IMC
+ * parameters.  Shared by the gfx950 engine (include/storm_gpu.h) and by the
+ * FP64 CPU oracle used as the correctness checker.
+ *
+ * Units are the reference's: CGS, temperature in K, opacities in cm^-1,
+ * energies in erg, times in s.
+ *
+ * What each block stands for in the reference:
+ *   storm_mesh_desc      MadCart::CartesianMesh3D(xEdges, yLo, yHi, ny, zLo, zHi, nz)
+ *                        as used by examples/marshak_wave/MarshakCommon.hpp:280-290;
+ *                        cell index (i*ny + j)*nz + k
+ *                        (examples/cartesian_parallel_check/main.cpp:54).
+ *   storm_material_desc  RadiationOpacityModel::CalcPlanckOpacity /
+ *                        CalcScatteringOpacity (radiation/RadiationOpacityModel.hpp:30-48)
+ *                        for the power-law family of
+ *                        examples/marshak_wave/MarshakOpacity.hpp, and the
+ *                        duck-typed EOS dT2cv / de2T (radiation/RadiationIMC.hpp:848,1648;
+ *                        examples/marshak_wave/MarshakCommon.hpp:128-172).
+ *   storm_boundary_desc  BoundaryCondition::apply per box face
+ *                        (boundary/SideTemperature.hpp:84-112, boundary/Vacuum.hpp:18-22,
+ *                        boundary/RigidBoundary.hpp:42-58) and the Planck surface
+ *                        source of SideTemperature::generateNewBoundaryParticles
+ *                        (boundary/SideTemperature.hpp:114-163).
+ *   storm_imc_params     RadiationIMCParameters (radiation/RadiationIMCParameters.hpp:12-56)
+ *                        plus CombPopulationControl(grid, Nmin, factor)
+ *                        (population/CombPopulationControl.hpp:75-77).
+ */
+#ifndef STORM_PROBLEM_H
+#define STORM_PROBLEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Box faces, in this order everywhere: x-, x+, y-, y+, z-, z+. */
+enum { STORM_XLO = 0, STORM_XHI = 1, STORM_YLO = 2, STORM_YHI = 3, STORM_ZLO = 4, STORM_ZHI = 5 };
+
+/* What happens to a packet that reaches a box face. */
+enum {
+  STORM_FACE_REFLECT = 0, /* specular reflection, the packet stays (REFLECT) */
+  STORM_FACE_ESCAPE = 1   /* the packet leaves the problem (REMOVE)        */
+};
+
+typedef struct storm_mesh_desc {
+  int32_t nx, ny, nz, reserved;
+  const double *x_edges; /* nx + 1 ascending */
+  const double *y_edges; /* ny + 1 ascending */
+  const double *z_edges; /* nz + 1 ascending */
+} storm_mesh_desc;
+
+/*
+ * sigma_a(c, T) = sigma_a0[c] * (max(T, T_floor) / T_ref)^(-a_temp_pow) * rho[c]^a_rho_pow
+ * sigma_s(c, T) = sigma_s0[c] * (max(T, T_floor) / T_ref)^(-s_temp_pow) * rho[c]^s_rho_pow
+ * EOS, energy per volume   e(T, rho) = eos_f * T^eos_beta * rho^(1 - eos_mu)
+ *      temperature         T(e, rho) = (max(e, 0) / (eos_f * rho^(1 - eos_mu)))^(1/eos_beta)
+ *      dT2cv (per mass)    eos_f * eos_beta * max(T, cv_T_floor)^(eos_beta - 1) * rho^(1 - eos_mu) / rho
+ * The Fleck factor uses dT2cv as returned, exactly as radiation/RadiationIMC.hpp:1648-1649 does.
+ */
+typedef struct storm_material_desc {
+  const double *sigma_a0; /* per cell, required */
+  const double *sigma_s0; /* per cell, NULL = no scattering */
+  const double *density;  /* per cell, NULL = 1 g/cm^3 */
+  double opacity_T_ref;
+  double opacity_T_floor;
+  double a_temp_pow, a_rho_pow;
+  double s_temp_pow, s_rho_pow;
+  double eos_f, eos_beta, eos_mu, eos_cv_T_floor;
+} storm_material_desc;
+
+typedef struct storm_boundary_desc {
+  int32_t face_kind[6];             /* STORM_FACE_* per box face */
+  int32_t source_face;              /* -1: none; STORM_XLO: Planck source entering along +x */
+  int32_t freeze_source_T4;         /* 1: T^4 taken once at the first step (SideTemperature.hpp:252) */
+  double source_temperature;        /* K */
+  uint64_t source_packets_per_face; /* Npercell of SideTemperature */
+} storm_boundary_desc;
+
+typedef struct storm_imc_params {
+  uint64_t new_photons_per_cell;      /* RadiationIMCParameters::newPhotonsPerCell */
+  int32_t with_random_walk;           /* withRandomWalk */
+  int32_t with_ddmc;                  /* withDDMC */
+  double rw_min_cell_optical_depth;   /* default 25 */
+  double rw_min_particle_optical_depth; /* default 5 */
+  double ddmc_min_cell_optical_depth; /* default 15 */
+  double ddmc_min_particle_optical_depth; /* default 5 */
+  int32_t comb;                       /* 0: NoPopulationControl, 1: CombPopulationControl */
+  int32_t reserved;
+  uint64_t comb_nmin;                 /* Nmin (default 20) */
+  double comb_factor;                 /* totalParticlesFactor (default 2.0) */
+} storm_imc_params;
+
+typedef struct storm_problem {
+  storm_mesh_desc mesh;
+  storm_material_desc material;
+  storm_boundary_desc boundary;
+  storm_imc_params imc;
+  const double *T0; /* initial material temperature per cell, K */
+  uint64_t seed;
+} storm_problem;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STORM_PROBLEM_H */

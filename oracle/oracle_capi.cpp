@@ -1,0 +1,226 @@
+// C entry points of the FP64 CPU oracle, for ctypes (tests, smoke, bench's
+// cpu_baseline).  TEST INFRASTRUCTURE ONLY -- never linked into the product.
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "oracle.hpp"
+
+using namespace storm_oracle;
+
+namespace {
+thread_local std::string g_err;
+
+CartesianGrid makeGrid(const storm_mesh_desc &m) {
+  std::vector<double> xe(m.x_edges, m.x_edges + m.nx + 1);
+  std::vector<double> ye(m.y_edges, m.y_edges + m.ny + 1);
+  std::vector<double> ze(m.z_edges, m.z_edges + m.nz + 1);
+  return CartesianGrid(std::move(xe), std::move(ye), std::move(ze));
+}
+
+Material makeMaterial(const storm_material_desc &d, size_t N) {
+  Material m;
+  m.sa0.assign(d.sigma_a0, d.sigma_a0 + N);
+  if (d.sigma_s0) m.ss0.assign(d.sigma_s0, d.sigma_s0 + N);
+  if (d.density) m.rho.assign(d.density, d.density + N);
+  m.Tref = d.opacity_T_ref;
+  m.Tfloor = d.opacity_T_floor;
+  m.apow = d.a_temp_pow;
+  m.arho = d.a_rho_pow;
+  m.spow = d.s_temp_pow;
+  m.srho = d.s_rho_pow;
+  m.eos_f = d.eos_f;
+  m.eos_beta = d.eos_beta;
+  m.eos_mu = d.eos_mu;
+  m.cvTfloor = d.eos_cv_T_floor;
+  return m;
+}
+}  // namespace
+
+extern "C" {
+
+// Per-step ledger layout (doubles): emitted, source, escaped, clamp,
+// censusIn, censusOut, censusComb, materialBefore, materialAfter, segments,
+// startParticles, endParticles, seconds
+#define ORACLE_LEDGER_STRIDE 13
+
+const char *oracle_last_error(void) { return g_err.c_str(); }
+
+// Run `nsteps` IMC steps of `prob` with time step `dt`.  `source_T` (nullable)
+// overrides the boundary temperature per step.  Outputs are per cell after the
+// last step (any may be NULL): T, internal energy, Erad time average,
+// census Erad (extensive), segment counts of the last step (as double), and a
+// per-step ledger (nsteps * ORACLE_LEDGER_STRIDE).
+int oracle_run(const storm_problem *prob, int nsteps, double dt, const double *source_T,
+               double *T_out, double *e_out, double *erad_out, double *census_out,
+               double *segments_out, double *ledger_out) {
+  try {
+    CartesianGrid grid = makeGrid(prob->mesh);
+    size_t N = grid.GetPointNo();
+    Material mat = makeMaterial(prob->material, N);
+    std::vector<Cell> cells(N);
+    std::vector<Extensives> ext(N);
+    for (size_t i = 0; i < N; ++i) {
+      double rho = mat.density(i), V = grid.GetVolume(i);
+      cells[i].temperature = prob->T0[i];
+      cells[i].internalEnergy = mat.eFromT(prob->T0[i], rho) * V;
+      ext[i].mass = rho * V;
+      ext[i].internal_energy = cells[i].internalEnergy;
+    }
+    storm_boundary_desc bd = prob->boundary;
+    FaceTableBoundary bc(grid, bd, prob->seed);
+    IMCParams ip;
+    ip.newPhotonsPerCell = prob->imc.new_photons_per_cell;
+    ip.withRandomWalk = prob->imc.with_random_walk != 0;
+    ip.withDDMC = prob->imc.with_ddmc != 0;
+    ip.rwMinCellOpticalDepth = prob->imc.rw_min_cell_optical_depth;
+    ip.rwMinParticleOpticalDepth = prob->imc.rw_min_particle_optical_depth;
+    ip.ddmcMinCellOpticalDepth = prob->imc.ddmc_min_cell_optical_depth;
+    ip.ddmcMinParticleOpticalDepth = prob->imc.ddmc_min_particle_optical_depth;
+    RadiationIMC imc(grid, &bc, cells, ext, mat, ip, prob->seed);
+    std::unique_ptr<Comb> comb;
+    if (prob->imc.comb) comb = std::make_unique<Comb>(grid, prob->imc.comb_nmin, prob->imc.comb_factor, prob->seed);
+    Manager mgr(grid, imc, comb.get(), bc);
+    std::vector<Particle> census;
+    for (int s = 0; s < nsteps; ++s) {
+      auto t0 = std::chrono::steady_clock::now();
+      double censusIn = 0, matBefore = 0;
+      for (const Particle &p : census) censusIn += p.weight;
+      for (size_t i = 0; i < N; ++i) matBefore += ext[i].internal_energy;
+      imc.ledger = EnergyLedger{};
+      bc.escapedEnergy = 0;
+      bc.sourceEnergy = 0;
+      if (source_T) bc.setSourceTemperature(source_T[s]);
+      census = mgr.step(std::move(census), dt);
+      auto t1 = std::chrono::steady_clock::now();
+      double censusOut = 0, censusComb = 0, matAfter = 0, segs = 0;
+      for (const Particle &p : mgr.lastPreCombCensus()) censusOut += p.weight;
+      for (const Particle &p : census) censusComb += p.weight;
+      for (size_t i = 0; i < N; ++i) matAfter += ext[i].internal_energy;
+      for (size_t i = 0; i < N; ++i) segs += static_cast<double>(mgr.cellsStepsCounters[i]);
+      if (ledger_out) {
+        double *L = ledger_out + static_cast<size_t>(s) * ORACLE_LEDGER_STRIDE;
+        L[0] = imc.ledger.emitted;
+        L[1] = bc.sourceEnergy;
+        L[2] = bc.escapedEnergy;
+        L[3] = imc.ledger.clamp;
+        L[4] = censusIn;
+        L[5] = censusOut;
+        L[6] = censusComb;
+        L[7] = matBefore;
+        L[8] = matAfter;
+        L[9] = segs;
+        L[10] = static_cast<double>(mgr.startParticleCount);
+        L[11] = static_cast<double>(mgr.endParticleCount);
+        L[12] = std::chrono::duration<double>(t1 - t0).count();
+      }
+    }
+    for (size_t i = 0; i < N; ++i) {
+      if (T_out) T_out[i] = cells[i].temperature;
+      if (e_out) e_out[i] = ext[i].internal_energy;
+      if (erad_out) erad_out[i] = imc.eradTimeAvg()[i];
+      if (census_out) census_out[i] = ext[i].Erad;
+      if (segments_out) segments_out[i] = static_cast<double>(mgr.cellsStepsCounters[i]);
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Per-cell preStep quantities for one step from T0 (no transport): Planck
+// opacity, Fleck factor, emission energy E_i, photons n_i.
+int oracle_prestep(const storm_problem *prob, double dt, double *sigma_out, double *fleck_out,
+                   double *emit_out, double *nphot_out) {
+  try {
+    CartesianGrid grid = makeGrid(prob->mesh);
+    size_t N = grid.GetPointNo();
+    Material mat = makeMaterial(prob->material, N);
+    std::vector<Cell> cells(N);
+    std::vector<Extensives> ext(N);
+    for (size_t i = 0; i < N; ++i) {
+      double rho = mat.density(i), V = grid.GetVolume(i);
+      cells[i].temperature = prob->T0[i];
+      cells[i].internalEnergy = mat.eFromT(prob->T0[i], rho) * V;
+      ext[i].mass = rho * V;
+      ext[i].internal_energy = cells[i].internalEnergy;
+    }
+    IMCParams ip;
+    ip.newPhotonsPerCell = prob->imc.new_photons_per_cell;
+    RadiationIMC imc(grid, nullptr, cells, ext, mat, ip, prob->seed);
+    imc.preStep(dt);
+    for (size_t i = 0; i < N; ++i) {
+      if (sigma_out) sigma_out[i] = imc.planckOpacities()[i];
+      if (fleck_out) fleck_out[i] = imc.factorFleck()[i];
+      if (emit_out) emit_out[i] = imc.emissionEnergy()[i];
+      if (nphot_out) nphot_out[i] = static_cast<double>(imc.photonsPerCell()[i]);
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Random-walk tables: tau[1024], survival[1024], radius[1024*256].
+int oracle_rw_tables(double *tau, double *survival, double *radius) {
+  try {
+    RandomWalk rw;
+    std::memcpy(tau, rw.tauTable().data(), rw.tauTable().size() * sizeof(double));
+    std::memcpy(survival, rw.survivalTable().data(), rw.survivalTable().size() * sizeof(double));
+    std::memcpy(radius, rw.radiusTable().data(), rw.radiusTable().size() * sizeof(double));
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// RandomWalk::sampleLeakTime / sampleRadius for a batch of inputs.
+int oracle_rw_sample(int n, const double *xi, const double *tau, double *leak_out, double *radius_out) {
+  try {
+    static RandomWalk rw;
+    for (int i = 0; i < n; ++i) {
+      if (leak_out) leak_out[i] = rw.sampleLeakTime(xi[i]);
+      if (radius_out) radius_out[i] = rw.sampleRadius(tau[i], xi[i]);
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Comb on a synthetic census in one cell-indexed grid: inputs are per-packet
+// cell and weight; outputs per-cell output count and output weight sum.
+int oracle_comb(const storm_problem *prob, int64_t n, const int64_t *cell, const double *weight,
+                double *count_out, double *wsum_out) {
+  try {
+    CartesianGrid grid = makeGrid(prob->mesh);
+    size_t N = grid.GetPointNo();
+    std::vector<Particle> ps(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) {
+      ps[i].id = static_cast<size_t>(i);
+      ps[i].cellIndex = static_cast<size_t>(cell[i]);
+      ps[i].location = grid.GetCellCM(ps[i].cellIndex);
+      ps[i].weight = weight[i];
+    }
+    Comb comb(grid, prob->imc.comb_nmin, prob->imc.comb_factor, prob->seed);
+    std::vector<Particle> out = comb.activate(ps);
+    for (size_t i = 0; i < N; ++i) {
+      count_out[i] = 0;
+      wsum_out[i] = 0;
+    }
+    for (const Particle &p : out) {
+      count_out[p.cellIndex] += 1;
+      wsum_out[p.cellIndex] += p.weight;
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,265 @@
+// CPU replay of the mixed-precision engine's transport and sourcing.
+//
+// TEST INFRASTRUCTURE ONLY.  It compiles the engine's shared host/device
+// headers (storm_amd/csrc/common/*.h) with g++ -ffp-contract=off and replays
+// packets from their Philox streams, so the tests can check that
+//   (1) the gfx950 kernels reproduce the integer tallies bit-exactly
+//       (replay_transport_f32 on the packets the GPU step started from), and
+//   (2) the engine's first-step physics agrees statistically with the FP64
+//       oracle before any GPU is involved (cpu_first_step).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../include/storm_problem.h"
+#include "../storm_amd/csrc/common/fixed.h"
+#include "../storm_amd/csrc/common/sourcing.h"
+#include "../storm_amd/csrc/common/transport.h"
+
+using namespace storm;
+
+namespace {
+thread_local std::string g_err;
+
+struct HostSink {
+  uint64_t *dep, *pint, *esc;
+  double *dep_fp, *pint_fp, *esc_fp;
+  uint32_t *segs;
+  int32_t eq;
+  void visit(uint32_t c, fx::Limbs d, fx::Limbs e, uint32_t s) {
+    dep[2 * c] += d.lo;
+    dep[2 * c + 1] += d.hi;
+    pint[2 * c] += e.lo;
+    pint[2 * c + 1] += e.hi;
+    segs[c] += s;
+  }
+  void visit_fp(uint32_t c, double d, double e, uint32_t s) {
+    dep_fp[c] += d;
+    pint_fp[c] += e;
+    segs[c] += s;
+  }
+  void escape(uint32_t c, double w) {
+    if (esc_fp) {
+      esc_fp[c] += w;
+    } else {
+      fx::Limbs l = fx::limbs_of(w, eq);
+      esc[2 * c] += l.lo;
+      esc[2 * c + 1] += l.hi;
+    }
+  }
+};
+
+template <typename Real>
+int run_all(std::vector<Packet<Real>> &pk, const MeshView<Real> &mv, const CellXS<Real> *xs, const StepConsts &kc,
+            int K, HostSink &sink, int64_t *outcomes) {
+  for (auto &p : pk) {
+    int o = kAlive;
+    while (o == kAlive) o = transport_packet<Real>(p, mv, xs, kc, K, sink);
+    outcomes[o] += 1;
+  }
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+const char *replay_last_error(void) { return g_err.c_str(); }
+
+// Philox4x32-10 block (for known-answer tests).
+void replay_philox(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4) {
+  Philox4 r = philox4x32_10(ctr4[0], ctr4[1], ctr4[2], ctr4[3], key2[0], key2[1]);
+  for (int i = 0; i < 4; ++i) out4[i] = r.v[i];
+}
+
+// The engine's FP32 elementary functions on a batch (for accuracy tests):
+// which 0 log, 1 exp, 2 expm1, 3 sin(2 pi x), 4 cos(2 pi x).
+void replay_fpmath(int32_t which, int64_t n, const float *x, float *y) {
+  for (int64_t i = 0; i < n; ++i) {
+    float s, c;
+    switch (which) {
+      case 0: y[i] = fpm::logf_d(x[i]); break;
+      case 1: y[i] = fpm::expf_d(x[i]); break;
+      case 2: y[i] = fpm::expm1f_d(x[i]); break;
+      case 3: fpm::sincos2pi(x[i], s, c); y[i] = s; break;
+      default: fpm::sincos2pi(x[i], s, c); y[i] = c; break;
+    }
+  }
+}
+
+// Exact replay of `n` packets (storm::Packet<float>) through the transport.
+// widths: nx + ny + nz floats; xs: 2 floats per cell; consts: seed, step, eq, et.
+// Tallies are accumulated into the given arrays (dep/pint/esc: 2 per cell, segs: 1 per cell).
+int replay_transport_f32(int64_t n, const void *packets, int32_t nx, int32_t ny, int32_t nz, const float *widths,
+                         const int32_t *face_kind, const float *xs, const int64_t *consts, int32_t K, uint64_t *dep,
+                         uint64_t *pint, uint64_t *esc, uint32_t *segs, int64_t *outcomes) {
+  MeshView<float> mv;
+  mv.nx = nx;
+  mv.ny = ny;
+  mv.nz = nz;
+  for (int f = 0; f < 6; ++f) mv.face_kind[f] = face_kind[f];
+  mv.width[0] = widths;
+  mv.width[1] = widths + nx;
+  mv.width[2] = widths + nx + ny;
+  StepConsts kc;
+  kc.seed = static_cast<uint64_t>(consts[0]);
+  kc.step = static_cast<uint32_t>(consts[1]);
+  kc.eq = static_cast<int32_t>(consts[2]);
+  kc.et = static_cast<int32_t>(consts[3]);
+  std::vector<Packet<float>> pk(static_cast<size_t>(n));
+  std::memcpy(pk.data(), packets, static_cast<size_t>(n) * sizeof(Packet<float>));
+  HostSink sink{dep, pint, esc, nullptr, nullptr, nullptr, segs, kc.eq};
+  return run_all<float>(pk, mv, reinterpret_cast<const CellXS<float> *>(xs), kc, K, sink, outcomes);
+}
+
+// First IMC step of the engine on the CPU (no census): preStep, photon
+// allocation, emission + boundary births and transport, in the engine's
+// arithmetic (precision 0: float + fixed point, 1: double).  Outputs per cell:
+// deposited energy (erg), Erad time average (erg/cm^3), segments; and the
+// escaped energy, census energy and packet count.
+int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_t step, double *dep_out,
+                   double *erad_out, double *segs_out, double *totals_out) {
+  try {
+    const int nx = p->mesh.nx, ny = p->mesh.ny, nz = p->mesh.nz;
+    const uint32_t N = static_cast<uint32_t>(nx) * ny * nz;
+    std::vector<double> V(N), E(N), f(N), sa(N), ss(N);
+    auto widthd = [&](const double *e, int i) { return e[i + 1] - e[i]; };
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k)
+          V[(static_cast<size_t>(i) * ny + j) * nz + k] =
+              widthd(p->mesh.x_edges, i) * widthd(p->mesh.y_edges, j) * widthd(p->mesh.z_edges, k);
+    const storm_material_desc &m = p->material;
+    double Etot = 0.0;
+    for (uint32_t c = 0; c < N; ++c) {
+      double T = p->T0[c];
+      double rho = m.density ? m.density[c] : 1.0;
+      auto tfac = [&](double pw) { return pw == 0.0 ? 1.0 : std::pow(std::fmax(T, m.opacity_T_floor) / m.opacity_T_ref, -pw); };
+      auto rfac = [&](double pw) { return (pw == 0.0 || !m.density) ? 1.0 : std::pow(rho, pw); };
+      sa[c] = m.sigma_a0[c] * tfac(m.a_temp_pow) * rfac(m.a_rho_pow);
+      ss[c] = m.sigma_s0 ? m.sigma_s0[c] * tfac(m.s_temp_pow) * rfac(m.s_rho_pow) : 0.0;
+      double cv = m.eos_f * m.eos_beta * std::pow(std::fmax(T, m.eos_cv_T_floor), m.eos_beta - 1.0) *
+                  std::pow(rho, 1.0 - m.eos_mu) / rho;
+      double T3 = T * T * T;
+      f[c] = 1.0 / (1.0 + (4.0 * units::arad * T3 * sa[c] * units::clight * dt) / cv);
+      E[c] = f[c] * V[c] * units::arad * (T3 * T) * sa[c] * dt * units::clight;
+      Etot += E[c];
+    }
+    const uint32_t npc = p->boundary.source_face == STORM_XLO ? static_cast<uint32_t>(p->boundary.source_packets_per_face) : 0u;
+    double src = 0.0;
+    const double Tb = p->boundary.source_temperature;
+    const double T4 = Tb * Tb * Tb * Tb;
+    std::vector<double> wface(npc ? static_cast<size_t>(ny) * nz : 0);
+    for (size_t fi = 0; fi < wface.size(); ++fi) {
+      int j = static_cast<int>(fi / nz), k = static_cast<int>(fi % nz);
+      wface[fi] = units::sigma_sb * T4 * widthd(p->mesh.y_edges, j) * widthd(p->mesh.z_edges, k) * dt / npc;
+      src += wface[fi] * npc;
+    }
+    const double Ebound = Etot + src;
+    const int32_t eq = fx::ceil_log2(Ebound) - fx::kHeadroomBits;
+    const int32_t et = fx::ceil_log2(Ebound * units::clight * dt) - fx::kHeadroomBits;
+    for (auto &w : wface) w = fx::quantize_down(w, eq);
+    // allocation
+    const uint64_t nmin = p->imc.new_photons_per_cell;
+    const uint64_t total = static_cast<uint64_t>(N) * nmin * 10ull;
+    std::vector<uint32_t> nph(N);
+    std::vector<double> wem(N);
+    for (uint32_t c = 0; c < N; ++c) {
+      uint64_t share = Etot > 0.0 ? static_cast<uint64_t>(E[c] / Etot * static_cast<double>(total)) : nmin;
+      uint64_t n = 0;
+      if (E[c] > 0.0) {
+        n = share < 20ull * nmin ? share : 20ull * nmin;
+        if (n < nmin) n = nmin;
+      }
+      double w = n ? fx::quantize_down(E[c] / static_cast<double>(n), eq) : 0.0;
+      if (w == 0.0) n = 0;
+      nph[c] = static_cast<uint32_t>(n);
+      wem[c] = w;
+    }
+    std::vector<uint64_t> dep(2 * N, 0), pint(2 * N, 0), esc(2 * N, 0);
+    std::vector<double> depf(N, 0), pintf(N, 0), escf(N, 0);
+    std::vector<uint32_t> segs(N, 0);
+    int64_t outc[4] = {0, 0, 0, 0};
+    double census = 0.0;
+    StepConsts kc{p->seed, step, eq, et};
+    auto go = [&](auto tag) {
+      using Real = decltype(tag);
+      std::vector<Real> w0(nx), w1(ny), w2(nz);
+      for (int i = 0; i < nx; ++i) w0[i] = static_cast<Real>(widthd(p->mesh.x_edges, i));
+      for (int i = 0; i < ny; ++i) w1[i] = static_cast<Real>(widthd(p->mesh.y_edges, i));
+      for (int i = 0; i < nz; ++i) w2[i] = static_cast<Real>(widthd(p->mesh.z_edges, i));
+      MeshView<Real> mv;
+      mv.nx = nx;
+      mv.ny = ny;
+      mv.nz = nz;
+      for (int q = 0; q < 6; ++q) mv.face_kind[q] = p->boundary.face_kind[q];
+      mv.width[0] = w0.data();
+      mv.width[1] = w1.data();
+      mv.width[2] = w2.data();
+      std::vector<CellXS<Real>> xs(N);
+      for (uint32_t c = 0; c < N; ++c) {
+        xs[c].sig_abs = static_cast<Real>(f[c] * sa[c]);
+        xs[c].sig_ev = static_cast<Real>(ss[c] + (1.0 - f[c]) * sa[c]);
+      }
+      const Real s0 = static_cast<Real>(units::clight * dt);
+      HostSink sink{dep.data(), pint.data(), esc.data(), nullptr, nullptr, nullptr, segs.data(), eq};
+      if (sizeof(Real) == 8) {
+        sink.dep_fp = depf.data();
+        sink.pint_fp = pintf.data();
+        sink.esc_fp = escf.data();
+      }
+      uint32_t id = 0;
+      std::vector<Packet<Real>> pk;
+      for (uint32_t c = 0; c < N; ++c) {
+        uint32_t k = c % nz, j = (c / nz) % ny, i = c / (static_cast<uint32_t>(ny) * nz);
+        for (uint32_t q = 0; q < nph[c]; ++q) {
+          Packet<Real> pp;
+          birth_emission<Real>(pp, id++, pack_ijk(i, j, k), wem[c], s0, p->seed, step);
+          pk.push_back(pp);
+        }
+      }
+      for (size_t fi = 0; fi < wface.size(); ++fi) {
+        for (uint32_t q = 0; q < npc; ++q) {
+          Packet<Real> pp;
+          birth_boundary<Real>(pp, id++, static_cast<uint32_t>(fi / nz), static_cast<uint32_t>(fi % nz), wface[fi], s0,
+                               p->seed, step);
+          pk.push_back(pp);
+        }
+      }
+      for (auto &pp : pk) {
+        int o = kAlive;
+        while (o == kAlive) o = transport_packet<Real>(pp, mv, xs.data(), kc, 1 << 20, sink);
+        outc[o] += 1;
+        if (o == kCensus) census += pp.w;
+      }
+    };
+    if (precision == 0)
+      go(float{});
+    else
+      go(double{});
+    double escaped = 0.0;
+    for (uint32_t c = 0; c < N; ++c) {
+      double d = precision == 0 ? fx::to_double(dep[2 * c], dep[2 * c + 1], eq) : depf[c];
+      double e = precision == 0 ? fx::to_double(pint[2 * c], pint[2 * c + 1], et) : pintf[c];
+      escaped += precision == 0 ? fx::to_double(esc[2 * c], esc[2 * c + 1], eq) : escf[c];
+      dep_out[c] = d;
+      erad_out[c] = e / (units::clight * dt * V[c]);
+      segs_out[c] = segs[c];
+    }
+    double emitted = 0.0;
+    for (uint32_t c = 0; c < N; ++c) emitted += wem[c] * nph[c];
+    totals_out[0] = escaped;
+    totals_out[1] = census;
+    totals_out[2] = static_cast<double>(outc[0] + outc[1] + outc[2] + outc[3]);
+    totals_out[3] = emitted;
+    totals_out[4] = src;
+    totals_out[5] = static_cast<double>(outc[kCensus]);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Exact fixed-point bookkeeping for packet weights and cell tallies.
+//
+// A step has an energy quantum Q = 2^eq erg chosen so that every energy the
+// step can move is below 2^90 Q.  Packet weights are FP64 values that are
+// kept integer multiples of Q at every point where they are tallied, so the
+// conversion weight -> integer is exact.  An integer below 2^92 is held as
+// two unsigned limbs, value = hi * 2^36 + lo with lo < 2^36; tallies are
+// per-cell pairs of u64 accumulators summed limb by limb with integer
+// atomics (no carries), which is exact and independent of the order of the
+// additions as long as a cell receives fewer than 2^28 additions per step.
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+#include "fpmath.h"
+#include "hd.h"
+
+namespace storm {
+namespace fx {
+
+constexpr int kLoBits = 36;
+constexpr uint64_t kLoMask = (1ull << kLoBits) - 1ull;
+constexpr int kHeadroomBits = 90;  // energies bounded by 2^90 quanta
+
+struct Limbs {
+  uint64_t lo, hi;
+};
+
+// Round a non-negative double down to a multiple of 2^e (exact).
+STORM_HD double quantize_down(double w, int e) {
+  return floor(w * fpm::pow2d(-e)) * fpm::pow2d(e);
+}
+// Round to the nearest multiple of 2^e (exact; ties to even).
+STORM_HD double quantize_nearest(double w, int e) {
+  return rint(w * fpm::pow2d(-e)) * fpm::pow2d(e);
+}
+
+// Limbs of w / 2^e for a w that is an integer multiple of 2^e below 2^92.
+STORM_HD Limbs limbs_of(double w, int e) {
+  double y = w * fpm::pow2d(-e);            // exact integer
+  double h = floor(y * fpm::pow2d(-kLoBits));
+  double l = y - h * fpm::pow2d(kLoBits);   // exact
+  Limbs r;
+  r.hi = static_cast<uint64_t>(h);
+  r.lo = static_cast<uint64_t>(l);
+  return r;
+}
+
+// a - b for a >= b, both normalised (lo < 2^36); result normalised.
+STORM_HD Limbs sub(Limbs a, Limbs b) {
+  Limbs r;
+  if (a.lo >= b.lo) {
+    r.lo = a.lo - b.lo;
+    r.hi = a.hi - b.hi;
+  } else {
+    r.lo = a.lo + (1ull << kLoBits) - b.lo;
+    r.hi = a.hi - b.hi - 1ull;
+  }
+  return r;
+}
+
+// Value of an accumulator pair (hi, lo sums) as a double times 2^e.
+STORM_HD double to_double(uint64_t lo_sum, uint64_t hi_sum, int e) {
+  // hi_sum * 2^36 + lo_sum, rounded once per term; the same expression is
+  // evaluated on the host and on the device.
+  double h = static_cast<double>(hi_sum) * fpm::pow2d(kLoBits);
+  double l = static_cast<double>(lo_sum);
+  return (h + l) * fpm::pow2d(e);
+}
+
+// Exponent of the smallest power of two >= x (x > 0, normal).
+STORM_HD int ceil_log2(double x) {
+  uint64_t b = fpm::d2u(x);
+  int e = static_cast<int>((b >> 52) & 0x7ffull) - 1023;
+  if ((b & ((1ull << 52) - 1ull)) != 0ull) e += 1;
+  return e;
+}
+
+}  // namespace fx
+}  // namespace storm

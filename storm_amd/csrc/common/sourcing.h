@@ -1,0 +1,69 @@
+// Packet birth for the two sources of a gray IMC step, shared by the gfx950
+// sourcing kernels and the CPU replay.
+//
+//   emission : RadiationIMC::generateParticles / generateSingleParticle
+//              (radiation/RadiationIMC.hpp:2405-2553): a uniform point in the
+//              cell (RandomInCellPositionSampler is uniform in a box,
+//              utils/RandomInCell.hpp:13-66), isotropic direction
+//              (RadiationOpacityModel.hpp:139-150), timeLeft = dt * U, weight
+//              E_i / n_i, initialWeight = weight.
+//   boundary : SideTemperature::generateNewBoundaryParticles
+//              (boundary/SideTemperature.hpp:114-163): a uniform point on the
+//              x- box face of the cell, mu = sqrt(U) about +x, phi = 2 pi U,
+//              timeLeft = dt * U, weight sigma_SB T^4 A dt / Npercell.
+// Each packet draws from its own stream keyed by (seed, id) with the birth tag.
+#pragma once
+
+#include <stdint.h>
+#include "hd.h"
+#include "philox.h"
+#include "transport.h"
+
+namespace storm {
+
+template <typename Real>
+STORM_HD void birth_emission(Packet<Real> &p, uint32_t id, uint32_t ijk, double w, Real s0, uint64_t seed,
+                             uint32_t step) {
+  using M = MathPolicy<Real>;
+  PacketStream rs = PacketStream::make(seed, id, step, kTagBirth);
+  Philox4 r0 = rs.block(0), r1 = rs.block(1);
+  p.u[0] = M::uniform(r0.v[0]);
+  p.u[1] = M::uniform(r0.v[1]);
+  p.u[2] = M::uniform(r0.v[2]);
+  p.s = s0 * M::uniform(r0.v[3]);
+  isotropic_dir<Real>(M::uniform(r1.v[0]), M::uniform(r1.v[1]), p.o);
+  p.w = w;
+  p.wmin = static_cast<Real>(w * 1e-3);
+  p.ijk = ijk;
+  p.id = id;
+  p.ctr = 0u;
+  p.flags = 0u;
+}
+
+template <typename Real>
+STORM_HD void birth_boundary(Packet<Real> &p, uint32_t id, uint32_t j, uint32_t k, double w, Real s0,
+                             uint64_t seed, uint32_t step) {
+  using M = MathPolicy<Real>;
+  PacketStream rs = PacketStream::make(seed, id, step, kTagBirth);
+  Philox4 r0 = rs.block(0), r1 = rs.block(1);
+  p.u[0] = Real(0);
+  p.u[1] = M::uniform(r0.v[0]);
+  p.u[2] = M::uniform(r0.v[1]);
+  Real mu = M::sqrt(M::uniform(r0.v[2]));
+  Real st2 = Real(1) - mu * mu;
+  Real st = M::sqrt(st2 > Real(0) ? st2 : Real(0));
+  Real sp, cp;
+  M::sincos2pi(M::uniform(r0.v[3]), sp, cp);
+  p.o[0] = mu;
+  p.o[1] = st * cp;
+  p.o[2] = st * sp;
+  p.s = s0 * M::uniform(r1.v[0]);
+  p.w = w;
+  p.wmin = static_cast<Real>(w * 1e-3);
+  p.ijk = pack_ijk(0u, j, k);
+  p.id = id;
+  p.ctr = 0u;
+  p.flags = 0u;
+}
+
+}  // namespace storm

@@ -1,0 +1,62 @@
+// Wavefront-level helpers for gfx950 (64 lanes): aggregated queue appends
+// and deterministic reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace storm {
+namespace gpu {
+
+// Rank of this lane among the lanes whose bit is set in `mask` below it.
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// Append: every active lane calls this; lanes with pred get consecutive slots
+// from `counter` with one atomic per wavefront.  Returns the slot or ~0u.
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *counter) {
+  const uint64_t exec = __builtin_amdgcn_ballot_w64(true);
+  const uint64_t want = __builtin_amdgcn_ballot_w64(pred);
+  const uint32_t total = static_cast<uint32_t>(__popcll(want));
+  uint32_t base = 0u;
+  if (lane_rank(exec) == 0u && total != 0u) base = atomicAdd(counter, total);
+  base = __builtin_amdgcn_readfirstlane(base);
+  return pred ? base + lane_rank(want) : ~0u;
+}
+
+// Integer sums are order independent: per-block LDS reduction, one atomic.
+template <int BLOCK>
+__device__ __forceinline__ void block_atomic_add_u64(uint64_t v, unsigned long long *dst) {
+  __shared__ uint64_t sh[BLOCK];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && sh[0] != 0ull) atomicAdd(dst, static_cast<unsigned long long>(sh[0]));
+  __syncthreads();
+}
+
+// Deterministic double sum: fixed grid, fixed per-thread stride order and a
+// fixed LDS tree; a second single-block launch adds the block partials.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sum_partials(const double *__restrict__ in, uint64_t n,
+                                                        double *__restrict__ partials) {
+  __shared__ double sh[BLOCK];
+  double acc = 0.0;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += stride) acc += in[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+}  // namespace gpu
+}  // namespace storm

@@ -1,0 +1,991 @@
+// Host orchestration of one IMC step on one MI355X and the C ABI
+// (include/storm_gpu.h).  One engine = one HIP stream; not re-entrant,
+// like the reference's serial manager (MonteCarloManagerSerial.hpp:269-270
+// keeps function-static buffers).
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+#include <rocprim/rocprim.hpp>
+
+#include "../../include/storm_gpu.h"
+#include "kernels.h"
+
+namespace storm {
+namespace gpu {
+
+thread_local std::string g_last_error;
+
+struct EngineError : std::runtime_error {
+  int code;
+  EngineError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                                            \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess) throw EngineError(2, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // grow without keeping the contents
+  void ensure(size_t m) {
+    if (m <= n && p) return;
+    release();
+    size_t cap = std::max<size_t>(m, 1);
+    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&p), cap * sizeof(T)));
+    n = cap;
+  }
+};
+
+inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t>((n + b - 1) / b); }
+
+template <typename Real>
+struct PacketBufs {
+  DevBuf<typename V4<Real>::T> pos, dir;
+  DevBuf<double> w;
+  DevBuf<uint4> meta;
+  void ensure(size_t n) {
+    pos.ensure(n);
+    dir.ensure(n);
+    w.ensure(n);
+    meta.ensure(n);
+  }
+  PacketSoA<Real> soa() { return PacketSoA<Real>{pos.p, dir.p, w.p, meta.p}; }
+  size_t cap() const { return meta.n; }
+};
+
+template <typename Real>
+struct CensusBufs {
+  DevBuf<typename V4<Real>::T> pos, dir;
+  DevBuf<double> w;
+  DevBuf<uint2> meta;
+  void ensure(size_t n) {
+    pos.ensure(n);
+    dir.ensure(n);
+    w.ensure(n);
+    meta.ensure(n);
+  }
+  CensusSoA<Real> soa() { return CensusSoA<Real>{pos.p, dir.p, w.p, meta.p}; }
+  size_t cap() const { return meta.n; }
+};
+
+struct PairAdd {
+  __host__ __device__ ulonglong2 operator()(const ulonglong2 &a, const ulonglong2 &b) const {
+    return make_ulonglong2(a.x + b.x, a.y + b.y);
+  }
+};
+
+// ---------------------------------------------------------------------------
+struct EngineBase {
+  virtual ~EngineBase() = default;
+  virtual void step(double dt) = 0;
+  virtual void get_cells(double *, double *, double *, double *, double *, double *) = 0;
+  virtual void get_segments(uint64_t *) = 0;
+  virtual void get_begin(uint64_t *) = 0;
+  virtual void get_ledger(storm_gpu_ledger *) = 0;
+  virtual void set_cell_state(const double *, const double *, const double *, const double *, const double *) = 0;
+  virtual uint64_t census_size() = 0;
+  virtual void export_census(storm_packet_record *, uint64_t, uint64_t *) = 0;
+  virtual void import_census(const storm_packet_record *, uint64_t) = 0;
+  virtual void debug_inputs(uint64_t *, void *, void *, int64_t *) = 0;
+  virtual void debug_tallies(uint64_t *, uint64_t *, uint64_t *, uint32_t *) = 0;
+  storm_gpu_counters counters{};
+  double source_T = 0.0;
+  int32_t block = 256;
+  int32_t seg_per_pass = 1 << 20;
+  bool keep_inputs = false;
+};
+
+template <typename Real>
+struct Engine final : EngineBase {
+  // problem (host copies)
+  int32_t nx, ny, nz;
+  uint32_t N;
+  std::vector<double> xe, ye, ze, Vh, sa0h, ss0h, rhoh;
+  storm_material_desc matd;
+  storm_boundary_desc bnd;
+  storm_imc_params imc;
+  uint64_t seed;
+  int32_t precision, material_mode;
+  bool frozen = false;
+  double frozenT4 = 0.0;
+  uint32_t step_index = 0;
+  int32_t eq_prev = -1074;
+  double census_total_prev = 0.0;  // erg
+  uint64_t n_census = 0;           // census resident in `cnext`
+
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // device: mesh and cells
+  DevBuf<Real> wx, wy, wz;
+  DevBuf<double> V, sa0, ss0, rho, T, e_mat, e_sync, fleck, sig_a, sig_s, Eemit, wemit, erad_avg, census_erad,
+      clamp, partials, host_sa, host_ss, host_cv, wface;
+  bool have_host_sa = false, have_host_ss = false, have_host_cv = false;
+  DevBuf<CellXS<Real>> xs;
+  DevBuf<uint32_t> nphot, err, cnt_u32, start_u32, ncopy, vals, vals_out, counters_u32;
+  DevBuf<uint64_t> offsets, keys, keys_out, offs64, nphot64;
+  DevBuf<unsigned long long> tally_u64, emit_limbs, W, requant, leaving;
+  DevBuf<double> tally_f64;
+  DevBuf<ulonglong2> limbs_sorted, cum;
+  DevBuf<CombCell> comb_cells;
+  DevBuf<unsigned char> temp;
+  Tallies tal{};
+
+  PacketBufs<Real> pa, pb;
+  CensusBufs<Real> cpre, cnext;
+
+  // kept transport inputs of the last step (debug / replay)
+  std::vector<Packet<Real>> kept_packets;
+  std::vector<CellXS<Real>> kept_xs;
+  StepConsts kept_consts{};
+
+  Engine(const storm_problem *p, int32_t prec, int32_t mmode) : precision(prec), material_mode(mmode) {
+    nx = p->mesh.nx;
+    ny = p->mesh.ny;
+    nz = p->mesh.nz;
+    if (nx < 1 || ny < 1 || nz < 1 || nx > 4096 || ny > 1024 || nz > 1024)
+      throw EngineError(1, "mesh dimensions must be 1..4096 x 1..1024 x 1..1024");
+    N = static_cast<uint32_t>(nx) * ny * nz;
+    xe.assign(p->mesh.x_edges, p->mesh.x_edges + nx + 1);
+    ye.assign(p->mesh.y_edges, p->mesh.y_edges + ny + 1);
+    ze.assign(p->mesh.z_edges, p->mesh.z_edges + nz + 1);
+    for (auto *e : {&xe, &ye, &ze})
+      for (size_t q = 1; q < e->size(); ++q)
+        if (!((*e)[q] > (*e)[q - 1])) throw EngineError(1, "mesh edges must be strictly increasing");
+    matd = p->material;
+    bnd = p->boundary;
+    imc = p->imc;
+    seed = p->seed;
+    if (imc.new_photons_per_cell == 0) throw EngineError(1, "RadiationIMC requires newPhotonsPerCell > 0");
+    if (imc.with_random_walk || imc.with_ddmc)
+      throw EngineError(3, "random walk / DDMC are not available in this engine build");
+    if (bnd.source_face != -1 && bnd.source_face != STORM_XLO)
+      throw EngineError(1, "only an x- Planck source face is supported (SideTemperature)");
+    if (!p->material.sigma_a0 && material_mode == STORM_GPU_MATERIAL_DEVICE)
+      throw EngineError(1, "material.sigma_a0 is required");
+    source_T = bnd.source_temperature;
+    Vh.resize(N);
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k)
+          Vh[(static_cast<size_t>(i) * ny + j) * nz + k] = (xe[i + 1] - xe[i]) * (ye[j + 1] - ye[j]) * (ze[k + 1] - ze[k]);
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreate(&ev0));
+    HIPCHECK(hipEventCreate(&ev1));
+    // mesh widths
+    auto up_w = [&](DevBuf<Real> &d, const std::vector<double> &e) {
+      std::vector<Real> w(e.size() - 1);
+      for (size_t q = 0; q + 1 < e.size(); ++q) w[q] = static_cast<Real>(e[q + 1] - e[q]);
+      d.ensure(w.size());
+      HIPCHECK(hipMemcpy(d.p, w.data(), w.size() * sizeof(Real), hipMemcpyHostToDevice));
+    };
+    up_w(wx, xe);
+    up_w(wy, ye);
+    up_w(wz, ze);
+    auto up = [&](DevBuf<double> &d, const double *src, size_t n) {
+      d.ensure(n);
+      HIPCHECK(hipMemcpy(d.p, src, n * sizeof(double), hipMemcpyHostToDevice));
+    };
+    up(V, Vh.data(), N);
+    if (p->material.sigma_a0) up(sa0, p->material.sigma_a0, N);
+    if (p->material.sigma_s0) up(ss0, p->material.sigma_s0, N);
+    if (p->material.density) up(rho, p->material.density, N);
+    // initial temperature and internal energy (e = EOS(T0) * V)
+    std::vector<double> e0(N);
+    for (uint32_t c = 0; c < N; ++c) {
+      double r = p->material.density ? p->material.density[c] : 1.0;
+      e0[c] = matd.eos_f * std::pow(p->T0[c], matd.eos_beta) * std::pow(r, 1.0 - matd.eos_mu) * Vh[c];
+    }
+    up(T, p->T0, N);
+    up(e_mat, e0.data(), N);
+    up(e_sync, e0.data(), N);
+    for (DevBuf<double> *b : {&fleck, &sig_a, &sig_s, &Eemit, &wemit, &erad_avg, &census_erad, &clamp})
+      b->ensure(N);
+    partials.ensure(1024);
+    xs.ensure(N);
+    nphot.ensure(N);
+    nphot64.ensure(N + 1);
+    offsets.ensure(N + 1);
+    err.ensure(1);
+    cnt_u32.ensure(N);
+    start_u32.ensure(N);
+    comb_cells.ensure(N);
+    emit_limbs.ensure(2 * static_cast<size_t>(N));
+    W.ensure(2 * static_cast<size_t>(N));
+    requant.ensure(2 * static_cast<size_t>(N));
+    leaving.ensure(1);
+    counters_u32.ensure(4);
+    tally_u64.ensure(7 * 2 * static_cast<size_t>(N));
+    tally_f64.ensure(3 * static_cast<size_t>(N));
+    DevBuf<uint32_t> *segs_begin = new DevBuf<uint32_t>();  // owned below
+    segs_begin_.reset(segs_begin);
+    segs_begin->ensure(2 * static_cast<size_t>(N));
+    unsigned long long *tu = tally_u64.p;
+    const size_t n2 = 2 * static_cast<size_t>(N);
+    tal.dep = tu;
+    tal.pint = tu + n2;
+    tal.esc = tu + 2 * n2;
+    tal.cin = tu + 3 * n2;
+    tal.cpre = tu + 4 * n2;
+    tal.cout = tu + 5 * n2;
+    tal.rem = tu + 6 * n2;
+    tal.dep_fp = tally_f64.p;
+    tal.pint_fp = tally_f64.p + N;
+    tal.esc_fp = tally_f64.p + 2 * static_cast<size_t>(N);
+    tal.segs = segs_begin->p;
+    tal.begin = segs_begin->p + N;
+    if (precision != STORM_GPU_FP64) {
+      tal.dep_fp = nullptr;
+      tal.pint_fp = nullptr;
+      tal.esc_fp = nullptr;
+    }
+    zero_u64(tally_u64.p, tally_u64.n);
+    zero_f64(tally_f64.p, tally_f64.n);
+    zero_u32(segs_begin->p, segs_begin->n);
+    zero_u64(requant.p, requant.n);
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+
+  ~Engine() override {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  std::unique_ptr<DevBuf<uint32_t>> segs_begin_;
+
+  void zero_u64(unsigned long long *p, size_t n) {
+    if (n) k_fill_u64<<<blocks_for(n, 256), 256, 0, stream>>>(p, n, 0ull);
+  }
+  void zero_u32(uint32_t *p, size_t n) {
+    if (n) k_fill_u32<<<blocks_for(n, 256), 256, 0, stream>>>(p, n, 0u);
+  }
+  void zero_f64(double *p, size_t n) {
+    if (n) k_fill_f64<<<blocks_for(n, 256), 256, 0, stream>>>(p, n, 0.0);
+  }
+
+  MatDev matdev() const {
+    MatDev m;
+    m.sa0 = sa0.p;
+    m.ss0 = ss0.p;
+    m.rho = rho.p;
+    m.Tref = matd.opacity_T_ref;
+    m.Tfloor = matd.opacity_T_floor;
+    m.apow = matd.a_temp_pow;
+    m.arho = matd.a_rho_pow;
+    m.spow = matd.s_temp_pow;
+    m.srho = matd.s_rho_pow;
+    m.eos_f = matd.eos_f;
+    m.eos_beta = matd.eos_beta;
+    m.eos_mu = matd.eos_mu;
+    m.cvTfloor = matd.eos_cv_T_floor;
+    return m;
+  }
+  MeshDims dims() const { return MeshDims{static_cast<uint32_t>(nx), static_cast<uint32_t>(ny), static_cast<uint32_t>(nz)}; }
+  MeshView<Real> meshview() const {
+    MeshView<Real> m;
+    m.nx = nx;
+    m.ny = ny;
+    m.nz = nz;
+    for (int f = 0; f < 6; ++f) m.face_kind[f] = bnd.face_kind[f];
+    m.width[0] = wx.p;
+    m.width[1] = wy.p;
+    m.width[2] = wz.p;
+    return m;
+  }
+
+  double deterministic_sum(const double *d, uint64_t n) {
+    constexpr int G = 512;
+    k_sum_partials<256><<<G, 256, 0, stream>>>(d, n, partials.p);
+    k_sum_partials<256><<<1, 256, 0, stream>>>(partials.p, G, partials.p + G);
+    double h = 0.0;
+    HIPCHECK(hipMemcpyAsync(&h, partials.p + G, sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    return h;
+  }
+
+  template <typename F>
+  void ensure_temp(F &&query) {
+    size_t bytes = 0;
+    query(nullptr, bytes);
+    temp.ensure(bytes + 16);
+    query(temp.p, bytes);
+  }
+
+  // ----------------------------------------------------------------- step
+  void step(double dt) override {
+    if (!(dt > 0.0)) throw EngineError(1, "dt must be positive");
+    auto t_start = std::chrono::steady_clock::now();
+    ++step_index;
+    counters = storm_gpu_counters{};
+    const MatDev md = matdev();
+    const MeshDims d = dims();
+    const Real s0 = static_cast<Real>(units::clight * dt);
+    zero_u32(err.p, 1);
+    eq_prev_ledger_ = eq_prev;
+    material_before_ = deterministic_sum(e_mat.p, N);
+
+    // preStep: opacities, Fleck factor, emission energies
+    k_prestep<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
+        N, T.p, md, have_host_sa ? host_sa.p : nullptr, have_host_ss ? host_ss.p : nullptr,
+        have_host_cv ? host_cv.p : nullptr, V.p, dt, xs.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p);
+    const double Etot = deterministic_sum(Eemit.p, N);
+
+    // boundary source: SideTemperature energy per face (T^4 frozen at the first call if asked)
+    const uint32_t npc = (bnd.source_face == STORM_XLO) ? static_cast<uint32_t>(bnd.source_packets_per_face) : 0u;
+    const uint32_t nfaces = npc ? static_cast<uint32_t>(ny) * nz : 0u;
+    double T4 = source_T * source_T * source_T * source_T;
+    if (bnd.freeze_source_T4) {
+      if (!frozen) {
+        frozenT4 = T4;
+        frozen = true;
+      }
+      T4 = frozenT4;
+    }
+    std::vector<double> wface_e(nfaces);
+    double source_total = 0.0;
+    for (uint32_t f = 0; f < nfaces; ++f) {
+      uint32_t j = f / nz, k = f % nz;
+      double A = (ye[j + 1] - ye[j]) * (ze[k + 1] - ze[k]);
+      wface_e[f] = units::sigma_sb * T4 * A * dt / npc;
+      source_total += wface_e[f] * npc;
+    }
+
+    // energy quantum of this step: every energy below 2^90 quanta
+    const double Ebound = census_total_prev + Etot + source_total;
+    const int32_t eq = Ebound > 0.0 ? fx::ceil_log2(Ebound) - fx::kHeadroomBits : -1000;
+    const int32_t et = Ebound > 0.0 ? fx::ceil_log2(Ebound * units::clight * dt) - fx::kHeadroomBits : -1000;
+    counters.energy_quantum_exp = eq;
+    for (uint32_t f = 0; f < nfaces; ++f) wface_e[f] = fx::quantize_down(wface_e[f], eq);
+    wface.ensure(std::max<uint32_t>(nfaces, 1));
+    if (nfaces)
+      HIPCHECK(hipMemcpyAsync(wface.p, wface_e.data(), nfaces * sizeof(double), hipMemcpyHostToDevice, stream));
+
+    // photon allocation and offsets
+    k_alloc<<<blocks_for(N, 256), 256, 0, stream>>>(N, Eemit.p, Etot, imc.new_photons_per_cell, eq, nphot.p,
+                                                    wemit.p, emit_limbs.p);
+    {
+      auto q = [&](void *tp, size_t &bytes) {
+        HIPCHECK(rocprim::exclusive_scan(tp, bytes, nphot.p, offsets.p, uint64_t(0), static_cast<size_t>(N) + 0,
+                                         rocprim::plus<uint64_t>(), stream));
+      };
+      ensure_temp(q);
+    }
+    uint64_t n_emit = 0;
+    {
+      uint64_t last_off = 0;
+      uint32_t last_n = 0;
+      HIPCHECK(hipMemcpyAsync(&last_off, offsets.p + (N - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipMemcpyAsync(&last_n, nphot.p + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      n_emit = last_off + last_n;
+      HIPCHECK(hipMemcpyAsync(offsets.p + N, &n_emit, sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    }
+    const uint64_t n_bc = static_cast<uint64_t>(nfaces) * npc;
+    const uint64_t n_total = n_census + n_emit + n_bc;
+    if (n_total >= (1ull << 32) - 1) throw EngineError(4, "more than 2^32 packets in one step");
+    counters.initial_particles = n_census;
+    counters.emitted_particles = n_emit;
+    counters.boundary_particles = n_bc;
+    counters.prestep_particles = n_emit + n_bc;
+    counters.start_particles = n_total;
+
+    // reset tallies
+    zero_u64(tally_u64.p, tally_u64.n);
+    if (precision == STORM_GPU_FP64) zero_f64(tally_f64.p, tally_f64.n);
+    zero_u32(tal.segs, 2 * static_cast<size_t>(N));
+    zero_u64(requant.p, requant.n);
+    zero_u64(leaving.p, 1);
+
+    // packets: [census | emission | boundary]
+    pa.ensure(n_total);
+    pb.ensure(n_total);
+    cpre.ensure(n_total);
+    if (n_census)
+      k_import<Real><<<blocks_for(n_census, 256), 256, 0, stream>>>(
+          static_cast<uint32_t>(n_census), cnext.soa(), pa.soa(), s0, eq_prev, eq, d, tal, requant.p);
+    if (n_emit)
+      k_emit<Real><<<blocks_for(n_emit, 256), 256, 0, stream>>>(
+          static_cast<uint32_t>(n_emit), N, offsets.p, wemit.p, pa.soa(), static_cast<uint32_t>(n_census), s0, d,
+          seed, step_index, tal.begin);
+    if (n_bc)
+      k_boundary<Real><<<blocks_for(n_bc, 256), 256, 0, stream>>>(
+          static_cast<uint32_t>(n_bc), npc, wface.p, pa.soa(), static_cast<uint32_t>(n_census + n_emit), s0, d, seed,
+          step_index, tal.begin);
+
+    StepConsts kc;
+    kc.seed = seed;
+    kc.step = step_index;
+    kc.eq = eq;
+    kc.et = et;
+    if (keep_inputs) {
+      HIPCHECK(hipStreamSynchronize(stream));
+      keep_transport_inputs(n_total, kc);
+    }
+
+    // transport passes until every packet is in census, absorbed or gone
+    const MeshView<Real> mv = meshview();
+    uint64_t n_in = n_total;
+    bool ping = true;
+    double t_transport = 0.0, t_max = 0.0;
+    uint32_t n_cen_total = 0;
+    zero_u32(counters_u32.p, 2);
+    while (n_in > 0) {
+      if (counters.transport_passes >= 100000)
+        throw EngineError(6, "transport did not finish within 100000 passes");
+      PacketBufs<Real> &src = ping ? pa : pb;
+      PacketBufs<Real> &dst = ping ? pb : pa;
+      zero_u32(counters_u32.p, 1);  // n_out
+      HIPCHECK(hipEventRecord(ev0, stream));
+      k_transport<Real><<<blocks_for(n_in, block), block, 0, stream>>>(
+          static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
+          xs.p, kc, seg_per_pass, tal, leaving.p);
+      HIPCHECK(hipEventRecord(ev1, stream));
+      uint32_t h[2];
+      HIPCHECK(hipMemcpyAsync(h, counters_u32.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+      t_transport += ms * 1e-3;
+      t_max = std::max(t_max, static_cast<double>(ms) * 1e-3);
+      counters.transport_launches++;
+      counters.transport_passes++;
+      n_in = h[0];
+      n_cen_total = h[1];
+      ping = !ping;
+    }
+    counters.seconds_transport = t_transport;
+    counters.seconds_transport_max_launch = t_max;
+    counters.census_before_comb = n_cen_total;
+
+    // population control (comb) or a deterministic pass-through
+    uint64_t n_out = comb(n_cen_total, eq);
+    n_census = n_out;
+    counters.end_particles = n_out;
+
+    // postStep
+    k_post<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
+        N, tal, eq, et, eq_prev, emit_limbs.p, requant.p, dt, V.p, md,
+        material_mode == STORM_GPU_MATERIAL_DEVICE ? 1 : 0, T.p, e_mat.p, e_sync.p, erad_avg.p, census_erad.p,
+        clamp.p, err.p, precision == STORM_GPU_FP64 ? 1 : 0);
+    census_total_prev = deterministic_sum(census_erad.p, N);
+    uint32_t herr = 0;
+    unsigned long long hleave = 0;
+    HIPCHECK(hipMemcpyAsync(&herr, err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipMemcpyAsync(&hleave, leaving.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    counters.leaving_particles = hleave;
+    eq_prev = eq;
+    last_eq_ = eq;
+    last_et_ = et;
+    last_dt_ = dt;
+    if (herr & kErrFleck) throw EngineError(5, "Invalid factor fleck in RadiationIMC::preStep");
+    if (herr & kErrNegativeEnergy) throw EngineError(5, "Negative material internal energy in RadiationIMC");
+    if (herr & kErrTallyOverflow) throw EngineError(5, "a cell received 2^28 tally additions in one step");
+    // segments
+    counters.segments = total_segments();
+    auto t_end = std::chrono::steady_clock::now();
+    counters.seconds_step = std::chrono::duration<double>(t_end - t_start).count();
+  }
+
+  int32_t last_eq_ = 0, last_et_ = 0;
+  double last_dt_ = 0.0;
+
+  uint64_t total_segments() {
+    std::vector<uint32_t> h(N);
+    HIPCHECK(hipMemcpy(h.data(), tal.segs, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    uint64_t s = 0;
+    for (uint32_t v : h) s += v;
+    return s;
+  }
+
+  // Comb (or pass-through) from `cpre` into `cnext`; returns the census size.
+  uint64_t comb(uint32_t n, int32_t eq) {
+    if (n == 0) return 0;
+    const MeshDims d = dims();
+    keys.ensure(n);
+    keys_out.ensure(n);
+    vals.ensure(n);
+    vals_out.ensure(n);
+    zero_u32(cnt_u32.p, N);
+    zero_u64(W.p, W.n);
+    const uint32_t salt = static_cast<uint32_t>(seed) * 0x2545F491u + step_index * 0x9E3779B9u;
+    k_comb_keys<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, cpre.soa(), d, N, salt, eq, keys.p, vals.p,
+                                                              cnt_u32.p, W.p);
+    int end_bit = 32;
+    while ((1ull << (end_bit - 32)) <= N) ++end_bit;
+    {
+      auto q = [&](void *tp, size_t &bytes) {
+        HIPCHECK(rocprim::radix_sort_pairs(tp, bytes, keys.p, keys_out.p, vals.p, vals_out.p, static_cast<size_t>(n),
+                                           0, end_bit, stream));
+      };
+      ensure_temp(q);
+    }
+    {
+      auto q = [&](void *tp, size_t &bytes) {
+        HIPCHECK(rocprim::exclusive_scan(tp, bytes, cnt_u32.p, start_u32.p, uint32_t(0), static_cast<size_t>(N),
+                                         rocprim::plus<uint32_t>(), stream));
+      };
+      ensure_temp(q);
+    }
+    uint32_t last_start = 0, last_cnt = 0;
+    HIPCHECK(hipMemcpyAsync(&last_start, start_u32.p + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipMemcpyAsync(&last_cnt, cnt_u32.p + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    // total census weight W = sum of W_c (exact integer) as double
+    std::vector<unsigned long long> hW(2 * static_cast<size_t>(N));
+    HIPCHECK(hipMemcpyAsync(hW.data(), W.p, hW.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    const uint32_t n_valid = last_start + last_cnt;
+    if (n_valid == 0) return 0;
+    unsigned __int128 Wsum = 0;
+    for (uint32_t c = 0; c < N; ++c)
+      Wsum += (static_cast<unsigned __int128>(hW[2 * c + 1]) << fx::kLoBits) + hW[2 * c];
+    const double Wtot = (static_cast<double>(static_cast<uint64_t>(Wsum >> 64)) * 18446744073709551616.0 +
+                         static_cast<double>(static_cast<uint64_t>(Wsum))) *
+                        fpm::pow2d(eq);
+    k_comb_cells<<<blocks_for(N, 256), 256, 0, stream>>>(N, cnt_u32.p, W.p, Wtot, imc.comb_nmin, imc.comb_factor, eq,
+                                                         imc.comb ? 1 : 0, seed, step_index, comb_cells.p, tal.rem);
+    limbs_sorted.ensure(n_valid);
+    cum.ensure(n_valid);
+    ncopy.ensure(n_valid);
+    offs64.ensure(n_valid);
+    k_comb_limbs<Real><<<blocks_for(n_valid, 256), 256, 0, stream>>>(n_valid, vals_out.p, cpre.soa(), eq,
+                                                                     limbs_sorted.p);
+    {
+      auto q = [&](void *tp, size_t &bytes) {
+        HIPCHECK(rocprim::inclusive_scan(tp, bytes, limbs_sorted.p, cum.p, static_cast<size_t>(n_valid), PairAdd(),
+                                         stream));
+      };
+      ensure_temp(q);
+    }
+    k_comb_count<Real><<<blocks_for(n_valid, 256), 256, 0, stream>>>(n_valid, keys_out.p, vals_out.p, cpre.soa(),
+                                                                     comb_cells.p, start_u32.p, cnt_u32.p, cum.p, eq,
+                                                                     ncopy.p);
+    {
+      auto q = [&](void *tp, size_t &bytes) {
+        HIPCHECK(rocprim::exclusive_scan(tp, bytes, ncopy.p, offs64.p, uint64_t(0), static_cast<size_t>(n_valid),
+                                         rocprim::plus<uint64_t>(), stream));
+      };
+      ensure_temp(q);
+    }
+    uint64_t last_off = 0;
+    uint32_t last_nc = 0;
+    HIPCHECK(hipMemcpyAsync(&last_off, offs64.p + (n_valid - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipMemcpyAsync(&last_nc, ncopy.p + (n_valid - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    const uint64_t n_out = last_off + last_nc;
+    if (n_out >= (1ull << 32) - 1) throw EngineError(4, "census larger than 2^32 packets");
+    cnext.ensure(n_out);
+    k_comb_write<Real><<<blocks_for(n_valid, 256), 256, 0, stream>>>(n_valid, keys_out.p, vals_out.p, cpre.soa(),
+                                                                     comb_cells.p, ncopy.p, offs64.p, cnext.soa(), eq,
+                                                                     tal);
+    return n_out;
+  }
+
+  void keep_transport_inputs(uint64_t n, const StepConsts &kc) {
+    kept_packets.resize(n);
+    std::vector<typename V4<Real>::T> hp(n), hd(n);
+    std::vector<double> hw(n);
+    std::vector<uint4> hm(n);
+    if (n) {
+      HIPCHECK(hipMemcpy(hp.data(), pa.pos.p, n * sizeof(hp[0]), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(hd.data(), pa.dir.p, n * sizeof(hd[0]), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(hw.data(), pa.w.p, n * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(hm.data(), pa.meta.p, n * sizeof(uint4), hipMemcpyDeviceToHost));
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+      Packet<Real> &p = kept_packets[i];
+      p.u[0] = hp[i].x;
+      p.u[1] = hp[i].y;
+      p.u[2] = hp[i].z;
+      p.s = hp[i].w;
+      p.o[0] = hd[i].x;
+      p.o[1] = hd[i].y;
+      p.o[2] = hd[i].z;
+      p.wmin = hd[i].w;
+      p.w = hw[i];
+      p.ijk = hm[i].x;
+      p.id = hm[i].y;
+      p.ctr = hm[i].z;
+      p.flags = hm[i].w;
+    }
+    kept_xs.resize(N);
+    HIPCHECK(hipMemcpy(kept_xs.data(), xs.p, N * sizeof(CellXS<Real>), hipMemcpyDeviceToHost));
+    kept_consts = kc;
+  }
+
+  void debug_inputs(uint64_t *n, void *packets, void *cell_xs, int64_t *consts4) override {
+    *n = kept_packets.size();
+    if (packets) std::memcpy(packets, kept_packets.data(), kept_packets.size() * sizeof(Packet<Real>));
+    if (cell_xs) std::memcpy(cell_xs, kept_xs.data(), kept_xs.size() * sizeof(CellXS<Real>));
+    if (consts4) {
+      consts4[0] = static_cast<int64_t>(kept_consts.seed);
+      consts4[1] = kept_consts.step;
+      consts4[2] = kept_consts.eq;
+      consts4[3] = kept_consts.et;
+    }
+  }
+
+  void debug_tallies(uint64_t *dep, uint64_t *pint, uint64_t *esc, uint32_t *segs) override {
+    const size_t n2 = 2 * static_cast<size_t>(N);
+    if (dep) HIPCHECK(hipMemcpy(dep, tal.dep, n2 * 8, hipMemcpyDeviceToHost));
+    if (pint) HIPCHECK(hipMemcpy(pint, tal.pint, n2 * 8, hipMemcpyDeviceToHost));
+    if (esc) HIPCHECK(hipMemcpy(esc, tal.esc, n2 * 8, hipMemcpyDeviceToHost));
+    if (segs) HIPCHECK(hipMemcpy(segs, tal.segs, N * 4, hipMemcpyDeviceToHost));
+  }
+
+  void get_cells(double *To, double *eo, double *erad, double *cen, double *fl, double *pl) override {
+    HIPCHECK(hipStreamSynchronize(stream));
+    auto dl = [&](double *dst, const DevBuf<double> &b) {
+      if (dst) HIPCHECK(hipMemcpy(dst, b.p, N * sizeof(double), hipMemcpyDeviceToHost));
+    };
+    dl(To, T);
+    dl(eo, e_mat);
+    dl(erad, erad_avg);
+    dl(cen, census_erad);
+    dl(fl, fleck);
+    dl(pl, sig_a);
+  }
+
+  void get_segments(uint64_t *out) override {
+    std::vector<uint32_t> h(N);
+    HIPCHECK(hipMemcpy(h.data(), tal.segs, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint32_t c = 0; c < N; ++c) out[c] = h[c];
+  }
+  void get_begin(uint64_t *out) override {
+    std::vector<uint32_t> h(N);
+    HIPCHECK(hipMemcpy(h.data(), tal.begin, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint32_t c = 0; c < N; ++c) out[c] = h[c];
+  }
+
+  static storm_gpu_u128 u128(unsigned __int128 v) {
+    storm_gpu_u128 r;
+    r.lo = static_cast<uint64_t>(v);
+    r.hi = static_cast<uint64_t>(v >> 64);
+    return r;
+  }
+
+  void get_ledger(storm_gpu_ledger *L) override {
+    const size_t n2 = 2 * static_cast<size_t>(N);
+    std::vector<unsigned long long> h(tally_u64.n), em(n2), rq(n2);
+    HIPCHECK(hipMemcpy(h.data(), tally_u64.p, h.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(em.data(), emit_limbs.p, n2 * 8, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(rq.data(), requant.p, n2 * 8, hipMemcpyDeviceToHost));
+    auto sum = [&](const unsigned long long *a) {
+      unsigned __int128 s = 0;
+      for (uint32_t c = 0; c < N; ++c) s += (static_cast<unsigned __int128>(a[2 * c + 1]) << fx::kLoBits) + a[2 * c];
+      return s;
+    };
+    const unsigned long long *b = h.data();
+    std::memset(L, 0, sizeof(*L));
+    L->census_in = u128(sum(b + 3 * n2));
+    L->emitted = u128(sum(em.data()));
+    L->deposited = u128(sum(b));
+    L->escaped = u128(sum(b + 2 * n2));
+    if (precision == STORM_GPU_FP64) {
+      // FP64 engine: deposits and escapes are FP64 sums, reported rounded to quanta
+      std::vector<double> f(tally_f64.n);
+      HIPCHECK(hipMemcpy(f.data(), tally_f64.p, f.size() * sizeof(double), hipMemcpyDeviceToHost));
+      long double dsum = 0.0L, esum = 0.0L;
+      for (uint32_t c = 0; c < N; ++c) {
+        dsum += f[c];
+        esum += f[2 * static_cast<size_t>(N) + c];
+      }
+      const long double Q = std::ldexp(1.0L, last_eq_);
+      L->deposited = u128(static_cast<unsigned __int128>(std::llround(static_cast<double>(dsum / Q) / 1.0)));
+      L->escaped = u128(static_cast<unsigned __int128>(std::llround(static_cast<double>(esum / Q))));
+    }
+    L->census_pre = u128(sum(b + 4 * n2));
+    L->census_out = u128(sum(b + 5 * n2));
+    L->comb_rem = u128(sum(b + 6 * n2));
+    // boundary source: Npc * w_face per face, exact
+    unsigned __int128 src = 0;
+    const uint32_t npc = (bnd.source_face == STORM_XLO) ? static_cast<uint32_t>(bnd.source_packets_per_face) : 0u;
+    if (npc) {
+      std::vector<double> wf(static_cast<size_t>(ny) * nz);
+      HIPCHECK(hipMemcpy(wf.data(), wface.p, wf.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (double w : wf) {
+        fx::Limbs l = fx::limbs_of(w, last_eq_);
+        src += ((static_cast<unsigned __int128>(l.hi) << fx::kLoBits) + l.lo) * npc;
+      }
+    }
+    L->source = u128(src);
+    L->eq = last_eq_;
+    L->eq_prev = eq_prev_ledger_;
+    double rq_sum = 0.0;
+    for (uint32_t c = 0; c < N; ++c) rq_sum += fx::to_double(rq[2 * c], rq[2 * c + 1], eq_prev_ledger_);
+    L->requant_rem = rq_sum;
+    std::vector<double> cl(N), em2(N);
+    HIPCHECK(hipMemcpy(cl.data(), clamp.p, N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(em2.data(), e_mat.p, N * sizeof(double), hipMemcpyDeviceToHost));
+    double cls = 0.0, ms = 0.0;
+    for (uint32_t c = 0; c < N; ++c) {
+      cls += cl[c];
+      ms += em2[c];
+    }
+    L->clamp = cls;
+    L->material_after = ms;
+    L->material_before = material_before_;
+  }
+  int32_t eq_prev_ledger_ = -1074;
+  double material_before_ = 0.0;
+
+  void set_cell_state(const double *Ti, const double *sa, const double *ss, const double *cv,
+                      const double *e) override {
+    auto up = [&](DevBuf<double> &d, const double *src) {
+      d.ensure(N);
+      HIPCHECK(hipMemcpy(d.p, src, N * sizeof(double), hipMemcpyHostToDevice));
+    };
+    if (Ti) up(T, Ti);
+    if (sa) {
+      up(host_sa, sa);
+      have_host_sa = true;
+    }
+    if (ss) {
+      up(host_ss, ss);
+      have_host_ss = true;
+    }
+    if (cv) {
+      up(host_cv, cv);
+      have_host_cv = true;
+    }
+    if (e) {
+      up(e_mat, e);
+      up(e_sync, e);
+    }
+  }
+
+  uint64_t census_size() override { return n_census; }
+
+  // global position of a census packet (cell lower corner + u * width)
+  void export_census(storm_packet_record *out, uint64_t cap, uint64_t *nw) override {
+    const uint64_t n = std::min<uint64_t>(cap, n_census);
+    std::vector<typename V4<Real>::T> hp(n), hd(n);
+    std::vector<double> hw(n);
+    std::vector<uint2> hm(n);
+    if (n) {
+      HIPCHECK(hipMemcpy(hp.data(), cnext.pos.p, n * sizeof(hp[0]), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(hd.data(), cnext.dir.p, n * sizeof(hd[0]), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(hw.data(), cnext.w.p, n * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(hm.data(), cnext.meta.p, n * sizeof(uint2), hipMemcpyDeviceToHost));
+    }
+    for (uint64_t q = 0; q < n; ++q) {
+      uint32_t i = ijk_i(hm[q].x), j = ijk_j(hm[q].x), k = ijk_k(hm[q].x);
+      storm_packet_record &r = out[q];
+      r.x = xe[i] + static_cast<double>(hp[q].x) * (xe[i + 1] - xe[i]);
+      r.y = ye[j] + static_cast<double>(hp[q].y) * (ye[j + 1] - ye[j]);
+      r.z = ze[k] + static_cast<double>(hp[q].z) * (ze[k + 1] - ze[k]);
+      r.vx = static_cast<double>(hd[q].x) * units::clight;
+      r.vy = static_cast<double>(hd[q].y) * units::clight;
+      r.vz = static_cast<double>(hd[q].z) * units::clight;
+      r.weight = hw[q];
+      r.frequency = 0.0;
+      r.cell = (static_cast<uint64_t>(i) * ny + j) * nz + k;
+      r.id = ~0ull;
+    }
+    *nw = n;
+  }
+
+  void import_census(const storm_packet_record *in, uint64_t n) override {
+    if (n >= (1ull << 32) - 1) throw EngineError(4, "census larger than 2^32 packets");
+    std::vector<typename V4<Real>::T> hp(n), hd(n);
+    std::vector<double> hw(n);
+    std::vector<uint2> hm(n);
+    auto locate = [](const std::vector<double> &e, double v) {
+      auto it = std::upper_bound(e.begin(), e.end(), v);
+      int idx = static_cast<int>(it - e.begin()) - 1;
+      return std::clamp(idx, 0, static_cast<int>(e.size()) - 2);
+    };
+    const double qe = fpm::pow2d(eq_prev);
+    for (uint64_t q = 0; q < n; ++q) {
+      const storm_packet_record &r = in[q];
+      int i = r.cell < N ? static_cast<int>(r.cell / (static_cast<uint64_t>(ny) * nz)) : locate(xe, r.x);
+      int j = r.cell < N ? static_cast<int>((r.cell / nz) % ny) : locate(ye, r.y);
+      int k = r.cell < N ? static_cast<int>(r.cell % nz) : locate(ze, r.z);
+      auto frac = [](double v, double lo, double hi) {
+        double f = (v - lo) / (hi - lo);
+        return f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+      };
+      hp[q].x = static_cast<Real>(frac(r.x, xe[i], xe[i + 1]));
+      hp[q].y = static_cast<Real>(frac(r.y, ye[j], ye[j + 1]));
+      hp[q].z = static_cast<Real>(frac(r.z, ze[k], ze[k + 1]));
+      hp[q].w = Real(0);
+      double vn = std::sqrt(r.vx * r.vx + r.vy * r.vy + r.vz * r.vz);
+      if (!(vn > 0.0)) throw EngineError(1, "census packet with zero velocity");
+      hd[q].x = static_cast<Real>(r.vx / vn);
+      hd[q].y = static_cast<Real>(r.vy / vn);
+      hd[q].z = static_cast<Real>(r.vz / vn);
+      hd[q].w = Real(0);
+      if (!std::isfinite(r.weight) || r.weight < 0.0) throw EngineError(1, "census packet weight is negative or not finite");
+      hw[q] = r.weight;
+      hm[q] = make_uint2(pack_ijk(i, j, k), static_cast<uint32_t>(q));
+      (void)qe;
+    }
+    cnext.ensure(std::max<uint64_t>(n, 1));
+    if (n) {
+      HIPCHECK(hipMemcpy(cnext.pos.p, hp.data(), n * sizeof(hp[0]), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(cnext.dir.p, hd.data(), n * sizeof(hd[0]), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(cnext.w.p, hw.data(), n * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(cnext.meta.p, hm.data(), n * sizeof(uint2), hipMemcpyHostToDevice));
+    }
+    n_census = n;
+    double tot = 0.0;
+    for (double w : hw) tot += w;
+    census_total_prev = tot;
+    // imported weights are arbitrary doubles: force re-quantisation at the next step
+    eq_prev = -1074;
+  }
+};
+
+EngineBase *as_engine(storm_gpu_engine *e) { return reinterpret_cast<EngineBase *>(e); }
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const EngineError &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return 9;
+  }
+}
+
+}  // namespace gpu
+}  // namespace storm
+
+using namespace storm::gpu;
+
+extern "C" {
+
+const char *storm_gpu_last_error(void) { return g_last_error.c_str(); }
+int storm_gpu_abi_version(void) { return STORM_GPU_ABI_VERSION; }
+
+int storm_gpu_device_info(char *buf, size_t len) {
+  return guarded([&] {
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, dev));
+    std::snprintf(buf, len, "%s | %s | CUs=%d | clock=%d kHz | mem=%zu MiB", prop.name, prop.gcnArchName,
+                  prop.multiProcessorCount, prop.clockRate, static_cast<size_t>(prop.totalGlobalMem >> 20));
+  });
+}
+
+int storm_gpu_create(const storm_problem *prob, int32_t precision, int32_t material_mode, storm_gpu_engine **out) {
+  return guarded([&] {
+    if (!prob || !out) throw EngineError(1, "null argument");
+    if (!prob->T0) throw EngineError(1, "T0 is required");
+    if (material_mode != STORM_GPU_MATERIAL_DEVICE && material_mode != STORM_GPU_MATERIAL_HOST)
+      throw EngineError(1, "unknown material mode");
+    EngineBase *e = nullptr;
+    if (precision == STORM_GPU_MIXED)
+      e = new Engine<float>(prob, precision, material_mode);
+    else if (precision == STORM_GPU_FP64)
+      e = new Engine<double>(prob, precision, material_mode);
+    else
+      throw EngineError(1, "unknown precision");
+    *out = reinterpret_cast<storm_gpu_engine *>(e);
+  });
+}
+
+int storm_gpu_destroy(storm_gpu_engine *eng) {
+  return guarded([&] { delete as_engine(eng); });
+}
+
+int storm_gpu_step(storm_gpu_engine *eng, double dt) {
+  return guarded([&] { as_engine(eng)->step(dt); });
+}
+
+int storm_gpu_set_source_temperature(storm_gpu_engine *eng, double T) {
+  return guarded([&] { as_engine(eng)->source_T = T; });
+}
+
+int storm_gpu_set_cell_state(storm_gpu_engine *eng, const double *T, const double *sa, const double *ss,
+                             const double *cv, const double *e) {
+  return guarded([&] { as_engine(eng)->set_cell_state(T, sa, ss, cv, e); });
+}
+
+int storm_gpu_get_cells(storm_gpu_engine *eng, double *T, double *e, double *erad, double *cen, double *fl,
+                        double *pl) {
+  return guarded([&] { as_engine(eng)->get_cells(T, e, erad, cen, fl, pl); });
+}
+
+int storm_gpu_get_segments(storm_gpu_engine *eng, uint64_t *per_cell) {
+  return guarded([&] { as_engine(eng)->get_segments(per_cell); });
+}
+
+int storm_gpu_get_beginning_counts(storm_gpu_engine *eng, uint64_t *per_cell) {
+  return guarded([&] { as_engine(eng)->get_begin(per_cell); });
+}
+
+int storm_gpu_get_counters(storm_gpu_engine *eng, storm_gpu_counters *out) {
+  return guarded([&] { *out = as_engine(eng)->counters; });
+}
+
+int storm_gpu_get_ledger(storm_gpu_engine *eng, storm_gpu_ledger *out) {
+  return guarded([&] { as_engine(eng)->get_ledger(out); });
+}
+
+int storm_gpu_census_size(storm_gpu_engine *eng, uint64_t *n) {
+  return guarded([&] { *n = as_engine(eng)->census_size(); });
+}
+
+int storm_gpu_export_census(storm_gpu_engine *eng, storm_packet_record *out, uint64_t cap, uint64_t *nw) {
+  return guarded([&] { as_engine(eng)->export_census(out, cap, nw); });
+}
+
+int storm_gpu_import_census(storm_gpu_engine *eng, const storm_packet_record *in, uint64_t n) {
+  return guarded([&] { as_engine(eng)->import_census(in, n); });
+}
+
+int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segments_per_pass) {
+  return guarded([&] {
+    if (block_size != 64 && block_size != 128 && block_size != 256)
+      throw EngineError(1, "block size must be 64, 128 or 256");
+    if (segments_per_pass < 1) throw EngineError(1, "segments per pass must be >= 1");
+    as_engine(eng)->block = block_size;
+    as_engine(eng)->seg_per_pass = segments_per_pass;
+  });
+}
+
+int storm_gpu_debug_keep_transport_inputs(storm_gpu_engine *eng, int32_t enable) {
+  return guarded([&] { as_engine(eng)->keep_inputs = enable != 0; });
+}
+
+int storm_gpu_debug_transport_inputs(storm_gpu_engine *eng, uint64_t *n, void *packets, void *cell_xs,
+                                     int64_t *consts4) {
+  return guarded([&] { as_engine(eng)->debug_inputs(n, packets, cell_xs, consts4); });
+}
+
+int storm_gpu_debug_tallies(storm_gpu_engine *eng, uint64_t *dep, uint64_t *pint, uint64_t *esc, uint32_t *segs) {
+  return guarded([&] { as_engine(eng)->debug_tallies(dep, pint, esc, segs); });
+}
+
+}  // extern "C"

@@ -1,0 +1,662 @@
+// gfx950 kernels of one IMC step.  Every kernel is templated on the tracking
+// type Real (float: mixed engine, double: plain FP64 engine).
+//
+//   k_prestep      RadiationIMC::preStep per-cell part (radiation/RadiationIMC.hpp:1632-1662)
+//                  and the emission energies of generateParticles (:2046-2060)
+//   k_alloc        photon allocation n_i (:2076-2087) and packet weights E_i/n_i (:2211)
+//   k_import       census re-entry (MonteCarloManagerSerial.hpp:393-405)
+//   k_emit         emission packets (generateSingleParticle :2516-2553, timeLeft :2408)
+//   k_boundary     Planck surface source (boundary/SideTemperature.hpp:114-163)
+//   k_transport    the segment loop (MonteCarloManagerSerial.hpp:266-383 + RadiationIMC::step)
+//   k_comb_*       CombPopulationControl::activate (population/CombPopulationControl.hpp:79-253)
+//   k_post         RadiationIMC::postStep (:1887-2029)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common/constants.h"
+#include "common/fixed.h"
+#include "common/philox.h"
+#include "common/sourcing.h"
+#include "common/transport.h"
+#include "device_util.h"
+
+namespace storm {
+namespace gpu {
+
+template <typename Real>
+struct V4;
+template <>
+struct V4<float> {
+  using T = float4;
+};
+template <>
+struct V4<double> {
+  using T = double4;
+};
+
+// Packet store (structure of arrays, one HBM stream per field group).
+template <typename Real>
+struct PacketSoA {
+  typename V4<Real>::T *pos;  // ux, uy, uz, s
+  typename V4<Real>::T *dir;  // ox, oy, oz, wmin
+  double *w;
+  uint4 *meta;                // ijk, id, ctr, flags
+};
+
+// Census store: what survives to the end of the step.
+template <typename Real>
+struct CensusSoA {
+  typename V4<Real>::T *pos;  // ux, uy, uz, -
+  typename V4<Real>::T *dir;  // ox, oy, oz, -
+  double *w;
+  uint2 *meta;                // ijk, id
+};
+
+template <typename Real>
+__device__ __forceinline__ Packet<Real> load_packet(const PacketSoA<Real> &s, uint32_t i) {
+  Packet<Real> p;
+  auto a = s.pos[i];
+  auto b = s.dir[i];
+  p.u[0] = a.x;
+  p.u[1] = a.y;
+  p.u[2] = a.z;
+  p.s = a.w;
+  p.o[0] = b.x;
+  p.o[1] = b.y;
+  p.o[2] = b.z;
+  p.wmin = b.w;
+  p.w = s.w[i];
+  uint4 m = s.meta[i];
+  p.ijk = m.x;
+  p.id = m.y;
+  p.ctr = m.z;
+  p.flags = m.w;
+  return p;
+}
+
+template <typename Real>
+__device__ __forceinline__ void store_packet(const PacketSoA<Real> &s, uint32_t i, const Packet<Real> &p) {
+  typename V4<Real>::T a, b;
+  a.x = p.u[0];
+  a.y = p.u[1];
+  a.z = p.u[2];
+  a.w = p.s;
+  b.x = p.o[0];
+  b.y = p.o[1];
+  b.z = p.o[2];
+  b.w = p.wmin;
+  s.pos[i] = a;
+  s.dir[i] = b;
+  s.w[i] = p.w;
+  s.meta[i] = make_uint4(p.ijk, p.id, p.ctr, p.flags);
+}
+
+template <typename Real>
+__device__ __forceinline__ void store_census(const CensusSoA<Real> &c, uint32_t i, const Packet<Real> &p) {
+  typename V4<Real>::T a, b;
+  a.x = p.u[0];
+  a.y = p.u[1];
+  a.z = p.u[2];
+  a.w = Real(0);
+  b.x = p.o[0];
+  b.y = p.o[1];
+  b.z = p.o[2];
+  b.w = Real(0);
+  c.pos[i] = a;
+  c.dir[i] = b;
+  c.w[i] = p.w;
+  c.meta[i] = make_uint2(p.ijk, p.id);
+}
+
+// Per-cell integer tallies: pairs (lo sum, hi sum) of fixed-point limbs.
+struct Tallies {
+  unsigned long long *dep;   // [2N] material deposit (transport + roulette)
+  unsigned long long *pint;  // [2N] path integral of the weight, erg*cm
+  unsigned long long *esc;   // [2N] escaped through REMOVE faces
+  unsigned long long *cin;   // [2N] census weight entering transport
+  unsigned long long *cpre;  // [2N] census weight before population control
+  unsigned long long *cout;  // [2N] census weight after population control
+  unsigned long long *rem;   // [2N] comb remainders (go to the material)
+  double *dep_fp;            // [N]  FP64 engine: deposit
+  double *pint_fp;           // [N]  FP64 engine: path integral
+  double *esc_fp;            // [N]  FP64 engine: escaped
+  uint32_t *segs;            // [N]  tracked segments (cellsStepsCounters)
+  uint32_t *begin;           // [N]  packets per cell at transport start
+};
+
+__device__ __forceinline__ void tally_limbs(unsigned long long *t, uint32_t c, fx::Limbs v) {
+  if (v.lo) atomicAdd(&t[2 * c], static_cast<unsigned long long>(v.lo));
+  if (v.hi) atomicAdd(&t[2 * c + 1], static_cast<unsigned long long>(v.hi));
+}
+__device__ __forceinline__ void tally_limbs_n(unsigned long long *t, uint32_t c, fx::Limbs v, uint64_t n) {
+  if (v.lo) atomicAdd(&t[2 * c], static_cast<unsigned long long>(v.lo * n));
+  if (v.hi) atomicAdd(&t[2 * c + 1], static_cast<unsigned long long>(v.hi * n));
+}
+
+// Device material model (storm_material_desc).
+struct MatDev {
+  const double *sa0, *ss0, *rho;
+  double Tref, Tfloor, apow, arho, spow, srho;
+  double eos_f, eos_beta, eos_mu, cvTfloor;
+
+  __device__ double density(uint32_t c) const { return rho ? rho[c] : 1.0; }
+  __device__ double tfac(double T, double p) const {
+    if (p == 0.0) return 1.0;
+    return pow(fmax(T, Tfloor) / Tref, -p);
+  }
+  __device__ double rfac(uint32_t c, double p) const {
+    if (p == 0.0 || !rho) return 1.0;
+    return pow(rho[c], p);
+  }
+  __device__ double sigma_a(uint32_t c, double T) const { return sa0[c] * tfac(T, apow) * rfac(c, arho); }
+  __device__ double sigma_s(uint32_t c, double T) const {
+    return ss0 ? ss0[c] * tfac(T, spow) * rfac(c, srho) : 0.0;
+  }
+  __device__ double dT2cv(double rhoc, double T) const {
+    double cvVol = eos_f * eos_beta * pow(fmax(T, cvTfloor), eos_beta - 1.0) * pow(rhoc, 1.0 - eos_mu);
+    return cvVol / rhoc;
+  }
+  __device__ double de2T(double rhoc, double specific) const {
+    double eVol = specific * rhoc;
+    double ratio = fmax(eVol, 0.0) / (eos_f * pow(rhoc, 1.0 - eos_mu));
+    return pow(fmax(ratio, 0.0), 1.0 / eos_beta);
+  }
+};
+
+// Error flags raised by kernels (checked by the host after the step).
+enum : uint32_t { kErrFleck = 1u, kErrNegativeEnergy = 2u, kErrTallyOverflow = 4u, kErrCapacity = 8u };
+
+// ------------------------------------------------------------------ preStep
+template <typename Real>
+__global__ __launch_bounds__(256) void k_prestep(uint32_t N, const double *__restrict__ T, MatDev mat,
+                                                 const double *__restrict__ host_sa,
+                                                 const double *__restrict__ host_ss,
+                                                 const double *__restrict__ host_cv,
+                                                 const double *__restrict__ V, double dt,
+                                                 CellXS<Real> *__restrict__ xs, double *__restrict__ fleck,
+                                                 double *__restrict__ sig_a_out, double *__restrict__ sig_s_out,
+                                                 double *__restrict__ Eemit, uint32_t *__restrict__ err) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const double Tc = T[c];
+  const double sa = host_sa ? host_sa[c] : mat.sigma_a(c, Tc);
+  const double ss = host_ss ? host_ss[c] : mat.sigma_s(c, Tc);
+  const double cv = host_cv ? host_cv[c] : mat.dT2cv(mat.density(c), Tc);
+  const double T3 = Tc * Tc * Tc;
+  const double f = 1.0 / (1.0 + (4.0 * units::arad * T3 * sa * units::clight * dt) / cv);
+  if (!(f >= 0.0 && f <= 1.0)) atomicOr(err, kErrFleck);
+  fleck[c] = f;
+  sig_a_out[c] = sa;
+  sig_s_out[c] = ss;
+  Eemit[c] = f * V[c] * units::arad * (T3 * Tc) * sa * dt * units::clight;
+  CellXS<Real> x;
+  x.sig_abs = static_cast<Real>(f * sa);
+  x.sig_ev = static_cast<Real>(ss + (1.0 - f) * sa);
+  xs[c] = x;
+}
+
+// photons per cell and packet weight, quantised down to the step quantum
+__global__ __launch_bounds__(256) void k_alloc(uint32_t N, const double *__restrict__ Eemit, double Etot,
+                                               uint64_t n_min, int32_t eq, uint32_t *__restrict__ nphot,
+                                               double *__restrict__ wemit,
+                                               unsigned long long *__restrict__ emit_limbs) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const double E = Eemit[c];
+  const uint64_t total = static_cast<uint64_t>(N) * n_min * 10ull;
+  uint64_t share = Etot > 0.0 ? static_cast<uint64_t>(E / Etot * static_cast<double>(total)) : n_min;
+  uint64_t n = 0;
+  if (E > 0.0) {
+    n = share < 20ull * n_min ? share : 20ull * n_min;
+    if (n < n_min) n = n_min;
+  }
+  double w = n ? fx::quantize_down(E / static_cast<double>(n), eq) : 0.0;
+  if (w == 0.0) n = 0;  // below one quantum: nothing to emit
+  nphot[c] = static_cast<uint32_t>(n);
+  wemit[c] = w;
+  fx::Limbs L = n ? fx::limbs_of(w, eq) : fx::Limbs{0, 0};
+  emit_limbs[2 * c] = L.lo * n;
+  emit_limbs[2 * c + 1] = L.hi * n;
+}
+
+// ------------------------------------------------------------------ sourcing
+template <typename Real>
+__device__ __forceinline__ Real uni(uint32_t x) {
+  return MathPolicy<Real>::uniform(x);
+}
+
+struct MeshDims {
+  uint32_t nx, ny, nz;
+};
+
+__device__ __forceinline__ uint32_t ijk_of_flat(uint32_t c, MeshDims d) {
+  uint32_t k = c % d.nz;
+  uint32_t t = c / d.nz;
+  uint32_t j = t % d.ny;
+  uint32_t i = t / d.ny;
+  return pack_ijk(i, j, k);
+}
+
+// Census re-entry: timeLeft = dt, initialWeight = |w| (MonteCarloManagerSerial.hpp:393-405).
+// Weights are re-quantised if the quantum grew; the remainder (in the old
+// quantum) goes to the material.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_import(uint32_t n, CensusSoA<Real> cen, PacketSoA<Real> out,
+                                                Real s0, int32_t eq_old, int32_t eq_new, MeshDims d,
+                                                Tallies t, unsigned long long *__restrict__ requant) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Packet<Real> p;
+  auto a = cen.pos[i];
+  auto b = cen.dir[i];
+  uint2 m = cen.meta[i];
+  p.u[0] = a.x;
+  p.u[1] = a.y;
+  p.u[2] = a.z;
+  p.s = s0;
+  p.o[0] = b.x;
+  p.o[1] = b.y;
+  p.o[2] = b.z;
+  double w = cen.w[i];
+  uint32_t c = (ijk_i(m.x) * d.ny + ijk_j(m.x)) * d.nz + ijk_k(m.x);
+  if (eq_new > eq_old) {
+    double wq = fx::quantize_down(w, eq_new);
+    if (wq != w) tally_limbs(requant, c, fx::limbs_of(w - wq, eq_old));  // exact in the old quantum
+    w = wq;
+  }
+  p.w = w;
+  p.wmin = static_cast<Real>(w * 1e-3);
+  p.ijk = m.x;
+  p.id = i;
+  p.ctr = 0u;
+  p.flags = 0u;
+  store_packet(out, i, p);
+  tally_limbs(t.cin, c, fx::limbs_of(w, eq_new));
+  atomicAdd(&t.begin[c], 1u);
+}
+
+// Emission packets: uniform position in the cell (RandomInCellPositionSampler
+// is uniform in the box), isotropic direction, timeLeft = dt * U.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_emit(uint32_t n_emit, uint32_t N, const uint64_t *__restrict__ offsets,
+                                              const double *__restrict__ wemit, PacketSoA<Real> out,
+                                              uint32_t base, Real s0, MeshDims d, uint64_t seed,
+                                              uint32_t step, uint32_t *__restrict__ begin) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_emit) return;
+  // cell owning packet q: last c with offsets[c] <= q
+  uint32_t lo = 0, hi = N;  // offsets[N] = n_emit
+  while (hi - lo > 1u) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (offsets[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t c = lo;
+  Packet<Real> p;
+  birth_emission<Real>(p, base + q, ijk_of_flat(c, d), wemit[c], s0, seed, step);
+  store_packet(out, base + q, p);
+  atomicAdd(&begin[c], 1u);
+}
+
+// Planck surface source on the x- box face: per face Npc packets of weight
+// sigma_SB T^4 A dt / Npc, uniform on the face, mu = sqrt(U) about +x.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_boundary(uint32_t n_bc, uint32_t npc, const double *__restrict__ wface,
+                                                  PacketSoA<Real> out, uint32_t base, Real s0, MeshDims d,
+                                                  uint64_t seed, uint32_t step, uint32_t *__restrict__ begin) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_bc) return;
+  const uint32_t f = q / npc;  // face index j * nz + k
+  const uint32_t j = f / d.nz, k = f % d.nz;
+  Packet<Real> p;
+  birth_boundary<Real>(p, base + q, j, k, wface[f], s0, seed, step);
+  store_packet(out, base + q, p);
+  atomicAdd(&begin[(0u * d.ny + j) * d.nz + k], 1u);
+}
+
+// ------------------------------------------------------------------ transport
+struct DeviceSink {
+  Tallies t;
+  int32_t eq;
+  __device__ void visit(uint32_t c, fx::Limbs dep, fx::Limbs pint, uint32_t segs) {
+    tally_limbs(t.dep, c, dep);
+    tally_limbs(t.pint, c, pint);
+    atomicAdd(&t.segs[c], segs);
+  }
+  __device__ void visit_fp(uint32_t c, double dep, double pint, uint32_t segs) {
+    if (dep != 0.0) atomicAdd(&t.dep_fp[c], dep);
+    if (pint != 0.0) atomicAdd(&t.pint_fp[c], pint);
+    atomicAdd(&t.segs[c], segs);
+  }
+  __device__ void escape(uint32_t c, double w) {
+    if (t.esc_fp) {
+      atomicAdd(&t.esc_fp[c], w);
+    } else {
+      tally_limbs(t.esc, c, fx::limbs_of(w, eq));
+    }
+  }
+};
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out,
+                                                   uint32_t *__restrict__ n_out, CensusSoA<Real> cen,
+                                                   uint32_t *__restrict__ n_cen, MeshView<Real> mesh,
+                                                   const CellXS<Real> *__restrict__ xs, StepConsts k,
+                                                   int32_t max_segments, Tallies t,
+                                                   unsigned long long *__restrict__ leaving) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n_in;
+  int outcome = kGone;
+  Packet<Real> p;
+  if (valid) {
+    p = load_packet(in, i);
+    DeviceSink sink{t, k.eq};
+    outcome = transport_packet<Real>(p, mesh, xs, k, max_segments, sink);
+  }
+  const bool alive = valid && outcome == kAlive;
+  const bool census = valid && outcome == kCensus;
+  uint32_t slot = wave_append(alive, n_out);
+  if (alive) store_packet(out, slot, p);
+  uint32_t cslot = wave_append(census, n_cen);
+  if (census) {
+    store_census(cen, cslot, p);
+    uint32_t c = flat_cell(mesh, p.ijk);
+    tally_limbs(t.cpre, c, fx::limbs_of(p.w, k.eq));
+  }
+  const uint64_t esc_mask = __builtin_amdgcn_ballot_w64(valid && outcome == kEscaped);
+  if (esc_mask && lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u)
+    atomicAdd(leaving, static_cast<unsigned long long>(__popcll(esc_mask)));
+}
+
+// ------------------------------------------------------------------ comb
+// Bijective 32-bit mix (odd multipliers and xor-shifts are invertible on
+// 2^32): gives every census packet a distinct pseudo-random rank within its
+// cell, replacing the reference's sort-by-id + std::shuffle
+// (CombPopulationControl.hpp:196-202) by a deterministic permutation.
+__device__ __forceinline__ uint32_t perm32(uint32_t x, uint32_t salt) {
+  x ^= salt;
+  x *= 0x9E3779B1u;
+  x ^= x >> 16;
+  x *= 0x85EBCA77u;
+  x ^= x >> 13;
+  x *= 0xC2B2AE3Du;
+  x ^= x >> 16;
+  return x;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_comb_keys(uint32_t n, CensusSoA<Real> cen, MeshDims d, uint32_t N,
+                                                   uint32_t salt, int32_t eq, uint64_t *__restrict__ keys,
+                                                   uint32_t *__restrict__ vals, uint32_t *__restrict__ cnt,
+                                                   unsigned long long *__restrict__ W) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint2 m = cen.meta[i];
+  double w = cen.w[i];
+  uint32_t c = (ijk_i(m.x) * d.ny + ijk_j(m.x)) * d.nz + ijk_k(m.x);
+  vals[i] = i;
+  if (w == 0.0) {  // weight-0 packets are dropped (CombPopulationControl.hpp:136-137)
+    keys[i] = static_cast<uint64_t>(N) << 32;
+    return;
+  }
+  keys[i] = (static_cast<uint64_t>(c) << 32) | perm32(m.y, salt);
+  atomicAdd(&cnt[c], 1u);
+  tally_limbs(W, c, fx::limbs_of(w, eq));
+}
+
+struct CombCell {
+  double w_out;     // COMB: tooth weight; SPLIT: ideal weight W_c / N_c
+  double r;         // comb offset in (0,1)
+  double Wd;        // W_c as double
+  uint32_t target;  // N_c
+  uint32_t mode;    // 0 empty, 1 split/pass-through, 2 comb
+};
+
+__device__ __forceinline__ unsigned __int128 limbs_value(const unsigned long long *W, uint32_t c) {
+  return (static_cast<unsigned __int128>(W[2 * c + 1]) << fx::kLoBits) + static_cast<unsigned __int128>(W[2 * c]);
+}
+
+// Largest integer <= v with at most 53 significant bits (exactly representable
+// as a double).
+__device__ __forceinline__ unsigned __int128 floor53(unsigned __int128 v) {
+  const uint64_t hi = static_cast<uint64_t>(v >> 64), lo = static_cast<uint64_t>(v);
+  const int bits = hi ? 128 - __clzll(hi) : (lo ? 64 - __clzll(lo) : 0);
+  if (bits <= 53) return v;
+  const int drop = bits - 53;
+  return (v >> drop) << drop;
+}
+
+// Exact double of an integer with at most 53 significant bits.
+__device__ __forceinline__ double u128_exact_double(unsigned __int128 v) {
+  return static_cast<double>(static_cast<uint64_t>(v >> 64)) * 18446744073709551616.0 +
+         static_cast<double>(static_cast<uint64_t>(v));
+}
+
+// Remainder limbs of W - q * n (requires q * n <= W).
+__device__ __forceinline__ fx::Limbs remainder_limbs(unsigned __int128 W, unsigned __int128 q, uint64_t n) {
+  unsigned __int128 rv = W - q * static_cast<unsigned __int128>(n);
+  fx::Limbs rl;
+  rl.lo = static_cast<uint64_t>(rv) & fx::kLoMask;
+  rl.hi = static_cast<uint64_t>(rv >> fx::kLoBits);
+  return rl;
+}
+
+// Per-cell comb targets (CombPopulationControl.hpp:157-206).
+__global__ __launch_bounds__(256) void k_comb_cells(uint32_t N, const uint32_t *__restrict__ cnt,
+                                                    const unsigned long long *__restrict__ W, double Wtot,
+                                                    uint64_t nmin, double factor, int32_t eq, int32_t comb_on,
+                                                    uint64_t seed, uint32_t step, CombCell *__restrict__ cells,
+                                                    unsigned long long *__restrict__ rem) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  CombCell cc;
+  cc.w_out = 0.0;
+  cc.r = 0.5;
+  cc.target = 0;
+  cc.mode = 0;
+  const uint32_t count = cnt[c];
+  const unsigned __int128 Wc = limbs_value(W, c);
+  const double Wd = static_cast<double>(static_cast<uint64_t>(Wc >> 64)) * 18446744073709551616.0 +
+                    static_cast<double>(static_cast<uint64_t>(Wc)) ;
+  cc.Wd = Wd * fpm::pow2d(eq);
+  if (count == 0u || !comb_on) {
+    cc.mode = count ? 1u : 0u;
+    cc.target = count;
+    cc.w_out = 1e308;  // never split
+    cells[c] = cc;
+    return;
+  }
+  const uint64_t Ntotal = static_cast<uint64_t>(static_cast<double>(static_cast<uint64_t>(N) * nmin) * factor);
+  uint64_t share = static_cast<uint64_t>(static_cast<double>(Ntotal) * cc.Wd / Wtot);
+  uint64_t NinCell = share > nmin ? share : nmin;
+  if (NinCell > 20ull * nmin) NinCell = 20ull * nmin;
+  cc.target = static_cast<uint32_t>(NinCell);
+  if (count <= NinCell) {
+    cc.mode = 1u;
+    cc.w_out = cc.Wd / static_cast<double>(NinCell);  // weight_ideal
+  } else {
+    cc.mode = 2u;
+    // tooth weight: floor(W_c / N_c) quanta, cut to 53 bits so it is a
+    // double; the remainder W_c - N_c * tooth goes to the material
+    const unsigned __int128 q = floor53(Wc / static_cast<unsigned __int128>(NinCell));
+    cc.w_out = u128_exact_double(q) * fpm::pow2d(eq);
+    tally_limbs(rem, c, remainder_limbs(Wc, q, NinCell));
+    PacketStream rs = PacketStream::make(seed, c, step, kTagComb);
+    Philox4 r = rs.block(0);
+    cc.r = u01_f64(r.v[0], r.v[1]);
+  }
+  cells[c] = cc;
+}
+
+__device__ __forceinline__ double u128_to_double(unsigned __int128 v) {
+  return static_cast<double>(static_cast<uint64_t>(v >> 64)) * 18446744073709551616.0 +
+         static_cast<double>(static_cast<uint64_t>(v));
+}
+
+__device__ __forceinline__ uint64_t comb_teeth(double C, double Wc, double target, double r) {
+  // #{k >= 0 : (k + r) * W/N < C} = ceil(C*N/W - r), clamped to [0, N]
+  double x = ceil(C * target / Wc - r);
+  if (!(x > 0.0)) return 0;
+  if (x > target) return static_cast<uint64_t>(target);
+  return static_cast<uint64_t>(x);
+}
+
+// Output copies per sorted census packet.  `cum` holds inclusive prefix sums
+// of the weight limbs over the sorted order; `start` the first sorted index
+// of each cell.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_comb_count(uint32_t n_valid, const uint64_t *__restrict__ keys,
+                                                    const uint32_t *__restrict__ vals, CensusSoA<Real> cen,
+                                                    const CombCell *__restrict__ cells,
+                                                    const uint32_t *__restrict__ start,
+                                                    const uint32_t *__restrict__ cnt,
+                                                    const ulonglong2 *__restrict__ cum, int32_t eq,
+                                                    uint32_t *__restrict__ ncopy) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_valid) return;
+  const uint32_t c = static_cast<uint32_t>(keys[i] >> 32);
+  const CombCell cc = cells[c];
+  const double w = cen.w[vals[i]];
+  uint32_t n = 1;
+  if (cc.mode == 1u) {
+    if (w > 2.0 * cc.w_out) n = static_cast<uint32_t>(ceil(w / cc.w_out));
+  } else if (cc.mode == 2u) {
+    const uint32_t s = start[c];
+    const bool last = (i + 1u == s + cnt[c]);
+    ulonglong2 a = cum[i];
+    unsigned __int128 Cin = (static_cast<unsigned __int128>(a.y) << fx::kLoBits) + a.x;
+    unsigned __int128 Cbase = 0;
+    if (s > 0u) {
+      ulonglong2 b = cum[s - 1u];
+      Cbase = (static_cast<unsigned __int128>(b.y) << fx::kLoBits) + b.x;
+    }
+    fx::Limbs wl = fx::limbs_of(w, eq);
+    unsigned __int128 wv = (static_cast<unsigned __int128>(wl.hi) << fx::kLoBits) + wl.lo;
+    unsigned __int128 Cincl = Cin - Cbase;
+    unsigned __int128 Cexcl = Cincl - wv;
+    const double Wc = cc.Wd * fpm::pow2d(-eq);
+    const double tgt = static_cast<double>(cc.target);
+    uint64_t k1 = last ? cc.target : comb_teeth(u128_to_double(Cincl), Wc, tgt, cc.r);
+    uint64_t k0 = comb_teeth(u128_to_double(Cexcl), Wc, tgt, cc.r);
+    n = static_cast<uint32_t>(k1 > k0 ? k1 - k0 : 0);
+  }
+  ncopy[i] = n;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_comb_write(uint32_t n_valid, const uint64_t *__restrict__ keys,
+                                                    const uint32_t *__restrict__ vals, CensusSoA<Real> cen,
+                                                    const CombCell *__restrict__ cells,
+                                                    const uint32_t *__restrict__ ncopy,
+                                                    const uint64_t *__restrict__ offs, CensusSoA<Real> out,
+                                                    int32_t eq, Tallies t) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_valid) return;
+  const uint32_t n = ncopy[i];
+  if (n == 0u) return;
+  const uint32_t c = static_cast<uint32_t>(keys[i] >> 32);
+  const CombCell cc = cells[c];
+  const uint32_t src = vals[i];
+  const double w = cen.w[src];
+  double wout = w;
+  if (cc.mode == 2u) {
+    wout = cc.w_out;
+  } else if (n > 1u) {
+    // split copies: floor(w / n) quanta each, remainder to the material
+    fx::Limbs wl = fx::limbs_of(w, eq);
+    const unsigned __int128 wv = (static_cast<unsigned __int128>(wl.hi) << fx::kLoBits) + wl.lo;
+    const unsigned __int128 q = floor53(wv / static_cast<unsigned __int128>(n));
+    wout = u128_exact_double(q) * fpm::pow2d(eq);
+    tally_limbs(t.rem, c, remainder_limbs(wv, q, n));
+  }
+  const auto a = cen.pos[src];
+  const auto b = cen.dir[src];
+  const uint2 m = cen.meta[src];
+  const uint64_t o = offs[i];
+  for (uint32_t q = 0; q < n; ++q) {
+    out.pos[o + q] = a;
+    out.dir[o + q] = b;
+    out.w[o + q] = wout;
+    out.meta[o + q] = make_uint2(m.x, static_cast<uint32_t>(o + q));
+  }
+  tally_limbs_n(t.cout, c, fx::limbs_of(wout, eq), n);
+}
+
+// (lo, hi) limbs of each sorted census weight, for the prefix sums
+template <typename Real>
+__global__ __launch_bounds__(256) void k_comb_limbs(uint32_t n_valid, const uint32_t *__restrict__ vals,
+                                                    CensusSoA<Real> cen, int32_t eq,
+                                                    ulonglong2 *__restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_valid) return;
+  fx::Limbs l = fx::limbs_of(cen.w[vals[i]], eq);
+  out[i] = make_ulonglong2(l.lo, l.hi);
+}
+
+// ------------------------------------------------------------------ postStep
+template <typename Real>
+__global__ __launch_bounds__(256) void k_post(uint32_t N, Tallies t, int32_t eq, int32_t et, int32_t eq_old,
+                                              const unsigned long long *__restrict__ emit_limbs,
+                                              const unsigned long long *__restrict__ requant, double dt,
+                                              const double *__restrict__ V, MatDev mat, int32_t device_eos,
+                                              double *__restrict__ T, double *__restrict__ e_mat,
+                                              double *__restrict__ e_sync, double *__restrict__ erad_avg,
+                                              double *__restrict__ census_erad, double *__restrict__ clamp,
+                                              uint32_t *__restrict__ err, int32_t fp64) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const double emit = fx::to_double(emit_limbs[2 * c], emit_limbs[2 * c + 1], eq);
+  double dep, pint;
+  if (fp64) {
+    dep = t.dep_fp[c];
+    pint = t.pint_fp[c];
+  } else {
+    dep = fx::to_double(t.dep[2 * c], t.dep[2 * c + 1], eq);
+    pint = fx::to_double(t.pint[2 * c], t.pint[2 * c + 1], et);
+  }
+  dep += fx::to_double(t.rem[2 * c], t.rem[2 * c + 1], eq);
+  dep += fx::to_double(requant[2 * c], requant[2 * c + 1], eq_old);
+  const double Vc = V[c];
+  erad_avg[c] = pint / (units::clight * dt * Vc);  // RadiationIMC.hpp:1895 (time integral = path / c)
+  census_erad[c] = fx::to_double(t.cout[2 * c], t.cout[2 * c + 1], eq);
+  double E = (e_mat[c] - emit) + dep;
+  double cl = 0.0;
+  if (E < 0.0) {  // throwIfNegativeInternalEnergy (RadiationIMC.hpp:771-817)
+    const double Tc = T[c];
+    double thermal = units::arad * (Tc * Tc * Tc * Tc) * Vc;
+    if (thermal < 1e-30) thermal = 1e-30;
+    if (fabs(E) / thermal < 0.1) {
+      double ne = fmax(0.0, e_sync[c]);
+      cl = ne - E;
+      E = ne;
+    } else {
+      atomicOr(err, kErrNegativeEnergy);
+    }
+  }
+  clamp[c] = cl;
+  e_mat[c] = E;
+  e_sync[c] = E;
+  if (device_eos) {
+    const double rho = mat.density(c);
+    T[c] = mat.de2T(rho, E / (rho * Vc));  // synchronizeMaterialCell (:830-866)
+  }
+  if (t.segs[c] >= (1u << 28)) atomicOr(err, kErrTallyOverflow);
+}
+
+__global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f64(double *p, uint64_t n, double v) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace gpu
+}  // namespace storm

@@ -1,0 +1,220 @@
+"""Problem descriptions (storm_problem.h) for the gray IMC workloads.
+
+Each builder returns a `ProblemSpec` holding numpy arrays and scalars; call
+`.to_c()` for the ctypes `Problem` (the spec keeps the arrays alive).
+
+Workloads (all CGS, T in K):
+  c1_slab   gray slab 100x1x1 over [0,1] cm, sigma_a = 1, cv = 1e15 erg/(cm^3 keV),
+            T0 = 1e-3 keV, SideTemperature 1 keV at x- (escape there, reflect
+            elsewhere), comb population control, dt = 1e-11 s.
+  c2_cube   gray 64^3 unit cube, sigma_a = sigma0_c * max(T, 0.1 keV)^-3 (T in keV),
+            sigma0_c = 10^(2U) per cell, rho = 1, cv = 2 a (1 keV)^2 T (so that
+            e(1 keV) = a (1 keV)^4), T0 = 1e-3 keV, 1 keV source at x-, every box
+            face escapes, ~1e7 packets per step, dt = 1e-11 s.
+  marshak   the power-law Marshak-wave family of the reference examples
+            (examples/marshak_wave/MarshakCommon.hpp:48-172), gray, no random walk.
+The 0.1 keV opacity floor of c2_cube is a recorded choice: at the literal
+1e-3 keV start the opacity sigma0 * 1e9 cm^-1 makes every cell an effective
+scatterer with ~1e4 events per absorption length.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+
+CLIGHT = 2.99792458e10
+K_BOLTZ = 1.380649e-16
+KEV = 1.602176634e-9
+KEV_K = KEV / K_BOLTZ
+SIGMA_SB = 5.670374419e-5
+ARAD = 4.0 * SIGMA_SB / CLIGHT
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _ptr(a):
+    if a is None:
+        return C.POINTER(C.c_double)()
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@dataclass
+class ProblemSpec:
+    name: str
+    x_edges: np.ndarray
+    y_edges: np.ndarray
+    z_edges: np.ndarray
+    sigma_a0: np.ndarray
+    T0: np.ndarray
+    dt: float
+    steps: int
+    sigma_s0: np.ndarray | None = None
+    density: np.ndarray | None = None
+    T_ref: float = 1.0
+    T_floor: float = 0.0
+    a_pow: float = 0.0
+    a_rho: float = 0.0
+    s_pow: float = 0.0
+    s_rho: float = 0.0
+    eos_f: float = 1.0
+    eos_beta: float = 1.0
+    eos_mu: float = 0.0
+    eos_cv_floor: float = 0.0
+    face_kind: tuple = (capi.FACE_ESCAPE,) + (capi.FACE_REFLECT,) * 5
+    source_face: int = capi.XLO
+    source_T: float = KEV_K
+    freeze_T4: int = 1
+    source_per_face: int = 100
+    new_photons_per_cell: int = 10
+    comb: int = 1
+    comb_nmin: int = 20
+    comb_factor: float = 2.0
+    with_rw: int = 0
+    with_ddmc: int = 0
+    rw_cell_tau: float = 25.0
+    rw_particle_tau: float = 5.0
+    ddmc_cell_tau: float = 15.0
+    ddmc_particle_tau: float = 5.0
+    seed: int = 42
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def shape(self):
+        return (len(self.x_edges) - 1, len(self.y_edges) - 1, len(self.z_edges) - 1)
+
+    @property
+    def ncells(self) -> int:
+        nx, ny, nz = self.shape
+        return nx * ny * nz
+
+    def volumes(self) -> np.ndarray:
+        dx = np.diff(self.x_edges)[:, None, None]
+        dy = np.diff(self.y_edges)[None, :, None]
+        dz = np.diff(self.z_edges)[None, None, :]
+        return (dx * dy * dz).reshape(-1)
+
+    def centers(self):
+        xc = 0.5 * (self.x_edges[1:] + self.x_edges[:-1])
+        yc = 0.5 * (self.y_edges[1:] + self.y_edges[:-1])
+        zc = 0.5 * (self.z_edges[1:] + self.z_edges[:-1])
+        X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
+        return X.reshape(-1), Y.reshape(-1), Z.reshape(-1)
+
+    def with_seed(self, seed: int) -> "ProblemSpec":
+        import copy
+        s = copy.copy(self)
+        s.seed = int(seed)
+        s._keep = []
+        return s
+
+    def to_c(self) -> capi.Problem:
+        arrs = [_f64(self.x_edges), _f64(self.y_edges), _f64(self.z_edges), _f64(self.sigma_a0),
+                None if self.sigma_s0 is None else _f64(self.sigma_s0),
+                None if self.density is None else _f64(self.density), _f64(self.T0)]
+        self._keep = arrs
+        xe, ye, ze, sa, ss, rho, t0 = arrs
+        p = capi.Problem()
+        nx, ny, nz = self.shape
+        p.mesh = capi.MeshDesc(nx, ny, nz, 0, _ptr(xe), _ptr(ye), _ptr(ze))
+        m = capi.MaterialDesc()
+        m.sigma_a0, m.sigma_s0, m.density = _ptr(sa), _ptr(ss), _ptr(rho)
+        m.opacity_T_ref, m.opacity_T_floor = self.T_ref, self.T_floor
+        m.a_temp_pow, m.a_rho_pow, m.s_temp_pow, m.s_rho_pow = self.a_pow, self.a_rho, self.s_pow, self.s_rho
+        m.eos_f, m.eos_beta, m.eos_mu, m.eos_cv_T_floor = self.eos_f, self.eos_beta, self.eos_mu, self.eos_cv_floor
+        p.material = m
+        b = capi.BoundaryDesc()
+        for f in range(6):
+            b.face_kind[f] = int(self.face_kind[f])
+        b.source_face = int(self.source_face)
+        b.freeze_source_T4 = int(self.freeze_T4)
+        b.source_temperature = float(self.source_T)
+        b.source_packets_per_face = int(self.source_per_face)
+        p.boundary = b
+        q = capi.ImcParams()
+        q.new_photons_per_cell = int(self.new_photons_per_cell)
+        q.with_random_walk, q.with_ddmc = int(self.with_rw), int(self.with_ddmc)
+        q.rw_min_cell_optical_depth, q.rw_min_particle_optical_depth = self.rw_cell_tau, self.rw_particle_tau
+        q.ddmc_min_cell_optical_depth, q.ddmc_min_particle_optical_depth = self.ddmc_cell_tau, self.ddmc_particle_tau
+        q.comb, q.comb_nmin, q.comb_factor = int(self.comb), int(self.comb_nmin), float(self.comb_factor)
+        p.imc = q
+        p.T0 = _ptr(t0)
+        p.seed = int(self.seed)
+        return p
+
+
+def c1_slab(seed: int = 0, packets_scale: float = 1.0, steps: int = 100, nx: int = 100) -> ProblemSpec:
+    """100x1x1 gray slab, ~1e5 packets per step at packets_scale = 1."""
+    n = nx
+    xe = np.linspace(0.0, 1.0, n + 1)
+    h = 1.0 / n
+    ye = np.array([0.0, h])
+    ze = np.array([0.0, h])
+    s = max(packets_scale, 1e-6)
+    return ProblemSpec(
+        name="c1_slab", x_edges=xe, y_edges=ye, z_edges=ze,
+        sigma_a0=np.ones(n), T0=np.full(n, 1e-3 * KEV_K), dt=1e-11, steps=steps,
+        eos_f=1e15 / KEV_K, eos_beta=1.0, eos_mu=0.0,
+        face_kind=(capi.FACE_ESCAPE,) + (capi.FACE_REFLECT,) * 5,
+        source_face=capi.XLO, source_T=KEV_K, freeze_T4=1,
+        source_per_face=max(1, int(round(25000 * s))),
+        new_photons_per_cell=max(1, int(round(25 * s))),
+        comb=1, comb_nmin=max(1, int(round(250 * s))), comb_factor=2.0, seed=seed)
+
+
+def c2_cube(seed: int = 0, n: int = 64, packets_scale: float = 1.0, steps: int = 50) -> ProblemSpec:
+    """n^3 gray cube with sigma_a = sigma0 T^-3; ~1e7 packets per step at n = 64."""
+    xe = np.linspace(0.0, 1.0, n + 1)
+    N = n ** 3
+    rng = np.random.default_rng(np.random.SeedSequence([int(seed), 0xC2]))
+    sigma0 = 10.0 ** (2.0 * rng.random(N))
+    s = max(packets_scale, 1e-6)
+    scale_cells = (64.0 / n) ** 3  # keep packets per cell comparable on smaller meshes
+    return ProblemSpec(
+        name="c2_cube", x_edges=xe, y_edges=xe.copy(), z_edges=xe.copy(),
+        sigma_a0=sigma0, T0=np.full(N, 1e-3 * KEV_K), dt=1e-11, steps=steps,
+        T_ref=KEV_K, T_floor=0.1 * KEV_K, a_pow=3.0,
+        eos_f=ARAD * KEV_K ** 2, eos_beta=2.0, eos_mu=0.0,
+        face_kind=(capi.FACE_ESCAPE,) * 6, source_face=capi.XLO, source_T=KEV_K, freeze_T4=0,
+        source_per_face=max(1, int(round(500 * s * min(scale_cells, 64.0) ** (1 / 3)))),
+        new_photons_per_cell=max(1, int(round(2 * s))),
+        comb=1, comb_nmin=max(1, int(round(10 * s))), comb_factor=1.0, seed=seed)
+
+
+MARSHAK = {
+    1: dict(L=0.2, kP0=0.1, kR0=100.0, alpha=3.0, beta=0.0, f=6.860085e14 / KEV_K ** 4, eb=4.0, mu=0.0,
+            Tc=1.008038, Te=1.0 / 3.0),
+    2: dict(L=0.2, kP0=100.0, kR0=100.0, alpha=3.0, beta=0.0, f=6.860085e14 / KEV_K ** 4, eb=4.0, mu=0.0,
+            Tc=1.014565, Te=1.0 / 3.0),
+}
+
+
+def marshak(problem: int = 2, nx: int = 64, seed: int = 0, new_photons: int = 15, boundary_photons: int = 100,
+            steps: int = 10, dt: float = 1e-12) -> ProblemSpec:
+    """Marshak wave 1 or 2 (examples/marshak_wave/MarshakCommon.hpp:48-172), gray, no random walk."""
+    P = MARSHAK[problem]
+    L = P["L"]
+    xe = np.linspace(0.0, L, nx + 1)
+    h = L / nx
+    T0 = 1e-3 * KEV_K
+    return ProblemSpec(
+        name=f"marshak{problem}", x_edges=xe, y_edges=np.array([0.0, h]), z_edges=np.array([0.0, h]),
+        sigma_a0=np.full(nx, P["kP0"]), sigma_s0=np.full(nx, max(P["kR0"] - P["kP0"], 0.0)),
+        T0=np.full(nx, T0), dt=dt, steps=steps,
+        T_ref=KEV_K, T_floor=1.0, a_pow=P["alpha"], s_pow=P["alpha"],
+        eos_f=P["f"], eos_beta=P["eb"], eos_mu=P["mu"], eos_cv_floor=1.0,
+        face_kind=(capi.FACE_ESCAPE,) + (capi.FACE_REFLECT,) * 5, source_face=capi.XLO,
+        source_T=P["Tc"] * (1e-15 * 1e9) ** P["Te"] * KEV_K, freeze_T4=0,
+        source_per_face=boundary_photons, new_photons_per_cell=new_photons,
+        comb=1, comb_nmin=15, comb_factor=6.0, seed=seed)
+
+
+def marshak_bath_T(problem: int, t: float) -> float:
+    P = MARSHAK[problem]
+    t_ns = max(t, 1e-20) * 1e9
+    return P["Tc"] * t_ns ** P["Te"] * KEV_K

@@ -1,0 +1,21 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs libstorm_amd.so kernels)")
+    config.addinivalue_line("markers", "slow: CPU test that takes more than ~20 s")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import pyoracle
+    if not os.path.exists(pyoracle.LIB):
+        pyoracle.build()
+    return pyoracle

@@ -1,0 +1,206 @@
+"""CPU tests of the checkers: the FP64 oracle and the CPU replay.
+
+What pins each oracle function (no reference code is imported or run: the
+reference needs Boost, `units`, `planck_integral` and MadCart, none of which
+exist here):
+  * Philox4x32-10: the three published Random123 known-answer vectors.
+  * Fleck factor / emission energy (RadiationIMC.hpp:1649, 2058): values
+    evaluated by hand for fixed inputs, plus f = 1/2 when cv = 4 a T^3 sigma c dt.
+  * photon allocation (:2076-2087): bounds [n, 20n] and the uniform case 10 n.
+  * transport + surface source + continuous absorption: slab transmission
+    2 E_3(tau) of a cosine-law source through a pure absorber.
+  * energy bookkeeping of a whole step: balance to 1e-12 relative.
+  * comb (CombPopulationControl.hpp:79-253): exact per-cell energy and the
+    target count.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy import special
+
+from storm_amd import capi
+from storm_amd import problems as P
+from tests.stats import check_ensembles
+
+
+def test_philox_known_answers(oracle):
+    L = oracle.rlib()
+    L.replay_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
+    cases = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, want in cases:
+        c = (C.c_uint32 * 4)(*ctr)
+        k = (C.c_uint32 * 2)(*key)
+        o = (C.c_uint32 * 4)()
+        L.replay_philox(c, k, o)
+        assert tuple(o) == want
+
+
+def test_fp32_math_accuracy(oracle):
+    L = oracle.rlib()
+    L.replay_fpmath.argtypes = [C.c_int32, C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    rng = np.random.default_rng(0)
+
+    def run(which, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty_like(x)
+        L.replay_fpmath(which, len(x), x.ctypes.data_as(C.POINTER(C.c_float)), y.ctypes.data_as(C.POINTER(C.c_float)))
+        return x.astype(np.float64), y.astype(np.float64)
+
+    x, y = run(0, rng.random(200000) * 10.0 ** rng.uniform(-30, 30, 200000))
+    ref = np.log(x)
+    m = np.abs(ref) > 1e-3
+    assert np.max(np.abs(y[m] - ref[m]) / np.abs(ref[m])) < 5e-7
+    x, y = run(1, -rng.random(200000) * 87)
+    assert np.max(np.abs(y - np.exp(x)) / np.exp(x)) < 5e-7
+    x, y = run(2, -rng.random(200000) * 10.0 ** rng.uniform(-8, 1, 200000))
+    ref = np.expm1(x)
+    assert np.max(np.abs(y - ref) / np.abs(ref)) < 5e-7
+    x, y = run(3, rng.random(100000))
+    assert np.max(np.abs(y - np.sin(2 * np.pi * x))) < 3e-7
+    x, y = run(4, rng.random(100000))
+    assert np.max(np.abs(y - np.cos(2 * np.pi * x))) < 3e-7
+    # special points
+    x, y = run(0, [1.0, 2.0, 0.5])
+    assert y[0] == 0.0 and abs(y[1] - math.log(2)) < 1e-7
+    x, y = run(3, [0.0, 0.25, 0.5, 0.75])
+    assert np.allclose(y, [0, 1, 0, -1], atol=1e-7)
+
+
+def _one_cell(T_keV, sigma, cv_per_K, dt=1e-11, V_side=0.01, n=10):
+    h = V_side
+    return P.ProblemSpec(
+        name="cell", x_edges=np.array([0.0, h]), y_edges=np.array([0.0, h]), z_edges=np.array([0.0, h]),
+        sigma_a0=np.array([sigma]), T0=np.array([T_keV * P.KEV_K]), dt=dt, steps=1,
+        eos_f=cv_per_K, eos_beta=1.0, face_kind=(0,) * 6, source_face=-1, source_per_face=0,
+        new_photons_per_cell=n, comb=0)
+
+
+def test_fleck_and_emission_hand_values(oracle):
+    # T = 1 keV, sigma = 1/cm, cv = 1e15 erg/(cm^3 keV), dt = 1e-11 s, V = 1e-6 cm^3:
+    # 4 a T^3 sigma c dt / cv = 0.1645281  ->  f = 0.8587169
+    # E = f V a T^4 sigma c dt = f * 4.113203e7 erg
+    r = oracle.prestep(_one_cell(1.0, 1.0, 1e15 / P.KEV_K))
+    assert r["fleck"][0] == pytest.approx(0.8587169, rel=2e-7)
+    assert r["emit"][0] == pytest.approx(0.8587169 * 4.113203e7, rel=1e-6)
+    # f = 1/2 exactly when cv = 4 a T^3 sigma c dt
+    T = 0.3 * P.KEV_K
+    cv = 4 * P.ARAD * T ** 3 * 2.0 * P.CLIGHT * 1e-11
+    r = oracle.prestep(_one_cell(0.3, 2.0, cv))
+    assert r["fleck"][0] == pytest.approx(0.5, rel=1e-13)
+    # optically thin limit: f -> 1, E -> V a T^4 sigma c dt
+    r = oracle.prestep(_one_cell(0.5, 1e-12, 1e15 / P.KEV_K))
+    T = 0.5 * P.KEV_K
+    assert r["fleck"][0] == pytest.approx(1.0, abs=1e-12)
+    assert r["emit"][0] == pytest.approx(1e-6 * P.ARAD * T ** 4 * 1e-12 * P.CLIGHT * 1e-11, rel=1e-12)
+
+
+def test_photon_allocation_bounds(oracle):
+    spec = P.c2_cube(seed=3, n=8)
+    r = oracle.prestep(spec)
+    n = spec.new_photons_per_cell
+    assert np.all((r["nphot"] >= n) & (r["nphot"] <= 20 * n))
+    # uniform cells -> share = 10 n each; the reference truncates
+    # (E_i / E_tot) * total with static_cast, so a rounded-down quotient
+    # gives 10 n - 1 (RadiationIMC.hpp:2080-2081)
+    uni = P.c1_slab(seed=0)
+    r = oracle.prestep(uni)
+    n10 = 10 * uni.new_photons_per_cell
+    assert np.all((r["nphot"] == n10) | (r["nphot"] == n10 - 1))
+
+
+def _balance(led, s):
+    lhs = led["material_before"][s] + led["census_in"][s] + led["source"][s] + led["clamp"][s]
+    rhs = led["material_after"][s] + led["census_comb"][s] + led["escaped"][s]
+    return abs(lhs - rhs) / lhs
+
+
+@pytest.mark.parametrize("make", [lambda: P.c1_slab(seed=2, packets_scale=0.05, steps=6),
+                                  lambda: P.c2_cube(seed=2, n=8, packets_scale=0.2, steps=4)], ids=["c1", "c2"])
+def test_oracle_energy_balance(oracle, make):
+    spec = make()
+    r = oracle.run(spec)
+    for s in range(spec.steps):
+        assert _balance(r["ledger"], s) < 1e-12
+
+
+def _beer_spec(tau, n=20, npc=100000, seed=11):
+    return P.ProblemSpec(
+        name="beer", x_edges=np.linspace(0, 1.0, n + 1), y_edges=np.array([0.0, 1.0]), z_edges=np.array([0.0, 1.0]),
+        sigma_a0=np.full(n, tau), T0=np.zeros(n), dt=1e-7, steps=1, eos_f=1e10, eos_beta=1.0,
+        face_kind=(capi.FACE_REFLECT, capi.FACE_ESCAPE) + (capi.FACE_REFLECT,) * 4,
+        source_face=capi.XLO, source_T=P.KEV_K, freeze_T4=0, source_per_face=npc,
+        new_photons_per_cell=1, comb=0, seed=seed)
+
+
+@pytest.mark.parametrize("tau", [0.5, 1.0, 2.0])
+def test_oracle_slab_transmission(oracle, tau):
+    spec = _beer_spec(tau)
+    r = oracle.run(spec)
+    L = r["ledger"]
+    frac = L["escaped"][0] / L["source"][0]
+    expect = 2.0 * special.expn(3, tau)
+    assert abs(frac - expect) < 4.0 * math.sqrt(expect * (1 - expect) / 1e5) + 2e-3, (frac, expect)
+
+
+def test_oracle_comb_conserves_and_targets(oracle):
+    spec = P.c1_slab(seed=0, nx=10)
+    spec.comb_nmin, spec.comb_factor = 20, 2.0
+    rng = np.random.default_rng(5)
+    cells = np.repeat(np.arange(10), [5, 100, 300, 40, 40, 41, 1, 0, 500, 60])
+    w = rng.exponential(1.0, len(cells)) * (1 + cells)
+    cnt, ws = oracle.comb(spec, cells, w)
+    Wc = np.bincount(cells, weights=w, minlength=10)
+    assert np.allclose(ws, Wc, rtol=1e-12)
+    Ntot = int(10 * 20 * 2.0)
+    target = np.minimum(400, np.maximum(20, np.floor(Ntot * Wc / Wc.sum()))).astype(int)
+    counts = np.bincount(cells, minlength=10)
+    combed = counts > target
+    assert np.all(cnt[combed] == target[combed])
+    assert np.all(cnt[~combed & (counts > 0)] >= counts[~combed & (counts > 0)])
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["mixed", "fp64"])
+@pytest.mark.parametrize("which", ["c1", "c2"])
+def test_engine_first_step_physics_on_cpu(oracle, precision, which):
+    """The engine's shared sourcing + transport code, run on the CPU, agrees
+    with the oracle's first step (per-cell Erad time average and segments)."""
+    if which == "c1":
+        make = lambda s: P.c1_slab(seed=s, packets_scale=0.05, steps=1)  # noqa: E731
+    else:
+        make = lambda s: P.c2_cube(seed=7, n=8, packets_scale=0.2, steps=1).with_seed(s)  # noqa: E731
+    o_erad, o_seg, e_erad, e_seg = [], [], [], []
+    for s in range(16):
+        spec = make(s)
+        o = oracle.run(spec, steps=1)
+        e = oracle.cpu_first_step(spec.with_seed(100 + s), precision)
+        o_erad.append(o["erad"])
+        o_seg.append(o["segments"])
+        e_erad.append(e["erad"])
+        e_seg.append(e["segments"])
+    s1, ok1 = check_ensembles(o_erad, e_erad, label="erad")
+    s2, ok2 = check_ensembles(o_seg, e_seg, label="segments")
+    assert ok1 and ok2, (s1, s2)
+
+
+def test_cpu_first_step_is_deterministic(oracle):
+    spec = P.c2_cube(seed=7, n=8, packets_scale=0.2, steps=1)
+    a = oracle.cpu_first_step(spec, 0)
+    b = oracle.cpu_first_step(spec, 0)
+    assert np.array_equal(a["dep"], b["dep"]) and np.array_equal(a["erad"], b["erad"])
+
+
+def test_cpu_first_step_conserves(oracle):
+    spec = P.c1_slab(seed=1, packets_scale=0.05, steps=1)
+    e = oracle.cpu_first_step(spec, 0)
+    total_in = e["emitted"] + e["source"]
+    total_out = e["dep"].sum() + e["escaped"] + e["census"]
+    assert abs(total_in - total_out) < 1e-12 * total_in
