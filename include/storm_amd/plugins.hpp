@@ -1,0 +1,422 @@
+// The reference's plugin surface for the IMC path, for use with
+// MonteCarloManagerGPU: same class names, template parameters, constructors
+// and public methods as
+//   physics/MonteCarloPhysics.hpp:16-53            MonteCarloPhysics
+//   boundary/BoundaryCondition.hpp:30-51           BoundaryCondition
+//   boundary/SideTemperature.hpp:156-191           SideTemperature
+//   boundary/Vacuum.hpp:10-45                      VacuumBoundary
+//   boundary/RigidBoundary.hpp:9-24                RigidBoundary
+//   population/PopulationControl.hpp:8-22          PopulationControl
+//   population/CombPopulationControl.hpp:20-32     CombPopulationControl
+//   population/NoPopulationControl.hpp:8-28        NoPopulationControl
+//   radiation/RadiationIMCParameters.hpp:12-56     RadiationIMCParameters
+//   radiation/RadiationCell.hpp:6-19               RadiationCell, SimpleExtensives
+//   radiation/RadiationOpacityModel.hpp:18-151     RadiationOpacityModel
+//   radiation/RadiationIMC.hpp:332-467             RadiationIMC
+// The physics itself runs in the gfx950 engine (include/storm_gpu.h); these
+// classes carry the configuration, the user's opacity / EOS objects (called
+// once per cell per step on the host, never per segment) and the results.
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <random>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../storm_problem.h"
+#include "Particle.hpp"
+#include "StormError.hpp"
+
+namespace STORM {
+
+// ------------------------------------------------------------------ physics
+template <typename T, typename Grid>
+class MonteCarloPhysics {
+ public:
+  using MCParticle = Particle<T, Grid>;
+  MonteCarloPhysics(const Grid &grid, const std::shared_ptr<class BoundaryConditionTag> & = nullptr) : grid(grid) {}
+  explicit MonteCarloPhysics(const Grid &grid) : grid(grid) {}
+  virtual ~MonteCarloPhysics() = default;
+  virtual void updateGridData(void) {}
+  virtual std::vector<MCParticle> preStep(double fullDt) = 0;
+  virtual StepResult<T, Grid> step(MCParticle &particle, std::vector<MCParticle> &particlesToAdd) = 0;
+  virtual void postStep(const std::vector<MCParticle> &particles, double fullDt) = 0;
+  virtual size_t getRandomWalkStepCount() const { return 0; }
+  virtual size_t getDDMCStepCount() const { return 0; }
+  virtual size_t getDDMCLeakCount() const { return 0; }
+  virtual size_t getDDMCCensusCount() const { return 0; }
+  virtual size_t getDDMCUpscatterCount() const { return 0; }
+  virtual size_t getDDMCFallbackCount() const { return 0; }
+  virtual std::string getAccelerationDebugInfo(size_t, double) const { return std::string(); }
+
+ protected:
+  const Grid &grid;
+};
+
+// ------------------------------------------------------------------ boundary
+enum class DDMCBoundaryFaceBehavior { Unsupported, ReflectingRigid };
+
+// What the GPU needs to know about a boundary: the fate of a packet on each
+// box face (x-, x+, y-, y+, z-, z+) and an optional Planck source on x-.
+struct GpuBoundaryInfo {
+  int face_kind[6] = {STORM_FACE_REFLECT, STORM_FACE_REFLECT, STORM_FACE_REFLECT,
+                      STORM_FACE_REFLECT, STORM_FACE_REFLECT, STORM_FACE_REFLECT};
+  bool source = false;
+  double temperature = 0.0;
+  size_t packets_per_face = 0;
+  bool freeze_T4 = false;
+};
+
+template <typename T, typename Grid>
+class BoundaryCondition {
+ public:
+  explicit BoundaryCondition(const Grid &grid) : grid(grid) {}
+  virtual ~BoundaryCondition() = default;
+  virtual ParticleStatus apply(Particle<T, Grid> &particle) = 0;
+  virtual std::vector<Particle<T, Grid>> generateNewBoundaryParticles(double fullDt) = 0;
+  virtual DDMCBoundaryFaceBehavior getDDMCBoundaryFaceBehavior(size_t, size_t, size_t) const {
+    return DDMCBoundaryFaceBehavior::Unsupported;
+  }
+  // Face table for the GPU; boundaries that do not describe themselves are
+  // probed through apply() by the manager.
+  virtual bool describeForGpu(GpuBoundaryInfo &) const { return false; }
+
+ protected:
+  const Grid &grid;
+};
+
+// Planck surface source on the x- face; packets reaching x- leave, the other
+// faces reflect (boundary/SideTemperature.hpp:84-163).  As in the reference,
+// T^4 is taken at the first step (`static const double T4`, :252) unless
+// freezeT4 is false (the MarshakBoundary behaviour,
+// examples/marshak_wave/MarshakBoundary.hpp:150-156).
+template <typename T, typename Grid>
+class SideTemperature : public BoundaryCondition<T, Grid> {
+ public:
+  SideTemperature(const Grid &grid, double temperature, size_t Npercell,
+                  const std::vector<double> &energyBoundaries = {}, bool freezeT4 = true)
+      : BoundaryCondition<T, Grid>(grid), temperature(temperature), Npercell(Npercell), freezeT4(freezeT4) {
+    if (!energyBoundaries.empty()) throw StormError("SideTemperature: multigroup sources are not supported on the GPU");
+  }
+  ParticleStatus apply(Particle<T, Grid> &) override {
+    throw StormError("SideTemperature::apply is evaluated on the GPU (MonteCarloManagerGPU)");
+  }
+  std::vector<Particle<T, Grid>> generateNewBoundaryParticles(double) override {
+    throw StormError("SideTemperature sources are generated on the GPU (MonteCarloManagerGPU)");
+  }
+  DDMCBoundaryFaceBehavior getDDMCBoundaryFaceBehavior(size_t, size_t, size_t) const override {
+    return DDMCBoundaryFaceBehavior::ReflectingRigid;
+  }
+  bool describeForGpu(GpuBoundaryInfo &info) const override {
+    info = GpuBoundaryInfo{};
+    info.face_kind[STORM_XLO] = STORM_FACE_ESCAPE;
+    info.source = Npercell > 0;
+    info.temperature = temperature;
+    info.packets_per_face = Npercell;
+    info.freeze_T4 = freezeT4;
+    return true;
+  }
+  void SetTemperature(double temp) { temperature = temp; }
+  double GetTemperature() const { return temperature; }
+
+ private:
+  double temperature;
+  size_t Npercell;
+  bool freezeT4;
+};
+
+// Every face removes the packet and the escaped energy is tallied (boundary/Vacuum.hpp:18-22).
+template <typename T, typename Grid>
+class VacuumBoundary : public BoundaryCondition<T, Grid> {
+ public:
+  using MCParticle = Particle<T, Grid>;
+  explicit VacuumBoundary(const Grid &grid) : BoundaryCondition<T, Grid>(grid) {}
+  ParticleStatus apply(MCParticle &particle) override {
+    escapedEnergy_ += particle.weight;
+    return ParticleStatus::REMOVE;
+  }
+  std::vector<MCParticle> generateNewBoundaryParticles(double) override { return {}; }
+  bool describeForGpu(GpuBoundaryInfo &info) const override {
+    info = GpuBoundaryInfo{};
+    for (int f = 0; f < 6; ++f) info.face_kind[f] = STORM_FACE_ESCAPE;
+    return true;
+  }
+  double getEscapedEnergy() const { return escapedEnergy_; }
+  void resetEscapedEnergy() { escapedEnergy_ = 0.0; }
+  void addEscapedEnergy(double e) { escapedEnergy_ += e; }
+
+ private:
+  double escapedEnergy_ = 0.0;
+};
+
+// Every face reflects (boundary/RigidBoundary.hpp:42-58).
+template <typename T, typename Grid>
+class RigidBoundary : public BoundaryCondition<T, Grid> {
+ public:
+  explicit RigidBoundary(const Grid &grid) : BoundaryCondition<T, Grid>(grid) {}
+  ParticleStatus apply(Particle<T, Grid> &) override {
+    throw StormError("RigidBoundary::apply is evaluated on the GPU (MonteCarloManagerGPU)");
+  }
+  std::vector<Particle<T, Grid>> generateNewBoundaryParticles(double) override { return {}; }
+  DDMCBoundaryFaceBehavior getDDMCBoundaryFaceBehavior(size_t, size_t, size_t) const override {
+    return DDMCBoundaryFaceBehavior::ReflectingRigid;
+  }
+  bool describeForGpu(GpuBoundaryInfo &info) const override {
+    info = GpuBoundaryInfo{};
+    return true;
+  }
+};
+
+// ------------------------------------------------------------------ population control
+template <typename T, typename Grid>
+class PopulationControl {
+ public:
+  using MCParticle = Particle<T, Grid>;
+  explicit PopulationControl(const Grid &grid) : grid(grid) {}
+  virtual ~PopulationControl() = default;
+  virtual std::vector<MCParticle> activate(const std::vector<MCParticle> &particles) = 0;
+
+ protected:
+  const Grid &grid;
+};
+
+// Comb population control: run on the GPU by the manager
+// (population/CombPopulationControl.hpp:79-253).
+template <typename T, typename Grid>
+class CombPopulationControl : public PopulationControl<T, Grid> {
+ public:
+  CombPopulationControl(const Grid &grid, size_t Nmin = 20, double totalParticlesFactor = 2.0)
+      : PopulationControl<T, Grid>(grid), Nmin(Nmin), totalParticlesFactor(totalParticlesFactor) {}
+  std::vector<Particle<T, Grid>> activate(const std::vector<Particle<T, Grid>> &) override {
+    throw StormError("CombPopulationControl runs on the GPU (MonteCarloManagerGPU)");
+  }
+  size_t getNmin() const { return Nmin; }
+  double getTotalParticlesFactor() const { return totalParticlesFactor; }
+
+ private:
+  size_t Nmin;
+  double totalParticlesFactor;
+};
+
+template <typename T, typename Grid>
+class NoPopulationControl : public PopulationControl<T, Grid> {
+ public:
+  using MCParticle = Particle<T, Grid>;
+  explicit NoPopulationControl(const Grid &grid) : PopulationControl<T, Grid>(grid) {}
+  std::vector<MCParticle> activate(const std::vector<MCParticle> &particles) override { return particles; }
+};
+
+// ------------------------------------------------------------------ radiation
+template <std::size_t NumGroups>
+struct RadiationIMCParameters {
+  std::size_t newPhotonsPerCell = 0;
+  bool withHydro = false;
+  bool diffusionPressureGradient = false;
+  bool MMC = false;
+  bool withMultigroupOpacity = false;
+  bool withRandomWalk = false;
+  double rwMinCellOpticalDepth = 25.0;
+  double rwMinParticleOpticalDepth = 5.0;
+  bool withDDMC = false;
+  double ddmcMinCellOpticalDepth = 15.0;
+  double ddmcMinParticleOpticalDepth = 5.0;
+  bool ddmcUseMultigroupPGRW = false;
+  bool noHydroFeedback = false;
+  bool withEgTimeAvg = false;
+  bool withCompton = false;
+  std::array<double, NumGroups + 1> energyBoundaries{};
+  bool energyBoundariesProvided = false;
+  struct PostProcessParameters {
+    bool enabled = false;
+    struct PolarizationParameters {
+      bool enabled = false;
+    } polarization;
+  } postProcess;
+};
+
+struct RadiationCell {
+  double temperature = 0;     // K
+  double cv = 0;              // unused by RadiationIMC
+  double internalEnergy = 0;  // erg (previous step's value, the negative-energy floor)
+  double Erad = 0;            // erg/g
+};
+
+struct SimpleExtensives {
+  double internal_energy = 0;  // erg
+  double mass = 0;             // g
+  double Erad = 0;             // erg
+};
+
+template <typename PointT, typename GridT, typename CellT, std::size_t NumGroups>
+class RadiationOpacityModel {
+ public:
+  using GroupArray = std::array<double, NumGroups>;
+  using GroupBoundaries = std::array<double, NumGroups + 1>;
+  virtual ~RadiationOpacityModel() = default;
+  virtual double CalcPlanckOpacity(const CellT &cell) = 0;
+  virtual double CalcAbsorptionOpacity(const CellT &cell, double) { return CalcPlanckOpacity(cell); }
+  virtual double CalcScatteringOpacity(const CellT &) { return 0.0; }
+  virtual double CalcScatteringOpacity(const CellT &cell, double) { return CalcScatteringOpacity(cell); }
+  virtual void reseed(std::uint64_t) {}
+  virtual std::size_t findGroup(double frequency, const GroupBoundaries &b) const {
+    for (std::size_t g = 0; g < NumGroups; ++g)
+      if (frequency < b[g + 1]) return g;
+    return NumGroups - 1;
+  }
+};
+
+namespace detail {
+template <typename C, typename = void>
+struct has_density : std::false_type {};
+template <typename C>
+struct has_density<C, std::void_t<decltype(std::declval<const C &>().density)>> : std::true_type {};
+template <typename C, typename = void>
+struct has_internal_energy_density : std::false_type {};
+template <typename C>
+struct has_internal_energy_density<C, std::void_t<decltype(std::declval<C &>().internalEnergy)>> : std::true_type {};
+template <typename C, typename = void>
+struct has_erad : std::false_type {};
+template <typename C>
+struct has_erad<C, std::void_t<decltype(std::declval<C &>().Erad)>> : std::true_type {};
+}  // namespace detail
+
+// What the manager needs from the physics, independent of its template
+// arguments: per-cell host tabulation before a step and the results after.
+class GpuRadiationPhysics {
+ public:
+  virtual ~GpuRadiationPhysics() = default;
+  virtual void gpuParameters(storm_imc_params &p) const = 0;
+  virtual std::uint64_t gpuSeed() const = 0;
+  // T, Planck opacity, scattering opacity, dT2cv, internal energy, mass per cell
+  virtual void gpuTabulate(double *T, double *sa, double *ss, double *cv, double *e, double *mass) = 0;
+  // new internal energy, Erad time average, census Erad (extensive), Fleck, Planck opacity
+  virtual void gpuAbsorb(const double *e, const double *eradAvg, const double *censusErad, const double *fleck,
+                         const double *planck) = 0;
+  virtual void gpuCounters(std::size_t rw, std::size_t ddmc) = 0;
+};
+
+template <typename PointT, typename GridT, typename CellT, typename ExtensivesT, typename EOST,
+          std::size_t NumGroups>
+class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiationPhysics {
+ public:
+  static_assert(NumGroups == 1, "the GPU engine is gray (NumGroups = 1)");
+  using Base = MonteCarloPhysics<PointT, GridT>;
+  using MCParticle = Particle<PointT, GridT>;
+  using Parameters = RadiationIMCParameters<NumGroups>;
+  using OpacityModel = RadiationOpacityModel<PointT, GridT, CellT, NumGroups>;
+
+  template <typename BoundaryPtr>
+  RadiationIMC(const GridT &grid, const BoundaryPtr &, std::vector<CellT> &cells,
+               std::vector<ExtensivesT> &extensives, std::shared_ptr<EOST> eos, std::shared_ptr<OpacityModel> opacity,
+               Parameters parameters, std::uint64_t seed = 42)
+      : Base(grid), cells_(cells), extensives_(extensives), eos_(std::move(eos)), opacity_(std::move(opacity)),
+        parameters_(parameters), seed_(seed) {
+    // constructor checks of radiation/RadiationIMC.hpp:585-602, 689-694
+    if (parameters_.newPhotonsPerCell == 0) throw StormError("RadiationIMC requires newPhotonsPerCell > 0");
+    if (!eos_) throw StormError("RadiationIMC requires a non-null EOS");
+    if (!opacity_) throw StormError("RadiationIMC requires a non-null opacity model");
+    if (cells_.size() < grid.GetPointNo()) throw StormError("RadiationIMC cells vector is smaller than the grid cell count");
+    if (extensives_.size() < grid.GetPointNo())
+      throw StormError("RadiationIMC extensives vector is smaller than the grid cell count");
+    if (parameters_.withCompton) throw StormError("RadiationIMC option withCompton is not implemented");
+    if (parameters_.postProcess.enabled) throw StormError("RadiationIMC option postProcess.enabled is not implemented");
+    if (parameters_.withHydro || parameters_.MMC) throw StormError("RadiationIMC: hydro coupling is not supported on the GPU");
+    if (parameters_.withMultigroupOpacity) throw StormError("RadiationIMC: the GPU engine is gray");
+    opacity_->reseed(seed + 1ULL);
+    Erad_time_avg_.assign(grid.GetPointNo(), 0.0);
+  }
+
+  std::vector<MCParticle> preStep(double) override { throw StormError(kGpuOnly); }
+  StepResult<PointT, GridT> step(MCParticle &, std::vector<MCParticle> &) override { throw StormError(kGpuOnly); }
+  void postStep(const std::vector<MCParticle> &, double) override { throw StormError(kGpuOnly); }
+
+  const std::vector<double> &getFactorFleck() const { return factorFleck_; }
+  const std::vector<double> &getPlanckOpacities() const { return planck_; }
+  const std::vector<double> &getEradTimeAvg() const { return Erad_time_avg_; }
+  std::vector<double> &getEradTimeAvg() { return Erad_time_avg_; }
+  const Parameters &getParameters() const { return parameters_; }
+  std::size_t getRandomWalkStepCount() const override { return rw_; }
+  std::size_t getDDMCStepCount() const override { return ddmc_; }
+  void setNewPhotonsPerCell(std::size_t n) { parameters_.newPhotonsPerCell = n; }
+
+  // ---------------------------------------------------------------- GpuRadiationPhysics
+  void gpuParameters(storm_imc_params &p) const override {
+    p.new_photons_per_cell = parameters_.newPhotonsPerCell;
+    p.with_random_walk = parameters_.withRandomWalk ? 1 : 0;
+    p.with_ddmc = parameters_.withDDMC ? 1 : 0;
+    p.rw_min_cell_optical_depth = parameters_.rwMinCellOpticalDepth;
+    p.rw_min_particle_optical_depth = parameters_.rwMinParticleOpticalDepth;
+    p.ddmc_min_cell_optical_depth = parameters_.ddmcMinCellOpticalDepth;
+    p.ddmc_min_particle_optical_depth = parameters_.ddmcMinParticleOpticalDepth;
+  }
+  std::uint64_t gpuSeed() const override { return seed_; }
+
+  void gpuTabulate(double *T, double *sa, double *ss, double *cv, double *e, double *mass) override {
+    const std::size_t N = this->grid.GetPointNo();
+    const std::vector<double> tracers;
+    const std::vector<std::string> names;
+    for (std::size_t i = 0; i < N; ++i) {
+      const CellT &cell = cells_[i];
+      T[i] = cell.temperature;
+      sa[i] = opacity_->CalcPlanckOpacity(cell);
+      ss[i] = opacity_->CalcScatteringOpacity(cell);
+      cv[i] = eos_->dT2cv(density(i), cell.temperature, tracers, names);
+      e[i] = extensives_[i].internal_energy;
+      mass[i] = extensives_[i].mass;
+      if constexpr (detail::has_internal_energy_density<CellT>::value) {
+        // the negative-energy floor of the step is the last synchronised value
+        // (RadiationIMC.hpp:794-797); the engine keeps e as that floor
+      }
+    }
+  }
+
+  void gpuAbsorb(const double *e, const double *eradAvg, const double *censusErad, const double *fleck,
+                 const double *planck) override {
+    const std::size_t N = this->grid.GetPointNo();
+    const std::vector<double> tracers;
+    const std::vector<std::string> names;
+    Erad_time_avg_.assign(eradAvg, eradAvg + N);
+    factorFleck_.assign(fleck, fleck + N);
+    planck_.assign(planck, planck + N);
+    for (std::size_t i = 0; i < N; ++i) {
+      // synchronizeMaterialCell (radiation/RadiationIMC.hpp:830-866)
+      extensives_[i].internal_energy = e[i];
+      CellT &cell = cells_[i];
+      const double specific = e[i] / extensives_[i].mass;
+      if constexpr (detail::has_internal_energy_density<CellT>::value) cell.internalEnergy = e[i];
+      cell.temperature = eos_->de2T(density(i), specific, tracers, names);
+      // census radiation energy (RadiationIMC.hpp:1989-2028)
+      if constexpr (detail::has_erad<ExtensivesT>::value) extensives_[i].Erad = censusErad[i];
+      if constexpr (detail::has_erad<CellT>::value)
+        cell.Erad = extensives_[i].mass > 0.0 ? censusErad[i] / extensives_[i].mass : 0.0;
+    }
+  }
+  void gpuCounters(std::size_t rw, std::size_t ddmc) override {
+    rw_ = rw;
+    ddmc_ = ddmc;
+  }
+
+ private:
+  static constexpr const char *kGpuOnly = "RadiationIMC runs on the GPU: drive it with MonteCarloManagerGPU";
+  double density(std::size_t i) const {
+    if constexpr (detail::has_density<CellT>::value) return cells_[i].density;
+    else return extensives_[i].mass / this->grid.GetVolume(i);
+  }
+
+  std::vector<CellT> &cells_;
+  std::vector<ExtensivesT> &extensives_;
+  std::shared_ptr<EOST> eos_;
+  std::shared_ptr<OpacityModel> opacity_;
+  Parameters parameters_;
+  std::uint64_t seed_;
+  std::vector<double> Erad_time_avg_, factorFleck_, planck_;
+  std::size_t rw_ = 0, ddmc_ = 0;
+};
+
+}  // namespace STORM

@@ -174,6 +174,9 @@ int storm_gpu_debug_transport_inputs(storm_gpu_engine *eng, uint64_t *n_packets,
                                      void *cell_xs, int64_t *consts4);
 int storm_gpu_debug_tallies(storm_gpu_engine *eng, uint64_t *dep, uint64_t *pint, uint64_t *esc,
                             uint32_t *segs);
+/* Per-cell random-walk / DDMC data of the last step (bytes written to
+ * `out`, which may be NULL to query the size). */
+int storm_gpu_debug_accel(storm_gpu_engine *eng, void *out, uint64_t *bytes);
 
 #ifdef __cplusplus
 }

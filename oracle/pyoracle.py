@@ -128,15 +128,20 @@ def rlib():
         L.replay_last_error.restype = C.c_char_p
         L.replay_transport_f32.argtypes = [C.c_int64, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                            C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_float),
-                                           C.POINTER(C.c_int64), C.c_int32, u64p, u64p, u64p,
+                                           C.c_void_p, dp, C.POINTER(C.c_int64), C.c_int32, u64p, u64p, u64p,
                                            C.POINTER(C.c_uint32), C.POINTER(C.c_int64)]
         L.cpu_first_step.argtypes = [C.c_void_p, C.c_double, C.c_int32, C.c_uint32, dp, dp, dp, dp]
+        L.replay_rw_tables.argtypes = [dp, dp, dp]
+        L.replay_rw_sample.argtypes = [C.c_int32, C.c_int64, dp, dp, dp, dp]
         _rlib = L
     return _rlib
 
 
-def replay_transport(spec, packets, n, xs, consts, segments_per_pass=1 << 20):
-    """Replay `n` engine packets (ctypes array of PacketF32) on the CPU; integer tallies."""
+def replay_transport(spec, packets, n, xs, consts, segments_per_pass=1 << 20, accel=None):
+    """Replay `n` engine packets (ctypes array of PacketF32) on the CPU; integer tallies.
+
+    `accel` is the engine's per-cell acceleration data (bytes) when the
+    problem has random walk or DDMC."""
     L = rlib()
     nx, ny, nz = spec.shape
     widths = np.concatenate([np.diff(spec.x_edges), np.diff(spec.y_edges), np.diff(spec.z_edges)]).astype(np.float32)
@@ -146,12 +151,16 @@ def replay_transport(spec, packets, n, xs, consts, segments_per_pass=1 << 20):
     pint = np.zeros(2 * N, dtype=np.uint64)
     esc = np.zeros(2 * N, dtype=np.uint64)
     segs = np.zeros(N, dtype=np.uint32)
-    outc = np.zeros(4, dtype=np.int64)
+    outc = np.zeros(6, dtype=np.int64)
     cst = np.asarray(consts, dtype=np.int64)
     xs32 = np.ascontiguousarray(xs, dtype=np.float32)
+    accbuf = None if accel is None else C.create_string_buffer(bytes(accel), len(accel))
+    accp = np.array([spec.with_rw, spec.with_ddmc, spec.rw_particle_tau, spec.ddmc_particle_tau], dtype=np.float64)
     rc = L.replay_transport_f32(n, C.cast(packets, C.c_void_p), nx, ny, nz,
                                 widths.ctypes.data_as(C.POINTER(C.c_float)), fk.ctypes.data_as(C.POINTER(C.c_int32)),
-                                xs32.ctypes.data_as(C.POINTER(C.c_float)), cst.ctypes.data_as(C.POINTER(C.c_int64)),
+                                xs32.ctypes.data_as(C.POINTER(C.c_float)),
+                                None if accbuf is None else C.cast(accbuf, C.c_void_p), _dp(accp),
+                                cst.ctypes.data_as(C.POINTER(C.c_int64)),
                                 int(segments_per_pass),
                                 dep.ctypes.data_as(C.POINTER(C.c_uint64)), pint.ctypes.data_as(C.POINTER(C.c_uint64)),
                                 esc.ctypes.data_as(C.POINTER(C.c_uint64)), segs.ctypes.data_as(C.POINTER(C.c_uint32)),
@@ -165,10 +174,26 @@ def cpu_first_step(spec, precision: int, step: int = 1):
     """The engine's first step (sourcing + transport, no census) on the CPU."""
     L = rlib()
     n = spec.ncells
-    dep, erad, segs, tot = np.empty(n), np.empty(n), np.empty(n), np.empty(6)
+    dep, erad, segs, tot = np.empty(n), np.empty(n), np.empty(n), np.empty(8)
     prob = spec.to_c()
     rc = L.cpu_first_step(C.byref(prob), spec.dt, precision, step, *(_dp(a) for a in (dep, erad, segs, tot)))
     if rc != 0:
         raise RuntimeError("replay: " + L.replay_last_error().decode())
     return dict(dep=dep, erad=erad, segments=segs, escaped=tot[0], census=tot[1], packets=tot[2],
-                emitted=tot[3], source=tot[4], census_packets=tot[5])
+                emitted=tot[3], source=tot[4], census_packets=tot[5], rw_steps=tot[6], ddmc_steps=tot[7])
+
+
+def engine_rw_tables():
+    L = rlib()
+    tau, surv, rad = np.empty(1024), np.empty(1024), np.empty(1024 * 256)
+    L.replay_rw_tables(_dp(tau), _dp(surv), _dp(rad))
+    return tau, surv, rad.reshape(1024, 256)
+
+
+def engine_rw_sample(precision, xi, tau):
+    L = rlib()
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    tau = np.ascontiguousarray(tau, dtype=np.float64)
+    leak, rad = np.empty_like(xi), np.empty_like(xi)
+    L.replay_rw_sample(precision, len(xi), _dp(xi), _dp(tau), _dp(leak), _dp(rad))
+    return leak, rad

@@ -10,11 +10,13 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "../include/storm_problem.h"
 #include "../storm_amd/csrc/common/fixed.h"
+#include "../storm_amd/csrc/common/random_walk.h"
 #include "../storm_amd/csrc/common/sourcing.h"
 #include "../storm_amd/csrc/common/transport.h"
 
@@ -28,6 +30,7 @@ struct HostSink {
   double *dep_fp, *pint_fp, *esc_fp;
   uint32_t *segs;
   int32_t eq;
+  uint64_t rw = 0, ddmc = 0;
   void visit(uint32_t c, fx::Limbs d, fx::Limbs e, uint32_t s) {
     dep[2 * c] += d.lo;
     dep[2 * c + 1] += d.hi;
@@ -49,17 +52,94 @@ struct HostSink {
       esc[2 * c + 1] += l.hi;
     }
   }
+  void rw_step() { ++rw; }
+  void ddmc_step() { ++ddmc; }
+};
+
+// Random-walk tables in the engine's precision (same host build as the engine).
+template <typename Real>
+struct HostTables {
+  std::vector<Real> tau, surv, rad;
+  HostTables() {
+    std::vector<double> t(rw::kTableSize), s(rw::kTableSize), r(rw::kTableSize * rw::kRadiusSize);
+    rw::build_tables(t.data(), s.data(), r.data());
+    tau.assign(t.begin(), t.end());
+    surv.assign(s.begin(), s.end());
+    rad.assign(r.begin(), r.end());
+  }
 };
 
 template <typename Real>
-int run_all(std::vector<Packet<Real>> &pk, const MeshView<Real> &mv, const CellXS<Real> *xs, const StepConsts &kc,
-            int K, HostSink &sink, int64_t *outcomes) {
-  for (auto &p : pk) {
-    int o = kAlive;
-    while (o == kAlive) o = transport_packet<Real>(p, mv, xs, kc, K, sink);
-    outcomes[o] += 1;
+AccelView<Real> make_accel_view(const CellAccel<Real> *cells, const HostTables<Real> *tb, int rw_on, int ddmc_on,
+                                double rw_ptau, double ddmc_ptau) {
+  AccelView<Real> a;
+  a.cell = cells;
+  a.rw_tau = tb ? tb->tau.data() : nullptr;
+  a.rw_survival = tb ? tb->surv.data() : nullptr;
+  a.rw_radius = tb ? tb->rad.data() : nullptr;
+  a.rw_log_tau_min = static_cast<Real>(std::log(rw::kTauMin));
+  a.rw_log_tau_max = static_cast<Real>(std::log(rw::kTauMax));
+  a.rw_particle_tau = rw_on ? static_cast<Real>(rw_ptau) : MathPolicy<Real>::inf();
+  a.ddmc_particle_tau = ddmc_on ? static_cast<Real>(ddmc_ptau) : MathPolicy<Real>::inf();
+  return a;
+}
+
+// Host restatement of the engine's per-cell acceleration data.
+template <typename Real>
+std::vector<CellAccel<Real>> build_accel(const storm_problem *p, const std::vector<double> &sa,
+                                         const std::vector<double> &ss) {
+  const int nx = p->mesh.nx, ny = p->mesh.ny, nz = p->mesh.nz;
+  const uint32_t N = static_cast<uint32_t>(nx) * ny * nz;
+  std::vector<CellAccel<Real>> out(N);
+  std::vector<double> Dt(N);
+  auto w = [&](int ax, int idx) {
+    const double *e = ax == 0 ? p->mesh.x_edges : (ax == 1 ? p->mesh.y_edges : p->mesh.z_edges);
+    return static_cast<double>(static_cast<Real>(e[idx + 1] - e[idx]));
+  };
+  for (uint32_t c = 0; c < N; ++c) {
+    int k = static_cast<int>(c % nz), j = static_cast<int>((c / nz) % ny),
+        i = static_cast<int>(c / (static_cast<uint32_t>(ny) * nz));
+    double dx = w(0, i), dy = w(1, j), dz = w(2, k);
+    double V = dx * dy * dz, S = 2.0 * (dy * dz + dx * dz + dx * dy), chord = 4.0 * V / S;
+    double st = sa[c] + ss[c];
+    CellAccel<Real> &a = out[c];
+    a.sig_t = static_cast<Real>(st);
+    a.total = Real(0);
+    for (int f = 0; f < 6; ++f) a.rate[f] = Real(0);
+    a.flags = 0u;
+    a.pad = 0u;
+    if (p->imc.with_random_walk && st * chord >= p->imc.rw_min_cell_optical_depth) a.flags |= 1u;
+    Dt[c] = st > 0.0 ? 1.0 / (3.0 * st) : 0.0;
+    if (p->imc.with_ddmc && st * chord >= p->imc.ddmc_min_cell_optical_depth && Dt[c] > 0.0) a.flags |= 2u;
   }
-  return 0;
+  const uint32_t n3[3] = {static_cast<uint32_t>(nx), static_cast<uint32_t>(ny), static_cast<uint32_t>(nz)};
+  const uint32_t stride[3] = {static_cast<uint32_t>(ny * nz), static_cast<uint32_t>(nz), 1u};
+  for (uint32_t c = 0; c < N; ++c) {
+    CellAccel<Real> &a = out[c];
+    if (!(a.flags & 2u)) continue;
+    uint32_t idx[3] = {c / stride[0], (c / nz) % ny, c % nz};
+    double total = 0.0;
+    for (int f = 0; f < 6; ++f) {
+      int ax = f >> 1;
+      bool up = f & 1;
+      if (up ? (idx[ax] + 1u >= n3[ax]) : (idx[ax] == 0u)) continue;
+      uint32_t nb = up ? c + stride[ax] : c - stride[ax];
+      double D1 = Dt[c], D2 = Dt[nb], Df = D1;
+      if (D2 > 0.0) Df = (D1 > 0.0 && D2 > 0.0) ? 2.0 * D1 * D2 / (D1 + D2) : std::fmax(D1, D2);
+      double ww = w(ax, static_cast<int>(idx[ax]));
+      double rate = Df / (ww * (0.5 * ww));
+      if (rate > 0.0 && std::isfinite(rate)) {
+        a.rate[f] = static_cast<Real>(rate);
+        total += rate;
+      }
+    }
+    a.total = static_cast<Real>(total);
+    if (!(total > 0.0)) {
+      a.flags &= ~2u;
+      for (int f = 0; f < 6; ++f) a.rate[f] = Real(0);
+    }
+  }
+  return out;
 }
 }  // namespace
 
@@ -88,36 +168,80 @@ void replay_fpmath(int32_t which, int64_t n, const float *x, float *y) {
   }
 }
 
+// The engine's random-walk tables (double) and its samplers in `precision`
+// (0 float, 1 double) on a batch.
+void replay_rw_tables(double *tau, double *survival, double *radius) { rw::build_tables(tau, survival, radius); }
+
+void replay_rw_sample(int32_t precision, int64_t n, const double *xi, const double *tau, double *leak,
+                      double *radius) {
+  auto go = [&](auto tag) {
+    using Real = decltype(tag);
+    HostTables<Real> tb;
+    rw::Tables<Real> t{tb.tau.data(), tb.surv.data(), tb.rad.data(), static_cast<Real>(std::log(rw::kTauMin)),
+                       static_cast<Real>(std::log(rw::kTauMax))};
+    for (int64_t i = 0; i < n; ++i) {
+      leak[i] = static_cast<double>(rw::sample_leak_time<Real>(t, static_cast<Real>(xi[i])));
+      radius[i] = static_cast<double>(rw::sample_radius<Real>(t, static_cast<Real>(tau[i]), static_cast<Real>(xi[i])));
+    }
+  };
+  if (precision == 0) go(float{});
+  else go(double{});
+}
+
 // Exact replay of `n` packets (storm::Packet<float>) through the transport.
-// widths: nx + ny + nz floats; xs: 2 floats per cell; consts: seed, step, eq, et.
-// Tallies are accumulated into the given arrays (dep/pint/esc: 2 per cell, segs: 1 per cell).
+// widths: nx + ny + nz floats; xs: 2 floats per cell; consts: seed, step, eq, et;
+// accel: N CellAccel<float> records or NULL; acc_params: rw_on, ddmc_on,
+// rw particle tau, ddmc particle tau.  Tallies are accumulated into the given
+// arrays (dep/pint/esc: 2 per cell, segs: 1 per cell); counts[0..3] outcomes,
+// counts[4] random-walk steps, counts[5] DDMC steps.
 int replay_transport_f32(int64_t n, const void *packets, int32_t nx, int32_t ny, int32_t nz, const float *widths,
-                         const int32_t *face_kind, const float *xs, const int64_t *consts, int32_t K, uint64_t *dep,
-                         uint64_t *pint, uint64_t *esc, uint32_t *segs, int64_t *outcomes) {
-  MeshView<float> mv;
-  mv.nx = nx;
-  mv.ny = ny;
-  mv.nz = nz;
-  for (int f = 0; f < 6; ++f) mv.face_kind[f] = face_kind[f];
-  mv.width[0] = widths;
-  mv.width[1] = widths + nx;
-  mv.width[2] = widths + nx + ny;
-  StepConsts kc;
-  kc.seed = static_cast<uint64_t>(consts[0]);
-  kc.step = static_cast<uint32_t>(consts[1]);
-  kc.eq = static_cast<int32_t>(consts[2]);
-  kc.et = static_cast<int32_t>(consts[3]);
-  std::vector<Packet<float>> pk(static_cast<size_t>(n));
-  std::memcpy(pk.data(), packets, static_cast<size_t>(n) * sizeof(Packet<float>));
-  HostSink sink{dep, pint, esc, nullptr, nullptr, nullptr, segs, kc.eq};
-  return run_all<float>(pk, mv, reinterpret_cast<const CellXS<float> *>(xs), kc, K, sink, outcomes);
+                         const int32_t *face_kind, const float *xs, const void *accel, const double *acc_params,
+                         const int64_t *consts, int32_t K, uint64_t *dep, uint64_t *pint, uint64_t *esc,
+                         uint32_t *segs, int64_t *counts) {
+  try {
+    MeshView<float> mv;
+    mv.nx = nx;
+    mv.ny = ny;
+    mv.nz = nz;
+    for (int f = 0; f < 6; ++f) mv.face_kind[f] = face_kind[f];
+    mv.width[0] = widths;
+    mv.width[1] = widths + nx;
+    mv.width[2] = widths + nx + ny;
+    StepConsts kc;
+    kc.seed = static_cast<uint64_t>(consts[0]);
+    kc.step = static_cast<uint32_t>(consts[1]);
+    kc.eq = static_cast<int32_t>(consts[2]);
+    kc.et = static_cast<int32_t>(consts[3]);
+    const int rw_on = acc_params ? static_cast<int>(acc_params[0]) : 0;
+    const int ddmc_on = acc_params ? static_cast<int>(acc_params[1]) : 0;
+    std::unique_ptr<HostTables<float>> tb(rw_on ? new HostTables<float>() : nullptr);
+    AccelView<float> av = make_accel_view<float>(static_cast<const CellAccel<float> *>(accel), tb.get(), rw_on,
+                                                 ddmc_on, acc_params ? acc_params[2] : 0.0,
+                                                 acc_params ? acc_params[3] : 0.0);
+    std::vector<Packet<float>> pk(static_cast<size_t>(n));
+    std::memcpy(pk.data(), packets, static_cast<size_t>(n) * sizeof(Packet<float>));
+    HostSink sink{dep, pint, esc, nullptr, nullptr, nullptr, segs, kc.eq};
+    for (auto &p : pk) {
+      int o = kAlive;
+      while (o == kAlive)
+        o = transport_packet<float>(p, mv, reinterpret_cast<const CellXS<float> *>(xs), av, kc, K, sink);
+      counts[o] += 1;
+    }
+    counts[4] += static_cast<int64_t>(sink.rw);
+    counts[5] += static_cast<int64_t>(sink.ddmc);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
 }
 
 // First IMC step of the engine on the CPU (no census): preStep, photon
 // allocation, emission + boundary births and transport, in the engine's
 // arithmetic (precision 0: float + fixed point, 1: double).  Outputs per cell:
-// deposited energy (erg), Erad time average (erg/cm^3), segments; and the
-// escaped energy, census energy and packet count.
+// deposited energy (erg), Erad time average (erg/cm^3), segments; totals:
+// escaped, census energy, packets, emitted, source, census packets,
+// random-walk steps, DDMC steps.
 int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_t step, double *dep_out,
                    double *erad_out, double *segs_out, double *totals_out) {
   try {
@@ -135,7 +259,9 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
     for (uint32_t c = 0; c < N; ++c) {
       double T = p->T0[c];
       double rho = m.density ? m.density[c] : 1.0;
-      auto tfac = [&](double pw) { return pw == 0.0 ? 1.0 : std::pow(std::fmax(T, m.opacity_T_floor) / m.opacity_T_ref, -pw); };
+      auto tfac = [&](double pw) {
+        return pw == 0.0 ? 1.0 : std::pow(std::fmax(T, m.opacity_T_floor) / m.opacity_T_ref, -pw);
+      };
       auto rfac = [&](double pw) { return (pw == 0.0 || !m.density) ? 1.0 : std::pow(rho, pw); };
       sa[c] = m.sigma_a0[c] * tfac(m.a_temp_pow) * rfac(m.a_rho_pow);
       ss[c] = m.sigma_s0 ? m.sigma_s0[c] * tfac(m.s_temp_pow) * rfac(m.s_rho_pow) : 0.0;
@@ -146,7 +272,8 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       E[c] = f[c] * V[c] * units::arad * (T3 * T) * sa[c] * dt * units::clight;
       Etot += E[c];
     }
-    const uint32_t npc = p->boundary.source_face == STORM_XLO ? static_cast<uint32_t>(p->boundary.source_packets_per_face) : 0u;
+    const uint32_t npc =
+        p->boundary.source_face == STORM_XLO ? static_cast<uint32_t>(p->boundary.source_packets_per_face) : 0u;
     double src = 0.0;
     const double Tb = p->boundary.source_temperature;
     const double T4 = Tb * Tb * Tb * Tb;
@@ -160,7 +287,6 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
     const int32_t eq = fx::ceil_log2(Ebound) - fx::kHeadroomBits;
     const int32_t et = fx::ceil_log2(Ebound * units::clight * dt) - fx::kHeadroomBits;
     for (auto &w : wface) w = fx::quantize_down(w, eq);
-    // allocation
     const uint64_t nmin = p->imc.new_photons_per_cell;
     const uint64_t total = static_cast<uint64_t>(N) * nmin * 10ull;
     std::vector<uint32_t> nph(N);
@@ -181,8 +307,10 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
     std::vector<double> depf(N, 0), pintf(N, 0), escf(N, 0);
     std::vector<uint32_t> segs(N, 0);
     int64_t outc[4] = {0, 0, 0, 0};
+    uint64_t nrw = 0, nddmc = 0;
     double census = 0.0;
     StepConsts kc{p->seed, step, eq, et};
+    const bool accel_on = p->imc.with_random_walk || p->imc.with_ddmc;
     auto go = [&](auto tag) {
       using Real = decltype(tag);
       std::vector<Real> w0(nx), w1(ny), w2(nz);
@@ -202,6 +330,16 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
         xs[c].sig_abs = static_cast<Real>(f[c] * sa[c]);
         xs[c].sig_ev = static_cast<Real>(ss[c] + (1.0 - f[c]) * sa[c]);
       }
+      std::vector<CellAccel<Real>> acc_cells;
+      std::unique_ptr<HostTables<Real>> tb;
+      if (accel_on) {
+        acc_cells = build_accel<Real>(p, sa, ss);
+        if (p->imc.with_random_walk) tb.reset(new HostTables<Real>());
+      }
+      AccelView<Real> av = make_accel_view<Real>(accel_on ? acc_cells.data() : nullptr, tb.get(),
+                                                 p->imc.with_random_walk, p->imc.with_ddmc,
+                                                 p->imc.rw_min_particle_optical_depth,
+                                                 p->imc.ddmc_min_particle_optical_depth);
       const Real s0 = static_cast<Real>(units::clight * dt);
       HostSink sink{dep.data(), pint.data(), esc.data(), nullptr, nullptr, nullptr, segs.data(), eq};
       if (sizeof(Real) == 8) {
@@ -222,17 +360,19 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       for (size_t fi = 0; fi < wface.size(); ++fi) {
         for (uint32_t q = 0; q < npc; ++q) {
           Packet<Real> pp;
-          birth_boundary<Real>(pp, id++, static_cast<uint32_t>(fi / nz), static_cast<uint32_t>(fi % nz), wface[fi], s0,
-                               p->seed, step);
+          birth_boundary<Real>(pp, id++, static_cast<uint32_t>(fi / nz), static_cast<uint32_t>(fi % nz), wface[fi],
+                               s0, p->seed, step);
           pk.push_back(pp);
         }
       }
       for (auto &pp : pk) {
         int o = kAlive;
-        while (o == kAlive) o = transport_packet<Real>(pp, mv, xs.data(), kc, 1 << 20, sink);
+        while (o == kAlive) o = transport_packet<Real>(pp, mv, xs.data(), av, kc, 1 << 20, sink);
         outc[o] += 1;
         if (o == kCensus) census += pp.w;
       }
+      nrw = sink.rw;
+      nddmc = sink.ddmc;
     };
     if (precision == 0)
       go(float{});
@@ -255,6 +395,8 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
     totals_out[3] = emitted;
     totals_out[4] = src;
     totals_out[5] = static_cast<double>(outc[kCensus]);
+    totals_out[6] = static_cast<double>(nrw);
+    totals_out[7] = static_cast<double>(nddmc);
     return 0;
   } catch (const std::exception &e) {
     g_err = e.what();

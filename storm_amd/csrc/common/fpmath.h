@@ -150,8 +150,10 @@ STORM_HD void sincos2pi(float u, float &s_out, float &c_out) {
   }
 }
 
-// Exact power of two as a double, 2^k for k in [-1022, 1023].
+// Exact power of two as a double, 2^k; k is clamped to the normal range
+// [-1022, 1023].
 STORM_HD double pow2d(int k) {
+  k = k < -1022 ? -1022 : (k > 1023 ? 1023 : k);
   return u2d(static_cast<uint64_t>(k + 1023) << 52);
 }
 

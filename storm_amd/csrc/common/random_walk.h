@@ -1,0 +1,162 @@
+// Random-walk tables and samplers (Fleck & Canfield diffusion in a sphere),
+// host/device.
+//
+// Tables (radiation/RandomWalk.hpp:20-79): 1024 dimensionless times tau
+// log-spaced on [1e-8, 64]; the survival probability S(tau) of a walker
+// started at the centre of a unit sphere with D = 1,
+//     S(tau) = 2 sum_{n>=1} (-1)^(n+1) exp(-n^2 pi^2 tau),
+// and for each tau the inverse of the radial CDF of the walker's position
+// given survival, tabulated on 256 uniform levels xi in [0, 1]:
+//     G(x, tau) = 2 / S(tau) sum_n exp(-n^2 pi^2 tau) (sin(n pi x) / (n pi) - x cos(n pi x))
+// (free-space form erf(a) - 2 a exp(-a^2) / sqrt(pi), a = x / (2 sqrt(tau)),
+// below tau = 0.01), inverted by linear interpolation on a 512-point x grid.
+// The series are truncated after 500 terms or once n^2 pi^2 tau > 700, as
+// the reference does.
+//
+// Samplers (RandomWalk.hpp:81-137): leak time by inverse interpolation of S
+// in log tau, radius by bilinear interpolation in (log tau, xi).
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+#include "hd.h"
+#include "mathpolicy.h"
+
+namespace storm {
+namespace rw {
+
+constexpr int kTableSize = 1024;
+constexpr int kRadiusSize = 256;
+constexpr double kTauMin = 1e-8;
+constexpr double kTauMax = 64.0;
+
+template <typename Real>
+struct Tables {
+  const Real *tau;       // [kTableSize]
+  const Real *survival;  // [kTableSize], decreasing
+  const Real *radius;    // [kTableSize * kRadiusSize]
+  Real log_tau_min, log_tau_max;
+};
+
+// Leak time for a uniform xi: smallest table index with S <= 1 - xi, then
+// log-linear interpolation between its neighbours.
+template <typename Real>
+STORM_HD Real sample_leak_time(const Tables<Real> &t, Real xi) {
+  using M = MathPolicy<Real>;
+  const Real target = Real(1) - xi;
+  if (target >= t.survival[0]) return t.tau[0];
+  if (target <= t.survival[kTableSize - 1]) return t.tau[kTableSize - 1];
+  int lo = 0, hi = kTableSize - 1;  // survival[lo] > target >= survival[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (t.survival[mid] > target) lo = mid;
+    else hi = mid;
+  }
+  const Real S0 = t.survival[hi - 1], S1 = t.survival[hi];
+  const Real frac = (S0 - target) / (S0 - S1);
+  const Real l0 = M::log(t.tau[hi - 1]), l1 = M::log(t.tau[hi]);
+  return M::exp(l0 + frac * (l1 - l0));
+}
+
+template <typename Real>
+STORM_HD Real sample_radius(const Tables<Real> &t, Real tau, Real xi) {
+  using M = MathPolicy<Real>;
+  const Real tmin = t.tau[0], tmax = t.tau[kTableSize - 1];
+  const Real tc = tau < tmin ? tmin : (tau > tmax ? tmax : tau);
+  const Real pos = (M::log(tc) - t.log_tau_min) / (t.log_tau_max - t.log_tau_min) * Real(kTableSize - 1);
+  int ti = static_cast<int>(pos);
+  if (ti > kTableSize - 2) ti = kTableSize - 2;
+  if (ti < 0) ti = 0;
+  const Real tf = pos - static_cast<Real>(ti);
+  const Real xc = xi < Real(0) ? Real(0) : (xi > Real(1) ? Real(1) : xi);
+  const Real xp = xc * Real(kRadiusSize - 1);
+  int xj = static_cast<int>(xp);
+  if (xj > kRadiusSize - 2) xj = kRadiusSize - 2;
+  const Real xf = xp - static_cast<Real>(xj);
+  const Real *r0 = t.radius + ti * kRadiusSize;
+  const Real *r1 = r0 + kRadiusSize;
+  const Real a = r0[xj] + xf * (r0[xj + 1] - r0[xj]);
+  const Real b = r1[xj] + xf * (r1[xj + 1] - r1[xj]);
+  const Real r = a + tf * (b - a);
+  return r < Real(0) ? Real(0) : (r > Real(1) ? Real(1) : r);
+}
+
+// ---------------------------------------------------------------- host build
+// (plain host functions: built once on the host and uploaded)
+inline double survival_series(double tau) {
+  if (tau <= 0.0 || tau < 1e-4) return 1.0;
+  const double pi = 3.14159265358979323846;
+  double sum = 0.0;
+  for (int n = 1; n <= 500; ++n) {
+    double arg = -static_cast<double>(n * n) * pi * pi * tau;
+    if (arg < -700.0) break;
+    double term = exp(arg);
+    sum += (n % 2 == 1) ? term : -term;
+  }
+  double s = 2.0 * sum;
+  return s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s);
+}
+
+inline double radial_cdf(double x, double tau) {
+  const double pi = 3.14159265358979323846;
+  if (tau <= 0.0) return x > 0.0 ? 1.0 : 0.0;
+  if (x <= 0.0) return 0.0;
+  if (x >= 1.0) return 1.0;
+  double v;
+  if (tau < 0.01) {
+    double a = x / (2.0 * sqrt(tau));
+    v = erf(a) - 2.0 * a * exp(-a * a) / sqrt(pi);
+  } else {
+    double S = survival_series(tau);
+    if (S < 1e-30) return 1.0;
+    double sum = 0.0;
+    for (int n = 1; n <= 500; ++n) {
+      double arg = -static_cast<double>(n * n) * pi * pi * tau;
+      if (arg < -700.0) break;
+      double npi = static_cast<double>(n) * pi, npx = npi * x;
+      sum += exp(arg) * (sin(npx) / npi - x * cos(npx));
+    }
+    v = 2.0 * sum / S;
+  }
+  return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+}
+
+// Fills tau[1024], survival[1024], radius[1024*256] in double precision.
+inline void build_tables(double *tau, double *survival, double *radius) {
+  const double lmin = log(kTauMin), lmax = log(kTauMax);
+  for (int i = 0; i < kTableSize; ++i) {
+    double f = static_cast<double>(i) / static_cast<double>(kTableSize - 1);
+    tau[i] = exp(lmin + f * (lmax - lmin));
+    survival[i] = survival_series(tau[i]);
+  }
+  constexpr int X = 512;
+  double g[X];
+  for (int i = 0; i < kTableSize; ++i) {
+    for (int k = 0; k < X; ++k) g[k] = radial_cdf(static_cast<double>(k) / (X - 1), tau[i]);
+    for (int j = 0; j < kRadiusSize; ++j) {
+      double xi = static_cast<double>(j) / (kRadiusSize - 1);
+      double &out = radius[i * kRadiusSize + j];
+      if (xi <= g[0]) {
+        out = 0.0;
+      } else if (xi >= g[X - 1]) {
+        out = 1.0;
+      } else {
+        int idx = 0;  // first k with g[k] >= xi
+        int lo = 0, hi = X - 1;
+        while (lo < hi) {
+          int mid = (lo + hi) >> 1;
+          if (g[mid] < xi) lo = mid + 1;
+          else hi = mid;
+        }
+        idx = lo < 1 ? 1 : lo;
+        double g0 = g[idx - 1], g1 = g[idx];
+        double x0 = static_cast<double>(idx - 1) / (X - 1), x1 = static_cast<double>(idx) / (X - 1);
+        double fr = (g1 > g0) ? (xi - g0) / (g1 - g0) : 0.5;
+        out = x0 + fr * (x1 - x0);
+      }
+    }
+  }
+}
+
+}  // namespace rw
+}  // namespace storm

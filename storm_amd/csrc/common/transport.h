@@ -3,21 +3,26 @@
 // One function, compiled for gfx950 (the transport kernels) and for the host
 // (the bit-exact replay used by the tests), advances one packet through up to
 // `max_segments` segments.  A segment is one call of the reference's
-// RadiationIMC::step (radiation/RadiationIMC.hpp:1700-1881): the distance to
-// the nearest cell face, to the next effective-scatter event and to census is
-// compared, the packet moves, its weight is attenuated by the continuous
-// absorption factor, the absorbed energy and the time-integrated radiation
-// energy are tallied in the cell, and the event is applied (face crossing with
-// the manager's cell move or boundary call, MonteCarloManagerSerial.hpp:318-357;
-// isotropic redirection; census; roulette below 1e-3 of the step's starting
-// weight, RadiationIMC.hpp:1812-1820).
+// RadiationIMC::step (radiation/RadiationIMC.hpp:1700-1881), counted like
+// cellsStepsCounters (MonteCarloManagerSerial.hpp:302-303):
+//   * in a random-walk cell, a random-walk step if the packet is deep enough
+//     (tryRandomWalkStep, :1065-1276);
+//   * else in a DDMC cell, a discrete-diffusion step (tryDDMCStep, :1415-1612);
+//   * else analytic tracking: the distance to the nearest cell face, to the
+//     next effective-scatter event and to census is compared, the packet
+//     moves, its weight is attenuated by the continuous absorption factor, the
+//     absorbed energy and the time-integrated radiation energy are tallied in
+//     the cell, and the event is applied (face crossing with the manager's
+//     cell move or boundary call, MonteCarloManagerSerial.hpp:318-357;
+//     isotropic redirection; census).  Roulette below 1e-3 of the step's
+//     starting weight (1e-4 in a random-walk step) deposits the rest.
 //
 // Layout decisions (MI355X): the position is the fraction of the cell width
 // on each axis (u in [0,1]^3), so a face crossing sets one coordinate to 0 or
 // 1 exactly instead of nudging by 1e-8 as the FP64 reference does; the
-// direction is a unit vector; path lengths are in cm (time * c).  The Real
-// type is float for the mixed engine (deterministic own math, fixed-point
-// tallies through the Sink) and double for the plain FP64 engine.
+// direction is a unit vector; times are carried as path lengths in cm
+// (time * c).  The Real type is float for the mixed engine (deterministic own
+// math, fixed-point tallies) and double for the plain FP64 engine.
 #pragma once
 
 #include <stdint.h>
@@ -26,6 +31,7 @@
 #include "fixed.h"
 #include "fpmath.h"
 #include "hd.h"
+#include "mathpolicy.h"
 #include "philox.h"
 
 namespace storm {
@@ -51,11 +57,37 @@ STORM_HD uint32_t flat_cell(const MeshView<Real> &m, uint32_t ijk) {
 }
 
 // Per-cell opacities used by a segment (cm^-1):
-//   sig_abs = f * sigma_a            continuous absorption (weight attenuation)
+//   sig_abs = f * sigma_a              continuous absorption (weight attenuation)
 //   sig_ev  = sigma_s + (1-f)*sigma_a  effective-scatter event opacity
 template <typename Real>
 struct CellXS {
   Real sig_abs, sig_ev;
+};
+
+// Per-cell data of the diffusion accelerations (gray):
+//   flags bit 0 random-walk cell, bit 1 DDMC cell
+//   sig_t  sigma_a + sigma_s (cm^-1)
+//   rate   DDMC leak rate per face / c (cm^-1), order x-, x+, y-, y+, z-, z+
+//   total  sum of rate
+template <typename Real>
+struct CellAccel {
+  Real sig_t;
+  Real total;
+  Real rate[6];
+  uint32_t flags;
+  uint32_t pad;
+};
+
+// Random-walk tables (see random_walk.h) and the particle thresholds.
+template <typename Real>
+struct AccelView {
+  const CellAccel<Real> *cell;  // null: no random walk and no DDMC in this problem
+  const Real *rw_tau;
+  const Real *rw_survival;
+  const Real *rw_radius;
+  Real rw_log_tau_min, rw_log_tau_max;
+  Real rw_particle_tau;    // rwMinParticleOpticalDepth
+  Real ddmc_particle_tau;  // ddmcMinParticleOpticalDepth
 };
 
 template <typename Real>
@@ -81,57 +113,21 @@ struct StepConsts {
 
 enum TransportOutcome : int { kAlive = 0, kCensus = 1, kGone = 2, kEscaped = 3 };
 
-// ---------------------------------------------------------------- math policy
-template <typename Real>
-struct MathPolicy;
+}  // namespace storm
 
-template <>
-struct MathPolicy<float> {
-  STORM_HD static float log(float x) { return fpm::logf_d(x); }
-  STORM_HD static float expm1(float x) { return fpm::expm1f_d(x); }
-  STORM_HD static void sincos2pi(float u, float &s, float &c) { fpm::sincos2pi(u, s, c); }
-  STORM_HD static float sqrt(float x) { return ::sqrtf(x); }
-  STORM_HD static float uniform(uint32_t x) { return u01_f32(x); }
-  STORM_HD static float inf() { return fpm::u2f(0x7f800000u); }
-};
+#include "random_walk.h"
 
-template <>
-struct MathPolicy<double> {
-  STORM_HD static double log(double x) { return ::log(x); }
-  STORM_HD static double expm1(double x) { return ::expm1(x); }
-  STORM_HD static void sincos2pi(double u, double &s, double &c) {
-    double a = 6.283185307179586476925286766559 * u;
-    s = ::sin(a);
-    c = ::cos(a);
-  }
-  STORM_HD static double sqrt(double x) { return ::sqrt(x); }
-  STORM_HD static double uniform(uint32_t x) { return (static_cast<double>(x) + 0.5) * 2.3283064365386963e-10; }
-  STORM_HD static double inf() { return fpm::u2d(0x7ff0000000000000ull); }
-};
-
-// Isotropic unit vector from two uniforms: mu = 2 u0 - 1, phi = 2 pi u1
-// (isotropicVelocity, radiation/RadiationOpacityModel.hpp:139-150).
-template <typename Real>
-STORM_HD void isotropic_dir(Real u0, Real u1, Real o[3]) {
-  Real mu = Real(2) * u0 - Real(1);
-  Real st2 = Real(1) - mu * mu;
-  Real st = MathPolicy<Real>::sqrt(st2 > Real(0) ? st2 : Real(0));
-  Real sp, cp;
-  MathPolicy<Real>::sincos2pi(u1, sp, cp);
-  o[0] = st * cp;
-  o[1] = st * sp;
-  o[2] = mu;
-}
+namespace storm {
 
 // ---------------------------------------------------------------- transport
 // Sink interface (duck-typed):
-//   visit(cell, dep_limbs_or_double, path_integral, segments)   cell visit ends
-//   escape(cell, weight)                                         packet escapes
-// For Real=float the energies are passed as fixed-point limbs (exact); for
-// Real=double as plain doubles.
+//   visit(cell, dep_limbs, path_limbs, segments)  (Real = float) or
+//   visit_fp(cell, dep, path, segments)           (Real = double): a cell visit ends
+//   escape(cell, weight)                           the packet leaves the problem
+//   rw_step() / ddmc_step()                        acceleration counters
 template <typename Real, typename Sink>
 STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const CellXS<Real> *xs,
-                              const StepConsts &k, int max_segments, Sink &sink) {
+                              const AccelView<Real> &acc, const StepConsts &k, int max_segments, Sink &sink) {
   using M = MathPolicy<Real>;
   constexpr bool fixed_point = sizeof(Real) == 4;
   const PacketStream rs = PacketStream::make(k.seed, p.id, k.step, kTagTransport);
@@ -153,6 +149,36 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
     visit_segs = 0;
     path_int = 0.0;
   };
+  // close the visit and move into the neighbour across face `fax`/`up`
+  auto enter_neighbour = [&](int fax, bool up) {
+    double wq = fixed_point ? fx::quantize_down(p.w, k.eq) : p.w;
+    flush(wq);
+    p.w = wq;
+    w_enter = wq;
+    uint32_t ni = ijk_i(p.ijk), nj = ijk_j(p.ijk), nk = ijk_k(p.ijk);
+    if (fax == 0) ni = up ? ni + 1u : ni - 1u;
+    else if (fax == 1) nj = up ? nj + 1u : nj - 1u;
+    else nk = up ? nk + 1u : nk - 1u;
+    p.ijk = pack_ijk(ni, nj, nk);
+    p.u[fax] = up ? Real(0) : Real(1);
+    cell = flat_cell(m, p.ijk);
+  };
+  // continuous absorption over a path d (cm); `tally_free` adds w * d to the
+  // path integral when nothing is absorbed (the analytic step and DDMC do,
+  // the random-walk step does not, RadiationIMC.hpp:1163-1165)
+  auto absorb = [&](Real sig_abs, Real d, bool tally_free) {
+    Real xabs = sig_abs * d;
+    Real frac = -M::expm1(-xabs);
+    double w0 = p.w;
+    if (xabs >= Real(1e-12)) {
+      path_int += w0 * static_cast<double>(frac / sig_abs);
+    } else if (tally_free) {
+      path_int += w0 * static_cast<double>(d);
+    } else if (sig_abs > Real(0)) {
+      path_int += w0 * static_cast<double>(frac / sig_abs);
+    }
+    p.w = w0 - w0 * static_cast<double>(frac);
+  };
 
   for (int seg = 0; seg < max_segments; ++seg) {
     const uint32_t i = ijk_i(p.ijk), j = ijk_j(p.ijk), kk = ijk_k(p.ijk);
@@ -160,9 +186,138 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
     const CellXS<Real> x = xs[cell];
     ++visit_segs;
 
-    // Random numbers of this segment: one Philox block.
+    // Random numbers of this segment: one Philox block (two for an
+    // acceleration step).
     Philox4 r = rs.block(p.ctr++);
 
+    if (acc.cell) {
+      const CellAccel<Real> ca = acc.cell[cell];
+      if (ca.flags) {
+        // distance to the nearest face (computeMinDistanceToFaces, :956-969)
+        Real Ro = M::inf();
+        for (int a = 0; a < 3; ++a) {
+          Real lo = p.u[a] * width[a], hi = (Real(1) - p.u[a]) * width[a];
+          Real dm = lo < hi ? lo : hi;
+          Ro = dm < Ro ? dm : Ro;
+        }
+        const Real tauR = Ro * ca.sig_t;
+        if ((ca.flags & 1u) && Ro > Real(0) && ca.sig_t > Real(0) && tauR >= acc.rw_particle_tau) {
+          // ---------------- random-walk step (tryRandomWalkStep, gray)
+          Philox4 r2 = rs.block(p.ctr++);
+          rw::Tables<Real> tb{acc.rw_tau, acc.rw_survival, acc.rw_radius, acc.rw_log_tau_min, acc.rw_log_tau_max};
+          const Real tauLeak = rw::sample_leak_time<Real>(tb, M::uniform(r.v[0]));
+          // tLeak = tau Ro^2 / D with D = c / (3 sigma_t): as a path, tau Ro^2 3 sigma_t
+          const Real sLeak = tauLeak * Ro * Ro * (Real(3) * ca.sig_t);
+          const bool leak = sLeak <= p.s;
+          const Real d = leak ? sLeak : p.s;
+          absorb(x.sig_abs, d, false);
+          p.s = leak ? p.s - d : Real(0);
+          if (p.s < Real(0)) p.s = Real(0);
+          sink.rw_step();
+          if (p.w < static_cast<double>(p.wmin * Real(0.1))) {
+            flush(0.0);
+            p.w = 0.0;
+            outcome = kGone;
+            break;
+          }
+          // new position on the sphere (leak) or inside it (census)
+          Real dir[3];
+          isotropic_dir<Real>(M::uniform(r.v[1]), M::uniform(r.v[2]), dir);
+          // posDir = (sin t cos phi, sin t sin phi, cos t), cos t = 2 U - 1
+          Real disp = Ro;
+          if (!leak) {
+            const Real tauPos = d / (Real(3) * ca.sig_t * Ro * Ro);
+            disp = Ro * rw::sample_radius<Real>(tb, tauPos, M::uniform(r.v[3]));
+          }
+          if (disp > Ro) disp = Ro;
+          Real c0[3] = {p.u[0], p.u[1], p.u[2]};
+          Real nu[3];
+          for (int a = 0; a < 3; ++a) {
+            nu[a] = c0[a] + disp * dir[a] / width[a];
+            nu[a] = nu[a] * Real(1 - 1e-6) + Real(1e-6) * Real(0.5);  // nudge toward the centre
+          }
+          for (int guard = 0; guard < 2000; ++guard) {
+            bool inside = true;
+            for (int a = 0; a < 3; ++a) inside = inside && nu[a] >= Real(0) && nu[a] <= Real(1);
+            if (inside) break;
+            disp = disp * Real(0.99);
+            for (int a = 0; a < 3; ++a) nu[a] = c0[a] + disp * dir[a] / width[a];
+          }
+          for (int a = 0; a < 3; ++a) p.u[a] = nu[a] < Real(0) ? Real(0) : (nu[a] > Real(1) ? Real(1) : nu[a]);
+          isotropic_dir<Real>(M::uniform(r2.v[0]), M::uniform(r2.v[1]), p.o);
+          if (!leak) {
+            double wq = fx::quantize_down(p.w, k.eq);
+            flush(wq);
+            p.w = wq;
+            outcome = kCensus;
+            break;
+          }
+          continue;
+        }
+        if ((ca.flags & 2u) && ca.total > Real(0) && tauR >= acc.ddmc_particle_tau) {
+          // ---------------- DDMC step (tryDDMCStep, gray)
+          const Real sEvent = (-M::log(M::uniform(r.v[0]))) / ca.total;
+          const bool census = p.s <= sEvent;
+          const Real d = census ? p.s : sEvent;
+          absorb(x.sig_abs, d, true);
+          p.s = census ? Real(0) : p.s - d;
+          if (p.s < Real(0)) p.s = Real(0);
+          sink.ddmc_step();
+          if (p.w < static_cast<double>(p.wmin)) {
+            flush(0.0);
+            p.w = 0.0;
+            outcome = kGone;
+            break;
+          }
+          if (census) {
+            double wq = fx::quantize_down(p.w, k.eq);
+            flush(wq);
+            p.w = wq;
+            outcome = kCensus;
+            break;
+          }
+          // leak through a face chosen in proportion to its rate
+          Real pick = M::uniform(r.v[1]) * ca.total;
+          int face = -1, lastf = -1;
+          for (int f = 0; f < 6; ++f) {
+            if (ca.rate[f] > Real(0)) {
+              lastf = f;
+              pick -= ca.rate[f];
+              if (face < 0 && pick <= Real(0)) face = f;
+            }
+          }
+          if (face < 0) face = lastf;
+          const int fax = face >> 1;
+          const bool up = (face & 1) != 0;
+          // direction: mu = sqrt(U) about the outward normal, basis as in :1565-1586
+          const Real mu = M::sqrt(M::uniform(r.v[2]));
+          const Real st2 = Real(1) - mu * mu;
+          const Real st = M::sqrt(st2 > Real(0) ? st2 : Real(0));
+          Real sp, cp;
+          M::sincos2pi(M::uniform(r.v[3]), sp, cp);
+          const Real sg = up ? Real(1) : Real(-1);
+          Real nd[3] = {Real(0), Real(0), Real(0)};
+          nd[fax] = sg * mu;
+          if (fax == 0) {          // e1 = y, e2 = n x y = sg z
+            nd[1] = st * cp;
+            nd[2] = sg * st * sp;
+          } else if (fax == 1) {   // e1 = x, e2 = n x x = -sg z
+            nd[0] = st * cp;
+            nd[2] = -sg * st * sp;
+          } else {                 // e1 = x, e2 = n x x = sg y
+            nd[0] = st * cp;
+            nd[1] = sg * st * sp;
+          }
+          for (int a = 0; a < 3; ++a) p.o[a] = nd[a];
+          // placed at the face centroid (:1588), in the neighbour
+          for (int a = 0; a < 3; ++a) p.u[a] = Real(0.5);
+          enter_neighbour(fax, up);
+          continue;
+        }
+      }
+    }
+
+    // ---------------- analytic step
     // Distance to the nearest face along the direction.
     Real dface = M::inf(), dsecond = M::inf();
     int fax = -1, sax = -1;
@@ -217,16 +372,7 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
     p.s -= d;
     if (ev == 2 || p.s < Real(0)) p.s = Real(0);
 
-    // Continuous absorption over the segment.
-    Real xabs = x.sig_abs * d;
-    Real frac = -M::expm1(-xabs);
-    double w0 = p.w;
-    if (xabs >= Real(1e-12)) {
-      path_int += w0 * static_cast<double>(frac / x.sig_abs);
-    } else {
-      path_int += w0 * static_cast<double>(d);
-    }
-    p.w = w0 - w0 * static_cast<double>(frac);
+    absorb(x.sig_abs, d, true);
 
     // Move (the face axis is set exactly below).
     for (int a = 0; a < 3; ++a) {
@@ -250,16 +396,7 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
       bool outside = up ? (idx + 1u >= n) : (idx == 0u);
       if (!outside) {
         // interior face: the packet enters the neighbour on the opposite face
-        double wq = fixed_point ? fx::quantize_down(p.w, k.eq) : p.w;
-        flush(wq);
-        p.w = wq;
-        w_enter = wq;
-        uint32_t ni = i, nj = j, nk = kk;
-        uint32_t nidx = up ? idx + 1u : idx - 1u;
-        if (fax == 0) ni = nidx; else if (fax == 1) nj = nidx; else nk = nidx;
-        p.ijk = pack_ijk(ni, nj, nk);
-        p.u[fax] = up ? Real(0) : Real(1);
-        cell = flat_cell(m, p.ijk);
+        enter_neighbour(fax, up);
       } else {
         int face = 2 * fax + (up ? 1 : 0);
         p.u[fax] = up ? Real(1) : Real(0);

@@ -111,6 +111,7 @@ struct EngineBase {
   virtual void import_census(const storm_packet_record *, uint64_t) = 0;
   virtual void debug_inputs(uint64_t *, void *, void *, int64_t *) = 0;
   virtual void debug_tallies(uint64_t *, uint64_t *, uint64_t *, uint32_t *) = 0;
+  virtual void debug_accel(void *, uint64_t *) = 0;
   storm_gpu_counters counters{};
   double source_T = 0.0;
   int32_t block = 256;
@@ -132,7 +133,7 @@ struct Engine final : EngineBase {
   bool frozen = false;
   double frozenT4 = 0.0;
   uint32_t step_index = 0;
-  int32_t eq_prev = -1074;
+  int32_t eq_prev = 0;  // quantum exponent of the resident census weights
   double census_total_prev = 0.0;  // erg
   uint64_t n_census = 0;           // census resident in `cnext`
 
@@ -151,6 +152,10 @@ struct Engine final : EngineBase {
   DevBuf<double> tally_f64;
   DevBuf<ulonglong2> limbs_sorted, cum;
   DevBuf<CombCell> comb_cells;
+  DevBuf<CellAccel<Real>> accel;
+  DevBuf<double> Dtil;
+  DevBuf<Real> rw_tau, rw_surv, rw_rad;
+  bool accel_on = false;
   DevBuf<unsigned char> temp;
   Tallies tal{};
 
@@ -180,8 +185,7 @@ struct Engine final : EngineBase {
     imc = p->imc;
     seed = p->seed;
     if (imc.new_photons_per_cell == 0) throw EngineError(1, "RadiationIMC requires newPhotonsPerCell > 0");
-    if (imc.with_random_walk || imc.with_ddmc)
-      throw EngineError(3, "random walk / DDMC are not available in this engine build");
+    accel_on = imc.with_random_walk || imc.with_ddmc;
     if (bnd.source_face != -1 && bnd.source_face != STORM_XLO)
       throw EngineError(1, "only an x- Planck source face is supported (SideTemperature)");
     if (!p->material.sigma_a0 && material_mode == STORM_GPU_MATERIAL_DEVICE)
@@ -236,8 +240,26 @@ struct Engine final : EngineBase {
     emit_limbs.ensure(2 * static_cast<size_t>(N));
     W.ensure(2 * static_cast<size_t>(N));
     requant.ensure(2 * static_cast<size_t>(N));
-    leaving.ensure(1);
+    leaving.ensure(4);  // escapes, random-walk steps, DDMC steps
     counters_u32.ensure(4);
+    if (accel_on) {
+      accel.ensure(N);
+      Dtil.ensure(N);
+      if (imc.with_random_walk) {
+        // random-walk tables built once on the host, like the RandomWalk
+        // constructor the reference calls from RadiationIMC's constructor
+        std::vector<double> t(rw::kTableSize), s(rw::kTableSize), r(rw::kTableSize * rw::kRadiusSize);
+        rw::build_tables(t.data(), s.data(), r.data());
+        auto up_r = [&](DevBuf<Real> &d, const std::vector<double> &v) {
+          std::vector<Real> h(v.begin(), v.end());
+          d.ensure(h.size());
+          HIPCHECK(hipMemcpy(d.p, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice));
+        };
+        up_r(rw_tau, t);
+        up_r(rw_surv, s);
+        up_r(rw_rad, r);
+      }
+    }
     tally_u64.ensure(7 * 2 * static_cast<size_t>(N));
     tally_f64.ensure(3 * static_cast<size_t>(N));
     DevBuf<uint32_t> *segs_begin = new DevBuf<uint32_t>();  // owned below
@@ -305,6 +327,21 @@ struct Engine final : EngineBase {
     m.cvTfloor = matd.eos_cv_T_floor;
     return m;
   }
+  AccelView<Real> accelview() const {
+    AccelView<Real> a;
+    a.cell = accel_on ? accel.p : nullptr;
+    a.rw_tau = rw_tau.p;
+    a.rw_survival = rw_surv.p;
+    a.rw_radius = rw_rad.p;
+    a.rw_log_tau_min = static_cast<Real>(std::log(rw::kTauMin));
+    a.rw_log_tau_max = static_cast<Real>(std::log(rw::kTauMax));
+    // a particle threshold of +inf switches a method off
+    a.rw_particle_tau = imc.with_random_walk ? static_cast<Real>(imc.rw_min_particle_optical_depth)
+                                             : MathPolicy<Real>::inf();
+    a.ddmc_particle_tau = imc.with_ddmc ? static_cast<Real>(imc.ddmc_min_particle_optical_depth)
+                                        : MathPolicy<Real>::inf();
+    return a;
+  }
   MeshDims dims() const { return MeshDims{static_cast<uint32_t>(nx), static_cast<uint32_t>(ny), static_cast<uint32_t>(nz)}; }
   MeshView<Real> meshview() const {
     MeshView<Real> m;
@@ -353,6 +390,13 @@ struct Engine final : EngineBase {
     k_prestep<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
         N, T.p, md, have_host_sa ? host_sa.p : nullptr, have_host_ss ? host_ss.p : nullptr,
         have_host_cv ? host_cv.p : nullptr, V.p, dt, xs.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p);
+    if (accel_on) {
+      const MeshView<Real> mv0 = meshview();
+      k_accel_cells<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
+          N, d, mv0, sig_a.p, sig_s.p, imc.with_random_walk, imc.rw_min_cell_optical_depth, imc.with_ddmc,
+          imc.ddmc_min_cell_optical_depth, Dtil.p, accel.p);
+      k_accel_rates<Real><<<blocks_for(N, 256), 256, 0, stream>>>(N, d, mv0, Dtil.p, accel.p);
+    }
     const double Etot = deterministic_sum(Eemit.p, N);
 
     // boundary source: SideTemperature energy per face (T^4 frozen at the first call if asked)
@@ -419,7 +463,7 @@ struct Engine final : EngineBase {
     if (precision == STORM_GPU_FP64) zero_f64(tally_f64.p, tally_f64.n);
     zero_u32(tal.segs, 2 * static_cast<size_t>(N));
     zero_u64(requant.p, requant.n);
-    zero_u64(leaving.p, 1);
+    zero_u64(leaving.p, 4);
 
     // packets: [census | emission | boundary]
     pa.ensure(n_total);
@@ -449,6 +493,7 @@ struct Engine final : EngineBase {
 
     // transport passes until every packet is in census, absorbed or gone
     const MeshView<Real> mv = meshview();
+    const AccelView<Real> av = accelview();
     uint64_t n_in = n_total;
     bool ping = true;
     double t_transport = 0.0, t_max = 0.0;
@@ -463,7 +508,7 @@ struct Engine final : EngineBase {
       HIPCHECK(hipEventRecord(ev0, stream));
       k_transport<Real><<<blocks_for(n_in, block), block, 0, stream>>>(
           static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
-          xs.p, kc, seg_per_pass, tal, leaving.p);
+          xs.p, av, kc, seg_per_pass, tal, leaving.p);
       HIPCHECK(hipEventRecord(ev1, stream));
       uint32_t h[2];
       HIPCHECK(hipMemcpyAsync(h, counters_u32.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
@@ -494,11 +539,13 @@ struct Engine final : EngineBase {
         clamp.p, err.p, precision == STORM_GPU_FP64 ? 1 : 0);
     census_total_prev = deterministic_sum(census_erad.p, N);
     uint32_t herr = 0;
-    unsigned long long hleave = 0;
+    unsigned long long hstats[4] = {0, 0, 0, 0};
     HIPCHECK(hipMemcpyAsync(&herr, err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipMemcpyAsync(&hleave, leaving.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipMemcpyAsync(hstats, leaving.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
-    counters.leaving_particles = hleave;
+    counters.leaving_particles = hstats[0];
+    counters.random_walk_steps = hstats[1];
+    counters.ddmc_steps = hstats[2];
     eq_prev = eq;
     last_eq_ = eq;
     last_et_ = et;
@@ -650,6 +697,12 @@ struct Engine final : EngineBase {
     }
   }
 
+  void debug_accel(void *out, uint64_t *bytes) override {
+    const uint64_t nb = accel_on ? static_cast<uint64_t>(N) * sizeof(CellAccel<Real>) : 0;
+    *bytes = nb;
+    if (out && nb) HIPCHECK(hipMemcpy(out, accel.p, nb, hipMemcpyDeviceToHost));
+  }
+
   void debug_tallies(uint64_t *dep, uint64_t *pint, uint64_t *esc, uint32_t *segs) override {
     const size_t n2 = 2 * static_cast<size_t>(N);
     if (dep) HIPCHECK(hipMemcpy(dep, tal.dep, n2 * 8, hipMemcpyDeviceToHost));
@@ -716,8 +769,16 @@ struct Engine final : EngineBase {
         esum += f[2 * static_cast<size_t>(N) + c];
       }
       const long double Q = std::ldexp(1.0L, last_eq_);
-      L->deposited = u128(static_cast<unsigned __int128>(std::llround(static_cast<double>(dsum / Q) / 1.0)));
-      L->escaped = u128(static_cast<unsigned __int128>(std::llround(static_cast<double>(esum / Q))));
+      auto to_u128 = [](long double x) {
+        if (!(x > 0.0L)) return static_cast<unsigned __int128>(0);
+        const long double two64 = std::ldexp(1.0L, 64);
+        long double hi = std::floor(x / two64);
+        long double lo = std::floor(x - hi * two64 + 0.5L);
+        return (static_cast<unsigned __int128>(static_cast<uint64_t>(hi)) << 64) +
+               static_cast<unsigned __int128>(static_cast<uint64_t>(lo));
+      };
+      L->deposited = u128(to_u128(dsum / Q));
+      L->escaped = u128(to_u128(esum / Q));
     }
     L->census_pre = u128(sum(b + 4 * n2));
     L->census_out = u128(sum(b + 5 * n2));
@@ -751,7 +812,7 @@ struct Engine final : EngineBase {
     L->material_after = ms;
     L->material_before = material_before_;
   }
-  int32_t eq_prev_ledger_ = -1074;
+  int32_t eq_prev_ledger_ = 0;
   double material_before_ = 0.0;
 
   void set_cell_state(const double *Ti, const double *sa, const double *ss, const double *cv,
@@ -820,7 +881,8 @@ struct Engine final : EngineBase {
       int idx = static_cast<int>(it - e.begin()) - 1;
       return std::clamp(idx, 0, static_cast<int>(e.size()) - 2);
     };
-    const double qe = fpm::pow2d(eq_prev);
+    std::vector<uint32_t> hcell(n);
+    double tot = 0.0;
     for (uint64_t q = 0; q < n; ++q) {
       const storm_packet_record &r = in[q];
       int i = r.cell < N ? static_cast<int>(r.cell / (static_cast<uint64_t>(ny) * nz)) : locate(xe, r.x);
@@ -842,8 +904,28 @@ struct Engine final : EngineBase {
       hd[q].w = Real(0);
       if (!std::isfinite(r.weight) || r.weight < 0.0) throw EngineError(1, "census packet weight is negative or not finite");
       hw[q] = r.weight;
+      tot += r.weight;
+      hcell[q] = static_cast<uint32_t>((static_cast<uint64_t>(i) * ny + j) * nz + k);
       hm[q] = make_uint2(pack_ijk(i, j, k), static_cast<uint32_t>(q));
-      (void)qe;
+    }
+    // imported weights are arbitrary doubles: put them on a quantum of their
+    // own total; the sub-quantum remainders go to the material of their cells
+    const int32_t eq_imp = tot > 0.0 ? fx::ceil_log2(tot) - fx::kHeadroomBits : 0;
+    std::vector<double> rem(N, 0.0);
+    bool any_rem = false;
+    for (uint64_t q = 0; q < n; ++q) {
+      double wq = fx::quantize_down(hw[q], eq_imp);
+      if (wq != hw[q]) {
+        rem[hcell[q]] += hw[q] - wq;
+        any_rem = true;
+      }
+      hw[q] = wq;
+    }
+    if (any_rem) {
+      std::vector<double> e(N);
+      HIPCHECK(hipMemcpy(e.data(), e_mat.p, N * sizeof(double), hipMemcpyDeviceToHost));
+      for (uint32_t c = 0; c < N; ++c) e[c] += rem[c];
+      HIPCHECK(hipMemcpy(e_mat.p, e.data(), N * sizeof(double), hipMemcpyHostToDevice));
     }
     cnext.ensure(std::max<uint64_t>(n, 1));
     if (n) {
@@ -853,11 +935,10 @@ struct Engine final : EngineBase {
       HIPCHECK(hipMemcpy(cnext.meta.p, hm.data(), n * sizeof(uint2), hipMemcpyHostToDevice));
     }
     n_census = n;
-    double tot = 0.0;
-    for (double w : hw) tot += w;
-    census_total_prev = tot;
-    // imported weights are arbitrary doubles: force re-quantisation at the next step
-    eq_prev = -1074;
+    double totq = 0.0;
+    for (double w : hw) totq += w;
+    census_total_prev = totq;
+    eq_prev = eq_imp;
   }
 };
 
@@ -982,6 +1063,10 @@ int storm_gpu_debug_keep_transport_inputs(storm_gpu_engine *eng, int32_t enable)
 int storm_gpu_debug_transport_inputs(storm_gpu_engine *eng, uint64_t *n, void *packets, void *cell_xs,
                                      int64_t *consts4) {
   return guarded([&] { as_engine(eng)->debug_inputs(n, packets, cell_xs, consts4); });
+}
+
+int storm_gpu_debug_accel(storm_gpu_engine *eng, void *out, uint64_t *bytes) {
+  return guarded([&] { as_engine(eng)->debug_accel(out, bytes); });
 }
 
 int storm_gpu_debug_tallies(storm_gpu_engine *eng, uint64_t *dep, uint64_t *pint, uint64_t *esc, uint32_t *segs) {

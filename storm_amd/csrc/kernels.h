@@ -320,6 +320,7 @@ __global__ __launch_bounds__(256) void k_boundary(uint32_t n_bc, uint32_t npc, c
 struct DeviceSink {
   Tallies t;
   int32_t eq;
+  uint32_t rw = 0, ddmc = 0;
   __device__ void visit(uint32_t c, fx::Limbs dep, fx::Limbs pint, uint32_t segs) {
     tally_limbs(t.dep, c, dep);
     tally_limbs(t.pint, c, pint);
@@ -337,23 +338,32 @@ struct DeviceSink {
       tally_limbs(t.esc, c, fx::limbs_of(w, eq));
     }
   }
+  __device__ void rw_step() { ++rw; }
+  __device__ void ddmc_step() { ++ddmc; }
 };
 
+// Sum over the active lanes of a wavefront, added once to *dst.
+__device__ __forceinline__ void wave_add_u32(uint32_t v, unsigned long long *dst) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (v && lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u) atomicAdd(dst, static_cast<unsigned long long>(v));
+}
+
+// stats: [0] escapes, [1] random-walk steps, [2] DDMC steps
 template <typename Real>
 __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out,
                                                    uint32_t *__restrict__ n_out, CensusSoA<Real> cen,
                                                    uint32_t *__restrict__ n_cen, MeshView<Real> mesh,
-                                                   const CellXS<Real> *__restrict__ xs, StepConsts k,
-                                                   int32_t max_segments, Tallies t,
-                                                   unsigned long long *__restrict__ leaving) {
+                                                   const CellXS<Real> *__restrict__ xs, AccelView<Real> acc,
+                                                   StepConsts k, int32_t max_segments, Tallies t,
+                                                   unsigned long long *__restrict__ stats) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_in;
   int outcome = kGone;
   Packet<Real> p;
+  DeviceSink sink{t, k.eq};
   if (valid) {
     p = load_packet(in, i);
-    DeviceSink sink{t, k.eq};
-    outcome = transport_packet<Real>(p, mesh, xs, k, max_segments, sink);
+    outcome = transport_packet<Real>(p, mesh, xs, acc, k, max_segments, sink);
   }
   const bool alive = valid && outcome == kAlive;
   const bool census = valid && outcome == kCensus;
@@ -365,9 +375,81 @@ __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real
     uint32_t c = flat_cell(mesh, p.ijk);
     tally_limbs(t.cpre, c, fx::limbs_of(p.w, k.eq));
   }
-  const uint64_t esc_mask = __builtin_amdgcn_ballot_w64(valid && outcome == kEscaped);
-  if (esc_mask && lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u)
-    atomicAdd(leaving, static_cast<unsigned long long>(__popcll(esc_mask)));
+  wave_add_u32((valid && outcome == kEscaped) ? 1u : 0u, &stats[0]);
+  if (acc.cell) {
+    wave_add_u32(sink.rw, &stats[1]);
+    wave_add_u32(sink.ddmc, &stats[2]);
+  }
+}
+
+// ------------------------------------------------------------------ accelerations
+// Per-cell random-walk / DDMC data (precomputeRandomWalkData, RadiationIMC.hpp:982-1063;
+// precomputeDDMCData, :1282-1413), gray.  Pass 1: sigma_t, D/c = 1/(3 sigma_t), flags.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_accel_cells(uint32_t N, MeshDims d, MeshView<Real> mv,
+                                                     const double *__restrict__ sig_a,
+                                                     const double *__restrict__ sig_s, int32_t rw_on,
+                                                     double rw_cell_tau, int32_t ddmc_on, double ddmc_cell_tau,
+                                                     double *__restrict__ Dtil, CellAccel<Real> *__restrict__ out) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const uint32_t ijk = ijk_of_flat(c, d);
+  const double dx = mv.width[0][ijk_i(ijk)], dy = mv.width[1][ijk_j(ijk)], dz = mv.width[2][ijk_k(ijk)];
+  const double V = dx * dy * dz;
+  const double S = 2.0 * (dy * dz + dx * dz + dx * dy);
+  const double chord = 4.0 * V / S;
+  const double st = sig_a[c] + sig_s[c];
+  CellAccel<Real> a;
+  a.sig_t = static_cast<Real>(st);
+  a.total = Real(0);
+  for (int f = 0; f < 6; ++f) a.rate[f] = Real(0);
+  a.flags = 0u;
+  a.pad = 0u;
+  if (rw_on && st * chord >= rw_cell_tau) a.flags |= 1u;
+  const double Dt = st > 0.0 ? 1.0 / (3.0 * st) : 0.0;
+  Dtil[c] = Dt;
+  if (ddmc_on && st * chord >= ddmc_cell_tau && Dt > 0.0) a.flags |= 2u;
+  out[c] = a;
+}
+
+// Pass 2: leak rate per face, D_f A / (V d) / c with the harmonic mean of
+// the two cells' D; faces on the box are skipped (RadiationIMC.hpp:1368-1371).
+template <typename Real>
+__global__ __launch_bounds__(256) void k_accel_rates(uint32_t N, MeshDims d, MeshView<Real> mv,
+                                                     const double *__restrict__ Dtil,
+                                                     CellAccel<Real> *__restrict__ out) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  CellAccel<Real> a = out[c];
+  if (!(a.flags & 2u)) return;
+  const uint32_t ijk = ijk_of_flat(c, d);
+  const uint32_t idx[3] = {ijk_i(ijk), ijk_j(ijk), ijk_k(ijk)};
+  const uint32_t n[3] = {d.nx, d.ny, d.nz};
+  const uint32_t stride[3] = {d.ny * d.nz, d.nz, 1u};
+  const double D1 = Dtil[c];
+  double total = 0.0;
+  for (int f = 0; f < 6; ++f) {
+    const int ax = f >> 1;
+    const bool up = f & 1;
+    a.rate[f] = Real(0);
+    if (up ? (idx[ax] + 1u >= n[ax]) : (idx[ax] == 0u)) continue;
+    const uint32_t nb = up ? c + stride[ax] : c - stride[ax];
+    const double D2 = Dtil[nb];
+    double Df = D1;
+    if (D2 > 0.0) Df = (D1 > 0.0 && D2 > 0.0) ? 2.0 * D1 * D2 / (D1 + D2) : fmax(D1, D2);
+    const double w = mv.width[ax][idx[ax]];
+    const double rate = Df / (w * (0.5 * w));  // A / V = 1 / w, face distance w / 2
+    if (rate > 0.0 && isfinite(rate)) {
+      a.rate[f] = static_cast<Real>(rate);
+      total += rate;
+    }
+  }
+  a.total = static_cast<Real>(total);
+  if (!(total > 0.0)) {
+    a.flags &= ~2u;
+    for (int f = 0; f < 6; ++f) a.rate[f] = Real(0);
+  }
+  out[c] = a;
 }
 
 // ------------------------------------------------------------------ comb

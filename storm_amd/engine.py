@@ -139,6 +139,15 @@ class GpuEngine:
                                                               xs.ctypes.data_as(C.c_void_p), consts))
         return pk, int(n.value), xs, [int(v) for v in consts]
 
+    def accel_data(self) -> bytes | None:
+        n = C.c_uint64()
+        self._check(self.lib.storm_gpu_debug_accel(self.h, None, C.byref(n)))
+        if n.value == 0:
+            return None
+        buf = C.create_string_buffer(n.value)
+        self._check(self.lib.storm_gpu_debug_accel(self.h, C.cast(buf, C.c_void_p), C.byref(n)))
+        return buf.raw
+
     def tallies(self):
         dep = np.empty(2 * self.N, dtype=np.uint64)
         pint = np.empty(2 * self.N, dtype=np.uint64)

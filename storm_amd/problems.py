@@ -186,6 +186,38 @@ def c2_cube(seed: int = 0, n: int = 64, packets_scale: float = 1.0, steps: int =
         comb=1, comb_nmin=max(1, int(round(10 * s))), comb_factor=1.0, seed=seed)
 
 
+def c3_step(seed: int = 0, nx: int = 256, nyz: int = 16, packets_scale: float = 1.0, steps: int = 100,
+            rw: bool = True, ddmc: bool = True) -> ProblemSpec:
+    """Two-material step-opacity slab, nx x nyz x nyz cells of 1/nx cm.
+
+    x < L/2: sigma_a = 1; x >= L/2: sigma_a = 1e4 and sigma_s = 1e4 (scattering
+    ratio 0.5).  T0 = 0.1 keV, cv = 1e15 erg/(cm^3 keV), 1 keV SideTemperature
+    on x- (escape there, reflect elsewhere).  Random walk and DDMC in cells of
+    optical thickness >= 10; the random walk needs a particle depth of 10 and
+    DDMC of 5, so packets at depths 5..10 take DDMC steps (with equal
+    thresholds DDMC is never reached: the reference tries the random walk
+    first, RadiationIMC.hpp:1723-1738)."""
+    h = 1.0 / nx
+    xe = np.linspace(0.0, 1.0, nx + 1)
+    ye = np.linspace(0.0, nyz * h, nyz + 1)
+    N = nx * nyz * nyz
+    xc = np.repeat(0.5 * (xe[1:] + xe[:-1]), nyz * nyz)
+    thick = xc >= 0.5
+    sa = np.where(thick, 1e4, 1.0)
+    ss = np.where(thick, 1e4, 0.0)
+    s = max(packets_scale, 1e-6)
+    return ProblemSpec(
+        name="c3_step", x_edges=xe, y_edges=ye, z_edges=ye.copy(),
+        sigma_a0=sa, sigma_s0=ss, T0=np.full(N, 0.1 * KEV_K), dt=1e-11, steps=steps,
+        eos_f=1e15 / KEV_K, eos_beta=1.0, eos_mu=0.0,
+        face_kind=(capi.FACE_ESCAPE,) + (capi.FACE_REFLECT,) * 5, source_face=capi.XLO, source_T=KEV_K,
+        freeze_T4=1, source_per_face=max(1, int(round(4000 * s))),
+        new_photons_per_cell=max(1, int(round(20 * s))),
+        comb=1, comb_nmin=max(1, int(round(20 * s))), comb_factor=1.0,
+        with_rw=int(rw), with_ddmc=int(ddmc), rw_cell_tau=10.0, rw_particle_tau=10.0,
+        ddmc_cell_tau=10.0, ddmc_particle_tau=5.0, seed=seed)
+
+
 MARSHAK = {
     1: dict(L=0.2, kP0=0.1, kR0=100.0, alpha=3.0, beta=0.0, f=6.860085e14 / KEV_K ** 4, eb=4.0, mu=0.0,
             Tc=1.008038, Te=1.0 / 3.0),

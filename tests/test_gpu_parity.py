@@ -64,6 +64,7 @@ def _assert_parity(ref, got, label):
 
 C1 = lambda s: P.c1_slab(seed=s, packets_scale=0.1, steps=5)  # noqa: E731
 C2 = lambda s: P.c2_cube(seed=7, n=8, packets_scale=0.2, steps=3).with_seed(s)  # noqa: E731
+C3 = lambda s: P.c3_step(seed=s, nx=32, nyz=2, packets_scale=0.1, steps=3)  # noqa: E731
 
 
 @pytest.fixture(scope="module")
@@ -86,6 +87,23 @@ def test_c1_slab_matches_oracle(c1_ref, precision):
 def test_c2_cube_matches_oracle(c2_ref, precision):
     got = gpu_ensemble(C2, range(NSEEDS), precision)
     _assert_parity(c2_ref, got, f"c2/{precision}")
+
+
+@pytest.fixture(scope="module")
+def c3_ref(oracle):
+    return oracle_ensemble(oracle, C3, range(1000, 1000 + NSEEDS))
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
+def test_c3_random_walk_ddmc_matches_oracle(c3_ref, precision):
+    """Step-opacity slab with random walk and DDMC in the thick half."""
+    got = gpu_ensemble(C3, range(NSEEDS), precision)
+    _assert_parity(c3_ref, got, f"c3/{precision}")
+    spec = C3(0)
+    with GpuEngine(spec, precision) as e:
+        e.step()
+        c = e.counters()
+    assert c["random_walk_steps"] > 0 and c["ddmc_steps"] > 0, c
 
 
 def _ledger_ok(L):
@@ -123,7 +141,7 @@ def test_fp64_energy_balance():
             assert L["census_pre"] == L["census_out"] + L["comb_rem"]
 
 
-@pytest.mark.parametrize("make", [C1, C2], ids=["c1", "c2"])
+@pytest.mark.parametrize("make", [C1, C2, C3], ids=["c1", "c2", "c3"])
 def test_replay_reproduces_tallies_bit_exactly(oracle, make):
     spec = make(5)
     with GpuEngine(spec, capi.MIXED) as e:
@@ -132,8 +150,11 @@ def test_replay_reproduces_tallies_bit_exactly(oracle, make):
             e.step()
         pk, n, xs, consts = e.transport_inputs()
         dep, pint, esc, segs = e.tallies()
-    rdep, rpint, resc, rsegs, outc = oracle.replay_transport(spec, pk, n, xs, consts)
-    assert n > 0 and outc.sum() == n
+        accel = e.accel_data()
+        cnt = e.counters()
+    rdep, rpint, resc, rsegs, outc = oracle.replay_transport(spec, pk, n, xs, consts, accel=accel)
+    assert n > 0 and outc[:4].sum() == n
+    assert outc[4] == cnt["random_walk_steps"] and outc[5] == cnt["ddmc_steps"]
     assert np.array_equal(segs.astype(np.uint64), rsegs.astype(np.uint64))
     assert np.array_equal(dep, rdep)
     assert np.array_equal(pint, rpint)
