@@ -172,7 +172,9 @@ int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segm
 int storm_gpu_debug_keep_transport_inputs(storm_gpu_engine *eng, int32_t enable);
 int storm_gpu_debug_transport_inputs(storm_gpu_engine *eng, uint64_t *n_packets, void *packets,
                                      void *cell_xs, int64_t *consts4);
-int storm_gpu_debug_tallies(storm_gpu_engine *eng, uint64_t *dep, uint64_t *pint, uint64_t *esc,
+/* Integer tallies, two limbs per cell: absorbed, rouletted, path integral
+ * (cells without absorption only), escaped; and the segment counts. */
+int storm_gpu_debug_tallies(storm_gpu_engine *eng, uint64_t *dep, uint64_t *rdep, uint64_t *pint, uint64_t *esc,
                             uint32_t *segs);
 /* Per-cell random-walk / DDMC data of the last step (bytes written to
  * `out`, which may be NULL to query the size). */

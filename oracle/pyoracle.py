@@ -129,7 +129,7 @@ def rlib():
         L.replay_transport_f32.argtypes = [C.c_int64, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                            C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_float),
                                            C.c_void_p, dp, C.POINTER(C.c_int64), C.c_int32, u64p, u64p, u64p,
-                                           C.POINTER(C.c_uint32), C.POINTER(C.c_int64)]
+                                           u64p, C.POINTER(C.c_uint32), C.POINTER(C.c_int64)]
         L.cpu_first_step.argtypes = [C.c_void_p, C.c_double, C.c_int32, C.c_uint32, dp, dp, dp, dp]
         L.replay_rw_tables.argtypes = [dp, dp, dp]
         L.replay_rw_sample.argtypes = [C.c_int32, C.c_int64, dp, dp, dp, dp]
@@ -138,7 +138,10 @@ def rlib():
 
 
 def replay_transport(spec, packets, n, xs, consts, segments_per_pass=1 << 20, accel=None):
-    """Replay `n` engine packets (ctypes array of PacketF32) on the CPU; integer tallies.
+    """Replay `n` engine packets (ctypes array of PacketF32) on the CPU.
+
+    Returns the integer tallies (dict of dep, rdep, pint, esc, segs, the
+    engine's layout) and the outcome counts.
 
     `accel` is the engine's per-cell acceleration data (bytes) when the
     problem has random walk or DDMC."""
@@ -147,10 +150,9 @@ def replay_transport(spec, packets, n, xs, consts, segments_per_pass=1 << 20, ac
     widths = np.concatenate([np.diff(spec.x_edges), np.diff(spec.y_edges), np.diff(spec.z_edges)]).astype(np.float32)
     fk = np.asarray(spec.face_kind, dtype=np.int32)
     N = spec.ncells
-    dep = np.zeros(2 * N, dtype=np.uint64)
-    pint = np.zeros(2 * N, dtype=np.uint64)
-    esc = np.zeros(2 * N, dtype=np.uint64)
-    segs = np.zeros(N, dtype=np.uint32)
+    t = {k: np.zeros(2 * N, dtype=np.uint64) for k in ("dep", "rdep", "pint", "esc")}
+    t["segs"] = np.zeros(N, dtype=np.uint32)
+    u64 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint64))  # noqa: E731
     outc = np.zeros(6, dtype=np.int64)
     cst = np.asarray(consts, dtype=np.int64)
     xs32 = np.ascontiguousarray(xs, dtype=np.float32)
@@ -162,12 +164,12 @@ def replay_transport(spec, packets, n, xs, consts, segments_per_pass=1 << 20, ac
                                 None if accbuf is None else C.cast(accbuf, C.c_void_p), _dp(accp),
                                 cst.ctypes.data_as(C.POINTER(C.c_int64)),
                                 int(segments_per_pass),
-                                dep.ctypes.data_as(C.POINTER(C.c_uint64)), pint.ctypes.data_as(C.POINTER(C.c_uint64)),
-                                esc.ctypes.data_as(C.POINTER(C.c_uint64)), segs.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                u64(t["dep"]), u64(t["rdep"]), u64(t["pint"]), u64(t["esc"]),
+                                t["segs"].ctypes.data_as(C.POINTER(C.c_uint32)),
                                 outc.ctypes.data_as(C.POINTER(C.c_int64)))
     if rc != 0:
         raise RuntimeError("replay: " + L.replay_last_error().decode())
-    return dep, pint, esc, segs, outc
+    return t, outc
 
 
 def cpu_first_step(spec, precision: int, step: int = 1):

@@ -26,7 +26,7 @@ namespace {
 thread_local std::string g_err;
 
 struct HostSink {
-  uint64_t *dep, *pint, *esc;
+  uint64_t *dep, *rdep, *pint, *esc;
   double *dep_fp, *pint_fp, *esc_fp;
   uint32_t *segs;
   int32_t eq;
@@ -37,6 +37,11 @@ struct HostSink {
     pint[2 * c] += e.lo;
     pint[2 * c + 1] += e.hi;
     segs[c] += s;
+  }
+  void roulette(uint32_t c, double w) {
+    fx::Limbs l = fx::limbs_of(w, eq);
+    rdep[2 * c] += l.lo;
+    rdep[2 * c + 1] += l.hi;
   }
   void visit_fp(uint32_t c, double d, double e, uint32_t s) {
     dep_fp[c] += d;
@@ -192,18 +197,20 @@ void replay_rw_sample(int32_t precision, int64_t n, const double *xi, const doub
 // widths: nx + ny + nz floats; xs: 2 floats per cell; consts: seed, step, eq, et;
 // accel: N CellAccel<float> records or NULL; acc_params: rw_on, ddmc_on,
 // rw particle tau, ddmc particle tau.  Tallies are accumulated into the given
-// arrays (dep/pint/esc: 2 per cell, segs: 1 per cell); counts[0..3] outcomes,
+// arrays (dep/rdep/pint/esc: 2 per cell, segs: 1 per cell); counts[0..3] outcomes,
 // counts[4] random-walk steps, counts[5] DDMC steps.
 int replay_transport_f32(int64_t n, const void *packets, int32_t nx, int32_t ny, int32_t nz, const float *widths,
                          const int32_t *face_kind, const float *xs, const void *accel, const double *acc_params,
-                         const int64_t *consts, int32_t K, uint64_t *dep, uint64_t *pint, uint64_t *esc,
-                         uint32_t *segs, int64_t *counts) {
+                         const int64_t *consts, int32_t K, uint64_t *dep, uint64_t *rdep, uint64_t *pint,
+                         uint64_t *esc, uint32_t *segs, int64_t *counts) {
   try {
     MeshView<float> mv;
     mv.nx = nx;
     mv.ny = ny;
     mv.nz = nz;
-    for (int f = 0; f < 6; ++f) mv.face_kind[f] = face_kind[f];
+    mv.escape_mask = 0u;
+    for (int f = 0; f < 6; ++f)
+      if (face_kind[f] == STORM_FACE_ESCAPE) mv.escape_mask |= 1u << f;
     mv.width[0] = widths;
     mv.width[1] = widths + nx;
     mv.width[2] = widths + nx + ny;
@@ -220,7 +227,7 @@ int replay_transport_f32(int64_t n, const void *packets, int32_t nx, int32_t ny,
                                                  acc_params ? acc_params[3] : 0.0);
     std::vector<Packet<float>> pk(static_cast<size_t>(n));
     std::memcpy(pk.data(), packets, static_cast<size_t>(n) * sizeof(Packet<float>));
-    HostSink sink{dep, pint, esc, nullptr, nullptr, nullptr, segs, kc.eq};
+    HostSink sink{dep, rdep, pint, esc, nullptr, nullptr, nullptr, segs, kc.eq};
     for (auto &p : pk) {
       int o = kAlive;
       while (o == kAlive)
@@ -303,7 +310,9 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       nph[c] = static_cast<uint32_t>(n);
       wem[c] = w;
     }
-    std::vector<uint64_t> dep(2 * N, 0), pint(2 * N, 0), esc(2 * N, 0);
+    std::vector<uint64_t> dep(2 * N, 0), rdep(2 * N, 0), pint(2 * N, 0), esc(2 * N, 0);
+    std::vector<float> sabs32(N);
+    for (uint32_t c = 0; c < N; ++c) sabs32[c] = static_cast<float>(f[c] * sa[c]);
     std::vector<double> depf(N, 0), pintf(N, 0), escf(N, 0);
     std::vector<uint32_t> segs(N, 0);
     int64_t outc[4] = {0, 0, 0, 0};
@@ -321,7 +330,9 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       mv.nx = nx;
       mv.ny = ny;
       mv.nz = nz;
-      for (int q = 0; q < 6; ++q) mv.face_kind[q] = p->boundary.face_kind[q];
+      mv.escape_mask = 0u;
+      for (int q = 0; q < 6; ++q)
+        if (p->boundary.face_kind[q] == STORM_FACE_ESCAPE) mv.escape_mask |= 1u << q;
       mv.width[0] = w0.data();
       mv.width[1] = w1.data();
       mv.width[2] = w2.data();
@@ -341,7 +352,7 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
                                                  p->imc.rw_min_particle_optical_depth,
                                                  p->imc.ddmc_min_particle_optical_depth);
       const Real s0 = static_cast<Real>(units::clight * dt);
-      HostSink sink{dep.data(), pint.data(), esc.data(), nullptr, nullptr, nullptr, segs.data(), eq};
+      HostSink sink{dep.data(), rdep.data(), pint.data(), esc.data(), nullptr, nullptr, nullptr, segs.data(), eq};
       if (sizeof(Real) == 8) {
         sink.dep_fp = depf.data();
         sink.pint_fp = pintf.data();
@@ -380,8 +391,13 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       go(double{});
     double escaped = 0.0;
     for (uint32_t c = 0; c < N; ++c) {
-      double d = precision == 0 ? fx::to_double(dep[2 * c], dep[2 * c + 1], eq) : depf[c];
-      double e = precision == 0 ? fx::to_double(pint[2 * c], pint[2 * c + 1], et) : pintf[c];
+      double d = depf[c], e = pintf[c];
+      if (precision == 0) {
+        // path integral = absorbed / sig_abs where the cell absorbs (the engine's postStep)
+        d = fx::to_double(dep[2 * c], dep[2 * c + 1], eq);
+        e = sabs32[c] > 0.0f ? d / static_cast<double>(sabs32[c]) : fx::to_double(pint[2 * c], pint[2 * c + 1], et);
+        d += fx::to_double(rdep[2 * c], rdep[2 * c + 1], eq);
+      }
       escaped += precision == 0 ? fx::to_double(esc[2 * c], esc[2 * c + 1], eq) : escf[c];
       dep_out[c] = d;
       erad_out[c] = e / (units::clight * dt * V[c]);

@@ -129,7 +129,7 @@ def load_library(path: str | None = None):
         "storm_gpu_set_tuning": ([vp, C.c_int32, C.c_int32], C.c_int),
         "storm_gpu_debug_keep_transport_inputs": ([vp, C.c_int32], C.c_int),
         "storm_gpu_debug_transport_inputs": ([vp, C.POINTER(C.c_uint64), vp, vp, C.POINTER(C.c_int64)], C.c_int),
-        "storm_gpu_debug_tallies": ([vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+        "storm_gpu_debug_tallies": ([vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)], C.c_int),
         "storm_gpu_debug_accel": ([vp, vp, C.POINTER(C.c_uint64)], C.c_int),
     }

@@ -43,11 +43,11 @@ STORM_HD uint32_t ijk_j(uint32_t p) { return (p >> 10) & 0x3ffu; }
 STORM_HD uint32_t ijk_k(uint32_t p) { return p & 0x3ffu; }
 
 // Mesh as seen by the transport: per-axis widths (cm) and the fate of a
-// packet at each box face (0 reflect, 1 escape), order x-, x+, y-, y+, z-, z+.
+// packet at each box face, order x-, x+, y-, y+, z-, z+.
 template <typename Real>
 struct MeshView {
   int32_t nx, ny, nz;
-  int32_t face_kind[6];
+  uint32_t escape_mask;  // bit f set: box face f removes the packet (else it reflects)
   const Real *width[3];  // width[a][idx] for a = x, y, z
 };
 
@@ -120,11 +120,29 @@ enum TransportOutcome : int { kAlive = 0, kCensus = 1, kGone = 2, kEscaped = 3 }
 namespace storm {
 
 // ---------------------------------------------------------------- transport
+// Component access by a runtime axis without indexing a local array (a
+// runtime index into a register array would put it in scratch memory).
+template <typename T>
+STORM_HD T sel3(int a, T x, T y, T z) {
+  return a == 0 ? x : (a == 1 ? y : z);
+}
+template <typename T>
+STORM_HD void put3(int a, T &x, T &y, T &z, T v) {
+  x = a == 0 ? v : x;
+  y = a == 1 ? v : y;
+  z = a == 2 ? v : z;
+}
+
 // Sink interface (duck-typed):
 //   visit(cell, dep_limbs, path_limbs, segments)  (Real = float) or
 //   visit_fp(cell, dep, path, segments)           (Real = double): a cell visit ends
-//   escape(cell, weight)                           the packet leaves the problem
-//   rw_step() / ddmc_step()                        acceleration counters
+//   roulette(cell, weight)                          Real = float: weight rouletted into the material
+//   escape(cell, weight)                            the packet leaves the problem
+//   rw_step() / ddmc_step()                         acceleration counters
+// The mixed engine tallies the path integral only in cells without
+// absorption: elsewhere it equals the absorbed energy divided by f sigma_a
+// (both are sums of w (1 - e^{-x}) over the same segments), which the
+// postStep evaluates from the exact deposit tally.
 template <typename Real, typename Sink>
 STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const CellXS<Real> *xs,
                               const AccelView<Real> &acc, const StepConsts &k, int max_segments, Sink &sink) {
@@ -132,16 +150,26 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
   constexpr bool fixed_point = sizeof(Real) == 4;
   const PacketStream rs = PacketStream::make(k.seed, p.id, k.step, kTagTransport);
 
-  uint32_t cell = flat_cell(m, p.ijk);
-  double w_enter = p.w;      // weight when the current cell visit began
-  double path_int = 0.0;     // sum of w * (1 - e^{-x}) / sig_abs over the visit, erg*cm
+  Real u0 = p.u[0], u1 = p.u[1], u2 = p.u[2];
+  Real o0 = p.o[0], o1 = p.o[1], o2 = p.o[2];
+  Real s = p.s;
+  double w = p.w;
+  uint32_t ijk = p.ijk;
+  uint32_t ctr = p.ctr;
+  const Real wmin = p.wmin;
+
+  uint32_t cell = flat_cell(m, ijk);
+  Real cell_sig_abs = xs[cell].sig_abs;
+  double w_enter = w;         // weight when the current cell visit began
+  double path_int = 0.0;      // sum of w * (1 - e^{-x}) / sig_abs over the visit, erg*cm
   uint32_t visit_segs = 0;
   int outcome = kAlive;
 
   auto flush = [&](double w_exit) {
     if constexpr (fixed_point) {
       fx::Limbs d = fx::sub(fx::limbs_of(w_enter, k.eq), fx::limbs_of(w_exit, k.eq));
-      fx::Limbs e = fx::limbs_of(fx::quantize_nearest(path_int, k.et), k.et);
+      fx::Limbs e{0u, 0u};
+      if (!(cell_sig_abs > Real(0))) e = fx::limbs_of(fx::quantize_nearest(path_int, k.et), k.et);
       sink.visit(cell, d, e, visit_segs);
     } else {
       sink.visit_fp(cell, w_enter - w_exit, path_int, visit_segs);
@@ -151,17 +179,18 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
   };
   // close the visit and move into the neighbour across face `fax`/`up`
   auto enter_neighbour = [&](int fax, bool up) {
-    double wq = fixed_point ? fx::quantize_down(p.w, k.eq) : p.w;
+    double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
     flush(wq);
-    p.w = wq;
+    w = wq;
     w_enter = wq;
-    uint32_t ni = ijk_i(p.ijk), nj = ijk_j(p.ijk), nk = ijk_k(p.ijk);
+    uint32_t ni = ijk_i(ijk), nj = ijk_j(ijk), nk = ijk_k(ijk);
     if (fax == 0) ni = up ? ni + 1u : ni - 1u;
     else if (fax == 1) nj = up ? nj + 1u : nj - 1u;
     else nk = up ? nk + 1u : nk - 1u;
-    p.ijk = pack_ijk(ni, nj, nk);
-    p.u[fax] = up ? Real(0) : Real(1);
-    cell = flat_cell(m, p.ijk);
+    ijk = pack_ijk(ni, nj, nk);
+    put3<Real>(fax, u0, u1, u2, up ? Real(0) : Real(1));
+    cell = flat_cell(m, ijk);
+    cell_sig_abs = xs[cell].sig_abs;
   };
   // continuous absorption over a path d (cm); `tally_free` adds w * d to the
   // path integral when nothing is absorbed (the analytic step and DDMC do,
@@ -169,7 +198,7 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
   auto absorb = [&](Real sig_abs, Real d, bool tally_free) {
     Real xabs = sig_abs * d;
     Real frac = -M::expm1(-xabs);
-    double w0 = p.w;
+    double w0 = w;
     if (xabs >= Real(1e-12)) {
       path_int += w0 * static_cast<double>(frac / sig_abs);
     } else if (tally_free) {
@@ -177,79 +206,99 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
     } else if (sig_abs > Real(0)) {
       path_int += w0 * static_cast<double>(frac / sig_abs);
     }
-    p.w = w0 - w0 * static_cast<double>(frac);
+    w = w0 - w0 * static_cast<double>(frac);
+  };
+  // roulette: the remaining weight goes to the material (RadiationIMC.hpp:1812-1820)
+  auto roulette = [&]() {
+    double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
+    flush(fixed_point ? wq : 0.0);
+    if constexpr (fixed_point) sink.roulette(cell, wq);
+    w = 0.0;
+    outcome = kGone;
+  };
+  auto census = [&]() {
+    // the weight is kept as a multiple of the quantum in both engines so
+    // that population control can work in exact integers
+    double wq = fx::quantize_down(w, k.eq);
+    flush(wq);
+    w = wq;
+    outcome = kCensus;
   };
 
   for (int seg = 0; seg < max_segments; ++seg) {
-    const uint32_t i = ijk_i(p.ijk), j = ijk_j(p.ijk), kk = ijk_k(p.ijk);
-    const Real width[3] = {m.width[0][i], m.width[1][j], m.width[2][kk]};
+    const uint32_t i = ijk_i(ijk), j = ijk_j(ijk), kk = ijk_k(ijk);
+    const Real w0w = m.width[0][i], w1w = m.width[1][j], w2w = m.width[2][kk];
     const CellXS<Real> x = xs[cell];
     ++visit_segs;
 
     // Random numbers of this segment: one Philox block (two for an
     // acceleration step).
-    Philox4 r = rs.block(p.ctr++);
+    Philox4 r = rs.block(ctr++);
 
     if (acc.cell) {
       const CellAccel<Real> ca = acc.cell[cell];
       if (ca.flags) {
         // distance to the nearest face (computeMinDistanceToFaces, :956-969)
-        Real Ro = M::inf();
-        for (int a = 0; a < 3; ++a) {
-          Real lo = p.u[a] * width[a], hi = (Real(1) - p.u[a]) * width[a];
-          Real dm = lo < hi ? lo : hi;
-          Ro = dm < Ro ? dm : Ro;
+        Real Ro;
+        {
+          Real a0 = u0 * w0w, b0 = (Real(1) - u0) * w0w;
+          Real a1 = u1 * w1w, b1 = (Real(1) - u1) * w1w;
+          Real a2 = u2 * w2w, b2 = (Real(1) - u2) * w2w;
+          Real m0 = a0 < b0 ? a0 : b0, m1 = a1 < b1 ? a1 : b1, m2 = a2 < b2 ? a2 : b2;
+          Ro = m0 < m1 ? m0 : m1;
+          Ro = m2 < Ro ? m2 : Ro;
         }
         const Real tauR = Ro * ca.sig_t;
         if ((ca.flags & 1u) && Ro > Real(0) && ca.sig_t > Real(0) && tauR >= acc.rw_particle_tau) {
           // ---------------- random-walk step (tryRandomWalkStep, gray)
-          Philox4 r2 = rs.block(p.ctr++);
+          Philox4 r2 = rs.block(ctr++);
           rw::Tables<Real> tb{acc.rw_tau, acc.rw_survival, acc.rw_radius, acc.rw_log_tau_min, acc.rw_log_tau_max};
           const Real tauLeak = rw::sample_leak_time<Real>(tb, M::uniform(r.v[0]));
           // tLeak = tau Ro^2 / D with D = c / (3 sigma_t): as a path, tau Ro^2 3 sigma_t
           const Real sLeak = tauLeak * Ro * Ro * (Real(3) * ca.sig_t);
-          const bool leak = sLeak <= p.s;
-          const Real d = leak ? sLeak : p.s;
+          const bool leak = sLeak <= s;
+          const Real d = leak ? sLeak : s;
           absorb(x.sig_abs, d, false);
-          p.s = leak ? p.s - d : Real(0);
-          if (p.s < Real(0)) p.s = Real(0);
+          s = leak ? s - d : Real(0);
+          if (s < Real(0)) s = Real(0);
           sink.rw_step();
-          if (p.w < static_cast<double>(p.wmin * Real(0.1))) {
-            flush(0.0);
-            p.w = 0.0;
-            outcome = kGone;
+          if (w < static_cast<double>(wmin * Real(0.1))) {
+            roulette();
             break;
           }
           // new position on the sphere (leak) or inside it (census)
           Real dir[3];
           isotropic_dir<Real>(M::uniform(r.v[1]), M::uniform(r.v[2]), dir);
-          // posDir = (sin t cos phi, sin t sin phi, cos t), cos t = 2 U - 1
           Real disp = Ro;
           if (!leak) {
             const Real tauPos = d / (Real(3) * ca.sig_t * Ro * Ro);
             disp = Ro * rw::sample_radius<Real>(tb, tauPos, M::uniform(r.v[3]));
           }
           if (disp > Ro) disp = Ro;
-          Real c0[3] = {p.u[0], p.u[1], p.u[2]};
-          Real nu[3];
-          for (int a = 0; a < 3; ++a) {
-            nu[a] = c0[a] + disp * dir[a] / width[a];
-            nu[a] = nu[a] * Real(1 - 1e-6) + Real(1e-6) * Real(0.5);  // nudge toward the centre
-          }
+          const Real c0 = u0, c1 = u1, c2 = u2;
+          // nudge toward the cell centre (RadiationIMC.hpp:1214-1215)
+          Real n0 = (c0 + disp * dir[0] / w0w) * Real(1 - 1e-6) + Real(1e-6) * Real(0.5);
+          Real n1 = (c1 + disp * dir[1] / w1w) * Real(1 - 1e-6) + Real(1e-6) * Real(0.5);
+          Real n2 = (c2 + disp * dir[2] / w2w) * Real(1 - 1e-6) + Real(1e-6) * Real(0.5);
           for (int guard = 0; guard < 2000; ++guard) {
-            bool inside = true;
-            for (int a = 0; a < 3; ++a) inside = inside && nu[a] >= Real(0) && nu[a] <= Real(1);
+            bool inside = n0 >= Real(0) && n0 <= Real(1) && n1 >= Real(0) && n1 <= Real(1) && n2 >= Real(0) &&
+                          n2 <= Real(1);
             if (inside) break;
             disp = disp * Real(0.99);
-            for (int a = 0; a < 3; ++a) nu[a] = c0[a] + disp * dir[a] / width[a];
+            n0 = c0 + disp * dir[0] / w0w;
+            n1 = c1 + disp * dir[1] / w1w;
+            n2 = c2 + disp * dir[2] / w2w;
           }
-          for (int a = 0; a < 3; ++a) p.u[a] = nu[a] < Real(0) ? Real(0) : (nu[a] > Real(1) ? Real(1) : nu[a]);
-          isotropic_dir<Real>(M::uniform(r2.v[0]), M::uniform(r2.v[1]), p.o);
+          u0 = n0 < Real(0) ? Real(0) : (n0 > Real(1) ? Real(1) : n0);
+          u1 = n1 < Real(0) ? Real(0) : (n1 > Real(1) ? Real(1) : n1);
+          u2 = n2 < Real(0) ? Real(0) : (n2 > Real(1) ? Real(1) : n2);
+          Real nd[3];
+          isotropic_dir<Real>(M::uniform(r2.v[0]), M::uniform(r2.v[1]), nd);
+          o0 = nd[0];
+          o1 = nd[1];
+          o2 = nd[2];
           if (!leak) {
-            double wq = fx::quantize_down(p.w, k.eq);
-            flush(wq);
-            p.w = wq;
-            outcome = kCensus;
+            census();
             break;
           }
           continue;
@@ -257,28 +306,26 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
         if ((ca.flags & 2u) && ca.total > Real(0) && tauR >= acc.ddmc_particle_tau) {
           // ---------------- DDMC step (tryDDMCStep, gray)
           const Real sEvent = (-M::log(M::uniform(r.v[0]))) / ca.total;
-          const bool census = p.s <= sEvent;
-          const Real d = census ? p.s : sEvent;
+          const bool cen = s <= sEvent;
+          const Real d = cen ? s : sEvent;
           absorb(x.sig_abs, d, true);
-          p.s = census ? Real(0) : p.s - d;
-          if (p.s < Real(0)) p.s = Real(0);
+          s = cen ? Real(0) : s - d;
+          if (s < Real(0)) s = Real(0);
           sink.ddmc_step();
-          if (p.w < static_cast<double>(p.wmin)) {
-            flush(0.0);
-            p.w = 0.0;
-            outcome = kGone;
+          if (w < static_cast<double>(wmin)) {
+            roulette();
             break;
           }
-          if (census) {
-            double wq = fx::quantize_down(p.w, k.eq);
-            flush(wq);
-            p.w = wq;
-            outcome = kCensus;
+          if (cen) {
+            census();
             break;
           }
           // leak through a face chosen in proportion to its rate
           Real pick = M::uniform(r.v[1]) * ca.total;
           int face = -1, lastf = -1;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
           for (int f = 0; f < 6; ++f) {
             if (ca.rate[f] > Real(0)) {
               lastf = f;
@@ -296,21 +343,23 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
           Real sp, cp;
           M::sincos2pi(M::uniform(r.v[3]), sp, cp);
           const Real sg = up ? Real(1) : Real(-1);
-          Real nd[3] = {Real(0), Real(0), Real(0)};
-          nd[fax] = sg * mu;
           if (fax == 0) {          // e1 = y, e2 = n x y = sg z
-            nd[1] = st * cp;
-            nd[2] = sg * st * sp;
+            o0 = sg * mu;
+            o1 = st * cp;
+            o2 = sg * st * sp;
           } else if (fax == 1) {   // e1 = x, e2 = n x x = -sg z
-            nd[0] = st * cp;
-            nd[2] = -sg * st * sp;
+            o0 = st * cp;
+            o1 = sg * mu;
+            o2 = -sg * st * sp;
           } else {                 // e1 = x, e2 = n x x = sg y
-            nd[0] = st * cp;
-            nd[1] = sg * st * sp;
+            o0 = st * cp;
+            o1 = sg * st * sp;
+            o2 = sg * mu;
           }
-          for (int a = 0; a < 3; ++a) p.o[a] = nd[a];
           // placed at the face centroid (:1588), in the neighbour
-          for (int a = 0; a < 3; ++a) p.u[a] = Real(0.5);
+          u0 = Real(0.5);
+          u1 = Real(0.5);
+          u2 = Real(0.5);
           enter_neighbour(fax, up);
           continue;
         }
@@ -318,40 +367,45 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
     }
 
     // ---------------- analytic step
-    // Distance to the nearest face along the direction.
-    Real dface = M::inf(), dsecond = M::inf();
-    int fax = -1, sax = -1;
-    for (int a = 0; a < 3; ++a) {
-      Real oa = p.o[a];
-      if (oa != Real(0)) {
-        Real frac = oa > Real(0) ? (Real(1) - p.u[a]) : p.u[a];
-        Real d = frac * width[a] / (oa > Real(0) ? oa : -oa);
-        if (d < dface) {
-          dsecond = dface;
-          sax = fax;
-          dface = d;
-          fax = a;
-        } else if (d < dsecond) {
-          dsecond = d;
-          sax = a;
-        }
-      }
+    // Distance to the nearest face along the direction (+inf along an axis
+    // the packet does not move on); ties go to the lower axis.
+    auto face_dist = [](Real ua, Real oa, Real wa) {
+      Real frac = oa > Real(0) ? (Real(1) - ua) : ua;
+      Real d = frac * wa / (oa > Real(0) ? oa : -oa);
+      return oa != Real(0) ? d : M::inf();
+    };
+    const Real d0 = face_dist(u0, o0, w0w), d1 = face_dist(u1, o1, w1w), d2 = face_dist(u2, o2, w2w);
+    int fax = 0, sax = 1;
+    Real dface = d0, dsecond = d1;
+    if (d1 < d0) {
+      fax = 1;
+      sax = 0;
+      dface = d1;
+      dsecond = d0;
+    }
+    if (d2 < dface) {
+      sax = fax;
+      dsecond = dface;
+      fax = 2;
+      dface = d2;
+    } else if (d2 < dsecond) {
+      sax = 2;
+      dsecond = d2;
     }
     if constexpr (fixed_point) {
       // Two faces within 1e-6 of each other along the path (an edge or corner
       // crossing): decide which one is hit first in FP64.
-      if (sax >= 0 && dsecond - dface <= Real(1e-6) * dsecond) {
-        double df[2];
-        int ax2[2] = {fax, sax};
-        for (int q = 0; q < 2; ++q) {
-          int a = ax2[q];
-          double oa = static_cast<double>(p.o[a]);
-          double fr = oa > 0.0 ? (1.0 - static_cast<double>(p.u[a])) : static_cast<double>(p.u[a]);
-          df[q] = fr * static_cast<double>(width[a]) / (oa > 0.0 ? oa : -oa);
-        }
-        if (df[1] < df[0]) {
+      if (dsecond < M::inf() && dsecond - dface <= Real(1e-6) * dsecond) {
+        auto dist64 = [&](int a) {
+          double oa = static_cast<double>(sel3(a, o0, o1, o2));
+          double ua = static_cast<double>(sel3(a, u0, u1, u2));
+          double fr = oa > 0.0 ? (1.0 - ua) : ua;
+          return fr * static_cast<double>(sel3(a, w0w, w1w, w2w)) / (oa > 0.0 ? oa : -oa);
+        };
+        double dA = dist64(fax), dB = dist64(sax);
+        if (dB < dA) {
           fax = sax;
-          dface = static_cast<Real>(df[1]);
+          dface = static_cast<Real>(dB);
         }
       }
     }
@@ -365,72 +419,84 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
       ev = 1;
       d = dcol;
     }
-    if (p.s < d) {
+    if (s < d) {
       ev = 2;
-      d = p.s;
+      d = s;
     }
-    p.s -= d;
-    if (ev == 2 || p.s < Real(0)) p.s = Real(0);
+    s -= d;
+    if (ev == 2 || s < Real(0)) s = Real(0);
 
     absorb(x.sig_abs, d, true);
 
     // Move (the face axis is set exactly below).
-    for (int a = 0; a < 3; ++a) {
-      if (ev == 0 && a == fax) continue;
-      Real nu = p.u[a] + p.o[a] * d / width[a];
-      p.u[a] = nu < Real(0) ? Real(0) : (nu > Real(1) ? Real(1) : nu);
+    {
+      Real n0 = u0 + o0 * d / w0w, n1 = u1 + o1 * d / w1w, n2 = u2 + o2 * d / w2w;
+      n0 = n0 < Real(0) ? Real(0) : (n0 > Real(1) ? Real(1) : n0);
+      n1 = n1 < Real(0) ? Real(0) : (n1 > Real(1) ? Real(1) : n1);
+      n2 = n2 < Real(0) ? Real(0) : (n2 > Real(1) ? Real(1) : n2);
+      if (!(ev == 0 && fax == 0)) u0 = n0;
+      if (!(ev == 0 && fax == 1)) u1 = n1;
+      if (!(ev == 0 && fax == 2)) u2 = n2;
     }
 
     // Roulette (RadiationIMC.hpp:1812-1820): the remaining weight goes to the material.
-    if (p.w < static_cast<double>(p.wmin)) {
-      flush(0.0);
-      p.w = 0.0;
-      outcome = kGone;
+    if (w < static_cast<double>(wmin)) {
+      roulette();
       break;
     }
 
     if (ev == 0) {
-      const bool up = p.o[fax] > Real(0);
-      uint32_t idx = fax == 0 ? i : (fax == 1 ? j : kk);
-      uint32_t n = static_cast<uint32_t>(fax == 0 ? m.nx : (fax == 1 ? m.ny : m.nz));
-      bool outside = up ? (idx + 1u >= n) : (idx == 0u);
+      const Real of = sel3(fax, o0, o1, o2);
+      const bool up = of > Real(0);
+      const uint32_t idx = sel3<uint32_t>(fax, i, j, kk);
+      const uint32_t n = static_cast<uint32_t>(sel3<int32_t>(fax, m.nx, m.ny, m.nz));
+      const bool outside = up ? (idx + 1u >= n) : (idx == 0u);
       if (!outside) {
         // interior face: the packet enters the neighbour on the opposite face
         enter_neighbour(fax, up);
       } else {
-        int face = 2 * fax + (up ? 1 : 0);
-        p.u[fax] = up ? Real(1) : Real(0);
-        if (m.face_kind[face] == 1) {
+        const int face = 2 * fax + (up ? 1 : 0);
+        put3<Real>(fax, u0, u1, u2, up ? Real(1) : Real(0));
+        if ((m.escape_mask >> face) & 1u) {
           // escape (REMOVE); the weight leaves the problem
-          double wq = fixed_point ? fx::quantize_down(p.w, k.eq) : p.w;
+          double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
           flush(wq);
           sink.escape(cell, wq);
-          p.w = 0.0;
+          w = 0.0;
           outcome = kEscaped;
           break;
         }
         // specular reflection on the box face (BoundaryCondition.hpp:87-108)
-        p.o[fax] = -p.o[fax];
+        put3<Real>(fax, o0, o1, o2, -of);
       }
     } else if (ev == 1) {
       // effective scatter: isotropic redirection (RadiationIMC.hpp:1827-1874)
-      isotropic_dir<Real>(M::uniform(r.v[2]), M::uniform(r.v[3]), p.o);
+      Real nd[3];
+      isotropic_dir<Real>(M::uniform(r.v[2]), M::uniform(r.v[3]), nd);
+      o0 = nd[0];
+      o1 = nd[1];
+      o2 = nd[2];
     } else {
-      // census: the weight is kept as a multiple of the quantum in both
-      // engines so that population control can work in exact integers
-      double wq = fx::quantize_down(p.w, k.eq);
-      flush(wq);
-      p.w = wq;
-      outcome = kCensus;
+      census();
       break;
     }
   }
   if (outcome == kAlive) {
     // segment budget used up: close the visit so the next pass starts clean
-    double wq = fixed_point ? fx::quantize_down(p.w, k.eq) : p.w;
+    double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
     flush(wq);
-    p.w = wq;
+    w = wq;
   }
+  p.u[0] = u0;
+  p.u[1] = u1;
+  p.u[2] = u2;
+  p.o[0] = o0;
+  p.o[1] = o1;
+  p.o[2] = o2;
+  p.s = s;
+  p.w = w;
+  p.ijk = ijk;
+  p.ctr = ctr;
   return outcome;
 }
 

@@ -110,7 +110,7 @@ struct EngineBase {
   virtual void export_census(storm_packet_record *, uint64_t, uint64_t *) = 0;
   virtual void import_census(const storm_packet_record *, uint64_t) = 0;
   virtual void debug_inputs(uint64_t *, void *, void *, int64_t *) = 0;
-  virtual void debug_tallies(uint64_t *, uint64_t *, uint64_t *, uint32_t *) = 0;
+  virtual void debug_tallies(uint64_t *, uint64_t *, uint64_t *, uint64_t *, uint32_t *) = 0;
   virtual void debug_accel(void *, uint64_t *) = 0;
   storm_gpu_counters counters{};
   double source_T = 0.0;
@@ -260,7 +260,7 @@ struct Engine final : EngineBase {
         up_r(rw_rad, r);
       }
     }
-    tally_u64.ensure(7 * 2 * static_cast<size_t>(N));
+    tally_u64.ensure(8 * 2 * static_cast<size_t>(N));
     tally_f64.ensure(3 * static_cast<size_t>(N));
     DevBuf<uint32_t> *segs_begin = new DevBuf<uint32_t>();  // owned below
     segs_begin_.reset(segs_begin);
@@ -274,6 +274,7 @@ struct Engine final : EngineBase {
     tal.cpre = tu + 4 * n2;
     tal.cout = tu + 5 * n2;
     tal.rem = tu + 6 * n2;
+    tal.rdep = tu + 7 * n2;
     tal.dep_fp = tally_f64.p;
     tal.pint_fp = tally_f64.p + N;
     tal.esc_fp = tally_f64.p + 2 * static_cast<size_t>(N);
@@ -348,7 +349,9 @@ struct Engine final : EngineBase {
     m.nx = nx;
     m.ny = ny;
     m.nz = nz;
-    for (int f = 0; f < 6; ++f) m.face_kind[f] = bnd.face_kind[f];
+    m.escape_mask = 0u;
+    for (int f = 0; f < 6; ++f)
+      if (bnd.face_kind[f] == STORM_FACE_ESCAPE) m.escape_mask |= 1u << f;
     m.width[0] = wx.p;
     m.width[1] = wy.p;
     m.width[2] = wz.p;
@@ -536,7 +539,7 @@ struct Engine final : EngineBase {
     k_post<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
         N, tal, eq, et, eq_prev, emit_limbs.p, requant.p, dt, V.p, md,
         material_mode == STORM_GPU_MATERIAL_DEVICE ? 1 : 0, T.p, e_mat.p, e_sync.p, erad_avg.p, census_erad.p,
-        clamp.p, err.p, precision == STORM_GPU_FP64 ? 1 : 0);
+        clamp.p, err.p, precision == STORM_GPU_FP64 ? 1 : 0, xs.p);
     census_total_prev = deterministic_sum(census_erad.p, N);
     uint32_t herr = 0;
     unsigned long long hstats[4] = {0, 0, 0, 0};
@@ -703,9 +706,10 @@ struct Engine final : EngineBase {
     if (out && nb) HIPCHECK(hipMemcpy(out, accel.p, nb, hipMemcpyDeviceToHost));
   }
 
-  void debug_tallies(uint64_t *dep, uint64_t *pint, uint64_t *esc, uint32_t *segs) override {
+  void debug_tallies(uint64_t *dep, uint64_t *rdep, uint64_t *pint, uint64_t *esc, uint32_t *segs) override {
     const size_t n2 = 2 * static_cast<size_t>(N);
     if (dep) HIPCHECK(hipMemcpy(dep, tal.dep, n2 * 8, hipMemcpyDeviceToHost));
+    if (rdep) HIPCHECK(hipMemcpy(rdep, tal.rdep, n2 * 8, hipMemcpyDeviceToHost));
     if (pint) HIPCHECK(hipMemcpy(pint, tal.pint, n2 * 8, hipMemcpyDeviceToHost));
     if (esc) HIPCHECK(hipMemcpy(esc, tal.esc, n2 * 8, hipMemcpyDeviceToHost));
     if (segs) HIPCHECK(hipMemcpy(segs, tal.segs, N * 4, hipMemcpyDeviceToHost));
@@ -757,7 +761,7 @@ struct Engine final : EngineBase {
     std::memset(L, 0, sizeof(*L));
     L->census_in = u128(sum(b + 3 * n2));
     L->emitted = u128(sum(em.data()));
-    L->deposited = u128(sum(b));
+    L->deposited = u128(sum(b) + sum(b + 7 * n2));
     L->escaped = u128(sum(b + 2 * n2));
     if (precision == STORM_GPU_FP64) {
       // FP64 engine: deposits and escapes are FP64 sums, reported rounded to quanta
@@ -1069,8 +1073,9 @@ int storm_gpu_debug_accel(storm_gpu_engine *eng, void *out, uint64_t *bytes) {
   return guarded([&] { as_engine(eng)->debug_accel(out, bytes); });
 }
 
-int storm_gpu_debug_tallies(storm_gpu_engine *eng, uint64_t *dep, uint64_t *pint, uint64_t *esc, uint32_t *segs) {
-  return guarded([&] { as_engine(eng)->debug_tallies(dep, pint, esc, segs); });
+int storm_gpu_debug_tallies(storm_gpu_engine *eng, uint64_t *dep, uint64_t *rdep, uint64_t *pint, uint64_t *esc,
+                            uint32_t *segs) {
+  return guarded([&] { as_engine(eng)->debug_tallies(dep, rdep, pint, esc, segs); });
 }
 
 }  // extern "C"

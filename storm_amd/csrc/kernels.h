@@ -112,13 +112,14 @@ __device__ __forceinline__ void store_census(const CensusSoA<Real> &c, uint32_t 
 
 // Per-cell integer tallies: pairs (lo sum, hi sum) of fixed-point limbs.
 struct Tallies {
-  unsigned long long *dep;   // [2N] material deposit (transport + roulette)
-  unsigned long long *pint;  // [2N] path integral of the weight, erg*cm
+  unsigned long long *dep;   // [2N] material deposit by continuous absorption
+  unsigned long long *pint;  // [2N] path integral of the weight, erg*cm (cells without absorption)
   unsigned long long *esc;   // [2N] escaped through REMOVE faces
   unsigned long long *cin;   // [2N] census weight entering transport
   unsigned long long *cpre;  // [2N] census weight before population control
   unsigned long long *cout;  // [2N] census weight after population control
   unsigned long long *rem;   // [2N] comb remainders (go to the material)
+  unsigned long long *rdep;  // [2N] rouletted weight (goes to the material)
   double *dep_fp;            // [N]  FP64 engine: deposit
   double *pint_fp;           // [N]  FP64 engine: path integral
   double *esc_fp;            // [N]  FP64 engine: escaped
@@ -326,6 +327,7 @@ struct DeviceSink {
     tally_limbs(t.pint, c, pint);
     atomicAdd(&t.segs[c], segs);
   }
+  __device__ void roulette(uint32_t c, double w) { tally_limbs(t.rdep, c, fx::limbs_of(w, eq)); }
   __device__ void visit_fp(uint32_t c, double dep, double pint, uint32_t segs) {
     if (dep != 0.0) atomicAdd(&t.dep_fp[c], dep);
     if (pint != 0.0) atomicAdd(&t.pint_fp[c], pint);
@@ -686,7 +688,8 @@ __global__ __launch_bounds__(256) void k_post(uint32_t N, Tallies t, int32_t eq,
                                               double *__restrict__ T, double *__restrict__ e_mat,
                                               double *__restrict__ e_sync, double *__restrict__ erad_avg,
                                               double *__restrict__ census_erad, double *__restrict__ clamp,
-                                              uint32_t *__restrict__ err, int32_t fp64) {
+                                              uint32_t *__restrict__ err, int32_t fp64,
+                                              const CellXS<Real> *__restrict__ xs) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
   const double emit = fx::to_double(emit_limbs[2 * c], emit_limbs[2 * c + 1], eq);
@@ -696,7 +699,12 @@ __global__ __launch_bounds__(256) void k_post(uint32_t N, Tallies t, int32_t eq,
     pint = t.pint_fp[c];
   } else {
     dep = fx::to_double(t.dep[2 * c], t.dep[2 * c + 1], eq);
-    pint = fx::to_double(t.pint[2 * c], t.pint[2 * c + 1], et);
+    // Each segment absorbs w (1 - e^{-x}) and adds w (1 - e^{-x}) / sig_abs to
+    // the path integral, so where sig_abs > 0 the integral is the absorbed
+    // energy over sig_abs (the transport skips that tally there).
+    const double sabs = static_cast<double>(xs[c].sig_abs);
+    pint = sabs > 0.0 ? dep / sabs : fx::to_double(t.pint[2 * c], t.pint[2 * c + 1], et);
+    dep += fx::to_double(t.rdep[2 * c], t.rdep[2 * c + 1], eq);
   }
   dep += fx::to_double(t.rem[2 * c], t.rem[2 * c + 1], eq);
   dep += fx::to_double(requant[2 * c], requant[2 * c + 1], eq_old);

@@ -149,14 +149,14 @@ class GpuEngine:
         return buf.raw
 
     def tallies(self):
-        dep = np.empty(2 * self.N, dtype=np.uint64)
-        pint = np.empty(2 * self.N, dtype=np.uint64)
-        esc = np.empty(2 * self.N, dtype=np.uint64)
-        segs = np.empty(self.N, dtype=np.uint32)
+        """Integer tallies of the last step: dict of dep, rdep, pint, esc (2 limbs
+        per cell) and segs."""
+        t = {k: np.empty(2 * self.N, dtype=np.uint64) for k in ("dep", "rdep", "pint", "esc")}
+        t["segs"] = np.empty(self.N, dtype=np.uint32)
         u64 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint64))
-        self._check(self.lib.storm_gpu_debug_tallies(self.h, u64(dep), u64(pint), u64(esc),
-                                                     segs.ctypes.data_as(C.POINTER(C.c_uint32))))
-        return dep, pint, esc, segs
+        self._check(self.lib.storm_gpu_debug_tallies(self.h, u64(t["dep"]), u64(t["rdep"]), u64(t["pint"]),
+                                                     u64(t["esc"]), t["segs"].ctypes.data_as(C.POINTER(C.c_uint32))))
+        return t
 
 
 def device_info() -> str:

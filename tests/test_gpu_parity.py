@@ -149,16 +149,14 @@ def test_replay_reproduces_tallies_bit_exactly(oracle, make):
             e.keep_transport_inputs(s == spec.steps - 1)
             e.step()
         pk, n, xs, consts = e.transport_inputs()
-        dep, pint, esc, segs = e.tallies()
+        got = e.tallies()
         accel = e.accel_data()
         cnt = e.counters()
-    rdep, rpint, resc, rsegs, outc = oracle.replay_transport(spec, pk, n, xs, consts, accel=accel)
+    ref, outc = oracle.replay_transport(spec, pk, n, xs, consts, accel=accel)
     assert n > 0 and outc[:4].sum() == n
     assert outc[4] == cnt["random_walk_steps"] and outc[5] == cnt["ddmc_steps"]
-    assert np.array_equal(segs.astype(np.uint64), rsegs.astype(np.uint64))
-    assert np.array_equal(dep, rdep)
-    assert np.array_equal(pint, rpint)
-    assert np.array_equal(esc, resc)
+    for k in ("segs", "dep", "rdep", "pint", "esc"):
+        assert np.array_equal(got[k].astype(np.uint64), ref[k].astype(np.uint64)), k
 
 
 def test_same_seed_is_bit_identical():
