@@ -32,13 +32,14 @@ B_SEG_FP64 = 224.0   # 2 * 80 B packet + 32 B cell + 2 * 16 B tally
 HBM_PEAK_GBS = 8000.0
 
 
-def run_engine(precision, steps, warmup, seed, log):
-    from storm_amd import capi
+def run_engine(precision, steps, warmup, seed, log, schedule=None):
     from storm_amd import problems as P
     from storm_amd.engine import GpuEngine
 
     spec = P.c2_cube(seed=seed)
     with GpuEngine(spec, precision) as e:
+        if schedule is not None:
+            e.set_tuning(schedule=schedule)
         for s in range(warmup):
             e.step()
             log(f"warmup step {s + 1}/{warmup}: {e.counters()['segments']} segments")
@@ -61,12 +62,13 @@ def run_engine(precision, steps, warmup, seed, log):
                 name=spec.name, N=spec.ncells)
 
 
-def cpu_baseline(log, budget_s=20.0):
-    """FP64 oracle, one core, on the same workload with 1/100 of the packets."""
+def cpu_baseline(log):
+    """FP64 oracle, one core, on the same workload with 3/20 of the packets
+    (about 15 s of CPU work)."""
     from oracle import pyoracle
     from storm_amd import problems as P
 
-    spec = P.c2_cube(seed=1, packets_scale=0.01, steps=2)
+    spec = P.c2_cube(seed=1, packets_scale=0.15, steps=2)
     t0 = time.perf_counter()
     r = pyoracle.run(spec)
     wall = time.perf_counter() - t0
@@ -74,7 +76,7 @@ def cpu_baseline(log, budget_s=20.0):
     secs = float(r["ledger"]["seconds"].sum())
     log(f"cpu baseline: {seg:.0f} segments in {secs:.2f} s (wall {wall:.2f} s)")
     return dict(value=seg / secs, unit="segments/s", cores=1, kind="port",
-                sample="64^3 cube, first 2 IMC steps at 1/100 of the packets (~1e5 per step), FP64 oracle, 1 thread")
+                sample="64^3 cube, first 2 IMC steps at 3/20 of the packets (~1e6 per step), FP64 oracle, 1 thread")
 
 
 def main():
@@ -101,6 +103,7 @@ def main():
     from storm_amd.engine import device_info
 
     log("device: " + device_info())
+    log("mixed engine (refill schedule)")
     mixed = run_engine(capi.MIXED, args.steps, args.warmup, args.seed, log)
     value = mixed["segments"] / mixed["wall"]
     ms_per_step = mixed["wall"] / args.steps * 1e3
@@ -133,18 +136,27 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_transport<float>",
+            "kernel": "k_transport_refill<float>",
             "bytes_per_segment": B_SEG_MIXED,
         },
     }
     if not args.no_fp64:
+        log("plain FP64 engine (one packet per thread)")
         f64 = run_engine(capi.FP64, args.steps, args.warmup, args.seed, log)
         out["fp64_gpu"] = {
             "value": f64["segments"] / f64["wall"],
             "ms_per_step": f64["wall"] / args.steps * 1e3,
             "transport_segments_per_s": f64["segments"] / f64["transport_s"],
+            "form": "plain: FP64 tracking and tallies, one packet per thread",
         }
         out["speedup_vs_fp64"] = value / out["fp64_gpu"]["value"]
+        log("FP64 engine on the refill schedule")
+        f64r = run_engine(capi.FP64, args.steps, args.warmup, args.seed, log, schedule=capi.SCHEDULE_REFILL)
+        out["fp64_gpu_refill"] = {
+            "value": f64r["segments"] / f64r["wall"],
+            "ms_per_step": f64r["wall"] / args.steps * 1e3,
+            "transport_segments_per_s": f64r["segments"] / f64r["transport_s"],
+        }
     out["transport_segments_per_s"] = mixed["segments"] / mixed["transport_s"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(log)

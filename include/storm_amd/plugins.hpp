@@ -35,30 +35,6 @@
 
 namespace STORM {
 
-// ------------------------------------------------------------------ physics
-template <typename T, typename Grid>
-class MonteCarloPhysics {
- public:
-  using MCParticle = Particle<T, Grid>;
-  MonteCarloPhysics(const Grid &grid, const std::shared_ptr<class BoundaryConditionTag> & = nullptr) : grid(grid) {}
-  explicit MonteCarloPhysics(const Grid &grid) : grid(grid) {}
-  virtual ~MonteCarloPhysics() = default;
-  virtual void updateGridData(void) {}
-  virtual std::vector<MCParticle> preStep(double fullDt) = 0;
-  virtual StepResult<T, Grid> step(MCParticle &particle, std::vector<MCParticle> &particlesToAdd) = 0;
-  virtual void postStep(const std::vector<MCParticle> &particles, double fullDt) = 0;
-  virtual size_t getRandomWalkStepCount() const { return 0; }
-  virtual size_t getDDMCStepCount() const { return 0; }
-  virtual size_t getDDMCLeakCount() const { return 0; }
-  virtual size_t getDDMCCensusCount() const { return 0; }
-  virtual size_t getDDMCUpscatterCount() const { return 0; }
-  virtual size_t getDDMCFallbackCount() const { return 0; }
-  virtual std::string getAccelerationDebugInfo(size_t, double) const { return std::string(); }
-
- protected:
-  const Grid &grid;
-};
-
 // ------------------------------------------------------------------ boundary
 enum class DDMCBoundaryFaceBehavior { Unsupported, ReflectingRigid };
 
@@ -171,6 +147,31 @@ class RigidBoundary : public BoundaryCondition<T, Grid> {
     info = GpuBoundaryInfo{};
     return true;
   }
+};
+
+// ------------------------------------------------------------------ physics
+template <typename T, typename Grid>
+class MonteCarloPhysics {
+ public:
+  using MCParticle = Particle<T, Grid>;
+  MonteCarloPhysics(const Grid &grid, const std::shared_ptr<BoundaryCondition<T, Grid>> &boundary)
+      : grid(grid), boundary(boundary) {}
+  virtual ~MonteCarloPhysics() = default;
+  virtual void updateGridData(void) {}
+  virtual std::vector<MCParticle> preStep(double fullDt) = 0;
+  virtual StepResult<T, Grid> step(MCParticle &particle, std::vector<MCParticle> &particlesToAdd) = 0;
+  virtual void postStep(const std::vector<MCParticle> &particles, double fullDt) = 0;
+  virtual size_t getRandomWalkStepCount() const { return 0; }
+  virtual size_t getDDMCStepCount() const { return 0; }
+  virtual size_t getDDMCLeakCount() const { return 0; }
+  virtual size_t getDDMCCensusCount() const { return 0; }
+  virtual size_t getDDMCUpscatterCount() const { return 0; }
+  virtual size_t getDDMCFallbackCount() const { return 0; }
+  virtual std::string getAccelerationDebugInfo(size_t, double) const { return std::string(); }
+
+ protected:
+  const Grid &grid;
+  std::shared_ptr<BoundaryCondition<T, Grid>> boundary;
 };
 
 // ------------------------------------------------------------------ population control
@@ -311,11 +312,10 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
   using Parameters = RadiationIMCParameters<NumGroups>;
   using OpacityModel = RadiationOpacityModel<PointT, GridT, CellT, NumGroups>;
 
-  template <typename BoundaryPtr>
-  RadiationIMC(const GridT &grid, const BoundaryPtr &, std::vector<CellT> &cells,
-               std::vector<ExtensivesT> &extensives, std::shared_ptr<EOST> eos, std::shared_ptr<OpacityModel> opacity,
-               Parameters parameters, std::uint64_t seed = 42)
-      : Base(grid), cells_(cells), extensives_(extensives), eos_(std::move(eos)), opacity_(std::move(opacity)),
+  RadiationIMC(const GridT &grid, const std::shared_ptr<BoundaryCondition<PointT, GridT>> &boundary,
+               std::vector<CellT> &cells, std::vector<ExtensivesT> &extensives, std::shared_ptr<EOST> eos,
+               std::shared_ptr<OpacityModel> opacity, Parameters parameters, std::uint64_t seed = 42)
+      : Base(grid, boundary), cells_(cells), extensives_(extensives), eos_(std::move(eos)), opacity_(std::move(opacity)),
         parameters_(parameters), seed_(seed) {
     // constructor checks of radiation/RadiationIMC.hpp:585-602, 689-694
     if (parameters_.newPhotonsPerCell == 0) throw StormError("RadiationIMC requires newPhotonsPerCell > 0");

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "_lib", "libstorm_amd.so")
 XLO, XHI, YLO, YHI, ZLO, ZHI = range(6)
 FACE_REFLECT, FACE_ESCAPE = 0, 1
 MIXED, FP64 = 0, 1
+SCHEDULE_PASS, SCHEDULE_REFILL = 0, 1
 MATERIAL_DEVICE, MATERIAL_HOST = 0, 1
 
 _dp = C.POINTER(C.c_double)
@@ -126,7 +127,7 @@ def load_library(path: str | None = None):
         "storm_gpu_census_size": ([vp, C.POINTER(C.c_uint64)], C.c_int),
         "storm_gpu_export_census": ([vp, C.POINTER(PacketRecord), C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
         "storm_gpu_import_census": ([vp, C.POINTER(PacketRecord), C.c_uint64], C.c_int),
-        "storm_gpu_set_tuning": ([vp, C.c_int32, C.c_int32], C.c_int),
+        "storm_gpu_set_tuning": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
         "storm_gpu_debug_keep_transport_inputs": ([vp, C.c_int32], C.c_int),
         "storm_gpu_debug_transport_inputs": ([vp, C.POINTER(C.c_uint64), vp, vp, C.POINTER(C.c_int64)], C.c_int),
         "storm_gpu_debug_tallies": ([vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),

@@ -143,29 +143,68 @@ STORM_HD void put3(int a, T &x, T &y, T &z, T v) {
 // absorption: elsewhere it equals the absorbed energy divided by f sigma_a
 // (both are sums of w (1 - e^{-x}) over the same segments), which the
 // postStep evaluates from the exact deposit tally.
-template <typename Real, typename Sink>
-STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const CellXS<Real> *xs,
-                              const AccelView<Real> &acc, const StepConsts &k, int max_segments, Sink &sink) {
+//
+// Tracker<Real> holds one packet's history between segments, so that a
+// kernel can advance a packet one segment at a time and give an idle lane a
+// new packet (transport_packet below runs a whole pass of one packet).
+template <typename Real>
+struct Tracker {
   using M = MathPolicy<Real>;
-  constexpr bool fixed_point = sizeof(Real) == 4;
-  const PacketStream rs = PacketStream::make(k.seed, p.id, k.step, kTagTransport);
+  static constexpr bool fixed_point = sizeof(Real) == 4;
 
-  Real u0 = p.u[0], u1 = p.u[1], u2 = p.u[2];
-  Real o0 = p.o[0], o1 = p.o[1], o2 = p.o[2];
-  Real s = p.s;
-  double w = p.w;
-  uint32_t ijk = p.ijk;
-  uint32_t ctr = p.ctr;
-  const Real wmin = p.wmin;
+  PacketStream rs;
+  Real u0, u1, u2, o0, o1, o2, s, wmin, cell_sig_abs;
+  double w, w_enter, path_int;  // w_enter: weight when the current cell visit began
+  uint32_t ijk, ctr, cell, visit_segs, id;
+  int outcome;
 
-  uint32_t cell = flat_cell(m, ijk);
-  Real cell_sig_abs = xs[cell].sig_abs;
-  double w_enter = w;         // weight when the current cell visit began
-  double path_int = 0.0;      // sum of w * (1 - e^{-x}) / sig_abs over the visit, erg*cm
-  uint32_t visit_segs = 0;
-  int outcome = kAlive;
+  STORM_HD void begin(const Packet<Real> &p, const MeshView<Real> &m, const CellXS<Real> *xs, const StepConsts &k) {
+    rs = PacketStream::make(k.seed, p.id, k.step, kTagTransport);
+    u0 = p.u[0];
+    u1 = p.u[1];
+    u2 = p.u[2];
+    o0 = p.o[0];
+    o1 = p.o[1];
+    o2 = p.o[2];
+    s = p.s;
+    w = p.w;
+    ijk = p.ijk;
+    ctr = p.ctr;
+    wmin = p.wmin;
+    id = p.id;
+    cell = flat_cell(m, ijk);
+    cell_sig_abs = xs[cell].sig_abs;
+    w_enter = w;
+    path_int = 0.0;  // sum of w * (1 - e^{-x}) / sig_abs over the visit, erg*cm
+    visit_segs = 0;
+    outcome = kAlive;
+  }
 
-  auto flush = [&](double w_exit) {
+  // Writes the state back; a packet still alive closes its visit so that the
+  // next pass starts clean.
+  template <typename Sink>
+  STORM_HD void end(Packet<Real> &p, const StepConsts &k, Sink &sink) {
+    if (outcome == kAlive) {
+      double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
+      flush(wq, k, sink);
+      w = wq;
+    }
+    p.u[0] = u0;
+    p.u[1] = u1;
+    p.u[2] = u2;
+    p.o[0] = o0;
+    p.o[1] = o1;
+    p.o[2] = o2;
+    p.s = s;
+    p.w = w;
+    p.wmin = wmin;
+    p.ijk = ijk;
+    p.ctr = ctr;
+    p.id = id;
+  }
+
+  template <typename Sink>
+  STORM_HD void flush(double w_exit, const StepConsts &k, Sink &sink) {
     if constexpr (fixed_point) {
       fx::Limbs d = fx::sub(fx::limbs_of(w_enter, k.eq), fx::limbs_of(w_exit, k.eq));
       fx::Limbs e{0u, 0u};
@@ -176,11 +215,14 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
     }
     visit_segs = 0;
     path_int = 0.0;
-  };
+  }
+
   // close the visit and move into the neighbour across face `fax`/`up`
-  auto enter_neighbour = [&](int fax, bool up) {
+  template <typename Sink>
+  STORM_HD void enter_neighbour(int fax, bool up, const MeshView<Real> &m, const CellXS<Real> *xs,
+                                const StepConsts &k, Sink &sink) {
     double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
-    flush(wq);
+    flush(wq, k, sink);
     w = wq;
     w_enter = wq;
     uint32_t ni = ijk_i(ijk), nj = ijk_j(ijk), nk = ijk_k(ijk);
@@ -191,11 +233,12 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
     put3<Real>(fax, u0, u1, u2, up ? Real(0) : Real(1));
     cell = flat_cell(m, ijk);
     cell_sig_abs = xs[cell].sig_abs;
-  };
+  }
+
   // continuous absorption over a path d (cm); `tally_free` adds w * d to the
   // path integral when nothing is absorbed (the analytic step and DDMC do,
   // the random-walk step does not, RadiationIMC.hpp:1163-1165)
-  auto absorb = [&](Real sig_abs, Real d, bool tally_free) {
+  STORM_HD void absorb(Real sig_abs, Real d, bool tally_free) {
     Real xabs = sig_abs * d;
     Real frac = -M::expm1(-xabs);
     double w0 = w;
@@ -207,25 +250,34 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
       path_int += w0 * static_cast<double>(frac / sig_abs);
     }
     w = w0 - w0 * static_cast<double>(frac);
-  };
+  }
+
   // roulette: the remaining weight goes to the material (RadiationIMC.hpp:1812-1820)
-  auto roulette = [&]() {
+  template <typename Sink>
+  STORM_HD void roulette(const StepConsts &k, Sink &sink) {
     double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
-    flush(fixed_point ? wq : 0.0);
+    flush(fixed_point ? wq : 0.0, k, sink);
     if constexpr (fixed_point) sink.roulette(cell, wq);
     w = 0.0;
     outcome = kGone;
-  };
-  auto census = [&]() {
+  }
+
+  template <typename Sink>
+  STORM_HD void census(const StepConsts &k, Sink &sink) {
     // the weight is kept as a multiple of the quantum in both engines so
     // that population control can work in exact integers
     double wq = fx::quantize_down(w, k.eq);
-    flush(wq);
+    flush(wq, k, sink);
     w = wq;
     outcome = kCensus;
-  };
+  }
 
-  for (int seg = 0; seg < max_segments; ++seg) {
+  // One segment of the history (a random-walk step, a DDMC step or an
+  // analytic step to the next face, scatter or census event); returns the
+  // outcome, kAlive while the packet goes on.
+  template <typename Sink>
+  STORM_HD int segment(const MeshView<Real> &m, const CellXS<Real> *xs, const AccelView<Real> &acc,
+                       const StepConsts &k, Sink &sink) {
     const uint32_t i = ijk_i(ijk), j = ijk_j(ijk), kk = ijk_k(ijk);
     const Real w0w = m.width[0][i], w1w = m.width[1][j], w2w = m.width[2][kk];
     const CellXS<Real> x = xs[cell];
@@ -263,8 +315,8 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
           if (s < Real(0)) s = Real(0);
           sink.rw_step();
           if (w < static_cast<double>(wmin * Real(0.1))) {
-            roulette();
-            break;
+            roulette(k, sink);
+            return outcome;
           }
           // new position on the sphere (leak) or inside it (census)
           Real dir[3];
@@ -297,11 +349,8 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
           o0 = nd[0];
           o1 = nd[1];
           o2 = nd[2];
-          if (!leak) {
-            census();
-            break;
-          }
-          continue;
+          if (!leak) census(k, sink);
+          return outcome;
         }
         if ((ca.flags & 2u) && ca.total > Real(0) && tauR >= acc.ddmc_particle_tau) {
           // ---------------- DDMC step (tryDDMCStep, gray)
@@ -313,12 +362,12 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
           if (s < Real(0)) s = Real(0);
           sink.ddmc_step();
           if (w < static_cast<double>(wmin)) {
-            roulette();
-            break;
+            roulette(k, sink);
+            return outcome;
           }
           if (cen) {
-            census();
-            break;
+            census(k, sink);
+            return outcome;
           }
           // leak through a face chosen in proportion to its rate
           Real pick = M::uniform(r.v[1]) * ca.total;
@@ -360,8 +409,8 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
           u0 = Real(0.5);
           u1 = Real(0.5);
           u2 = Real(0.5);
-          enter_neighbour(fax, up);
-          continue;
+          enter_neighbour(fax, up, m, xs, k, sink);
+          return outcome;
         }
       }
     }
@@ -441,8 +490,8 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
 
     // Roulette (RadiationIMC.hpp:1812-1820): the remaining weight goes to the material.
     if (w < static_cast<double>(wmin)) {
-      roulette();
-      break;
+      roulette(k, sink);
+      return outcome;
     }
 
     if (ev == 0) {
@@ -453,21 +502,21 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
       const bool outside = up ? (idx + 1u >= n) : (idx == 0u);
       if (!outside) {
         // interior face: the packet enters the neighbour on the opposite face
-        enter_neighbour(fax, up);
+        enter_neighbour(fax, up, m, xs, k, sink);
       } else {
         const int face = 2 * fax + (up ? 1 : 0);
         put3<Real>(fax, u0, u1, u2, up ? Real(1) : Real(0));
         if ((m.escape_mask >> face) & 1u) {
           // escape (REMOVE); the weight leaves the problem
           double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
-          flush(wq);
+          flush(wq, k, sink);
           sink.escape(cell, wq);
           w = 0.0;
           outcome = kEscaped;
-          break;
+        } else {
+          // specular reflection on the box face (BoundaryCondition.hpp:87-108)
+          put3<Real>(fax, o0, o1, o2, -of);
         }
-        // specular reflection on the box face (BoundaryCondition.hpp:87-108)
-        put3<Real>(fax, o0, o1, o2, -of);
       }
     } else if (ev == 1) {
       // effective scatter: isotropic redirection (RadiationIMC.hpp:1827-1874)
@@ -477,27 +526,22 @@ STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const Ce
       o1 = nd[1];
       o2 = nd[2];
     } else {
-      census();
-      break;
+      census(k, sink);
     }
+    return outcome;
   }
-  if (outcome == kAlive) {
-    // segment budget used up: close the visit so the next pass starts clean
-    double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
-    flush(wq);
-    w = wq;
-  }
-  p.u[0] = u0;
-  p.u[1] = u1;
-  p.u[2] = u2;
-  p.o[0] = o0;
-  p.o[1] = o1;
-  p.o[2] = o2;
-  p.s = s;
-  p.w = w;
-  p.ijk = ijk;
-  p.ctr = ctr;
-  return outcome;
+};
+
+// One transport pass of a packet: at most `max_segments` segments, then the
+// packet is written back (alive, in census, absorbed or escaped).
+template <typename Real, typename Sink>
+STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const CellXS<Real> *xs,
+                              const AccelView<Real> &acc, const StepConsts &k, int max_segments, Sink &sink) {
+  Tracker<Real> t;
+  t.begin(p, m, xs, k);
+  for (int seg = 0; seg < max_segments && t.outcome == kAlive; ++seg) t.segment(m, xs, acc, k, sink);
+  t.end(p, k, sink);
+  return t.outcome;
 }
 
 }  // namespace storm

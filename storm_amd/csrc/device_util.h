@@ -26,6 +26,27 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *counter) {
   return pred ? base + lane_rank(want) : ~0u;
 }
 
+// Segmented wavefront sum: adds `v` over each run of consecutive lanes with
+// equal `key` (inclusive scan restarted at each run head, 6 shuffle steps).
+// All 64 lanes must be active.  Returns true on the last lane of a run,
+// whose `v` then holds the run total: one atomic per run instead of one per
+// lane when the keys come sorted (packets of one cell side by side).
+template <typename T>
+__device__ __forceinline__ bool wave_run_sum(uint32_t key, T &v) {
+  const uint32_t lane = __lane_id();
+  const uint32_t prev = __shfl_up(key, 1u, 64);
+  const uint32_t next = __shfl_down(key, 1u, 64);
+  const uint64_t heads = __builtin_amdgcn_ballot_w64(lane == 0u || prev != key);
+  const uint64_t upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
+  const uint32_t start = 63u - static_cast<uint32_t>(__clzll(static_cast<long long>(heads & upto)));
+#pragma unroll
+  for (uint32_t off = 1; off < 64u; off <<= 1) {
+    const T o = __shfl_up(v, off, 64);
+    if (lane >= start + off) v += o;
+  }
+  return lane == 63u || next != key;
+}
+
 // Integer sums are order independent: per-block LDS reduction, one atomic.
 template <int BLOCK>
 __device__ __forceinline__ void block_atomic_add_u64(uint64_t v, unsigned long long *dst) {

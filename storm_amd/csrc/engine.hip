@@ -49,11 +49,13 @@ struct DevBuf {
     p = nullptr;
     n = 0;
   }
-  // grow without keeping the contents
+  // grow without keeping the contents; by at least half the capacity, so a
+  // census that grows step by step reallocates only a few times
   void ensure(size_t m) {
     if (m <= n && p) return;
+    const size_t grown = n + n / 2;
     release();
-    size_t cap = std::max<size_t>(m, 1);
+    size_t cap = std::max<size_t>(std::max(m, grown), 1);
     HIPCHECK(hipMalloc(reinterpret_cast<void **>(&p), cap * sizeof(T)));
     n = cap;
   }
@@ -116,6 +118,7 @@ struct EngineBase {
   double source_T = 0.0;
   int32_t block = 256;
   int32_t seg_per_pass = 1 << 20;
+  int32_t schedule = STORM_GPU_SCHEDULE_REFILL;
   bool keep_inputs = false;
 };
 
@@ -186,6 +189,8 @@ struct Engine final : EngineBase {
     seed = p->seed;
     if (imc.new_photons_per_cell == 0) throw EngineError(1, "RadiationIMC requires newPhotonsPerCell > 0");
     accel_on = imc.with_random_walk || imc.with_ddmc;
+    // the FP64 engine is the plain form by default: one packet per thread
+    schedule = precision == STORM_GPU_FP64 ? STORM_GPU_SCHEDULE_PASS : STORM_GPU_SCHEDULE_REFILL;
     if (bnd.source_face != -1 && bnd.source_face != STORM_XLO)
       throw EngineError(1, "only an x- Planck source face is supported (SideTemperature)");
     if (!p->material.sigma_a0 && material_mode == STORM_GPU_MATERIAL_DEVICE)
@@ -358,6 +363,21 @@ struct Engine final : EngineBase {
     return m;
   }
 
+  // Workgroups of the refill transport that fit on the device at once.
+  uint32_t resident_cache[3] = {0, 0, 0};
+  uint32_t resident_blocks(int32_t b) {
+    uint32_t &r = resident_cache[b == 64 ? 0 : (b == 128 ? 1 : 2)];
+    if (!r) {
+      int dev = 0, per_cu = 0;
+      HIPCHECK(hipGetDevice(&dev));
+      hipDeviceProp_t prop;
+      HIPCHECK(hipGetDeviceProperties(&prop, dev));
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport_refill<Real>, b, 0));
+      r = static_cast<uint32_t>(std::max(per_cu, 1) * std::max(prop.multiProcessorCount, 1));
+    }
+    return r;
+  }
+
   double deterministic_sum(const double *d, uint64_t n) {
     constexpr int G = 512;
     k_sum_partials<256><<<G, 256, 0, stream>>>(d, n, partials.p);
@@ -454,7 +474,9 @@ struct Engine final : EngineBase {
     }
     const uint64_t n_bc = static_cast<uint64_t>(nfaces) * npc;
     const uint64_t n_total = n_census + n_emit + n_bc;
-    if (n_total >= (1ull << 32) - 1) throw EngineError(4, "more than 2^32 packets in one step");
+    // 32-bit packet indices; the refill queue head may run past the count by
+    // one wavefront of slots per resident wavefront
+    if (n_total >= (1ull << 32) - (1ull << 24)) throw EngineError(4, "more than 2^32 - 2^24 packets in one step");
     counters.initial_particles = n_census;
     counters.emitted_particles = n_emit;
     counters.boundary_particles = n_bc;
@@ -508,10 +530,18 @@ struct Engine final : EngineBase {
       PacketBufs<Real> &src = ping ? pa : pb;
       PacketBufs<Real> &dst = ping ? pb : pa;
       zero_u32(counters_u32.p, 1);  // n_out
+      if (schedule == STORM_GPU_SCHEDULE_REFILL) zero_u32(counters_u32.p + 2, 1);  // queue head
       HIPCHECK(hipEventRecord(ev0, stream));
-      k_transport<Real><<<blocks_for(n_in, block), block, 0, stream>>>(
-          static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
-          xs.p, av, kc, seg_per_pass, tal, leaving.p);
+      if (schedule == STORM_GPU_SCHEDULE_REFILL) {
+        const uint32_t grid = std::min<uint32_t>(blocks_for(n_in, block), resident_blocks(block));
+        k_transport_refill<Real><<<grid, block, 0, stream>>>(
+            static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
+            xs.p, av, kc, seg_per_pass, tal, leaving.p, counters_u32.p + 2);
+      } else {
+        k_transport<Real><<<blocks_for(n_in, block), block, 0, stream>>>(
+            static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
+            xs.p, av, kc, seg_per_pass, tal, leaving.p);
+      }
       HIPCHECK(hipEventRecord(ev1, stream));
       uint32_t h[2];
       HIPCHECK(hipMemcpyAsync(h, counters_u32.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
@@ -1050,13 +1080,16 @@ int storm_gpu_import_census(storm_gpu_engine *eng, const storm_packet_record *in
   return guarded([&] { as_engine(eng)->import_census(in, n); });
 }
 
-int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segments_per_pass) {
+int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segments_per_pass, int32_t schedule) {
   return guarded([&] {
     if (block_size != 64 && block_size != 128 && block_size != 256)
       throw EngineError(1, "block size must be 64, 128 or 256");
     if (segments_per_pass < 1) throw EngineError(1, "segments per pass must be >= 1");
+    if (schedule != STORM_GPU_SCHEDULE_PASS && schedule != STORM_GPU_SCHEDULE_REFILL)
+      throw EngineError(1, "schedule must be STORM_GPU_SCHEDULE_PASS or STORM_GPU_SCHEDULE_REFILL");
     as_engine(eng)->block = block_size;
     as_engine(eng)->seg_per_pass = segments_per_pass;
+    as_engine(eng)->schedule = schedule;
   });
 }
 

@@ -247,35 +247,49 @@ template <typename Real>
 __global__ __launch_bounds__(256) void k_import(uint32_t n, CensusSoA<Real> cen, PacketSoA<Real> out,
                                                 Real s0, int32_t eq_old, int32_t eq_new, MeshDims d,
                                                 Tallies t, unsigned long long *__restrict__ requant) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  Packet<Real> p;
-  auto a = cen.pos[i];
-  auto b = cen.dir[i];
-  uint2 m = cen.meta[i];
-  p.u[0] = a.x;
-  p.u[1] = a.y;
-  p.u[2] = a.z;
-  p.s = s0;
-  p.o[0] = b.x;
-  p.o[1] = b.y;
-  p.o[2] = b.z;
-  double w = cen.w[i];
-  uint32_t c = (ijk_i(m.x) * d.ny + ijk_j(m.x)) * d.nz + ijk_k(m.x);
-  if (eq_new > eq_old) {
-    double wq = fx::quantize_down(w, eq_new);
-    if (wq != w) tally_limbs(requant, c, fx::limbs_of(w - wq, eq_old));  // exact in the old quantum
-    w = wq;
+  // no early return: the per-cell sums below need every lane of the wavefront
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  uint32_t c = 0xFFFFFFFFu;
+  fx::Limbs L{0u, 0u};
+  if (valid) {
+    Packet<Real> p;
+    auto a = cen.pos[i];
+    auto b = cen.dir[i];
+    uint2 m = cen.meta[i];
+    p.u[0] = a.x;
+    p.u[1] = a.y;
+    p.u[2] = a.z;
+    p.s = s0;
+    p.o[0] = b.x;
+    p.o[1] = b.y;
+    p.o[2] = b.z;
+    double w = cen.w[i];
+    c = (ijk_i(m.x) * d.ny + ijk_j(m.x)) * d.nz + ijk_k(m.x);
+    if (eq_new > eq_old) {
+      double wq = fx::quantize_down(w, eq_new);
+      if (wq != w) tally_limbs(requant, c, fx::limbs_of(w - wq, eq_old));  // exact in the old quantum
+      w = wq;
+    }
+    p.w = w;
+    p.wmin = static_cast<Real>(w * 1e-3);
+    p.ijk = m.x;
+    p.id = i;
+    p.ctr = 0u;
+    p.flags = 0u;
+    store_packet(out, i, p);
+    L = fx::limbs_of(w, eq_new);
   }
-  p.w = w;
-  p.wmin = static_cast<Real>(w * 1e-3);
-  p.ijk = m.x;
-  p.id = i;
-  p.ctr = 0u;
-  p.flags = 0u;
-  store_packet(out, i, p);
-  tally_limbs(t.cin, c, fx::limbs_of(w, eq_new));
-  atomicAdd(&t.begin[c], 1u);
+  // the census comes sorted by cell after population control: one atomic per run
+  unsigned long long lo = L.lo, hi = L.hi;
+  uint32_t one = valid ? 1u : 0u;
+  wave_run_sum(c, lo);
+  wave_run_sum(c, hi);
+  if (wave_run_sum(c, one) && valid) {
+    if (lo) atomicAdd(&t.cin[2 * c], lo);
+    if (hi) atomicAdd(&t.cin[2 * c + 1], hi);
+    atomicAdd(&t.begin[c], one);
+  }
 }
 
 // Emission packets: uniform position in the cell (RandomInCellPositionSampler
@@ -285,20 +299,24 @@ __global__ __launch_bounds__(256) void k_emit(uint32_t n_emit, uint32_t N, const
                                               const double *__restrict__ wemit, PacketSoA<Real> out,
                                               uint32_t base, Real s0, MeshDims d, uint64_t seed,
                                               uint32_t step, uint32_t *__restrict__ begin) {
-  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n_emit) return;
-  // cell owning packet q: last c with offsets[c] <= q
-  uint32_t lo = 0, hi = N;  // offsets[N] = n_emit
-  while (hi - lo > 1u) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (offsets[mid] <= q) lo = mid;
-    else hi = mid;
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = q < n_emit;
+  uint32_t c = 0xFFFFFFFFu;
+  if (valid) {
+    // cell owning packet q: last c with offsets[c] <= q
+    uint32_t lo = 0, hi = N;  // offsets[N] = n_emit
+    while (hi - lo > 1u) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (offsets[mid] <= q) lo = mid;
+      else hi = mid;
+    }
+    c = lo;
+    Packet<Real> p;
+    birth_emission<Real>(p, base + q, ijk_of_flat(c, d), wemit[c], s0, seed, step);
+    store_packet(out, base + q, p);
   }
-  const uint32_t c = lo;
-  Packet<Real> p;
-  birth_emission<Real>(p, base + q, ijk_of_flat(c, d), wemit[c], s0, seed, step);
-  store_packet(out, base + q, p);
-  atomicAdd(&begin[c], 1u);
+  uint32_t one = valid ? 1u : 0u;
+  if (wave_run_sum(c, one) && valid) atomicAdd(&begin[c], one);
 }
 
 // Planck surface source on the x- box face: per face Npc packets of weight
@@ -307,14 +325,17 @@ template <typename Real>
 __global__ __launch_bounds__(256) void k_boundary(uint32_t n_bc, uint32_t npc, const double *__restrict__ wface,
                                                   PacketSoA<Real> out, uint32_t base, Real s0, MeshDims d,
                                                   uint64_t seed, uint32_t step, uint32_t *__restrict__ begin) {
-  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n_bc) return;
-  const uint32_t f = q / npc;  // face index j * nz + k
-  const uint32_t j = f / d.nz, k = f % d.nz;
-  Packet<Real> p;
-  birth_boundary<Real>(p, base + q, j, k, wface[f], s0, seed, step);
-  store_packet(out, base + q, p);
-  atomicAdd(&begin[(0u * d.ny + j) * d.nz + k], 1u);
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = q < n_bc;
+  const uint32_t f = valid ? q / npc : 0xFFFFFFFFu;  // face index j * nz + k = cell (0, j, k)
+  if (valid) {
+    const uint32_t j = f / d.nz, k = f % d.nz;
+    Packet<Real> p;
+    birth_boundary<Real>(p, base + q, j, k, wface[f], s0, seed, step);
+    store_packet(out, base + q, p);
+  }
+  uint32_t one = valid ? 1u : 0u;
+  if (wave_run_sum(f, one) && valid) atomicAdd(&begin[f], one);
 }
 
 // ------------------------------------------------------------------ transport
@@ -378,6 +399,71 @@ __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real
     tally_limbs(t.cpre, c, fx::limbs_of(p.w, k.eq));
   }
   wave_add_u32((valid && outcome == kEscaped) ? 1u : 0u, &stats[0]);
+  if (acc.cell) {
+    wave_add_u32(sink.rw, &stats[1]);
+    wave_add_u32(sink.ddmc, &stats[2]);
+  }
+}
+
+// Persistent transport with lane refill.  Each wavefront advances its 64
+// packets one segment at a time; a lane whose packet ended (census, absorbed,
+// escaped, or `max_segments` segments in this launch) takes the next packet
+// of the input queue (`*queue` counts handed-out packets, one atomic per
+// wavefront).  Lanes stay busy until the queue is drained, instead of idling
+// until the longest history of their wavefront ends.  Per packet the result
+// is that of k_transport with the same `max_segments`.
+// stats: [0] escapes, [1] random-walk steps, [2] DDMC steps
+template <typename Real>
+__global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out,
+                                                          uint32_t *__restrict__ n_out, CensusSoA<Real> cen,
+                                                          uint32_t *__restrict__ n_cen, MeshView<Real> mesh,
+                                                          const CellXS<Real> *__restrict__ xs, AccelView<Real> acc,
+                                                          StepConsts k, int32_t max_segments, Tallies t,
+                                                          unsigned long long *__restrict__ stats,
+                                                          uint32_t *__restrict__ queue) {
+  DeviceSink sink{t, k.eq};
+  Tracker<Real> tr;
+  bool busy = false;
+  bool drained = false;  // uniform across the wavefront
+  int32_t nseg = 0;
+  uint32_t escapes = 0;
+  for (;;) {
+    if (!drained) {
+      const uint32_t slot = wave_append(!busy, queue);
+      const bool got = !busy && slot < n_in;
+      if (__builtin_amdgcn_ballot_w64(!busy && slot >= n_in)) drained = true;
+      if (got) {
+        const Packet<Real> p = load_packet(in, slot);
+        tr.begin(p, mesh, xs, k);
+        busy = true;
+        nseg = 0;
+      }
+    }
+    // every lane idle: the queue was drained on this or an earlier round
+    if (__builtin_amdgcn_ballot_w64(busy) == 0ull) break;
+    int outcome = kAlive;
+    if (busy) {
+      outcome = tr.segment(mesh, xs, acc, k, sink);
+      ++nseg;
+    }
+    const bool fin = busy && (outcome != kAlive || nseg >= max_segments);
+    Packet<Real> p;
+    if (fin) {
+      tr.end(p, k, sink);
+      busy = false;
+    }
+    const bool census = fin && outcome == kCensus;
+    const bool survivor = fin && outcome == kAlive;
+    escapes += (fin && outcome == kEscaped) ? 1u : 0u;
+    const uint32_t cslot = wave_append(census, n_cen);
+    if (census) {
+      store_census(cen, cslot, p);
+      tally_limbs(t.cpre, flat_cell(mesh, p.ijk), fx::limbs_of(p.w, k.eq));
+    }
+    const uint32_t oslot = wave_append(survivor, n_out);
+    if (survivor) store_packet(out, oslot, p);
+  }
+  wave_add_u32(escapes, &stats[0]);
   if (acc.cell) {
     wave_add_u32(sink.rw, &stats[1]);
     wave_add_u32(sink.ddmc, &stats[2]);

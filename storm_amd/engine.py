@@ -67,8 +67,12 @@ class GpuEngine:
     def set_source_temperature(self, T: float):
         self._check(self.lib.storm_gpu_set_source_temperature(self.h, float(T)))
 
-    def set_tuning(self, block: int = 256, segments_per_pass: int = 1 << 20):
-        self._check(self.lib.storm_gpu_set_tuning(self.h, int(block), int(segments_per_pass)))
+    def set_tuning(self, block: int = 256, segments_per_pass: int = 1 << 20, schedule: int | None = None):
+        """Transport launch shape; `schedule` is capi.SCHEDULE_PASS or capi.SCHEDULE_REFILL
+        (default: refill for the mixed engine, pass for FP64)."""
+        if schedule is None:
+            schedule = capi.SCHEDULE_PASS if self.precision == capi.FP64 else capi.SCHEDULE_REFILL
+        self._check(self.lib.storm_gpu_set_tuning(self.h, int(block), int(segments_per_pass), int(schedule)))
 
     def set_cell_state(self, T=None, sigma_a=None, sigma_s=None, cv=None, e=None):
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (T, sigma_a, sigma_s, cv, e)]

@@ -44,4 +44,21 @@ def test_struct_layouts_match_header_sizes():
 
 def test_abi_version_without_gpu():
     lib = capi.load_library()
-    assert lib.storm_gpu_abi_version() == 1
+    assert lib.storm_gpu_abi_version() == 2
+
+
+def test_cpp_manager_program_fails_loudly_without_engine():
+    """tests/cpp/manager_slab (user code over MonteCarloManagerGPU.hpp) links
+    the engine library; without a GPU it must stop with the engine's error,
+    never fall back to anything."""
+    import json
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "manager_slab")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/_build/manager_slab is not built (run __graft_entry__.build())")
+    out = subprocess.run([exe, "0", "0", "1", "0", "1", "10", "10", "2", "4"], capture_output=True, text=True,
+                         timeout=120)
+    if out.returncode == 0:  # a GPU is present: the run must be real
+        assert len(json.loads(out.stdout)["runs"]) == 1
+    else:
+        assert out.returncode == 1 and "gfx950 engine" in out.stderr, (out.returncode, out.stderr)

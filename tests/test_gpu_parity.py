@@ -14,7 +14,10 @@ Gates (per the engine's contract):
 from __future__ import annotations
 
 import concurrent.futures as cf
+import json
 import math
+import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -227,3 +230,112 @@ def test_c2_full_size_one_step_invariants():
     assert c["segments"] == int(segs.sum()) > c["start_particles"]
     assert c["start_particles"] == c["emitted_particles"] + c["boundary_particles"] + c["initial_particles"]
     assert c["boundary_particles"] == 64 * 64 * spec.source_per_face
+
+
+def _run_tallies(spec, precision, **tuning):
+    with GpuEngine(spec, precision) as e:
+        if tuning:
+            e.set_tuning(**tuning)
+        for _ in range(spec.steps):
+            e.step()
+        return e.cells(), e.tallies(), e.counters()
+
+
+@pytest.mark.parametrize("make", [C2, C3], ids=["c2", "c3"])
+def test_refill_schedule_matches_pass_schedule(make):
+    """The persistent refill transport and the one-packet-per-thread pass
+    give bit-identical tallies and cell states: a packet's history depends
+    only on its id, the step and the segment budget, never on the lane."""
+    spec = make(12)
+    a = _run_tallies(spec, capi.MIXED, schedule=capi.SCHEDULE_PASS)
+    b = _run_tallies(spec, capi.MIXED, schedule=capi.SCHEDULE_REFILL, block=128)
+    for k in ("T", "e", "erad", "census"):
+        assert np.array_equal(a[0][k], b[0][k]), k
+    for k in ("dep", "rdep", "pint", "esc", "segs"):
+        assert np.array_equal(a[1][k], b[1][k]), k
+    assert a[2]["segments"] == b[2]["segments"]
+
+
+@pytest.mark.parametrize("schedule", [capi.SCHEDULE_PASS, capi.SCHEDULE_REFILL], ids=["pass", "refill"])
+def test_small_segment_budget_replays_exactly(oracle, schedule):
+    """With 3 segments per launch the packets go through many launches; the
+    CPU replay with the same budget reproduces the tallies bit for bit."""
+    spec = C3(8)
+    with GpuEngine(spec, capi.MIXED) as e:
+        e.set_tuning(block=256, segments_per_pass=3, schedule=schedule)
+        for s in range(spec.steps):
+            e.keep_transport_inputs(s == spec.steps - 1)
+            e.step()
+        pk, n, xs, consts = e.transport_inputs()
+        got = e.tallies()
+        accel = e.accel_data()
+        cnt = e.counters()
+    assert cnt["transport_launches"] > 3
+    ref, outc = oracle.replay_transport(spec, pk, n, xs, consts, segments_per_pass=3, accel=accel)
+    assert outc[:4].sum() == n
+    for k in ("segs", "dep", "rdep", "pint", "esc"):
+        assert np.array_equal(got[k].astype(np.uint64), ref[k].astype(np.uint64)), k
+
+
+def test_beginning_counts_per_cell():
+    """Packets per cell at transport start (one atomic per run of a cell's
+    packets): they add up to the step's packets, the x- column holds the
+    boundary packets, and after the first step the census of each cell."""
+    spec = C2(13)
+    with GpuEngine(spec, capi.MIXED) as e:
+        e.step()
+        b1 = e.beginning_counts()
+        c1 = e.counters()
+        assert int(b1.sum()) == c1["start_particles"]
+        nx, ny, nz = spec.shape
+        assert (b1.reshape(nx, ny, nz)[0] >= spec.source_per_face).all()
+        rec = e.export_census()
+        e.step()
+        b2 = e.beginning_counts()
+        c2 = e.counters()
+    assert int(b2.sum()) == c2["start_particles"]
+    cen = np.bincount(rec["cell"].astype(np.int64), minlength=spec.ncells)
+    assert (b2 >= cen).all()
+
+
+# ---------------------------------------------------------------- C++ drop-in manager
+MANAGER_EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "cpp", "_build",
+                           "manager_slab")
+
+
+def _run_manager(spec, precision, resident, seeds):
+    """Runs tests/cpp/manager_slab.cpp (reference-style user code over
+    MonteCarloManagerGPU.hpp) on the slab `spec` for each seed."""
+    args = [MANAGER_EXE, str(precision), str(int(resident)), str(spec.steps), str(seeds[0]), str(len(seeds)),
+            str(spec.shape[0]), str(spec.source_per_face), str(spec.new_photons_per_cell), str(spec.comb_nmin)]
+    print("manager:", " ".join(args), flush=True)
+    out = subprocess.run(args, capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0, out.stderr
+    runs = json.loads(out.stdout)["runs"]
+    return {k: np.array([r[k] for r in runs]) for k in ("T", "erad", "census")}, runs
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
+def test_cpp_manager_matches_oracle(c1_ref, precision):
+    """The C++ manager (host-tabulated RadiationIMC plugins, census returned
+    to the caller as std::vector<Particle> and passed back every step) agrees
+    with the FP64 oracle like the engine does."""
+    got, _ = _run_manager(C1(0), precision, False, list(range(NSEEDS)))
+    _assert_parity(c1_ref, got, f"c1-manager/{precision}")
+
+
+def test_cpp_manager_resident_is_the_engine():
+    """With the census resident in HBM the manager's step is the engine's
+    step: host-tabulated opacity, cv and EOS give the same doubles as the
+    device material model, so the results are bit-identical."""
+    spec = C1(3)
+    got, runs = _run_manager(spec, capi.MIXED, True, [3])
+    with GpuEngine(spec, capi.MIXED) as e:
+        for _ in range(spec.steps):
+            e.step()
+        c = e.cells()
+        n = e.census_size()
+    assert runs[0]["census_packets"] == n
+    assert np.array_equal(got["T"][0], c["T"])
+    assert np.array_equal(got["erad"][0], c["erad"])
+    assert np.allclose(got["census"][0], c["census"], rtol=1e-12, atol=0)
