@@ -218,32 +218,100 @@ def c3_step(seed: int = 0, nx: int = 256, nyz: int = 16, packets_scale: float = 
         ddmc_cell_tau=10.0, ddmc_particle_tau=5.0, seed=seed)
 
 
+# Marshak-wave problems 1-4 (examples/marshak_wave/MarshakCommon.hpp:48-112):
+# kappa_P = kP0 (keV/T)^alpha rho^beta, scattering kappa_R - kappa_P,
+# e = f T^eb rho^(1 - mu) per volume, T_bath = Tc (t/ns)^Te keV.
 MARSHAK = {
-    1: dict(L=0.2, kP0=0.1, kR0=100.0, alpha=3.0, beta=0.0, f=6.860085e14 / KEV_K ** 4, eb=4.0, mu=0.0,
-            Tc=1.008038, Te=1.0 / 3.0),
-    2: dict(L=0.2, kP0=100.0, kR0=100.0, alpha=3.0, beta=0.0, f=6.860085e14 / KEV_K ** 4, eb=4.0, mu=0.0,
-            Tc=1.014565, Te=1.0 / 3.0),
+    1: dict(L=0.2, x0=0.0, kP0=0.1, kR0=100.0, alpha=3.0, beta=0.0, f=6.860085e14 / KEV_K ** 4, eb=4.0, mu=0.0,
+            Tc=1.008038, Te=1.0 / 3.0, dt0=1e-15),
+    2: dict(L=0.2, x0=0.0, kP0=100.0, kR0=100.0, alpha=3.0, beta=0.0, f=6.860085e14 / KEV_K ** 4, eb=4.0, mu=0.0,
+            Tc=1.014565, Te=1.0 / 3.0, dt0=1e-15),
+    3: dict(L=1.0, x0=0.0, kP0=0.1, kR0=40.0, alpha=1.5, beta=1.2, f=1e14 / KEV_K ** 3.4, eb=3.4, mu=0.14,
+            Tc=1.0470478, Te=86.0 / 57.0, dt0=1e-15),
+    4: dict(L=1.0, x0=1e-5, kP0=0.001, kR0=2.0, alpha=4.5, beta=1.9, f=1e14 / KEV_K ** 6.0, eb=6.0, mu=0.3,
+            Tc=1.01008116, Te=14.0 / 139.0, dt0=1e-17),
 }
+MARSHAK_T_FINAL = 1e-9
 
 
-def marshak(problem: int = 2, nx: int = 64, seed: int = 0, new_photons: int = 15, boundary_photons: int = 100,
-            steps: int = 10, dt: float = 1e-12) -> ProblemSpec:
-    """Marshak wave 1 or 2 (examples/marshak_wave/MarshakCommon.hpp:48-172), gray, no random walk."""
+def marshak_density(problem: int, x):
+    """ComputeDensity (MarshakCommon.hpp:114-123)."""
+    x = np.maximum(np.asarray(x, dtype=np.float64), 1e-30)
+    if problem == 3:
+        return x ** (20.0 / 19.0)
+    if problem == 4:
+        return x ** (-40.0 / 139.0)
+    return np.ones_like(x)
+
+
+def marshak4_edges(x0: float, xmax: float) -> np.ndarray:
+    """BuildMarshak4MeshEdges (MarshakCommon.hpp:256-271): 512 cells growing by 1.00969..."""
+    e = np.empty(513)
+    e[0] = x0
+    w = 7.0e-5
+    for i in range(512):
+        e[i + 1] = xmax if i == 511 else e[i] + w
+        w *= 1.0096903847403653
+    return e
+
+
+def marshak(problem: int = 2, nx: int = 256, seed: int = 0, new_photons: int = 15, boundary_photons: int = 100,
+            steps: int = 10, dt: float | None = None, random_walk: bool = True) -> ProblemSpec:
+    """Marshak wave 1-4 as RunMarshakWave sets it up (MarshakCommon.hpp:273-330): gray,
+    random walk on, comb(15, 6.0), x- drive with escape, reflecting elsewhere."""
     P = MARSHAK[problem]
-    L = P["L"]
-    xe = np.linspace(0.0, L, nx + 1)
-    h = L / nx
+    xmax = P["x0"] + P["L"]
+    if problem == 4:
+        xe = marshak4_edges(P["x0"], xmax)
+        h = xmax / 512.0
+    else:
+        xe = np.linspace(0.0, xmax, nx + 1)
+        h = xmax / nx
+    n = len(xe) - 1
+    xc = 0.5 * (xe[1:] + xe[:-1])
+    rho = marshak_density(problem, xc)
     T0 = 1e-3 * KEV_K
     return ProblemSpec(
         name=f"marshak{problem}", x_edges=xe, y_edges=np.array([0.0, h]), z_edges=np.array([0.0, h]),
-        sigma_a0=np.full(nx, P["kP0"]), sigma_s0=np.full(nx, max(P["kR0"] - P["kP0"], 0.0)),
-        T0=np.full(nx, T0), dt=dt, steps=steps,
-        T_ref=KEV_K, T_floor=1.0, a_pow=P["alpha"], s_pow=P["alpha"],
+        sigma_a0=np.full(n, P["kP0"]), sigma_s0=np.full(n, max(P["kR0"] - P["kP0"], 0.0)),
+        density=None if problem in (1, 2) else rho,
+        T0=np.full(n, T0), dt=P["dt0"] if dt is None else dt, steps=steps,
+        T_ref=KEV_K, T_floor=1.0, a_pow=P["alpha"], s_pow=P["alpha"], a_rho=P["beta"], s_rho=P["beta"],
         eos_f=P["f"], eos_beta=P["eb"], eos_mu=P["mu"], eos_cv_floor=1.0,
         face_kind=(capi.FACE_ESCAPE,) + (capi.FACE_REFLECT,) * 5, source_face=capi.XLO,
-        source_T=P["Tc"] * (1e-15 * 1e9) ** P["Te"] * KEV_K, freeze_T4=0,
+        source_T=marshak_bath_T(problem, P["dt0"]), freeze_T4=0,
         source_per_face=boundary_photons, new_photons_per_cell=new_photons,
-        comb=1, comb_nmin=15, comb_factor=6.0, seed=seed)
+        comb=1, comb_nmin=15, comb_factor=6.0, with_rw=int(random_walk), seed=seed)
+
+
+def marshak_schedule(problem: int):
+    """The reference's time steps to 1 ns (MarshakCommon.hpp:343-357):
+    yields (t_now, dt) with the bath temperature taken at t_now."""
+    dt0 = MARSHAK[problem]["dt0"]
+    t, dt = 0.0, dt0
+    while t < MARSHAK_T_FINAL:
+        t_now = max(t, dt0)
+        dt = min(dt, MARSHAK_T_FINAL - t)
+        yield t_now, dt
+        t += dt
+        dt = min(max(dt0, t * 1e-3), 5e-11)
+
+
+def marshak_l1(x_sim, T_sim, ref) -> float:
+    """ComputeL1 (MarshakCommon.hpp:192-228): mean relative |T_sim - T_ref| over
+    the reference points with T_ref > 1e-4 keV, T_sim linearly interpolated."""
+    l1, cnt, j = 0.0, 0, 0
+    for xr, Tr in zip(ref[:, 0], ref[:, 1]):
+        while j + 1 < len(x_sim) and x_sim[j + 1] < xr:
+            j += 1
+        if j + 1 >= len(x_sim):
+            break
+        fr = (xr - x_sim[j]) / (x_sim[j + 1] - x_sim[j])
+        Ts = T_sim[j] + fr * (T_sim[j + 1] - T_sim[j])
+        if Tr / KEV_K > 1e-4:
+            l1 += abs(Ts / KEV_K - Tr / KEV_K) / (Tr / KEV_K)
+            cnt += 1
+    return l1 / cnt if cnt else -1.0
 
 
 def marshak_bath_T(problem: int, t: float) -> float:
