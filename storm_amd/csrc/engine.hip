@@ -146,12 +146,12 @@ struct Engine final : EngineBase {
   // device: mesh and cells
   DevBuf<Real> wx, wy, wz;
   DevBuf<double> V, sa0, ss0, rho, T, e_mat, e_sync, fleck, sig_a, sig_s, Eemit, wemit, erad_avg, census_erad,
-      clamp, partials, host_sa, host_ss, host_cv, wface;
+      clamp, partials, host_sa, host_ss, host_cv, wface, dsc;
   bool have_host_sa = false, have_host_ss = false, have_host_cv = false;
   DevBuf<CellXS<Real>> xs;
   DevBuf<uint32_t> nphot, err, cnt_u32, start_u32, ncopy, vals, vals_out, counters_u32;
   DevBuf<uint64_t> offsets, keys, keys_out, offs64, nphot64;
-  DevBuf<unsigned long long> tally_u64, emit_limbs, W, requant, leaving;
+  DevBuf<unsigned long long> tally_u64, emit_limbs, W, requant, leaving, scal;
   DevBuf<double> tally_f64;
   DevBuf<ulonglong2> limbs_sorted, cum;
   DevBuf<CombCell> comb_cells;
@@ -245,7 +245,9 @@ struct Engine final : EngineBase {
     emit_limbs.ensure(2 * static_cast<size_t>(N));
     W.ensure(2 * static_cast<size_t>(N));
     requant.ensure(2 * static_cast<size_t>(N));
-    leaving.ensure(4);  // escapes, random-walk steps, DDMC steps
+    leaving.ensure(4);  // escapes, random-walk steps, DDMC steps, segments
+    scal.ensure(4);     // total census weight (low, high limb)
+    dsc.ensure(2);      // material energy before the step, census energy after it
     counters_u32.ensure(4);
     if (accel_on) {
       accel.ensure(N);
@@ -378,12 +380,16 @@ struct Engine final : EngineBase {
     return r;
   }
 
+  // Deterministic FP64 sum (fixed grid and tree) of d[0..n) into *out (device).
+  static constexpr int kSumGrid = 512;
+  void sum_async(const double *d, uint64_t n, double *out) {
+    k_sum_partials<256><<<kSumGrid, 256, 0, stream>>>(d, n, partials.p);
+    k_sum_partials<256><<<1, 256, 0, stream>>>(partials.p, kSumGrid, out);
+  }
   double deterministic_sum(const double *d, uint64_t n) {
-    constexpr int G = 512;
-    k_sum_partials<256><<<G, 256, 0, stream>>>(d, n, partials.p);
-    k_sum_partials<256><<<1, 256, 0, stream>>>(partials.p, G, partials.p + G);
+    sum_async(d, n, partials.p + kSumGrid);
     double h = 0.0;
-    HIPCHECK(hipMemcpyAsync(&h, partials.p + G, sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipMemcpyAsync(&h, partials.p + kSumGrid, sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     return h;
   }
@@ -407,7 +413,7 @@ struct Engine final : EngineBase {
     const Real s0 = static_cast<Real>(units::clight * dt);
     zero_u32(err.p, 1);
     eq_prev_ledger_ = eq_prev;
-    material_before_ = deterministic_sum(e_mat.p, N);
+    sum_async(e_mat.p, N, dsc.p);  // material energy before the step, read by the ledger
 
     // preStep: opacities, Fleck factor, emission energies
     k_prestep<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
@@ -570,12 +576,19 @@ struct Engine final : EngineBase {
         N, tal, eq, et, eq_prev, emit_limbs.p, requant.p, dt, V.p, md,
         material_mode == STORM_GPU_MATERIAL_DEVICE ? 1 : 0, T.p, e_mat.p, e_sync.p, erad_avg.p, census_erad.p,
         clamp.p, err.p, precision == STORM_GPU_FP64 ? 1 : 0, xs.p);
-    census_total_prev = deterministic_sum(census_erad.p, N);
+    // census energy for the next step's quantum, total segments, error flags
+    // and counters: one synchronisation
+    sum_async(census_erad.p, N, dsc.p + 1);
+    k_total_u32<<<std::min<uint32_t>(blocks_for(N, 256), 1024u), 256, 0, stream>>>(N, tal.segs, leaving.p + 3);
     uint32_t herr = 0;
     unsigned long long hstats[4] = {0, 0, 0, 0};
+    double hcensus = 0.0;
+    HIPCHECK(hipMemcpyAsync(&hcensus, dsc.p + 1, sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipMemcpyAsync(&herr, err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipMemcpyAsync(hstats, leaving.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
+    census_total_prev = hcensus;
+    counters.segments = hstats[3];
     counters.leaving_particles = hstats[0];
     counters.random_walk_steps = hstats[1];
     counters.ddmc_steps = hstats[2];
@@ -586,22 +599,12 @@ struct Engine final : EngineBase {
     if (herr & kErrFleck) throw EngineError(5, "Invalid factor fleck in RadiationIMC::preStep");
     if (herr & kErrNegativeEnergy) throw EngineError(5, "Negative material internal energy in RadiationIMC");
     if (herr & kErrTallyOverflow) throw EngineError(5, "a cell received 2^28 tally additions in one step");
-    // segments
-    counters.segments = total_segments();
     auto t_end = std::chrono::steady_clock::now();
     counters.seconds_step = std::chrono::duration<double>(t_end - t_start).count();
   }
 
   int32_t last_eq_ = 0, last_et_ = 0;
   double last_dt_ = 0.0;
-
-  uint64_t total_segments() {
-    std::vector<uint32_t> h(N);
-    HIPCHECK(hipMemcpy(h.data(), tal.segs, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    uint64_t s = 0;
-    for (uint32_t v : h) s += v;
-    return s;
-  }
 
   // Comb (or pass-through) from `cpre` into `cnext`; returns the census size.
   uint64_t comb(uint32_t n, int32_t eq) {
@@ -632,23 +635,18 @@ struct Engine final : EngineBase {
       };
       ensure_temp(q);
     }
-    uint32_t last_start = 0, last_cnt = 0;
-    HIPCHECK(hipMemcpyAsync(&last_start, start_u32.p + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipMemcpyAsync(&last_cnt, cnt_u32.p + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    // total census weight W = sum of W_c (exact integer) as double
-    std::vector<unsigned long long> hW(2 * static_cast<size_t>(N));
-    HIPCHECK(hipMemcpyAsync(hW.data(), W.p, hW.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    // total census weight (exact integer) on the device
+    zero_u64(scal.p, 2);
+    k_total_limbs<<<std::min<uint32_t>(blocks_for(N, 256), 1024u), 256, 0, stream>>>(N, W.p, scal.p);
+    uint32_t last[2] = {0, 0};
+    HIPCHECK(hipMemcpyAsync(&last[0], start_u32.p + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipMemcpyAsync(&last[1], cnt_u32.p + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
-    const uint32_t n_valid = last_start + last_cnt;
+    const uint32_t n_valid = last[0] + last[1];
     if (n_valid == 0) return 0;
-    unsigned __int128 Wsum = 0;
-    for (uint32_t c = 0; c < N; ++c)
-      Wsum += (static_cast<unsigned __int128>(hW[2 * c + 1]) << fx::kLoBits) + hW[2 * c];
-    const double Wtot = (static_cast<double>(static_cast<uint64_t>(Wsum >> 64)) * 18446744073709551616.0 +
-                         static_cast<double>(static_cast<uint64_t>(Wsum))) *
-                        fpm::pow2d(eq);
-    k_comb_cells<<<blocks_for(N, 256), 256, 0, stream>>>(N, cnt_u32.p, W.p, Wtot, imc.comb_nmin, imc.comb_factor, eq,
-                                                         imc.comb ? 1 : 0, seed, step_index, comb_cells.p, tal.rem);
+    k_comb_cells<<<blocks_for(N, 256), 256, 0, stream>>>(N, cnt_u32.p, W.p, scal.p, imc.comb_nmin, imc.comb_factor,
+                                                         eq, imc.comb ? 1 : 0, seed, step_index, comb_cells.p,
+                                                         tal.rem);
     limbs_sorted.ensure(n_valid);
     cum.ensure(n_valid);
     ncopy.ensure(n_valid);
@@ -844,10 +842,9 @@ struct Engine final : EngineBase {
     }
     L->clamp = cls;
     L->material_after = ms;
-    L->material_before = material_before_;
+    HIPCHECK(hipMemcpy(&L->material_before, dsc.p, sizeof(double), hipMemcpyDeviceToHost));
   }
   int32_t eq_prev_ledger_ = 0;
-  double material_before_ = 0.0;
 
   void set_cell_state(const double *Ti, const double *sa, const double *ss, const double *cv,
                       const double *e) override {

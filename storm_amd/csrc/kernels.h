@@ -615,7 +615,8 @@ __device__ __forceinline__ fx::Limbs remainder_limbs(unsigned __int128 W, unsign
 
 // Per-cell comb targets (CombPopulationControl.hpp:157-206).
 __global__ __launch_bounds__(256) void k_comb_cells(uint32_t N, const uint32_t *__restrict__ cnt,
-                                                    const unsigned long long *__restrict__ W, double Wtot,
+                                                    const unsigned long long *__restrict__ W,
+                                                    const unsigned long long *__restrict__ Wsum,
                                                     uint64_t nmin, double factor, int32_t eq, int32_t comb_on,
                                                     uint64_t seed, uint32_t step, CombCell *__restrict__ cells,
                                                     unsigned long long *__restrict__ rem) {
@@ -627,6 +628,11 @@ __global__ __launch_bounds__(256) void k_comb_cells(uint32_t N, const uint32_t *
   cc.target = 0;
   cc.mode = 0;
   const uint32_t count = cnt[c];
+  // total census weight (exact integer, k_total_limbs), as a double in erg
+  const unsigned __int128 Wall = (static_cast<unsigned __int128>(Wsum[1]) << fx::kLoBits) + Wsum[0];
+  const double Wtot = (static_cast<double>(static_cast<uint64_t>(Wall >> 64)) * 18446744073709551616.0 +
+                       static_cast<double>(static_cast<uint64_t>(Wall))) *
+                      fpm::pow2d(eq);
   const unsigned __int128 Wc = limbs_value(W, c);
   const double Wd = static_cast<double>(static_cast<uint64_t>(Wc >> 64)) * 18446744073709551616.0 +
                     static_cast<double>(static_cast<uint64_t>(Wc)) ;
@@ -819,6 +825,55 @@ __global__ __launch_bounds__(256) void k_post(uint32_t N, Tallies t, int32_t eq,
     T[c] = mat.de2T(rho, E / (rho * Vc));  // synchronizeMaterialCell (:830-866)
   }
   if (t.segs[c] >= (1u << 28)) atomicOr(err, kErrTallyOverflow);
+}
+
+// Exact total of per-cell limb pairs: out[0] += low limbs, out[1] += high
+// limbs, each block's part normalised first (low part below 2^36), so the
+// total is (out[1] << 36) + out[0] with no overflow.
+__global__ __launch_bounds__(256) void k_total_limbs(uint32_t N, const unsigned long long *__restrict__ L,
+                                                     unsigned long long *__restrict__ out) {
+  constexpr unsigned long long kMask = (1ull << fx::kLoBits) - 1ull;
+  unsigned long long lo = 0, hi = 0;
+  for (uint32_t c = blockIdx.x * 256u + threadIdx.x; c < N; c += gridDim.x * 256u) {
+    lo += L[2 * c];
+    hi += L[2 * c + 1] + (lo >> fx::kLoBits);
+    lo &= kMask;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo += __shfl_xor(lo, off, 64);
+    hi += __shfl_xor(hi, off, 64);
+  }
+  hi += lo >> fx::kLoBits;
+  lo &= kMask;
+  __shared__ unsigned long long slo[4], shi[4];
+  const uint32_t w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63u) == 0u) {
+    slo[w] = lo;
+    shi[w] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    lo = slo[0] + slo[1] + slo[2] + slo[3];
+    hi = shi[0] + shi[1] + shi[2] + shi[3] + (lo >> fx::kLoBits);
+    lo &= kMask;
+    if (lo) atomicAdd(&out[0], lo);
+    if (hi) atomicAdd(&out[1], hi);
+  }
+}
+
+// out[0] += sum of v[0..N) (integer, order independent)
+__global__ __launch_bounds__(256) void k_total_u32(uint32_t N, const uint32_t *__restrict__ v,
+                                                   unsigned long long *__restrict__ out) {
+  unsigned long long s = 0;
+  for (uint32_t c = blockIdx.x * 256u + threadIdx.x; c < N; c += gridDim.x * 256u) s += v[c];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ unsigned long long sh[4];
+  if ((threadIdx.x & 63u) == 0u) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s = sh[0] + sh[1] + sh[2] + sh[3];
+    if (s) atomicAdd(out, s);
+  }
 }
 
 __global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v) {
