@@ -153,7 +153,9 @@ struct Tracker {
   static constexpr bool fixed_point = sizeof(Real) == 4;
 
   PacketStream rs;
-  Real u0, u1, u2, o0, o1, o2, s, wmin, cell_sig_abs;
+  Real u0, u1, u2, o0, o1, o2, s, wmin;
+  // the current cell's data, loaded when the packet enters it
+  Real cell_sig_abs, cell_sig_ev, cw0, cw1, cw2;
   double w, w_enter, path_int;  // w_enter: weight when the current cell visit began
   uint32_t ijk, ctr, cell, visit_segs, id;
   int outcome;
@@ -172,8 +174,7 @@ struct Tracker {
     ctr = p.ctr;
     wmin = p.wmin;
     id = p.id;
-    cell = flat_cell(m, ijk);
-    cell_sig_abs = xs[cell].sig_abs;
+    load_cell(m, xs);
     w_enter = w;
     path_int = 0.0;  // sum of w * (1 - e^{-x}) / sig_abs over the visit, erg*cm
     visit_segs = 0;
@@ -231,8 +232,17 @@ struct Tracker {
     else nk = up ? nk + 1u : nk - 1u;
     ijk = pack_ijk(ni, nj, nk);
     put3<Real>(fax, u0, u1, u2, up ? Real(0) : Real(1));
+    load_cell(m, xs);
+  }
+
+  STORM_HD void load_cell(const MeshView<Real> &m, const CellXS<Real> *xs) {
     cell = flat_cell(m, ijk);
-    cell_sig_abs = xs[cell].sig_abs;
+    const CellXS<Real> x = xs[cell];
+    cell_sig_abs = x.sig_abs;
+    cell_sig_ev = x.sig_ev;
+    cw0 = m.width[0][ijk_i(ijk)];
+    cw1 = m.width[1][ijk_j(ijk)];
+    cw2 = m.width[2][ijk_k(ijk)];
   }
 
   // continuous absorption over a path d (cm); `tally_free` adds w * d to the
@@ -242,6 +252,10 @@ struct Tracker {
     Real xabs = sig_abs * d;
     Real frac = -M::expm1(-xabs);
     double w0 = w;
+    w = w0 - w0 * static_cast<double>(frac);
+    // the mixed engine derives the path integral of absorbing cells from
+    // their deposit (see flush)
+    if (fixed_point && sig_abs > Real(0)) return;
     if (xabs >= Real(1e-12)) {
       path_int += w0 * static_cast<double>(frac / sig_abs);
     } else if (tally_free) {
@@ -249,7 +263,6 @@ struct Tracker {
     } else if (sig_abs > Real(0)) {
       path_int += w0 * static_cast<double>(frac / sig_abs);
     }
-    w = w0 - w0 * static_cast<double>(frac);
   }
 
   // roulette: the remaining weight goes to the material (RadiationIMC.hpp:1812-1820)
@@ -279,8 +292,8 @@ struct Tracker {
   STORM_HD int segment(const MeshView<Real> &m, const CellXS<Real> *xs, const AccelView<Real> &acc,
                        const StepConsts &k, Sink &sink) {
     const uint32_t i = ijk_i(ijk), j = ijk_j(ijk), kk = ijk_k(ijk);
-    const Real w0w = m.width[0][i], w1w = m.width[1][j], w2w = m.width[2][kk];
-    const CellXS<Real> x = xs[cell];
+    const Real w0w = cw0, w1w = cw1, w2w = cw2;
+    const Real sig_abs = cell_sig_abs;
     ++visit_segs;
 
     // Random numbers of this segment: one Philox block (two for an
@@ -310,7 +323,7 @@ struct Tracker {
           const Real sLeak = tauLeak * Ro * Ro * (Real(3) * ca.sig_t);
           const bool leak = sLeak <= s;
           const Real d = leak ? sLeak : s;
-          absorb(x.sig_abs, d, false);
+          absorb(sig_abs, d, false);
           s = leak ? s - d : Real(0);
           if (s < Real(0)) s = Real(0);
           sink.rw_step();
@@ -357,7 +370,7 @@ struct Tracker {
           const Real sEvent = (-M::log(M::uniform(r.v[0]))) / ca.total;
           const bool cen = s <= sEvent;
           const Real d = cen ? s : sEvent;
-          absorb(x.sig_abs, d, true);
+          absorb(sig_abs, d, true);
           s = cen ? Real(0) : s - d;
           if (s < Real(0)) s = Real(0);
           sink.ddmc_step();
@@ -460,7 +473,7 @@ struct Tracker {
     }
     // Distance to the next effective-scatter event (-ln xi / sigma_ev).
     Real xi = M::uniform(r.v[0]);
-    Real dcol = x.sig_ev > Real(0) ? (-M::log(xi)) / x.sig_ev : M::inf();
+    Real dcol = cell_sig_ev > Real(0) ? (-M::log(xi)) / cell_sig_ev : M::inf();
     // Event: face < scatter < census on ties, as the reference's min_element.
     int ev = 0;
     Real d = dface;
@@ -475,7 +488,7 @@ struct Tracker {
     s -= d;
     if (ev == 2 || s < Real(0)) s = Real(0);
 
-    absorb(x.sig_abs, d, true);
+    absorb(sig_abs, d, true);
 
     // Move (the face axis is set exactly below).
     {
