@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define STORM_GPU_ABI_VERSION 2
+#define STORM_GPU_ABI_VERSION 3
 
 typedef struct storm_gpu_engine storm_gpu_engine;
 
@@ -186,6 +186,12 @@ int storm_gpu_debug_tallies(storm_gpu_engine *eng, uint64_t *dep, uint64_t *rdep
 /* Per-cell random-walk / DDMC data of the last step (bytes written to
  * `out`, which may be NULL to query the size). */
 int storm_gpu_debug_accel(storm_gpu_engine *eng, void *out, uint64_t *bytes);
+/* Multigroup data of the last step in the engine's tracking type (float for
+ * STORM_GPU_MIXED): group count, energy exponent, bounds in keV (groups + 1),
+ * per-cell emission CDFs (cells * (groups + 1)) and scattering opacities.
+ * Pointers may be NULL; nothing is written but *groups when gray. */
+int storm_gpu_debug_spectrum(storm_gpu_engine *eng, int32_t *groups, double *nu_pow, void *bounds_kev, void *cdf,
+                             void *sig_s);
 
 #ifdef __cplusplus
 }

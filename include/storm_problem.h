@@ -26,6 +26,13 @@
  *   storm_imc_params     RadiationIMCParameters (radiation/RadiationIMCParameters.hpp:12-56)
  *                        plus CombPopulationControl(grid, Nmin, factor)
  *                        (population/CombPopulationControl.hpp:75-77).
+ *   storm_group_desc     multigroup opacity (withMultigroupOpacity, energyBoundaries):
+ *                        CalcAbsorptionOpacity(cell, frequency),
+ *                        CalcPlanckOpacity and GetThermalEnergy of the
+ *                        opacity-weighted Planck family of
+ *                        examples/densmore2012/DensmoreOpacity.hpp:38-110, and
+ *                        the Planck group sampling of multigroup boundary
+ *                        sources (boundary/SideTemperature.hpp:55-80, 150-153).
  */
 #ifndef STORM_PROBLEM_H
 #define STORM_PROBLEM_H
@@ -93,6 +100,30 @@ typedef struct storm_imc_params {
   double comb_factor;                 /* totalParticlesFactor (default 2.0) */
 } storm_imc_params;
 
+/*
+ * Multigroup (num_groups >= 2; 0 or 1 = gray).  Packets carry a photon
+ * energy E (erg) within [bounds[0], bounds[G]].
+ *   sigma_a(c, T, E) = sigma_a(c, T) * (max(E, bounds[0]) / nu_ref)^(-nu_pow)
+ *                      (sigma_a(c, T) of storm_material_desc; scattering is gray)
+ *   Planck opacity   = sum_g sigma_a(c, T, Ec_g) B_g / sum_g B_g, Ec_g the group
+ *                      centre, B_g the Planck fraction of group g at
+ *                      kT = k max(T, opacity_T_floor)   (1e-20 if sum_g B_g = 0)
+ *   emission / effective-scatter energy: group g with probability
+ *                      proportional to sigma_a(c, T, Ec_g) B_g, E uniform in the
+ *                      group (uniform over the whole range if the sum is 0)
+ *   boundary source energy: group g with probability B_g(T_source), E
+ *                      uniform in the group.
+ * Random walk and DDMC are gray-only here (the partial-gray random walk of
+ * RadiationIMC.hpp:1008-1061 is not implemented).
+ */
+typedef struct storm_group_desc {
+  int32_t num_groups;
+  int32_t reserved;
+  const double *bounds; /* num_groups + 1 ascending photon energies, erg */
+  double nu_pow;
+  double nu_ref;        /* erg */
+} storm_group_desc;
+
 typedef struct storm_problem {
   storm_mesh_desc mesh;
   storm_material_desc material;
@@ -100,6 +131,7 @@ typedef struct storm_problem {
   storm_imc_params imc;
   const double *T0; /* initial material temperature per cell, K */
   uint64_t seed;
+  storm_group_desc groups;
 } storm_problem;
 
 #ifdef __cplusplus

@@ -45,6 +45,7 @@
 
 #include "../include/storm_problem.h"
 #include "../storm_amd/csrc/common/constants.h"
+#include "../storm_amd/csrc/common/planck.h"
 
 namespace storm_oracle {
 
@@ -457,7 +458,70 @@ struct Material {
     if (p == 0.0 || rho.empty()) return 1.0;
     return std::pow(rho[c], p);
   }
-  double CalcPlanckOpacity(size_t c, double T) const { return sa0[c] * tfac(T, apow) * rfac(c, arho); }
+  // gray absorption opacity sigma_a(c, T)
+  double grayOpacity(size_t c, double T) const { return sa0[c] * tfac(T, apow) * rfac(c, arho); }
+
+  // Multigroup (storm_group_desc): group boundaries in erg, sigma_a(c, T, E)
+  // = grayOpacity * (max(E, E_0) / nu_ref)^(-nu_pow)
+  // (DensmoreOpacity::CalcAbsorptionOpacity, examples/densmore2012/DensmoreOpacity.hpp:56-64).
+  std::vector<double> bounds;
+  double nu_pow = 0.0, nu_ref = 1.0;
+  size_t groups() const { return bounds.size() > 2 ? bounds.size() - 1 : 1; }
+  bool multigroup() const { return groups() > 1; }
+  double CalcAbsorptionOpacity(size_t c, double T, double E) const {
+    return grayOpacity(c, T) * std::pow(std::max(E, bounds[0]) / nu_ref, -nu_pow);
+  }
+  // Cumulative sigma_a(Ec_g) B_g over groups (computeCumulativePlanck,
+  // DensmoreOpacity.hpp:128-148) and the Planck weights B_g.
+  void cumulativePlanck(size_t c, double T, std::vector<double> &cdf, std::vector<double> &Bg) const {
+    const size_t G = groups();
+    cdf.assign(G + 1, 0.0);
+    Bg.assign(G, 0.0);
+    const double kT = units::k_boltz * std::max(T, Tfloor);
+    for (size_t g = 0; g < G; ++g) {
+      const double a = bounds[g] / kT, b = bounds[g + 1] / kT;
+      const double bg = (a >= 0.0 && b > a && std::isfinite(b)) ? storm::planck::integral(a, b) : 0.0;
+      const double sg = CalcAbsorptionOpacity(c, T, 0.5 * (bounds[g] + bounds[g + 1]));
+      Bg[g] = bg;
+      const double w = (sg > 0.0 && std::isfinite(sg) && std::isfinite(bg)) ? sg * bg : 0.0;
+      cdf[g + 1] = cdf[g] + w;
+    }
+  }
+  // CalcPlanckOpacity (DensmoreOpacity.hpp:38-55 for multigroup; gray: sigma_a(c, T))
+  double CalcPlanckOpacity(size_t c, double T) const {
+    if (!multigroup()) return grayOpacity(c, T);
+    std::vector<double> cdf, Bg;
+    cumulativePlanck(c, T, cdf, Bg);
+    double weighted = cdf.back(), total = 0.0;
+    for (double b : Bg) total += b;
+    return total > 0.0 ? weighted / total : 1e-20;
+  }
+  // GetThermalEnergy (DensmoreOpacity.hpp:66-77, base class
+  // radiation/RadiationOpacityModel.hpp:88-92 when the weights vanish)
+  double GetThermalEnergy(size_t c, double T, double random) const {
+    std::vector<double> cdf, Bg;
+    cumulativePlanck(c, T, cdf, Bg);
+    return thermalEnergyFromCumulative(cdf, random);
+  }
+  // the same from a cumulative computed once per step (the cell temperature
+  // is constant during transport)
+  double thermalEnergyFromCumulative(const std::vector<double> &cdf, double random) const {
+    const size_t G = groups();
+    const double total = cdf[G];
+    if (!(total > 0.0) || !std::isfinite(total)) return bounds[0] + random * (bounds[G] - bounds[0]);
+    const double r = std::clamp(random, 0.0, std::nextafter(1.0, 0.0));
+    return linearInterpolation(cdf, bounds, r * total);
+  }
+  // LinearInterpolation (utils/LinearInterpolation.hpp:17-42)
+  static double linearInterpolation(const std::vector<double> &x, const std::vector<double> &y, double xi) {
+    auto it = std::upper_bound(x.begin(), x.end(), xi);
+    if (it == x.end()) return y.back();
+    if (it == x.begin()) return y.front();
+    const size_t i = static_cast<size_t>(it - x.begin());
+    if (std::abs(*it - xi) < 1e-12 * std::max(1.0, std::abs(xi))) return y[i];
+    return y[i] + (xi - *it) * (y[i - 1] - y[i]) / (*(it - 1) - *it);
+  }
+
   double CalcScatteringOpacity(size_t c, double T) const {
     if (ss0.empty()) return 0.0;
     return ss0[c] * tfac(T, spow) * rfac(c, srho);
@@ -495,6 +559,8 @@ class FaceTableBoundary {
 
   // SetTemperature (boundary/SideTemperature.hpp:183); ignored once T^4 is frozen
   void setSourceTemperature(double T) { desc_.source_temperature = T; }
+  // multigroup source: group boundaries in erg
+  void setGroupBounds(const std::vector<double> &b) { bounds_ = b; }
 
   Status apply(Particle &p) {
     Status status = DONE;
@@ -529,6 +595,13 @@ class FaceTableBoundary {
     }
     size_t N = grid_.GetPointNo();
     size_t Npc = desc_.source_packets_per_face;
+    std::vector<double> Bcdf;
+    if (!bounds_.empty()) {
+      const size_t G = bounds_.size() - 1;
+      const double kT = units::k_boltz * T;
+      Bcdf.assign(G + 1, 0.0);
+      for (size_t g = 0; g < G; ++g) Bcdf[g + 1] = Bcdf[g] + storm::planck::integral(bounds_[g] / kT, bounds_[g + 1] / kT);
+    }
     for (size_t i = 0; i < N; i++) {
       const Vec3 point = grid_.GetMeshPoint(i);
       for (size_t faceIdx : grid_.GetCellFaces(i)) {
@@ -549,6 +622,16 @@ class FaceTableBoundary {
               np.velocity.z = _1mmu * std::sin(theta);
               np.velocity *= units::clight;
               np.frequency = 0;
+              if (!bounds_.empty()) {
+                // DensmoreBoundary::generateNewBoundaryParticles
+                // (examples/densmore2012/DensmoreBoundary.hpp:96-130): group by
+                // the Planck fractions at T, energy uniform in the group
+                const size_t G = bounds_.size() - 1;
+                double r = re_.unit() * Bcdf.back();
+                size_t g = 0;
+                while (g + 1 < G && Bcdf[g + 1] < r) ++g;
+                np.frequency = bounds_[g] + re_.unit() * (bounds_[g + 1] - bounds_[g]);
+              }
               np.weight = energyToProduce;
               np.initialWeight = np.weight;
               np.timeLeft = fullDt * re_.unit();
@@ -598,6 +681,7 @@ class FaceTableBoundary {
 
   const CartesianGrid &grid_;
   storm_boundary_desc desc_;
+  std::vector<double> bounds_;
   Rng re_, faceRe_;
   bool frozen_ = false;
   double frozenT4_ = 0.0;
@@ -874,6 +958,8 @@ class RadiationIMC {
       : grid_(grid), boundary_(boundary), cells_(cells), ext_(ext), mat_(mat), params_(params),
         rng_(seed), sampler_(grid) {
     if (params_.newPhotonsPerCell == 0) throw OracleError("RadiationIMC requires newPhotonsPerCell > 0");
+    if (mat_.multigroup() && (params_.withRandomWalk || params_.withDDMC))
+      throw OracleError("random walk and DDMC are gray-only (no partial-gray random walk)");
     if (params_.withRandomWalk) rw_ = std::make_unique<RandomWalk>();
     Erad_time_avg_.assign(grid.GetPointNo(), 0.0);
     updateGridData();
@@ -922,6 +1008,11 @@ class RadiationIMC {
       factorFleck_[i] = 1.0 / (1.0 + (4.0 * units::arad * (T * T * T) * planck_[i] * units::clight * fullDt) / cv);
       if (factorFleck_[i] < 0.0 || factorFleck_[i] > 1.0) throw OracleError("Invalid factor fleck in RadiationIMC::preStep");
     }
+    if (mat_.multigroup()) {
+      thermalCdf_.assign(N, {});
+      std::vector<double> Bg;
+      for (size_t i = 0; i < N; ++i) mat_.cumulativePlanck(i, cells_[i].temperature, thermalCdf_[i], Bg);
+    }
     if (params_.withRandomWalk || params_.withDDMC) updateGridData();
     if (params_.withRandomWalk) {
       precomputeRandomWalkData();
@@ -968,6 +1059,7 @@ class RadiationIMC {
       for (size_t j = 0; j < n; ++j) {
         Particle p = generateSingleParticle(i);
         p.timeLeft = fullDt * rng_.unitOpen();
+        if (mat_.multigroup()) p.frequency = mat_.thermalEnergyFromCumulative(thermalCdf_[i], rng_.unitOpen());
         p.weight = perPhoton;
         p.initialWeight = std::abs(p.weight);
         out.push_back(p);
@@ -1042,6 +1134,11 @@ class RadiationIMC {
       if (tryDDMCStep(p, fn)) return fn;
     }
     double sa = planck_[c];
+    if (mat_.multigroup()) {
+      // clampFrequencyToBounds (RadiationIMC.hpp:868-872), CalcAbsorptionOpacity (:1753)
+      double E = std::clamp(p.frequency, mat_.bounds.front(), mat_.bounds.back());
+      sa = mat_.CalcAbsorptionOpacity(c, cells_[c].temperature, E);
+    }
     double ss = mat_.CalcScatteringOpacity(c, cells_[c].temperature);
     double eff = (1.0 - factorFleck_[c]) * sa;
     double eventOp = ss + eff;
@@ -1075,8 +1172,14 @@ class RadiationIMC {
       fn.nextCellIndex = nextC;
     } else if (ev == 1) {
       double eventRandom = rng_.unitOpen() * eventOp;
-      (void)eventRandom;  // both branches redirect isotropically (RadiationIMC.hpp:1834-1842)
+      // both branches redirect isotropically (RadiationIMC.hpp:1834-1842); an
+      // effective scatter re-emits at a thermal energy (:1843-1852)
+      const bool effective = !(eventRandom < ss);
       p.velocity = isotropicVelocity(rng_);
+      if (mat_.multigroup()) {
+        p.frequency = std::clamp(p.frequency, mat_.bounds.front(), mat_.bounds.back());
+        if (effective) p.frequency = mat_.thermalEnergyFromCumulative(thermalCdf_[c], rng_.unitOpen());
+      }
       fn.change = NO_CELL_MOVE;
     } else {
       fn.change = DONE;
@@ -1354,6 +1457,7 @@ class RadiationIMC {
   std::vector<char> rwEligible_;
   std::vector<double> rwSigmaT_;
   std::vector<DDMCCellData> ddmc_;
+  std::vector<std::vector<double>> thermalCdf_;
 };
 
 // ------------------------------------------------------------------ manager

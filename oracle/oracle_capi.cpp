@@ -18,8 +18,13 @@ CartesianGrid makeGrid(const storm_mesh_desc &m) {
   return CartesianGrid(std::move(xe), std::move(ye), std::move(ze));
 }
 
-Material makeMaterial(const storm_material_desc &d, size_t N) {
+Material makeMaterial(const storm_material_desc &d, size_t N, const storm_group_desc *g = nullptr) {
   Material m;
+  if (g && g->num_groups >= 2) {
+    m.bounds.assign(g->bounds, g->bounds + g->num_groups + 1);
+    m.nu_pow = g->nu_pow;
+    m.nu_ref = g->nu_ref;
+  }
   m.sa0.assign(d.sigma_a0, d.sigma_a0 + N);
   if (d.sigma_s0) m.ss0.assign(d.sigma_s0, d.sigma_s0 + N);
   if (d.density) m.rho.assign(d.density, d.density + N);
@@ -57,7 +62,7 @@ int oracle_run(const storm_problem *prob, int nsteps, double dt, const double *s
   try {
     CartesianGrid grid = makeGrid(prob->mesh);
     size_t N = grid.GetPointNo();
-    Material mat = makeMaterial(prob->material, N);
+    Material mat = makeMaterial(prob->material, N, &prob->groups);
     std::vector<Cell> cells(N);
     std::vector<Extensives> ext(N);
     for (size_t i = 0; i < N; ++i) {
@@ -69,6 +74,7 @@ int oracle_run(const storm_problem *prob, int nsteps, double dt, const double *s
     }
     storm_boundary_desc bd = prob->boundary;
     FaceTableBoundary bc(grid, bd, prob->seed);
+    if (mat.multigroup()) bc.setGroupBounds(mat.bounds);
     IMCParams ip;
     ip.newPhotonsPerCell = prob->imc.new_photons_per_cell;
     ip.withRandomWalk = prob->imc.with_random_walk != 0;
@@ -136,7 +142,7 @@ int oracle_prestep(const storm_problem *prob, double dt, double *sigma_out, doub
   try {
     CartesianGrid grid = makeGrid(prob->mesh);
     size_t N = grid.GetPointNo();
-    Material mat = makeMaterial(prob->material, N);
+    Material mat = makeMaterial(prob->material, N, &prob->groups);
     std::vector<Cell> cells(N);
     std::vector<Extensives> ext(N);
     for (size_t i = 0; i < N; ++i) {

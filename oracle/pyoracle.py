@@ -125,26 +125,30 @@ def rlib():
         L = C.CDLL(REPLAY_LIB)
         dp = C.POINTER(C.c_double)
         u64p = C.POINTER(C.c_uint64)
+        fp = C.POINTER(C.c_float)
         L.replay_last_error.restype = C.c_char_p
         L.replay_transport_f32.argtypes = [C.c_int64, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                            C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_float),
-                                           C.c_void_p, dp, C.POINTER(C.c_int64), C.c_int32, u64p, u64p, u64p,
-                                           u64p, C.POINTER(C.c_uint32), C.POINTER(C.c_int64)]
+                                           C.c_void_p, dp, C.POINTER(C.c_int64), C.c_int32, C.c_int32, C.c_double,
+                                           fp, fp, fp, u64p, u64p, u64p, u64p, C.POINTER(C.c_uint32),
+                                           C.POINTER(C.c_int64)]
         L.cpu_first_step.argtypes = [C.c_void_p, C.c_double, C.c_int32, C.c_uint32, dp, dp, dp, dp]
         L.replay_rw_tables.argtypes = [dp, dp, dp]
+        L.replay_planck.argtypes = [C.c_int64, dp, dp, dp]
         L.replay_rw_sample.argtypes = [C.c_int32, C.c_int64, dp, dp, dp, dp]
         _rlib = L
     return _rlib
 
 
-def replay_transport(spec, packets, n, xs, consts, segments_per_pass=1 << 20, accel=None):
+def replay_transport(spec, packets, n, xs, consts, segments_per_pass=1 << 20, accel=None, spectrum=None):
     """Replay `n` engine packets (ctypes array of PacketF32) on the CPU.
 
     Returns the integer tallies (dict of dep, rdep, pint, esc, segs, the
     engine's layout) and the outcome counts.
 
     `accel` is the engine's per-cell acceleration data (bytes) when the
-    problem has random walk or DDMC."""
+    problem has random walk or DDMC; `spectrum` the engine's multigroup data
+    (GpuEngine.spectrum_data()) when it is multigroup."""
     L = rlib()
     nx, ny, nz = spec.shape
     widths = np.concatenate([np.diff(spec.x_edges), np.diff(spec.y_edges), np.diff(spec.z_edges)]).astype(np.float32)
@@ -158,12 +162,18 @@ def replay_transport(spec, packets, n, xs, consts, segments_per_pass=1 << 20, ac
     xs32 = np.ascontiguousarray(xs, dtype=np.float32)
     accbuf = None if accel is None else C.create_string_buffer(bytes(accel), len(accel))
     accp = np.array([spec.with_rw, spec.with_ddmc, spec.rw_particle_tau, spec.ddmc_particle_tau], dtype=np.float64)
+    spec_arrays = [] if spectrum is None else [np.ascontiguousarray(spectrum[k], dtype=np.float32)
+                                                for k in ("bounds_kev", "cdf", "sig_s")]
+    spec_ptrs = [a.ctypes.data_as(C.POINTER(C.c_float)) for a in spec_arrays] or [None, None, None]
     rc = L.replay_transport_f32(n, C.cast(packets, C.c_void_p), nx, ny, nz,
                                 widths.ctypes.data_as(C.POINTER(C.c_float)), fk.ctypes.data_as(C.POINTER(C.c_int32)),
                                 xs32.ctypes.data_as(C.POINTER(C.c_float)),
                                 None if accbuf is None else C.cast(accbuf, C.c_void_p), _dp(accp),
                                 cst.ctypes.data_as(C.POINTER(C.c_int64)),
                                 int(segments_per_pass),
+                                0 if spectrum is None else int(spectrum["groups"]),
+                                0.0 if spectrum is None else float(spectrum["nu_pow"]),
+                                *spec_ptrs,
                                 u64(t["dep"]), u64(t["rdep"]), u64(t["pint"]), u64(t["esc"]),
                                 t["segs"].ctypes.data_as(C.POINTER(C.c_uint32)),
                                 outc.ctypes.data_as(C.POINTER(C.c_int64)))
@@ -199,3 +209,13 @@ def engine_rw_sample(precision, xi, tau):
     leak, rad = np.empty_like(xi), np.empty_like(xi)
     L.replay_rw_sample(precision, len(xi), _dp(xi), _dp(tau), _dp(leak), _dp(rad))
     return leak, rad
+
+
+def engine_planck(a, b):
+    """The engine's Planck integral 15/pi^4 int_a^b x^3/(e^x-1) dx (CPU build)."""
+    L = rlib()
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.empty_like(a)
+    L.replay_planck(len(a), _dp(a), _dp(b), _dp(out))
+    return out

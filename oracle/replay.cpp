@@ -16,6 +16,7 @@
 
 #include "../include/storm_problem.h"
 #include "../storm_amd/csrc/common/fixed.h"
+#include "../storm_amd/csrc/common/planck.h"
 #include "../storm_amd/csrc/common/random_walk.h"
 #include "../storm_amd/csrc/common/sourcing.h"
 #include "../storm_amd/csrc/common/transport.h"
@@ -177,6 +178,11 @@ void replay_fpmath(int32_t which, int64_t n, const float *x, float *y) {
 // (0 float, 1 double) on a batch.
 void replay_rw_tables(double *tau, double *survival, double *radius) { rw::build_tables(tau, survival, radius); }
 
+// The engine's normalised Planck integral over [a_i, b_i] (in units of kT).
+void replay_planck(int64_t n, const double *a, const double *b, double *out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = planck::integral(a[i], b[i]);
+}
+
 void replay_rw_sample(int32_t precision, int64_t n, const double *xi, const double *tau, double *leak,
                       double *radius) {
   auto go = [&](auto tag) {
@@ -196,12 +202,15 @@ void replay_rw_sample(int32_t precision, int64_t n, const double *xi, const doub
 // Exact replay of `n` packets (storm::Packet<float>) through the transport.
 // widths: nx + ny + nz floats; xs: 2 floats per cell; consts: seed, step, eq, et;
 // accel: N CellAccel<float> records or NULL; acc_params: rw_on, ddmc_on,
-// rw particle tau, ddmc particle tau.  Tallies are accumulated into the given
-// arrays (dep/rdep/pint/esc: 2 per cell, segs: 1 per cell); counts[0..3] outcomes,
-// counts[4] random-walk steps, counts[5] DDMC steps.
+// rw particle tau, ddmc particle tau.  Multigroup (groups >= 2): nu_pow,
+// group bounds in keV (groups + 1), the per-cell emission CDFs
+// (N * (groups + 1)) and scattering opacities (N).  Tallies are accumulated
+// into the given arrays (dep/rdep/pint/esc: 2 per cell, segs: 1 per cell);
+// counts[0..3] outcomes, counts[4] random-walk steps, counts[5] DDMC steps.
 int replay_transport_f32(int64_t n, const void *packets, int32_t nx, int32_t ny, int32_t nz, const float *widths,
                          const int32_t *face_kind, const float *xs, const void *accel, const double *acc_params,
-                         const int64_t *consts, int32_t K, uint64_t *dep, uint64_t *rdep, uint64_t *pint,
+                         const int64_t *consts, int32_t K, int32_t groups, double nu_pow, const float *bounds_kev,
+                         const float *cdf, const float *sig_s, uint64_t *dep, uint64_t *rdep, uint64_t *pint,
                          uint64_t *esc, uint32_t *segs, int64_t *counts) {
   try {
     MeshView<float> mv;
@@ -214,6 +223,16 @@ int replay_transport_f32(int64_t n, const void *packets, int32_t nx, int32_t ny,
     mv.width[0] = widths;
     mv.width[1] = widths + nx;
     mv.width[2] = widths + nx + ny;
+    mv.spec = SpectrumView<float>{};
+    mv.spec.groups = groups;
+    if (groups >= 2) {
+      mv.spec.nu_pow = static_cast<float>(nu_pow);
+      mv.spec.e_lo = bounds_kev[0];
+      mv.spec.e_hi = bounds_kev[groups];
+      mv.spec.bounds = bounds_kev;
+      mv.spec.cdf = cdf;
+      mv.spec.sig_s = sig_s;
+    }
     StepConsts kc;
     kc.seed = static_cast<uint64_t>(consts[0]);
     kc.step = static_cast<uint32_t>(consts[1]);
@@ -262,6 +281,9 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
           V[(static_cast<size_t>(i) * ny + j) * nz + k] =
               widthd(p->mesh.x_edges, i) * widthd(p->mesh.y_edges, j) * widthd(p->mesh.z_edges, k);
     const storm_material_desc &m = p->material;
+    const int32_t G = p->groups.num_groups >= 2 ? p->groups.num_groups : 0;
+    const double *gb = p->groups.bounds;
+    std::vector<double> cdfd(G ? static_cast<size_t>(N) * (G + 1) : 0), s1(N, 0.0);
     double Etot = 0.0;
     for (uint32_t c = 0; c < N; ++c) {
       double T = p->T0[c];
@@ -275,8 +297,31 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       double cv = m.eos_f * m.eos_beta * std::pow(std::fmax(T, m.eos_cv_T_floor), m.eos_beta - 1.0) *
                   std::pow(rho, 1.0 - m.eos_mu) / rho;
       double T3 = T * T * T;
-      f[c] = 1.0 / (1.0 + (4.0 * units::arad * T3 * sa[c] * units::clight * dt) / cv);
-      E[c] = f[c] * V[c] * units::arad * (T3 * T) * sa[c] * dt * units::clight;
+      double sP = sa[c];
+      if (G) {
+        // the engine's multigroup preStep (k_prestep_mg)
+        const double kT = units::k_boltz * std::fmax(T, m.opacity_T_floor);
+        double acc = 0.0, sumB = 0.0;
+        double *my = &cdfd[static_cast<size_t>(c) * (G + 1)];
+        my[0] = 0.0;
+        for (int32_t g = 0; g < G; ++g) {
+          const double Bg = planck::integral(gb[g] / kT, gb[g + 1] / kT);
+          const double Ec = 0.5 * (gb[g] + gb[g + 1]);
+          const double sg = sa[c] * std::pow(std::fmax(Ec, gb[0]) / p->groups.nu_ref, -p->groups.nu_pow);
+          const double wg = (sg > 0.0 && std::isfinite(sg) && std::isfinite(Bg)) ? sg * Bg : 0.0;
+          sumB += Bg;
+          acc += wg;
+          my[g + 1] = acc;
+        }
+        const bool valid = acc > 0.0 && std::isfinite(acc);
+        for (int32_t g = 1; g <= G; ++g) my[g] = valid ? my[g] / acc : (gb[g] - gb[0]) / (gb[G] - gb[0]);
+        my[G] = 1.0;
+        sP = sumB > 0.0 ? acc / sumB : 1e-20;
+        s1[c] = sa[c] * std::pow(units::kev / p->groups.nu_ref, -p->groups.nu_pow);
+      }
+      f[c] = 1.0 / (1.0 + (4.0 * units::arad * T3 * sP * units::clight * dt) / cv);
+      E[c] = f[c] * V[c] * units::arad * (T3 * T) * sP * dt * units::clight;
+      if (!G) s1[c] = sa[c];
       Etot += E[c];
     }
     const uint32_t npc =
@@ -312,7 +357,7 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
     }
     std::vector<uint64_t> dep(2 * N, 0), rdep(2 * N, 0), pint(2 * N, 0), esc(2 * N, 0);
     std::vector<float> sabs32(N);
-    for (uint32_t c = 0; c < N; ++c) sabs32[c] = static_cast<float>(f[c] * sa[c]);
+    for (uint32_t c = 0; c < N; ++c) sabs32[c] = G ? 0.0f : static_cast<float>(f[c] * sa[c]);
     std::vector<double> depf(N, 0), pintf(N, 0), escf(N, 0);
     std::vector<uint32_t> segs(N, 0);
     int64_t outc[4] = {0, 0, 0, 0};
@@ -338,8 +383,25 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       mv.width[2] = w2.data();
       std::vector<CellXS<Real>> xs(N);
       for (uint32_t c = 0; c < N; ++c) {
-        xs[c].sig_abs = static_cast<Real>(f[c] * sa[c]);
-        xs[c].sig_ev = static_cast<Real>(ss[c] + (1.0 - f[c]) * sa[c]);
+        xs[c].sig_abs = static_cast<Real>(f[c] * s1[c]);
+        xs[c].sig_ev = G ? static_cast<Real>((1.0 - f[c]) * s1[c]) : static_cast<Real>(ss[c] + (1.0 - f[c]) * sa[c]);
+      }
+      std::vector<Real> cdfr(cdfd.begin(), cdfd.end()), ssr(ss.begin(), ss.end()), bk(G ? G + 1 : 0), bcdf;
+      for (int32_t g = 0; G && g <= G; ++g) bk[g] = static_cast<Real>(gb[g] / units::kev);
+      mv.spec = SpectrumView<Real>{};
+      mv.spec.groups = G;
+      if (G) {
+        mv.spec.nu_pow = static_cast<Real>(p->groups.nu_pow);
+        mv.spec.e_lo = bk[0];
+        mv.spec.e_hi = bk[G];
+        mv.spec.bounds = bk.data();
+        mv.spec.cdf = cdfr.data();
+        mv.spec.sig_s = ssr.data();
+        const double kT = units::k_boltz * Tb;
+        std::vector<double> acc(G + 1, 0.0);
+        for (int32_t g = 0; g < G; ++g) acc[g + 1] = acc[g] + planck::integral(gb[g] / kT, gb[g + 1] / kT);
+        bcdf.resize(G + 1);
+        for (int32_t g = 0; g <= G; ++g) bcdf[g] = acc[G] > 0.0 ? static_cast<Real>(acc[g] / acc[G]) : Real(0);
       }
       std::vector<CellAccel<Real>> acc_cells;
       std::unique_ptr<HostTables<Real>> tb;
@@ -364,7 +426,7 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
         uint32_t k = c % nz, j = (c / nz) % ny, i = c / (static_cast<uint32_t>(ny) * nz);
         for (uint32_t q = 0; q < nph[c]; ++q) {
           Packet<Real> pp;
-          birth_emission<Real>(pp, id++, pack_ijk(i, j, k), wem[c], s0, p->seed, step);
+          birth_emission<Real>(pp, id++, pack_ijk(i, j, k), wem[c], s0, p->seed, step, mv.spec, c);
           pk.push_back(pp);
         }
       }
@@ -372,7 +434,7 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
         for (uint32_t q = 0; q < npc; ++q) {
           Packet<Real> pp;
           birth_boundary<Real>(pp, id++, static_cast<uint32_t>(fi / nz), static_cast<uint32_t>(fi % nz), wface[fi],
-                               s0, p->seed, step);
+                               s0, p->seed, step, mv.spec, G ? bcdf.data() : nullptr);
           pk.push_back(pp);
         }
       }

@@ -51,9 +51,14 @@ class ImcParams(C.Structure):
                 ("comb_nmin", C.c_uint64), ("comb_factor", C.c_double)]
 
 
+class GroupDesc(C.Structure):
+    _fields_ = [("num_groups", C.c_int32), ("reserved", C.c_int32), ("bounds", _dp),
+                ("nu_pow", C.c_double), ("nu_ref", C.c_double)]
+
+
 class Problem(C.Structure):
     _fields_ = [("mesh", MeshDesc), ("material", MaterialDesc), ("boundary", BoundaryDesc),
-                ("imc", ImcParams), ("T0", _dp), ("seed", C.c_uint64)]
+                ("imc", ImcParams), ("T0", _dp), ("seed", C.c_uint64), ("groups", GroupDesc)]
 
 
 class PacketRecord(C.Structure):
@@ -94,7 +99,7 @@ class PacketF32(C.Structure):
     """storm::Packet<float> (storm_amd/csrc/common/transport.h)."""
     _fields_ = [("u", C.c_float * 3), ("s", C.c_float), ("o", C.c_float * 3), ("wmin", C.c_float),
                 ("w", C.c_double), ("ijk", C.c_uint32), ("id", C.c_uint32), ("ctr", C.c_uint32),
-                ("flags", C.c_uint32)]
+                ("flags", C.c_uint32), ("nu", C.c_float), ("pad", C.c_uint32)]
 
 
 _LIB = None
@@ -133,6 +138,7 @@ def load_library(path: str | None = None):
         "storm_gpu_debug_tallies": ([vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)], C.c_int),
         "storm_gpu_debug_accel": ([vp, vp, C.POINTER(C.c_uint64)], C.c_int),
+        "storm_gpu_debug_spectrum": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_double), vp, vp, vp], C.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(lib, name)

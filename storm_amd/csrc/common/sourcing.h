@@ -11,6 +11,10 @@
 //              (boundary/SideTemperature.hpp:114-163): a uniform point on the
 //              x- box face of the cell, mu = sqrt(U) about +x, phi = 2 pi U,
 //              timeLeft = dt * U, weight sigma_SB T^4 A dt / Npercell.
+// Multigroup: an emitted packet takes its energy from the cell's emission
+// CDF (GetThermalEnergy, RadiationIMC.hpp:2496-2500), a boundary packet from
+// the Planck fractions of the source temperature with the energy uniform in
+// the group (examples/densmore2012/DensmoreBoundary.hpp:113-125).
 // Each packet draws from its own stream keyed by (seed, id) with the birth tag.
 #pragma once
 
@@ -23,7 +27,7 @@ namespace storm {
 
 template <typename Real>
 STORM_HD void birth_emission(Packet<Real> &p, uint32_t id, uint32_t ijk, double w, Real s0, uint64_t seed,
-                             uint32_t step) {
+                             uint32_t step, const SpectrumView<Real> &sp, uint32_t cell) {
   using M = MathPolicy<Real>;
   PacketStream rs = PacketStream::make(seed, id, step, kTagBirth);
   Philox4 r0 = rs.block(0), r1 = rs.block(1);
@@ -38,11 +42,19 @@ STORM_HD void birth_emission(Packet<Real> &p, uint32_t id, uint32_t ijk, double 
   p.id = id;
   p.ctr = 0u;
   p.flags = 0u;
+  p.nu = Real(0);
+  if (sp.groups >= 2) {
+    const int32_t G = sp.groups;
+    p.nu = spectrum_quantile<Real>(sp.cdf + static_cast<size_t>(cell) * static_cast<size_t>(G + 1), sp.bounds, G,
+                                   M::uniform(r1.v[2]));
+  }
 }
 
+// `bcdf`: normalised Planck fractions of the source temperature over the
+// groups (groups + 1 entries), or null when gray.
 template <typename Real>
 STORM_HD void birth_boundary(Packet<Real> &p, uint32_t id, uint32_t j, uint32_t k, double w, Real s0,
-                             uint64_t seed, uint32_t step) {
+                             uint64_t seed, uint32_t step, const SpectrumView<Real> &spec, const Real *bcdf) {
   using M = MathPolicy<Real>;
   PacketStream rs = PacketStream::make(seed, id, step, kTagBirth);
   Philox4 r0 = rs.block(0), r1 = rs.block(1);
@@ -64,6 +76,14 @@ STORM_HD void birth_boundary(Packet<Real> &p, uint32_t id, uint32_t j, uint32_t 
   p.id = id;
   p.ctr = 0u;
   p.flags = 0u;
+  p.nu = Real(0);
+  if (spec.groups >= 2 && bcdf) {
+    const int32_t G = spec.groups;
+    const Real r = M::uniform(r1.v[1]);
+    int32_t g = 0;
+    while (g + 1 < G && bcdf[g + 1] < r) ++g;
+    p.nu = spec.bounds[g] + M::uniform(r1.v[2]) * (spec.bounds[g + 1] - spec.bounds[g]);
+  }
 }
 
 }  // namespace storm

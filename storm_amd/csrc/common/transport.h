@@ -42,13 +42,50 @@ STORM_HD uint32_t ijk_i(uint32_t p) { return p >> 20; }
 STORM_HD uint32_t ijk_j(uint32_t p) { return (p >> 10) & 0x3ffu; }
 STORM_HD uint32_t ijk_k(uint32_t p) { return p & 0x3ffu; }
 
-// Mesh as seen by the transport: per-axis widths (cm) and the fate of a
-// packet at each box face, order x-, x+, y-, y+, z-, z+.
+// Multigroup spectrum as seen by the transport (groups < 2: gray).  Photon
+// energies are in keV.  A packet of energy E in cell c sees
+//   sigma_a(c, E) = sigma_a(c, 1 keV) * max(E, e_lo)^(-nu_pow)
+// (the cell table then holds f sigma_a(c, 1 keV) and (1-f) sigma_a(c, 1 keV))
+// and re-emits after an effective scatter at an energy drawn from the cell's
+// emission CDF: cdf[c*(groups+1) + g] over the group bounds, linear within a
+// group (DensmoreOpacity::GetThermalEnergy, examples/densmore2012/DensmoreOpacity.hpp:66-77).
+template <typename Real>
+struct SpectrumView {
+  int32_t groups;
+  Real nu_pow;
+  Real e_lo, e_hi;     // bounds[0], bounds[groups]
+  const Real *bounds;  // groups + 1
+  const Real *cdf;     // per cell, groups + 1, cdf[0] = 0 and cdf[groups] = 1
+  const Real *sig_s;   // per cell scattering opacity (gray)
+};
+
+// Energy (keV) at the quantile `u` of a normalised group CDF, linear within
+// the group (LinearInterpolation, utils/LinearInterpolation.hpp:17-42, of
+// the CDF onto the bounds).
+template <typename Real>
+STORM_HD Real spectrum_quantile(const Real *cdf, const Real *bounds, int32_t groups, Real u) {
+  // smallest i in [1, groups] with cdf[i] > u
+  int32_t lo = 0, hi = groups;
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (cdf[mid] > u) hi = mid;
+    else lo = mid;
+  }
+  const Real c0 = cdf[hi - 1], c1 = cdf[hi];
+  const Real b0 = bounds[hi - 1], b1 = bounds[hi];
+  const Real t = c1 > c0 ? (u - c0) / (c1 - c0) : Real(0.5);
+  const Real e = b0 + t * (b1 - b0);
+  return e < b0 ? b0 : (e > b1 ? b1 : e);
+}
+
+// Mesh as seen by the transport: per-axis widths (cm), the fate of a packet
+// at each box face (order x-, x+, y-, y+, z-, z+) and the spectrum.
 template <typename Real>
 struct MeshView {
   int32_t nx, ny, nz;
   uint32_t escape_mask;  // bit f set: box face f removes the packet (else it reflects)
   const Real *width[3];  // width[a][idx] for a = x, y, z
+  SpectrumView<Real> spec;
 };
 
 template <typename Real>
@@ -101,6 +138,7 @@ struct Packet {
   uint32_t id;    // packet id within the step (keys the random stream)
   uint32_t ctr;   // Philox blocks consumed by transport so far
   uint32_t flags;
+  Real nu;        // photon energy, keV (multigroup; 0 when gray)
 };
 
 // Scalars shared by every packet of a step.
@@ -153,12 +191,14 @@ struct Tracker {
   static constexpr bool fixed_point = sizeof(Real) == 4;
 
   PacketStream rs;
-  Real u0, u1, u2, o0, o1, o2, s, wmin;
-  // the current cell's data, loaded when the packet enters it
-  Real cell_sig_abs, cell_sig_ev, cw0, cw1, cw2;
+  Real u0, u1, u2, o0, o1, o2, s, wmin, nu;
+  // the current cell's data, loaded when the packet enters it (and, in
+  // multigroup, when its energy changes)
+  Real cell_sig_abs, cell_sig_ev, cell_sig_s, cw0, cw1, cw2;
   double w, w_enter, path_int;  // w_enter: weight when the current cell visit began
   uint32_t ijk, ctr, cell, visit_segs, id;
   int outcome;
+  bool derive_pint;
 
   STORM_HD void begin(const Packet<Real> &p, const MeshView<Real> &m, const CellXS<Real> *xs, const StepConsts &k) {
     rs = PacketStream::make(k.seed, p.id, k.step, kTagTransport);
@@ -174,6 +214,7 @@ struct Tracker {
     ctr = p.ctr;
     wmin = p.wmin;
     id = p.id;
+    nu = p.nu;
     load_cell(m, xs);
     w_enter = w;
     path_int = 0.0;  // sum of w * (1 - e^{-x}) / sig_abs over the visit, erg*cm
@@ -202,6 +243,13 @@ struct Tracker {
     p.ijk = ijk;
     p.ctr = ctr;
     p.id = id;
+    p.nu = nu;
+  }
+
+  // The mixed engine tallies the path integral only where postStep cannot
+  // derive it from the deposit (gray cells without absorption).
+  STORM_HD bool derives_path_integral(const MeshView<Real> &m) const {
+    return fixed_point && m.spec.groups < 2 && cell_sig_abs > Real(0);
   }
 
   template <typename Sink>
@@ -209,7 +257,7 @@ struct Tracker {
     if constexpr (fixed_point) {
       fx::Limbs d = fx::sub(fx::limbs_of(w_enter, k.eq), fx::limbs_of(w_exit, k.eq));
       fx::Limbs e{0u, 0u};
-      if (!(cell_sig_abs > Real(0))) e = fx::limbs_of(fx::quantize_nearest(path_int, k.et), k.et);
+      if (!derive_pint) e = fx::limbs_of(fx::quantize_nearest(path_int, k.et), k.et);
       sink.visit(cell, d, e, visit_segs);
     } else {
       sink.visit_fp(cell, w_enter - w_exit, path_int, visit_segs);
@@ -237,12 +285,27 @@ struct Tracker {
 
   STORM_HD void load_cell(const MeshView<Real> &m, const CellXS<Real> *xs) {
     cell = flat_cell(m, ijk);
-    const CellXS<Real> x = xs[cell];
-    cell_sig_abs = x.sig_abs;
-    cell_sig_ev = x.sig_ev;
     cw0 = m.width[0][ijk_i(ijk)];
     cw1 = m.width[1][ijk_j(ijk)];
     cw2 = m.width[2][ijk_k(ijk)];
+    load_opacity(m, xs);
+  }
+
+  // opacities of the current cell at the packet's energy
+  STORM_HD void load_opacity(const MeshView<Real> &m, const CellXS<Real> *xs) {
+    const CellXS<Real> x = xs[cell];
+    cell_sig_abs = x.sig_abs;
+    cell_sig_ev = x.sig_ev;
+    cell_sig_s = Real(0);
+    if (m.spec.groups >= 2) {
+      // clampFrequencyToBounds (RadiationIMC.hpp:868-872) and E^-nu_pow
+      Real e = nu < m.spec.e_lo ? m.spec.e_lo : (nu > m.spec.e_hi ? m.spec.e_hi : nu);
+      const Real fac = M::exp(-m.spec.nu_pow * M::log(e));
+      cell_sig_s = m.spec.sig_s[cell];
+      cell_sig_abs = x.sig_abs * fac;
+      cell_sig_ev = cell_sig_s + x.sig_ev * fac;
+    }
+    derive_pint = derives_path_integral(m);
   }
 
   // continuous absorption over a path d (cm); `tally_free` adds w * d to the
@@ -253,9 +316,9 @@ struct Tracker {
     Real frac = -M::expm1(-xabs);
     double w0 = w;
     w = w0 - w0 * static_cast<double>(frac);
-    // the mixed engine derives the path integral of absorbing cells from
-    // their deposit (see flush)
-    if (fixed_point && sig_abs > Real(0)) return;
+    // the mixed engine derives the path integral of absorbing gray cells
+    // from their deposit (see flush)
+    if (derive_pint) return;
     if (xabs >= Real(1e-12)) {
       path_int += w0 * static_cast<double>(frac / sig_abs);
     } else if (tally_free) {
@@ -532,12 +595,25 @@ struct Tracker {
         }
       }
     } else if (ev == 1) {
-      // effective scatter: isotropic redirection (RadiationIMC.hpp:1827-1874)
+      // scatter: isotropic redirection (RadiationIMC.hpp:1827-1874)
       Real nd[3];
       isotropic_dir<Real>(M::uniform(r.v[2]), M::uniform(r.v[3]), nd);
       o0 = nd[0];
       o1 = nd[1];
       o2 = nd[2];
+      if (m.spec.groups >= 2) {
+        // an effective (absorption + re-emission) scatter takes a new energy
+        // from the cell's emission spectrum (:1843-1852); an elastic one
+        // (probability sigma_s / event opacity) keeps it
+        const bool elastic = M::uniform(r.v[1]) * cell_sig_ev < cell_sig_s;
+        if (!elastic) {
+          const Philox4 r2 = rs.block(ctr++);
+          const int32_t G = m.spec.groups;
+          nu = spectrum_quantile<Real>(m.spec.cdf + static_cast<size_t>(cell) * static_cast<size_t>(G + 1),
+                                       m.spec.bounds, G, M::uniform(r2.v[0]));
+          load_opacity(m, xs);
+        }
+      }
     } else {
       census(k, sink);
     }

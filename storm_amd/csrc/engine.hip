@@ -68,13 +68,16 @@ struct PacketBufs {
   DevBuf<typename V4<Real>::T> pos, dir;
   DevBuf<double> w;
   DevBuf<uint4> meta;
+  DevBuf<Real> nu;
+  bool with_nu = false;
   void ensure(size_t n) {
     pos.ensure(n);
     dir.ensure(n);
     w.ensure(n);
     meta.ensure(n);
+    if (with_nu) nu.ensure(n);
   }
-  PacketSoA<Real> soa() { return PacketSoA<Real>{pos.p, dir.p, w.p, meta.p}; }
+  PacketSoA<Real> soa() { return PacketSoA<Real>{pos.p, dir.p, w.p, meta.p, with_nu ? nu.p : nullptr}; }
   size_t cap() const { return meta.n; }
 };
 
@@ -114,6 +117,7 @@ struct EngineBase {
   virtual void debug_inputs(uint64_t *, void *, void *, int64_t *) = 0;
   virtual void debug_tallies(uint64_t *, uint64_t *, uint64_t *, uint64_t *, uint32_t *) = 0;
   virtual void debug_accel(void *, uint64_t *) = 0;
+  virtual void debug_spectrum(int32_t *, double *, void *, void *, void *) = 0;
   storm_gpu_counters counters{};
   double source_T = 0.0;
   int32_t block = 256;
@@ -159,6 +163,12 @@ struct Engine final : EngineBase {
   DevBuf<double> Dtil;
   DevBuf<Real> rw_tau, rw_surv, rw_rad;
   bool accel_on = false;
+  // multigroup (storm_group_desc)
+  int32_t groups = 0;
+  double nu_pow = 0.0, nu_ref = 1.0;
+  std::vector<double> bounds_h;  // erg
+  DevBuf<double> bounds_erg;
+  DevBuf<Real> bounds_kev, cdf, sig_s_cell, bcdf;
   DevBuf<unsigned char> temp;
   Tallies tal{};
 
@@ -189,6 +199,20 @@ struct Engine final : EngineBase {
     seed = p->seed;
     if (imc.new_photons_per_cell == 0) throw EngineError(1, "RadiationIMC requires newPhotonsPerCell > 0");
     accel_on = imc.with_random_walk || imc.with_ddmc;
+    if (p->groups.num_groups >= 2) {
+      groups = p->groups.num_groups;
+      if (groups > 4096) throw EngineError(1, "at most 4096 groups");
+      if (!p->groups.bounds) throw EngineError(1, "groups.bounds is required");
+      bounds_h.assign(p->groups.bounds, p->groups.bounds + groups + 1);
+      for (int32_t g = 0; g < groups; ++g)
+        if (!(bounds_h[g + 1] > bounds_h[g]) || !(bounds_h[0] > 0.0))
+          throw EngineError(1, "group bounds must be positive and strictly increasing");
+      nu_pow = p->groups.nu_pow;
+      nu_ref = p->groups.nu_ref;
+      if (!(nu_ref > 0.0)) throw EngineError(1, "groups.nu_ref must be positive");
+      if (accel_on)
+        throw EngineError(1, "random walk and DDMC are gray-only (the partial-gray random walk is not implemented)");
+    }
     // the FP64 engine is the plain form by default: one packet per thread
     schedule = precision == STORM_GPU_FP64 ? STORM_GPU_SCHEDULE_PASS : STORM_GPU_SCHEDULE_REFILL;
     if (bnd.source_face != -1 && bnd.source_face != STORM_XLO)
@@ -249,6 +273,17 @@ struct Engine final : EngineBase {
     scal.ensure(4);     // total census weight (low, high limb)
     dsc.ensure(2);      // material energy before the step, census energy after it
     counters_u32.ensure(4);
+    if (groups >= 2) {
+      up(bounds_erg, bounds_h.data(), bounds_h.size());
+      std::vector<Real> bk(bounds_h.size());
+      for (size_t g = 0; g < bk.size(); ++g) bk[g] = static_cast<Real>(bounds_h[g] / units::kev);
+      bounds_kev.ensure(bk.size());
+      HIPCHECK(hipMemcpy(bounds_kev.p, bk.data(), bk.size() * sizeof(Real), hipMemcpyHostToDevice));
+      cdf.ensure(static_cast<size_t>(N) * static_cast<size_t>(groups + 1));
+      sig_s_cell.ensure(N);
+      bcdf.ensure(static_cast<size_t>(groups + 1));
+      pa.with_nu = pb.with_nu = true;
+    }
     if (accel_on) {
       accel.ensure(N);
       Dtil.ensure(N);
@@ -362,7 +397,21 @@ struct Engine final : EngineBase {
     m.width[0] = wx.p;
     m.width[1] = wy.p;
     m.width[2] = wz.p;
+    m.spec = spectrum();
     return m;
+  }
+  SpectrumView<Real> spectrum() const {
+    SpectrumView<Real> s{};
+    s.groups = groups;
+    if (groups >= 2) {
+      s.nu_pow = static_cast<Real>(nu_pow);
+      s.e_lo = static_cast<Real>(bounds_h.front() / units::kev);
+      s.e_hi = static_cast<Real>(bounds_h.back() / units::kev);
+      s.bounds = bounds_kev.p;
+      s.cdf = cdf.p;
+      s.sig_s = sig_s_cell.p;
+    }
+    return s;
   }
 
   // Workgroups of the refill transport that fit on the device at once.
@@ -416,9 +465,16 @@ struct Engine final : EngineBase {
     sum_async(e_mat.p, N, dsc.p);  // material energy before the step, read by the ledger
 
     // preStep: opacities, Fleck factor, emission energies
-    k_prestep<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
-        N, T.p, md, have_host_sa ? host_sa.p : nullptr, have_host_ss ? host_ss.p : nullptr,
-        have_host_cv ? host_cv.p : nullptr, V.p, dt, xs.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p);
+    if (groups >= 2) {
+      k_prestep_mg<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
+          N, T.p, md, have_host_sa ? host_sa.p : nullptr, have_host_ss ? host_ss.p : nullptr,
+          have_host_cv ? host_cv.p : nullptr, V.p, dt, groups, bounds_erg.p, nu_pow, nu_ref, xs.p, cdf.p,
+          sig_s_cell.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p);
+    } else {
+      k_prestep<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
+          N, T.p, md, have_host_sa ? host_sa.p : nullptr, have_host_ss ? host_ss.p : nullptr,
+          have_host_cv ? host_cv.p : nullptr, V.p, dt, xs.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p);
+    }
     if (accel_on) {
       const MeshView<Real> mv0 = meshview();
       k_accel_cells<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
@@ -454,6 +510,18 @@ struct Engine final : EngineBase {
     const int32_t et = Ebound > 0.0 ? fx::ceil_log2(Ebound * units::clight * dt) - fx::kHeadroomBits : -1000;
     counters.energy_quantum_exp = eq;
     for (uint32_t f = 0; f < nfaces; ++f) wface_e[f] = fx::quantize_down(wface_e[f], eq);
+    if (groups >= 2 && nfaces) {
+      // Planck fractions of the source temperature per group
+      // (examples/densmore2012/DensmoreBoundary.hpp:96-104)
+      const double kT = units::k_boltz * source_T;
+      std::vector<double> acc(groups + 1, 0.0);
+      for (int32_t g = 0; g < groups; ++g)
+        acc[g + 1] = acc[g] + planck::integral(bounds_h[g] / kT, bounds_h[g + 1] / kT);
+      std::vector<Real> h(groups + 1);
+      for (int32_t g = 0; g <= groups; ++g) h[g] = acc[groups] > 0.0 ? static_cast<Real>(acc[g] / acc[groups]) : Real(0);
+      HIPCHECK(hipMemcpyAsync(bcdf.p, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+    }
     wface.ensure(std::max<uint32_t>(nfaces, 1));
     if (nfaces)
       HIPCHECK(hipMemcpyAsync(wface.p, wface_e.data(), nfaces * sizeof(double), hipMemcpyHostToDevice, stream));
@@ -506,11 +574,11 @@ struct Engine final : EngineBase {
     if (n_emit)
       k_emit<Real><<<blocks_for(n_emit, 256), 256, 0, stream>>>(
           static_cast<uint32_t>(n_emit), N, offsets.p, wemit.p, pa.soa(), static_cast<uint32_t>(n_census), s0, d,
-          seed, step_index, tal.begin);
+          seed, step_index, tal.begin, spectrum());
     if (n_bc)
       k_boundary<Real><<<blocks_for(n_bc, 256), 256, 0, stream>>>(
           static_cast<uint32_t>(n_bc), npc, wface.p, pa.soa(), static_cast<uint32_t>(n_census + n_emit), s0, d, seed,
-          step_index, tal.begin);
+          step_index, tal.begin, spectrum(), groups >= 2 ? bcdf.p : nullptr);
 
     StepConsts kc;
     kc.seed = seed;
@@ -575,7 +643,7 @@ struct Engine final : EngineBase {
     k_post<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
         N, tal, eq, et, eq_prev, emit_limbs.p, requant.p, dt, V.p, md,
         material_mode == STORM_GPU_MATERIAL_DEVICE ? 1 : 0, T.p, e_mat.p, e_sync.p, erad_avg.p, census_erad.p,
-        clamp.p, err.p, precision == STORM_GPU_FP64 ? 1 : 0, xs.p);
+        clamp.p, err.p, precision == STORM_GPU_FP64 ? 1 : 0, xs.p, groups >= 2 ? 0 : 1);
     // census energy for the next step's quantum, total segments, error flags
     // and counters: one synchronisation
     sum_async(census_erad.p, N, dsc.p + 1);
@@ -689,7 +757,9 @@ struct Engine final : EngineBase {
     std::vector<typename V4<Real>::T> hp(n), hd(n);
     std::vector<double> hw(n);
     std::vector<uint4> hm(n);
+    std::vector<Real> hn(groups >= 2 ? n : 0);
     if (n) {
+      if (groups >= 2) HIPCHECK(hipMemcpy(hn.data(), pa.nu.p, n * sizeof(Real), hipMemcpyDeviceToHost));
       HIPCHECK(hipMemcpy(hp.data(), pa.pos.p, n * sizeof(hp[0]), hipMemcpyDeviceToHost));
       HIPCHECK(hipMemcpy(hd.data(), pa.dir.p, n * sizeof(hd[0]), hipMemcpyDeviceToHost));
       HIPCHECK(hipMemcpy(hw.data(), pa.w.p, n * sizeof(double), hipMemcpyDeviceToHost));
@@ -710,6 +780,7 @@ struct Engine final : EngineBase {
       p.id = hm[i].y;
       p.ctr = hm[i].z;
       p.flags = hm[i].w;
+      p.nu = groups >= 2 ? hn[i] : Real(0);
     }
     kept_xs.resize(N);
     HIPCHECK(hipMemcpy(kept_xs.data(), xs.p, N * sizeof(CellXS<Real>), hipMemcpyDeviceToHost));
@@ -726,6 +797,16 @@ struct Engine final : EngineBase {
       consts4[2] = kept_consts.eq;
       consts4[3] = kept_consts.et;
     }
+  }
+
+  void debug_spectrum(int32_t *g, double *npow, void *bounds, void *cdf_out, void *sig_s_out) override {
+    *g = groups;
+    if (npow) *npow = nu_pow;
+    if (groups < 2) return;
+    const size_t G1 = static_cast<size_t>(groups) + 1;
+    if (bounds) HIPCHECK(hipMemcpy(bounds, bounds_kev.p, G1 * sizeof(Real), hipMemcpyDeviceToHost));
+    if (cdf_out) HIPCHECK(hipMemcpy(cdf_out, cdf.p, G1 * N * sizeof(Real), hipMemcpyDeviceToHost));
+    if (sig_s_out) HIPCHECK(hipMemcpy(sig_s_out, sig_s_cell.p, N * sizeof(Real), hipMemcpyDeviceToHost));
   }
 
   void debug_accel(void *out, uint64_t *bytes) override {
@@ -895,7 +976,7 @@ struct Engine final : EngineBase {
       r.vy = static_cast<double>(hd[q].y) * units::clight;
       r.vz = static_cast<double>(hd[q].z) * units::clight;
       r.weight = hw[q];
-      r.frequency = 0.0;
+      r.frequency = groups >= 2 ? static_cast<double>(hp[q].w) * units::kev : 0.0;
       r.cell = (static_cast<uint64_t>(i) * ny + j) * nz + k;
       r.id = ~0ull;
     }
@@ -927,6 +1008,11 @@ struct Engine final : EngineBase {
       hp[q].y = static_cast<Real>(frac(r.y, ye[j], ye[j + 1]));
       hp[q].z = static_cast<Real>(frac(r.z, ze[k], ze[k + 1]));
       hp[q].w = Real(0);
+      if (groups >= 2) {
+        // photon energy in keV, clamped to the group range (clampFrequencyToBounds)
+        const double E = std::min(std::max(r.frequency, bounds_h.front()), bounds_h.back());
+        hp[q].w = static_cast<Real>(E / units::kev);
+      }
       double vn = std::sqrt(r.vx * r.vx + r.vy * r.vy + r.vz * r.vz);
       if (!(vn > 0.0)) throw EngineError(1, "census packet with zero velocity");
       hd[q].x = static_cast<Real>(r.vx / vn);
@@ -1101,6 +1187,14 @@ int storm_gpu_debug_transport_inputs(storm_gpu_engine *eng, uint64_t *n, void *p
 
 int storm_gpu_debug_accel(storm_gpu_engine *eng, void *out, uint64_t *bytes) {
   return guarded([&] { as_engine(eng)->debug_accel(out, bytes); });
+}
+
+int storm_gpu_debug_spectrum(storm_gpu_engine *eng, int32_t *groups, double *nu_pow, void *bounds_kev, void *cdf,
+                             void *sig_s) {
+  return guarded([&] {
+    if (!groups) throw EngineError(1, "groups must not be NULL");
+    as_engine(eng)->debug_spectrum(groups, nu_pow, bounds_kev, cdf, sig_s);
+  });
 }
 
 int storm_gpu_debug_tallies(storm_gpu_engine *eng, uint64_t *dep, uint64_t *rdep, uint64_t *pint, uint64_t *esc,

@@ -18,6 +18,7 @@
 #include "common/constants.h"
 #include "common/fixed.h"
 #include "common/philox.h"
+#include "common/planck.h"
 #include "common/sourcing.h"
 #include "common/transport.h"
 #include "device_util.h"
@@ -43,12 +44,13 @@ struct PacketSoA {
   typename V4<Real>::T *dir;  // ox, oy, oz, wmin
   double *w;
   uint4 *meta;                // ijk, id, ctr, flags
+  Real *nu;                   // photon energy, keV (multigroup only, else null)
 };
 
 // Census store: what survives to the end of the step.
 template <typename Real>
 struct CensusSoA {
-  typename V4<Real>::T *pos;  // ux, uy, uz, -
+  typename V4<Real>::T *pos;  // ux, uy, uz, photon energy (keV, multigroup)
   typename V4<Real>::T *dir;  // ox, oy, oz, -
   double *w;
   uint2 *meta;                // ijk, id
@@ -73,6 +75,7 @@ __device__ __forceinline__ Packet<Real> load_packet(const PacketSoA<Real> &s, ui
   p.id = m.y;
   p.ctr = m.z;
   p.flags = m.w;
+  p.nu = s.nu ? s.nu[i] : Real(0);
   return p;
 }
 
@@ -91,6 +94,7 @@ __device__ __forceinline__ void store_packet(const PacketSoA<Real> &s, uint32_t 
   s.dir[i] = b;
   s.w[i] = p.w;
   s.meta[i] = make_uint4(p.ijk, p.id, p.ctr, p.flags);
+  if (s.nu) s.nu[i] = p.nu;
 }
 
 template <typename Real>
@@ -99,7 +103,7 @@ __device__ __forceinline__ void store_census(const CensusSoA<Real> &c, uint32_t 
   a.x = p.u[0];
   a.y = p.u[1];
   a.z = p.u[2];
-  a.w = Real(0);
+  a.w = p.nu;
   b.x = p.o[0];
   b.y = p.o[1];
   b.z = p.o[2];
@@ -198,6 +202,62 @@ __global__ __launch_bounds__(256) void k_prestep(uint32_t N, const double *__res
   xs[c] = x;
 }
 
+// Multigroup preStep per cell (storm_group_desc): the Planck opacity
+// sum_g sigma(Ec_g) B_g / sum_g B_g (DensmoreOpacity::CalcPlanckOpacity,
+// examples/densmore2012/DensmoreOpacity.hpp:38-55) feeds the Fleck factor and
+// the emission energy; the normalised cumulative of sigma(Ec_g) B_g is the
+// emission spectrum (:66-77, :128-148); the cell table holds f sigma_a and
+// (1-f) sigma_a at 1 keV, scaled per packet by E^-nu_pow.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_prestep_mg(
+    uint32_t N, const double *__restrict__ T, MatDev mat, const double *__restrict__ host_sa,
+    const double *__restrict__ host_ss, const double *__restrict__ host_cv, const double *__restrict__ V, double dt,
+    int32_t G, const double *__restrict__ bounds, double nu_pow, double nu_ref, CellXS<Real> *__restrict__ xs,
+    Real *__restrict__ cdf, Real *__restrict__ sig_s_cell, double *__restrict__ fleck,
+    double *__restrict__ sig_a_out, double *__restrict__ sig_s_out, double *__restrict__ Eemit,
+    uint32_t *__restrict__ err) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const double Tc = T[c];
+  const double sa_gray = host_sa ? host_sa[c] : mat.sigma_a(c, Tc);
+  const double ss = host_ss ? host_ss[c] : mat.sigma_s(c, Tc);
+  const double cv = host_cv ? host_cv[c] : mat.dT2cv(mat.density(c), Tc);
+  const double kT = units::k_boltz * fmax(Tc, mat.Tfloor);
+  Real *my = cdf + static_cast<size_t>(c) * static_cast<size_t>(G + 1);
+  double acc = 0.0, sumB = 0.0;
+  my[0] = Real(0);
+  for (int32_t g = 0; g < G; ++g) {
+    const double a = bounds[g] / kT, b = bounds[g + 1] / kT;
+    const double Bg = planck::integral(a, b);
+    const double Ec = 0.5 * (bounds[g] + bounds[g + 1]);
+    const double sg = sa_gray * pow(fmax(Ec, bounds[0]) / nu_ref, -nu_pow);
+    const double wg = (sg > 0.0 && isfinite(sg) && isfinite(Bg)) ? sg * Bg : 0.0;
+    sumB += Bg;
+    acc += wg;
+    my[g + 1] = static_cast<Real>(acc);
+  }
+  const bool valid = acc > 0.0 && isfinite(acc);
+  for (int32_t g = 1; g <= G; ++g) {
+    my[g] = valid ? static_cast<Real>(static_cast<double>(my[g]) / acc)
+                  : static_cast<Real>((bounds[g] - bounds[0]) / (bounds[G] - bounds[0]));
+  }
+  my[G] = Real(1);
+  const double sP = sumB > 0.0 ? acc / sumB : 1e-20;
+  const double T3 = Tc * Tc * Tc;
+  const double f = 1.0 / (1.0 + (4.0 * units::arad * T3 * sP * units::clight * dt) / cv);
+  if (!(f >= 0.0 && f <= 1.0)) atomicOr(err, kErrFleck);
+  fleck[c] = f;
+  sig_a_out[c] = sP;
+  sig_s_out[c] = ss;
+  Eemit[c] = f * V[c] * units::arad * (T3 * Tc) * sP * dt * units::clight;
+  const double s1 = sa_gray * pow(units::kev / nu_ref, -nu_pow);  // sigma_a at 1 keV
+  CellXS<Real> x;
+  x.sig_abs = static_cast<Real>(f * s1);
+  x.sig_ev = static_cast<Real>((1.0 - f) * s1);
+  xs[c] = x;
+  sig_s_cell[c] = static_cast<Real>(ss);
+}
+
 // photons per cell and packet weight, quantised down to the step quantum
 __global__ __launch_bounds__(256) void k_alloc(uint32_t N, const double *__restrict__ Eemit, double Etot,
                                                uint64_t n_min, int32_t eq, uint32_t *__restrict__ nphot,
@@ -277,6 +337,7 @@ __global__ __launch_bounds__(256) void k_import(uint32_t n, CensusSoA<Real> cen,
     p.id = i;
     p.ctr = 0u;
     p.flags = 0u;
+    p.nu = a.w;
     store_packet(out, i, p);
     L = fx::limbs_of(w, eq_new);
   }
@@ -298,7 +359,7 @@ template <typename Real>
 __global__ __launch_bounds__(256) void k_emit(uint32_t n_emit, uint32_t N, const uint64_t *__restrict__ offsets,
                                               const double *__restrict__ wemit, PacketSoA<Real> out,
                                               uint32_t base, Real s0, MeshDims d, uint64_t seed,
-                                              uint32_t step, uint32_t *__restrict__ begin) {
+                                              uint32_t step, uint32_t *__restrict__ begin, SpectrumView<Real> sp) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = q < n_emit;
   uint32_t c = 0xFFFFFFFFu;
@@ -312,7 +373,7 @@ __global__ __launch_bounds__(256) void k_emit(uint32_t n_emit, uint32_t N, const
     }
     c = lo;
     Packet<Real> p;
-    birth_emission<Real>(p, base + q, ijk_of_flat(c, d), wemit[c], s0, seed, step);
+    birth_emission<Real>(p, base + q, ijk_of_flat(c, d), wemit[c], s0, seed, step, sp, c);
     store_packet(out, base + q, p);
   }
   uint32_t one = valid ? 1u : 0u;
@@ -324,14 +385,15 @@ __global__ __launch_bounds__(256) void k_emit(uint32_t n_emit, uint32_t N, const
 template <typename Real>
 __global__ __launch_bounds__(256) void k_boundary(uint32_t n_bc, uint32_t npc, const double *__restrict__ wface,
                                                   PacketSoA<Real> out, uint32_t base, Real s0, MeshDims d,
-                                                  uint64_t seed, uint32_t step, uint32_t *__restrict__ begin) {
+                                                  uint64_t seed, uint32_t step, uint32_t *__restrict__ begin,
+                                                  SpectrumView<Real> sp, const Real *__restrict__ bcdf) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = q < n_bc;
   const uint32_t f = valid ? q / npc : 0xFFFFFFFFu;  // face index j * nz + k = cell (0, j, k)
   if (valid) {
     const uint32_t j = f / d.nz, k = f % d.nz;
     Packet<Real> p;
-    birth_boundary<Real>(p, base + q, j, k, wface[f], s0, seed, step);
+    birth_boundary<Real>(p, base + q, j, k, wface[f], s0, seed, step, sp, bcdf);
     store_packet(out, base + q, p);
   }
   uint32_t one = valid ? 1u : 0u;
@@ -781,7 +843,7 @@ __global__ __launch_bounds__(256) void k_post(uint32_t N, Tallies t, int32_t eq,
                                               double *__restrict__ e_sync, double *__restrict__ erad_avg,
                                               double *__restrict__ census_erad, double *__restrict__ clamp,
                                               uint32_t *__restrict__ err, int32_t fp64,
-                                              const CellXS<Real> *__restrict__ xs) {
+                                              const CellXS<Real> *__restrict__ xs, int32_t gray) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
   const double emit = fx::to_double(emit_limbs[2 * c], emit_limbs[2 * c + 1], eq);
@@ -795,7 +857,7 @@ __global__ __launch_bounds__(256) void k_post(uint32_t N, Tallies t, int32_t eq,
     // the path integral, so where sig_abs > 0 the integral is the absorbed
     // energy over sig_abs (the transport skips that tally there).
     const double sabs = static_cast<double>(xs[c].sig_abs);
-    pint = sabs > 0.0 ? dep / sabs : fx::to_double(t.pint[2 * c], t.pint[2 * c + 1], et);
+    pint = (gray && sabs > 0.0) ? dep / sabs : fx::to_double(t.pint[2 * c], t.pint[2 * c + 1], et);
     dep += fx::to_double(t.rdep[2 * c], t.rdep[2 * c + 1], eq);
   }
   dep += fx::to_double(t.rem[2 * c], t.rem[2 * c + 1], eq);

@@ -143,6 +143,22 @@ class GpuEngine:
                                                               xs.ctypes.data_as(C.c_void_p), consts))
         return pk, int(n.value), xs, [int(v) for v in consts]
 
+    def spectrum_data(self):
+        """Multigroup data of the last step (float32 arrays for the mixed
+        engine): dict(groups, nu_pow, bounds_kev, cdf, sig_s), or None when gray."""
+        G = C.c_int32()
+        npow = C.c_double()
+        self._check(self.lib.storm_gpu_debug_spectrum(self.h, C.byref(G), C.byref(npow), None, None, None))
+        if G.value < 2:
+            return None
+        dt = np.float32 if self.precision == capi.MIXED else np.float64
+        b = np.empty(G.value + 1, dtype=dt)
+        cdf = np.empty(self.N * (G.value + 1), dtype=dt)
+        ss = np.empty(self.N, dtype=dt)
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        self._check(self.lib.storm_gpu_debug_spectrum(self.h, C.byref(G), C.byref(npow), vp(b), vp(cdf), vp(ss)))
+        return dict(groups=G.value, nu_pow=npow.value, bounds_kev=b, cdf=cdf, sig_s=ss)
+
     def accel_data(self) -> bytes | None:
         n = C.c_uint64()
         self._check(self.lib.storm_gpu_debug_accel(self.h, None, C.byref(n)))

@@ -82,6 +82,9 @@ class ProblemSpec:
     ddmc_cell_tau: float = 15.0
     ddmc_particle_tau: float = 5.0
     seed: int = 42
+    group_bounds: np.ndarray | None = None  # photon energies, erg (multigroup when >= 3 entries)
+    nu_pow: float = 0.0
+    nu_ref: float = 1.0
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -116,9 +119,10 @@ class ProblemSpec:
     def to_c(self) -> capi.Problem:
         arrs = [_f64(self.x_edges), _f64(self.y_edges), _f64(self.z_edges), _f64(self.sigma_a0),
                 None if self.sigma_s0 is None else _f64(self.sigma_s0),
-                None if self.density is None else _f64(self.density), _f64(self.T0)]
+                None if self.density is None else _f64(self.density), _f64(self.T0),
+                None if self.group_bounds is None else _f64(self.group_bounds)]
         self._keep = arrs
-        xe, ye, ze, sa, ss, rho, t0 = arrs
+        xe, ye, ze, sa, ss, rho, t0, gb = arrs
         p = capi.Problem()
         nx, ny, nz = self.shape
         p.mesh = capi.MeshDesc(nx, ny, nz, 0, _ptr(xe), _ptr(ye), _ptr(ze))
@@ -145,6 +149,11 @@ class ProblemSpec:
         p.imc = q
         p.T0 = _ptr(t0)
         p.seed = int(self.seed)
+        g = capi.GroupDesc()
+        g.num_groups = 0 if gb is None else len(gb) - 1
+        g.bounds = _ptr(gb)
+        g.nu_pow, g.nu_ref = float(self.nu_pow), float(self.nu_ref)
+        p.groups = g
         return p
 
 
@@ -312,6 +321,32 @@ def marshak_l1(x_sim, T_sim, ref) -> float:
             l1 += abs(Ts / KEV_K - Tr / KEV_K) / (Tr / KEV_K)
             cnt += 1
     return l1 / cnt if cnt else -1.0
+
+
+def densmore(nx: int = 512, seed: int = 0, new_photons: int = 16, boundary_photons: int = 100, steps: int = 200,
+             groups: int = 30, comb_nmin: int = 200, comb_factor: float = 5.0) -> ProblemSpec:
+    """Densmore et al. (2012) step-opacity benchmark as examples/densmore2012/main.cpp:128-200
+    sets it up: slab [0, 3] cm, sigma(E) = sigma0 / (sqrt(kT) E^3) with sigma0 = 10
+    keV^3.5/cm for x < 2 cm and 1000 beyond, cv = 1e15 erg/(cm^3 keV), T0 = 1 eV,
+    a 1 keV x- source (escape there, reflect elsewhere), 30 log-spaced groups over
+    1e-4..1e2 keV, dt = 5e-12 s, comb(200, 5).  The reference also switches the
+    random walk on (partial-gray in multigroup); it is off here."""
+    L, xstep = 3.0, 2.0
+    xe = np.linspace(0.0, L, nx + 1)
+    h = L / nx
+    xc = 0.5 * (xe[1:] + xe[:-1])
+    sa0 = np.where(xc < xstep, 10.0, 1000.0)
+    bounds = KEV * 1e-4 * (1e6 ** (np.arange(groups + 1) / groups))
+    bounds[-1] = KEV * 1e2
+    return ProblemSpec(
+        name="densmore", x_edges=xe, y_edges=np.array([0.0, h]), z_edges=np.array([0.0, h]),
+        sigma_a0=sa0, T0=np.full(nx, KEV_K / 1000.0), dt=5e-12, steps=steps,
+        T_ref=KEV_K, T_floor=1.0, a_pow=0.5,
+        eos_f=1e15 / KEV_K, eos_beta=1.0, eos_mu=0.0,
+        face_kind=(capi.FACE_ESCAPE,) + (capi.FACE_REFLECT,) * 5, source_face=capi.XLO, source_T=KEV_K,
+        freeze_T4=0, source_per_face=boundary_photons, new_photons_per_cell=new_photons,
+        comb=1, comb_nmin=comb_nmin, comb_factor=comb_factor, seed=seed,
+        group_bounds=bounds, nu_pow=3.0, nu_ref=KEV)
 
 
 def marshak_bath_T(problem: int, t: float) -> float:

@@ -32,9 +32,9 @@ def test_binding_signatures_cover_declarations():
 
 
 def test_struct_layouts_match_header_sizes():
-    # storm_packet_record: 8 doubles + 2 u64 = 80 bytes; Packet<float> = 8 floats + 1 double + 4 u32 = 56 bytes
+    # storm_packet_record: 8 doubles + 2 u64 = 80 bytes; Packet<float> = 8 floats + 1 double + 4 u32 + energy, padded = 64 bytes
     assert C.sizeof(capi.PacketRecord) == 80
-    assert C.sizeof(capi.PacketF32) == 56
+    assert C.sizeof(capi.PacketF32) == 64
     hdr = open(os.path.join(ROOT, "include", "storm_problem.h")).read()
     for field in ("x_edges", "sigma_a0", "face_kind", "new_photons_per_cell", "comb_nmin", "T0", "seed"):
         assert re.search(r"\b" + field + r"\b", hdr), field
@@ -44,7 +44,7 @@ def test_struct_layouts_match_header_sizes():
 
 def test_abi_version_without_gpu():
     lib = capi.load_library()
-    assert lib.storm_gpu_abi_version() == 2
+    assert lib.storm_gpu_abi_version() == 3
 
 
 def test_cpp_manager_program_fails_loudly_without_engine():

@@ -168,13 +168,22 @@ def test_oracle_comb_conserves_and_targets(oracle):
     assert np.all(cnt[~combed & (counts > 0)] >= counts[~combed & (counts > 0)])
 
 
+def _densmore_small(s):
+    # 16-cell Densmore slab at 0.3 keV so that cells emit across the groups
+    spec = P.densmore(nx=16, seed=s, steps=1, new_photons=6, boundary_photons=300, comb_nmin=4, comb_factor=1.0)
+    spec.T0 = np.full(16, 0.3 * P.KEV_K)
+    return spec
+
+
 @pytest.mark.parametrize("precision", [0, 1], ids=["mixed", "fp64"])
-@pytest.mark.parametrize("which", ["c1", "c2"])
+@pytest.mark.parametrize("which", ["c1", "c2", "multigroup"])
 def test_engine_first_step_physics_on_cpu(oracle, precision, which):
     """The engine's shared sourcing + transport code, run on the CPU, agrees
     with the oracle's first step (per-cell Erad time average and segments)."""
     if which == "c1":
         make = lambda s: P.c1_slab(seed=s, packets_scale=0.05, steps=1)  # noqa: E731
+    elif which == "multigroup":
+        make = _densmore_small
     else:
         make = lambda s: P.c2_cube(seed=7, n=8, packets_scale=0.2, steps=1).with_seed(s)  # noqa: E731
     o_erad, o_seg, e_erad, e_seg = [], [], [], []
