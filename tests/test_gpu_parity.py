@@ -70,6 +70,13 @@ C2 = lambda s: P.c2_cube(seed=7, n=8, packets_scale=0.2, steps=3).with_seed(s)  
 C3 = lambda s: P.c3_step(seed=s, nx=32, nyz=2, packets_scale=0.1, steps=3)  # noqa: E731
 
 
+def MG(s):
+    """16-cell, 30-group Densmore slab started at 0.3 keV, 3 steps."""
+    spec = P.densmore(nx=16, seed=s, steps=3, new_photons=6, boundary_photons=300, comb_nmin=8, comb_factor=1.0)
+    spec.T0 = np.full(16, 0.3 * P.KEV_K)
+    return spec
+
+
 @pytest.fixture(scope="module")
 def c1_ref(oracle):
     return oracle_ensemble(oracle, C1, range(1000, 1000 + NSEEDS))
@@ -95,6 +102,18 @@ def test_c2_cube_matches_oracle(c2_ref, precision):
 @pytest.fixture(scope="module")
 def c3_ref(oracle):
     return oracle_ensemble(oracle, C3, range(1000, 1000 + NSEEDS))
+
+
+@pytest.fixture(scope="module")
+def mg_ref(oracle):
+    return oracle_ensemble(oracle, MG, range(1000, 1000 + NSEEDS))
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
+def test_multigroup_matches_oracle(mg_ref, precision):
+    """30-group transport with energy-dependent opacity and re-emission."""
+    got = gpu_ensemble(MG, range(NSEEDS), precision)
+    _assert_parity(mg_ref, got, f"mg/{precision}")
 
 
 @pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
@@ -144,7 +163,7 @@ def test_fp64_energy_balance():
             assert L["census_pre"] == L["census_out"] + L["comb_rem"]
 
 
-@pytest.mark.parametrize("make", [C1, C2, C3], ids=["c1", "c2", "c3"])
+@pytest.mark.parametrize("make", [C1, C2, C3, MG], ids=["c1", "c2", "c3", "multigroup"])
 def test_replay_reproduces_tallies_bit_exactly(oracle, make):
     spec = make(5)
     with GpuEngine(spec, capi.MIXED) as e:
@@ -154,8 +173,9 @@ def test_replay_reproduces_tallies_bit_exactly(oracle, make):
         pk, n, xs, consts = e.transport_inputs()
         got = e.tallies()
         accel = e.accel_data()
+        spectrum = e.spectrum_data()
         cnt = e.counters()
-    ref, outc = oracle.replay_transport(spec, pk, n, xs, consts, accel=accel)
+    ref, outc = oracle.replay_transport(spec, pk, n, xs, consts, accel=accel, spectrum=spectrum)
     assert n > 0 and outc[:4].sum() == n
     assert outc[4] == cnt["random_walk_steps"] and outc[5] == cnt["ddmc_steps"]
     for k in ("segs", "dep", "rdep", "pint", "esc"):
