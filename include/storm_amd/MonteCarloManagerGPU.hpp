@@ -34,6 +34,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <type_traits>
 #include <string>
 #include <utility>
 #include <vector>
@@ -46,6 +47,15 @@
 namespace STORM {
 
 enum class GpuPrecision { Mixed = STORM_GPU_MIXED, FP64 = STORM_GPU_FP64 };
+
+namespace detail {
+template <typename G, typename = void>
+struct has_axis_edges : std::false_type {};
+template <typename G>
+struct has_axis_edges<G, std::void_t<decltype(std::declval<const G &>().xEdges()),
+                                     decltype(std::declval<const G &>().yEdges()),
+                                     decltype(std::declval<const G &>().zEdges())>> : std::true_type {};
+}  // namespace detail
 
 template <typename T, typename Grid>
 class MonteCarloManagerGPU {
@@ -189,9 +199,18 @@ class MonteCarloManagerGPU {
     if (rc != 0) throw StormError(std::string("gfx950 engine: ") + storm_gpu_last_error());
   }
 
-  // Edges from the cell centres: on a Cartesian grid each axis' centres are
-  // the midpoints of consecutive edges, starting at the box corner.
+  // Axis edges: taken as they are from grids that expose them, else
+  // recovered from the cell centres (on a Cartesian grid each axis' centres
+  // are the midpoints of consecutive edges, starting at the box corner).
   void describeMesh() {
+    if constexpr (detail::has_axis_edges<Grid>::value) {
+      auto copy = [](const auto &src, std::vector<double> &dst) { dst.assign(src.begin(), src.end()); };
+      copy(grid_.xEdges(), xe_);
+      copy(grid_.yEdges(), ye_);
+      copy(grid_.zEdges(), ze_);
+      checkCellOrder();
+      return;
+    }
     const auto box = grid_.GetBoxCoordinates();
     auto axis = [&](auto get, double lo, double hi, std::vector<double> &edges) {
       std::vector<double> c;
@@ -211,6 +230,10 @@ class MonteCarloManagerGPU {
     axis([](const T &p) { return static_cast<double>(p.x); }, box.first.x, box.second.x, xe_);
     axis([](const T &p) { return static_cast<double>(p.y); }, box.first.y, box.second.y, ye_);
     axis([](const T &p) { return static_cast<double>(p.z); }, box.first.z, box.second.z, ze_);
+    checkCellOrder();
+  }
+
+  void checkCellOrder() {
     const size_t nx = xe_.size() - 1, ny = ye_.size() - 1, nz = ze_.size() - 1;
     if (nx * ny * nz != Ncells_) throw StormError("MonteCarloManagerGPU: the grid is not a full Cartesian box");
     for (size_t i = 0; i < nx; ++i)

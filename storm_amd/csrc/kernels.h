@@ -470,10 +470,14 @@ __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real
 // Persistent transport with lane refill.  Each wavefront advances its 64
 // packets one segment at a time; a lane whose packet ended (census, absorbed,
 // escaped, or `max_segments` segments in this launch) takes the next packet
-// of the input queue (`*queue` counts handed-out packets, one atomic per
-// wavefront).  Lanes stay busy until the queue is drained, instead of idling
-// until the longest history of their wavefront ends.  Per packet the result
-// is that of k_transport with the same `max_segments`.
+// from its wavefront's pool of queued packets.  The pool is refilled with
+// kRefillChunk consecutive packets by one atomic on `*queue` (the count of
+// handed-out packets) when it runs dry, so the shared counter sees one atomic
+// per chunk rather than one per lane refill.  Lanes stay busy until the
+// queue is drained, instead of idling until the longest history of their
+// wavefront ends.  Per packet the result is that of k_transport with the
+// same `max_segments`.
+constexpr uint32_t kRefillChunk = 64;
 // stats: [0] escapes, [1] random-walk steps, [2] DDMC steps
 template <typename Real>
 __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out,
@@ -486,19 +490,39 @@ __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketS
   DeviceSink sink{t, k.eq};
   Tracker<Real> tr;
   bool busy = false;
-  bool drained = false;  // uniform across the wavefront
+  // uniform across the wavefront: the pool [pool_next, pool_end) of queued
+  // packets this wavefront owns, and whether the queue is exhausted
+  bool drained = false;
+  uint32_t pool_next = 0, pool_end = 0;
   int32_t nseg = 0;
   uint32_t escapes = 0;
   for (;;) {
     if (!drained) {
-      const uint32_t slot = wave_append(!busy, queue);
-      const bool got = !busy && slot < n_in;
-      if (__builtin_amdgcn_ballot_w64(!busy && slot >= n_in)) drained = true;
-      if (got) {
-        const Packet<Real> p = load_packet(in, slot);
-        tr.begin(p, mesh, xs, k);
-        busy = true;
-        nseg = 0;
+      const uint64_t idle = __builtin_amdgcn_ballot_w64(!busy);
+      if (idle) {
+        if (pool_next >= pool_end) {
+          uint32_t base = 0u;
+          if (lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u) base = atomicAdd(queue, kRefillChunk);
+          base = __builtin_amdgcn_readfirstlane(base);
+          if (base >= n_in) {
+            drained = true;
+          } else {
+            pool_next = base;
+            pool_end = base + kRefillChunk < n_in ? base + kRefillChunk : n_in;
+          }
+        }
+        if (!drained) {
+          const uint32_t avail = pool_end - pool_next;
+          const uint32_t want = static_cast<uint32_t>(__popcll(idle));
+          const uint32_t r = lane_rank(idle);
+          if (!busy && r < avail) {
+            const Packet<Real> p = load_packet(in, pool_next + r);
+            tr.begin(p, mesh, xs, k);
+            busy = true;
+            nseg = 0;
+          }
+          pool_next += want < avail ? want : avail;
+        }
       }
     }
     // every lane idle: the queue was drained on this or an earlier round
