@@ -346,8 +346,10 @@ def test_cpp_manager_matches_oracle(c1_ref, precision):
 
 def test_cpp_manager_resident_is_the_engine():
     """With the census resident in HBM the manager's step is the engine's
-    step: host-tabulated opacity, cv and EOS give the same doubles as the
-    device material model, so the results are bit-identical."""
+    step on host-tabulated material data: the opacity and cv are the same
+    doubles as the device material model's, and the host EOS differs from the
+    device's only in the rounding of its power function, so the packet
+    histories coincide and the cell results agree to rounding."""
     spec = C1(3)
     got, runs = _run_manager(spec, capi.MIXED, True, [3])
     with GpuEngine(spec, capi.MIXED) as e:
@@ -356,6 +358,6 @@ def test_cpp_manager_resident_is_the_engine():
         c = e.cells()
         n = e.census_size()
     assert runs[0]["census_packets"] == n
-    assert np.array_equal(got["T"][0], c["T"])
-    assert np.array_equal(got["erad"][0], c["erad"])
+    assert np.allclose(got["T"][0], c["T"], rtol=1e-12, atol=0)
+    assert np.allclose(got["erad"][0], c["erad"], rtol=1e-12, atol=0)
     assert np.allclose(got["census"][0], c["census"], rtol=1e-12, atol=0)
