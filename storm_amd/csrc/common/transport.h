@@ -182,10 +182,15 @@ STORM_HD void put3(int a, T &x, T &y, T &z, T v) {
 // (both are sums of w (1 - e^{-x}) over the same segments), which the
 // postStep evaluates from the exact deposit tally.
 //
-// Tracker<Real> holds one packet's history between segments, so that a
+// Tracker<Real, F> holds one packet's history between segments, so that a
 // kernel can advance a packet one segment at a time and give an idle lane a
 // new packet (transport_packet below runs a whole pass of one packet).
-template <typename Real>
+// F selects the code compiled in: kFeatAccel (random walk and DDMC) and
+// kFeatMultigroup; a problem without them runs a leaner kernel with the same
+// results.
+enum : uint32_t { kFeatAccel = 1u, kFeatMultigroup = 2u, kFeatAll = 3u };
+
+template <typename Real, uint32_t F = kFeatAll>
 struct Tracker {
   using M = MathPolicy<Real>;
   static constexpr bool fixed_point = sizeof(Real) == 4;
@@ -249,7 +254,10 @@ struct Tracker {
   // The mixed engine tallies the path integral only where postStep cannot
   // derive it from the deposit (gray cells without absorption).
   STORM_HD bool derives_path_integral(const MeshView<Real> &m) const {
-    return fixed_point && m.spec.groups < 2 && cell_sig_abs > Real(0);
+    if constexpr ((F & kFeatMultigroup) != 0u) {
+      if (m.spec.groups >= 2) return false;
+    }
+    return fixed_point && cell_sig_abs > Real(0);
   }
 
   template <typename Sink>
@@ -297,7 +305,7 @@ struct Tracker {
     cell_sig_abs = x.sig_abs;
     cell_sig_ev = x.sig_ev;
     cell_sig_s = Real(0);
-    if (m.spec.groups >= 2) {
+    if ((F & kFeatMultigroup) != 0u && m.spec.groups >= 2) {
       // clampFrequencyToBounds (RadiationIMC.hpp:868-872) and E^-nu_pow
       Real e = nu < m.spec.e_lo ? m.spec.e_lo : (nu > m.spec.e_hi ? m.spec.e_hi : nu);
       const Real fac = M::exp(-m.spec.nu_pow * M::log(e));
@@ -363,7 +371,7 @@ struct Tracker {
     // acceleration step).
     Philox4 r = rs.block(ctr++);
 
-    if (acc.cell) {
+    if ((F & kFeatAccel) != 0u && acc.cell) {
       const CellAccel<Real> ca = acc.cell[cell];
       if (ca.flags) {
         // distance to the nearest face (computeMinDistanceToFaces, :956-969)
@@ -601,7 +609,7 @@ struct Tracker {
       o0 = nd[0];
       o1 = nd[1];
       o2 = nd[2];
-      if (m.spec.groups >= 2) {
+      if ((F & kFeatMultigroup) != 0u && m.spec.groups >= 2) {
         // an effective (absorption + re-emission) scatter takes a new energy
         // from the cell's emission spectrum (:1843-1852); an elastic one
         // (probability sigma_s / event opacity) keeps it
@@ -623,10 +631,10 @@ struct Tracker {
 
 // One transport pass of a packet: at most `max_segments` segments, then the
 // packet is written back (alive, in census, absorbed or escaped).
-template <typename Real, typename Sink>
+template <typename Real, uint32_t F = kFeatAll, typename Sink>
 STORM_HD int transport_packet(Packet<Real> &p, const MeshView<Real> &m, const CellXS<Real> *xs,
                               const AccelView<Real> &acc, const StepConsts &k, int max_segments, Sink &sink) {
-  Tracker<Real> t;
+  Tracker<Real, F> t;
   t.begin(p, m, xs, k);
   for (int seg = 0; seg < max_segments && t.outcome == kAlive; ++seg) t.segment(m, xs, acc, k, sink);
   t.end(p, k, sink);

@@ -414,19 +414,47 @@ struct Engine final : EngineBase {
     return s;
   }
 
-  // Workgroups of the refill transport that fit on the device at once.
-  uint32_t resident_cache[3] = {0, 0, 0};
+  // Workgroups of a refill transport kernel that fit on the device at once.
+  uint32_t resident_cache[4][3] = {};
+  template <uint32_t F>
   uint32_t resident_blocks(int32_t b) {
-    uint32_t &r = resident_cache[b == 64 ? 0 : (b == 128 ? 1 : 2)];
+    uint32_t &r = resident_cache[F][b == 64 ? 0 : (b == 128 ? 1 : 2)];
     if (!r) {
       int dev = 0, per_cu = 0;
       HIPCHECK(hipGetDevice(&dev));
       hipDeviceProp_t prop;
       HIPCHECK(hipGetDeviceProperties(&prop, dev));
-      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport_refill<Real>, b, 0));
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport_refill<Real, F>, b, 0));
       r = static_cast<uint32_t>(std::max(per_cu, 1) * std::max(prop.multiProcessorCount, 1));
     }
     return r;
+  }
+
+  // One transport launch on the kernel variant compiled for this problem's
+  // features (random walk / DDMC, multigroup).
+  template <uint32_t F>
+  void launch_transport_f(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshView<Real> &mv,
+                          const AccelView<Real> &av, const StepConsts &kc) {
+    if (schedule == STORM_GPU_SCHEDULE_REFILL) {
+      const uint32_t grid = std::min<uint32_t>(blocks_for(n_in, block), resident_blocks<F>(block));
+      k_transport_refill<Real, F><<<grid, block, 0, stream>>>(
+          static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
+          xs.p, av, kc, seg_per_pass, tal, leaving.p, counters_u32.p + 2);
+    } else {
+      k_transport<Real, F><<<blocks_for(n_in, block), block, 0, stream>>>(
+          static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
+          xs.p, av, kc, seg_per_pass, tal, leaving.p);
+    }
+  }
+  void launch_transport(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshView<Real> &mv,
+                        const AccelView<Real> &av, const StepConsts &kc) {
+    const uint32_t f = (accel_on ? kFeatAccel : 0u) | (groups >= 2 ? kFeatMultigroup : 0u);
+    switch (f) {
+      case 0u: launch_transport_f<0u>(n_in, src, dst, mv, av, kc); break;
+      case kFeatAccel: launch_transport_f<kFeatAccel>(n_in, src, dst, mv, av, kc); break;
+      case kFeatMultigroup: launch_transport_f<kFeatMultigroup>(n_in, src, dst, mv, av, kc); break;
+      default: launch_transport_f<kFeatAll>(n_in, src, dst, mv, av, kc); break;
+    }
   }
 
   // Deterministic FP64 sum (fixed grid and tree) of d[0..n) into *out (device).
@@ -606,16 +634,7 @@ struct Engine final : EngineBase {
       zero_u32(counters_u32.p, 1);  // n_out
       if (schedule == STORM_GPU_SCHEDULE_REFILL) zero_u32(counters_u32.p + 2, 1);  // queue head
       HIPCHECK(hipEventRecord(ev0, stream));
-      if (schedule == STORM_GPU_SCHEDULE_REFILL) {
-        const uint32_t grid = std::min<uint32_t>(blocks_for(n_in, block), resident_blocks(block));
-        k_transport_refill<Real><<<grid, block, 0, stream>>>(
-            static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
-            xs.p, av, kc, seg_per_pass, tal, leaving.p, counters_u32.p + 2);
-      } else {
-        k_transport<Real><<<blocks_for(n_in, block), block, 0, stream>>>(
-            static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
-            xs.p, av, kc, seg_per_pass, tal, leaving.p);
-      }
+      launch_transport(n_in, src, dst, mv, av, kc);
       HIPCHECK(hipEventRecord(ev1, stream));
       uint32_t h[2];
       HIPCHECK(hipMemcpyAsync(h, counters_u32.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));

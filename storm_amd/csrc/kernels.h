@@ -434,7 +434,7 @@ __device__ __forceinline__ void wave_add_u32(uint32_t v, unsigned long long *dst
 }
 
 // stats: [0] escapes, [1] random-walk steps, [2] DDMC steps
-template <typename Real>
+template <typename Real, uint32_t F>
 __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out,
                                                    uint32_t *__restrict__ n_out, CensusSoA<Real> cen,
                                                    uint32_t *__restrict__ n_cen, MeshView<Real> mesh,
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real
   DeviceSink sink{t, k.eq};
   if (valid) {
     p = load_packet(in, i);
-    outcome = transport_packet<Real>(p, mesh, xs, acc, k, max_segments, sink);
+    outcome = transport_packet<Real, F>(p, mesh, xs, acc, k, max_segments, sink);
   }
   const bool alive = valid && outcome == kAlive;
   const bool census = valid && outcome == kCensus;
@@ -479,7 +479,7 @@ __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real
 // same `max_segments`.
 constexpr uint32_t kRefillChunk = 64;
 // stats: [0] escapes, [1] random-walk steps, [2] DDMC steps
-template <typename Real>
+template <typename Real, uint32_t F>
 __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out,
                                                           uint32_t *__restrict__ n_out, CensusSoA<Real> cen,
                                                           uint32_t *__restrict__ n_cen, MeshView<Real> mesh,
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketS
                                                           unsigned long long *__restrict__ stats,
                                                           uint32_t *__restrict__ queue) {
   DeviceSink sink{t, k.eq};
-  Tracker<Real> tr;
+  Tracker<Real, F> tr;
   bool busy = false;
   // uniform across the wavefront: the pool [pool_next, pool_end) of queued
   // packets this wavefront owns, and whether the queue is exhausted
