@@ -258,6 +258,8 @@ class MonteCarloManagerGPU {
     bnd_.freeze_source_T4 = info.freeze_T4 ? 1 : 0;
     bnd_.source_temperature = info.temperature;
     bnd_.source_packets_per_face = info.packets_per_face;
+    bnd_.source_xhi = info.source_xhi ? 1 : 0;
+    bnd_.source_temperature_xhi = info.temperature_xhi;
     if (auto *st = dynamic_cast<SideTemperature<T, Grid> *>(boundary_.get()))
       sourceT_ = [st]() { return st->GetTemperature(); };
   }

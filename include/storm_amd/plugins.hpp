@@ -4,6 +4,7 @@
 //   physics/MonteCarloPhysics.hpp:16-53            MonteCarloPhysics
 //   boundary/BoundaryCondition.hpp:30-51           BoundaryCondition
 //   boundary/SideTemperature.hpp:156-191           SideTemperature
+//   boundary/TwoSidesTemperature.hpp:20-58         TwoSidesTemperature
 //   boundary/Vacuum.hpp:10-45                      VacuumBoundary
 //   boundary/RigidBoundary.hpp:9-24                RigidBoundary
 //   population/PopulationControl.hpp:8-22          PopulationControl
@@ -43,10 +44,12 @@ enum class DDMCBoundaryFaceBehavior { Unsupported, ReflectingRigid };
 struct GpuBoundaryInfo {
   int face_kind[6] = {STORM_FACE_REFLECT, STORM_FACE_REFLECT, STORM_FACE_REFLECT,
                       STORM_FACE_REFLECT, STORM_FACE_REFLECT, STORM_FACE_REFLECT};
-  bool source = false;
+  bool source = false;        // Planck source on x-
   double temperature = 0.0;
   size_t packets_per_face = 0;
   bool freeze_T4 = false;
+  bool source_xhi = false;    // Planck source on x+ as well (same packets per face)
+  double temperature_xhi = 0.0;
 };
 
 template <typename T, typename Grid>
@@ -105,6 +108,43 @@ class SideTemperature : public BoundaryCondition<T, Grid> {
   double temperature;
   size_t Npercell;
   bool freezeT4;
+};
+
+// Planck sources on both x faces, which remove packets; the other faces
+// reflect.  T^4 of each side is taken at the first step
+// (`static const double T4_L/T4_R`, boundary/TwoSidesTemperature.hpp:158-159).
+template <typename T, typename Grid>
+class TwoSidesTemperature : public BoundaryCondition<T, Grid> {
+ public:
+  TwoSidesTemperature(const Grid &grid, double temperatureLeft, double temperatureRight, size_t Npercell,
+                      const std::vector<double> &energyBoundaries = {})
+      : BoundaryCondition<T, Grid>(grid), temperatureLeft(temperatureLeft), temperatureRight(temperatureRight),
+        Npercell(Npercell) {
+    if (!energyBoundaries.empty())
+      throw StormError("TwoSidesTemperature: multigroup sources go through storm_group_desc, not the C++ manager");
+  }
+  ParticleStatus apply(Particle<T, Grid> &) override {
+    throw StormError("TwoSidesTemperature::apply is evaluated on the GPU (MonteCarloManagerGPU)");
+  }
+  std::vector<Particle<T, Grid>> generateNewBoundaryParticles(double) override {
+    throw StormError("TwoSidesTemperature sources are generated on the GPU (MonteCarloManagerGPU)");
+  }
+  bool describeForGpu(GpuBoundaryInfo &info) const override {
+    info = GpuBoundaryInfo{};
+    info.face_kind[STORM_XLO] = STORM_FACE_ESCAPE;
+    info.face_kind[STORM_XHI] = STORM_FACE_ESCAPE;
+    info.source = Npercell > 0;
+    info.source_xhi = Npercell > 0;
+    info.temperature = temperatureLeft;
+    info.temperature_xhi = temperatureRight;
+    info.packets_per_face = Npercell;
+    info.freeze_T4 = true;
+    return true;
+  }
+
+ private:
+  double temperatureLeft, temperatureRight;
+  size_t Npercell;
 };
 
 // Every face removes the packet and the escaped energy is tallied (boundary/Vacuum.hpp:18-22).
@@ -335,6 +375,51 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
   std::vector<MCParticle> preStep(double) override { throw StormError(kGpuOnly); }
   StepResult<PointT, GridT> step(MCParticle &, std::vector<MCParticle> &) override { throw StormError(kGpuOnly); }
   void postStep(const std::vector<MCParticle> &, double) override { throw StormError(kGpuOnly); }
+
+  // generateInitialParticles (radiation/RadiationIMC.hpp:2559-2612), gray:
+  // particlesPerCell packets per cell carrying the cell's radiation energy
+  // (extensives' Erad, else cells' Erad times mass), uniform in the cell box,
+  // isotropic, for the first call of the manager's step.
+  std::vector<MCParticle> generateInitialParticles(std::size_t particlesPerCell) {
+    std::vector<MCParticle> out;
+    if (particlesPerCell == 0) return out;
+    const std::size_t N = this->grid.GetPointNo();
+    std::mt19937_64 rng(seed_ ^ 0x9E3779B97F4A7C15ull);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    for (std::size_t i = 0; i < N; ++i) {
+      double E = 0.0;
+      if constexpr (detail::has_erad<ExtensivesT>::value) E = extensives_[i].Erad;
+      else if constexpr (detail::has_erad<CellT>::value) E = cells_[i].Erad * extensives_[i].mass;
+      if (!(E > 0.0)) continue;
+      // the cell's box: per axis, the extreme face centroids
+      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+      for (auto f : this->grid.GetCellFaces(i)) {
+        const PointT fc = this->grid.FaceCM(f);
+        const double v[3] = {fc.x, fc.y, fc.z};
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = std::min(lo[a], v[a]);
+          hi[a] = std::max(hi[a], v[a]);
+        }
+      }
+      for (std::size_t j = 0; j < particlesPerCell; ++j) {
+        MCParticle p;
+        p.cellIndex = i;
+        p.frequency = 0.0;
+        // uniform in the cell (RandomInCellPositionSampler on a box cell)
+        p.location = PointT(lo[0] + U(rng) * (hi[0] - lo[0]), lo[1] + U(rng) * (hi[1] - lo[1]),
+                            lo[2] + U(rng) * (hi[2] - lo[2]));
+        // isotropic (RadiationOpacityModel.hpp:139-150)
+        const double phi = 6.283185307179586 * U(rng), mu = 2.0 * U(rng) - 1.0;
+        const double st = std::sqrt(std::max(0.0, 1.0 - mu * mu));
+        constexpr double c = 2.99792458e10;  // cm/s
+        p.velocity = PointT(st * std::cos(phi) * c, st * std::sin(phi) * c, mu * c);
+        p.weight = E / static_cast<double>(particlesPerCell);
+        p.initialWeight = p.weight;
+        out.push_back(p);
+      }
+    }
+    return out;
+  }
 
   const std::vector<double> &getFactorFleck() const { return factorFleck_; }
   const std::vector<double> &getPlanckOpacities() const { return planck_; }

@@ -84,6 +84,10 @@ typedef struct storm_boundary_desc {
   int32_t freeze_source_T4;         /* 1: T^4 taken once at the first step (SideTemperature.hpp:252) */
   double source_temperature;        /* K */
   uint64_t source_packets_per_face; /* Npercell of SideTemperature */
+  int32_t source_xhi;               /* 1: a second source on x+ entering along -x, same Npercell
+                                       (TwoSidesTemperature, boundary/TwoSidesTemperature.hpp:156-199) */
+  int32_t reserved;
+  double source_temperature_xhi;    /* K */
 } storm_boundary_desc;
 
 typedef struct storm_imc_params {

@@ -581,26 +581,35 @@ class FaceTableBoundary {
   }
 
   // SideTemperature::generateNewBoundaryParticles (boundary/SideTemperature.hpp:114-163)
+  // and, with source_xhi, the x+ side of TwoSidesTemperature
+  // (boundary/TwoSidesTemperature.hpp:156-199: T^4 of each side, mu = sqrt(U)
+  // along the inward normal).
   std::vector<Particle> generateNewBoundaryParticles(double fullDt) {
     std::vector<Particle> out;
-    if (desc_.source_face != STORM_XLO || desc_.source_packets_per_face == 0) return out;
-    double T = desc_.source_temperature;
-    double T4 = T * T * T * T;
+    const bool lo = desc_.source_face == STORM_XLO, hi = desc_.source_xhi != 0;
+    if ((!lo && !hi) || desc_.source_packets_per_face == 0) return out;
+    const double Ts[2] = {desc_.source_temperature, desc_.source_temperature_xhi};
+    double T4[2] = {Ts[0] * Ts[0] * Ts[0] * Ts[0], Ts[1] * Ts[1] * Ts[1] * Ts[1]};
     if (desc_.freeze_source_T4) {
       if (!frozen_) {
-        frozenT4_ = T4;
+        frozenT4_[0] = T4[0];
+        frozenT4_[1] = T4[1];
         frozen_ = true;
       }
-      T4 = frozenT4_;
+      T4[0] = frozenT4_[0];
+      T4[1] = frozenT4_[1];
     }
     size_t N = grid_.GetPointNo();
     size_t Npc = desc_.source_packets_per_face;
-    std::vector<double> Bcdf;
+    std::vector<double> Bcdf[2];
     if (!bounds_.empty()) {
       const size_t G = bounds_.size() - 1;
-      const double kT = units::k_boltz * T;
-      Bcdf.assign(G + 1, 0.0);
-      for (size_t g = 0; g < G; ++g) Bcdf[g + 1] = Bcdf[g] + storm::planck::integral(bounds_[g] / kT, bounds_[g + 1] / kT);
+      for (int s = 0; s < 2; ++s) {
+        const double kT = units::k_boltz * Ts[s];
+        Bcdf[s].assign(G + 1, 0.0);
+        for (size_t g = 0; g < G; ++g)
+          Bcdf[s][g + 1] = Bcdf[s][g] + storm::planck::integral(bounds_[g] / kT, bounds_[g + 1] / kT);
+      }
     }
     for (size_t i = 0; i < N; i++) {
       const Vec3 point = grid_.GetMeshPoint(i);
@@ -609,13 +618,16 @@ class FaceTableBoundary {
         size_t neighborIdx = (nb.first == i) ? nb.second : nb.first;
         if (neighborIdx >= N && grid_.IsPointOutsideBox(neighborIdx)) {
           Vec3 normal = normalize(grid_.GetMeshPoint(neighborIdx) - point);
-          if (normal.x < -0.99) {
-            double energyToProduce = units::sigma_sb * T4 * grid_.GetArea(faceIdx) * fullDt / Npc;
+          int side = -1;
+          if (normal.x < -0.99 && lo) side = 0;
+          if (normal.x > 0.99 && hi) side = 1;
+          if (side >= 0) {
+            double energyToProduce = units::sigma_sb * T4[side] * grid_.GetArea(faceIdx) * fullDt / Npc;
             for (size_t j = 0; j < Npc; j++) {
               Particle np;
               np.location = RandomPointOnFace(grid_, faceIdx, faceRe_);
               double mu = std::sqrt(re_.unit());
-              np.velocity.x = mu;
+              np.velocity.x = side == 0 ? mu : -mu;
               double _1mmu = std::sqrt(1 - mu * mu);
               double theta = 2 * M_PI * re_.unit();
               np.velocity.y = _1mmu * std::cos(theta);
@@ -627,9 +639,9 @@ class FaceTableBoundary {
                 // (examples/densmore2012/DensmoreBoundary.hpp:96-130): group by
                 // the Planck fractions at T, energy uniform in the group
                 const size_t G = bounds_.size() - 1;
-                double r = re_.unit() * Bcdf.back();
+                double r = re_.unit() * Bcdf[side].back();
                 size_t g = 0;
-                while (g + 1 < G && Bcdf[g + 1] < r) ++g;
+                while (g + 1 < G && Bcdf[side][g + 1] < r) ++g;
                 np.frequency = bounds_[g] + re_.unit() * (bounds_[g + 1] - bounds_[g]);
               }
               np.weight = energyToProduce;
@@ -684,7 +696,7 @@ class FaceTableBoundary {
   std::vector<double> bounds_;
   Rng re_, faceRe_;
   bool frozen_ = false;
-  double frozenT4_ = 0.0;
+  double frozenT4_[2] = {0.0, 0.0};
 };
 
 // ------------------------------------------------------------------ random walk tables

@@ -324,14 +324,25 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       if (!G) s1[c] = sa[c];
       Etot += E[c];
     }
-    const uint32_t npc =
-        p->boundary.source_face == STORM_XLO ? static_cast<uint32_t>(p->boundary.source_packets_per_face) : 0u;
+    // source sides in the engine's order: x- then x+
+    std::vector<double> sideT;
+    std::vector<int> sideHi;
+    if (p->boundary.source_face == STORM_XLO) {
+      sideT.push_back(p->boundary.source_temperature);
+      sideHi.push_back(0);
+    }
+    if (p->boundary.source_xhi) {
+      sideT.push_back(p->boundary.source_temperature_xhi);
+      sideHi.push_back(1);
+    }
+    const uint32_t npc = sideT.empty() ? 0u : static_cast<uint32_t>(p->boundary.source_packets_per_face);
+    const size_t per_side = static_cast<size_t>(ny) * nz;
     double src = 0.0;
-    const double Tb = p->boundary.source_temperature;
-    const double T4 = Tb * Tb * Tb * Tb;
-    std::vector<double> wface(npc ? static_cast<size_t>(ny) * nz : 0);
+    std::vector<double> wface(npc ? per_side * sideT.size() : 0);
     for (size_t fi = 0; fi < wface.size(); ++fi) {
-      int j = static_cast<int>(fi / nz), k = static_cast<int>(fi % nz);
+      const size_t s = fi / per_side, jk = fi % per_side;
+      const double T4 = sideT[s] * sideT[s] * sideT[s] * sideT[s];
+      int j = static_cast<int>(jk / nz), k = static_cast<int>(jk % nz);
       wface[fi] = units::sigma_sb * T4 * widthd(p->mesh.y_edges, j) * widthd(p->mesh.z_edges, k) * dt / npc;
       src += wface[fi] * npc;
     }
@@ -397,11 +408,12 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
         mv.spec.bounds = bk.data();
         mv.spec.cdf = cdfr.data();
         mv.spec.sig_s = ssr.data();
-        const double kT = units::k_boltz * Tb;
-        std::vector<double> acc(G + 1, 0.0);
-        for (int32_t g = 0; g < G; ++g) acc[g + 1] = acc[g] + planck::integral(gb[g] / kT, gb[g + 1] / kT);
-        bcdf.resize(G + 1);
-        for (int32_t g = 0; g <= G; ++g) bcdf[g] = acc[G] > 0.0 ? static_cast<Real>(acc[g] / acc[G]) : Real(0);
+        for (double Ts : sideT) {
+          const double kT = units::k_boltz * Ts;
+          std::vector<double> acc(G + 1, 0.0);
+          for (int32_t g = 0; g < G; ++g) acc[g + 1] = acc[g] + planck::integral(gb[g] / kT, gb[g + 1] / kT);
+          for (int32_t g = 0; g <= G; ++g) bcdf.push_back(acc[G] > 0.0 ? static_cast<Real>(acc[g] / acc[G]) : Real(0));
+        }
       }
       std::vector<CellAccel<Real>> acc_cells;
       std::unique_ptr<HostTables<Real>> tb;
@@ -433,8 +445,11 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       for (size_t fi = 0; fi < wface.size(); ++fi) {
         for (uint32_t q = 0; q < npc; ++q) {
           Packet<Real> pp;
-          birth_boundary<Real>(pp, id++, static_cast<uint32_t>(fi / nz), static_cast<uint32_t>(fi % nz), wface[fi],
-                               s0, p->seed, step, mv.spec, G ? bcdf.data() : nullptr);
+          const size_t s = fi / per_side, jk = fi % per_side;
+          const bool hi = sideHi[s] != 0;
+          birth_boundary<Real>(pp, id++, hi, hi ? static_cast<uint32_t>(nx - 1) : 0u, static_cast<uint32_t>(jk / nz),
+                               static_cast<uint32_t>(jk % nz), wface[fi], s0, p->seed, step, mv.spec,
+                               G ? bcdf.data() + s * static_cast<size_t>(G + 1) : nullptr);
           pk.push_back(pp);
         }
       }

@@ -38,7 +38,8 @@ class MaterialDesc(C.Structure):
 class BoundaryDesc(C.Structure):
     _fields_ = [("face_kind", C.c_int32 * 6), ("source_face", C.c_int32),
                 ("freeze_source_T4", C.c_int32), ("source_temperature", C.c_double),
-                ("source_packets_per_face", C.c_uint64)]
+                ("source_packets_per_face", C.c_uint64), ("source_xhi", C.c_int32), ("reserved", C.c_int32),
+                ("source_temperature_xhi", C.c_double)]
 
 
 class ImcParams(C.Structure):

@@ -50,15 +50,19 @@ STORM_HD void birth_emission(Packet<Real> &p, uint32_t id, uint32_t ijk, double 
   }
 }
 
-// `bcdf`: normalised Planck fractions of the source temperature over the
-// groups (groups + 1 entries), or null when gray.
+// `i` is the cell column of the face: 0 for the x- face (entering along
+// +x), nx - 1 for the x+ face of TwoSidesTemperature (entering along -x,
+// boundary/TwoSidesTemperature.hpp:176-180).  `bcdf`: normalised Planck
+// fractions of the source temperature over the groups (groups + 1
+// entries), or null when gray.
 template <typename Real>
-STORM_HD void birth_boundary(Packet<Real> &p, uint32_t id, uint32_t j, uint32_t k, double w, Real s0,
-                             uint64_t seed, uint32_t step, const SpectrumView<Real> &spec, const Real *bcdf) {
+STORM_HD void birth_boundary(Packet<Real> &p, uint32_t id, bool xhi, uint32_t i, uint32_t j, uint32_t k, double w,
+                             Real s0, uint64_t seed, uint32_t step, const SpectrumView<Real> &spec,
+                             const Real *bcdf) {
   using M = MathPolicy<Real>;
   PacketStream rs = PacketStream::make(seed, id, step, kTagBirth);
   Philox4 r0 = rs.block(0), r1 = rs.block(1);
-  p.u[0] = Real(0);
+  p.u[0] = xhi ? Real(1) : Real(0);
   p.u[1] = M::uniform(r0.v[0]);
   p.u[2] = M::uniform(r0.v[1]);
   Real mu = M::sqrt(M::uniform(r0.v[2]));
@@ -66,13 +70,13 @@ STORM_HD void birth_boundary(Packet<Real> &p, uint32_t id, uint32_t j, uint32_t 
   Real st = M::sqrt(st2 > Real(0) ? st2 : Real(0));
   Real sp, cp;
   M::sincos2pi(M::uniform(r0.v[3]), sp, cp);
-  p.o[0] = mu;
+  p.o[0] = xhi ? -mu : mu;
   p.o[1] = st * cp;
   p.o[2] = st * sp;
   p.s = s0 * M::uniform(r1.v[0]);
   p.w = w;
   p.wmin = static_cast<Real>(w * 1e-3);
-  p.ijk = pack_ijk(0u, j, k);
+  p.ijk = pack_ijk(i, j, k);
   p.id = id;
   p.ctr = 0u;
   p.flags = 0u;

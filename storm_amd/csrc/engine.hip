@@ -138,7 +138,8 @@ struct Engine final : EngineBase {
   uint64_t seed;
   int32_t precision, material_mode;
   bool frozen = false;
-  double frozenT4 = 0.0;
+  std::vector<double> frozenT4s;
+  double source_T_hi = 0.0;
   uint32_t step_index = 0;
   int32_t eq_prev = 0;  // quantum exponent of the resident census weights
   double census_total_prev = 0.0;  // erg
@@ -216,7 +217,9 @@ struct Engine final : EngineBase {
     // the FP64 engine is the plain form by default: one packet per thread
     schedule = precision == STORM_GPU_FP64 ? STORM_GPU_SCHEDULE_PASS : STORM_GPU_SCHEDULE_REFILL;
     if (bnd.source_face != -1 && bnd.source_face != STORM_XLO)
-      throw EngineError(1, "only an x- Planck source face is supported (SideTemperature)");
+      throw EngineError(1, "source_face must be -1 or STORM_XLO (an x+ source is source_xhi)");
+    if (bnd.source_xhi && !(bnd.source_temperature_xhi >= 0.0)) throw EngineError(1, "invalid x+ source temperature");
+    source_T_hi = bnd.source_temperature_xhi;
     if (!p->material.sigma_a0 && material_mode == STORM_GPU_MATERIAL_DEVICE)
       throw EngineError(1, "material.sigma_a0 is required");
     source_T = bnd.source_temperature;
@@ -512,23 +515,31 @@ struct Engine final : EngineBase {
     }
     const double Etot = deterministic_sum(Eemit.p, N);
 
-    // boundary source: SideTemperature energy per face (T^4 frozen at the first call if asked)
-    const uint32_t npc = (bnd.source_face == STORM_XLO) ? static_cast<uint32_t>(bnd.source_packets_per_face) : 0u;
-    const uint32_t nfaces = npc ? static_cast<uint32_t>(ny) * nz : 0u;
-    double T4 = source_T * source_T * source_T * source_T;
+    // boundary sources: SideTemperature / TwoSidesTemperature energy per face
+    // (T^4 frozen at the first call if asked), sides x- then x+
+    const uint32_t per_side = static_cast<uint32_t>(ny) * nz;
+    std::vector<double> side_T;
+    if (bnd.source_face == STORM_XLO) side_T.push_back(source_T);
+    const uint32_t xhi_from = static_cast<uint32_t>(side_T.size());
+    if (bnd.source_xhi) side_T.push_back(source_T_hi);
+    const uint32_t npc = side_T.empty() ? 0u : static_cast<uint32_t>(bnd.source_packets_per_face);
+    const uint32_t nfaces = npc ? per_side * static_cast<uint32_t>(side_T.size()) : 0u;
+    std::vector<double> side_T4(side_T.size());
+    for (size_t s = 0; s < side_T.size(); ++s) side_T4[s] = side_T[s] * side_T[s] * side_T[s] * side_T[s];
     if (bnd.freeze_source_T4) {
       if (!frozen) {
-        frozenT4 = T4;
+        frozenT4s = side_T4;
         frozen = true;
       }
-      T4 = frozenT4;
+      side_T4 = frozenT4s;
     }
     std::vector<double> wface_e(nfaces);
     double source_total = 0.0;
     for (uint32_t f = 0; f < nfaces; ++f) {
-      uint32_t j = f / nz, k = f % nz;
+      const uint32_t s = f / per_side, jk = f % per_side;
+      uint32_t j = jk / nz, k = jk % nz;
       double A = (ye[j + 1] - ye[j]) * (ze[k + 1] - ze[k]);
-      wface_e[f] = units::sigma_sb * T4 * A * dt / npc;
+      wface_e[f] = units::sigma_sb * side_T4[s] * A * dt / npc;
       source_total += wface_e[f] * npc;
     }
 
@@ -539,14 +550,18 @@ struct Engine final : EngineBase {
     counters.energy_quantum_exp = eq;
     for (uint32_t f = 0; f < nfaces; ++f) wface_e[f] = fx::quantize_down(wface_e[f], eq);
     if (groups >= 2 && nfaces) {
-      // Planck fractions of the source temperature per group
+      // Planck fractions of each source temperature per group
       // (examples/densmore2012/DensmoreBoundary.hpp:96-104)
-      const double kT = units::k_boltz * source_T;
-      std::vector<double> acc(groups + 1, 0.0);
-      for (int32_t g = 0; g < groups; ++g)
-        acc[g + 1] = acc[g] + planck::integral(bounds_h[g] / kT, bounds_h[g + 1] / kT);
-      std::vector<Real> h(groups + 1);
-      for (int32_t g = 0; g <= groups; ++g) h[g] = acc[groups] > 0.0 ? static_cast<Real>(acc[g] / acc[groups]) : Real(0);
+      std::vector<Real> h;
+      for (double Ts : side_T) {
+        const double kT = units::k_boltz * Ts;
+        std::vector<double> acc(groups + 1, 0.0);
+        for (int32_t g = 0; g < groups; ++g)
+          acc[g + 1] = acc[g] + planck::integral(bounds_h[g] / kT, bounds_h[g + 1] / kT);
+        for (int32_t g = 0; g <= groups; ++g)
+          h.push_back(acc[groups] > 0.0 ? static_cast<Real>(acc[g] / acc[groups]) : Real(0));
+      }
+      bcdf.ensure(h.size());
       HIPCHECK(hipMemcpyAsync(bcdf.p, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice, stream));
       HIPCHECK(hipStreamSynchronize(stream));
     }
@@ -606,7 +621,7 @@ struct Engine final : EngineBase {
     if (n_bc)
       k_boundary<Real><<<blocks_for(n_bc, 256), 256, 0, stream>>>(
           static_cast<uint32_t>(n_bc), npc, wface.p, pa.soa(), static_cast<uint32_t>(n_census + n_emit), s0, d, seed,
-          step_index, tal.begin, spectrum(), groups >= 2 ? bcdf.p : nullptr);
+          step_index, tal.begin, spectrum(), groups >= 2 ? bcdf.p : nullptr, xhi_from);
 
     StepConsts kc;
     kc.seed = seed;
@@ -917,9 +932,10 @@ struct Engine final : EngineBase {
     L->comb_rem = u128(sum(b + 6 * n2));
     // boundary source: Npc * w_face per face, exact
     unsigned __int128 src = 0;
-    const uint32_t npc = (bnd.source_face == STORM_XLO) ? static_cast<uint32_t>(bnd.source_packets_per_face) : 0u;
+    const uint32_t sides = (bnd.source_face == STORM_XLO ? 1u : 0u) + (bnd.source_xhi ? 1u : 0u);
+    const uint32_t npc = sides ? static_cast<uint32_t>(bnd.source_packets_per_face) : 0u;
     if (npc) {
-      std::vector<double> wf(static_cast<size_t>(ny) * nz);
+      std::vector<double> wf(static_cast<size_t>(ny) * nz * sides);
       HIPCHECK(hipMemcpy(wf.data(), wface.p, wf.size() * sizeof(double), hipMemcpyDeviceToHost));
       for (double w : wf) {
         fx::Limbs l = fx::limbs_of(w, last_eq_);

@@ -383,21 +383,32 @@ __global__ __launch_bounds__(256) void k_emit(uint32_t n_emit, uint32_t N, const
 // Planck surface source on the x- box face: per face Npc packets of weight
 // sigma_SB T^4 A dt / Npc, uniform on the face, mu = sqrt(U) about +x.
 template <typename Real>
+// Faces f = s * ny * nz + j * nz + k for the source sides s (0 .. nsides-1);
+// xhi_from: the first side index that is the x+ face (nsides if none);
+// bcdf holds groups + 1 entries per side.
 __global__ __launch_bounds__(256) void k_boundary(uint32_t n_bc, uint32_t npc, const double *__restrict__ wface,
                                                   PacketSoA<Real> out, uint32_t base, Real s0, MeshDims d,
                                                   uint64_t seed, uint32_t step, uint32_t *__restrict__ begin,
-                                                  SpectrumView<Real> sp, const Real *__restrict__ bcdf) {
+                                                  SpectrumView<Real> sp, const Real *__restrict__ bcdf,
+                                                  uint32_t xhi_from) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = q < n_bc;
-  const uint32_t f = valid ? q / npc : 0xFFFFFFFFu;  // face index j * nz + k = cell (0, j, k)
+  uint32_t cell = 0xFFFFFFFFu;
   if (valid) {
-    const uint32_t j = f / d.nz, k = f % d.nz;
+    const uint32_t f = q / npc;
+    const uint32_t per_side = d.ny * d.nz;
+    const uint32_t side = f / per_side, jk = f % per_side;
+    const uint32_t j = jk / d.nz, k = jk % d.nz;
+    const bool xhi = side >= xhi_from;
+    const uint32_t i = xhi ? d.nx - 1u : 0u;
+    cell = (i * d.ny + j) * d.nz + k;
     Packet<Real> p;
-    birth_boundary<Real>(p, base + q, j, k, wface[f], s0, seed, step, sp, bcdf);
+    birth_boundary<Real>(p, base + q, xhi, i, j, k, wface[f], s0, seed, step, sp,
+                         bcdf ? bcdf + static_cast<size_t>(side) * static_cast<size_t>(sp.groups + 1) : nullptr);
     store_packet(out, base + q, p);
   }
   uint32_t one = valid ? 1u : 0u;
-  if (wave_run_sum(f, one) && valid) atomicAdd(&begin[f], one);
+  if (wave_run_sum(cell, one) && valid) atomicAdd(&begin[cell], one);
 }
 
 // ------------------------------------------------------------------ transport

@@ -69,6 +69,8 @@ class ProblemSpec:
     face_kind: tuple = (capi.FACE_ESCAPE,) + (capi.FACE_REFLECT,) * 5
     source_face: int = capi.XLO
     source_T: float = KEV_K
+    source_xhi: int = 0          # a second source on x+ (TwoSidesTemperature)
+    source_T_hi: float = 0.0
     freeze_T4: int = 1
     source_per_face: int = 100
     new_photons_per_cell: int = 10
@@ -139,6 +141,8 @@ class ProblemSpec:
         b.freeze_source_T4 = int(self.freeze_T4)
         b.source_temperature = float(self.source_T)
         b.source_packets_per_face = int(self.source_per_face)
+        b.source_xhi = int(self.source_xhi)
+        b.source_temperature_xhi = float(self.source_T_hi)
         p.boundary = b
         q = capi.ImcParams()
         q.new_photons_per_cell = int(self.new_photons_per_cell)

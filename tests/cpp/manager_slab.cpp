@@ -6,6 +6,7 @@
 //
 // usage: manager_slab <precision 0|1> <resident 0|1> <steps> <seed0> <nseeds>
 //                     <nx> <boundary packets> <photons per cell> <comb Nmin>
+//                     [<x+ temperature, keV>: TwoSidesTemperature instead]
 // Prints one JSON object: per seed the cell temperatures (K), time-averaged
 // Erad (erg/cm^3) and census Erad (erg), plus the manager's counters.
 #include <cstdio>
@@ -61,10 +62,12 @@ void print_array(const char *name, const std::vector<double> &a, bool last = fal
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc != 10) {
-    std::fprintf(stderr, "usage: %s precision resident steps seed0 nseeds nx npc nphot nmin\n", argv[0]);
+  if (argc != 10 && argc != 11) {
+    std::fprintf(stderr, "usage: %s precision resident steps seed0 nseeds nx npc nphot nmin [T_right_keV]\n", argv[0]);
     return 2;
   }
+  const bool two_sides = argc == 11;
+  const double T_right = two_sides ? std::atof(argv[10]) * kKevK : 0.0;
   const int precision = std::atoi(argv[1]);
   const bool resident = std::atoi(argv[2]) != 0;
   const int steps = std::atoi(argv[3]);
@@ -91,7 +94,11 @@ int main(int argc, char **argv) {
       }
       STORM::RadiationIMCParameters<1> params;
       params.newPhotonsPerCell = nphot;
-      auto boundary = std::make_shared<STORM::SideTemperature<Vec3, Grid>>(grid, kKevK, npc);
+      std::shared_ptr<STORM::BoundaryCondition<Vec3, Grid>> boundary;
+      if (two_sides)
+        boundary = std::make_shared<STORM::TwoSidesTemperature<Vec3, Grid>>(grid, kKevK, T_right, npc);
+      else
+        boundary = std::make_shared<STORM::SideTemperature<Vec3, Grid>>(grid, kKevK, npc);
       auto physics = std::make_shared<IMC>(grid, boundary, cells, ext, std::make_shared<ConstantCvEOS>(),
                                            std::make_shared<UnitOpacity>(), params,
                                            static_cast<std::uint64_t>(seed0 + s));

@@ -328,6 +328,8 @@ def _run_manager(spec, precision, resident, seeds):
     MonteCarloManagerGPU.hpp) on the slab `spec` for each seed."""
     args = [MANAGER_EXE, str(precision), str(int(resident)), str(spec.steps), str(seeds[0]), str(len(seeds)),
             str(spec.shape[0]), str(spec.source_per_face), str(spec.new_photons_per_cell), str(spec.comb_nmin)]
+    if spec.source_xhi:
+        args.append(repr(spec.source_T_hi / P.KEV_K))
     print("manager:", " ".join(args), flush=True)
     out = subprocess.run(args, capture_output=True, text=True, timeout=240)
     assert out.returncode == 0, out.stderr
@@ -361,3 +363,26 @@ def test_cpp_manager_resident_is_the_engine():
     assert np.allclose(got["T"][0], c["T"], rtol=1e-12, atol=0)
     assert np.allclose(got["erad"][0], c["erad"], rtol=1e-12, atol=0)
     assert np.allclose(got["census"][0], c["census"], rtol=1e-12, atol=0)
+
+
+def C1_TWO(s):
+    """C1 slab with TwoSidesTemperature: 1 keV on x-, 0.5 keV on x+."""
+    spec = C1(s)
+    spec.source_xhi, spec.source_T_hi = 1, 0.5 * P.KEV_K
+    spec.face_kind = (capi.FACE_ESCAPE, capi.FACE_ESCAPE) + (capi.FACE_REFLECT,) * 4
+    return spec
+
+
+@pytest.fixture(scope="module")
+def c1_two_ref(oracle):
+    return oracle_ensemble(oracle, C1_TWO, range(1000, 1000 + NSEEDS))
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
+def test_two_sided_source_matches_oracle(c1_two_ref, precision):
+    """Planck sources on both x faces (TwoSidesTemperature), through the engine
+    and through the C++ manager with the TwoSidesTemperature plugin."""
+    got = gpu_ensemble(C1_TWO, range(NSEEDS), precision)
+    _assert_parity(c1_two_ref, got, f"c1-two/{precision}")
+    got_cpp, _ = _run_manager(C1_TWO(0), precision, False, list(range(NSEEDS)))
+    _assert_parity(c1_two_ref, got_cpp, f"c1-two-manager/{precision}")

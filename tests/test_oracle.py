@@ -175,8 +175,16 @@ def _densmore_small(s):
     return spec
 
 
+def _two_sides(s):
+    # C1 slab with TwoSidesTemperature: 1 keV on x-, 0.5 keV on x+, both x faces escape
+    spec = P.c1_slab(seed=s, packets_scale=0.05, steps=1)
+    spec.source_xhi, spec.source_T_hi = 1, 0.5 * P.KEV_K
+    spec.face_kind = (capi.FACE_ESCAPE, capi.FACE_ESCAPE) + (capi.FACE_REFLECT,) * 4
+    return spec
+
+
 @pytest.mark.parametrize("precision", [0, 1], ids=["mixed", "fp64"])
-@pytest.mark.parametrize("which", ["c1", "c2", "multigroup"])
+@pytest.mark.parametrize("which", ["c1", "c2", "multigroup", "two_sides"])
 def test_engine_first_step_physics_on_cpu(oracle, precision, which):
     """The engine's shared sourcing + transport code, run on the CPU, agrees
     with the oracle's first step (per-cell Erad time average and segments)."""
@@ -184,6 +192,8 @@ def test_engine_first_step_physics_on_cpu(oracle, precision, which):
         make = lambda s: P.c1_slab(seed=s, packets_scale=0.05, steps=1)  # noqa: E731
     elif which == "multigroup":
         make = _densmore_small
+    elif which == "two_sides":
+        make = _two_sides
     else:
         make = lambda s: P.c2_cube(seed=7, n=8, packets_scale=0.2, steps=1).with_seed(s)  # noqa: E731
     o_erad, o_seg, e_erad, e_seg = [], [], [], []
