@@ -3,8 +3,11 @@
 //
 // Vendor libm / device math libraries differ between host and device, so the
 // mixed-precision engine uses only these routines, built from IEEE-754 +, -,
-// *, / and sqrt (correctly rounded on both sides; every translation unit that
-// includes this header is compiled with -ffp-contract=off so no FMA is formed).
+// *, /, sqrt and fused multiply-add, all correctly rounded on both sides.
+// Every translation unit that includes this header is compiled with
+// -ffp-contract=off, so the compiler forms no FMA of its own; the polynomial
+// steps below ask for one explicitly (fma_f: v_fma_f32 on the device, the
+// correctly rounded fmaf of the C library on the host).
 // Methods:
 //   log   : exponent/mantissa split with m in [sqrt(1/2), sqrt(2)), then
 //           log(m) = 2 atanh(s), s = (m-1)/(m+1), atanh series to s^9;
@@ -19,6 +22,7 @@
 // host and device produce the same bits.
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
 #include <string.h>
 #include "hd.h"
@@ -47,6 +51,15 @@ STORM_HD double u2d(uint64_t u) {
   return x;
 }
 
+// a * b + c with one rounding
+STORM_HD float fma_f(float a, float b, float c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_fmaf(a, b, c);
+#else
+  return ::fmaf(a, b, c);
+#endif
+}
+
 constexpr float kLn2Hi = 0.693115234375f;          // 12 significant bits
 constexpr float kLn2Lo = 3.194618329871446e-05f;   // ln2 - kLn2Hi
 constexpr float kLog2e = 1.4426950216293335f;
@@ -73,13 +86,13 @@ STORM_HD float logf_d(float x) {
   float s = f / (2.0f + f);
   float z = s * s;
   float p = 1.0f / 9.0f;
-  p = 1.0f / 7.0f + z * p;
-  p = 1.0f / 5.0f + z * p;
-  p = 1.0f / 3.0f + z * p;
-  p = 1.0f + z * p;
+  p = fma_f(z, p, 1.0f / 7.0f);
+  p = fma_f(z, p, 1.0f / 5.0f);
+  p = fma_f(z, p, 1.0f / 3.0f);
+  p = fma_f(z, p, 1.0f);
   float lm = 2.0f * s * p;
   float ef = static_cast<float>(e);
-  return ef * kLn2Hi + (lm + ef * kLn2Lo);
+  return fma_f(ef, kLn2Hi, fma_f(ef, kLn2Lo, lm));
 }
 
 // e^y.  Flushes to 0 below -87 and saturates to +inf above 88.
@@ -89,16 +102,16 @@ STORM_HD float expf_d(float y) {
   float kf = y * kLog2e;
   int k = static_cast<int>(kf + (kf >= 0.0f ? 0.5f : -0.5f));
   float fk = static_cast<float>(k);
-  float r = (y - fk * kLn2Hi) - fk * kLn2Lo;
+  float r = fma_f(-fk, kLn2Lo, fma_f(-fk, kLn2Hi, y));
   float p = 1.0f / 40320.0f;
-  p = 1.0f / 5040.0f + r * p;
-  p = 1.0f / 720.0f + r * p;
-  p = 1.0f / 120.0f + r * p;
-  p = 1.0f / 24.0f + r * p;
-  p = 1.0f / 6.0f + r * p;
-  p = 0.5f + r * p;
-  p = 1.0f + r * p;
-  p = 1.0f + r * p;
+  p = fma_f(r, p, 1.0f / 5040.0f);
+  p = fma_f(r, p, 1.0f / 720.0f);
+  p = fma_f(r, p, 1.0f / 120.0f);
+  p = fma_f(r, p, 1.0f / 24.0f);
+  p = fma_f(r, p, 1.0f / 6.0f);
+  p = fma_f(r, p, 0.5f);
+  p = fma_f(r, p, 1.0f);
+  p = fma_f(r, p, 1.0f);
   // scale by 2^k in two halves so that k down to -126-ish stays representable
   int k1 = k / 2;
   int k2 = k - k1;
@@ -112,14 +125,14 @@ STORM_HD float expm1f_d(float y) {
   float ay = y < 0.0f ? -y : y;
   if (ay < 0.34657359f) {
     float p = 1.0f / 362880.0f;
-    p = 1.0f / 40320.0f + y * p;
-    p = 1.0f / 5040.0f + y * p;
-    p = 1.0f / 720.0f + y * p;
-    p = 1.0f / 120.0f + y * p;
-    p = 1.0f / 24.0f + y * p;
-    p = 1.0f / 6.0f + y * p;
-    p = 0.5f + y * p;
-    return y + (y * y) * p;
+    p = fma_f(y, p, 1.0f / 40320.0f);
+    p = fma_f(y, p, 1.0f / 5040.0f);
+    p = fma_f(y, p, 1.0f / 720.0f);
+    p = fma_f(y, p, 1.0f / 120.0f);
+    p = fma_f(y, p, 1.0f / 24.0f);
+    p = fma_f(y, p, 1.0f / 6.0f);
+    p = fma_f(y, p, 0.5f);
+    return fma_f(y * y, p, y);
   }
   return expf_d(y) - 1.0f;
 }
@@ -132,16 +145,16 @@ STORM_HD void sincos2pi(float u, float &s_out, float &c_out) {
   float a = t * kHalfPi;
   float a2 = a * a;
   float sp = 1.0f / 362880.0f;
-  sp = -1.0f / 5040.0f + a2 * sp;
-  sp = 1.0f / 120.0f + a2 * sp;
-  sp = -1.0f / 6.0f + a2 * sp;
-  float s = a + (a * a2) * sp;
+  sp = fma_f(a2, sp, -1.0f / 5040.0f);
+  sp = fma_f(a2, sp, 1.0f / 120.0f);
+  sp = fma_f(a2, sp, -1.0f / 6.0f);
+  float s = fma_f(a * a2, sp, a);
   float cp = -1.0f / 3628800.0f;
-  cp = 1.0f / 40320.0f + a2 * cp;
-  cp = -1.0f / 720.0f + a2 * cp;
-  cp = 1.0f / 24.0f + a2 * cp;
-  cp = -0.5f + a2 * cp;
-  float c = 1.0f + a2 * cp;
+  cp = fma_f(a2, cp, 1.0f / 40320.0f);
+  cp = fma_f(a2, cp, -1.0f / 720.0f);
+  cp = fma_f(a2, cp, 1.0f / 24.0f);
+  cp = fma_f(a2, cp, -0.5f);
+  float c = fma_f(a2, cp, 1.0f);
   switch (q & 3) {
     case 0: s_out = s; c_out = c; break;
     case 1: s_out = c; c_out = -s; break;

@@ -197,9 +197,11 @@ struct Tracker {
 
   PacketStream rs;
   Real u0, u1, u2, o0, o1, o2, s, wmin, nu;
+  Real io0, io1, io2;  // 1/|o| per axis (+inf along an axis the packet does not move on)
   // the current cell's data, loaded when the packet enters it (and, in
-  // multigroup, when its energy changes)
-  Real cell_sig_abs, cell_sig_ev, cell_sig_s, cw0, cw1, cw2;
+  // multigroup, when its energy changes); reciprocals so that a segment
+  // multiplies instead of dividing
+  Real cell_sig_abs, cell_sig_ev, cell_sig_s, inv_sig_ev, cw0, cw1, cw2, icw0, icw1, icw2;
   double w, w_enter, path_int;  // w_enter: weight when the current cell visit began
   uint32_t ijk, ctr, cell, visit_segs, id;
   int outcome;
@@ -210,9 +212,7 @@ struct Tracker {
     u0 = p.u[0];
     u1 = p.u[1];
     u2 = p.u[2];
-    o0 = p.o[0];
-    o1 = p.o[1];
-    o2 = p.o[2];
+    set_dir(p.o[0], p.o[1], p.o[2]);
     s = p.s;
     w = p.w;
     ijk = p.ijk;
@@ -296,7 +296,19 @@ struct Tracker {
     cw0 = m.width[0][ijk_i(ijk)];
     cw1 = m.width[1][ijk_j(ijk)];
     cw2 = m.width[2][ijk_k(ijk)];
+    icw0 = Real(1) / cw0;
+    icw1 = Real(1) / cw1;
+    icw2 = Real(1) / cw2;
     load_opacity(m, xs);
+  }
+
+  STORM_HD void set_dir(Real a, Real b, Real c) {
+    o0 = a;
+    o1 = b;
+    o2 = c;
+    io0 = a != Real(0) ? Real(1) / (a > Real(0) ? a : -a) : M::inf();
+    io1 = b != Real(0) ? Real(1) / (b > Real(0) ? b : -b) : M::inf();
+    io2 = c != Real(0) ? Real(1) / (c > Real(0) ? c : -c) : M::inf();
   }
 
   // opacities of the current cell at the packet's energy
@@ -313,6 +325,7 @@ struct Tracker {
       cell_sig_abs = x.sig_abs * fac;
       cell_sig_ev = cell_sig_s + x.sig_ev * fac;
     }
+    inv_sig_ev = cell_sig_ev > Real(0) ? Real(1) / cell_sig_ev : Real(0);
     derive_pint = derives_path_integral(m);
   }
 
@@ -413,26 +426,24 @@ struct Tracker {
           if (disp > Ro) disp = Ro;
           const Real c0 = u0, c1 = u1, c2 = u2;
           // nudge toward the cell centre (RadiationIMC.hpp:1214-1215)
-          Real n0 = (c0 + disp * dir[0] / w0w) * Real(1 - 1e-6) + Real(1e-6) * Real(0.5);
-          Real n1 = (c1 + disp * dir[1] / w1w) * Real(1 - 1e-6) + Real(1e-6) * Real(0.5);
-          Real n2 = (c2 + disp * dir[2] / w2w) * Real(1 - 1e-6) + Real(1e-6) * Real(0.5);
+          Real n0 = (c0 + disp * dir[0] * icw0) * Real(1 - 1e-6) + Real(1e-6) * Real(0.5);
+          Real n1 = (c1 + disp * dir[1] * icw1) * Real(1 - 1e-6) + Real(1e-6) * Real(0.5);
+          Real n2 = (c2 + disp * dir[2] * icw2) * Real(1 - 1e-6) + Real(1e-6) * Real(0.5);
           for (int guard = 0; guard < 2000; ++guard) {
             bool inside = n0 >= Real(0) && n0 <= Real(1) && n1 >= Real(0) && n1 <= Real(1) && n2 >= Real(0) &&
                           n2 <= Real(1);
             if (inside) break;
             disp = disp * Real(0.99);
-            n0 = c0 + disp * dir[0] / w0w;
-            n1 = c1 + disp * dir[1] / w1w;
-            n2 = c2 + disp * dir[2] / w2w;
+            n0 = c0 + disp * dir[0] * icw0;
+            n1 = c1 + disp * dir[1] * icw1;
+            n2 = c2 + disp * dir[2] * icw2;
           }
           u0 = n0 < Real(0) ? Real(0) : (n0 > Real(1) ? Real(1) : n0);
           u1 = n1 < Real(0) ? Real(0) : (n1 > Real(1) ? Real(1) : n1);
           u2 = n2 < Real(0) ? Real(0) : (n2 > Real(1) ? Real(1) : n2);
           Real nd[3];
           isotropic_dir<Real>(M::uniform(r2.v[0]), M::uniform(r2.v[1]), nd);
-          o0 = nd[0];
-          o1 = nd[1];
-          o2 = nd[2];
+          set_dir(nd[0], nd[1], nd[2]);
           if (!leak) census(k, sink);
           return outcome;
         }
@@ -477,17 +488,11 @@ struct Tracker {
           M::sincos2pi(M::uniform(r.v[3]), sp, cp);
           const Real sg = up ? Real(1) : Real(-1);
           if (fax == 0) {          // e1 = y, e2 = n x y = sg z
-            o0 = sg * mu;
-            o1 = st * cp;
-            o2 = sg * st * sp;
+            set_dir(sg * mu, st * cp, sg * st * sp);
           } else if (fax == 1) {   // e1 = x, e2 = n x x = -sg z
-            o0 = st * cp;
-            o1 = sg * mu;
-            o2 = -sg * st * sp;
+            set_dir(st * cp, sg * mu, -sg * st * sp);
           } else {                 // e1 = x, e2 = n x x = sg y
-            o0 = st * cp;
-            o1 = sg * st * sp;
-            o2 = sg * mu;
+            set_dir(st * cp, sg * st * sp, sg * mu);
           }
           // placed at the face centroid (:1588), in the neighbour
           u0 = Real(0.5);
@@ -502,12 +507,13 @@ struct Tracker {
     // ---------------- analytic step
     // Distance to the nearest face along the direction (+inf along an axis
     // the packet does not move on); ties go to the lower axis.
-    auto face_dist = [](Real ua, Real oa, Real wa) {
+    auto face_dist = [](Real ua, Real oa, Real wa, Real ioa) {
       Real frac = oa > Real(0) ? (Real(1) - ua) : ua;
-      Real d = frac * wa / (oa > Real(0) ? oa : -oa);
+      Real d = frac * wa * ioa;
       return oa != Real(0) ? d : M::inf();
     };
-    const Real d0 = face_dist(u0, o0, w0w), d1 = face_dist(u1, o1, w1w), d2 = face_dist(u2, o2, w2w);
+    const Real d0 = face_dist(u0, o0, w0w, io0), d1 = face_dist(u1, o1, w1w, io1),
+               d2 = face_dist(u2, o2, w2w, io2);
     int fax = 0, sax = 1;
     Real dface = d0, dsecond = d1;
     if (d1 < d0) {
@@ -544,7 +550,7 @@ struct Tracker {
     }
     // Distance to the next effective-scatter event (-ln xi / sigma_ev).
     Real xi = M::uniform(r.v[0]);
-    Real dcol = cell_sig_ev > Real(0) ? (-M::log(xi)) / cell_sig_ev : M::inf();
+    Real dcol = cell_sig_ev > Real(0) ? (-M::log(xi)) * inv_sig_ev : M::inf();
     // Event: face < scatter < census on ties, as the reference's min_element.
     int ev = 0;
     Real d = dface;
@@ -563,7 +569,7 @@ struct Tracker {
 
     // Move (the face axis is set exactly below).
     {
-      Real n0 = u0 + o0 * d / w0w, n1 = u1 + o1 * d / w1w, n2 = u2 + o2 * d / w2w;
+      Real n0 = u0 + o0 * d * icw0, n1 = u1 + o1 * d * icw1, n2 = u2 + o2 * d * icw2;
       n0 = n0 < Real(0) ? Real(0) : (n0 > Real(1) ? Real(1) : n0);
       n1 = n1 < Real(0) ? Real(0) : (n1 > Real(1) ? Real(1) : n1);
       n2 = n2 < Real(0) ? Real(0) : (n2 > Real(1) ? Real(1) : n2);
@@ -606,9 +612,7 @@ struct Tracker {
       // scatter: isotropic redirection (RadiationIMC.hpp:1827-1874)
       Real nd[3];
       isotropic_dir<Real>(M::uniform(r.v[2]), M::uniform(r.v[3]), nd);
-      o0 = nd[0];
-      o1 = nd[1];
-      o2 = nd[2];
+      set_dir(nd[0], nd[1], nd[2]);
       if ((F & kFeatMultigroup) != 0u && m.spec.groups >= 2) {
         // an effective (absorption + re-emission) scatter takes a new energy
         // from the cell's emission spectrum (:1843-1852); an elastic one
