@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define STORM_GPU_ABI_VERSION 3
+#define STORM_GPU_ABI_VERSION 4
 
 typedef struct storm_gpu_engine storm_gpu_engine;
 
@@ -75,6 +75,8 @@ typedef struct storm_gpu_counters {
   double seconds_step;         /* host wall time of the last storm_gpu_step          */
   double seconds_transport;    /* GPU time of the transport kernels (HIP events)     */
   double seconds_transport_max_launch; /* longest single transport launch              */
+  uint64_t max_packet_blocks;  /* most random-number blocks one packet used this step
+                                  (about its segments): the longest history */
 } storm_gpu_counters;
 
 /* Exact energy ledger of the last step, in units of its quantum 2^eq erg

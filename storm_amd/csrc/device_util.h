@@ -47,6 +47,15 @@ __device__ __forceinline__ bool wave_run_sum(uint32_t key, T &v) {
   return lane == 63u || next != key;
 }
 
+// Maximum over the wavefront's lanes, one atomic per wavefront.
+__device__ __forceinline__ void wave_max_u32(uint32_t v, unsigned long long *dst) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  if (v && lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u) atomicMax(dst, static_cast<unsigned long long>(v));
+}
+
 // Integer sums are order independent: per-block LDS reduction, one atomic.
 template <int BLOCK>
 __device__ __forceinline__ void block_atomic_add_u64(uint64_t v, unsigned long long *dst) {

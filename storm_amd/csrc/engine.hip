@@ -272,7 +272,7 @@ struct Engine final : EngineBase {
     emit_limbs.ensure(2 * static_cast<size_t>(N));
     W.ensure(2 * static_cast<size_t>(N));
     requant.ensure(2 * static_cast<size_t>(N));
-    leaving.ensure(4);  // escapes, random-walk steps, DDMC steps, segments
+    leaving.ensure(5);  // escapes, random-walk steps, DDMC steps, segments, longest history
     scal.ensure(4);     // total census weight (low, high limb)
     dsc.ensure(2);      // material energy before the step, census energy after it
     counters_u32.ensure(4);
@@ -605,7 +605,7 @@ struct Engine final : EngineBase {
     if (precision == STORM_GPU_FP64) zero_f64(tally_f64.p, tally_f64.n);
     zero_u32(tal.segs, 2 * static_cast<size_t>(N));
     zero_u64(requant.p, requant.n);
-    zero_u64(leaving.p, 4);
+    zero_u64(leaving.p, 5);
 
     // packets: [census | emission | boundary]
     pa.ensure(n_total);
@@ -683,14 +683,15 @@ struct Engine final : EngineBase {
     sum_async(census_erad.p, N, dsc.p + 1);
     k_total_u32<<<std::min<uint32_t>(blocks_for(N, 256), 1024u), 256, 0, stream>>>(N, tal.segs, leaving.p + 3);
     uint32_t herr = 0;
-    unsigned long long hstats[4] = {0, 0, 0, 0};
+    unsigned long long hstats[5] = {0, 0, 0, 0, 0};
     double hcensus = 0.0;
     HIPCHECK(hipMemcpyAsync(&hcensus, dsc.p + 1, sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipMemcpyAsync(&herr, err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipMemcpyAsync(hstats, leaving.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipMemcpyAsync(hstats, leaving.p, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     census_total_prev = hcensus;
     counters.segments = hstats[3];
+    counters.max_packet_blocks = hstats[4];
     counters.leaving_particles = hstats[0];
     counters.random_walk_steps = hstats[1];
     counters.ddmc_steps = hstats[2];

@@ -472,6 +472,7 @@ __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real
     tally_limbs(t.cpre, c, fx::limbs_of(p.w, k.eq));
   }
   wave_add_u32((valid && outcome == kEscaped) ? 1u : 0u, &stats[0]);
+  wave_max_u32(valid ? p.ctr : 0u, &stats[4]);
   if (acc.cell) {
     wave_add_u32(sink.rw, &stats[1]);
     wave_add_u32(sink.ddmc, &stats[2]);
@@ -506,7 +507,7 @@ __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketS
   bool drained = false;
   uint32_t pool_next = 0, pool_end = 0;
   int32_t nseg = 0;
-  uint32_t escapes = 0;
+  uint32_t escapes = 0, longest = 0;
   for (;;) {
     if (!drained) {
       const uint64_t idle = __builtin_amdgcn_ballot_w64(!busy);
@@ -552,6 +553,7 @@ __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketS
     const bool census = fin && outcome == kCensus;
     const bool survivor = fin && outcome == kAlive;
     escapes += (fin && outcome == kEscaped) ? 1u : 0u;
+    if (fin && p.ctr > longest) longest = p.ctr;
     const uint32_t cslot = wave_append(census, n_cen);
     if (census) {
       store_census(cen, cslot, p);
@@ -561,6 +563,7 @@ __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketS
     if (survivor) store_packet(out, oslot, p);
   }
   wave_add_u32(escapes, &stats[0]);
+  wave_max_u32(longest, &stats[4]);
   if (acc.cell) {
     wave_add_u32(sink.rw, &stats[1]);
     wave_add_u32(sink.ddmc, &stats[2]);
