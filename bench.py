@@ -10,10 +10,14 @@ segments/s of the mixed-precision engine.  The plain FP64 engine runs the
 same W + K steps for comparison, and the FP64 CPU oracle is timed on a
 bounded sample of the same workload (`cpu_baseline`).
 
-Roofline: the transport kernel is HBM-bound by construction (no MFMA; per
-segment ~100 FP32 ops against 144 algorithmic bytes).  achieved = segments of
-a launch * 144 B / launch duration (HIP events on the engine's stream),
-averaged over the timed launches.
+Roofline: no MFMA (nothing is a dense contraction), so the roofline is HBM
+bandwidth against the algorithmic bytes of the metric's model: 144 B per
+segment (a 48 B packet read and written, a 16 B cell record, two 16 B
+tallies).  achieved = segments * 144 B / transport time (HIP events on the
+engine's stream around each launch), over the timed steps.  The history
+kernel keeps a packet in registers for its whole history, so its measured
+HBM traffic per segment is far below the model (rocprof FETCH_SIZE /
+WRITE_SIZE).
 """
 from __future__ import annotations
 
@@ -32,7 +36,7 @@ B_SEG_FP64 = 224.0   # 2 * 80 B packet + 32 B cell + 2 * 16 B tally
 HBM_PEAK_GBS = 8000.0
 
 
-def run_engine(precision, steps, warmup, seed, log, schedule=None):
+def run_engine(precision, steps, warmup, seed, log, schedule=None, copy_mib=0):
     from storm_amd import problems as P
     from storm_amd.engine import GpuEngine
 
@@ -88,6 +92,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--schedule", choices=("pass", "refill"), default="pass",
+                    help="transport schedule of the mixed engine")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the run, one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -103,8 +111,9 @@ def main():
     from storm_amd.engine import device_info
 
     log("device: " + device_info())
-    log("mixed engine (refill schedule)")
-    mixed = run_engine(capi.MIXED, args.steps, args.warmup, args.seed, log)
+    sched = capi.SCHEDULE_PASS if args.schedule == "pass" else capi.SCHEDULE_REFILL
+    log(f"mixed engine ({args.schedule} schedule)")
+    mixed = run_engine(capi.MIXED, args.steps, args.warmup, args.seed, log, schedule=sched)
     value = mixed["segments"] / mixed["wall"]
     ms_per_step = mixed["wall"] / args.steps * 1e3
     achieved = mixed["segments"] * B_SEG_MIXED / mixed["transport_s"] / 1e9
@@ -136,7 +145,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_transport_refill<float>",
+            "kernel": "k_transport<float, 0u>" if args.schedule == "pass" else "k_transport_refill<float, 0u>",
             "bytes_per_segment": B_SEG_MIXED,
         },
     }
@@ -150,17 +159,24 @@ def main():
             "form": "plain: FP64 tracking and tallies, one packet per thread",
         }
         out["speedup_vs_fp64"] = value / out["fp64_gpu"]["value"]
-        log("FP64 engine on the refill schedule")
-        f64r = run_engine(capi.FP64, args.steps, args.warmup, args.seed, log, schedule=capi.SCHEDULE_REFILL)
-        out["fp64_gpu_refill"] = {
-            "value": f64r["segments"] / f64r["wall"],
-            "ms_per_step": f64r["wall"] / args.steps * 1e3,
-            "transport_segments_per_s": f64r["segments"] / f64r["transport_s"],
-        }
+
     out["transport_segments_per_s"] = mixed["segments"] / mixed["transport_s"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(log)
+    if args.copy_mib > 0:
+        copy_calibration(args.copy_mib, log)
     print(json.dumps(out))
+
+
+def copy_calibration(mib, log):
+    """One device-to-device copy of `mib` MiB through torch (a plain copy
+    kernel of known traffic, to calibrate the HBM byte counters)."""
+    import torch
+    a = torch.empty(mib << 20, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {mib} MiB device to device")
 
 
 if __name__ == "__main__":

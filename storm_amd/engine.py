@@ -68,10 +68,9 @@ class GpuEngine:
         self._check(self.lib.storm_gpu_set_source_temperature(self.h, float(T)))
 
     def set_tuning(self, block: int = 256, segments_per_pass: int = 1 << 20, schedule: int | None = None):
-        """Transport launch shape; `schedule` is capi.SCHEDULE_PASS or capi.SCHEDULE_REFILL
-        (default: refill for the mixed engine, pass for FP64)."""
+        """Transport launch shape; `schedule` is capi.SCHEDULE_PASS (default) or capi.SCHEDULE_REFILL."""
         if schedule is None:
-            schedule = capi.SCHEDULE_PASS if self.precision == capi.FP64 else capi.SCHEDULE_REFILL
+            schedule = capi.SCHEDULE_PASS
         self._check(self.lib.storm_gpu_set_tuning(self.h, int(block), int(segments_per_pass), int(schedule)))
 
     def set_cell_state(self, T=None, sigma_a=None, sigma_s=None, cv=None, e=None):
