@@ -95,7 +95,7 @@ def main():
     ap.add_argument("--schedule", choices=("pass", "refill"), default="pass",
                     help="transport schedule of the mixed engine")
     ap.add_argument("--copy-mib", type=int, default=0,
-                    help="after the run, one device-to-device copy of this size (counter calibration)")
+                    help="after the run, one plain copy kernel over this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -169,14 +169,12 @@ def main():
 
 
 def copy_calibration(mib, log):
-    """One device-to-device copy of `mib` MiB through torch (a plain copy
-    kernel of known traffic, to calibrate the HBM byte counters)."""
-    import torch
-    a = torch.empty(mib << 20, dtype=torch.uint8, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    log(f"copied {mib} MiB device to device")
+    """One run of the library's plain copy kernel over `mib` MiB (reads and
+    writes each byte once: a kernel of known traffic, to scale the HBM byte
+    counters of a profiler run)."""
+    from storm_amd.engine import calibration_copy
+    ms = calibration_copy(mib << 20)
+    log(f"calibration copy: {mib} MiB in, {mib} MiB out, {ms:.3f} ms ({2 * (mib << 20) / ms / 1e6:.0f} GB/s)")
 
 
 if __name__ == "__main__":

@@ -194,6 +194,13 @@ int storm_gpu_debug_accel(storm_gpu_engine *eng, void *out, uint64_t *bytes);
 int storm_gpu_debug_spectrum(storm_gpu_engine *eng, int32_t *groups, double *nu_pow, void *bounds_kev, void *cdf,
                              void *sig_s);
 
+/* Measurement aid with no counterpart in the reference: one plain copy of
+ * `bytes` (a multiple of 16) between two fresh device buffers by a kernel
+ * that reads and writes each byte once (k_copy_calibration), so that a
+ * profiler's HBM byte counters can be scaled against known traffic.  *ms
+ * (may be NULL) receives the kernel time. */
+int storm_gpu_calibration_copy(uint64_t bytes, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -141,6 +141,7 @@ def load_library(path: str | None = None):
                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)], C.c_int),
         "storm_gpu_debug_accel": ([vp, vp, C.POINTER(C.c_uint64)], C.c_int),
         "storm_gpu_debug_spectrum": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_double), vp, vp, vp], C.c_int),
+        "storm_gpu_calibration_copy": ([C.c_uint64, C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(lib, name)

@@ -291,6 +291,26 @@ struct Tracker {
     load_cell(m, xs);
   }
 
+  // The same move with the neighbour's data already loaded (the analytic
+  // step loads it before the segment's arithmetic, to overlap the latency):
+  // `nijk`/`ncell` its index, `nw` its width along `fax`, `nx`/`nsig_s` its
+  // opacities.
+  template <typename Sink>
+  STORM_HD void enter_loaded_neighbour(int fax, bool up, uint32_t nijk, uint32_t ncell, Real nw,
+                                       const CellXS<Real> &nx, Real nsig_s, const MeshView<Real> &m,
+                                       const StepConsts &k, Sink &sink) {
+    double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
+    flush(wq, k, sink);
+    w = wq;
+    w_enter = wq;
+    ijk = nijk;
+    cell = ncell;
+    put3<Real>(fax, u0, u1, u2, up ? Real(0) : Real(1));
+    put3<Real>(fax, cw0, cw1, cw2, nw);
+    put3<Real>(fax, icw0, icw1, icw2, Real(1) / nw);
+    set_opacity(m, nx, nsig_s);
+  }
+
   STORM_HD void load_cell(const MeshView<Real> &m, const CellXS<Real> *xs) {
     cell = flat_cell(m, ijk);
     cw0 = m.width[0][ijk_i(ijk)];
@@ -313,7 +333,12 @@ struct Tracker {
 
   // opacities of the current cell at the packet's energy
   STORM_HD void load_opacity(const MeshView<Real> &m, const CellXS<Real> *xs) {
-    const CellXS<Real> x = xs[cell];
+    const bool mg = (F & kFeatMultigroup) != 0u && m.spec.groups >= 2;
+    set_opacity(m, xs[cell], mg ? m.spec.sig_s[cell] : Real(0));
+  }
+  // from the cell's gray opacities `x` and scattering opacity `sig_s`
+  // (multigroup only)
+  STORM_HD void set_opacity(const MeshView<Real> &m, const CellXS<Real> &x, Real sig_s) {
     cell_sig_abs = x.sig_abs;
     cell_sig_ev = x.sig_ev;
     cell_sig_s = Real(0);
@@ -321,7 +346,7 @@ struct Tracker {
       // clampFrequencyToBounds (RadiationIMC.hpp:868-872) and E^-nu_pow
       Real e = nu < m.spec.e_lo ? m.spec.e_lo : (nu > m.spec.e_hi ? m.spec.e_hi : nu);
       const Real fac = M::exp(-m.spec.nu_pow * M::log(e));
-      cell_sig_s = m.spec.sig_s[cell];
+      cell_sig_s = sig_s;
       cell_sig_abs = x.sig_abs * fac;
       cell_sig_ev = cell_sig_s + x.sig_ev * fac;
     }
@@ -380,10 +405,9 @@ struct Tracker {
     const Real sig_abs = cell_sig_abs;
     ++visit_segs;
 
-    // Random numbers of this segment: one Philox block (two for an
-    // acceleration step).
-    Philox4 r = rs.block(ctr++);
-
+    // Random numbers of a segment: one Philox block, block ctr (two for a
+    // random-walk step, three for a multigroup re-emission), drawn in each
+    // branch below after the loads it does not depend on are issued.
     if ((F & kFeatAccel) != 0u && acc.cell) {
       const CellAccel<Real> ca = acc.cell[cell];
       if (ca.flags) {
@@ -400,7 +424,8 @@ struct Tracker {
         const Real tauR = Ro * ca.sig_t;
         if ((ca.flags & 1u) && Ro > Real(0) && ca.sig_t > Real(0) && tauR >= acc.rw_particle_tau) {
           // ---------------- random-walk step (tryRandomWalkStep, gray)
-          Philox4 r2 = rs.block(ctr++);
+          const Philox4 r = rs.block(ctr++);
+          const Philox4 r2 = rs.block(ctr++);
           rw::Tables<Real> tb{acc.rw_tau, acc.rw_survival, acc.rw_radius, acc.rw_log_tau_min, acc.rw_log_tau_max};
           const Real tauLeak = rw::sample_leak_time<Real>(tb, M::uniform(r.v[0]));
           // tLeak = tau Ro^2 / D with D = c / (3 sigma_t): as a path, tau Ro^2 3 sigma_t
@@ -449,6 +474,7 @@ struct Tracker {
         }
         if ((ca.flags & 2u) && ca.total > Real(0) && tauR >= acc.ddmc_particle_tau) {
           // ---------------- DDMC step (tryDDMCStep, gray)
+          const Philox4 r = rs.block(ctr++);
           const Real sEvent = (-M::log(M::uniform(r.v[0]))) / ca.total;
           const bool cen = s <= sEvent;
           const Real d = cen ? s : sEvent;
@@ -548,6 +574,27 @@ struct Tracker {
         }
       }
     }
+    // The neighbour across the nearest face (this cell again when that face
+    // is on the box boundary): its data is loaded now and used only if the
+    // segment ends on that face, so that the load latency overlaps the
+    // random numbers and the attenuation below.
+    const Real of = sel3(fax, o0, o1, o2);
+    const bool up = of > Real(0);
+    const uint32_t idx = sel3<uint32_t>(fax, i, j, kk);
+    const uint32_t nax = static_cast<uint32_t>(sel3<int32_t>(fax, m.nx, m.ny, m.nz));
+    const bool outside = up ? (idx + 1u >= nax) : (idx == 0u);
+    const uint32_t nidx = outside ? idx : (up ? idx + 1u : idx - 1u);
+    const uint32_t snz = static_cast<uint32_t>(m.nz);
+    const uint32_t cstride = sel3<uint32_t>(fax, static_cast<uint32_t>(m.ny) * snz, snz, 1u);
+    const uint32_t istride = sel3<uint32_t>(fax, 1u << 20, 1u << 10, 1u);
+    const uint32_t ncell = outside ? cell : (up ? cell + cstride : cell - cstride);
+    const uint32_t nijk = outside ? ijk : (up ? ijk + istride : ijk - istride);
+    const CellXS<Real> nxs = xs[ncell];
+    const Real nw = sel3<const Real *>(fax, m.width[0], m.width[1], m.width[2])[nidx];
+    Real nsig_s = Real(0);
+    if ((F & kFeatMultigroup) != 0u && m.spec.groups >= 2) nsig_s = m.spec.sig_s[ncell];
+
+    const Philox4 r = rs.block(ctr++);
     // Distance to the next effective-scatter event (-ln xi / sigma_ev).
     Real xi = M::uniform(r.v[0]);
     Real dcol = cell_sig_ev > Real(0) ? (-M::log(xi)) * inv_sig_ev : M::inf();
@@ -585,14 +632,9 @@ struct Tracker {
     }
 
     if (ev == 0) {
-      const Real of = sel3(fax, o0, o1, o2);
-      const bool up = of > Real(0);
-      const uint32_t idx = sel3<uint32_t>(fax, i, j, kk);
-      const uint32_t n = static_cast<uint32_t>(sel3<int32_t>(fax, m.nx, m.ny, m.nz));
-      const bool outside = up ? (idx + 1u >= n) : (idx == 0u);
       if (!outside) {
         // interior face: the packet enters the neighbour on the opposite face
-        enter_neighbour(fax, up, m, xs, k, sink);
+        enter_loaded_neighbour(fax, up, nijk, ncell, nw, nxs, nsig_s, m, k, sink);
       } else {
         const int face = 2 * fax + (up ? 1 : 0);
         put3<Real>(fax, u0, u1, u2, up ? Real(1) : Real(0));

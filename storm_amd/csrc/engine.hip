@@ -1133,6 +1133,39 @@ int storm_gpu_device_info(char *buf, size_t len) {
   });
 }
 
+int storm_gpu_calibration_copy(uint64_t bytes, double *ms) {
+  return guarded([&] {
+    if (bytes == 0 || bytes % 16u != 0u) throw EngineError(1, "copy size must be a positive multiple of 16 bytes");
+    if (bytes > (uint64_t(16) << 30)) throw EngineError(1, "copy size above 16 GiB");
+    const uint64_t n = bytes / 16u;
+    void *a = nullptr, *b = nullptr;
+    HIPCHECK(hipMalloc(&a, bytes));
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+      (void)hipFree(a);
+      throw EngineError(2, "device allocation failed");
+    }
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    const uint32_t block = 256;
+    const uint64_t grid = (n + block - 1) / block;
+    k_fill_u32<<<static_cast<uint32_t>((bytes / 4 + block - 1) / block), block>>>(static_cast<uint32_t *>(a),
+                                                                                   bytes / 4, 0x5a5a5a5au);
+    HIPCHECK(hipEventRecord(e0, nullptr));
+    k_copy_calibration<<<static_cast<uint32_t>(grid), block>>>(static_cast<const uint4 *>(a), static_cast<uint4 *>(b),
+                                                               n);
+    HIPCHECK(hipEventRecord(e1, nullptr));
+    HIPCHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    HIPCHECK(hipEventDestroy(e0));
+    HIPCHECK(hipEventDestroy(e1));
+    HIPCHECK(hipFree(a));
+    HIPCHECK(hipFree(b));
+  });
+}
+
 int storm_gpu_create(const storm_problem *prob, int32_t precision, int32_t material_mode, storm_gpu_engine **out) {
   return guarded([&] {
     if (!prob || !out) throw EngineError(1, "null argument");

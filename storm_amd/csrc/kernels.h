@@ -989,5 +989,13 @@ __global__ void k_fill_f64(double *p, uint64_t n, double v) {
   if (i < n) p[i] = v;
 }
 
+// Plain copy of n 16-byte words, one per thread: a kernel whose HBM traffic
+// is known exactly (16 n bytes read, 16 n written), to scale byte counters.
+__global__ __launch_bounds__(256) void k_copy_calibration(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                          uint64_t n) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 }  // namespace gpu
 }  // namespace storm

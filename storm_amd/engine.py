@@ -185,3 +185,14 @@ def device_info() -> str:
     if rc != 0:
         raise StormError(lib.storm_gpu_last_error().decode())
     return buf.value.decode()
+
+
+def calibration_copy(nbytes: int) -> float:
+    """One copy of `nbytes` by the library's plain copy kernel (reads and
+    writes each byte once); returns its time in ms."""
+    lib = capi.load_library()
+    ms = C.c_double(0.0)
+    rc = lib.storm_gpu_calibration_copy(C.c_uint64(nbytes), C.byref(ms))
+    if rc != 0:
+        raise StormError(lib.storm_gpu_last_error().decode())
+    return ms.value
