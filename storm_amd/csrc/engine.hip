@@ -158,6 +158,10 @@ struct Engine final : EngineBase {
   DevBuf<uint64_t> offsets, keys, keys_out, offs64, nphot64;
   DevBuf<unsigned long long> tally_u64, emit_limbs, W, requant, leaving, scal;
   DevBuf<double> tally_f64;
+  // per-XCD replicas 1..kTallyReplicas-1 of the per-visit tallies
+  DevBuf<unsigned long long> rep_u64;
+  DevBuf<double> rep_f64;
+  DevBuf<uint32_t> rep_u32;
   DevBuf<ulonglong2> limbs_sorted, cum;
   DevBuf<CombCell> comb_cells;
   DevBuf<CellAccel<Real>> accel;
@@ -330,6 +334,23 @@ struct Engine final : EngineBase {
       tal.dep_fp = nullptr;
       tal.pint_fp = nullptr;
       tal.esc_fp = nullptr;
+    }
+    {
+      const size_t R1 = kTallyReplicas - 1;
+      tal.ncell = N;
+      rep_u32.ensure(R1 * N);
+      tal.segs_rep = rep_u32.p;
+      if (precision == STORM_GPU_FP64) {
+        rep_f64.ensure(R1 * 2 * static_cast<size_t>(N));
+        tal.dep_fp_rep = rep_f64.p;
+        tal.pint_fp_rep = rep_f64.p + R1 * N;
+        tal.dep_rep = tal.pint_rep = nullptr;
+      } else {
+        rep_u64.ensure(R1 * 2 * n2);
+        tal.dep_rep = rep_u64.p;
+        tal.pint_rep = rep_u64.p + R1 * n2;
+        tal.dep_fp_rep = tal.pint_fp_rep = nullptr;
+      }
     }
     zero_u64(tally_u64.p, tally_u64.n);
     zero_f64(tally_f64.p, tally_f64.n);
@@ -605,6 +626,9 @@ struct Engine final : EngineBase {
     zero_u64(tally_u64.p, tally_u64.n);
     if (precision == STORM_GPU_FP64) zero_f64(tally_f64.p, tally_f64.n);
     zero_u32(tal.segs, 2 * static_cast<size_t>(N));
+    zero_u32(rep_u32.p, rep_u32.n);
+    if (precision == STORM_GPU_FP64) zero_f64(rep_f64.p, rep_f64.n);
+    else zero_u64(rep_u64.p, rep_u64.n);
     zero_u64(requant.p, requant.n);
     zero_u64(leaving.p, 5);
 
@@ -665,6 +689,7 @@ struct Engine final : EngineBase {
       n_cen_total = h[1];
       ping = !ping;
     }
+    k_fold_replicas<<<blocks_for(N, 256), 256, 0, stream>>>(tal, kTallyReplicas);
     counters.seconds_transport = t_transport;
     counters.seconds_transport_max_launch = t_max;
     counters.census_before_comb = n_cen_total;

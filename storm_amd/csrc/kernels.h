@@ -129,7 +129,25 @@ struct Tallies {
   double *esc_fp;            // [N]  FP64 engine: escaped
   uint32_t *segs;            // [N]  tracked segments (cellsStepsCounters)
   uint32_t *begin;           // [N]  packets per cell at transport start
+  // Replicas 1..kTallyReplicas-1 of the per-visit tallies (dep, pint, segs;
+  // dep_fp, pint_fp in the FP64 engine), one per XCD, each laid out as its
+  // replica-0 array above; k_fold_replicas adds them into replica 0 after
+  // the transport.  A wavefront adds into the replica of the XCD it runs on,
+  // so that the atomics of different XCDs never share a cache line.
+  unsigned long long *dep_rep, *pint_rep;  // [(R-1) * 2N]
+  double *dep_fp_rep, *pint_fp_rep;        // [(R-1) * N]
+  uint32_t *segs_rep;                      // [(R-1) * N]
+  uint32_t ncell;
 };
+
+// Tally replicas: one per XCD of an MI355X (8 XCDs of 32 CUs).
+constexpr uint32_t kTallyReplicas = 8;
+
+// Index of the XCD this wavefront runs on: hardware register XCC_ID
+// (s_getreg id 20), bits [3:0].
+__device__ __forceinline__ uint32_t xcd_index() {
+  return static_cast<uint32_t>(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20)) & 0xfu;
+}
 
 __device__ __forceinline__ void tally_limbs(unsigned long long *t, uint32_t c, fx::Limbs v) {
   if (v.lo) atomicAdd(&t[2 * c], static_cast<unsigned long long>(v.lo));
@@ -416,16 +434,31 @@ struct DeviceSink {
   Tallies t;
   int32_t eq;
   uint32_t rw = 0, ddmc = 0;
-  __device__ void visit(uint32_t c, fx::Limbs dep, fx::Limbs pint, uint32_t segs) {
-    tally_limbs(t.dep, c, dep);
-    tally_limbs(t.pint, c, pint);
-    atomicAdd(&t.segs[c], segs);
+  // this wavefront's replica of the per-visit tallies
+  unsigned long long *dep, *pint;
+  double *dep_fp, *pint_fp;
+  uint32_t *segs;
+  __device__ DeviceSink(const Tallies &tt, int32_t e) : t(tt), eq(e) {
+    // the integer replicas exist in the mixed engine, the FP64 ones in the
+    // FP64 engine (the other pair of pointers is null)
+    const uint32_t r = xcd_index() % kTallyReplicas;
+    const size_t n = tt.ncell;
+    dep = (r && tt.dep_rep) ? tt.dep_rep + (r - 1) * 2 * n : tt.dep;
+    pint = (r && tt.pint_rep) ? tt.pint_rep + (r - 1) * 2 * n : tt.pint;
+    segs = r ? tt.segs_rep + (r - 1) * n : tt.segs;
+    dep_fp = (r && tt.dep_fp_rep) ? tt.dep_fp_rep + (r - 1) * n : tt.dep_fp;
+    pint_fp = (r && tt.pint_fp_rep) ? tt.pint_fp_rep + (r - 1) * n : tt.pint_fp;
+  }
+  __device__ void visit(uint32_t c, fx::Limbs d, fx::Limbs e, uint32_t n) {
+    tally_limbs(dep, c, d);
+    tally_limbs(pint, c, e);
+    atomicAdd(&segs[c], n);
   }
   __device__ void roulette(uint32_t c, double w) { tally_limbs(t.rdep, c, fx::limbs_of(w, eq)); }
-  __device__ void visit_fp(uint32_t c, double dep, double pint, uint32_t segs) {
-    if (dep != 0.0) atomicAdd(&t.dep_fp[c], dep);
-    if (pint != 0.0) atomicAdd(&t.pint_fp[c], pint);
-    atomicAdd(&t.segs[c], segs);
+  __device__ void visit_fp(uint32_t c, double d, double e, uint32_t n) {
+    if (d != 0.0) atomicAdd(&dep_fp[c], d);
+    if (e != 0.0) atomicAdd(&pint_fp[c], e);
+    atomicAdd(&segs[c], n);
   }
   __device__ void escape(uint32_t c, double w) {
     if (t.esc_fp) {
@@ -456,7 +489,7 @@ __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real
   const bool valid = i < n_in;
   int outcome = kGone;
   Packet<Real> p;
-  DeviceSink sink{t, k.eq};
+  DeviceSink sink(t, k.eq);
   if (valid) {
     p = load_packet(in, i);
     outcome = transport_packet<Real, F>(p, mesh, xs, acc, k, max_segments, sink);
@@ -499,7 +532,7 @@ __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketS
                                                           StepConsts k, int32_t max_segments, Tallies t,
                                                           unsigned long long *__restrict__ stats,
                                                           uint32_t *__restrict__ queue) {
-  DeviceSink sink{t, k.eq};
+  DeviceSink sink(t, k.eq);
   Tracker<Real, F> tr;
   bool busy = false;
   // uniform across the wavefront: the pool [pool_next, pool_end) of queued
@@ -987,6 +1020,40 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
 __global__ void k_fill_f64(double *p, uint64_t n, double v) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// Adds tally replicas 1..R-1 into replica 0, cell by cell in replica order
+// (integer sums are exact; the FP64 engine's sums are in a fixed order).
+__global__ __launch_bounds__(256) void k_fold_replicas(Tallies t, uint32_t reps) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= t.ncell) return;
+  const size_t n = t.ncell;
+  uint32_t sg = t.segs[c];
+  for (uint32_t r = 0; r + 1 < reps; ++r) sg += t.segs_rep[r * n + c];
+  t.segs[c] = sg;
+  if (t.dep_rep) {
+    unsigned long long d0 = t.dep[2 * c], d1 = t.dep[2 * c + 1];
+    unsigned long long p0 = t.pint[2 * c], p1 = t.pint[2 * c + 1];
+    for (uint32_t r = 0; r + 1 < reps; ++r) {
+      d0 += t.dep_rep[r * 2 * n + 2 * c];
+      d1 += t.dep_rep[r * 2 * n + 2 * c + 1];
+      p0 += t.pint_rep[r * 2 * n + 2 * c];
+      p1 += t.pint_rep[r * 2 * n + 2 * c + 1];
+    }
+    t.dep[2 * c] = d0;
+    t.dep[2 * c + 1] = d1;
+    t.pint[2 * c] = p0;
+    t.pint[2 * c + 1] = p1;
+  }
+  if (t.dep_fp_rep) {
+    double a = t.dep_fp[c], b = t.pint_fp[c];
+    for (uint32_t r = 0; r + 1 < reps; ++r) {
+      a += t.dep_fp_rep[r * n + c];
+      b += t.pint_fp_rep[r * n + c];
+    }
+    t.dep_fp[c] = a;
+    t.pint_fp[c] = b;
+  }
 }
 
 // Plain copy of n 16-byte words, one per thread: a kernel whose HBM traffic
