@@ -430,6 +430,16 @@ __global__ __launch_bounds__(256) void k_boundary(uint32_t n_bc, uint32_t npc, c
 }
 
 // ------------------------------------------------------------------ transport
+// Packets enter the transport ordered by cell (the census comes sorted from
+// population control, emission is generated cell by cell), so the 64
+// consecutive packets of a wavefront would start in one or a few cells and
+// their tally atomics would hit the same addresses together.  Slot q of the
+// launch takes packet spread(q): lane l of wavefront w takes packet
+// l * nw + w (nw = ceil(n / 64)), so a wavefront's packets are n / 64 apart.
+// Slots with spread(q) >= n hold no packet; there are fewer than 64.
+__device__ __forceinline__ uint32_t spread_waves(uint32_t n) { return (n + 63u) / 64u; }
+__device__ __forceinline__ uint32_t spread(uint32_t q, uint32_t nw) { return (q & 63u) * nw + (q >> 6); }
+
 struct DeviceSink {
   Tallies t;
   int32_t eq;
@@ -485,8 +495,10 @@ __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real
                                                    const CellXS<Real> *__restrict__ xs, AccelView<Real> acc,
                                                    StepConsts k, int32_t max_segments, Tallies t,
                                                    unsigned long long *__restrict__ stats) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < n_in;
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nw = spread_waves(n_in);
+  const uint32_t i = spread(q, nw);
+  const bool valid = (q >> 6) < nw && i < n_in;
   int outcome = kGone;
   Packet<Real> p;
   DeviceSink sink(t, k.eq);
@@ -515,10 +527,11 @@ __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real
 // Persistent transport with lane refill.  Each wavefront advances its 64
 // packets one segment at a time; a lane whose packet ended (census, absorbed,
 // escaped, or `max_segments` segments in this launch) takes the next packet
-// from its wavefront's pool of queued packets.  The pool is refilled with
-// kRefillChunk consecutive packets by one atomic on `*queue` (the count of
-// handed-out packets) when it runs dry, so the shared counter sees one atomic
-// per chunk rather than one per lane refill.  Lanes stay busy until the
+// from its wavefront's pool of queued slots.  The pool is refilled with
+// kRefillChunk consecutive slots (packets spread(q), as in k_transport) by
+// one atomic on `*queue` (the count of handed-out slots) when it runs dry,
+// so the shared counter sees one atomic per chunk rather than one per lane
+// refill.  Lanes stay busy until the
 // queue is drained, instead of idling until the longest history of their
 // wavefront ends.  Per packet the result is that of k_transport with the
 // same `max_segments`.
@@ -535,8 +548,11 @@ __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketS
   DeviceSink sink(t, k.eq);
   Tracker<Real, F> tr;
   bool busy = false;
+  // the queue hands out slots q in [0, 64 nw); slot q holds packet spread(q)
+  const uint32_t nw = spread_waves(n_in);
+  const uint32_t nq = 64u * nw;
   // uniform across the wavefront: the pool [pool_next, pool_end) of queued
-  // packets this wavefront owns, and whether the queue is exhausted
+  // slots this wavefront owns, and whether the queue is exhausted
   bool drained = false;
   uint32_t pool_next = 0, pool_end = 0;
   int32_t nseg = 0;
@@ -549,19 +565,20 @@ __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketS
           uint32_t base = 0u;
           if (lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u) base = atomicAdd(queue, kRefillChunk);
           base = __builtin_amdgcn_readfirstlane(base);
-          if (base >= n_in) {
+          if (base >= nq) {
             drained = true;
           } else {
             pool_next = base;
-            pool_end = base + kRefillChunk < n_in ? base + kRefillChunk : n_in;
+            pool_end = base + kRefillChunk < nq ? base + kRefillChunk : nq;
           }
         }
         if (!drained) {
           const uint32_t avail = pool_end - pool_next;
           const uint32_t want = static_cast<uint32_t>(__popcll(idle));
           const uint32_t r = lane_rank(idle);
-          if (!busy && r < avail) {
-            const Packet<Real> p = load_packet(in, pool_next + r);
+          const uint32_t pk = spread(pool_next + r, nw);
+          if (!busy && r < avail && pk < n_in) {
+            const Packet<Real> p = load_packet(in, pk);
             tr.begin(p, mesh, xs, k);
             busy = true;
             nseg = 0;
