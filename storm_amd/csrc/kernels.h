@@ -431,14 +431,59 @@ __global__ __launch_bounds__(256) void k_boundary(uint32_t n_bc, uint32_t npc, c
 
 // ------------------------------------------------------------------ transport
 // Packets enter the transport ordered by cell (the census comes sorted from
-// population control, emission is generated cell by cell), so the 64
-// consecutive packets of a wavefront would start in one or a few cells and
-// their tally atomics would hit the same addresses together.  Slot q of the
-// launch takes packet spread(q): lane l of wavefront w takes packet
-// l * nw + w (nw = ceil(n / 64)), so a wavefront's packets are n / 64 apart.
-// Slots with spread(q) >= n hold no packet; there are fewer than 64.
+// population control, emission is generated cell by cell).  k_transport
+// keeps that order (a block's visits then fall in few cells, which its LDS
+// tally cache exploits).  k_transport_refill has no block-local cache and
+// spreads instead, so that the 64 lanes of a wavefront do not send their
+// device-memory atomics to the same addresses together: queue slot q holds
+// packet spread(q), lane l of wavefront w taking packet l * nw + w
+// (nw = ceil(n / 64)).  Slots with spread(q) >= n hold no packet; there are
+// fewer than 64.
 __device__ __forceinline__ uint32_t spread_waves(uint32_t n) { return (n + 63u) / 64u; }
 __device__ __forceinline__ uint32_t spread(uint32_t q, uint32_t nw) { return (q & 63u) * nw + (q >> 6); }
+
+// Block-local cache of the per-visit tallies in LDS.  Global atomics on
+// device memory are performed at the memory controller (every one a DRAM
+// read-modify-write), so a visit's three atomics are the transport's most
+// expensive operation.  The packets of one block start in neighbouring cells
+// (the input is ordered by cell) and in optically thick media stay close for
+// many segments, so a block's visits fall in few cells: they are summed in a
+// small open-addressed table in LDS (cell -> sums) and written to device
+// memory once per cell when the block's last wavefront ends.  A visit whose
+// cell finds no slot within kCacheProbes probes goes straight to device
+// memory.  Integer sums are exact in any order; the FP64 engine's sums
+// (doubles in v[0], v[1]) change only their order.
+constexpr uint32_t kCacheSlots = 512;  // power of two
+constexpr uint32_t kCacheProbes = 4;
+struct TallyCache {
+  uint32_t tag[kCacheSlots];  // cell + 1; 0: free
+  uint32_t segs[kCacheSlots];
+  unsigned long long v[4][kCacheSlots];  // dep lo, dep hi, pint lo, pint hi (FP64: dep, pint)
+  uint32_t waves_done;
+};
+
+__device__ __forceinline__ void cache_clear(TallyCache &L) {
+  for (uint32_t s = threadIdx.x; s < kCacheSlots; s += blockDim.x) {
+    L.tag[s] = 0u;
+    L.segs[s] = 0u;
+    L.v[0][s] = L.v[1][s] = L.v[2][s] = L.v[3][s] = 0ull;
+  }
+  if (threadIdx.x == 0) L.waves_done = 0u;
+}
+
+// Slot of cell c (claimed if free), or -1 when the probed slots hold other cells.
+__device__ __forceinline__ int cache_slot(TallyCache &L, uint32_t c) {
+  const uint32_t key = c + 1u;
+  const uint32_t h = (c * 0x9E3779B1u) >> 23;  // 9 bits: kCacheSlots = 512
+#pragma unroll
+  for (uint32_t probe = 0; probe < kCacheProbes; ++probe) {
+    const uint32_t s = (h + probe) & (kCacheSlots - 1u);
+    uint32_t cur = __hip_atomic_load(&L.tag[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == 0u) cur = atomicCAS(&L.tag[s], 0u, key);
+    if (cur == 0u || cur == key) return static_cast<int>(s);
+  }
+  return -1;
+}
 
 struct DeviceSink {
   Tallies t;
@@ -448,6 +493,7 @@ struct DeviceSink {
   unsigned long long *dep, *pint;
   double *dep_fp, *pint_fp;
   uint32_t *segs;
+  TallyCache *cache = nullptr;  // the block's LDS cache, if any
   __device__ DeviceSink(const Tallies &tt, int32_t e) : t(tt), eq(e) {
     // the integer replicas exist in the mixed engine, the FP64 ones in the
     // FP64 engine (the other pair of pointers is null)
@@ -460,15 +506,56 @@ struct DeviceSink {
     pint_fp = (r && tt.pint_fp_rep) ? tt.pint_fp_rep + (r - 1) * n : tt.pint_fp;
   }
   __device__ void visit(uint32_t c, fx::Limbs d, fx::Limbs e, uint32_t n) {
+    const int s = cache ? cache_slot(*cache, c) : -1;
+    if (s >= 0) {
+      if (d.lo) atomicAdd(&cache->v[0][s], static_cast<unsigned long long>(d.lo));
+      if (d.hi) atomicAdd(&cache->v[1][s], static_cast<unsigned long long>(d.hi));
+      if (e.lo) atomicAdd(&cache->v[2][s], static_cast<unsigned long long>(e.lo));
+      if (e.hi) atomicAdd(&cache->v[3][s], static_cast<unsigned long long>(e.hi));
+      atomicAdd(&cache->segs[s], n);
+      return;
+    }
     tally_limbs(dep, c, d);
     tally_limbs(pint, c, e);
     atomicAdd(&segs[c], n);
   }
   __device__ void roulette(uint32_t c, double w) { tally_limbs(t.rdep, c, fx::limbs_of(w, eq)); }
   __device__ void visit_fp(uint32_t c, double d, double e, uint32_t n) {
+    const int s = cache ? cache_slot(*cache, c) : -1;
+    if (s >= 0) {
+      if (d != 0.0) atomicAdd(reinterpret_cast<double *>(&cache->v[0][s]), d);
+      if (e != 0.0) atomicAdd(reinterpret_cast<double *>(&cache->v[1][s]), e);
+      atomicAdd(&cache->segs[s], n);
+      return;
+    }
     if (d != 0.0) atomicAdd(&dep_fp[c], d);
     if (e != 0.0) atomicAdd(&pint_fp[c], e);
     atomicAdd(&segs[c], n);
+  }
+  // Called by every wavefront of the block once its packets are done: the
+  // last one writes the cache to device memory.
+  __device__ void finish_block() {
+    if (!cache) return;
+    uint32_t done = 0u;
+    if (lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u)
+      done = __hip_atomic_fetch_add(&cache->waves_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    done = __builtin_amdgcn_readfirstlane(done);
+    if (done + 1u != (blockDim.x + 63u) / 64u) return;
+    for (uint32_t s = __lane_id(); s < kCacheSlots; s += 64u) {
+      const uint32_t key = __hip_atomic_load(&cache->tag[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (!key) continue;
+      const uint32_t c = key - 1u;
+      if (dep_fp) {
+        const double d = __longlong_as_double(static_cast<long long>(cache->v[0][s]));
+        const double e = __longlong_as_double(static_cast<long long>(cache->v[1][s]));
+        if (d != 0.0) atomicAdd(&dep_fp[c], d);
+        if (e != 0.0) atomicAdd(&pint_fp[c], e);
+      } else {
+        tally_limbs(dep, c, fx::Limbs{cache->v[0][s], cache->v[1][s]});
+        tally_limbs(pint, c, fx::Limbs{cache->v[2][s], cache->v[3][s]});
+      }
+      atomicAdd(&segs[c], cache->segs[s]);
+    }
   }
   __device__ void escape(uint32_t c, double w) {
     if (t.esc_fp) {
@@ -495,17 +582,20 @@ __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real
                                                    const CellXS<Real> *__restrict__ xs, AccelView<Real> acc,
                                                    StepConsts k, int32_t max_segments, Tallies t,
                                                    unsigned long long *__restrict__ stats) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t nw = spread_waves(n_in);
-  const uint32_t i = spread(q, nw);
-  const bool valid = (q >> 6) < nw && i < n_in;
+  __shared__ TallyCache cache;
+  cache_clear(cache);
+  __syncthreads();
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n_in;
   int outcome = kGone;
   Packet<Real> p;
   DeviceSink sink(t, k.eq);
+  sink.cache = &cache;
   if (valid) {
     p = load_packet(in, i);
     outcome = transport_packet<Real, F>(p, mesh, xs, acc, k, max_segments, sink);
   }
+  sink.finish_block();
   const bool alive = valid && outcome == kAlive;
   const bool census = valid && outcome == kCensus;
   uint32_t slot = wave_append(alive, n_out);
