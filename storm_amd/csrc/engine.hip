@@ -120,7 +120,7 @@ struct EngineBase {
   virtual void debug_spectrum(int32_t *, double *, void *, void *, void *) = 0;
   storm_gpu_counters counters{};
   double source_T = 0.0;
-  int32_t block = 256;
+  int32_t block = 512;  // k_transport; k_transport_refill uses at most 256
   int32_t seg_per_pass = 1 << 20;
   int32_t schedule = STORM_GPU_SCHEDULE_PASS;
   bool keep_inputs = false;
@@ -461,12 +461,13 @@ struct Engine final : EngineBase {
   void launch_transport_f(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshView<Real> &mv,
                           const AccelView<Real> &av, const StepConsts &kc) {
     if (schedule == STORM_GPU_SCHEDULE_REFILL) {
-      const uint32_t grid = std::min<uint32_t>(blocks_for(n_in, block), resident_blocks<F>(block));
-      k_transport_refill<Real, F><<<grid, block, 0, stream>>>(
+      const int32_t rb = std::min<int32_t>(block, 256);
+      const uint32_t grid = std::min<uint32_t>(blocks_for(n_in, rb), resident_blocks<F>(rb));
+      k_transport_refill<Real, F><<<grid, rb, 0, stream>>>(
           static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
           xs.p, av, kc, seg_per_pass, tal, leaving.p, counters_u32.p + 2);
     } else {
-      k_transport<Real, F><<<blocks_for(n_in, block), block, 0, stream>>>(
+      k_transport<Real, F><<<blocks_for(n_in, block), block, cache_bytes(block), stream>>>(
           static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
           xs.p, av, kc, seg_per_pass, tal, leaving.p);
     }
@@ -1260,8 +1261,8 @@ int storm_gpu_import_census(storm_gpu_engine *eng, const storm_packet_record *in
 
 int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segments_per_pass, int32_t schedule) {
   return guarded([&] {
-    if (block_size != 64 && block_size != 128 && block_size != 256)
-      throw EngineError(1, "block size must be 64, 128 or 256");
+    if (block_size != 64 && block_size != 128 && block_size != 256 && block_size != 512)
+      throw EngineError(1, "block size must be 64, 128, 256 or 512");
     if (segments_per_pass < 1) throw EngineError(1, "segments per pass must be >= 1");
     if (schedule != STORM_GPU_SCHEDULE_PASS && schedule != STORM_GPU_SCHEDULE_REFILL)
       throw EngineError(1, "schedule must be STORM_GPU_SCHEDULE_PASS or STORM_GPU_SCHEDULE_REFILL");

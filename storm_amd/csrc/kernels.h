@@ -451,38 +451,64 @@ __device__ __forceinline__ uint32_t spread(uint32_t q, uint32_t nw) { return (q 
 // small open-addressed table in LDS (cell -> sums) and written to device
 // memory once per cell when the block's last wavefront ends.  A visit whose
 // cell finds no slot within kCacheProbes probes goes straight to device
-// memory.  Integer sums are exact in any order; the FP64 engine's sums
-// (doubles in v[0], v[1]) change only their order.
-constexpr uint32_t kCacheSlots = 512;  // power of two
-constexpr uint32_t kCacheProbes = 4;
+// memory.  The census weight of packets ending the step in census (cpre)
+// goes through the same table.  Integer sums are exact in any order; the
+// FP64 engine's sums (doubles in v[0], v[1]) change only their order.
+// The table lives in dynamic LDS: 2 slots per thread of the block (block
+// sizes are powers of two), 56 B per slot.
+constexpr uint32_t kCacheSlotsPerThread = 2;
+constexpr uint32_t kCacheProbes = 8;
+constexpr uint32_t kCacheSlotBytes = 6 * 8 + 2 * 4;
+inline __host__ __device__ uint32_t cache_bytes(uint32_t block) {
+  return kCacheSlotsPerThread * block * kCacheSlotBytes + 16u;
+}
 struct TallyCache {
-  uint32_t tag[kCacheSlots];  // cell + 1; 0: free
-  uint32_t segs[kCacheSlots];
-  unsigned long long v[4][kCacheSlots];  // dep lo, dep hi, pint lo, pint hi (FP64: dep, pint)
-  uint32_t waves_done;
+  uint32_t bits, mask;
+  unsigned long long *v;  // v[f * slots + s]: dep lo, dep hi, pint lo, pint hi (FP64: dep, pint), cpre lo, cpre hi
+  uint32_t *tag;          // cell + 1; 0: free
+  uint32_t *segs;
+  uint32_t *waves_done;
+  __device__ __forceinline__ unsigned long long &at(uint32_t f, uint32_t s) const { return v[f * (mask + 1u) + s]; }
 };
 
-__device__ __forceinline__ void cache_clear(TallyCache &L) {
-  for (uint32_t s = threadIdx.x; s < kCacheSlots; s += blockDim.x) {
+// Lays the table out in `lds` (cache_bytes(blockDim.x) bytes) and clears it;
+// the caller synchronises the block before first use.
+__device__ __forceinline__ TallyCache cache_init(unsigned char *lds) {
+  TallyCache L;
+  const uint32_t slots = kCacheSlotsPerThread * blockDim.x;
+  L.bits = 31u - static_cast<uint32_t>(__clz(static_cast<int>(slots)));
+  L.mask = slots - 1u;
+  L.v = reinterpret_cast<unsigned long long *>(lds);
+  L.tag = reinterpret_cast<uint32_t *>(lds + 6 * 8 * slots);
+  L.segs = L.tag + slots;
+  L.waves_done = L.segs + slots;
+  for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
     L.tag[s] = 0u;
     L.segs[s] = 0u;
-    L.v[0][s] = L.v[1][s] = L.v[2][s] = L.v[3][s] = 0ull;
+    for (uint32_t f = 0; f < 6; ++f) L.at(f, s) = 0ull;
   }
-  if (threadIdx.x == 0) L.waves_done = 0u;
+  if (threadIdx.x == 0) *L.waves_done = 0u;
+  return L;
+}
+
+// Adds into the table (workgroup scope: the table is the block's own).
+template <typename T>
+__device__ __forceinline__ void lds_add(T *p, T v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Slot of cell c (claimed if free), or -1 when the probed slots hold other cells.
-__device__ __forceinline__ int cache_slot(TallyCache &L, uint32_t c) {
+__device__ __forceinline__ int cache_slot(uint32_t *tag, uint32_t bits, uint32_t mask, uint32_t c) {
   const uint32_t key = c + 1u;
-  const uint32_t h = (c * 0x9E3779B1u) >> 23;  // 9 bits: kCacheSlots = 512
-#pragma unroll
-  for (uint32_t probe = 0; probe < kCacheProbes; ++probe) {
-    const uint32_t s = (h + probe) & (kCacheSlots - 1u);
-    uint32_t cur = __hip_atomic_load(&L.tag[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (cur == 0u) cur = atomicCAS(&L.tag[s], 0u, key);
-    if (cur == 0u || cur == key) return static_cast<int>(s);
+  const uint32_t h = (c * 0x9E3779B1u) >> (32u - bits);
+  int found = -1;
+  for (uint32_t probe = 0; probe < kCacheProbes && found < 0; ++probe) {
+    const uint32_t s = (h + probe) & mask;
+    uint32_t cur = __hip_atomic_load(&tag[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == 0u) cur = atomicCAS(&tag[s], 0u, key);
+    if (cur == 0u || cur == key) found = static_cast<int>(s);
   }
-  return -1;
+  return found;
 }
 
 struct DeviceSink {
@@ -493,7 +519,7 @@ struct DeviceSink {
   unsigned long long *dep, *pint;
   double *dep_fp, *pint_fp;
   uint32_t *segs;
-  TallyCache *cache = nullptr;  // the block's LDS cache, if any
+  TallyCache cache{};  // the block's LDS cache (cache.v null: none)
   __device__ DeviceSink(const Tallies &tt, int32_t e) : t(tt), eq(e) {
     // the integer replicas exist in the mixed engine, the FP64 ones in the
     // FP64 engine (the other pair of pointers is null)
@@ -506,13 +532,13 @@ struct DeviceSink {
     pint_fp = (r && tt.pint_fp_rep) ? tt.pint_fp_rep + (r - 1) * n : tt.pint_fp;
   }
   __device__ void visit(uint32_t c, fx::Limbs d, fx::Limbs e, uint32_t n) {
-    const int s = cache ? cache_slot(*cache, c) : -1;
+    const int s = cache.v ? cache_slot(cache.tag, cache.bits, cache.mask, c) : -1;
     if (s >= 0) {
-      if (d.lo) atomicAdd(&cache->v[0][s], static_cast<unsigned long long>(d.lo));
-      if (d.hi) atomicAdd(&cache->v[1][s], static_cast<unsigned long long>(d.hi));
-      if (e.lo) atomicAdd(&cache->v[2][s], static_cast<unsigned long long>(e.lo));
-      if (e.hi) atomicAdd(&cache->v[3][s], static_cast<unsigned long long>(e.hi));
-      atomicAdd(&cache->segs[s], n);
+      if (d.lo) lds_add(&cache.at(0, s), static_cast<unsigned long long>(d.lo));
+      if (d.hi) lds_add(&cache.at(1, s), static_cast<unsigned long long>(d.hi));
+      if (e.lo) lds_add(&cache.at(2, s), static_cast<unsigned long long>(e.lo));
+      if (e.hi) lds_add(&cache.at(3, s), static_cast<unsigned long long>(e.hi));
+      lds_add(&cache.segs[s], n);
       return;
     }
     tally_limbs(dep, c, d);
@@ -521,40 +547,51 @@ struct DeviceSink {
   }
   __device__ void roulette(uint32_t c, double w) { tally_limbs(t.rdep, c, fx::limbs_of(w, eq)); }
   __device__ void visit_fp(uint32_t c, double d, double e, uint32_t n) {
-    const int s = cache ? cache_slot(*cache, c) : -1;
+    const int s = cache.v ? cache_slot(cache.tag, cache.bits, cache.mask, c) : -1;
     if (s >= 0) {
-      if (d != 0.0) atomicAdd(reinterpret_cast<double *>(&cache->v[0][s]), d);
-      if (e != 0.0) atomicAdd(reinterpret_cast<double *>(&cache->v[1][s]), e);
-      atomicAdd(&cache->segs[s], n);
+      if (d != 0.0) lds_add(reinterpret_cast<double *>(&cache.at(0, s)), d);
+      if (e != 0.0) lds_add(reinterpret_cast<double *>(&cache.at(1, s)), e);
+      lds_add(&cache.segs[s], n);
       return;
     }
     if (d != 0.0) atomicAdd(&dep_fp[c], d);
     if (e != 0.0) atomicAdd(&pint_fp[c], e);
     atomicAdd(&segs[c], n);
   }
+  // census weight of a packet ending the step in census, cell c
+  __device__ void census_weight(uint32_t c, fx::Limbs w) {
+    const int s = cache.v ? cache_slot(cache.tag, cache.bits, cache.mask, c) : -1;
+    if (s >= 0) {
+      if (w.lo) lds_add(&cache.at(4, s), static_cast<unsigned long long>(w.lo));
+      if (w.hi) lds_add(&cache.at(5, s), static_cast<unsigned long long>(w.hi));
+      return;
+    }
+    tally_limbs(t.cpre, c, w);
+  }
   // Called by every wavefront of the block once its packets are done: the
   // last one writes the cache to device memory.
   __device__ void finish_block() {
-    if (!cache) return;
+    if (!cache.v) return;
     uint32_t done = 0u;
     if (lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u)
-      done = __hip_atomic_fetch_add(&cache->waves_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      done = __hip_atomic_fetch_add(cache.waves_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     done = __builtin_amdgcn_readfirstlane(done);
     if (done + 1u != (blockDim.x + 63u) / 64u) return;
-    for (uint32_t s = __lane_id(); s < kCacheSlots; s += 64u) {
-      const uint32_t key = __hip_atomic_load(&cache->tag[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (uint32_t s = __lane_id(); s <= cache.mask; s += 64u) {
+      const uint32_t key = __hip_atomic_load(&cache.tag[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (!key) continue;
       const uint32_t c = key - 1u;
       if (dep_fp) {
-        const double d = __longlong_as_double(static_cast<long long>(cache->v[0][s]));
-        const double e = __longlong_as_double(static_cast<long long>(cache->v[1][s]));
+        const double d = __longlong_as_double(static_cast<long long>(cache.at(0, s)));
+        const double e = __longlong_as_double(static_cast<long long>(cache.at(1, s)));
         if (d != 0.0) atomicAdd(&dep_fp[c], d);
         if (e != 0.0) atomicAdd(&pint_fp[c], e);
       } else {
-        tally_limbs(dep, c, fx::Limbs{cache->v[0][s], cache->v[1][s]});
-        tally_limbs(pint, c, fx::Limbs{cache->v[2][s], cache->v[3][s]});
+        tally_limbs(dep, c, fx::Limbs{cache.at(0, s), cache.at(1, s)});
+        tally_limbs(pint, c, fx::Limbs{cache.at(2, s), cache.at(3, s)});
       }
-      atomicAdd(&segs[c], cache->segs[s]);
+      if (cache.segs[s]) atomicAdd(&segs[c], cache.segs[s]);
+      tally_limbs(t.cpre, c, fx::Limbs{cache.at(4, s), cache.at(5, s)});
     }
   }
   __device__ void escape(uint32_t c, double w) {
@@ -576,26 +613,25 @@ __device__ __forceinline__ void wave_add_u32(uint32_t v, unsigned long long *dst
 
 // stats: [0] escapes, [1] random-walk steps, [2] DDMC steps
 template <typename Real, uint32_t F>
-__global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out,
+__global__ __launch_bounds__(512) void k_transport(uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out,
                                                    uint32_t *__restrict__ n_out, CensusSoA<Real> cen,
                                                    uint32_t *__restrict__ n_cen, MeshView<Real> mesh,
                                                    const CellXS<Real> *__restrict__ xs, AccelView<Real> acc,
                                                    StepConsts k, int32_t max_segments, Tallies t,
                                                    unsigned long long *__restrict__ stats) {
-  __shared__ TallyCache cache;
-  cache_clear(cache);
+  extern __shared__ __align__(16) unsigned char lds[];
+  const TallyCache cache = cache_init(lds);
   __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_in;
   int outcome = kGone;
   Packet<Real> p;
   DeviceSink sink(t, k.eq);
-  sink.cache = &cache;
+  sink.cache = cache;
   if (valid) {
     p = load_packet(in, i);
     outcome = transport_packet<Real, F>(p, mesh, xs, acc, k, max_segments, sink);
   }
-  sink.finish_block();
   const bool alive = valid && outcome == kAlive;
   const bool census = valid && outcome == kCensus;
   uint32_t slot = wave_append(alive, n_out);
@@ -603,9 +639,9 @@ __global__ __launch_bounds__(256) void k_transport(uint32_t n_in, PacketSoA<Real
   uint32_t cslot = wave_append(census, n_cen);
   if (census) {
     store_census(cen, cslot, p);
-    uint32_t c = flat_cell(mesh, p.ijk);
-    tally_limbs(t.cpre, c, fx::limbs_of(p.w, k.eq));
+    sink.census_weight(flat_cell(mesh, p.ijk), fx::limbs_of(p.w, k.eq));
   }
+  sink.finish_block();
   wave_add_u32((valid && outcome == kEscaped) ? 1u : 0u, &stats[0]);
   wave_max_u32(valid ? p.ctr : 0u, &stats[4]);
   if (acc.cell) {
