@@ -34,6 +34,22 @@ METRIC = "tracked segments/s and s per IMC step, mixed vs FP64 GPU at matched ta
 B_SEG_MIXED = 144.0  # bytes per segment: 2 * 48 B packet + 16 B cell + 2 * 16 B tally
 B_SEG_FP64 = 224.0   # 2 * 80 B packet + 32 B cell + 2 * 16 B tally
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per transport launch, measured by rocprofv3 (FETCH_SIZE and
+# WRITE_SIZE, scaled by the same counters on a plain copy kernel) on this
+# benchmark's own command; the summary holds the figures and the command.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_transport_c2_cube_counters.md")
+
+
+def measured_traffic():
+    """(bytes per launch, source) from the committed counter summary, or (None, None)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1]), os.path.relpath(TRAFFIC_PROFILE, ROOT)
+    except OSError:
+        pass
+    return None, None
 
 
 def run_engine(precision, steps, warmup, seed, log, schedule=None, copy_mib=0):
@@ -117,6 +133,7 @@ def main():
     value = mixed["segments"] / mixed["wall"]
     ms_per_step = mixed["wall"] / args.steps * 1e3
     achieved = mixed["segments"] * B_SEG_MIXED / mixed["transport_s"] / 1e9
+    traffic, traffic_src = measured_traffic() if args.schedule == "pass" else (None, None)
     out = {
         "metric": METRIC,
         "value": value,
@@ -144,7 +161,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": "k_transport<float, 0u>" if args.schedule == "pass" else "k_transport_refill<float, 0u>",
             "bytes_per_segment": B_SEG_MIXED,
         },

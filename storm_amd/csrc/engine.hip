@@ -49,13 +49,14 @@ struct DevBuf {
     p = nullptr;
     n = 0;
   }
-  // grow without keeping the contents; by at least half the capacity, so a
-  // census that grows step by step reallocates only a few times
+  // Grow without keeping the contents, to a power of two at least twice the
+  // old capacity: a census that grows step by step then reallocates a few
+  // times only (hipFree waits for the device and costs about a millisecond).
   void ensure(size_t m) {
     if (m <= n && p) return;
-    const size_t grown = n + n / 2;
+    size_t cap = 1;
+    while (cap < m || cap < 2 * n) cap <<= 1;
     release();
-    size_t cap = std::max<size_t>(std::max(m, grown), 1);
     HIPCHECK(hipMalloc(reinterpret_cast<void **>(&p), cap * sizeof(T)));
     n = cap;
   }
