@@ -177,7 +177,9 @@ int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segm
 
 /* Keep a copy of the next step's transport inputs (initial packets, cell
  * opacities, step constants) and its integer tallies, for an exact CPU replay
- * of the packet random streams.  Sizes first with NULL buffers. */
+ * of the packet random streams.  Sizes first with NULL buffers.  After a step
+ * fails because a packet exceeded the segment limit (error 6), the kept
+ * packets are the (first 4096) packets still in flight instead. */
 int storm_gpu_debug_keep_transport_inputs(storm_gpu_engine *eng, int32_t enable);
 int storm_gpu_debug_transport_inputs(storm_gpu_engine *eng, uint64_t *n_packets, void *packets,
                                      void *cell_xs, int64_t *consts4);

@@ -182,6 +182,30 @@ def replay_transport(spec, packets, n, xs, consts, segments_per_pass=1 << 20, ac
     return t, outc
 
 
+def trace_packet(spec, packet, xs, consts, accel=None, max_seg=100000):
+    """Segment-by-segment states of one engine packet (PacketF32) through the
+    mixed engine's transport on the CPU: array (k, 12) of u0 u1 u2 o0 o1 o2 s
+    w cell ijk ctr outcome after each segment."""
+    L = rlib()
+    f = L.replay_trace_f32
+    f.restype = C.c_int64
+    widths = np.concatenate([np.diff(spec.x_edges), np.diff(spec.y_edges), np.diff(spec.z_edges)]).astype(np.float32)
+    fk = np.asarray(spec.face_kind, dtype=np.int32)
+    cst = np.asarray(consts, dtype=np.int64)
+    xs32 = np.ascontiguousarray(xs, dtype=np.float32)
+    accbuf = None if accel is None else C.create_string_buffer(bytes(accel), len(accel))
+    accp = np.array([spec.with_rw, spec.with_ddmc, spec.rw_particle_tau, spec.ddmc_particle_tau], dtype=np.float64)
+    out = np.zeros((max_seg, 12), dtype=np.float64)
+    nx, ny, nz = spec.shape
+    k = f(C.byref(packet), nx, ny, nz, widths.ctypes.data_as(C.POINTER(C.c_float)),
+          fk.ctypes.data_as(C.POINTER(C.c_int32)), xs32.ctypes.data_as(C.POINTER(C.c_float)),
+          None if accbuf is None else C.cast(accbuf, C.c_void_p), _dp(accp), cst.ctypes.data_as(C.POINTER(C.c_int64)),
+          C.c_int64(max_seg), _dp(out))
+    if k < 0:
+        raise RuntimeError("replay: " + L.replay_last_error().decode())
+    return out[:k]
+
+
 def cpu_first_step(spec, precision: int, step: int = 1):
     """The engine's first step (sourcing + transport, no census) on the CPU."""
     L = rlib()

@@ -262,6 +262,60 @@ int replay_transport_f32(int64_t n, const void *packets, int32_t nx, int32_t ny,
   }
 }
 
+// Segment-by-segment trace of one packet through the engine's transport
+// (mixed engine): after each segment, 12 doubles: u0 u1 u2 o0 o1 o2 s w cell
+// ijk ctr outcome.  At most max_seg segments; returns the number written, or
+// -1 on error.
+int64_t replay_trace_f32(const void *packet, int32_t nx, int32_t ny, int32_t nz, const float *widths,
+                         const int32_t *face_kind, const float *xs, const void *accel, const double *acc_params,
+                         const int64_t *consts, int64_t max_seg, double *trace) {
+  try {
+    MeshView<float> mv;
+    mv.nx = nx;
+    mv.ny = ny;
+    mv.nz = nz;
+    mv.escape_mask = 0u;
+    for (int f = 0; f < 6; ++f)
+      if (face_kind[f] == STORM_FACE_ESCAPE) mv.escape_mask |= 1u << f;
+    mv.width[0] = widths;
+    mv.width[1] = widths + nx;
+    mv.width[2] = widths + nx + ny;
+    mv.spec = SpectrumView<float>{};
+    StepConsts kc;
+    kc.seed = static_cast<uint64_t>(consts[0]);
+    kc.step = static_cast<uint32_t>(consts[1]);
+    kc.eq = static_cast<int32_t>(consts[2]);
+    kc.et = static_cast<int32_t>(consts[3]);
+    const int rw_on = acc_params ? static_cast<int>(acc_params[0]) : 0;
+    const int ddmc_on = acc_params ? static_cast<int>(acc_params[1]) : 0;
+    std::unique_ptr<HostTables<float>> tb(rw_on ? new HostTables<float>() : nullptr);
+    AccelView<float> av = make_accel_view<float>(static_cast<const CellAccel<float> *>(accel), tb.get(), rw_on,
+                                                 ddmc_on, acc_params ? acc_params[2] : 0.0,
+                                                 acc_params ? acc_params[3] : 0.0);
+    Packet<float> p;
+    std::memcpy(&p, packet, sizeof(p));
+    const size_t N = static_cast<size_t>(nx) * ny * nz;
+    std::vector<uint64_t> d(2 * N), r(2 * N), q(2 * N), e(2 * N);
+    std::vector<uint32_t> sg(N);
+    HostSink sink{d.data(), r.data(), q.data(), e.data(), nullptr, nullptr, nullptr, sg.data(), kc.eq};
+    Tracker<float> t;
+    t.begin(p, mv, reinterpret_cast<const CellXS<float> *>(xs), kc);
+    int64_t k = 0;
+    while (k < max_seg && t.outcome == kAlive) {
+      t.segment(mv, reinterpret_cast<const CellXS<float> *>(xs), av, kc, sink);
+      double *o = trace + 12 * k;
+      o[0] = t.u0; o[1] = t.u1; o[2] = t.u2;
+      o[3] = t.o0; o[4] = t.o1; o[5] = t.o2;
+      o[6] = t.s; o[7] = t.w; o[8] = t.cell; o[9] = t.ijk; o[10] = t.ctr; o[11] = t.outcome;
+      ++k;
+    }
+    return k;
+  } catch (const std::exception &ex) {
+    g_err = ex.what();
+    return -1;
+  }
+}
+
 // First IMC step of the engine on the CPU (no census): preStep, photon
 // allocation, emission + boundary births and transport, in the engine's
 // arithmetic (precision 0: float + fixed point, 1: double).  Outputs per cell:

@@ -171,6 +171,35 @@ STORM_HD void put3(int a, T &x, T &y, T &z, T v) {
   z = a == 2 ? v : z;
 }
 
+// Cell-local coordinates are fractions of the cell width.  A float moved by
+// less than half a unit in its last place keeps its value, and near u = 1
+// that unit is 2^-24 of the width: in an optically thick cell a collision
+// flight is far shorter.  A packet lying on a face (after a reflection, say)
+// would then stay pinned there, at distance 0 from the face, never eligible
+// for the random walk, and with its time budget no longer decreasing either
+// once a flight is below the budget's own unit: a history without end (the
+// FP64 reference pins the same way once a flight is below the unit of its
+// absolute coordinates).  So a packet on a face that moves inward by less
+// than a unit is taken one unit inside (the smallest normal number above 0
+// on the lower face), far below the cell size but enough for the random walk
+// to take over in such a cell; elsewhere a sub-unit move of `delta` is taken
+// as one whole unit with probability |delta| / unit (from 8 random bits
+// `r8`), so that the expected move is exact.
+STORM_HD float move_coord(float u, float n, float delta, uint32_t r8) {
+  if (n != u || delta == 0.0f) return n;
+  if (delta > 0.0f ? !(u < 1.0f) : !(u > 0.0f)) return n;
+  if (u == 0.0f) return 1.17549435e-38f;
+  if (u == 1.0f) return 0.99999994f;
+  uint32_t b;
+  __builtin_memcpy(&b, &u, sizeof(b));
+  b = delta > 0.0f ? b + 1u : b - 1u;
+  float next;
+  __builtin_memcpy(&next, &b, sizeof(next));
+  const float unit = delta > 0.0f ? next - u : u - next;
+  const float p = (delta > 0.0f ? delta : -delta) / unit;
+  return (static_cast<float>(r8) + 0.5f) * 0.00390625f < p ? next : u;
+}
+
 // Sink interface (duck-typed):
 //   visit(cell, dep_limbs, path_limbs, segments)  (Real = float) or
 //   visit_fp(cell, dep, path, segments)           (Real = double): a cell visit ends
@@ -616,10 +645,17 @@ struct Tracker {
 
     // Move (the face axis is set exactly below).
     {
-      Real n0 = u0 + o0 * d * icw0, n1 = u1 + o1 * d * icw1, n2 = u2 + o2 * d * icw2;
+      const Real m0 = o0 * d * icw0, m1 = o1 * d * icw1, m2 = o2 * d * icw2;
+      Real n0 = u0 + m0, n1 = u1 + m1, n2 = u2 + m2;
       n0 = n0 < Real(0) ? Real(0) : (n0 > Real(1) ? Real(1) : n0);
       n1 = n1 < Real(0) ? Real(0) : (n1 > Real(1) ? Real(1) : n1);
       n2 = n2 < Real(0) ? Real(0) : (n2 > Real(1) ? Real(1) : n2);
+      if constexpr (fixed_point) {
+        // the low 8 bits of these words are not used by the uniforms above
+        n0 = move_coord(u0, n0, m0, r.v[1] & 0xffu);
+        n1 = move_coord(u1, n1, m1, r.v[2] & 0xffu);
+        n2 = move_coord(u2, n2, m2, r.v[3] & 0xffu);
+      }
       if (!(ev == 0 && fax == 0)) u0 = n0;
       if (!(ev == 0 && fax == 1)) u1 = n1;
       if (!(ev == 0 && fax == 2)) u2 = n2;

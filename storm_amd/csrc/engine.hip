@@ -64,6 +64,8 @@ struct DevBuf {
 
 inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t>((n + b - 1) / b); }
 
+constexpr uint64_t kMaxSegmentsPerStep = uint64_t(1) << 24;
+
 template <typename Real>
 struct PacketBufs {
   DevBuf<typename V4<Real>::T> pos, dir;
@@ -668,9 +670,18 @@ struct Engine final : EngineBase {
     double t_transport = 0.0, t_max = 0.0;
     uint32_t n_cen_total = 0;
     zero_u32(counters_u32.p, 2);
+    // A packet may take at most kMaxSegmentsPerStep segments in one step (a
+    // history that long is a problem set-up the random walk should cover, and
+    // one lane running it would keep the GPU busy for minutes).
+    const uint64_t max_passes = (kMaxSegmentsPerStep + static_cast<uint64_t>(seg_per_pass) - 1) /
+                                    static_cast<uint64_t>(seg_per_pass) + 1;
     while (n_in > 0) {
-      if (counters.transport_passes >= 100000)
-        throw EngineError(6, "transport did not finish within 100000 passes");
+      if (counters.transport_passes >= max_passes) {
+        // the packets still in flight, for storm_gpu_debug_transport_inputs
+        keep_packets(ping ? pa : pb, std::min<uint64_t>(n_in, 4096), kc);
+        throw EngineError(6, "transport did not finish: a packet exceeded 2^24 segments in one step "
+                             "(optically thick cells without the random walk?)");
+      }
       PacketBufs<Real> &src = ping ? pa : pb;
       PacketBufs<Real> &dst = ping ? pb : pa;
       zero_u32(counters_u32.p, 1);  // n_out
@@ -815,18 +826,21 @@ struct Engine final : EngineBase {
     return n_out;
   }
 
-  void keep_transport_inputs(uint64_t n, const StepConsts &kc) {
+  void keep_transport_inputs(uint64_t n, const StepConsts &kc) { keep_packets(pa, n, kc); }
+  // Host copy of the first n packets of `src` with the cell data and step
+  // constants they are tracked with.
+  void keep_packets(PacketBufs<Real> &src, uint64_t n, const StepConsts &kc) {
     kept_packets.resize(n);
     std::vector<typename V4<Real>::T> hp(n), hd(n);
     std::vector<double> hw(n);
     std::vector<uint4> hm(n);
     std::vector<Real> hn(groups >= 2 ? n : 0);
     if (n) {
-      if (groups >= 2) HIPCHECK(hipMemcpy(hn.data(), pa.nu.p, n * sizeof(Real), hipMemcpyDeviceToHost));
-      HIPCHECK(hipMemcpy(hp.data(), pa.pos.p, n * sizeof(hp[0]), hipMemcpyDeviceToHost));
-      HIPCHECK(hipMemcpy(hd.data(), pa.dir.p, n * sizeof(hd[0]), hipMemcpyDeviceToHost));
-      HIPCHECK(hipMemcpy(hw.data(), pa.w.p, n * sizeof(double), hipMemcpyDeviceToHost));
-      HIPCHECK(hipMemcpy(hm.data(), pa.meta.p, n * sizeof(uint4), hipMemcpyDeviceToHost));
+      if (groups >= 2) HIPCHECK(hipMemcpy(hn.data(), src.nu.p, n * sizeof(Real), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(hp.data(), src.pos.p, n * sizeof(hp[0]), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(hd.data(), src.dir.p, n * sizeof(hd[0]), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(hw.data(), src.w.p, n * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(hm.data(), src.meta.p, n * sizeof(uint4), hipMemcpyDeviceToHost));
     }
     for (uint64_t i = 0; i < n; ++i) {
       Packet<Real> &p = kept_packets[i];
