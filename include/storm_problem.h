@@ -117,8 +117,9 @@ typedef struct storm_imc_params {
  *                      group (uniform over the whole range if the sum is 0)
  *   boundary source energy: group g with probability B_g(T_source), E
  *                      uniform in the group.
- * Random walk and DDMC are gray-only here (the partial-gray random walk of
- * RadiationIMC.hpp:1008-1061 is not implemented).
+ * With imc.with_random_walk, packets in the optically thick low groups take
+ * the partial-gray random walk (RadiationIMC.hpp:1008-1061, 1065-1276).  DDMC
+ * is gray-only here.
  */
 typedef struct storm_group_desc {
   int32_t num_groups;

@@ -970,8 +970,8 @@ class RadiationIMC {
       : grid_(grid), boundary_(boundary), cells_(cells), ext_(ext), mat_(mat), params_(params),
         rng_(seed), sampler_(grid) {
     if (params_.newPhotonsPerCell == 0) throw OracleError("RadiationIMC requires newPhotonsPerCell > 0");
-    if (mat_.multigroup() && (params_.withRandomWalk || params_.withDDMC))
-      throw OracleError("random walk and DDMC are gray-only (no partial-gray random walk)");
+    if (mat_.multigroup() && params_.withDDMC)
+      throw OracleError("DDMC is gray-only (no multigroup DDMC)");
     if (params_.withRandomWalk) rw_ = std::make_unique<RandomWalk>();
     Erad_time_avg_.assign(grid.GetPointNo(), 0.0);
     updateGridData();
@@ -1254,18 +1254,58 @@ class RadiationIMC {
     return s;
   }
 
-  // precomputeRandomWalkData (radiation/RadiationIMC.hpp:982-1063), gray
+  // precomputeRandomWalkData (radiation/RadiationIMC.hpp:982-1063): gray, and
+  // the partial-gray random walk of multigroup runs (:1008-1060): the groups
+  // from the lowest up, as long as each is optically thick over the cell's
+  // mean chord, form the diffusive part; their Planck-weighted opacities
+  // drive the walk.
   void precomputeRandomWalkData() {
     size_t N = grid_.GetPointNo();
     rwEligible_.assign(N, 0);
     rwSigmaT_.assign(N, 0.0);
+    if (mat_.multigroup()) pgrw_.assign(N, PGRWCellData{});
     for (size_t i = 0; i < N; ++i) {
       double ss = mat_.CalcScatteringOpacity(i, cells_[i].temperature);
       double sT = planck_[i] + ss;
       rwSigmaT_[i] = sT;
       double S = surfaceArea(i), V = grid_.GetVolume(i);
       double chord = (S > 0.0) ? 4.0 * V / S : 0.0;
-      rwEligible_[i] = (sT * chord >= params_.rwMinCellOpticalDepth) ? 1 : 0;
+      if (!mat_.multigroup()) {
+        rwEligible_[i] = (sT * chord >= params_.rwMinCellOpticalDepth) ? 1 : 0;
+        continue;
+      }
+      const size_t G = mat_.groups();
+      const double kT = units::k_boltz * cells_[i].temperature;
+      double allAB = 0.0, sB = 0.0, sBA = 0.0, sBT = 0.0, sBoT = 0.0;
+      size_t cutoff = 0;
+      bool nonDiffusive = false;
+      for (size_t g = 0; g < G; ++g) {
+        const double a = mat_.bounds[g] / kT, b = mat_.bounds[g + 1] / kT;
+        const double Bg = (a > 0.0 && b > a) ? storm::planck::integral(a, b) : 1.0 / static_cast<double>(G);
+        const double sA = mat_.CalcAbsorptionOpacity(i, cells_[i].temperature,
+                                                     0.5 * (mat_.bounds[g] + mat_.bounds[g + 1]));
+        const double sTg = sA + ss;
+        allAB += sA * Bg;
+        if (!nonDiffusive && sTg * chord >= params_.rwMinCellOpticalDepth) {
+          cutoff = g + 1;
+          sB += Bg;
+          sBA += Bg * sA;
+          sBT += Bg * sTg;
+          if (sTg > 0.0) sBoT += Bg / sTg;
+        } else {
+          nonDiffusive = true;
+        }
+      }
+      if (cutoff > 0 && sB > 0.0) {
+        PGRWCellData &d = pgrw_[i];
+        d.groupCutoff = cutoff;
+        d.sigmaA = sBA / sB;
+        d.sigmaT = sBT / sB;
+        d.D = (units::clight / 3.0) * sBoT / sB;
+        d.gamma = (allAB > 0.0) ? sBA / allAB : 1.0;
+        rwSigmaT_[i] = d.sigmaT;
+        rwEligible_[i] = 1;
+      }
     }
   }
 
@@ -1273,16 +1313,36 @@ class RadiationIMC {
   bool tryRandomWalkStep(Particle &p, StepResult &fn) {
     size_t c = p.cellIndex;
     double Ro = minDistanceToFaces(c, p.location);
+    const bool pg = mat_.multigroup();
     double sigmaT = rwSigmaT_[c];
     double sa = planck_[c];
     double D = (sigmaT > 0.0) ? units::clight / (3.0 * sigmaT) : 0.0;
+    double gamma = 1.0;
+    size_t cutoff = 0;
+    if (pg) {
+      const PGRWCellData &d = pgrw_[c];
+      sigmaT = d.sigmaT;
+      sa = d.sigmaA;
+      D = d.D;
+      gamma = d.gamma;
+      cutoff = d.groupCutoff;
+    }
     bool doRW = (Ro > 0.0 && sigmaT > 0.0 && D > 0.0 && Ro * sigmaT >= params_.rwMinParticleOpticalDepth);
+    if (doRW && pg) {
+      // only packets in the diffusive groups walk (:1094-1114)
+      const double E = std::clamp(p.frequency, mat_.bounds.front(), mat_.bounds.back());
+      if (E >= mat_.bounds[cutoff]) doRW = false;
+    }
     if (!doRW) return false;
     double f = factorFleck_[c];
     double tauLeak = rw_->sampleLeakTime(rng_.unitOpen());
     double tLeak = tauLeak * Ro * Ro / D;
     double tCensus = p.timeLeft;
     double tUp = std::numeric_limits<double>::max();  // gray: no upscatter
+    if (pg && gamma < 1.0 && sa > 0.0 && f > 0.0) {
+      // absorption re-emitted above the diffusive groups ends the walk (:1126-1131)
+      tUp = -std::log(rng_.unitOpen()) / (units::clight * (1.0 - f) * sa * (1.0 - gamma));
+    }
     int ev;
     double dt;
     if (tLeak <= tCensus && tLeak <= tUp) { ev = 0; dt = tLeak; }
@@ -1325,6 +1385,20 @@ class RadiationIMC {
       }
     }
     p.velocity = isotropicVelocity(rng_);
+    if (ev == 2) {
+      // re-emission above the cutoff (:1237-1252): the cell's emission
+      // spectrum restricted to the non-diffusive groups
+      const std::vector<double> &cdf = thermalCdf_[c];
+      const size_t G = mat_.groups();
+      const double atCut = cdf[cutoff], total = cdf[G];
+      if (total > atCut) {
+        const double lo = atCut / total;
+        p.frequency = mat_.thermalEnergyFromCumulative(cdf, lo + rng_.unitOpen() * (1.0 - lo));
+      } else {
+        p.frequency = std::nextafter(mat_.bounds[cutoff], std::numeric_limits<double>::max());
+      }
+      p.frequency = std::clamp(p.frequency, mat_.bounds.front(), mat_.bounds.back());
+    }
     if (ev == 1) fn.change = DONE;
     return true;
   }
@@ -1468,6 +1542,12 @@ class RadiationIMC {
   std::unique_ptr<RandomWalk> rw_;
   std::vector<char> rwEligible_;
   std::vector<double> rwSigmaT_;
+  // partial-gray random walk data (PGRWCellData, RadiationIMC.hpp:1044-1051)
+  struct PGRWCellData {
+    size_t groupCutoff = 0;
+    double sigmaA = 0.0, sigmaT = 0.0, D = 0.0, gamma = 1.0;
+  };
+  std::vector<PGRWCellData> pgrw_;
   std::vector<DDMCCellData> ddmc_;
   std::vector<std::vector<double>> thermalCdf_;
 };

@@ -113,7 +113,11 @@ std::vector<CellAccel<Real>> build_accel(const storm_problem *p, const std::vect
     a.total = Real(0);
     for (int f = 0; f < 6; ++f) a.rate[f] = Real(0);
     a.flags = 0u;
-    a.pad = 0u;
+    a.gcut = 0u;
+    a.sig_d = a.sig_t;
+    a.sig_abs = Real(0);
+    a.up_rate = Real(0);
+    a.e_cut = std::numeric_limits<Real>::infinity();
     if (p->imc.with_random_walk && st * chord >= p->imc.rw_min_cell_optical_depth) a.flags |= 1u;
     Dt[c] = st > 0.0 ? 1.0 / (3.0 * st) : 0.0;
     if (p->imc.with_ddmc && st * chord >= p->imc.ddmc_min_cell_optical_depth && Dt[c] > 0.0) a.flags |= 2u;
@@ -143,6 +147,68 @@ std::vector<CellAccel<Real>> build_accel(const storm_problem *p, const std::vect
     if (!(total > 0.0)) {
       a.flags &= ~2u;
       for (int f = 0; f < 6; ++f) a.rate[f] = Real(0);
+    }
+  }
+  return out;
+}
+
+// The engine's partial-gray random-walk cell data (k_prestep_mg): T, gray
+// sigma_a and sigma_s, Fleck factors per cell.
+template <typename Real>
+std::vector<CellAccel<Real>> build_accel_mg(const storm_problem *p, const std::vector<double> &sa,
+                                            const std::vector<double> &ss, const std::vector<double> &f) {
+  const int nx = p->mesh.nx, ny = p->mesh.ny, nz = p->mesh.nz;
+  const uint32_t N = static_cast<uint32_t>(nx) * ny * nz;
+  const int32_t G = p->groups.num_groups;
+  const double *gb = p->groups.bounds;
+  std::vector<CellAccel<Real>> out(N);
+  auto w = [&](int ax, int idx) {
+    const double *e = ax == 0 ? p->mesh.x_edges : (ax == 1 ? p->mesh.y_edges : p->mesh.z_edges);
+    return static_cast<double>(static_cast<Real>(e[idx + 1] - e[idx]));
+  };
+  for (uint32_t c = 0; c < N; ++c) {
+    int k = static_cast<int>(c % nz), j = static_cast<int>((c / nz) % ny),
+        i = static_cast<int>(c / (static_cast<uint32_t>(ny) * nz));
+    const double dx = w(0, i), dy = w(1, j), dz = w(2, k);
+    const double chord = 4.0 * dx * dy * dz / (2.0 * (dy * dz + dx * dz + dx * dy));
+    const double kTrw = units::k_boltz * p->T0[c];
+    double allAB = 0.0, dB = 0.0, dBA = 0.0, dBT = 0.0, dBoT = 0.0;
+    uint32_t cut = 0;
+    bool thin = false;
+    for (int32_t g = 0; g < G; ++g) {
+      const double Ec = 0.5 * (gb[g] + gb[g + 1]);
+      const double sg = sa[c] * std::pow(std::fmax(Ec, gb[0]) / p->groups.nu_ref, -p->groups.nu_pow);
+      const double ar = gb[g] / kTrw, br = gb[g + 1] / kTrw;
+      const double Br = (ar > 0.0 && br > ar) ? planck::integral(ar, br) : 1.0 / static_cast<double>(G);
+      const double st = sg + ss[c];
+      allAB += sg * Br;
+      if (!thin && st * chord >= p->imc.rw_min_cell_optical_depth) {
+        cut = static_cast<uint32_t>(g) + 1u;
+        dB += Br;
+        dBA += Br * sg;
+        dBT += Br * st;
+        if (st > 0.0) dBoT += Br / st;
+      } else {
+        thin = true;
+      }
+    }
+    CellAccel<Real> &r = out[c];
+    r.total = Real(0);
+    for (int q = 0; q < 6; ++q) r.rate[q] = Real(0);
+    r.flags = 0u;
+    r.gcut = 0u;
+    r.sig_t = r.sig_d = r.sig_abs = r.up_rate = Real(0);
+    r.e_cut = std::numeric_limits<Real>::infinity();
+    if (cut > 0u && dB > 0.0) {
+      const double sA = dBA / dB, sT = dBT / dB;
+      const double gamma = allAB > 0.0 ? dBA / allAB : 1.0;
+      r.flags = 1u;
+      r.gcut = cut;
+      r.sig_t = static_cast<Real>(sT);
+      r.sig_d = static_cast<Real>(dBoT > 0.0 ? dB / dBoT : 0.0);
+      r.sig_abs = static_cast<Real>(f[c] * sA);
+      r.up_rate = static_cast<Real>((gamma < 1.0 && sA > 0.0 && f[c] > 0.0) ? (1.0 - f[c]) * sA * (1.0 - gamma) : 0.0);
+      r.e_cut = static_cast<Real>(gb[cut] / units::kev);
     }
   }
   return out;
@@ -472,7 +538,7 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       std::vector<CellAccel<Real>> acc_cells;
       std::unique_ptr<HostTables<Real>> tb;
       if (accel_on) {
-        acc_cells = build_accel<Real>(p, sa, ss);
+        acc_cells = G ? build_accel_mg<Real>(p, sa, ss, f) : build_accel<Real>(p, sa, ss);
         if (p->imc.with_random_walk) tb.reset(new HostTables<Real>());
       }
       AccelView<Real> av = make_accel_view<Real>(accel_on ? acc_cells.data() : nullptr, tb.get(),

@@ -101,18 +101,35 @@ struct CellXS {
   Real sig_abs, sig_ev;
 };
 
-// Per-cell data of the diffusion accelerations (gray):
-//   flags bit 0 random-walk cell, bit 1 DDMC cell
-//   sig_t  sigma_a + sigma_s (cm^-1)
-//   rate   DDMC leak rate per face / c (cm^-1), order x-, x+, y-, y+, z-, z+
-//   total  sum of rate
+// Per-cell data of the diffusion accelerations:
+//   flags   bit 0 random-walk cell, bit 1 DDMC cell
+//   sig_t   sigma_a + sigma_s (cm^-1); multigroup: the Planck mean of
+//           sigma_a(E) + sigma_s over the diffusive groups (the walk's
+//           particle threshold opacity)
+//   rate    DDMC leak rate per face / c (cm^-1), order x-, x+, y-, y+, z-, z+
+//   total   sum of rate
+//   gcut    multigroup: the diffusive groups are 0 .. gcut-1
+//   sig_d   opacity of the walk's diffusion coefficient, D = c / (3 sig_d)
+//           (gray: sig_t; multigroup: the Planck-weighted harmonic mean of
+//           sigma_a(E) + sigma_s over the diffusive groups)
+//   sig_abs multigroup: f times the Planck mean of sigma_a(E) over the
+//           diffusive groups, the continuous absorption during a walk
+//   up_rate multigroup: (1 - f) sigma_a,mean (1 - gamma) (cm^-1), the rate of
+//           re-emission above the diffusive groups, which ends a walk
+//   e_cut   multigroup: lower bound (keV) of group gcut; packets at or above
+//           it do not walk (+inf gray)
+// (precomputeRandomWalkData, radiation/RadiationIMC.hpp:983-1063)
 template <typename Real>
 struct CellAccel {
   Real sig_t;
   Real total;
   Real rate[6];
   uint32_t flags;
-  uint32_t pad;
+  uint32_t gcut;
+  Real sig_d;
+  Real sig_abs;
+  Real up_rate;
+  Real e_cut;
 };
 
 // Random-walk tables (see random_walk.h) and the particle thresholds.
@@ -198,6 +215,22 @@ STORM_HD float move_coord(float u, float n, float delta, uint32_t r8) {
   const float unit = delta > 0.0f ? next - u : u - next;
   const float p = (delta > 0.0f ? delta : -delta) / unit;
   return (static_cast<float>(r8) + 0.5f) * 0.00390625f < p ? next : u;
+}
+
+// The next representable number above a positive x.
+STORM_HD float next_above(float x) {
+  uint32_t b;
+  __builtin_memcpy(&b, &x, sizeof(b));
+  ++b;
+  __builtin_memcpy(&x, &b, sizeof(x));
+  return x;
+}
+STORM_HD double next_above(double x) {
+  uint64_t b;
+  __builtin_memcpy(&b, &x, sizeof(b));
+  ++b;
+  __builtin_memcpy(&x, &b, sizeof(x));
+  return x;
 }
 
 // Sink interface (duck-typed):
@@ -451,18 +484,31 @@ struct Tracker {
           Ro = m2 < Ro ? m2 : Ro;
         }
         const Real tauR = Ro * ca.sig_t;
-        if ((ca.flags & 1u) && Ro > Real(0) && ca.sig_t > Real(0) && tauR >= acc.rw_particle_tau) {
-          // ---------------- random-walk step (tryRandomWalkStep, gray)
+        // multigroup: the partial-gray walk, for packets in the diffusive groups only (:1094-1114)
+        const bool pgrw = (F & kFeatMultigroup) != 0u && m.spec.groups >= 2;
+        const bool walks = !pgrw || nu < ca.e_cut;
+        if ((ca.flags & 1u) && walks && Ro > Real(0) && ca.sig_t > Real(0) && tauR >= acc.rw_particle_tau) {
+          // ---------------- random-walk step (tryRandomWalkStep)
           const Philox4 r = rs.block(ctr++);
           const Philox4 r2 = rs.block(ctr++);
           rw::Tables<Real> tb{acc.rw_tau, acc.rw_survival, acc.rw_radius, acc.rw_log_tau_min, acc.rw_log_tau_max};
           const Real tauLeak = rw::sample_leak_time<Real>(tb, M::uniform(r.v[0]));
-          // tLeak = tau Ro^2 / D with D = c / (3 sigma_t): as a path, tau Ro^2 3 sigma_t
-          const Real sLeak = tauLeak * Ro * Ro * (Real(3) * ca.sig_t);
-          const bool leak = sLeak <= s;
-          const Real d = leak ? sLeak : s;
-          absorb(sig_abs, d, false);
-          s = leak ? s - d : Real(0);
+          // tLeak = tau Ro^2 / D with D = c / (3 sig_d): as a path, tau Ro^2 3 sig_d
+          const Real sig_d = pgrw ? ca.sig_d : ca.sig_t;
+          const Real sLeak = tauLeak * Ro * Ro * (Real(3) * sig_d);
+          // multigroup: re-emission above the diffusive groups (:1126-1131)
+          Real sUp = M::inf();
+          if (pgrw && ca.up_rate > Real(0)) sUp = (-M::log(M::uniform(r2.v[2]))) / ca.up_rate;
+          // leak, census, upscatter; ties in that order (:1133-1152)
+          int rev = 0;
+          Real d = sLeak;
+          if (!(sLeak <= s && sLeak <= sUp)) {
+            rev = s <= sUp ? 1 : 2;
+            d = rev == 1 ? s : sUp;
+          }
+          const bool leak = rev == 0;
+          absorb(pgrw ? ca.sig_abs : sig_abs, d, false);
+          s = rev == 1 ? Real(0) : s - d;
           if (s < Real(0)) s = Real(0);
           sink.rw_step();
           if (w < static_cast<double>(wmin * Real(0.1))) {
@@ -474,7 +520,7 @@ struct Tracker {
           isotropic_dir<Real>(M::uniform(r.v[1]), M::uniform(r.v[2]), dir);
           Real disp = Ro;
           if (!leak) {
-            const Real tauPos = d / (Real(3) * ca.sig_t * Ro * Ro);
+            const Real tauPos = d / (Real(3) * sig_d * Ro * Ro);
             disp = Ro * rw::sample_radius<Real>(tb, tauPos, M::uniform(r.v[3]));
           }
           if (disp > Ro) disp = Ro;
@@ -498,7 +544,21 @@ struct Tracker {
           Real nd[3];
           isotropic_dir<Real>(M::uniform(r2.v[0]), M::uniform(r2.v[1]), nd);
           set_dir(nd[0], nd[1], nd[2]);
-          if (!leak) census(k, sink);
+          if ((F & kFeatMultigroup) != 0u && rev == 2) {
+            // a new energy from the cell's emission spectrum above the
+            // diffusive groups (:1237-1252)
+            const int32_t G = m.spec.groups;
+            const Real *cdfc = m.spec.cdf + static_cast<size_t>(cell) * static_cast<size_t>(G + 1);
+            const Real lo = cdfc[ca.gcut];
+            if (lo < Real(1)) {
+              nu = spectrum_quantile<Real>(cdfc, m.spec.bounds, G, lo + M::uniform(r2.v[3]) * (Real(1) - lo));
+            } else {
+              nu = next_above(m.spec.bounds[ca.gcut]);
+            }
+            nu = nu < m.spec.e_lo ? m.spec.e_lo : (nu > m.spec.e_hi ? m.spec.e_hi : nu);
+            load_opacity(m, xs);
+          }
+          if (rev == 1) census(k, sink);
           return outcome;
         }
         if ((ca.flags & 2u) && ca.total > Real(0) && tauR >= acc.ddmc_particle_tau) {

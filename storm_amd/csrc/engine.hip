@@ -218,8 +218,7 @@ struct Engine final : EngineBase {
       nu_pow = p->groups.nu_pow;
       nu_ref = p->groups.nu_ref;
       if (!(nu_ref > 0.0)) throw EngineError(1, "groups.nu_ref must be positive");
-      if (accel_on)
-        throw EngineError(1, "random walk and DDMC are gray-only (the partial-gray random walk is not implemented)");
+      if (imc.with_ddmc) throw EngineError(1, "DDMC is gray-only (multigroup DDMC is not implemented)");
     }
     // one packet per thread per pass: measured faster than the refill
     // schedule on the 64^3 cube for both engines
@@ -526,13 +525,14 @@ struct Engine final : EngineBase {
       k_prestep_mg<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
           N, T.p, md, have_host_sa ? host_sa.p : nullptr, have_host_ss ? host_ss.p : nullptr,
           have_host_cv ? host_cv.p : nullptr, V.p, dt, groups, bounds_erg.p, nu_pow, nu_ref, xs.p, cdf.p,
-          sig_s_cell.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p);
+          sig_s_cell.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p, d, meshview(), imc.rw_min_cell_optical_depth,
+          accel_on ? accel.p : nullptr);
     } else {
       k_prestep<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
           N, T.p, md, have_host_sa ? host_sa.p : nullptr, have_host_ss ? host_ss.p : nullptr,
           have_host_cv ? host_cv.p : nullptr, V.p, dt, xs.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p);
     }
-    if (accel_on) {
+    if (accel_on && groups < 2) {
       const MeshView<Real> mv0 = meshview();
       k_accel_cells<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
           N, d, mv0, sig_a.p, sig_s.p, imc.with_random_walk, imc.rw_min_cell_optical_depth, imc.with_ddmc,

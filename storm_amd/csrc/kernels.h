@@ -191,6 +191,18 @@ struct MatDev {
 // Error flags raised by kernels (checked by the host after the step).
 enum : uint32_t { kErrFleck = 1u, kErrNegativeEnergy = 2u, kErrTallyOverflow = 4u, kErrCapacity = 8u };
 
+struct MeshDims {
+  uint32_t nx, ny, nz;
+};
+
+__device__ __forceinline__ uint32_t ijk_of_flat(uint32_t c, MeshDims d) {
+  uint32_t k = c % d.nz;
+  uint32_t t = c / d.nz;
+  uint32_t j = t % d.ny;
+  uint32_t i = t / d.ny;
+  return pack_ijk(i, j, k);
+}
+
 // ------------------------------------------------------------------ preStep
 template <typename Real>
 __global__ __launch_bounds__(256) void k_prestep(uint32_t N, const double *__restrict__ T, MatDev mat,
@@ -233,7 +245,8 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
     int32_t G, const double *__restrict__ bounds, double nu_pow, double nu_ref, CellXS<Real> *__restrict__ xs,
     Real *__restrict__ cdf, Real *__restrict__ sig_s_cell, double *__restrict__ fleck,
     double *__restrict__ sig_a_out, double *__restrict__ sig_s_out, double *__restrict__ Eemit,
-    uint32_t *__restrict__ err) {
+    uint32_t *__restrict__ err, MeshDims dims, MeshView<Real> mv, double rw_cell_tau,
+    CellAccel<Real> *__restrict__ rw_out) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
   const double Tc = T[c];
@@ -243,6 +256,19 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
   const double kT = units::k_boltz * fmax(Tc, mat.Tfloor);
   Real *my = cdf + static_cast<size_t>(c) * static_cast<size_t>(G + 1);
   double acc = 0.0, sumB = 0.0;
+  // partial-gray random walk (precomputeRandomWalkData, RadiationIMC.hpp:1008-1060):
+  // the groups from the lowest up while each is optically thick over the
+  // cell's mean chord; Planck-weighted sums over them
+  double chord = 0.0;
+  if (rw_out) {
+    const uint32_t ijk = ijk_of_flat(c, dims);
+    const double dx = mv.width[0][ijk_i(ijk)], dy = mv.width[1][ijk_j(ijk)], dz = mv.width[2][ijk_k(ijk)];
+    chord = 4.0 * dx * dy * dz / (2.0 * (dy * dz + dx * dz + dx * dy));
+  }
+  const double kTrw = units::k_boltz * Tc;
+  double allAB = 0.0, dB = 0.0, dBA = 0.0, dBT = 0.0, dBoT = 0.0;
+  uint32_t cut = 0;
+  bool thin = false;
   my[0] = Real(0);
   for (int32_t g = 0; g < G; ++g) {
     const double a = bounds[g] / kT, b = bounds[g + 1] / kT;
@@ -253,6 +279,21 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
     sumB += Bg;
     acc += wg;
     my[g + 1] = static_cast<Real>(acc);
+    if (rw_out) {
+      const double ar = bounds[g] / kTrw, br = bounds[g + 1] / kTrw;
+      const double Br = (ar > 0.0 && br > ar) ? planck::integral(ar, br) : 1.0 / static_cast<double>(G);
+      const double st = sg + ss;
+      allAB += sg * Br;
+      if (!thin && st * chord >= rw_cell_tau) {
+        cut = static_cast<uint32_t>(g) + 1u;
+        dB += Br;
+        dBA += Br * sg;
+        dBT += Br * st;
+        if (st > 0.0) dBoT += Br / st;
+      } else {
+        thin = true;
+      }
+    }
   }
   const bool valid = acc > 0.0 && isfinite(acc);
   for (int32_t g = 1; g <= G; ++g) {
@@ -274,6 +315,27 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
   x.sig_ev = static_cast<Real>((1.0 - f) * s1);
   xs[c] = x;
   sig_s_cell[c] = static_cast<Real>(ss);
+  if (rw_out) {
+    CellAccel<Real> r;
+    r.total = Real(0);
+    for (int q = 0; q < 6; ++q) r.rate[q] = Real(0);
+    r.flags = 0u;
+    r.gcut = 0u;
+    r.sig_t = r.sig_d = r.sig_abs = r.up_rate = Real(0);
+    r.e_cut = static_cast<Real>(INFINITY);
+    if (cut > 0u && dB > 0.0) {
+      const double sA = dBA / dB, sT = dBT / dB;
+      const double gamma = allAB > 0.0 ? dBA / allAB : 1.0;
+      r.flags = 1u;
+      r.gcut = cut;
+      r.sig_t = static_cast<Real>(sT);
+      r.sig_d = static_cast<Real>(dBoT > 0.0 ? dB / dBoT : 0.0);
+      r.sig_abs = static_cast<Real>(f * sA);
+      r.up_rate = static_cast<Real>((gamma < 1.0 && sA > 0.0 && f > 0.0) ? (1.0 - f) * sA * (1.0 - gamma) : 0.0);
+      r.e_cut = static_cast<Real>(bounds[cut] / units::kev);
+    }
+    rw_out[c] = r;
+  }
 }
 
 // photons per cell and packet weight, quantised down to the step quantum
@@ -306,17 +368,6 @@ __device__ __forceinline__ Real uni(uint32_t x) {
   return MathPolicy<Real>::uniform(x);
 }
 
-struct MeshDims {
-  uint32_t nx, ny, nz;
-};
-
-__device__ __forceinline__ uint32_t ijk_of_flat(uint32_t c, MeshDims d) {
-  uint32_t k = c % d.nz;
-  uint32_t t = c / d.nz;
-  uint32_t j = t % d.ny;
-  uint32_t i = t / d.ny;
-  return pack_ijk(i, j, k);
-}
 
 // Census re-entry: timeLeft = dt, initialWeight = |w| (MonteCarloManagerSerial.hpp:393-405).
 // Weights are re-quantised if the quantum grew; the remainder (in the old
@@ -768,7 +819,11 @@ __global__ __launch_bounds__(256) void k_accel_cells(uint32_t N, MeshDims d, Mes
   a.total = Real(0);
   for (int f = 0; f < 6; ++f) a.rate[f] = Real(0);
   a.flags = 0u;
-  a.pad = 0u;
+  a.gcut = 0u;
+  a.sig_d = a.sig_t;
+  a.sig_abs = Real(0);
+  a.up_rate = Real(0);
+  a.e_cut = static_cast<Real>(INFINITY);
   if (rw_on && st * chord >= rw_cell_tau) a.flags |= 1u;
   const double Dt = st > 0.0 ? 1.0 / (3.0 * st) : 0.0;
   Dtil[c] = Dt;

@@ -357,13 +357,13 @@ def marshak_shape(x_sim, T_sim, ref):
 
 
 def densmore(nx: int = 512, seed: int = 0, new_photons: int = 16, boundary_photons: int = 100, steps: int = 200,
-             groups: int = 30, comb_nmin: int = 200, comb_factor: float = 5.0) -> ProblemSpec:
+             groups: int = 30, comb_nmin: int = 200, comb_factor: float = 5.0, random_walk: bool = True) -> ProblemSpec:
     """Densmore et al. (2012) step-opacity benchmark as examples/densmore2012/main.cpp:128-200
     sets it up: slab [0, 3] cm, sigma(E) = sigma0 / (sqrt(kT) E^3) with sigma0 = 10
     keV^3.5/cm for x < 2 cm and 1000 beyond, cv = 1e15 erg/(cm^3 keV), T0 = 1 eV,
     a 1 keV x- source (escape there, reflect elsewhere), 30 log-spaced groups over
-    1e-4..1e2 keV, dt = 5e-12 s, comb(200, 5).  The reference also switches the
-    random walk on (partial-gray in multigroup); it is off here."""
+    1e-4..1e2 keV, dt = 5e-12 s, comb(200, 5), and the random walk on (partial-gray
+    in multigroup, main.cpp:208-209) with the default thresholds."""
     L, xstep = 3.0, 2.0
     xe = np.linspace(0.0, L, nx + 1)
     h = L / nx
@@ -378,7 +378,7 @@ def densmore(nx: int = 512, seed: int = 0, new_photons: int = 16, boundary_photo
         eos_f=1e15 / KEV_K, eos_beta=1.0, eos_mu=0.0,
         face_kind=(capi.FACE_ESCAPE,) + (capi.FACE_REFLECT,) * 5, source_face=capi.XLO, source_T=KEV_K,
         freeze_T4=0, source_per_face=boundary_photons, new_photons_per_cell=new_photons,
-        comb=1, comb_nmin=comb_nmin, comb_factor=comb_factor, seed=seed,
+        comb=1, comb_nmin=comb_nmin, comb_factor=comb_factor, seed=seed, with_rw=int(random_walk),
         group_bounds=bounds, nu_pow=3.0, nu_ref=KEV)
 
 

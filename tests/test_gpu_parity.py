@@ -72,8 +72,16 @@ C3 = lambda s: P.c3_step(seed=s, nx=32, nyz=2, packets_scale=0.1, steps=3)  # no
 
 def MG(s):
     """16-cell, 30-group Densmore slab started at 0.3 keV, 3 steps."""
-    spec = P.densmore(nx=16, seed=s, steps=3, new_photons=6, boundary_photons=300, comb_nmin=8, comb_factor=1.0)
+    spec = P.densmore(nx=16, seed=s, steps=3, new_photons=6, boundary_photons=300, comb_nmin=8, comb_factor=1.0,
+                      random_walk=False)
     spec.T0 = np.full(16, 0.3 * P.KEV_K)
+    return spec
+
+
+def MG_RW(s):
+    """The same with the partial-gray random walk."""
+    spec = MG(s)
+    spec.with_rw = 1
     return spec
 
 
@@ -109,11 +117,23 @@ def mg_ref(oracle):
     return oracle_ensemble(oracle, MG, range(1000, 1000 + NSEEDS))
 
 
+@pytest.fixture(scope="module")
+def mg_rw_ref(oracle):
+    return oracle_ensemble(oracle, MG_RW, range(1000, 1000 + NSEEDS))
+
+
 @pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
 def test_multigroup_matches_oracle(mg_ref, precision):
     """30-group transport with energy-dependent opacity and re-emission."""
     got = gpu_ensemble(MG, range(NSEEDS), precision)
     _assert_parity(mg_ref, got, f"mg/{precision}")
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
+def test_multigroup_random_walk_matches_oracle(mg_rw_ref, precision):
+    """The same with the partial-gray random walk (RadiationIMC.hpp:1008-1060, 1065-1276)."""
+    got = gpu_ensemble(MG_RW, range(NSEEDS), precision)
+    _assert_parity(mg_rw_ref, got, f"mg_rw/{precision}")
 
 
 @pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
@@ -163,7 +183,7 @@ def test_fp64_energy_balance():
             assert L["census_pre"] == L["census_out"] + L["comb_rem"]
 
 
-@pytest.mark.parametrize("make", [C1, C2, C3, MG], ids=["c1", "c2", "c3", "multigroup"])
+@pytest.mark.parametrize("make", [C1, C2, C3, MG, MG_RW], ids=["c1", "c2", "c3", "multigroup", "multigroup_rw"])
 def test_replay_reproduces_tallies_bit_exactly(oracle, make):
     spec = make(5)
     with GpuEngine(spec, capi.MIXED) as e:

@@ -170,8 +170,16 @@ def test_oracle_comb_conserves_and_targets(oracle):
 
 def _densmore_small(s):
     # 16-cell Densmore slab at 0.3 keV so that cells emit across the groups
-    spec = P.densmore(nx=16, seed=s, steps=1, new_photons=6, boundary_photons=300, comb_nmin=4, comb_factor=1.0)
+    spec = P.densmore(nx=16, seed=s, steps=1, new_photons=6, boundary_photons=300, comb_nmin=4, comb_factor=1.0,
+                      random_walk=False)
     spec.T0 = np.full(16, 0.3 * P.KEV_K)
+    return spec
+
+
+def _densmore_small_rw(s):
+    # the same with the partial-gray random walk (the reference's setting)
+    spec = _densmore_small(s)
+    spec.with_rw = 1
     return spec
 
 
@@ -184,7 +192,7 @@ def _two_sides(s):
 
 
 @pytest.mark.parametrize("precision", [0, 1], ids=["mixed", "fp64"])
-@pytest.mark.parametrize("which", ["c1", "c2", "multigroup", "two_sides"])
+@pytest.mark.parametrize("which", ["c1", "c2", "multigroup", "multigroup_rw", "two_sides"])
 def test_engine_first_step_physics_on_cpu(oracle, precision, which):
     """The engine's shared sourcing + transport code, run on the CPU, agrees
     with the oracle's first step (per-cell Erad time average and segments)."""
@@ -192,6 +200,8 @@ def test_engine_first_step_physics_on_cpu(oracle, precision, which):
         make = lambda s: P.c1_slab(seed=s, packets_scale=0.05, steps=1)  # noqa: E731
     elif which == "multigroup":
         make = _densmore_small
+    elif which == "multigroup_rw":
+        make = _densmore_small_rw
     elif which == "two_sides":
         make = _two_sides
     else:

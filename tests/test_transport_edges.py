@@ -44,9 +44,16 @@ def thick_slab():
     return spec
 
 
+# storm::CellAccel<float> (storm_amd/csrc/common/transport.h)
+CELL_ACCEL_F32 = np.dtype([("sig_t", "<f4"), ("total", "<f4"), ("rate", "<f4", 6), ("flags", "<u4"), ("gcut", "<u4"),
+                           ("sig_d", "<f4"), ("sig_abs", "<f4"), ("up_rate", "<f4"), ("e_cut", "<f4")])
+
+
 def accel_bytes(n, sig_t):
-    rec = np.zeros(n, dtype=[("sig_t", "<f4"), ("total", "<f4"), ("rate", "<f4", 6), ("flags", "<u4"), ("pad", "<u4")])
+    rec = np.zeros(n, dtype=CELL_ACCEL_F32)
     rec["sig_t"] = sig_t
+    rec["sig_d"] = sig_t  # gray walk: D = c / (3 sigma_t)
+    rec["e_cut"] = np.inf
     rec["flags"] = 1  # random-walk cell
     return rec.tobytes()
 
