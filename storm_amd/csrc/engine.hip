@@ -159,7 +159,7 @@ struct Engine final : EngineBase {
   DevBuf<CellXS<Real>> xs;
   DevBuf<uint32_t> nphot, err, cnt_u32, start_u32, ncopy, vals, vals_out, counters_u32;
   DevBuf<uint64_t> offsets, keys, keys_out, offs64, nphot64;
-  DevBuf<unsigned long long> tally_u64, emit_limbs, W, requant, leaving, scal;
+  DevBuf<unsigned long long> tally_u64, emit_limbs, requant, leaving, scal;
   DevBuf<double> tally_f64;
   // per-XCD replicas 1..kTallyReplicas-1 of the per-visit tallies
   DevBuf<unsigned long long> rep_u64;
@@ -277,7 +277,6 @@ struct Engine final : EngineBase {
     start_u32.ensure(N);
     comb_cells.ensure(N);
     emit_limbs.ensure(2 * static_cast<size_t>(N));
-    W.ensure(2 * static_cast<size_t>(N));
     requant.ensure(2 * static_cast<size_t>(N));
     leaving.ensure(5);  // escapes, random-walk steps, DDMC steps, segments, longest history
     scal.ensure(4);     // total census weight (low, high limb)
@@ -457,6 +456,20 @@ struct Engine final : EngineBase {
     return r;
   }
 
+  // Tally-cache slots of k_transport<Real, F> at this block size: the share
+  // of the CU's LDS of each block its registers let run there.
+  uint32_t pass_slots_cache[4][4] = {};
+  template <uint32_t F>
+  uint32_t pass_cache_slots(int32_t b, uint32_t nf) {
+    uint32_t &r = pass_slots_cache[F][b == 64 ? 0 : (b == 128 ? 1 : (b == 256 ? 2 : 3))];
+    if (!r) {
+      int per_cu = 0;
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<Real, F>, b, 0));
+      r = cache_slots(static_cast<uint32_t>(b), nf, static_cast<uint32_t>(std::max(per_cu, 1)));
+    }
+    return r;
+  }
+
   // One transport launch on the kernel variant compiled for this problem's
   // features (random walk / DDMC, multigroup).
   template <uint32_t F>
@@ -469,9 +482,11 @@ struct Engine final : EngineBase {
           static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
           xs.p, av, kc, seg_per_pass, tal, leaving.p, counters_u32.p + 2);
     } else {
-      k_transport<Real, F><<<blocks_for(n_in, block), block, cache_bytes(block), stream>>>(
+      const uint32_t nf = (precision == STORM_GPU_MIXED && groups >= 2) ? 4u : 2u;
+      const uint32_t slots = pass_cache_slots<F>(block, nf);
+      k_transport<Real, F><<<blocks_for(n_in, block), block, cache_bytes(slots, nf), stream>>>(
           static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
-          xs.p, av, kc, seg_per_pass, tal, leaving.p);
+          xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf);
     }
   }
   void launch_transport(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshView<Real> &mv,
@@ -757,10 +772,12 @@ struct Engine final : EngineBase {
     vals.ensure(n);
     vals_out.ensure(n);
     zero_u32(cnt_u32.p, N);
-    zero_u64(W.p, W.n);
+    // per-cell census weight: the ledger's census_pre tally
+    unsigned long long *Wc = tal.cpre;
+    zero_u64(Wc, 2 * static_cast<size_t>(N));
     const uint32_t salt = static_cast<uint32_t>(seed) * 0x2545F491u + step_index * 0x9E3779B9u;
     k_comb_keys<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, cpre.soa(), d, N, salt, eq, keys.p, vals.p,
-                                                              cnt_u32.p, W.p);
+                                                              cnt_u32.p, Wc);
     int end_bit = 32;
     while ((1ull << (end_bit - 32)) <= N) ++end_bit;
     {
@@ -779,14 +796,14 @@ struct Engine final : EngineBase {
     }
     // total census weight (exact integer) on the device
     zero_u64(scal.p, 2);
-    k_total_limbs<<<std::min<uint32_t>(blocks_for(N, 256), 1024u), 256, 0, stream>>>(N, W.p, scal.p);
+    k_total_limbs<<<std::min<uint32_t>(blocks_for(N, 256), 1024u), 256, 0, stream>>>(N, Wc, scal.p);
     uint32_t last[2] = {0, 0};
     HIPCHECK(hipMemcpyAsync(&last[0], start_u32.p + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipMemcpyAsync(&last[1], cnt_u32.p + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     const uint32_t n_valid = last[0] + last[1];
     if (n_valid == 0) return 0;
-    k_comb_cells<<<blocks_for(N, 256), 256, 0, stream>>>(N, cnt_u32.p, W.p, scal.p, imc.comb_nmin, imc.comb_factor,
+    k_comb_cells<<<blocks_for(N, 256), 256, 0, stream>>>(N, cnt_u32.p, Wc, scal.p, imc.comb_nmin, imc.comb_factor,
                                                          eq, imc.comb ? 1 : 0, seed, step_index, comb_cells.p,
                                                          tal.rem);
     limbs_sorted.ensure(n_valid);

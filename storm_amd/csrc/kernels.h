@@ -120,7 +120,7 @@ struct Tallies {
   unsigned long long *pint;  // [2N] path integral of the weight, erg*cm (cells without absorption)
   unsigned long long *esc;   // [2N] escaped through REMOVE faces
   unsigned long long *cin;   // [2N] census weight entering transport
-  unsigned long long *cpre;  // [2N] census weight before population control
+  unsigned long long *cpre;  // [2N] census weight before population control (k_comb_keys)
   unsigned long long *cout;  // [2N] census weight after population control
   unsigned long long *rem;   // [2N] comb remainders (go to the material)
   unsigned long long *rdep;  // [2N] rouletted weight (goes to the material)
@@ -502,41 +502,51 @@ __device__ __forceinline__ uint32_t spread(uint32_t q, uint32_t nw) { return (q 
 // small open-addressed table in LDS (cell -> sums) and written to device
 // memory once per cell when the block's last wavefront ends.  A visit whose
 // cell finds no slot within kCacheProbes probes goes straight to device
-// memory.  The census weight of packets ending the step in census (cpre)
-// goes through the same table.  Integer sums are exact in any order; the
-// FP64 engine's sums (doubles in v[0], v[1]) change only their order.
-// The table lives in dynamic LDS: 2 slots per thread of the block (block
-// sizes are powers of two), 56 B per slot.
-constexpr uint32_t kCacheSlotsPerThread = 2;
+// memory.  Integer sums are exact in any order; the FP64 engine's sums
+// (doubles in v[0], v[1]) change only their order.
+// The table lives in dynamic LDS: `slots` (a power of two) slots of `nf`
+// 64-bit sums, a tag and a segment count each.  nf = 2 in the mixed gray
+// engine (the two deposit limbs; the path integral, tallied only in cells
+// without absorption, goes to device memory), 4 in the mixed multigroup
+// engine (deposit and path-integral limbs), 2 in the FP64 engine (deposit and
+// path integral as doubles).  The host sizes it to what a block's share of
+// the CU's 160 KB of LDS holds (cache_slots).
 constexpr uint32_t kCacheProbes = 8;
-constexpr uint32_t kCacheSlotBytes = 6 * 8 + 2 * 4;
-inline __host__ __device__ uint32_t cache_bytes(uint32_t block) {
-  return kCacheSlotsPerThread * block * kCacheSlotBytes + 16u;
+inline __host__ __device__ uint32_t cache_bytes(uint32_t slots, uint32_t nf) {
+  return slots * (8u * nf + 8u) + 16u;
+}
+// Largest power-of-two slot count within 4 per thread and the LDS share of
+// one block when `blocks_per_cu` blocks are resident.
+inline uint32_t cache_slots(uint32_t block, uint32_t nf, uint32_t blocks_per_cu) {
+  const uint32_t budget = (160u * 1024u) / (blocks_per_cu ? blocks_per_cu : 1u) - 64u;
+  uint32_t s = 1;
+  while (2 * s <= 4 * block && cache_bytes(2 * s, nf) <= budget) s *= 2;
+  return s;
 }
 struct TallyCache {
-  uint32_t bits, mask;
-  unsigned long long *v;  // v[f * slots + s]: dep lo, dep hi, pint lo, pint hi (FP64: dep, pint), cpre lo, cpre hi
+  uint32_t bits, mask, nf;
+  unsigned long long *v;  // v[f * slots + s], f < nf
   uint32_t *tag;          // cell + 1; 0: free
   uint32_t *segs;
   uint32_t *waves_done;
   __device__ __forceinline__ unsigned long long &at(uint32_t f, uint32_t s) const { return v[f * (mask + 1u) + s]; }
 };
 
-// Lays the table out in `lds` (cache_bytes(blockDim.x) bytes) and clears it;
+// Lays the table out in `lds` (cache_bytes(slots, nf) bytes) and clears it;
 // the caller synchronises the block before first use.
-__device__ __forceinline__ TallyCache cache_init(unsigned char *lds) {
+__device__ __forceinline__ TallyCache cache_init(unsigned char *lds, uint32_t slots, uint32_t nf) {
   TallyCache L;
-  const uint32_t slots = kCacheSlotsPerThread * blockDim.x;
   L.bits = 31u - static_cast<uint32_t>(__clz(static_cast<int>(slots)));
   L.mask = slots - 1u;
+  L.nf = nf;
   L.v = reinterpret_cast<unsigned long long *>(lds);
-  L.tag = reinterpret_cast<uint32_t *>(lds + 6 * 8 * slots);
+  L.tag = reinterpret_cast<uint32_t *>(lds + 8 * nf * slots);
   L.segs = L.tag + slots;
   L.waves_done = L.segs + slots;
   for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
     L.tag[s] = 0u;
     L.segs[s] = 0u;
-    for (uint32_t f = 0; f < 6; ++f) L.at(f, s) = 0ull;
+    for (uint32_t f = 0; f < nf; ++f) L.at(f, s) = 0ull;
   }
   if (threadIdx.x == 0) *L.waves_done = 0u;
   return L;
@@ -587,8 +597,12 @@ struct DeviceSink {
     if (s >= 0) {
       if (d.lo) lds_add(&cache.at(0, s), static_cast<unsigned long long>(d.lo));
       if (d.hi) lds_add(&cache.at(1, s), static_cast<unsigned long long>(d.hi));
-      if (e.lo) lds_add(&cache.at(2, s), static_cast<unsigned long long>(e.lo));
-      if (e.hi) lds_add(&cache.at(3, s), static_cast<unsigned long long>(e.hi));
+      if (cache.nf >= 4) {
+        if (e.lo) lds_add(&cache.at(2, s), static_cast<unsigned long long>(e.lo));
+        if (e.hi) lds_add(&cache.at(3, s), static_cast<unsigned long long>(e.hi));
+      } else {
+        tally_limbs(pint, c, e);
+      }
       lds_add(&cache.segs[s], n);
       return;
     }
@@ -608,16 +622,6 @@ struct DeviceSink {
     if (d != 0.0) atomicAdd(&dep_fp[c], d);
     if (e != 0.0) atomicAdd(&pint_fp[c], e);
     atomicAdd(&segs[c], n);
-  }
-  // census weight of a packet ending the step in census, cell c
-  __device__ void census_weight(uint32_t c, fx::Limbs w) {
-    const int s = cache.v ? cache_slot(cache.tag, cache.bits, cache.mask, c) : -1;
-    if (s >= 0) {
-      if (w.lo) lds_add(&cache.at(4, s), static_cast<unsigned long long>(w.lo));
-      if (w.hi) lds_add(&cache.at(5, s), static_cast<unsigned long long>(w.hi));
-      return;
-    }
-    tally_limbs(t.cpre, c, w);
   }
   // Called by every wavefront of the block once its packets are done: the
   // last one writes the cache to device memory.
@@ -639,10 +643,9 @@ struct DeviceSink {
         if (e != 0.0) atomicAdd(&pint_fp[c], e);
       } else {
         tally_limbs(dep, c, fx::Limbs{cache.at(0, s), cache.at(1, s)});
-        tally_limbs(pint, c, fx::Limbs{cache.at(2, s), cache.at(3, s)});
+        if (cache.nf >= 4) tally_limbs(pint, c, fx::Limbs{cache.at(2, s), cache.at(3, s)});
       }
       if (cache.segs[s]) atomicAdd(&segs[c], cache.segs[s]);
-      tally_limbs(t.cpre, c, fx::Limbs{cache.at(4, s), cache.at(5, s)});
     }
   }
   __device__ void escape(uint32_t c, double w) {
@@ -669,9 +672,10 @@ __global__ __launch_bounds__(512) void k_transport(uint32_t n_in, PacketSoA<Real
                                                    uint32_t *__restrict__ n_cen, MeshView<Real> mesh,
                                                    const CellXS<Real> *__restrict__ xs, AccelView<Real> acc,
                                                    StepConsts k, int32_t max_segments, Tallies t,
-                                                   unsigned long long *__restrict__ stats) {
+                                                   unsigned long long *__restrict__ stats, uint32_t cache_slots,
+                                                   uint32_t cache_fields) {
   extern __shared__ __align__(16) unsigned char lds[];
-  const TallyCache cache = cache_init(lds);
+  const TallyCache cache = cache_init(lds, cache_slots, cache_fields);
   __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n_in;
@@ -688,10 +692,7 @@ __global__ __launch_bounds__(512) void k_transport(uint32_t n_in, PacketSoA<Real
   uint32_t slot = wave_append(alive, n_out);
   if (alive) store_packet(out, slot, p);
   uint32_t cslot = wave_append(census, n_cen);
-  if (census) {
-    store_census(cen, cslot, p);
-    sink.census_weight(flat_cell(mesh, p.ijk), fx::limbs_of(p.w, k.eq));
-  }
+  if (census) store_census(cen, cslot, p);
   sink.finish_block();
   wave_add_u32((valid && outcome == kEscaped) ? 1u : 0u, &stats[0]);
   wave_max_u32(valid ? p.ctr : 0u, &stats[4]);
@@ -782,10 +783,7 @@ __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketS
     escapes += (fin && outcome == kEscaped) ? 1u : 0u;
     if (fin && p.ctr > longest) longest = p.ctr;
     const uint32_t cslot = wave_append(census, n_cen);
-    if (census) {
-      store_census(cen, cslot, p);
-      tally_limbs(t.cpre, flat_cell(mesh, p.ijk), fx::limbs_of(p.w, k.eq));
-    }
+    if (census) store_census(cen, cslot, p);
     const uint32_t oslot = wave_append(survivor, n_out);
     if (survivor) store_packet(out, oslot, p);
   }
