@@ -152,7 +152,7 @@ def main():
             "cells": mixed["N"],
             "packets_per_step": mixed["packets"] / args.steps,
             "segments_per_step": mixed["segments"] / args.steps,
-            "arithmetic": "f32 cell-local tracking, 92-bit fixed-point tallies, f64 material update",
+            "arithmetic": "f32 cell-local tracking, 64-bit fixed-point tallies, f64 material update",
             "parallelism": "1 GPU",
         },
         "roofline": {

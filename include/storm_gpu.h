@@ -32,7 +32,7 @@ typedef struct storm_gpu_engine storm_gpu_engine;
 
 /* Arithmetic of the transport. */
 enum {
-  STORM_GPU_MIXED = 0, /* FP32 cell-local tracking, own deterministic math, 92-bit fixed-point tallies */
+  STORM_GPU_MIXED = 0, /* FP32 cell-local tracking, own deterministic math, 64-bit fixed-point tallies */
   STORM_GPU_FP64 = 1   /* plain FP64 tracking, device libm, FP64 atomic tallies                      */
 };
 
@@ -183,8 +183,9 @@ int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segm
 int storm_gpu_debug_keep_transport_inputs(storm_gpu_engine *eng, int32_t enable);
 int storm_gpu_debug_transport_inputs(storm_gpu_engine *eng, uint64_t *n_packets, void *packets,
                                      void *cell_xs, int64_t *consts4);
-/* Integer tallies, two limbs per cell: absorbed, rouletted, path integral
- * (cells without absorption only), escaped; and the segment counts. */
+/* Integer tallies in quanta of the step, two words per cell (value =
+ * hi * 2^36 + lo): absorbed, rouletted, path integral (cells without
+ * absorption only), escaped; and the segment counts. */
 int storm_gpu_debug_tallies(storm_gpu_engine *eng, uint64_t *dep, uint64_t *rdep, uint64_t *pint, uint64_t *esc,
                             uint32_t *segs);
 /* Per-cell random-walk / DDMC data of the last step (bytes written to

@@ -32,11 +32,10 @@ struct HostSink {
   uint32_t *segs;
   int32_t eq;
   uint64_t rw = 0, ddmc = 0;
+  // whole values into the low word, as the engine's transport adds them
   void visit(uint32_t c, fx::Limbs d, fx::Limbs e, uint32_t s) {
-    dep[2 * c] += d.lo;
-    dep[2 * c + 1] += d.hi;
-    pint[2 * c] += e.lo;
-    pint[2 * c + 1] += e.hi;
+    dep[2 * c] += fx::value_of(d);
+    pint[2 * c] += fx::value_of(e);
     segs[c] += s;
   }
   void roulette(uint32_t c, double w) {

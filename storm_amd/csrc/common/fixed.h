@@ -1,13 +1,18 @@
 // Exact fixed-point bookkeeping for packet weights and cell tallies.
 //
 // A step has an energy quantum Q = 2^eq erg chosen so that every energy the
-// step can move is below 2^90 Q.  Packet weights are FP64 values that are
+// step can move is below 2^62 Q.  Packet weights are FP64 values that are
 // kept integer multiples of Q at every point where they are tallied, so the
-// conversion weight -> integer is exact.  An integer below 2^92 is held as
-// two unsigned limbs, value = hi * 2^36 + lo with lo < 2^36; tallies are
-// per-cell pairs of u64 accumulators summed limb by limb with integer
-// atomics (no carries), which is exact and independent of the order of the
-// additions as long as a cell receives fewer than 2^28 additions per step.
+// conversion weight -> integer is exact, and any sum of the step's energy
+// flows (a cell's deposits, the census, the ledger's totals) fits one u64.
+// (Q is 2^-62 of the step's energy: a weight of 1e-7 of it keeps 39 bits,
+// far finer than the FP32 tracking.)  A value is written as two limbs,
+// value = hi * 2^36 + lo with lo < 2^36; tallies are per-cell pairs of u64
+// accumulators, value = hi_sum * 2^36 + lo_sum.  The transport's per-visit
+// tallies add whole values (value_of) into the low word, one atomic each; the
+// other tallies add limb by limb (no carries), exact as long as a cell
+// receives fewer than 2^28 of those additions per step.  Integer sums are
+// exact and independent of the order of the additions.
 #pragma once
 
 #include <stdint.h>
@@ -20,7 +25,7 @@ namespace fx {
 
 constexpr int kLoBits = 36;
 constexpr uint64_t kLoMask = (1ull << kLoBits) - 1ull;
-constexpr int kHeadroomBits = 90;  // energies bounded by 2^90 quanta
+constexpr int kHeadroomBits = 62;  // energies bounded by 2^62 quanta
 
 struct Limbs {
   uint64_t lo, hi;
@@ -35,7 +40,7 @@ STORM_HD double quantize_nearest(double w, int e) {
   return rint(w * fpm::pow2d(-e)) * fpm::pow2d(e);
 }
 
-// Limbs of w / 2^e for a w that is an integer multiple of 2^e below 2^92.
+// Limbs of w / 2^e for a w that is an integer multiple of 2^e below 2^62.
 STORM_HD Limbs limbs_of(double w, int e) {
   double y = w * fpm::pow2d(-e);            // exact integer
   double h = floor(y * fpm::pow2d(-kLoBits));
@@ -45,6 +50,9 @@ STORM_HD Limbs limbs_of(double w, int e) {
   r.lo = static_cast<uint64_t>(l);
   return r;
 }
+
+// The integer hi * 2^36 + lo (below 2^62 for the step's energies).
+STORM_HD uint64_t value_of(Limbs l) { return (l.hi << kLoBits) + l.lo; }
 
 // a - b for a >= b, both normalised (lo < 2^36); result normalised.
 STORM_HD Limbs sub(Limbs a, Limbs b) {

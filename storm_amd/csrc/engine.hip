@@ -9,9 +9,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include <hip/hip_runtime.h>
@@ -99,9 +101,12 @@ struct CensusBufs {
   size_t cap() const { return meta.n; }
 };
 
+// Sum of two (lo, hi) limb pairs with lo < 2^36, normalised the same way (the
+// representation is unique, so the operator is associative).
 struct PairAdd {
   __host__ __device__ ulonglong2 operator()(const ulonglong2 &a, const ulonglong2 &b) const {
-    return make_ulonglong2(a.x + b.x, a.y + b.y);
+    const unsigned long long lo = a.x + b.x;
+    return make_ulonglong2(lo & fx::kLoMask, a.y + b.y + (lo >> fx::kLoBits));
   }
 };
 
@@ -157,9 +162,9 @@ struct Engine final : EngineBase {
       clamp, partials, host_sa, host_ss, host_cv, wface, dsc;
   bool have_host_sa = false, have_host_ss = false, have_host_cv = false;
   DevBuf<CellXS<Real>> xs;
-  DevBuf<uint32_t> nphot, err, cnt_u32, start_u32, ncopy, vals, vals_out, counters_u32;
-  DevBuf<uint64_t> offsets, keys, keys_out, offs64, nphot64;
-  DevBuf<unsigned long long> tally_u64, emit_limbs, requant, leaving, scal;
+  DevBuf<uint32_t> nphot, err, run_start, run_end, ncopy, keys, keys_out, counters_u32;
+  DevBuf<uint64_t> offsets, vals, vals_out, offs64, nphot64;
+  DevBuf<unsigned long long> tally_u64, emit_limbs, requant, leaving;
   DevBuf<double> tally_f64;
   // per-XCD replicas 1..kTallyReplicas-1 of the per-visit tallies
   DevBuf<unsigned long long> rep_u64;
@@ -167,6 +172,14 @@ struct Engine final : EngineBase {
   DevBuf<uint32_t> rep_u32;
   DevBuf<ulonglong2> limbs_sorted, cum;
   DevBuf<CombCell> comb_cells;
+  DevBuf<int32_t> qexp;
+  // scalars read back by the host: gathered on the device, one copy into
+  // pinned memory, one synchronisation
+  DevBuf<unsigned long long> mail;
+  unsigned long long *mail_h = nullptr;
+  // pinned staging of the boundary face weights
+  double *wface_h = nullptr;
+  size_t wface_h_n = 0;
   DevBuf<CellAccel<Real>> accel;
   DevBuf<double> Dtil;
   DevBuf<Real> rw_tau, rw_surv, rw_rad;
@@ -194,6 +207,8 @@ struct Engine final : EngineBase {
     nz = p->mesh.nz;
     if (nx < 1 || ny < 1 || nz < 1 || nx > 4096 || ny > 1024 || nz > 1024)
       throw EngineError(1, "mesh dimensions must be 1..4096 x 1..1024 x 1..1024");
+    // cell indices stay below 2^31 in every tally offset (2 c + 1)
+    if (static_cast<uint64_t>(nx) * ny * nz > (1ull << 30)) throw EngineError(1, "more than 2^30 cells");
     N = static_cast<uint32_t>(nx) * ny * nz;
     xe.assign(p->mesh.x_edges, p->mesh.x_edges + nx + 1);
     ye.assign(p->mesh.y_edges, p->mesh.y_edges + ny + 1);
@@ -273,14 +288,16 @@ struct Engine final : EngineBase {
     nphot64.ensure(N + 1);
     offsets.ensure(N + 1);
     err.ensure(1);
-    cnt_u32.ensure(N);
-    start_u32.ensure(N);
+    run_start.ensure(N);
+    run_end.ensure(N);
     comb_cells.ensure(N);
     emit_limbs.ensure(2 * static_cast<size_t>(N));
     requant.ensure(2 * static_cast<size_t>(N));
     leaving.ensure(5);  // escapes, random-walk steps, DDMC steps, segments, longest history
-    scal.ensure(4);     // total census weight (low, high limb)
-    dsc.ensure(2);      // material energy before the step, census energy after it
+    dsc.ensure(3);      // material energy before the step, census energy after it, emission total
+    qexp.ensure(2);     // energy quantum exponents of the step
+    mail.ensure(8);
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&mail_h), 8 * sizeof(unsigned long long)));
     counters_u32.ensure(4);
     if (groups >= 2) {
       up(bounds_erg, bounds_h.data(), bounds_h.size());
@@ -362,6 +379,8 @@ struct Engine final : EngineBase {
 
   ~Engine() override {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (mail_h) (void)hipHostFree(mail_h);
+    if (wface_h) (void)hipHostFree(wface_h);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -482,7 +501,7 @@ struct Engine final : EngineBase {
           static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
           xs.p, av, kc, seg_per_pass, tal, leaving.p, counters_u32.p + 2);
     } else {
-      const uint32_t nf = (precision == STORM_GPU_MIXED && groups >= 2) ? 4u : 2u;
+      const uint32_t nf = (precision == STORM_GPU_MIXED && groups < 2) ? 1u : 2u;
       const uint32_t slots = pass_cache_slots<F>(block, nf);
       k_transport<Real, F><<<blocks_for(n_in, block), block, cache_bytes(slots, nf), stream>>>(
           static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
@@ -506,12 +525,26 @@ struct Engine final : EngineBase {
     k_sum_partials<256><<<kSumGrid, 256, 0, stream>>>(d, n, partials.p);
     k_sum_partials<256><<<1, 256, 0, stream>>>(partials.p, kSumGrid, out);
   }
-  double deterministic_sum(const double *d, uint64_t n) {
-    sum_async(d, n, partials.p + kSumGrid);
-    double h = 0.0;
-    HIPCHECK(hipMemcpyAsync(&h, partials.p + kSumGrid, sizeof(double), hipMemcpyDeviceToHost, stream));
+
+  // Reads up to 8 device scalars (4 or 8 bytes each) with one copy into
+  // pinned memory and one synchronisation.
+  const unsigned long long *read_scalars(std::initializer_list<std::pair<const void *, uint32_t>> refs) {
+    ScalarRefs r{};
+    uint32_t n = 0;
+    for (const auto &x : refs) {
+      r.p[n] = x.first;
+      r.bytes[n] = x.second;
+      ++n;
+    }
+    k_gather_scalars<<<1, 64, 0, stream>>>(r, n, mail.p);
+    HIPCHECK(hipMemcpyAsync(mail_h, mail.p, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
-    return h;
+    return mail_h;
+  }
+  static double as_double(unsigned long long v) {
+    double d;
+    std::memcpy(&d, &v, sizeof d);
+    return d;
   }
 
   template <typename F>
@@ -554,7 +587,7 @@ struct Engine final : EngineBase {
           imc.ddmc_min_cell_optical_depth, Dtil.p, accel.p);
       k_accel_rates<Real><<<blocks_for(N, 256), 256, 0, stream>>>(N, d, mv0, Dtil.p, accel.p);
     }
-    const double Etot = deterministic_sum(Eemit.p, N);
+    sum_async(Eemit.p, N, dsc.p + 2);  // emission total, read by k_quantum and k_alloc
 
     // boundary sources: SideTemperature / TwoSidesTemperature energy per face
     // (T^4 frozen at the first call if asked), sides x- then x+
@@ -584,10 +617,24 @@ struct Engine final : EngineBase {
       source_total += wface_e[f] * npc;
     }
 
-    // energy quantum of this step: every energy below 2^90 quanta
-    const double Ebound = census_total_prev + Etot + source_total;
-    const int32_t eq = Ebound > 0.0 ? fx::ceil_log2(Ebound) - fx::kHeadroomBits : -1000;
-    const int32_t et = Ebound > 0.0 ? fx::ceil_log2(Ebound * units::clight * dt) - fx::kHeadroomBits : -1000;
+    // energy quantum of this step (every energy below 2^62 quanta), photon
+    // allocation and offsets; one read-back for the quantum and the count
+    k_quantum<<<1, 64, 0, stream>>>(dsc.p + 2, census_total_prev, source_total, units::clight, dt, qexp.p);
+    k_alloc<<<blocks_for(N, 256), 256, 0, stream>>>(N, Eemit.p, dsc.p + 2, imc.new_photons_per_cell, qexp.p,
+                                                    nphot.p, wemit.p, emit_limbs.p);
+    {
+      auto q = [&](void *tp, size_t &bytes) {
+        HIPCHECK(rocprim::exclusive_scan(tp, bytes, nphot.p, offsets.p, uint64_t(0), static_cast<size_t>(N) + 0,
+                                         rocprim::plus<uint64_t>(), stream));
+      };
+      ensure_temp(q);
+    }
+    const unsigned long long *hs =
+        read_scalars({{offsets.p + (N - 1), 8u}, {nphot.p + (N - 1), 4u}, {qexp.p, 4u}, {qexp.p + 1, 4u}});
+    const uint64_t n_emit = hs[0] + hs[1];
+    const int32_t eq = static_cast<int32_t>(static_cast<uint32_t>(hs[2]));
+    const int32_t et = static_cast<int32_t>(static_cast<uint32_t>(hs[3]));
+    k_fill_u64<<<1, 64, 0, stream>>>(reinterpret_cast<unsigned long long *>(offsets.p + N), 1, n_emit);
     counters.energy_quantum_exp = eq;
     for (uint32_t f = 0; f < nfaces; ++f) wface_e[f] = fx::quantize_down(wface_e[f], eq);
     if (groups >= 2 && nfaces) {
@@ -607,28 +654,17 @@ struct Engine final : EngineBase {
       HIPCHECK(hipStreamSynchronize(stream));
     }
     wface.ensure(std::max<uint32_t>(nfaces, 1));
-    if (nfaces)
-      HIPCHECK(hipMemcpyAsync(wface.p, wface_e.data(), nfaces * sizeof(double), hipMemcpyHostToDevice, stream));
-
-    // photon allocation and offsets
-    k_alloc<<<blocks_for(N, 256), 256, 0, stream>>>(N, Eemit.p, Etot, imc.new_photons_per_cell, eq, nphot.p,
-                                                    wemit.p, emit_limbs.p);
-    {
-      auto q = [&](void *tp, size_t &bytes) {
-        HIPCHECK(rocprim::exclusive_scan(tp, bytes, nphot.p, offsets.p, uint64_t(0), static_cast<size_t>(N) + 0,
-                                         rocprim::plus<uint64_t>(), stream));
-      };
-      ensure_temp(q);
-    }
-    uint64_t n_emit = 0;
-    {
-      uint64_t last_off = 0;
-      uint32_t last_n = 0;
-      HIPCHECK(hipMemcpyAsync(&last_off, offsets.p + (N - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
-      HIPCHECK(hipMemcpyAsync(&last_n, nphot.p + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-      HIPCHECK(hipStreamSynchronize(stream));
-      n_emit = last_off + last_n;
-      HIPCHECK(hipMemcpyAsync(offsets.p + N, &n_emit, sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    if (nfaces) {
+      // the stream is idle here (read_scalars above), so the staging buffer
+      // of the last step is free
+      if (wface_h_n < nfaces) {
+        if (wface_h) HIPCHECK(hipHostFree(wface_h));
+        wface_h = nullptr;
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&wface_h), nfaces * sizeof(double)));
+        wface_h_n = nfaces;
+      }
+      std::memcpy(wface_h, wface_e.data(), nfaces * sizeof(double));
+      HIPCHECK(hipMemcpyAsync(wface.p, wface_h, nfaces * sizeof(double), hipMemcpyHostToDevice, stream));
     }
     const uint64_t n_bc = static_cast<uint64_t>(nfaces) * npc;
     const uint64_t n_total = n_census + n_emit + n_bc;
@@ -704,9 +740,10 @@ struct Engine final : EngineBase {
       HIPCHECK(hipEventRecord(ev0, stream));
       launch_transport(n_in, src, dst, mv, av, kc);
       HIPCHECK(hipEventRecord(ev1, stream));
-      uint32_t h[2];
-      HIPCHECK(hipMemcpyAsync(h, counters_u32.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipMemcpyAsync(mail_h, counters_u32.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
       HIPCHECK(hipStreamSynchronize(stream));
+      uint32_t h[2];
+      std::memcpy(h, mail_h, sizeof h);
       float ms = 0.f;
       HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
       t_transport += ms * 1e-3;
@@ -736,13 +773,12 @@ struct Engine final : EngineBase {
     // and counters: one synchronisation
     sum_async(census_erad.p, N, dsc.p + 1);
     k_total_u32<<<std::min<uint32_t>(blocks_for(N, 256), 1024u), 256, 0, stream>>>(N, tal.segs, leaving.p + 3);
-    uint32_t herr = 0;
-    unsigned long long hstats[5] = {0, 0, 0, 0, 0};
-    double hcensus = 0.0;
-    HIPCHECK(hipMemcpyAsync(&hcensus, dsc.p + 1, sizeof(double), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipMemcpyAsync(&herr, err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipMemcpyAsync(hstats, leaving.p, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipStreamSynchronize(stream));
+    const unsigned long long *hp = read_scalars({{dsc.p + 1, 8u}, {err.p, 4u}, {leaving.p, 8u}, {leaving.p + 1, 8u},
+                                                 {leaving.p + 2, 8u}, {leaving.p + 3, 8u}, {leaving.p + 4, 8u}});
+    const double hcensus = as_double(hp[0]);
+    const uint32_t herr = static_cast<uint32_t>(hp[1]);
+    unsigned long long hstats[5];
+    for (int k = 0; k < 5; ++k) hstats[k] = hp[2 + k];
     census_total_prev = hcensus;
     counters.segments = hstats[3];
     counters.max_packet_blocks = hstats[4];
@@ -771,75 +807,57 @@ struct Engine final : EngineBase {
     keys_out.ensure(n);
     vals.ensure(n);
     vals_out.ensure(n);
-    zero_u32(cnt_u32.p, N);
-    // per-cell census weight: the ledger's census_pre tally
-    unsigned long long *Wc = tal.cpre;
-    zero_u64(Wc, 2 * static_cast<size_t>(N));
+    limbs_sorted.ensure(n);
+    cum.ensure(n);
+    ncopy.ensure(n);
+    offs64.ensure(n);
     const uint32_t salt = static_cast<uint32_t>(seed) * 0x2545F491u + step_index * 0x9E3779B9u;
-    k_comb_keys<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, cpre.soa(), d, N, salt, eq, keys.p, vals.p,
-                                                              cnt_u32.p, Wc);
-    int end_bit = 32;
-    while ((1ull << (end_bit - 32)) <= N) ++end_bit;
+    // 32-bit keys: cell (cbits bits, N itself included for dropped
+    // packets) above the top pbits bits of the rank; ties by k_comb_ties
+    uint32_t cbits = 1;
+    while (cbits < 32u && (static_cast<uint64_t>(N) >> cbits) != 0u) ++cbits;
+    const uint32_t pbits = 32u - cbits;
+    k_comb_keys<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, cpre.soa(), d, N, salt, pbits, keys.p, vals.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
         HIPCHECK(rocprim::radix_sort_pairs(tp, bytes, keys.p, keys_out.p, vals.p, vals_out.p, static_cast<size_t>(n),
-                                           0, end_bit, stream));
+                                           0, 32, stream));
       };
       ensure_temp(q);
     }
+    k_comb_ties<<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits ? N << pbits : N);
+    // runs of cells over the sorted census, weight limbs and their prefix sums
+    zero_u32(run_end.p, N);
+    k_comb_sorted<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits, cpre.soa(), eq, N,
+                                                                limbs_sorted.p, run_start.p, run_end.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
-        HIPCHECK(rocprim::exclusive_scan(tp, bytes, cnt_u32.p, start_u32.p, uint32_t(0), static_cast<size_t>(N),
-                                         rocprim::plus<uint32_t>(), stream));
-      };
-      ensure_temp(q);
-    }
-    // total census weight (exact integer) on the device
-    zero_u64(scal.p, 2);
-    k_total_limbs<<<std::min<uint32_t>(blocks_for(N, 256), 1024u), 256, 0, stream>>>(N, Wc, scal.p);
-    uint32_t last[2] = {0, 0};
-    HIPCHECK(hipMemcpyAsync(&last[0], start_u32.p + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipMemcpyAsync(&last[1], cnt_u32.p + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipStreamSynchronize(stream));
-    const uint32_t n_valid = last[0] + last[1];
-    if (n_valid == 0) return 0;
-    k_comb_cells<<<blocks_for(N, 256), 256, 0, stream>>>(N, cnt_u32.p, Wc, scal.p, imc.comb_nmin, imc.comb_factor,
-                                                         eq, imc.comb ? 1 : 0, seed, step_index, comb_cells.p,
-                                                         tal.rem);
-    limbs_sorted.ensure(n_valid);
-    cum.ensure(n_valid);
-    ncopy.ensure(n_valid);
-    offs64.ensure(n_valid);
-    k_comb_limbs<Real><<<blocks_for(n_valid, 256), 256, 0, stream>>>(n_valid, vals_out.p, cpre.soa(), eq,
-                                                                     limbs_sorted.p);
-    {
-      auto q = [&](void *tp, size_t &bytes) {
-        HIPCHECK(rocprim::inclusive_scan(tp, bytes, limbs_sorted.p, cum.p, static_cast<size_t>(n_valid), PairAdd(),
+        HIPCHECK(rocprim::inclusive_scan(tp, bytes, limbs_sorted.p, cum.p, static_cast<size_t>(n), PairAdd(),
                                          stream));
       };
       ensure_temp(q);
     }
-    k_comb_count<Real><<<blocks_for(n_valid, 256), 256, 0, stream>>>(n_valid, keys_out.p, vals_out.p, cpre.soa(),
-                                                                     comb_cells.p, start_u32.p, cnt_u32.p, cum.p, eq,
-                                                                     ncopy.p);
+    // per-cell targets; W_c into the ledger's census_pre tally
+    k_comb_cells<<<blocks_for(N, 256), 256, 0, stream>>>(N, run_start.p, run_end.p, cum.p, n, imc.comb_nmin,
+                                                         imc.comb_factor, eq, imc.comb ? 1 : 0, seed, step_index,
+                                                         comb_cells.p, tal.cpre, tal.rem);
+    k_comb_count<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits, cpre.soa(), comb_cells.p,
+                                                               run_start.p, run_end.p, cum.p, eq, N, ncopy.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
-        HIPCHECK(rocprim::exclusive_scan(tp, bytes, ncopy.p, offs64.p, uint64_t(0), static_cast<size_t>(n_valid),
+        HIPCHECK(rocprim::exclusive_scan(tp, bytes, ncopy.p, offs64.p, uint64_t(0), static_cast<size_t>(n),
                                          rocprim::plus<uint64_t>(), stream));
       };
       ensure_temp(q);
     }
-    uint64_t last_off = 0;
-    uint32_t last_nc = 0;
-    HIPCHECK(hipMemcpyAsync(&last_off, offs64.p + (n_valid - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipMemcpyAsync(&last_nc, ncopy.p + (n_valid - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipStreamSynchronize(stream));
-    const uint64_t n_out = last_off + last_nc;
+    const unsigned long long *hc = read_scalars({{offs64.p + (n - 1), 8u}, {ncopy.p + (n - 1), 4u}});
+    const uint64_t n_out = hc[0] + hc[1];
     if (n_out >= (1ull << 32) - 1) throw EngineError(4, "census larger than 2^32 packets");
+    if (n_out == 0) return 0;
     cnext.ensure(n_out);
-    k_comb_write<Real><<<blocks_for(n_valid, 256), 256, 0, stream>>>(n_valid, keys_out.p, vals_out.p, cpre.soa(),
-                                                                     comb_cells.p, ncopy.p, offs64.p, cnext.soa(), eq,
-                                                                     tal);
+    k_comb_write<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits, cpre.soa(),
+                                                               comb_cells.p, ncopy.p, offs64.p, cnext.soa(), eq, N,
+                                                               tal);
     return n_out;
   }
 

@@ -338,13 +338,28 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
   }
 }
 
+// Energy quantum exponents of a step: every energy the step can move (the
+// census energy of the last step, this step's emission total and the boundary
+// source total) is below 2^62 quanta; q[0] for energies, q[1] for path
+// integrals (energy x c dt).  The host reads them back.
+__global__ void k_quantum(const double *__restrict__ Etot, double census_prev, double source_total, double c,
+                          double dt, int32_t *__restrict__ q) {
+  if (threadIdx.x != 0u || blockIdx.x != 0u) return;
+  const double Ebound = census_prev + *Etot + source_total;
+  q[0] = Ebound > 0.0 ? fx::ceil_log2(Ebound) - fx::kHeadroomBits : -1000;
+  q[1] = Ebound > 0.0 ? fx::ceil_log2(Ebound * c * dt) - fx::kHeadroomBits : -1000;
+}
+
 // photons per cell and packet weight, quantised down to the step quantum
-__global__ __launch_bounds__(256) void k_alloc(uint32_t N, const double *__restrict__ Eemit, double Etot,
-                                               uint64_t n_min, int32_t eq, uint32_t *__restrict__ nphot,
+__global__ __launch_bounds__(256) void k_alloc(uint32_t N, const double *__restrict__ Eemit,
+                                               const double *__restrict__ Etot_p, uint64_t n_min,
+                                               const int32_t *__restrict__ q, uint32_t *__restrict__ nphot,
                                                double *__restrict__ wemit,
                                                unsigned long long *__restrict__ emit_limbs) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
+  const double Etot = *Etot_p;
+  const int32_t eq = q[0];
   const double E = Eemit[c];
   const uint64_t total = static_cast<uint64_t>(N) * n_min * 10ull;
   uint64_t share = Etot > 0.0 ? static_cast<uint64_t>(E / Etot * static_cast<double>(total)) : n_min;
@@ -505,22 +520,22 @@ __device__ __forceinline__ uint32_t spread(uint32_t q, uint32_t nw) { return (q 
 // memory.  Integer sums are exact in any order; the FP64 engine's sums
 // (doubles in v[0], v[1]) change only their order.
 // The table lives in dynamic LDS: `slots` (a power of two) slots of `nf`
-// 64-bit sums, a tag and a segment count each.  nf = 2 in the mixed gray
-// engine (the two deposit limbs; the path integral, tallied only in cells
-// without absorption, goes to device memory), 4 in the mixed multigroup
-// engine (deposit and path-integral limbs), 2 in the FP64 engine (deposit and
+// 64-bit sums, a tag and a segment count each.  nf = 1 in the mixed gray
+// engine (the deposit, in quanta; the path integral, tallied only in cells
+// without absorption, goes to device memory), 2 in the mixed multigroup
+// engine (deposit and path integral), 2 in the FP64 engine (deposit and
 // path integral as doubles).  The host sizes it to what a block's share of
 // the CU's 160 KB of LDS holds (cache_slots).
 constexpr uint32_t kCacheProbes = 8;
 inline __host__ __device__ uint32_t cache_bytes(uint32_t slots, uint32_t nf) {
   return slots * (8u * nf + 8u) + 16u;
 }
-// Largest power-of-two slot count within 4 per thread and the LDS share of
+// Largest power-of-two slot count within 8 per thread and the LDS share of
 // one block when `blocks_per_cu` blocks are resident.
 inline uint32_t cache_slots(uint32_t block, uint32_t nf, uint32_t blocks_per_cu) {
   const uint32_t budget = (160u * 1024u) / (blocks_per_cu ? blocks_per_cu : 1u) - 64u;
   uint32_t s = 1;
-  while (2 * s <= 4 * block && cache_bytes(2 * s, nf) <= budget) s *= 2;
+  while (2 * s <= 8 * block && cache_bytes(2 * s, nf) <= budget) s *= 2;
   return s;
 }
 struct TallyCache {
@@ -592,22 +607,23 @@ struct DeviceSink {
     dep_fp = (r && tt.dep_fp_rep) ? tt.dep_fp_rep + (r - 1) * n : tt.dep_fp;
     pint_fp = (r && tt.pint_fp_rep) ? tt.pint_fp_rep + (r - 1) * n : tt.pint_fp;
   }
+  // Whole values (below 2^62 quanta, as is every sum of them) into the low
+  // word of the cell's pair: one atomic per tally.
   __device__ void visit(uint32_t c, fx::Limbs d, fx::Limbs e, uint32_t n) {
+    const unsigned long long dv = fx::value_of(d), ev = fx::value_of(e);
     const int s = cache.v ? cache_slot(cache.tag, cache.bits, cache.mask, c) : -1;
     if (s >= 0) {
-      if (d.lo) lds_add(&cache.at(0, s), static_cast<unsigned long long>(d.lo));
-      if (d.hi) lds_add(&cache.at(1, s), static_cast<unsigned long long>(d.hi));
-      if (cache.nf >= 4) {
-        if (e.lo) lds_add(&cache.at(2, s), static_cast<unsigned long long>(e.lo));
-        if (e.hi) lds_add(&cache.at(3, s), static_cast<unsigned long long>(e.hi));
-      } else {
-        tally_limbs(pint, c, e);
+      if (dv) lds_add(&cache.at(0, s), dv);
+      if (cache.nf >= 2) {
+        if (ev) lds_add(&cache.at(1, s), ev);
+      } else if (ev) {
+        atomicAdd(&pint[2 * c], ev);
       }
       lds_add(&cache.segs[s], n);
       return;
     }
-    tally_limbs(dep, c, d);
-    tally_limbs(pint, c, e);
+    if (dv) atomicAdd(&dep[2 * c], dv);
+    if (ev) atomicAdd(&pint[2 * c], ev);
     atomicAdd(&segs[c], n);
   }
   __device__ void roulette(uint32_t c, double w) { tally_limbs(t.rdep, c, fx::limbs_of(w, eq)); }
@@ -642,8 +658,8 @@ struct DeviceSink {
         if (d != 0.0) atomicAdd(&dep_fp[c], d);
         if (e != 0.0) atomicAdd(&pint_fp[c], e);
       } else {
-        tally_limbs(dep, c, fx::Limbs{cache.at(0, s), cache.at(1, s)});
-        if (cache.nf >= 4) tally_limbs(pint, c, fx::Limbs{cache.at(2, s), cache.at(3, s)});
+        if (cache.at(0, s)) atomicAdd(&dep[2 * c], cache.at(0, s));
+        if (cache.nf >= 2 && cache.at(1, s)) atomicAdd(&pint[2 * c], cache.at(1, s));
       }
       if (cache.segs[s]) atomicAdd(&segs[c], cache.segs[s]);
     }
@@ -885,24 +901,90 @@ __device__ __forceinline__ uint32_t perm32(uint32_t x, uint32_t salt) {
   return x;
 }
 
+// Sort keys of the census: the cell in the high bits and the top `pbits`
+// bits of the packet's rank within its cell below it (32-bit keys: four radix
+// passes).  The value carries the full 32-bit rank above the packet's index,
+// for k_comb_ties.  Weight-0 packets get cell N and sort last.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_comb_keys(uint32_t n, CensusSoA<Real> cen, MeshDims d, uint32_t N,
-                                                   uint32_t salt, int32_t eq, uint64_t *__restrict__ keys,
-                                                   uint32_t *__restrict__ vals, uint32_t *__restrict__ cnt,
-                                                   unsigned long long *__restrict__ W) {
+                                                   uint32_t salt, uint32_t pbits, uint32_t *__restrict__ keys,
+                                                   uint64_t *__restrict__ vals) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint2 m = cen.meta[i];
-  double w = cen.w[i];
-  uint32_t c = (ijk_i(m.x) * d.ny + ijk_j(m.x)) * d.nz + ijk_k(m.x);
-  vals[i] = i;
-  if (w == 0.0) {  // weight-0 packets are dropped (CombPopulationControl.hpp:136-137)
-    keys[i] = static_cast<uint64_t>(N) << 32;
+  const uint32_t rank = perm32(m.y, salt);
+  vals[i] = (static_cast<uint64_t>(rank) << 32) | i;
+  // weight-0 packets are dropped (CombPopulationControl.hpp:136-137)
+  const uint32_t c = cen.w[i] == 0.0 ? N : (ijk_i(m.x) * d.ny + ijk_j(m.x)) * d.nz + ijk_k(m.x);
+  keys[i] = pbits ? (c << pbits) | (rank >> (32u - pbits)) : c;
+}
+
+// Orders each run of equal sort keys by the full rank (the value's high
+// word; ranks are distinct), so that the order inside a cell is the order of
+// the full 32-bit ranks whatever order the census was written in.  Runs are
+// short (a few packets share the top rank bits of a cell); long ones, from a
+// cell holding far more packets than 2^pbits, are heap-sorted.
+__global__ __launch_bounds__(256) void k_comb_ties(uint32_t n, const uint32_t *__restrict__ keys,
+                                                   uint64_t *__restrict__ vals, uint32_t dropped_key) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1u >= n) return;
+  const uint32_t k = keys[i];
+  if (k >= dropped_key || keys[i + 1u] != k || (i > 0u && keys[i - 1u] == k)) return;
+  uint32_t j = i + 2u;
+  while (j < n && keys[j] == k) ++j;
+  uint64_t *v = vals + i;
+  const uint32_t L = j - i;
+  if (L <= 32u) {
+    for (uint32_t a = 1; a < L; ++a) {
+      const uint64_t x = v[a];
+      uint32_t b = a;
+      while (b > 0u && v[b - 1u] > x) {
+        v[b] = v[b - 1u];
+        --b;
+      }
+      v[b] = x;
+    }
     return;
   }
-  keys[i] = (static_cast<uint64_t>(c) << 32) | perm32(m.y, salt);
-  atomicAdd(&cnt[c], 1u);
-  tally_limbs(W, c, fx::limbs_of(w, eq));
+  // heap sort (ascending)
+  auto sift = [&](uint32_t root, uint32_t len) {
+    const uint64_t x = v[root];
+    for (;;) {
+      uint32_t ch = 2u * root + 1u;
+      if (ch >= len) break;
+      if (ch + 1u < len && v[ch + 1u] > v[ch]) ++ch;
+      if (v[ch] <= x) break;
+      v[root] = v[ch];
+      root = ch;
+    }
+    v[root] = x;
+  };
+  for (uint32_t r = L / 2u; r-- > 0u;) sift(r, L);
+  for (uint32_t e = L - 1u; e > 0u; --e) {
+    const uint64_t t = v[0];
+    v[0] = v[e];
+    v[e] = t;
+    sift(0u, e);
+  }
+}
+
+// Over the sorted census: the (lo, hi) limbs of each weight, for the prefix
+// sums, and each cell's run [start, end) of sorted indices (end stays 0 for a
+// cell with no packet).
+template <typename Real>
+__global__ __launch_bounds__(256) void k_comb_sorted(uint32_t n, const uint32_t *__restrict__ keys,
+                                                     const uint64_t *__restrict__ vals, uint32_t pbits,
+                                                     CensusSoA<Real> cen, int32_t eq, uint32_t N,
+                                                     ulonglong2 *__restrict__ limbs, uint32_t *__restrict__ start,
+                                                     uint32_t *__restrict__ end) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = keys[i] >> pbits;
+  fx::Limbs l = fx::limbs_of(cen.w[static_cast<uint32_t>(vals[i])], eq);
+  limbs[i] = make_ulonglong2(l.lo, l.hi);
+  if (c >= N) return;
+  if (i == 0u || (keys[i - 1u] >> pbits) != c) start[c] = i;
+  if (i + 1u == n || (keys[i + 1u] >> pbits) != c) end[c] = i + 1u;
 }
 
 struct CombCell {
@@ -913,8 +995,10 @@ struct CombCell {
   uint32_t mode;    // 0 empty, 1 split/pass-through, 2 comb
 };
 
-__device__ __forceinline__ unsigned __int128 limbs_value(const unsigned long long *W, uint32_t c) {
-  return (static_cast<unsigned __int128>(W[2 * c + 1]) << fx::kLoBits) + static_cast<unsigned __int128>(W[2 * c]);
+// Exact value of an inclusive prefix sum of limb pairs.
+__device__ __forceinline__ unsigned __int128 cum_value(const ulonglong2 *cum, uint32_t i) {
+  const ulonglong2 a = cum[i];
+  return (static_cast<unsigned __int128>(a.y) << fx::kLoBits) + static_cast<unsigned __int128>(a.x);
 }
 
 // Largest integer <= v with at most 53 significant bits (exactly representable
@@ -942,12 +1026,21 @@ __device__ __forceinline__ fx::Limbs remainder_limbs(unsigned __int128 W, unsign
   return rl;
 }
 
-// Per-cell comb targets (CombPopulationControl.hpp:157-206).
-__global__ __launch_bounds__(256) void k_comb_cells(uint32_t N, const uint32_t *__restrict__ cnt,
-                                                    const unsigned long long *__restrict__ W,
-                                                    const unsigned long long *__restrict__ Wsum,
-                                                    uint64_t nmin, double factor, int32_t eq, int32_t comb_on,
-                                                    uint64_t seed, uint32_t step, CombCell *__restrict__ cells,
+__device__ __forceinline__ double u128_to_double(unsigned __int128 v) {
+  return static_cast<double>(static_cast<uint64_t>(v >> 64)) * 18446744073709551616.0 +
+         static_cast<double>(static_cast<uint64_t>(v));
+}
+
+// Per-cell comb targets (CombPopulationControl.hpp:157-206).  The cell's
+// census weight W_c (CombPopulationControl.hpp:139-141) is the difference of
+// the prefix sums over its run of sorted packets; it is also stored, as limbs,
+// for the energy ledger.
+__global__ __launch_bounds__(256) void k_comb_cells(uint32_t N, const uint32_t *__restrict__ start,
+                                                    const uint32_t *__restrict__ end,
+                                                    const ulonglong2 *__restrict__ cum, uint32_t n, uint64_t nmin,
+                                                    double factor, int32_t eq, int32_t comb_on, uint64_t seed,
+                                                    uint32_t step, CombCell *__restrict__ cells,
+                                                    unsigned long long *__restrict__ W,
                                                     unsigned long long *__restrict__ rem) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
@@ -956,15 +1049,17 @@ __global__ __launch_bounds__(256) void k_comb_cells(uint32_t N, const uint32_t *
   cc.r = 0.5;
   cc.target = 0;
   cc.mode = 0;
-  const uint32_t count = cnt[c];
-  // total census weight (exact integer, k_total_limbs), as a double in erg
-  const unsigned __int128 Wall = (static_cast<unsigned __int128>(Wsum[1]) << fx::kLoBits) + Wsum[0];
-  const double Wtot = (static_cast<double>(static_cast<uint64_t>(Wall >> 64)) * 18446744073709551616.0 +
-                       static_cast<double>(static_cast<uint64_t>(Wall))) *
-                      fpm::pow2d(eq);
-  const unsigned __int128 Wc = limbs_value(W, c);
-  const double Wd = static_cast<double>(static_cast<uint64_t>(Wc >> 64)) * 18446744073709551616.0 +
-                    static_cast<double>(static_cast<uint64_t>(Wc)) ;
+  const uint32_t e = end[c];
+  const uint32_t s0 = e ? start[c] : 0u;
+  const uint32_t count = e - s0;
+  // total census weight (exact integer), as a double in erg
+  const double Wtot = u128_to_double(cum_value(cum, n - 1u)) * fpm::pow2d(eq);
+  const unsigned __int128 Wc =
+      count ? cum_value(cum, e - 1u) - (s0 ? cum_value(cum, s0 - 1u) : static_cast<unsigned __int128>(0))
+            : static_cast<unsigned __int128>(0);
+  W[2 * c] = static_cast<unsigned long long>(static_cast<uint64_t>(Wc) & fx::kLoMask);
+  W[2 * c + 1] = static_cast<unsigned long long>(static_cast<uint64_t>(Wc >> fx::kLoBits));
+  const double Wd = u128_to_double(Wc);
   cc.Wd = Wd * fpm::pow2d(eq);
   if (count == 0u || !comb_on) {
     cc.mode = count ? 1u : 0u;
@@ -995,11 +1090,6 @@ __global__ __launch_bounds__(256) void k_comb_cells(uint32_t N, const uint32_t *
   cells[c] = cc;
 }
 
-__device__ __forceinline__ double u128_to_double(unsigned __int128 v) {
-  return static_cast<double>(static_cast<uint64_t>(v >> 64)) * 18446744073709551616.0 +
-         static_cast<double>(static_cast<uint64_t>(v));
-}
-
 __device__ __forceinline__ uint64_t comb_teeth(double C, double Wc, double target, double r) {
   // #{k >= 0 : (k + r) * W/N < C} = ceil(C*N/W - r), clamped to [0, N]
   double x = ceil(C * target / Wc - r);
@@ -1008,96 +1098,100 @@ __device__ __forceinline__ uint64_t comb_teeth(double C, double Wc, double targe
   return static_cast<uint64_t>(x);
 }
 
-// Output copies per sorted census packet.  `cum` holds inclusive prefix sums
-// of the weight limbs over the sorted order; `start` the first sorted index
-// of each cell.
+// Output copies per sorted census packet (0 for the dropped weight-0 packets
+// at the end).  `cum` holds inclusive prefix sums of the weight limbs over the
+// sorted order; [start, end) is each cell's run.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_comb_count(uint32_t n_valid, const uint64_t *__restrict__ keys,
-                                                    const uint32_t *__restrict__ vals, CensusSoA<Real> cen,
+__global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *__restrict__ keys,
+                                                    const uint64_t *__restrict__ vals, uint32_t pbits,
+                                                    CensusSoA<Real> cen,
                                                     const CombCell *__restrict__ cells,
                                                     const uint32_t *__restrict__ start,
-                                                    const uint32_t *__restrict__ cnt,
-                                                    const ulonglong2 *__restrict__ cum, int32_t eq,
+                                                    const uint32_t *__restrict__ end,
+                                                    const ulonglong2 *__restrict__ cum, int32_t eq, uint32_t N,
                                                     uint32_t *__restrict__ ncopy) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_valid) return;
-  const uint32_t c = static_cast<uint32_t>(keys[i] >> 32);
+  if (i >= n) return;
+  const uint32_t c = keys[i] >> pbits;
+  if (c >= N) {
+    ncopy[i] = 0u;
+    return;
+  }
   const CombCell cc = cells[c];
-  const double w = cen.w[vals[i]];
-  uint32_t n = 1;
+  const double w = cen.w[static_cast<uint32_t>(vals[i])];
+  uint32_t k = 1;
   if (cc.mode == 1u) {
-    if (w > 2.0 * cc.w_out) n = static_cast<uint32_t>(ceil(w / cc.w_out));
+    if (w > 2.0 * cc.w_out) k = static_cast<uint32_t>(ceil(w / cc.w_out));
   } else if (cc.mode == 2u) {
     const uint32_t s = start[c];
-    const bool last = (i + 1u == s + cnt[c]);
-    ulonglong2 a = cum[i];
-    unsigned __int128 Cin = (static_cast<unsigned __int128>(a.y) << fx::kLoBits) + a.x;
-    unsigned __int128 Cbase = 0;
-    if (s > 0u) {
-      ulonglong2 b = cum[s - 1u];
-      Cbase = (static_cast<unsigned __int128>(b.y) << fx::kLoBits) + b.x;
-    }
+    const bool last = (i + 1u == end[c]);
+    const unsigned __int128 Cbase = s > 0u ? cum_value(cum, s - 1u) : static_cast<unsigned __int128>(0);
     fx::Limbs wl = fx::limbs_of(w, eq);
     unsigned __int128 wv = (static_cast<unsigned __int128>(wl.hi) << fx::kLoBits) + wl.lo;
-    unsigned __int128 Cincl = Cin - Cbase;
+    unsigned __int128 Cincl = cum_value(cum, i) - Cbase;
     unsigned __int128 Cexcl = Cincl - wv;
     const double Wc = cc.Wd * fpm::pow2d(-eq);
     const double tgt = static_cast<double>(cc.target);
     uint64_t k1 = last ? cc.target : comb_teeth(u128_to_double(Cincl), Wc, tgt, cc.r);
     uint64_t k0 = comb_teeth(u128_to_double(Cexcl), Wc, tgt, cc.r);
-    n = static_cast<uint32_t>(k1 > k0 ? k1 - k0 : 0);
+    k = static_cast<uint32_t>(k1 > k0 ? k1 - k0 : 0);
   }
-  ncopy[i] = n;
+  ncopy[i] = k;
 }
 
+// Writes the combed census.  The census weight out per cell is summed over
+// each run of equal cells inside the wavefront (the keys are sorted, so a
+// cell's lanes are side by side) and added once per run, instead of once per
+// packet into the same two words.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_comb_write(uint32_t n_valid, const uint64_t *__restrict__ keys,
-                                                    const uint32_t *__restrict__ vals, CensusSoA<Real> cen,
+__global__ __launch_bounds__(256) void k_comb_write(uint32_t n, const uint32_t *__restrict__ keys,
+                                                    const uint64_t *__restrict__ vals, uint32_t pbits,
+                                                    CensusSoA<Real> cen,
                                                     const CombCell *__restrict__ cells,
                                                     const uint32_t *__restrict__ ncopy,
                                                     const uint64_t *__restrict__ offs, CensusSoA<Real> out,
-                                                    int32_t eq, Tallies t) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_valid) return;
-  const uint32_t n = ncopy[i];
-  if (n == 0u) return;
-  const uint32_t c = static_cast<uint32_t>(keys[i] >> 32);
-  const CombCell cc = cells[c];
-  const uint32_t src = vals[i];
-  const double w = cen.w[src];
-  double wout = w;
-  if (cc.mode == 2u) {
-    wout = cc.w_out;
-  } else if (n > 1u) {
-    // split copies: floor(w / n) quanta each, remainder to the material
-    fx::Limbs wl = fx::limbs_of(w, eq);
-    const unsigned __int128 wv = (static_cast<unsigned __int128>(wl.hi) << fx::kLoBits) + wl.lo;
-    const unsigned __int128 q = floor53(wv / static_cast<unsigned __int128>(n));
-    wout = u128_exact_double(q) * fpm::pow2d(eq);
-    tally_limbs(t.rem, c, remainder_limbs(wv, q, n));
+                                                    int32_t eq, uint32_t N, Tallies t) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t c = 0xFFFFFFFFu;  // lanes past the end sort after every cell
+  unsigned long long lo = 0, hi = 0;
+  if (i < n) {
+    c = keys[i] >> pbits;
+    const uint32_t k = ncopy[i];
+    if (k) {
+      const CombCell cc = cells[c];
+      const uint32_t src = static_cast<uint32_t>(vals[i]);
+      const double w = cen.w[src];
+      double wout = w;
+      if (cc.mode == 2u) {
+        wout = cc.w_out;
+      } else if (k > 1u) {
+        // split copies: floor(w / k) quanta each, remainder to the material
+        fx::Limbs wl = fx::limbs_of(w, eq);
+        const unsigned __int128 wv = (static_cast<unsigned __int128>(wl.hi) << fx::kLoBits) + wl.lo;
+        const unsigned __int128 q = floor53(wv / static_cast<unsigned __int128>(k));
+        wout = u128_exact_double(q) * fpm::pow2d(eq);
+        tally_limbs(t.rem, c, remainder_limbs(wv, q, k));
+      }
+      const auto a = cen.pos[src];
+      const auto b = cen.dir[src];
+      const uint2 m = cen.meta[src];
+      const uint64_t o = offs[i];
+      for (uint32_t q = 0; q < k; ++q) {
+        out.pos[o + q] = a;
+        out.dir[o + q] = b;
+        out.w[o + q] = wout;
+        out.meta[o + q] = make_uint2(m.x, static_cast<uint32_t>(o + q));
+      }
+      const fx::Limbs l = fx::limbs_of(wout, eq);
+      lo = static_cast<unsigned long long>(l.lo) * k;
+      hi = static_cast<unsigned long long>(l.hi) * k;
+    }
   }
-  const auto a = cen.pos[src];
-  const auto b = cen.dir[src];
-  const uint2 m = cen.meta[src];
-  const uint64_t o = offs[i];
-  for (uint32_t q = 0; q < n; ++q) {
-    out.pos[o + q] = a;
-    out.dir[o + q] = b;
-    out.w[o + q] = wout;
-    out.meta[o + q] = make_uint2(m.x, static_cast<uint32_t>(o + q));
+  wave_run_sum(c, lo);
+  if (wave_run_sum(c, hi) && c < N) {
+    if (lo) atomicAdd(&t.cout[2 * c], lo);
+    if (hi) atomicAdd(&t.cout[2 * c + 1], hi);
   }
-  tally_limbs_n(t.cout, c, fx::limbs_of(wout, eq), n);
-}
-
-// (lo, hi) limbs of each sorted census weight, for the prefix sums
-template <typename Real>
-__global__ __launch_bounds__(256) void k_comb_limbs(uint32_t n_valid, const uint32_t *__restrict__ vals,
-                                                    CensusSoA<Real> cen, int32_t eq,
-                                                    ulonglong2 *__restrict__ out) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_valid) return;
-  fx::Limbs l = fx::limbs_of(cen.w[vals[i]], eq);
-  out[i] = make_ulonglong2(l.lo, l.hi);
 }
 
 // ------------------------------------------------------------------ postStep
@@ -1156,40 +1250,6 @@ __global__ __launch_bounds__(256) void k_post(uint32_t N, Tallies t, int32_t eq,
   if (t.segs[c] >= (1u << 28)) atomicOr(err, kErrTallyOverflow);
 }
 
-// Exact total of per-cell limb pairs: out[0] += low limbs, out[1] += high
-// limbs, each block's part normalised first (low part below 2^36), so the
-// total is (out[1] << 36) + out[0] with no overflow.
-__global__ __launch_bounds__(256) void k_total_limbs(uint32_t N, const unsigned long long *__restrict__ L,
-                                                     unsigned long long *__restrict__ out) {
-  constexpr unsigned long long kMask = (1ull << fx::kLoBits) - 1ull;
-  unsigned long long lo = 0, hi = 0;
-  for (uint32_t c = blockIdx.x * 256u + threadIdx.x; c < N; c += gridDim.x * 256u) {
-    lo += L[2 * c];
-    hi += L[2 * c + 1] + (lo >> fx::kLoBits);
-    lo &= kMask;
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    lo += __shfl_xor(lo, off, 64);
-    hi += __shfl_xor(hi, off, 64);
-  }
-  hi += lo >> fx::kLoBits;
-  lo &= kMask;
-  __shared__ unsigned long long slo[4], shi[4];
-  const uint32_t w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63u) == 0u) {
-    slo[w] = lo;
-    shi[w] = hi;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    lo = slo[0] + slo[1] + slo[2] + slo[3];
-    hi = shi[0] + shi[1] + shi[2] + shi[3] + (lo >> fx::kLoBits);
-    lo &= kMask;
-    if (lo) atomicAdd(&out[0], lo);
-    if (hi) atomicAdd(&out[1], hi);
-  }
-}
-
 // out[0] += sum of v[0..N) (integer, order independent)
 __global__ __launch_bounds__(256) void k_total_u32(uint32_t N, const uint32_t *__restrict__ v,
                                                    unsigned long long *__restrict__ out) {
@@ -1203,6 +1263,18 @@ __global__ __launch_bounds__(256) void k_total_u32(uint32_t N, const uint32_t *_
     s = sh[0] + sh[1] + sh[2] + sh[3];
     if (s) atomicAdd(out, s);
   }
+}
+
+// Up to 8 device scalars of 4 or 8 bytes, gathered for one copy to the host.
+struct ScalarRefs {
+  const void *p[8];
+  uint32_t bytes[8];
+};
+__global__ void k_gather_scalars(ScalarRefs r, uint32_t n, unsigned long long *__restrict__ out) {
+  const uint32_t t = threadIdx.x;
+  if (t >= n) return;
+  out[t] = r.bytes[t] == 8u ? *static_cast<const unsigned long long *>(r.p[t])
+                            : static_cast<unsigned long long>(*static_cast<const uint32_t *>(r.p[t]));
 }
 
 __global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v) {

@@ -215,6 +215,52 @@ def test_same_seed_is_bit_identical():
     assert np.array_equal(outs[0][1], outs[1][1])
 
 
+def test_comb_order_in_a_crowded_cell_is_deterministic():
+    """120 000 census packets in one cell of a 2^20-cell mesh.  The comb's
+    32-bit sort keys keep 11 bits of a packet's rank there, so ~30-60 packets
+    share a key and the tie pass orders them by their full rank (insertion
+    sort for runs up to 32, heap sort above).  The transport writes the census
+    in a different order from run to run; the combed census, the tallies and
+    the energy ledger must come out the same every time."""
+    nx, ny, nz = 128, 128, 64
+    N = nx * ny * nz
+    spec = P.ProblemSpec(
+        name="crowd", x_edges=np.linspace(0, 1.0, nx + 1), y_edges=np.linspace(0, 1.0, ny + 1),
+        z_edges=np.linspace(0, 0.5, nz + 1), sigma_a0=np.full(N, 1e-3), T0=np.zeros(N), dt=1e-13, steps=1,
+        face_kind=(capi.FACE_REFLECT,) * 6, source_face=-1, source_per_face=0, new_photons_per_cell=1,
+        comb=1, comb_nmin=20, comb_factor=1.0, seed=5)
+    rng = np.random.default_rng(17)
+    n = 120_000
+    recs = np.zeros(n, dtype=np.ctypeslib.as_array((capi.PacketRecord * 1)()).dtype)
+    recs["x"] = rng.random(n) / nx
+    recs["y"] = rng.random(n) / ny
+    recs["z"] = rng.random(n) * 0.5 / nz
+    mu = 2.0 * rng.random(n) - 1.0
+    phi = 2.0 * np.pi * rng.random(n)
+    st = np.sqrt(1.0 - mu * mu)
+    c_light = 2.99792458e10
+    recs["vx"], recs["vy"], recs["vz"] = c_light * mu, c_light * st * np.cos(phi), c_light * st * np.sin(phi)
+    recs["weight"] = 1e-3
+    recs["cell"] = 0
+    recs["id"] = np.arange(n)
+    outs = []
+    for _ in range(3):
+        with GpuEngine(spec, capi.MIXED) as e:
+            e.import_census(recs)
+            e.step()
+            L = e.ledger()
+            ok, detail = _ledger_ok(L)
+            assert ok, detail
+            c = e.counters()
+            assert c["census_before_comb"] > 50_000 and c["end_particles"] < c["census_before_comb"], c
+            outs.append((e.export_census(), e.tallies(), L))
+    for k in range(1, len(outs)):
+        assert np.array_equal(outs[0][0], outs[k][0])
+        for f in outs[0][1]:
+            assert np.array_equal(outs[0][1][f], outs[k][1][f]), f
+        assert outs[0][2] == outs[k][2]
+
+
 @pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
 def test_beer_lambert_transmission(precision):
     """Pure absorber slab (f = 1, T0 = 0: no emission) lit by the x- surface
