@@ -166,7 +166,7 @@ int storm_gpu_export_census(storm_gpu_engine *eng, storm_packet_record *out, uin
 int storm_gpu_import_census(storm_gpu_engine *eng, const storm_packet_record *in, uint64_t n);
 
 /* Launch parameters of the transport: block size (64, 128, 256, 512; the
- * default 512; REFILL uses at most 256), segments per packet and launch, and
+ * default 256; REFILL uses at most 256), segments per packet and launch, and
  * the schedule: PASS (the default) runs one packet per thread for a launch,
  * REFILL keeps every lane busy by handing a lane whose packet ended the next
  * queued packet.  Results per packet do not depend on the schedule or the

@@ -67,6 +67,10 @@ struct DevBuf {
 inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t>((n + b - 1) / b); }
 
 constexpr uint64_t kMaxSegmentsPerStep = uint64_t(1) << 24;
+// k_transport: 64-packet chunks per wavefront of a block (a block owns
+// kChunksPerWave * block packets of the cell-ordered input and one LDS
+// tally table for all of them).
+constexpr uint32_t kChunksPerWave = 2;
 
 template <typename Real>
 struct PacketBufs {
@@ -128,7 +132,7 @@ struct EngineBase {
   virtual void debug_spectrum(int32_t *, double *, void *, void *, void *) = 0;
   storm_gpu_counters counters{};
   double source_T = 0.0;
-  int32_t block = 512;  // k_transport; k_transport_refill uses at most 256
+  int32_t block = 256;  // k_transport; k_transport_refill uses at most 256
   int32_t seg_per_pass = 1 << 20;
   int32_t schedule = STORM_GPU_SCHEDULE_PASS;
   bool keep_inputs = false;
@@ -484,7 +488,7 @@ struct Engine final : EngineBase {
     if (!r) {
       int per_cu = 0;
       HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<Real, F>, b, 0));
-      r = cache_slots(static_cast<uint32_t>(b), nf, static_cast<uint32_t>(std::max(per_cu, 1)));
+      r = cache_slots(static_cast<uint32_t>(b) * kChunksPerWave, nf, static_cast<uint32_t>(std::max(per_cu, 1)));
     }
     return r;
   }
@@ -503,9 +507,10 @@ struct Engine final : EngineBase {
     } else {
       const uint32_t nf = (precision == STORM_GPU_MIXED && groups < 2) ? 1u : 2u;
       const uint32_t slots = pass_cache_slots<F>(block, nf);
-      k_transport<Real, F><<<blocks_for(n_in, block), block, cache_bytes(slots, nf), stream>>>(
+      const uint32_t chunks = kChunksPerWave * static_cast<uint32_t>(block / 64);
+      k_transport<Real, F><<<blocks_for(n_in, 64u * chunks), block, cache_bytes(slots, nf), stream>>>(
           static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
-          xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf);
+          xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
     }
   }
   void launch_transport(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshView<Real> &mv,

@@ -530,12 +530,12 @@ constexpr uint32_t kCacheProbes = 8;
 inline __host__ __device__ uint32_t cache_bytes(uint32_t slots, uint32_t nf) {
   return slots * (8u * nf + 8u) + 16u;
 }
-// Largest power-of-two slot count within 8 per thread and the LDS share of
-// one block when `blocks_per_cu` blocks are resident.
-inline uint32_t cache_slots(uint32_t block, uint32_t nf, uint32_t blocks_per_cu) {
+// Largest power-of-two slot count within 4 per packet of the block's range
+// and the LDS share of one block when `blocks_per_cu` blocks are resident.
+inline uint32_t cache_slots(uint32_t packets, uint32_t nf, uint32_t blocks_per_cu) {
   const uint32_t budget = (160u * 1024u) / (blocks_per_cu ? blocks_per_cu : 1u) - 64u;
   uint32_t s = 1;
-  while (2 * s <= 8 * block && cache_bytes(2 * s, nf) <= budget) s *= 2;
+  while (2 * s <= 4 * packets && cache_bytes(2 * s, nf) <= budget) s *= 2;
   return s;
 }
 struct TallyCache {
@@ -544,6 +544,7 @@ struct TallyCache {
   uint32_t *tag;          // cell + 1; 0: free
   uint32_t *segs;
   uint32_t *waves_done;
+  uint32_t *next_chunk;   // k_transport's chunks handed out so far
   __device__ __forceinline__ unsigned long long &at(uint32_t f, uint32_t s) const { return v[f * (mask + 1u) + s]; }
 };
 
@@ -558,12 +559,16 @@ __device__ __forceinline__ TallyCache cache_init(unsigned char *lds, uint32_t sl
   L.tag = reinterpret_cast<uint32_t *>(lds + 8 * nf * slots);
   L.segs = L.tag + slots;
   L.waves_done = L.segs + slots;
+  L.next_chunk = L.waves_done + 1;
   for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
     L.tag[s] = 0u;
     L.segs[s] = 0u;
     for (uint32_t f = 0; f < nf; ++f) L.at(f, s) = 0ull;
   }
-  if (threadIdx.x == 0) *L.waves_done = 0u;
+  if (threadIdx.x == 0) {
+    *L.waves_done = 0u;
+    *L.next_chunk = 0u;
+  }
   return L;
 }
 
@@ -681,7 +686,14 @@ __device__ __forceinline__ void wave_add_u32(uint32_t v, unsigned long long *dst
   if (v && lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u) atomicAdd(dst, static_cast<unsigned long long>(v));
 }
 
-// stats: [0] escapes, [1] random-walk steps, [2] DDMC steps
+// One launch over n_in packets.  Block b owns the packets
+// [b * chunks * 64, (b + 1) * chunks * 64) and hands them to its wavefronts
+// 64 at a time (one packet per lane, tracked until it ends or has taken
+// `max_segments` segments); a wavefront whose packets ended takes the next
+// chunk.  The packets come ordered by cell, so the block's range is a compact
+// region and its LDS tally table serves all of it; each cell it touched is
+// written to device memory once, when the block's last wavefront ends.
+// stats: [0] escapes, [1] random-walk steps, [2] DDMC steps, [4] longest history
 template <typename Real, uint32_t F>
 __global__ __launch_bounds__(512) void k_transport(uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out,
                                                    uint32_t *__restrict__ n_out, CensusSoA<Real> cen,
@@ -689,29 +701,41 @@ __global__ __launch_bounds__(512) void k_transport(uint32_t n_in, PacketSoA<Real
                                                    const CellXS<Real> *__restrict__ xs, AccelView<Real> acc,
                                                    StepConsts k, int32_t max_segments, Tallies t,
                                                    unsigned long long *__restrict__ stats, uint32_t cache_slots,
-                                                   uint32_t cache_fields) {
+                                                   uint32_t cache_fields, uint32_t chunks) {
   extern __shared__ __align__(16) unsigned char lds[];
   const TallyCache cache = cache_init(lds, cache_slots, cache_fields);
   __syncthreads();
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < n_in;
-  int outcome = kGone;
-  Packet<Real> p;
   DeviceSink sink(t, k.eq);
   sink.cache = cache;
-  if (valid) {
-    p = load_packet(in, i);
-    outcome = transport_packet<Real, F>(p, mesh, xs, acc, k, max_segments, sink);
+  const uint32_t lane = __lane_id();
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * chunks * 64u;
+  uint32_t escaped = 0, longest = 0;
+  for (;;) {
+    uint32_t ch = 0;
+    if (lane == 0u) ch = __hip_atomic_fetch_add(cache.next_chunk, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    ch = __builtin_amdgcn_readfirstlane(ch);
+    if (ch >= chunks) break;
+    const uint64_t i = base + static_cast<uint64_t>(ch) * 64u + lane;
+    const bool valid = i < n_in;
+    int outcome = kGone;
+    Packet<Real> p;
+    if (valid) {
+      p = load_packet(in, static_cast<uint32_t>(i));
+      outcome = transport_packet<Real, F>(p, mesh, xs, acc, k, max_segments, sink);
+    }
+    const bool alive = valid && outcome == kAlive;
+    const bool census = valid && outcome == kCensus;
+    uint32_t slot = wave_append(alive, n_out);
+    if (alive) store_packet(out, slot, p);
+    uint32_t cslot = wave_append(census, n_cen);
+    if (census) store_census(cen, cslot, p);
+    escaped += (valid && outcome == kEscaped) ? 1u : 0u;
+    if (valid && p.ctr > longest) longest = p.ctr;
+    if (base + static_cast<uint64_t>(ch + 1u) * 64u >= n_in) break;  // the last chunk of the input
   }
-  const bool alive = valid && outcome == kAlive;
-  const bool census = valid && outcome == kCensus;
-  uint32_t slot = wave_append(alive, n_out);
-  if (alive) store_packet(out, slot, p);
-  uint32_t cslot = wave_append(census, n_cen);
-  if (census) store_census(cen, cslot, p);
   sink.finish_block();
-  wave_add_u32((valid && outcome == kEscaped) ? 1u : 0u, &stats[0]);
-  wave_max_u32(valid ? p.ctr : 0u, &stats[4]);
+  wave_add_u32(escaped, &stats[0]);
+  wave_max_u32(longest, &stats[4]);
   if (acc.cell) {
     wave_add_u32(sink.rw, &stats[1]);
     wave_add_u32(sink.ddmc, &stats[2]);

@@ -67,7 +67,7 @@ class GpuEngine:
     def set_source_temperature(self, T: float):
         self._check(self.lib.storm_gpu_set_source_temperature(self.h, float(T)))
 
-    def set_tuning(self, block: int = 512, segments_per_pass: int = 1 << 20, schedule: int | None = None):
+    def set_tuning(self, block: int = 256, segments_per_pass: int = 1 << 20, schedule: int | None = None):
         """Transport launch shape; `schedule` is capi.SCHEDULE_PASS (default) or capi.SCHEDULE_REFILL."""
         if schedule is None:
             schedule = capi.SCHEDULE_PASS
