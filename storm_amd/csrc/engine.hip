@@ -38,6 +38,17 @@ struct EngineError : std::runtime_error {
     if (e_ != hipSuccess) throw EngineError(2, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// While an engine steps, the buffers it outgrows are kept (RetiredBuffers)
+// instead of freed: hipFree waits for all work on the device and unmapping a
+// large buffer costs ~0.3 ms, so a census growing through a power of two
+// would stall the step for milliseconds.  Buffers grow by doubling, so what
+// is kept is less than what is in use.
+struct Retired {
+  void *p;
+  size_t bytes;
+};
+thread_local std::vector<Retired> *t_retired = nullptr;
+
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
@@ -47,7 +58,10 @@ struct DevBuf {
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (t_retired) t_retired->push_back(Retired{p, n * sizeof(T)});
+      else (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
   }
@@ -382,6 +396,7 @@ struct Engine final : EngineBase {
   }
 
   ~Engine() override {
+    free_retired();
     if (stream) (void)hipStreamSynchronize(stream);
     if (mail_h) (void)hipHostFree(mail_h);
     if (wface_h) (void)hipHostFree(wface_h);
@@ -561,9 +576,32 @@ struct Engine final : EngineBase {
   }
 
   // ----------------------------------------------------------------- step
+  // Buffers outgrown during steps, freed with the engine, or at the end of a
+  // step once they hold more than kRetiredLimit bytes (also when the step
+  // throws).
+  static constexpr size_t kRetiredLimit = size_t(16) << 30;
+  std::vector<Retired> retired;
+  void free_retired() {
+    if (retired.empty()) return;
+    (void)hipStreamSynchronize(stream);
+    for (const Retired &r : retired) (void)hipFree(r.p);
+    retired.clear();
+  }
+  struct StepRetire {
+    Engine &e;
+    explicit StepRetire(Engine &en) : e(en) { t_retired = &e.retired; }
+    ~StepRetire() {
+      t_retired = nullptr;
+      size_t bytes = 0;
+      for (const Retired &r : e.retired) bytes += r.bytes;
+      if (bytes > kRetiredLimit) e.free_retired();
+    }
+  };
+
   void step(double dt) override {
     if (!(dt > 0.0)) throw EngineError(1, "dt must be positive");
     auto t_start = std::chrono::steady_clock::now();
+    StepRetire retire(*this);
     ++step_index;
     counters = storm_gpu_counters{};
     const MatDev md = matdev();
