@@ -71,7 +71,7 @@ typedef struct storm_gpu_counters {
   uint64_t random_walk_steps;  /* getRandomWalkStepCount */
   uint64_t ddmc_steps;         /* getDDMCStepCount       */
   int32_t energy_quantum_exp;  /* Q = 2^eq erg of the last step */
-  int32_t reserved;
+  int32_t transport_block;     /* block size the transport ran with in the last step */
   double seconds_step;         /* host wall time of the last storm_gpu_step          */
   double seconds_transport;    /* GPU time of the transport kernels (HIP events)     */
   double seconds_transport_max_launch; /* longest single transport launch              */
@@ -166,11 +166,16 @@ int storm_gpu_export_census(storm_gpu_engine *eng, storm_packet_record *out, uin
 int storm_gpu_import_census(storm_gpu_engine *eng, const storm_packet_record *in, uint64_t n);
 
 /* Launch parameters of the transport: block size (64, 128, 256, 512; the
- * default 256; REFILL uses at most 256), segments per packet and launch, and
- * the schedule: PASS (the default) runs one packet per thread for a launch,
- * REFILL keeps every lane busy by handing a lane whose packet ended the next
- * queued packet.  Results per packet do not depend on the schedule or the
- * block size, only on segments_per_pass. */
+ * default 256; REFILL uses at most 256; 0: chosen on line, below), segments
+ * per packet and launch, and the schedule: PASS (the default) runs one packet
+ * per thread for a launch, REFILL keeps every lane busy by handing a lane
+ * whose packet ended the next queued packet.  Results per packet do not depend
+ * on the schedule or the block size, only on segments_per_pass.  With block
+ * size 0 the next six steps run the transport with blocks of 128, 256, 512,
+ * 512, 256 and 128 threads (an order that cancels a steady trend in the
+ * workload) and the engine then keeps the size with the highest mean rate of
+ * transport segments per second (counters.transport_block shows it).  The
+ * results are those of any fixed size. */
 #define STORM_GPU_SCHEDULE_PASS 0
 #define STORM_GPU_SCHEDULE_REFILL 1
 int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segments_per_pass, int32_t schedule);

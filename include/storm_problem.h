@@ -118,8 +118,9 @@ typedef struct storm_imc_params {
  *   boundary source energy: group g with probability B_g(T_source), E
  *                      uniform in the group.
  * With imc.with_random_walk, packets in the optically thick low groups take
- * the partial-gray random walk (RadiationIMC.hpp:1008-1061, 1065-1276).  DDMC
- * is gray-only here.
+ * the partial-gray random walk (RadiationIMC.hpp:1008-1061, 1065-1276).  With
+ * imc.with_ddmc, DDMC runs on the Planck-mean opacity and keeps the photon
+ * energy (the reference without ddmcUseMultigroupPGRW, :1343-1350).
  */
 typedef struct storm_group_desc {
   int32_t num_groups;

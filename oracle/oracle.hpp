@@ -970,8 +970,6 @@ class RadiationIMC {
       : grid_(grid), boundary_(boundary), cells_(cells), ext_(ext), mat_(mat), params_(params),
         rng_(seed), sampler_(grid) {
     if (params_.newPhotonsPerCell == 0) throw OracleError("RadiationIMC requires newPhotonsPerCell > 0");
-    if (mat_.multigroup() && params_.withDDMC)
-      throw OracleError("DDMC is gray-only (no multigroup DDMC)");
     if (params_.withRandomWalk) rw_ = std::make_unique<RandomWalk>();
     Erad_time_avg_.assign(grid.GetPointNo(), 0.0);
     updateGridData();
@@ -1403,7 +1401,9 @@ class RadiationIMC {
     return true;
   }
 
-  // precomputeDDMCData (radiation/RadiationIMC.hpp:1282-1413), gray
+  // precomputeDDMCData (radiation/RadiationIMC.hpp:1282-1413) without
+  // ddmcUseMultigroupPGRW (its default, RadiationIMCParameters.hpp:26): the
+  // Planck mean opacity, also in a multigroup problem (:1343-1350)
   void precomputeDDMCData() {
     size_t N = grid_.GetPointNo();
     ddmc_.assign(N, DDMCCellData{});
@@ -1454,7 +1454,8 @@ class RadiationIMC {
     }
   }
 
-  // tryDDMCStep (radiation/RadiationIMC.hpp:1415-1612), gray (no PGRW)
+  // tryDDMCStep (radiation/RadiationIMC.hpp:1415-1612) without
+  // ddmcUseMultigroupPGRW: Planck-mean absorption, the photon energy is kept
   bool tryDDMCStep(Particle &p, StepResult &fn) {
     size_t c = p.cellIndex;
     const DDMCCellData &d = ddmc_[c];

@@ -90,12 +90,14 @@ AccelView<Real> make_accel_view(const CellAccel<Real> *cells, const HostTables<R
 }
 
 // Host restatement of the engine's per-cell acceleration data.
+// The engine's DDMC cell data (k_accel_cells, k_accel_rates): total opacity
+// st (gray: sigma_a + sigma_s; multigroup: the Planck mean sigma_P +
+// sigma_s) and continuous absorption fabs per cell.
 template <typename Real>
-std::vector<CellAccel<Real>> build_accel(const storm_problem *p, const std::vector<double> &sa,
-                                         const std::vector<double> &ss) {
+void add_ddmc(const storm_problem *p, std::vector<CellAccel<Real>> &out, const std::vector<double> &st,
+              const std::vector<double> &fabs) {
   const int nx = p->mesh.nx, ny = p->mesh.ny, nz = p->mesh.nz;
   const uint32_t N = static_cast<uint32_t>(nx) * ny * nz;
-  std::vector<CellAccel<Real>> out(N);
   std::vector<double> Dt(N);
   auto w = [&](int ax, int idx) {
     const double *e = ax == 0 ? p->mesh.x_edges : (ax == 1 ? p->mesh.y_edges : p->mesh.z_edges);
@@ -106,20 +108,13 @@ std::vector<CellAccel<Real>> build_accel(const storm_problem *p, const std::vect
         i = static_cast<int>(c / (static_cast<uint32_t>(ny) * nz));
     double dx = w(0, i), dy = w(1, j), dz = w(2, k);
     double V = dx * dy * dz, S = 2.0 * (dy * dz + dx * dz + dx * dy), chord = 4.0 * V / S;
-    double st = sa[c] + ss[c];
     CellAccel<Real> &a = out[c];
-    a.sig_t = static_cast<Real>(st);
     a.total = Real(0);
     for (int f = 0; f < 6; ++f) a.rate[f] = Real(0);
-    a.flags = 0u;
-    a.gcut = 0u;
-    a.sig_d = a.sig_t;
-    a.sig_abs = Real(0);
-    a.up_rate = Real(0);
-    a.e_cut = std::numeric_limits<Real>::infinity();
-    if (p->imc.with_random_walk && st * chord >= p->imc.rw_min_cell_optical_depth) a.flags |= 1u;
-    Dt[c] = st > 0.0 ? 1.0 / (3.0 * st) : 0.0;
-    if (p->imc.with_ddmc && st * chord >= p->imc.ddmc_min_cell_optical_depth && Dt[c] > 0.0) a.flags |= 2u;
+    a.ddmc_sig_t = static_cast<Real>(st[c]);
+    a.ddmc_abs = static_cast<Real>(fabs[c]);
+    Dt[c] = st[c] > 0.0 ? 1.0 / (3.0 * st[c]) : 0.0;
+    if (p->imc.with_ddmc && st[c] * chord >= p->imc.ddmc_min_cell_optical_depth && Dt[c] > 0.0) a.flags |= 2u;
   }
   const uint32_t n3[3] = {static_cast<uint32_t>(nx), static_cast<uint32_t>(ny), static_cast<uint32_t>(nz)};
   const uint32_t stride[3] = {static_cast<uint32_t>(ny * nz), static_cast<uint32_t>(nz), 1u};
@@ -148,6 +143,37 @@ std::vector<CellAccel<Real>> build_accel(const storm_problem *p, const std::vect
       for (int f = 0; f < 6; ++f) a.rate[f] = Real(0);
     }
   }
+}
+
+template <typename Real>
+std::vector<CellAccel<Real>> build_accel(const storm_problem *p, const std::vector<double> &sa,
+                                         const std::vector<double> &ss, const std::vector<double> &f) {
+  const int nx = p->mesh.nx, ny = p->mesh.ny, nz = p->mesh.nz;
+  const uint32_t N = static_cast<uint32_t>(nx) * ny * nz;
+  std::vector<CellAccel<Real>> out(N);
+  std::vector<double> st(N), fabs(N);
+  auto w = [&](int ax, int idx) {
+    const double *e = ax == 0 ? p->mesh.x_edges : (ax == 1 ? p->mesh.y_edges : p->mesh.z_edges);
+    return static_cast<double>(static_cast<Real>(e[idx + 1] - e[idx]));
+  };
+  for (uint32_t c = 0; c < N; ++c) {
+    int k = static_cast<int>(c % nz), j = static_cast<int>((c / nz) % ny),
+        i = static_cast<int>(c / (static_cast<uint32_t>(ny) * nz));
+    double dx = w(0, i), dy = w(1, j), dz = w(2, k);
+    double V = dx * dy * dz, S = 2.0 * (dy * dz + dx * dz + dx * dy), chord = 4.0 * V / S;
+    st[c] = sa[c] + ss[c];
+    fabs[c] = f[c] * sa[c];
+    CellAccel<Real> &a = out[c];
+    a.sig_t = static_cast<Real>(st[c]);
+    a.flags = 0u;
+    a.gcut = 0u;
+    a.sig_d = a.sig_t;
+    a.sig_abs = Real(0);
+    a.up_rate = Real(0);
+    a.e_cut = std::numeric_limits<Real>::infinity();
+    if (p->imc.with_random_walk && st[c] * chord >= p->imc.rw_min_cell_optical_depth) a.flags |= 1u;
+  }
+  add_ddmc<Real>(p, out, st, fabs);
   return out;
 }
 
@@ -155,7 +181,8 @@ std::vector<CellAccel<Real>> build_accel(const storm_problem *p, const std::vect
 // sigma_a and sigma_s, Fleck factors per cell.
 template <typename Real>
 std::vector<CellAccel<Real>> build_accel_mg(const storm_problem *p, const std::vector<double> &sa,
-                                            const std::vector<double> &ss, const std::vector<double> &f) {
+                                            const std::vector<double> &ss, const std::vector<double> &f,
+                                            const std::vector<double> &sP) {
   const int nx = p->mesh.nx, ny = p->mesh.ny, nz = p->mesh.nz;
   const uint32_t N = static_cast<uint32_t>(nx) * ny * nz;
   const int32_t G = p->groups.num_groups;
@@ -181,7 +208,7 @@ std::vector<CellAccel<Real>> build_accel_mg(const storm_problem *p, const std::v
       const double Br = (ar > 0.0 && br > ar) ? planck::integral(ar, br) : 1.0 / static_cast<double>(G);
       const double st = sg + ss[c];
       allAB += sg * Br;
-      if (!thin && st * chord >= p->imc.rw_min_cell_optical_depth) {
+      if (!thin && p->imc.with_random_walk && st * chord >= p->imc.rw_min_cell_optical_depth) {
         cut = static_cast<uint32_t>(g) + 1u;
         dB += Br;
         dBA += Br * sg;
@@ -198,6 +225,7 @@ std::vector<CellAccel<Real>> build_accel_mg(const storm_problem *p, const std::v
     r.gcut = 0u;
     r.sig_t = r.sig_d = r.sig_abs = r.up_rate = Real(0);
     r.e_cut = std::numeric_limits<Real>::infinity();
+    r.ddmc_sig_t = r.ddmc_abs = Real(0);
     if (cut > 0u && dB > 0.0) {
       const double sA = dBA / dB, sT = dBT / dB;
       const double gamma = allAB > 0.0 ? dBA / allAB : 1.0;
@@ -210,6 +238,15 @@ std::vector<CellAccel<Real>> build_accel_mg(const storm_problem *p, const std::v
       r.e_cut = static_cast<Real>(gb[cut] / units::kev);
     }
   }
+  if (p->imc.with_ddmc) {
+    // DDMC on the Planck means (k_accel_cells after k_prestep_mg)
+    std::vector<double> st(N), fabs(N);
+    for (uint32_t c = 0; c < N; ++c) {
+      st[c] = sP[c] + ss[c];
+      fabs[c] = f[c] * sP[c];
+    }
+    add_ddmc<Real>(p, out, st, fabs);
+  }
   return out;
 }
 }  // namespace
@@ -217,6 +254,10 @@ std::vector<CellAccel<Real>> build_accel_mg(const storm_problem *p, const std::v
 extern "C" {
 
 const char *replay_last_error(void) { return g_err.c_str(); }
+
+// Bytes of one storm::CellAccel<float> record (the layout the replay entry
+// points and storm_gpu_debug_accel use).
+uint64_t replay_cell_accel_bytes(void) { return sizeof(CellAccel<float>); }
 
 // Philox4x32-10 block (for known-answer tests).
 void replay_philox(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4) {
@@ -402,7 +443,7 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
     const storm_material_desc &m = p->material;
     const int32_t G = p->groups.num_groups >= 2 ? p->groups.num_groups : 0;
     const double *gb = p->groups.bounds;
-    std::vector<double> cdfd(G ? static_cast<size_t>(N) * (G + 1) : 0), s1(N, 0.0);
+    std::vector<double> cdfd(G ? static_cast<size_t>(N) * (G + 1) : 0), s1(N, 0.0), sPc(N, 0.0);
     double Etot = 0.0;
     for (uint32_t c = 0; c < N; ++c) {
       double T = p->T0[c];
@@ -436,6 +477,7 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
         for (int32_t g = 1; g <= G; ++g) my[g] = valid ? my[g] / acc : (gb[g] - gb[0]) / (gb[G] - gb[0]);
         my[G] = 1.0;
         sP = sumB > 0.0 ? acc / sumB : 1e-20;
+        sPc[c] = sP;
         s1[c] = sa[c] * std::pow(units::kev / p->groups.nu_ref, -p->groups.nu_pow);
       }
       f[c] = 1.0 / (1.0 + (4.0 * units::arad * T3 * sP * units::clight * dt) / cv);
@@ -537,7 +579,7 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       std::vector<CellAccel<Real>> acc_cells;
       std::unique_ptr<HostTables<Real>> tb;
       if (accel_on) {
-        acc_cells = G ? build_accel_mg<Real>(p, sa, ss, f) : build_accel<Real>(p, sa, ss);
+        acc_cells = G ? build_accel_mg<Real>(p, sa, ss, f, sPc) : build_accel<Real>(p, sa, ss, f);
         if (p->imc.with_random_walk) tb.reset(new HostTables<Real>());
       }
       AccelView<Real> av = make_accel_view<Real>(accel_on ? acc_cells.data() : nullptr, tb.get(),

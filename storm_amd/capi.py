@@ -77,7 +77,7 @@ class Counters(C.Structure):
                 ("segments", C.c_uint64), ("transport_passes", C.c_uint64),
                 ("transport_launches", C.c_uint64), ("random_walk_steps", C.c_uint64),
                 ("ddmc_steps", C.c_uint64), ("energy_quantum_exp", C.c_int32),
-                ("reserved", C.c_int32), ("seconds_step", C.c_double),
+                ("transport_block", C.c_int32), ("seconds_step", C.c_double),
                 ("seconds_transport", C.c_double), ("seconds_transport_max_launch", C.c_double),
                 ("max_packet_blocks", C.c_uint64)]
 

@@ -118,7 +118,12 @@ struct CellXS {
 //           re-emission above the diffusive groups, which ends a walk
 //   e_cut   multigroup: lower bound (keV) of group gcut; packets at or above
 //           it do not walk (+inf gray)
-// (precomputeRandomWalkData, radiation/RadiationIMC.hpp:983-1063)
+//   ddmc_sig_t  DDMC: sigma_P + sigma_s, the Planck-mean total opacity (the
+//           particle threshold's opacity; equal to sig_t in a gray problem)
+//   ddmc_abs    DDMC in a multigroup problem: f sigma_P, the continuous
+//           absorption during a DDMC step (a gray step uses the cell's own)
+// (precomputeRandomWalkData, radiation/RadiationIMC.hpp:983-1063;
+// precomputeDDMCData without ddmcUseMultigroupPGRW, :1282-1351)
 template <typename Real>
 struct CellAccel {
   Real sig_t;
@@ -130,6 +135,8 @@ struct CellAccel {
   Real sig_abs;
   Real up_rate;
   Real e_cut;
+  Real ddmc_sig_t;
+  Real ddmc_abs;
 };
 
 // Random-walk tables (see random_walk.h) and the particle thresholds.
@@ -561,13 +568,15 @@ struct Tracker {
           if (rev == 1) census(k, sink);
           return outcome;
         }
-        if ((ca.flags & 2u) && ca.total > Real(0) && tauR >= acc.ddmc_particle_tau) {
-          // ---------------- DDMC step (tryDDMCStep, gray)
+        if ((ca.flags & 2u) && ca.total > Real(0) && Ro * ca.ddmc_sig_t >= acc.ddmc_particle_tau) {
+          // ---------------- DDMC step (tryDDMCStep; in a multigroup problem
+          // with the Planck-mean data and the photon energy kept, as the
+          // reference does without ddmcUseMultigroupPGRW)
           const Philox4 r = rs.block(ctr++);
           const Real sEvent = (-M::log(M::uniform(r.v[0]))) / ca.total;
           const bool cen = s <= sEvent;
           const Real d = cen ? s : sEvent;
-          absorb(sig_abs, d, true);
+          absorb((F & kFeatMultigroup) != 0u ? ca.ddmc_abs : sig_abs, d, true);
           s = cen ? Real(0) : s - d;
           if (s < Real(0)) s = Real(0);
           sink.ddmc_step();

@@ -147,6 +147,25 @@ struct EngineBase {
   storm_gpu_counters counters{};
   double source_T = 0.0;
   int32_t block = 256;  // k_transport; k_transport_refill uses at most 256
+  // On-line choice of the block size (storm_gpu_set_tuning with block 0):
+  // steps tried so far (-1: off) and the summed rates of each candidate.
+  static constexpr int32_t kTuneOrder[6] = {128, 256, 512, 512, 256, 128};
+  int32_t tune_step = -1;
+  double tune_rate[3] = {0.0, 0.0, 0.0};
+  void tune_before_step() {
+    if (tune_step >= 0) block = kTuneOrder[tune_step];
+  }
+  void tune_after_step(double segments, double seconds) {
+    if (tune_step < 0) return;
+    const int32_t b = kTuneOrder[tune_step];
+    tune_rate[b == 128 ? 0 : (b == 256 ? 1 : 2)] += seconds > 0.0 ? segments / seconds : 0.0;
+    if (++tune_step < 6) return;
+    int best = 1;
+    for (int k = 0; k < 3; ++k)
+      if (tune_rate[k] > tune_rate[best]) best = k;
+    block = best == 0 ? 128 : (best == 1 ? 256 : 512);
+    tune_step = -1;
+  }
   int32_t seg_per_pass = 1 << 20;
   int32_t schedule = STORM_GPU_SCHEDULE_PASS;
   bool keep_inputs = false;
@@ -157,6 +176,7 @@ struct Engine final : EngineBase {
   // problem (host copies)
   int32_t nx, ny, nz;
   uint32_t N;
+  Morton morton{};  // Z-order of the cells (the combed census order)
   std::vector<double> xe, ye, ze, Vh, sa0h, ss0h, rhoh;
   storm_material_desc matd;
   storm_boundary_desc bnd;
@@ -180,7 +200,7 @@ struct Engine final : EngineBase {
       clamp, partials, host_sa, host_ss, host_cv, wface, dsc;
   bool have_host_sa = false, have_host_ss = false, have_host_cv = false;
   DevBuf<CellXS<Real>> xs;
-  DevBuf<uint32_t> nphot, err, run_start, run_end, ncopy, keys, keys_out, counters_u32;
+  DevBuf<uint32_t> nphot, nphot_z, err, run_start, run_end, ncopy, keys, keys_out, counters_u32;
   DevBuf<uint64_t> offsets, vals, vals_out, offs64, nphot64;
   DevBuf<unsigned long long> tally_u64, emit_limbs, requant, leaving;
   DevBuf<double> tally_f64;
@@ -228,6 +248,18 @@ struct Engine final : EngineBase {
     // cell indices stay below 2^31 in every tally offset (2 c + 1)
     if (static_cast<uint64_t>(nx) * ny * nz > (1ull << 30)) throw EngineError(1, "more than 2^30 cells");
     N = static_cast<uint32_t>(nx) * ny * nz;
+    {
+      const int32_t n3[3] = {nx, ny, nz};
+      morton.total = 0;
+      for (int a = 0; a < 3; ++a) {
+        uint32_t b = 0;
+        while ((1ll << b) < n3[a]) ++b;
+        morton.bits[a] = b;
+        morton.total += b;
+      }
+      // the comb's keys hold the code and one more value (dropped packets)
+      if (morton.total > 31) throw EngineError(1, "mesh too large for 32-bit Z-order cell codes");
+    }
     xe.assign(p->mesh.x_edges, p->mesh.x_edges + nx + 1);
     ye.assign(p->mesh.y_edges, p->mesh.y_edges + ny + 1);
     ze.assign(p->mesh.z_edges, p->mesh.z_edges + nz + 1);
@@ -251,7 +283,6 @@ struct Engine final : EngineBase {
       nu_pow = p->groups.nu_pow;
       nu_ref = p->groups.nu_ref;
       if (!(nu_ref > 0.0)) throw EngineError(1, "groups.nu_ref must be positive");
-      if (imc.with_ddmc) throw EngineError(1, "DDMC is gray-only (multigroup DDMC is not implemented)");
     }
     // one packet per thread per pass: measured faster than the refill
     // schedule on the 64^3 cube for both engines
@@ -304,7 +335,8 @@ struct Engine final : EngineBase {
     xs.ensure(N);
     nphot.ensure(N);
     nphot64.ensure(N + 1);
-    offsets.ensure(N + 1);
+    nphot_z.ensure(size_t(1) << morton.total);
+    offsets.ensure((size_t(1) << morton.total) + 1);
     err.ensure(1);
     run_start.ensure(N);
     run_end.ensure(N);
@@ -616,18 +648,18 @@ struct Engine final : EngineBase {
       k_prestep_mg<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
           N, T.p, md, have_host_sa ? host_sa.p : nullptr, have_host_ss ? host_ss.p : nullptr,
           have_host_cv ? host_cv.p : nullptr, V.p, dt, groups, bounds_erg.p, nu_pow, nu_ref, xs.p, cdf.p,
-          sig_s_cell.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p, d, meshview(), imc.rw_min_cell_optical_depth,
-          accel_on ? accel.p : nullptr);
+          sig_s_cell.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p, d, meshview(),
+          imc.with_random_walk ? imc.rw_min_cell_optical_depth : HUGE_VAL, accel_on ? accel.p : nullptr);
     } else {
       k_prestep<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
           N, T.p, md, have_host_sa ? host_sa.p : nullptr, have_host_ss ? host_ss.p : nullptr,
           have_host_cv ? host_cv.p : nullptr, V.p, dt, xs.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p);
     }
-    if (accel_on && groups < 2) {
+    if (accel_on && (groups < 2 || imc.with_ddmc)) {
       const MeshView<Real> mv0 = meshview();
       k_accel_cells<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
-          N, d, mv0, sig_a.p, sig_s.p, imc.with_random_walk, imc.rw_min_cell_optical_depth, imc.with_ddmc,
-          imc.ddmc_min_cell_optical_depth, Dtil.p, accel.p);
+          N, d, mv0, sig_a.p, sig_s.p, fleck.p, imc.with_random_walk, imc.rw_min_cell_optical_depth, imc.with_ddmc,
+          imc.ddmc_min_cell_optical_depth, groups >= 2 ? 1 : 0, Dtil.p, accel.p);
       k_accel_rates<Real><<<blocks_for(N, 256), 256, 0, stream>>>(N, d, mv0, Dtil.p, accel.p);
     }
     sum_async(Eemit.p, N, dsc.p + 2);  // emission total, read by k_quantum and k_alloc
@@ -665,19 +697,23 @@ struct Engine final : EngineBase {
     k_quantum<<<1, 64, 0, stream>>>(dsc.p + 2, census_total_prev, source_total, units::clight, dt, qexp.p);
     k_alloc<<<blocks_for(N, 256), 256, 0, stream>>>(N, Eemit.p, dsc.p + 2, imc.new_photons_per_cell, qexp.p,
                                                     nphot.p, wemit.p, emit_limbs.p);
+    // photon counts in the cells' Z order, so emission is laid out like the
+    // combed census
+    const uint32_t M = 1u << morton.total;
+    k_morton_counts<<<blocks_for(M, 256), 256, 0, stream>>>(M, morton, d, nphot.p, nphot_z.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
-        HIPCHECK(rocprim::exclusive_scan(tp, bytes, nphot.p, offsets.p, uint64_t(0), static_cast<size_t>(N) + 0,
+        HIPCHECK(rocprim::exclusive_scan(tp, bytes, nphot_z.p, offsets.p, uint64_t(0), static_cast<size_t>(M),
                                          rocprim::plus<uint64_t>(), stream));
       };
       ensure_temp(q);
     }
     const unsigned long long *hs =
-        read_scalars({{offsets.p + (N - 1), 8u}, {nphot.p + (N - 1), 4u}, {qexp.p, 4u}, {qexp.p + 1, 4u}});
+        read_scalars({{offsets.p + (M - 1), 8u}, {nphot_z.p + (M - 1), 4u}, {qexp.p, 4u}, {qexp.p + 1, 4u}});
     const uint64_t n_emit = hs[0] + hs[1];
     const int32_t eq = static_cast<int32_t>(static_cast<uint32_t>(hs[2]));
     const int32_t et = static_cast<int32_t>(static_cast<uint32_t>(hs[3]));
-    k_fill_u64<<<1, 64, 0, stream>>>(reinterpret_cast<unsigned long long *>(offsets.p + N), 1, n_emit);
+    k_fill_u64<<<1, 64, 0, stream>>>(reinterpret_cast<unsigned long long *>(offsets.p + M), 1, n_emit);
     counters.energy_quantum_exp = eq;
     for (uint32_t f = 0; f < nfaces; ++f) wface_e[f] = fx::quantize_down(wface_e[f], eq);
     if (groups >= 2 && nfaces) {
@@ -739,7 +775,7 @@ struct Engine final : EngineBase {
           static_cast<uint32_t>(n_census), cnext.soa(), pa.soa(), s0, eq_prev, eq, d, tal, requant.p);
     if (n_emit)
       k_emit<Real><<<blocks_for(n_emit, 256), 256, 0, stream>>>(
-          static_cast<uint32_t>(n_emit), N, offsets.p, wemit.p, pa.soa(), static_cast<uint32_t>(n_census), s0, d,
+          static_cast<uint32_t>(n_emit), M, offsets.p, wemit.p, pa.soa(), static_cast<uint32_t>(n_census), s0, d, morton,
           seed, step_index, tal.begin, spectrum());
     if (n_bc)
       k_boundary<Real><<<blocks_for(n_bc, 256), 256, 0, stream>>>(
@@ -757,6 +793,8 @@ struct Engine final : EngineBase {
     }
 
     // transport passes until every packet is in census, absorbed or gone
+    tune_before_step();
+    counters.transport_block = block;
     const MeshView<Real> mv = meshview();
     const AccelView<Real> av = accelview();
     uint64_t n_in = n_total;
@@ -835,6 +873,7 @@ struct Engine final : EngineBase {
     if (herr & kErrFleck) throw EngineError(5, "Invalid factor fleck in RadiationIMC::preStep");
     if (herr & kErrNegativeEnergy) throw EngineError(5, "Negative material internal energy in RadiationIMC");
     if (herr & kErrTallyOverflow) throw EngineError(5, "a cell received 2^28 tally additions in one step");
+    tune_after_step(static_cast<double>(counters.segments), counters.seconds_transport);
     auto t_end = std::chrono::steady_clock::now();
     counters.seconds_step = std::chrono::duration<double>(t_end - t_start).count();
   }
@@ -855,12 +894,11 @@ struct Engine final : EngineBase {
     ncopy.ensure(n);
     offs64.ensure(n);
     const uint32_t salt = static_cast<uint32_t>(seed) * 0x2545F491u + step_index * 0x9E3779B9u;
-    // 32-bit keys: cell (cbits bits, N itself included for dropped
-    // packets) above the top pbits bits of the rank; ties by k_comb_ties
-    uint32_t cbits = 1;
-    while (cbits < 32u && (static_cast<uint64_t>(N) >> cbits) != 0u) ++cbits;
-    const uint32_t pbits = 32u - cbits;
-    k_comb_keys<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, cpre.soa(), d, N, salt, pbits, keys.p, vals.p);
+    // 32-bit keys: the cell's Z-order code (morton.total bits, and 2^total
+    // for dropped packets) above the top pbits bits of the rank; ties by
+    // k_comb_ties
+    const uint32_t pbits = 31u - morton.total;
+    k_comb_keys<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, cpre.soa(), morton, salt, pbits, keys.p, vals.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
         HIPCHECK(rocprim::radix_sort_pairs(tp, bytes, keys.p, keys_out.p, vals.p, vals_out.p, static_cast<size_t>(n),
@@ -868,10 +906,10 @@ struct Engine final : EngineBase {
       };
       ensure_temp(q);
     }
-    k_comb_ties<<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits ? N << pbits : N);
+    k_comb_ties<<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, (1u << morton.total) << pbits);
     // runs of cells over the sorted census, weight limbs and their prefix sums
     zero_u32(run_end.p, N);
-    k_comb_sorted<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits, cpre.soa(), eq, N,
+    k_comb_sorted<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits, cpre.soa(), eq, morton, d,
                                                                 limbs_sorted.p, run_start.p, run_end.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
@@ -885,7 +923,7 @@ struct Engine final : EngineBase {
                                                          imc.comb_factor, eq, imc.comb ? 1 : 0, seed, step_index,
                                                          comb_cells.p, tal.cpre, tal.rem);
     k_comb_count<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits, cpre.soa(), comb_cells.p,
-                                                               run_start.p, run_end.p, cum.p, eq, N, ncopy.p);
+                                                               run_start.p, run_end.p, cum.p, eq, morton, d, ncopy.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
         HIPCHECK(rocprim::exclusive_scan(tp, bytes, ncopy.p, offs64.p, uint64_t(0), static_cast<size_t>(n),
@@ -899,8 +937,8 @@ struct Engine final : EngineBase {
     if (n_out == 0) return 0;
     cnext.ensure(n_out);
     k_comb_write<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits, cpre.soa(),
-                                                               comb_cells.p, ncopy.p, offs64.p, cnext.soa(), eq, N,
-                                                               tal);
+                                                               comb_cells.p, ncopy.p, offs64.p, cnext.soa(), eq, morton,
+                                                               d, tal);
     return n_out;
   }
 
@@ -1354,12 +1392,19 @@ int storm_gpu_import_census(storm_gpu_engine *eng, const storm_packet_record *in
 
 int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segments_per_pass, int32_t schedule) {
   return guarded([&] {
-    if (block_size != 64 && block_size != 128 && block_size != 256 && block_size != 512)
-      throw EngineError(1, "block size must be 64, 128, 256 or 512");
+    if (block_size != 0 && block_size != 64 && block_size != 128 && block_size != 256 && block_size != 512)
+      throw EngineError(1, "block size must be 0 (chosen on line), 64, 128, 256 or 512");
     if (segments_per_pass < 1) throw EngineError(1, "segments per pass must be >= 1");
     if (schedule != STORM_GPU_SCHEDULE_PASS && schedule != STORM_GPU_SCHEDULE_REFILL)
       throw EngineError(1, "schedule must be STORM_GPU_SCHEDULE_PASS or STORM_GPU_SCHEDULE_REFILL");
-    as_engine(eng)->block = block_size;
+    EngineBase *e = as_engine(eng);
+    if (block_size == 0) {
+      e->tune_step = 0;
+      for (double &r : e->tune_rate) r = 0.0;
+    } else {
+      e->tune_step = -1;
+      e->block = block_size;
+    }
     as_engine(eng)->seg_per_pass = segments_per_pass;
     as_engine(eng)->schedule = schedule;
   });

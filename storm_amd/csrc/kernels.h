@@ -203,6 +203,53 @@ __device__ __forceinline__ uint32_t ijk_of_flat(uint32_t c, MeshDims d) {
   return pack_ijk(i, j, k);
 }
 
+// Z-order (Morton) code of a cell: the bits of k, j and i interleaved from
+// the lowest up, an axis dropping out when its bits are used up (bits[a] =
+// ceil(log2(n_a))).  The combed census is written in this order, so the
+// packets one transport block takes are a compact 3-D region of the mesh
+// rather than a line of cells, and touch fewer distinct cells.
+struct Morton {
+  uint32_t bits[3];  // i, j, k
+  uint32_t total;
+};
+__host__ __device__ inline uint32_t morton_encode(uint32_t i, uint32_t j, uint32_t k, Morton m) {
+  uint32_t code = 0, pos = 0;
+  for (uint32_t b = 0; pos < m.total; ++b) {
+    if (b < m.bits[2]) code |= ((k >> b) & 1u) << pos++;
+    if (b < m.bits[1]) code |= ((j >> b) & 1u) << pos++;
+    if (b < m.bits[0]) code |= ((i >> b) & 1u) << pos++;
+  }
+  return code;
+}
+__host__ __device__ inline void morton_ijk(uint32_t code, Morton m, uint32_t &i, uint32_t &j, uint32_t &k) {
+  uint32_t pos = 0;
+  i = j = k = 0;
+  for (uint32_t b = 0; pos < m.total; ++b) {
+    if (b < m.bits[2]) k |= ((code >> pos++) & 1u) << b;
+    if (b < m.bits[1]) j |= ((code >> pos++) & 1u) << b;
+    if (b < m.bits[0]) i |= ((code >> pos++) & 1u) << b;
+  }
+}
+__host__ __device__ inline uint32_t morton_cell(uint32_t code, Morton m, MeshDims d) {
+  uint32_t i, j, k;
+  morton_ijk(code, m, i, j, k);
+  return (i * d.ny + j) * d.nz + k;
+}
+
+// Photon counts in Z order: out[code] for every code below 2^total (0 where
+// the code is outside the mesh), so that emission is laid out like the
+// combed census.
+__global__ __launch_bounds__(256) void k_morton_counts(uint32_t M, Morton mo, MeshDims d,
+                                                       const uint32_t *__restrict__ nphot,
+                                                       uint32_t *__restrict__ out) {
+  const uint32_t code = blockIdx.x * blockDim.x + threadIdx.x;
+  if (code >= M) return;
+  uint32_t i, j, k;
+  morton_ijk(code, mo, i, j, k);
+  out[code] = (i < d.nx && j < d.ny && k < d.nz) ? nphot[(i * d.ny + j) * d.nz + k] : 0u;
+}
+
+
 // ------------------------------------------------------------------ preStep
 template <typename Real>
 __global__ __launch_bounds__(256) void k_prestep(uint32_t N, const double *__restrict__ T, MatDev mat,
@@ -323,6 +370,7 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
     r.gcut = 0u;
     r.sig_t = r.sig_d = r.sig_abs = r.up_rate = Real(0);
     r.e_cut = static_cast<Real>(INFINITY);
+    r.ddmc_sig_t = r.ddmc_abs = Real(0);
     if (cut > 0u && dB > 0.0) {
       const double sA = dBA / dB, sT = dBT / dB;
       const double gamma = allAB > 0.0 ? dBA / allAB : 1.0;
@@ -440,22 +488,24 @@ __global__ __launch_bounds__(256) void k_import(uint32_t n, CensusSoA<Real> cen,
 // Emission packets: uniform position in the cell (RandomInCellPositionSampler
 // is uniform in the box), isotropic direction, timeLeft = dt * U.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_emit(uint32_t n_emit, uint32_t N, const uint64_t *__restrict__ offsets,
+// Emission packets in Z order of their cells: `offsets` is the exclusive scan
+// of the photon counts over the M = 2^total codes (k_morton_counts).
+__global__ __launch_bounds__(256) void k_emit(uint32_t n_emit, uint32_t M, const uint64_t *__restrict__ offsets,
                                               const double *__restrict__ wemit, PacketSoA<Real> out,
-                                              uint32_t base, Real s0, MeshDims d, uint64_t seed,
+                                              uint32_t base, Real s0, MeshDims d, Morton mo, uint64_t seed,
                                               uint32_t step, uint32_t *__restrict__ begin, SpectrumView<Real> sp) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = q < n_emit;
   uint32_t c = 0xFFFFFFFFu;
   if (valid) {
-    // cell owning packet q: last c with offsets[c] <= q
-    uint32_t lo = 0, hi = N;  // offsets[N] = n_emit
+    // code owning packet q: last code with offsets[code] <= q
+    uint32_t lo = 0, hi = M;  // offsets[M] = n_emit
     while (hi - lo > 1u) {
       uint32_t mid = (lo + hi) >> 1;
       if (offsets[mid] <= q) lo = mid;
       else hi = mid;
     }
-    c = lo;
+    c = morton_cell(lo, mo, d);
     Packet<Real> p;
     birth_emission<Real>(p, base + q, ijk_of_flat(c, d), wemit[c], s0, seed, step, sp, c);
     store_packet(out, base + q, p);
@@ -839,11 +889,16 @@ __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketS
 // Per-cell random-walk / DDMC data (precomputeRandomWalkData, RadiationIMC.hpp:982-1063;
 // precomputeDDMCData, :1282-1413), gray.  Pass 1: sigma_t, D/c = 1/(3 sigma_t), flags.
 template <typename Real>
+// In a multigroup problem (`walk_set`) the random-walk fields come from
+// k_prestep_mg and only the DDMC ones are set here, from the Planck mean
+// sigma_P = sig_a (the reference's DDMC without ddmcUseMultigroupPGRW).
 __global__ __launch_bounds__(256) void k_accel_cells(uint32_t N, MeshDims d, MeshView<Real> mv,
                                                      const double *__restrict__ sig_a,
-                                                     const double *__restrict__ sig_s, int32_t rw_on,
+                                                     const double *__restrict__ sig_s,
+                                                     const double *__restrict__ fleck, int32_t rw_on,
                                                      double rw_cell_tau, int32_t ddmc_on, double ddmc_cell_tau,
-                                                     double *__restrict__ Dtil, CellAccel<Real> *__restrict__ out) {
+                                                     int32_t walk_set, double *__restrict__ Dtil,
+                                                     CellAccel<Real> *__restrict__ out) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
   const uint32_t ijk = ijk_of_flat(c, d);
@@ -853,16 +908,23 @@ __global__ __launch_bounds__(256) void k_accel_cells(uint32_t N, MeshDims d, Mes
   const double chord = 4.0 * V / S;
   const double st = sig_a[c] + sig_s[c];
   CellAccel<Real> a;
-  a.sig_t = static_cast<Real>(st);
+  if (walk_set) {
+    a = out[c];
+    a.flags &= 1u;
+  } else {
+    a.sig_t = static_cast<Real>(st);
+    a.flags = 0u;
+    a.gcut = 0u;
+    a.sig_d = a.sig_t;
+    a.sig_abs = Real(0);
+    a.up_rate = Real(0);
+    a.e_cut = static_cast<Real>(INFINITY);
+    if (rw_on && st * chord >= rw_cell_tau) a.flags |= 1u;
+  }
   a.total = Real(0);
   for (int f = 0; f < 6; ++f) a.rate[f] = Real(0);
-  a.flags = 0u;
-  a.gcut = 0u;
-  a.sig_d = a.sig_t;
-  a.sig_abs = Real(0);
-  a.up_rate = Real(0);
-  a.e_cut = static_cast<Real>(INFINITY);
-  if (rw_on && st * chord >= rw_cell_tau) a.flags |= 1u;
+  a.ddmc_sig_t = static_cast<Real>(st);
+  a.ddmc_abs = static_cast<Real>(fleck[c] * sig_a[c]);
   const double Dt = st > 0.0 ? 1.0 / (3.0 * st) : 0.0;
   Dtil[c] = Dt;
   if (ddmc_on && st * chord >= ddmc_cell_tau && Dt > 0.0) a.flags |= 2u;
@@ -925,13 +987,14 @@ __device__ __forceinline__ uint32_t perm32(uint32_t x, uint32_t salt) {
   return x;
 }
 
-// Sort keys of the census: the cell in the high bits and the top `pbits`
-// bits of the packet's rank within its cell below it (32-bit keys: four radix
-// passes).  The value carries the full 32-bit rank above the packet's index,
-// for k_comb_ties.  Weight-0 packets get cell N and sort last.
+// Sort keys of the census: the cell's Morton code in the high bits and the
+// top `pbits` bits of the packet's rank within its cell below it (32-bit
+// keys: four radix passes).  The value carries the full 32-bit rank above
+// the packet's index, for k_comb_ties.  Weight-0 packets get code
+// 2^total and sort last.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_comb_keys(uint32_t n, CensusSoA<Real> cen, MeshDims d, uint32_t N,
-                                                   uint32_t salt, uint32_t pbits, uint32_t *__restrict__ keys,
+__global__ __launch_bounds__(256) void k_comb_keys(uint32_t n, CensusSoA<Real> cen, Morton mo, uint32_t salt,
+                                                   uint32_t pbits, uint32_t *__restrict__ keys,
                                                    uint64_t *__restrict__ vals) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -939,8 +1002,8 @@ __global__ __launch_bounds__(256) void k_comb_keys(uint32_t n, CensusSoA<Real> c
   const uint32_t rank = perm32(m.y, salt);
   vals[i] = (static_cast<uint64_t>(rank) << 32) | i;
   // weight-0 packets are dropped (CombPopulationControl.hpp:136-137)
-  const uint32_t c = cen.w[i] == 0.0 ? N : (ijk_i(m.x) * d.ny + ijk_j(m.x)) * d.nz + ijk_k(m.x);
-  keys[i] = pbits ? (c << pbits) | (rank >> (32u - pbits)) : c;
+  const uint32_t code = cen.w[i] == 0.0 ? (1u << mo.total) : morton_encode(ijk_i(m.x), ijk_j(m.x), ijk_k(m.x), mo);
+  keys[i] = pbits ? (code << pbits) | (rank >> (32u - pbits)) : code;
 }
 
 // Orders each run of equal sort keys by the full rank (the value's high
@@ -998,17 +1061,18 @@ __global__ __launch_bounds__(256) void k_comb_ties(uint32_t n, const uint32_t *_
 template <typename Real>
 __global__ __launch_bounds__(256) void k_comb_sorted(uint32_t n, const uint32_t *__restrict__ keys,
                                                      const uint64_t *__restrict__ vals, uint32_t pbits,
-                                                     CensusSoA<Real> cen, int32_t eq, uint32_t N,
+                                                     CensusSoA<Real> cen, int32_t eq, Morton mo, MeshDims d,
                                                      ulonglong2 *__restrict__ limbs, uint32_t *__restrict__ start,
                                                      uint32_t *__restrict__ end) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t c = keys[i] >> pbits;
+  const uint32_t code = keys[i] >> pbits;
   fx::Limbs l = fx::limbs_of(cen.w[static_cast<uint32_t>(vals[i])], eq);
   limbs[i] = make_ulonglong2(l.lo, l.hi);
-  if (c >= N) return;
-  if (i == 0u || (keys[i - 1u] >> pbits) != c) start[c] = i;
-  if (i + 1u == n || (keys[i + 1u] >> pbits) != c) end[c] = i + 1u;
+  if (code >> mo.total) return;  // dropped
+  const uint32_t c = morton_cell(code, mo, d);
+  if (i == 0u || (keys[i - 1u] >> pbits) != code) start[c] = i;
+  if (i + 1u == n || (keys[i + 1u] >> pbits) != code) end[c] = i + 1u;
 }
 
 struct CombCell {
@@ -1132,15 +1196,16 @@ __global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *
                                                     const CombCell *__restrict__ cells,
                                                     const uint32_t *__restrict__ start,
                                                     const uint32_t *__restrict__ end,
-                                                    const ulonglong2 *__restrict__ cum, int32_t eq, uint32_t N,
-                                                    uint32_t *__restrict__ ncopy) {
+                                                    const ulonglong2 *__restrict__ cum, int32_t eq, Morton mo,
+                                                    MeshDims d, uint32_t *__restrict__ ncopy) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t c = keys[i] >> pbits;
-  if (c >= N) {
+  const uint32_t code = keys[i] >> pbits;
+  if (code >> mo.total) {  // dropped
     ncopy[i] = 0u;
     return;
   }
+  const uint32_t c = morton_cell(code, mo, d);
   const CombCell cc = cells[c];
   const double w = cen.w[static_cast<uint32_t>(vals[i])];
   uint32_t k = 1;
@@ -1174,12 +1239,13 @@ __global__ __launch_bounds__(256) void k_comb_write(uint32_t n, const uint32_t *
                                                     const CombCell *__restrict__ cells,
                                                     const uint32_t *__restrict__ ncopy,
                                                     const uint64_t *__restrict__ offs, CensusSoA<Real> out,
-                                                    int32_t eq, uint32_t N, Tallies t) {
+                                                    int32_t eq, Morton mo, MeshDims d, Tallies t) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t c = 0xFFFFFFFFu;  // lanes past the end sort after every cell
+  uint32_t c = 0xFFFFFFFFu;  // lanes past the end and dropped packets: no cell
   unsigned long long lo = 0, hi = 0;
   if (i < n) {
-    c = keys[i] >> pbits;
+    const uint32_t code = keys[i] >> pbits;
+    if (!(code >> mo.total)) c = morton_cell(code, mo, d);
     const uint32_t k = ncopy[i];
     if (k) {
       const CombCell cc = cells[c];
@@ -1212,7 +1278,7 @@ __global__ __launch_bounds__(256) void k_comb_write(uint32_t n, const uint32_t *
     }
   }
   wave_run_sum(c, lo);
-  if (wave_run_sum(c, hi) && c < N) {
+  if (wave_run_sum(c, hi) && c != 0xFFFFFFFFu) {
     if (lo) atomicAdd(&t.cout[2 * c], lo);
     if (hi) atomicAdd(&t.cout[2 * c + 1], hi);
   }

@@ -68,7 +68,8 @@ class GpuEngine:
         self._check(self.lib.storm_gpu_set_source_temperature(self.h, float(T)))
 
     def set_tuning(self, block: int = 256, segments_per_pass: int = 1 << 20, schedule: int | None = None):
-        """Transport launch shape; `schedule` is capi.SCHEDULE_PASS (default) or capi.SCHEDULE_REFILL."""
+        """Transport launch shape; `schedule` is capi.SCHEDULE_PASS (default) or capi.SCHEDULE_REFILL;
+        block 0 lets the engine pick the block size over the next six steps."""
         if schedule is None:
             schedule = capi.SCHEDULE_PASS
         self._check(self.lib.storm_gpu_set_tuning(self.h, int(block), int(segments_per_pass), int(schedule)))

@@ -85,6 +85,14 @@ def MG_RW(s):
     return spec
 
 
+def MG_DDMC(s):
+    """The same with DDMC instead (Planck-mean data, photon energy kept: the
+    reference without ddmcUseMultigroupPGRW)."""
+    spec = MG(s)
+    spec.with_ddmc = 1
+    return spec
+
+
 @pytest.fixture(scope="module")
 def c1_ref(oracle):
     return oracle_ensemble(oracle, C1, range(1000, 1000 + NSEEDS))
@@ -122,6 +130,11 @@ def mg_rw_ref(oracle):
     return oracle_ensemble(oracle, MG_RW, range(1000, 1000 + NSEEDS))
 
 
+@pytest.fixture(scope="module")
+def mg_ddmc_ref(oracle):
+    return oracle_ensemble(oracle, MG_DDMC, range(1000, 1000 + NSEEDS))
+
+
 @pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
 def test_multigroup_matches_oracle(mg_ref, precision):
     """30-group transport with energy-dependent opacity and re-emission."""
@@ -134,6 +147,17 @@ def test_multigroup_random_walk_matches_oracle(mg_rw_ref, precision):
     """The same with the partial-gray random walk (RadiationIMC.hpp:1008-1060, 1065-1276)."""
     got = gpu_ensemble(MG_RW, range(NSEEDS), precision)
     _assert_parity(mg_rw_ref, got, f"mg_rw/{precision}")
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
+def test_multigroup_ddmc_matches_oracle(mg_ddmc_ref, precision):
+    """The same with DDMC (RadiationIMC.hpp:1282-1612 without ddmcUseMultigroupPGRW)."""
+    got = gpu_ensemble(MG_DDMC, range(NSEEDS), precision)
+    _assert_parity(mg_ddmc_ref, got, f"mg_ddmc/{precision}")
+    with GpuEngine(MG_DDMC(0), precision) as e:
+        e.step()
+        c = e.counters()
+    assert c["ddmc_steps"] > 0, c
 
 
 @pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
@@ -183,7 +207,8 @@ def test_fp64_energy_balance():
             assert L["census_pre"] == L["census_out"] + L["comb_rem"]
 
 
-@pytest.mark.parametrize("make", [C1, C2, C3, MG, MG_RW], ids=["c1", "c2", "c3", "multigroup", "multigroup_rw"])
+@pytest.mark.parametrize("make", [C1, C2, C3, MG, MG_RW, MG_DDMC],
+                         ids=["c1", "c2", "c3", "multigroup", "multigroup_rw", "multigroup_ddmc"])
 def test_replay_reproduces_tallies_bit_exactly(oracle, make):
     spec = make(5)
     with GpuEngine(spec, capi.MIXED) as e:
@@ -340,6 +365,33 @@ def test_refill_schedule_matches_pass_schedule(make):
     for k in ("dep", "rdep", "pint", "esc", "segs"):
         assert np.array_equal(a[1][k], b[1][k]), k
     assert a[2]["segments"] == b[2]["segments"]
+
+
+def test_online_block_size_choice_keeps_results():
+    """Block size 0: the first six steps try 128, 256, 512, 512, 256, 128
+    threads per block, then one is kept; the cell states and tallies are
+    those of a fixed block size, step by step."""
+    spec = C2(13)
+    spec.steps = 8
+    fixed, tuned, blocks = [], [], []
+    with GpuEngine(spec, capi.MIXED) as e:
+        e.set_tuning(block=256)
+        for _ in range(spec.steps):
+            e.step()
+            fixed.append((e.cells(), e.tallies()))
+    with GpuEngine(spec, capi.MIXED) as e:
+        e.set_tuning(block=0)
+        for _ in range(spec.steps):
+            e.step()
+            tuned.append((e.cells(), e.tallies()))
+            blocks.append(e.counters()["transport_block"])
+    assert blocks[:6] == [128, 256, 512, 512, 256, 128], blocks
+    assert blocks[6] == blocks[7] and blocks[6] in (128, 256, 512), blocks
+    for (ca, ta), (cb, tb) in zip(fixed, tuned):
+        for k in ("T", "e", "erad", "census"):
+            assert np.array_equal(ca[k], cb[k]), k
+        for k in ("dep", "rdep", "pint", "esc", "segs"):
+            assert np.array_equal(ta[k], tb[k]), k
 
 
 @pytest.mark.parametrize("schedule", [capi.SCHEDULE_PASS, capi.SCHEDULE_REFILL], ids=["pass", "refill"])

@@ -46,7 +46,14 @@ def thick_slab():
 
 # storm::CellAccel<float> (storm_amd/csrc/common/transport.h)
 CELL_ACCEL_F32 = np.dtype([("sig_t", "<f4"), ("total", "<f4"), ("rate", "<f4", 6), ("flags", "<u4"), ("gcut", "<u4"),
-                           ("sig_d", "<f4"), ("sig_abs", "<f4"), ("up_rate", "<f4"), ("e_cut", "<f4")])
+                           ("sig_d", "<f4"), ("sig_abs", "<f4"), ("up_rate", "<f4"), ("e_cut", "<f4"),
+                           ("ddmc_sig_t", "<f4"), ("ddmc_abs", "<f4")])
+
+
+def test_cell_accel_record_layout(oracle):
+    """The record above is the engine's: a stale layout would misread every
+    cell after the first."""
+    assert CELL_ACCEL_F32.itemsize == oracle.rlib().replay_cell_accel_bytes()
 
 
 def accel_bytes(n, sig_t):
@@ -55,6 +62,7 @@ def accel_bytes(n, sig_t):
     rec["sig_d"] = sig_t  # gray walk: D = c / (3 sigma_t)
     rec["e_cut"] = np.inf
     rec["flags"] = 1  # random-walk cell
+    rec["ddmc_sig_t"] = sig_t
     return rec.tobytes()
 
 
