@@ -99,7 +99,7 @@ typedef struct storm_imc_params {
   double ddmc_min_cell_optical_depth; /* default 15 */
   double ddmc_min_particle_optical_depth; /* default 5 */
   int32_t comb;                       /* 0: NoPopulationControl, 1: CombPopulationControl */
-  int32_t reserved;
+  int32_t ddmc_multigroup_pgrw;       /* ddmcUseMultigroupPGRW (default 0) */
   uint64_t comb_nmin;                 /* Nmin (default 20) */
   double comb_factor;                 /* totalParticlesFactor (default 2.0) */
 } storm_imc_params;
@@ -120,7 +120,10 @@ typedef struct storm_imc_params {
  * With imc.with_random_walk, packets in the optically thick low groups take
  * the partial-gray random walk (RadiationIMC.hpp:1008-1061, 1065-1276).  With
  * imc.with_ddmc, DDMC runs on the Planck-mean opacity and keeps the photon
- * energy (the reference without ddmcUseMultigroupPGRW, :1343-1350).
+ * energy (the reference without ddmcUseMultigroupPGRW, :1343-1350); with
+ * imc.ddmc_multigroup_pgrw as well, on the diffusive groups' averages, and
+ * only packets below their cutoff energy take DDMC steps (:1301-1342,
+ * :1437-1450).
  */
 typedef struct storm_group_desc {
   int32_t num_groups;

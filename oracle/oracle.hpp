@@ -938,6 +938,7 @@ struct IMCParams {  // radiation/RadiationIMCParameters.hpp:12-56 (gray subset)
   bool withDDMC = false;
   double ddmcMinCellOpticalDepth = 15.0;
   double ddmcMinParticleOpticalDepth = 5.0;
+  bool ddmcUseMultigroupPGRW = false;
   bool noHydroFeedback = false;
 };
 
@@ -962,6 +963,7 @@ class RadiationIMC {
   struct DDMCCellData {
     bool eligible = false;
     double sigmaA = 0.0, sigmaT = 0.0, diffusionCoefficient = 0.0, gamma = 1.0, totalLeakRate = 0.0;
+    size_t groupCutoff = 0;
     std::vector<DDMCFaceLeak> faceLeaks;
   };
 
@@ -1413,6 +1415,40 @@ class RadiationIMC {
       double V = grid_.GetVolume(i), S = surfaceArea(i);
       if (V <= 0.0 || S <= 0.0) continue;
       double chord = 4.0 * V / S;
+      if (params_.ddmcUseMultigroupPGRW && mat_.multigroup()) {
+        // (:1301-1342) the diffusive groups from the lowest up, equal
+        // weights 1/G; D the harmonic mean over them
+        const double kT = units::k_boltz * cells_[i].temperature;
+        if (kT <= 0.0) continue;
+        const size_t G = mat_.groups();
+        double sB = 0.0, sBA = 0.0, sBT = 0.0, sBoT = 0.0;
+        size_t cutoff = 0;
+        bool nonDiffusive = false;
+        for (size_t g = 0; g < G; ++g) {
+          const double sA = mat_.CalcAbsorptionOpacity(i, cells_[i].temperature,
+                                                       0.5 * (mat_.bounds[g] + mat_.bounds[g + 1]));
+          const double sTg = sA + ss;
+          const double Bg = 1.0 / static_cast<double>(G);
+          if (!nonDiffusive && sTg * chord >= params_.ddmcMinCellOpticalDepth) {
+            cutoff = g + 1;
+            sB += Bg;
+            sBA += Bg * sA;
+            sBT += Bg * sTg;
+            if (sTg > 0.0) sBoT += Bg / sTg;
+          } else {
+            nonDiffusive = true;
+          }
+        }
+        if (cutoff > 0 && sB > 0.0) {
+          d.groupCutoff = cutoff;
+          d.sigmaA = sBA / sB;
+          d.sigmaT = sBT / sB;
+          d.diffusionCoefficient = (units::clight / 3.0) * sBoT / sB;
+          d.gamma = 1.0;
+          d.eligible = (d.sigmaT > 0.0 && d.diffusionCoefficient > 0.0);
+        }
+        continue;
+      }
       d.sigmaA = planck_[i];
       d.sigmaT = d.sigmaA + ss;
       d.diffusionCoefficient = (d.sigmaT > 0.0) ? units::clight / (3.0 * d.sigmaT) : 0.0;
@@ -1462,7 +1498,15 @@ class RadiationIMC {
     if (!d.eligible || d.totalLeakRate <= 0.0 || d.faceLeaks.empty()) return false;
     double Ro = minDistanceToFaces(c, p.location);
     if (Ro * d.sigmaT < params_.ddmcMinParticleOpticalDepth) return false;
+    if (params_.ddmcUseMultigroupPGRW && mat_.multigroup()) {
+      // (:1437-1450) only packets below the cutoff group
+      if (d.groupCutoff == 0 || d.groupCutoff > mat_.groups()) return false;
+      const double coFreq = std::clamp(p.frequency, mat_.bounds.front(), mat_.bounds.back());
+      if (coFreq >= mat_.bounds[d.groupCutoff]) return false;
+    }
     double f = factorFleck_[c];
+    // gamma = 1 in both forms of the data, so the upscatter rate
+    // c (1 - f) sigma_A (1 - gamma) of :1455-1458 is zero
     double eventRate = d.totalLeakRate;
     double tEvent = -std::log(rng_.unitOpen()) / eventRate;
     double tCensus = p.timeLeft;

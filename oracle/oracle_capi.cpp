@@ -83,6 +83,7 @@ int oracle_run(const storm_problem *prob, int nsteps, double dt, const double *s
     ip.rwMinParticleOpticalDepth = prob->imc.rw_min_particle_optical_depth;
     ip.ddmcMinCellOpticalDepth = prob->imc.ddmc_min_cell_optical_depth;
     ip.ddmcMinParticleOpticalDepth = prob->imc.ddmc_min_particle_optical_depth;
+    ip.ddmcUseMultigroupPGRW = prob->imc.ddmc_multigroup_pgrw != 0;
     RadiationIMC imc(grid, &bc, cells, ext, mat, ip, prob->seed);
     std::unique_ptr<Comb> comb;
     if (prob->imc.comb) comb = std::make_unique<Comb>(grid, prob->imc.comb_nmin, prob->imc.comb_factor, prob->seed);

@@ -93,9 +93,15 @@ AccelView<Real> make_accel_view(const CellAccel<Real> *cells, const HostTables<R
 // The engine's DDMC cell data (k_accel_cells, k_accel_rates): total opacity
 // st (gray: sigma_a + sigma_s; multigroup: the Planck mean sigma_P +
 // sigma_s) and continuous absorption fabs per cell.
+// With `pg` (ddmc_multigroup_pgrw) the per-cell D / c, eligibility and
+// energy cutoff come from the partial-gray sums instead.
+struct PartialGrayDDMC {
+  std::vector<double> Dt, ecut;
+  std::vector<char> elig;
+};
 template <typename Real>
 void add_ddmc(const storm_problem *p, std::vector<CellAccel<Real>> &out, const std::vector<double> &st,
-              const std::vector<double> &fabs) {
+              const std::vector<double> &fabs, const PartialGrayDDMC *pg = nullptr) {
   const int nx = p->mesh.nx, ny = p->mesh.ny, nz = p->mesh.nz;
   const uint32_t N = static_cast<uint32_t>(nx) * ny * nz;
   std::vector<double> Dt(N);
@@ -113,6 +119,13 @@ void add_ddmc(const storm_problem *p, std::vector<CellAccel<Real>> &out, const s
     for (int f = 0; f < 6; ++f) a.rate[f] = Real(0);
     a.ddmc_sig_t = static_cast<Real>(st[c]);
     a.ddmc_abs = static_cast<Real>(fabs[c]);
+    if (pg) {
+      a.ddmc_e_cut = static_cast<Real>(pg->ecut[c]);
+      Dt[c] = pg->Dt[c];
+      if (p->imc.with_ddmc && pg->elig[c]) a.flags |= 2u;
+      continue;
+    }
+    a.ddmc_e_cut = std::numeric_limits<Real>::infinity();
     Dt[c] = st[c] > 0.0 ? 1.0 / (3.0 * st[c]) : 0.0;
     if (p->imc.with_ddmc && st[c] * chord >= p->imc.ddmc_min_cell_optical_depth && Dt[c] > 0.0) a.flags |= 2u;
   }
@@ -188,6 +201,14 @@ std::vector<CellAccel<Real>> build_accel_mg(const storm_problem *p, const std::v
   const int32_t G = p->groups.num_groups;
   const double *gb = p->groups.bounds;
   std::vector<CellAccel<Real>> out(N);
+  const bool pgrw = p->imc.with_ddmc && p->imc.ddmc_multigroup_pgrw;
+  PartialGrayDDMC pg;
+  std::vector<double> qst(N, 0.0), qabs(N, 0.0);
+  if (pgrw) {
+    pg.Dt.assign(N, 0.0);
+    pg.ecut.assign(N, 0.0);
+    pg.elig.assign(N, 0);
+  }
   auto w = [&](int ax, int idx) {
     const double *e = ax == 0 ? p->mesh.x_edges : (ax == 1 ? p->mesh.y_edges : p->mesh.z_edges);
     return static_cast<double>(static_cast<Real>(e[idx + 1] - e[idx]));
@@ -201,6 +222,9 @@ std::vector<CellAccel<Real>> build_accel_mg(const storm_problem *p, const std::v
     double allAB = 0.0, dB = 0.0, dBA = 0.0, dBT = 0.0, dBoT = 0.0;
     uint32_t cut = 0;
     bool thin = false;
+    double qB = 0.0, qBA = 0.0, qBT = 0.0, qBoT = 0.0;
+    uint32_t qcut = 0;
+    bool qthin = !pgrw || !(p->T0[c] > 0.0);
     for (int32_t g = 0; g < G; ++g) {
       const double Ec = 0.5 * (gb[g] + gb[g + 1]);
       const double sg = sa[c] * std::pow(std::fmax(Ec, gb[0]) / p->groups.nu_ref, -p->groups.nu_pow);
@@ -216,6 +240,26 @@ std::vector<CellAccel<Real>> build_accel_mg(const storm_problem *p, const std::v
         if (st > 0.0) dBoT += Br / st;
       } else {
         thin = true;
+      }
+      if (!qthin && st * chord >= p->imc.ddmc_min_cell_optical_depth) {
+        const double Bq = 1.0 / static_cast<double>(G);
+        qcut = static_cast<uint32_t>(g) + 1u;
+        qB += Bq;
+        qBA += Bq * sg;
+        qBT += Bq * st;
+        if (st > 0.0) qBoT += Bq / st;
+      } else {
+        qthin = true;
+      }
+    }
+    if (pgrw) {
+      pg.ecut[c] = std::numeric_limits<double>::infinity();
+      if (qcut > 0u && qB > 0.0) {
+        qst[c] = qBT / qB;
+        qabs[c] = f[c] * (qBA / qB);
+        pg.Dt[c] = qBoT / (3.0 * qB);
+        pg.ecut[c] = static_cast<double>(static_cast<Real>(gb[qcut] / units::kev));
+        pg.elig[c] = (qst[c] > 0.0 && pg.Dt[c] > 0.0) ? 1 : 0;
       }
     }
     CellAccel<Real> &r = out[c];
@@ -238,7 +282,9 @@ std::vector<CellAccel<Real>> build_accel_mg(const storm_problem *p, const std::v
       r.e_cut = static_cast<Real>(gb[cut] / units::kev);
     }
   }
-  if (p->imc.with_ddmc) {
+  if (p->imc.with_ddmc && pgrw) {
+    add_ddmc<Real>(p, out, qst, qabs, &pg);  // partial-gray DDMC (k_prestep_mg)
+  } else if (p->imc.with_ddmc) {
     // DDMC on the Planck means (k_accel_cells after k_prestep_mg)
     std::vector<double> st(N), fabs(N);
     for (uint32_t c = 0; c < N; ++c) {

@@ -48,7 +48,7 @@ class ImcParams(C.Structure):
                 ("rw_min_particle_optical_depth", C.c_double),
                 ("ddmc_min_cell_optical_depth", C.c_double),
                 ("ddmc_min_particle_optical_depth", C.c_double),
-                ("comb", C.c_int32), ("reserved", C.c_int32),
+                ("comb", C.c_int32), ("ddmc_multigroup_pgrw", C.c_int32),
                 ("comb_nmin", C.c_uint64), ("comb_factor", C.c_double)]
 
 

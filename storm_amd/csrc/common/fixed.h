@@ -40,6 +40,18 @@ STORM_HD double quantize_nearest(double w, int e) {
   return rint(w * fpm::pow2d(-e)) * fpm::pow2d(e);
 }
 
+// v >= 0 rounded to a multiple of 2^e without bias: up with probability equal
+// to the fraction left over, taking u24 (24 uniform bits) as the draw.  A
+// path integral of a very opaque cell can be far below one quantum; rounding
+// it to nearest would drop it every time.
+STORM_HD double quantize_unbiased(double v, int e, uint32_t u24) {
+  const double y = v * fpm::pow2d(-e);
+  const double fl = floor(y);
+  const double frac = y - fl;  // exact
+  const double u = static_cast<double>(u24 & 0xFFFFFFu) * 0x1p-24;
+  return (u < frac ? fl + 1.0 : fl) * fpm::pow2d(e);
+}
+
 // Limbs of w / 2^e for a w that is an integer multiple of 2^e below 2^62.
 STORM_HD Limbs limbs_of(double w, int e) {
   double y = w * fpm::pow2d(-e);            // exact integer

@@ -121,7 +121,11 @@ struct CellXS {
 //   ddmc_sig_t  DDMC: sigma_P + sigma_s, the Planck-mean total opacity (the
 //           particle threshold's opacity; equal to sig_t in a gray problem)
 //   ddmc_abs    DDMC in a multigroup problem: f sigma_P, the continuous
-//           absorption during a DDMC step (a gray step uses the cell's own)
+//           absorption during a DDMC step (a gray step uses the cell's own);
+//           with ddmc_multigroup_pgrw f sigma_A of the diffusive groups
+//   ddmc_e_cut  DDMC with ddmc_multigroup_pgrw: lower bound (keV) of the
+//           first non-diffusive group; packets at or above it take no DDMC
+//           step (+inf otherwise)
 // (precomputeRandomWalkData, radiation/RadiationIMC.hpp:983-1063;
 // precomputeDDMCData without ddmcUseMultigroupPGRW, :1282-1351)
 template <typename Real>
@@ -137,6 +141,7 @@ struct CellAccel {
   Real e_cut;
   Real ddmc_sig_t;
   Real ddmc_abs;
+  Real ddmc_e_cut;
 };
 
 // Random-walk tables (see random_walk.h) and the particle thresholds.
@@ -334,7 +339,15 @@ struct Tracker {
     if constexpr (fixed_point) {
       fx::Limbs d = fx::sub(fx::limbs_of(w_enter, k.eq), fx::limbs_of(w_exit, k.eq));
       fx::Limbs e{0u, 0u};
-      if (!derive_pint) e = fx::limbs_of(fx::quantize_nearest(path_int, k.et), k.et);
+      if (!derive_pint) {
+        // the rounding draw: a hash of the packet, its stream position and
+        // the cell (no random-stream draw, so the history is unchanged)
+        uint32_t h = id * 0x9E3779B1u ^ (ctr + 0x7F4A7C15u) * 0x85EBCA77u ^ cell * 0xC2B2AE3Du;
+        h ^= h >> 15;
+        h *= 0x2C1B3C6Du;
+        h ^= h >> 12;
+        e = fx::limbs_of(fx::quantize_unbiased(path_int, k.et, h >> 8), k.et);
+      }
       sink.visit(cell, d, e, visit_segs);
     } else {
       sink.visit_fp(cell, w_enter - w_exit, path_int, visit_segs);
@@ -568,7 +581,8 @@ struct Tracker {
           if (rev == 1) census(k, sink);
           return outcome;
         }
-        if ((ca.flags & 2u) && ca.total > Real(0) && Ro * ca.ddmc_sig_t >= acc.ddmc_particle_tau) {
+        if ((ca.flags & 2u) && ca.total > Real(0) && Ro * ca.ddmc_sig_t >= acc.ddmc_particle_tau &&
+            ((F & kFeatMultigroup) == 0u || nu < ca.ddmc_e_cut)) {
           // ---------------- DDMC step (tryDDMCStep; in a multigroup problem
           // with the Planck-mean data and the photon energy kept, as the
           // reference does without ddmcUseMultigroupPGRW)

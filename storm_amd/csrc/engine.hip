@@ -649,7 +649,9 @@ struct Engine final : EngineBase {
           N, T.p, md, have_host_sa ? host_sa.p : nullptr, have_host_ss ? host_ss.p : nullptr,
           have_host_cv ? host_cv.p : nullptr, V.p, dt, groups, bounds_erg.p, nu_pow, nu_ref, xs.p, cdf.p,
           sig_s_cell.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p, d, meshview(),
-          imc.with_random_walk ? imc.rw_min_cell_optical_depth : HUGE_VAL, accel_on ? accel.p : nullptr);
+          imc.with_random_walk ? imc.rw_min_cell_optical_depth : HUGE_VAL, accel_on ? accel.p : nullptr,
+          (imc.with_ddmc && imc.ddmc_multigroup_pgrw) ? imc.ddmc_min_cell_optical_depth : -1.0,
+          accel_on ? Dtil.p : nullptr);
     } else {
       k_prestep<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
           N, T.p, md, have_host_sa ? host_sa.p : nullptr, have_host_ss ? host_ss.p : nullptr,
@@ -659,7 +661,8 @@ struct Engine final : EngineBase {
       const MeshView<Real> mv0 = meshview();
       k_accel_cells<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
           N, d, mv0, sig_a.p, sig_s.p, fleck.p, imc.with_random_walk, imc.rw_min_cell_optical_depth, imc.with_ddmc,
-          imc.ddmc_min_cell_optical_depth, groups >= 2 ? 1 : 0, Dtil.p, accel.p);
+          imc.ddmc_min_cell_optical_depth, groups >= 2 ? 1 : 0, imc.ddmc_multigroup_pgrw ? 1 : 0, Dtil.p,
+          accel.p);
       k_accel_rates<Real><<<blocks_for(N, 256), 256, 0, stream>>>(N, d, mv0, Dtil.p, accel.p);
     }
     sum_async(Eemit.p, N, dsc.p + 2);  // emission total, read by k_quantum and k_alloc

@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
     Real *__restrict__ cdf, Real *__restrict__ sig_s_cell, double *__restrict__ fleck,
     double *__restrict__ sig_a_out, double *__restrict__ sig_s_out, double *__restrict__ Eemit,
     uint32_t *__restrict__ err, MeshDims dims, MeshView<Real> mv, double rw_cell_tau,
-    CellAccel<Real> *__restrict__ rw_out) {
+    CellAccel<Real> *__restrict__ rw_out, double ddmc_pgrw_tau, double *__restrict__ Dtil) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
   const double Tc = T[c];
@@ -316,6 +316,11 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
   double allAB = 0.0, dB = 0.0, dBA = 0.0, dBT = 0.0, dBoT = 0.0;
   uint32_t cut = 0;
   bool thin = false;
+  // partial-gray DDMC (ddmcUseMultigroupPGRW, RadiationIMC.hpp:1301-1342):
+  // the diffusive groups from the lowest up with equal weights 1/G
+  double qB = 0.0, qBA = 0.0, qBT = 0.0, qBoT = 0.0;
+  uint32_t qcut = 0;
+  bool qthin = !(ddmc_pgrw_tau >= 0.0) || !(Tc > 0.0);
   my[0] = Real(0);
   for (int32_t g = 0; g < G; ++g) {
     const double a = bounds[g] / kT, b = bounds[g + 1] / kT;
@@ -339,6 +344,16 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
         if (st > 0.0) dBoT += Br / st;
       } else {
         thin = true;
+      }
+      if (!qthin && st * chord >= ddmc_pgrw_tau) {
+        const double Bq = 1.0 / static_cast<double>(G);
+        qcut = static_cast<uint32_t>(g) + 1u;
+        qB += Bq;
+        qBA += Bq * sg;
+        qBT += Bq * st;
+        if (st > 0.0) qBoT += Bq / st;
+      } else {
+        qthin = true;
       }
     }
   }
@@ -371,10 +386,21 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
     r.sig_t = r.sig_d = r.sig_abs = r.up_rate = Real(0);
     r.e_cut = static_cast<Real>(INFINITY);
     r.ddmc_sig_t = r.ddmc_abs = Real(0);
+    r.ddmc_e_cut = static_cast<Real>(INFINITY);
+    if (qcut > 0u && qB > 0.0) {
+      const double qT = qBT / qB, qD = qBoT / (3.0 * qB);
+      r.ddmc_sig_t = static_cast<Real>(qT);
+      r.ddmc_abs = static_cast<Real>(f * (qBA / qB));
+      r.ddmc_e_cut = static_cast<Real>(bounds[qcut] / units::kev);
+      Dtil[c] = qD;
+      if (qT > 0.0 && qD > 0.0) r.flags |= 4u;
+    } else if (ddmc_pgrw_tau >= 0.0) {
+      Dtil[c] = 0.0;
+    }
     if (cut > 0u && dB > 0.0) {
       const double sA = dBA / dB, sT = dBT / dB;
       const double gamma = allAB > 0.0 ? dBA / allAB : 1.0;
-      r.flags = 1u;
+      r.flags |= 1u;
       r.gcut = cut;
       r.sig_t = static_cast<Real>(sT);
       r.sig_d = static_cast<Real>(dBoT > 0.0 ? dB / dBoT : 0.0);
@@ -897,8 +923,8 @@ __global__ __launch_bounds__(256) void k_accel_cells(uint32_t N, MeshDims d, Mes
                                                      const double *__restrict__ sig_s,
                                                      const double *__restrict__ fleck, int32_t rw_on,
                                                      double rw_cell_tau, int32_t ddmc_on, double ddmc_cell_tau,
-                                                     int32_t walk_set, double *__restrict__ Dtil,
-                                                     CellAccel<Real> *__restrict__ out) {
+                                                     int32_t walk_set, int32_t ddmc_pgrw,
+                                                     double *__restrict__ Dtil, CellAccel<Real> *__restrict__ out) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
   const uint32_t ijk = ijk_of_flat(c, d);
@@ -908,8 +934,10 @@ __global__ __launch_bounds__(256) void k_accel_cells(uint32_t N, MeshDims d, Mes
   const double chord = 4.0 * V / S;
   const double st = sig_a[c] + sig_s[c];
   CellAccel<Real> a;
+  uint32_t pre = 0u;
   if (walk_set) {
     a = out[c];
+    pre = a.flags;
     a.flags &= 1u;
   } else {
     a.sig_t = static_cast<Real>(st);
@@ -923,8 +951,15 @@ __global__ __launch_bounds__(256) void k_accel_cells(uint32_t N, MeshDims d, Mes
   }
   a.total = Real(0);
   for (int f = 0; f < 6; ++f) a.rate[f] = Real(0);
+  if (walk_set && ddmc_pgrw) {
+    // partial-gray DDMC data and D / c from k_prestep_mg (bit 2: eligible)
+    if (ddmc_on && (pre & 4u)) a.flags |= 2u;
+    out[c] = a;
+    return;
+  }
   a.ddmc_sig_t = static_cast<Real>(st);
   a.ddmc_abs = static_cast<Real>(fleck[c] * sig_a[c]);
+  a.ddmc_e_cut = static_cast<Real>(INFINITY);
   const double Dt = st > 0.0 ? 1.0 / (3.0 * st) : 0.0;
   Dtil[c] = Dt;
   if (ddmc_on && st * chord >= ddmc_cell_tau && Dt > 0.0) a.flags |= 2u;

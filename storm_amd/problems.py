@@ -79,6 +79,7 @@ class ProblemSpec:
     comb_factor: float = 2.0
     with_rw: int = 0
     with_ddmc: int = 0
+    ddmc_pgrw: int = 0           # ddmcUseMultigroupPGRW (multigroup DDMC on the diffusive groups)
     rw_cell_tau: float = 25.0
     rw_particle_tau: float = 5.0
     ddmc_cell_tau: float = 15.0
@@ -150,6 +151,7 @@ class ProblemSpec:
         q.rw_min_cell_optical_depth, q.rw_min_particle_optical_depth = self.rw_cell_tau, self.rw_particle_tau
         q.ddmc_min_cell_optical_depth, q.ddmc_min_particle_optical_depth = self.ddmc_cell_tau, self.ddmc_particle_tau
         q.comb, q.comb_nmin, q.comb_factor = int(self.comb), int(self.comb_nmin), float(self.comb_factor)
+        q.ddmc_multigroup_pgrw = int(self.ddmc_pgrw)
         p.imc = q
         p.T0 = _ptr(t0)
         p.seed = int(self.seed)

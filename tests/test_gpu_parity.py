@@ -93,6 +93,14 @@ def MG_DDMC(s):
     return spec
 
 
+def MG_DDMC_PG(s):
+    """Partial-gray DDMC (ddmcUseMultigroupPGRW): data from the diffusive
+    groups, packets above the cutoff take analytic steps."""
+    spec = MG_DDMC(s)
+    spec.ddmc_pgrw = 1
+    return spec
+
+
 @pytest.fixture(scope="module")
 def c1_ref(oracle):
     return oracle_ensemble(oracle, C1, range(1000, 1000 + NSEEDS))
@@ -135,6 +143,11 @@ def mg_ddmc_ref(oracle):
     return oracle_ensemble(oracle, MG_DDMC, range(1000, 1000 + NSEEDS))
 
 
+@pytest.fixture(scope="module")
+def mg_ddmc_pg_ref(oracle):
+    return oracle_ensemble(oracle, MG_DDMC_PG, range(1000, 1000 + NSEEDS))
+
+
 @pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
 def test_multigroup_matches_oracle(mg_ref, precision):
     """30-group transport with energy-dependent opacity and re-emission."""
@@ -155,6 +168,17 @@ def test_multigroup_ddmc_matches_oracle(mg_ddmc_ref, precision):
     got = gpu_ensemble(MG_DDMC, range(NSEEDS), precision)
     _assert_parity(mg_ddmc_ref, got, f"mg_ddmc/{precision}")
     with GpuEngine(MG_DDMC(0), precision) as e:
+        e.step()
+        c = e.counters()
+    assert c["ddmc_steps"] > 0, c
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
+def test_multigroup_partial_gray_ddmc_matches_oracle(mg_ddmc_pg_ref, precision):
+    """Partial-gray DDMC (RadiationIMC.hpp:1301-1342, 1437-1450)."""
+    got = gpu_ensemble(MG_DDMC_PG, range(NSEEDS), precision)
+    _assert_parity(mg_ddmc_pg_ref, got, f"mg_ddmc_pg/{precision}")
+    with GpuEngine(MG_DDMC_PG(0), precision) as e:
         e.step()
         c = e.counters()
     assert c["ddmc_steps"] > 0, c
@@ -207,8 +231,9 @@ def test_fp64_energy_balance():
             assert L["census_pre"] == L["census_out"] + L["comb_rem"]
 
 
-@pytest.mark.parametrize("make", [C1, C2, C3, MG, MG_RW, MG_DDMC],
-                         ids=["c1", "c2", "c3", "multigroup", "multigroup_rw", "multigroup_ddmc"])
+@pytest.mark.parametrize("make", [C1, C2, C3, MG, MG_RW, MG_DDMC, MG_DDMC_PG],
+                         ids=["c1", "c2", "c3", "multigroup", "multigroup_rw", "multigroup_ddmc",
+                              "multigroup_ddmc_pg"])
 def test_replay_reproduces_tallies_bit_exactly(oracle, make):
     spec = make(5)
     with GpuEngine(spec, capi.MIXED) as e:

@@ -190,6 +190,13 @@ def _densmore_small_ddmc(s):
     return spec
 
 
+def _densmore_small_ddmc_pg(s):
+    # partial-gray DDMC (ddmcUseMultigroupPGRW)
+    spec = _densmore_small_ddmc(s)
+    spec.ddmc_pgrw = 1
+    return spec
+
+
 def _two_sides(s):
     # C1 slab with TwoSidesTemperature: 1 keV on x-, 0.5 keV on x+, both x faces escape
     spec = P.c1_slab(seed=s, packets_scale=0.05, steps=1)
@@ -199,7 +206,8 @@ def _two_sides(s):
 
 
 @pytest.mark.parametrize("precision", [0, 1], ids=["mixed", "fp64"])
-@pytest.mark.parametrize("which", ["c1", "c2", "multigroup", "multigroup_rw", "multigroup_ddmc", "two_sides"])
+@pytest.mark.parametrize("which", ["c1", "c2", "multigroup", "multigroup_rw", "multigroup_ddmc", "multigroup_ddmc_pg",
+                                   "two_sides"])
 def test_engine_first_step_physics_on_cpu(oracle, precision, which):
     """The engine's shared sourcing + transport code, run on the CPU, agrees
     with the oracle's first step (per-cell Erad time average and segments)."""
@@ -211,6 +219,8 @@ def test_engine_first_step_physics_on_cpu(oracle, precision, which):
         make = _densmore_small_rw
     elif which == "multigroup_ddmc":
         make = _densmore_small_ddmc
+    elif which == "multigroup_ddmc_pg":
+        make = _densmore_small_ddmc_pg
     elif which == "two_sides":
         make = _two_sides
     else:

@@ -47,7 +47,7 @@ def thick_slab():
 # storm::CellAccel<float> (storm_amd/csrc/common/transport.h)
 CELL_ACCEL_F32 = np.dtype([("sig_t", "<f4"), ("total", "<f4"), ("rate", "<f4", 6), ("flags", "<u4"), ("gcut", "<u4"),
                            ("sig_d", "<f4"), ("sig_abs", "<f4"), ("up_rate", "<f4"), ("e_cut", "<f4"),
-                           ("ddmc_sig_t", "<f4"), ("ddmc_abs", "<f4")])
+                           ("ddmc_sig_t", "<f4"), ("ddmc_abs", "<f4"), ("ddmc_e_cut", "<f4")])
 
 
 def test_cell_accel_record_layout(oracle):
@@ -63,6 +63,7 @@ def accel_bytes(n, sig_t):
     rec["e_cut"] = np.inf
     rec["flags"] = 1  # random-walk cell
     rec["ddmc_sig_t"] = sig_t
+    rec["ddmc_e_cut"] = np.inf
     return rec.tobytes()
 
 
