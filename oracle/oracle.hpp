@@ -86,9 +86,12 @@ inline Vec3 normalize(const Vec3 &v) {
 }
 
 // ------------------------------------------------------------------ grid
-// Cartesian mesh with the Grid methods the hot path calls (SURVEY Appendix A
-// lists them; evidence lines are cited there).  Outside "mirror" points have
-// indices N + face, GetMeshPoint() reflects the inside centre across the face.
+// Cartesian mesh with the Grid methods the hot path calls (GetPointNo,
+// GetCellFaces, GetFaceNeighbors, Normal, FaceCM, GetArea, GetVolume,
+// GetMeshPoint, IsPointOutsideBox; MonteCarloManagerSerial.hpp:134-135,
+// MonteCarloPhysics.hpp:75-97, RadiationIMC.hpp:1001, 1368-1378).  Outside
+// "mirror" points have indices N + face, GetMeshPoint() reflects the inside
+// centre across the face.
 class CartesianGrid {
  public:
   struct Face {
