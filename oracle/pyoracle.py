@@ -128,6 +128,8 @@ def rlib():
         fp = C.POINTER(C.c_float)
         L.replay_last_error.restype = C.c_char_p
         L.replay_cell_accel_bytes.restype = C.c_uint64
+        L.replay_quantize_unbiased.restype = C.c_double
+        L.replay_quantize_unbiased.argtypes = [C.c_double, C.c_int32, C.c_uint32]
         L.replay_transport_f32.argtypes = [C.c_int64, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                            C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_float),
                                            C.c_void_p, dp, C.POINTER(C.c_int64), C.c_int32, C.c_int32, C.c_double,

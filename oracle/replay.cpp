@@ -305,6 +305,9 @@ const char *replay_last_error(void) { return g_err.c_str(); }
 // points and storm_gpu_debug_accel use).
 uint64_t replay_cell_accel_bytes(void) { return sizeof(CellAccel<float>); }
 
+// The engine's unbiased rounding of path integrals (fx::quantize_unbiased).
+double replay_quantize_unbiased(double v, int32_t e, uint32_t u24) { return fx::quantize_unbiased(v, e, u24); }
+
 // Philox4x32-10 block (for known-answer tests).
 void replay_philox(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4) {
   Philox4 r = philox4x32_10(ctr4[0], ctr4[1], ctr4[2], ctr4[3], key2[0], key2[1]);

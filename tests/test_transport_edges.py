@@ -94,3 +94,18 @@ def test_packet_on_upper_face_of_thick_cell_reaches_census(oracle, sig_ev):
     assert int(outc[1]) == n, outc  # kCensus
     # off the face within a few flights, then a few random-walk steps
     assert segs < 200 * n, segs
+
+
+def test_path_integral_rounding_is_unbiased(oracle):
+    """fx::quantize_unbiased: a multiple of 2^e next to v, rounded up with
+    probability equal to the leftover fraction -- over the 2^24 equally spaced
+    draws the mean is v to within 2^-24 quanta, and exact multiples stay."""
+    q = oracle.rlib().replay_quantize_unbiased
+    e = -40
+    Q = 2.0 ** e
+    for v in (0.3 * Q, 1e-6 * Q, 17.75 * Q, 12345.0 * Q):
+        fl = math.floor(v / Q) * Q
+        outs = np.array([q(v, e, u) for u in range(0, 1 << 24, 1 << 12)])
+        assert set(np.unique(outs)) <= {fl, fl + Q}
+        assert abs(outs.mean() - v) <= Q / 4096 + 1e-300
+    assert q(5 * Q, e, 0) == 5 * Q and q(5 * Q, e, (1 << 24) - 1) == 5 * Q
