@@ -168,7 +168,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": "k_transport<float, 0u>" if args.schedule == "pass" else "k_transport_refill<float, 0u>",
+            "kernel": "k_transport_w6<float>" if args.schedule == "pass" else "k_transport_refill<float, 0u>",
             "bytes_per_segment": B_SEG_MIXED,
         },
     }

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <initializer_list>
+#include <type_traits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -534,11 +535,18 @@ struct Engine final : EngineBase {
     uint32_t &r = pass_slots_cache[F][b == 64 ? 0 : (b == 128 ? 1 : (b == 256 ? 2 : 3))];
     if (!r) {
       int per_cu = 0;
-      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<Real, F>, b, 0));
+      if constexpr (kUseW6<F>)
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport_w6<Real>, b, 0));
+      else
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<Real, F>, b, 0));
       r = cache_slots(static_cast<uint32_t>(b) * kChunksPerWave, nf, static_cast<uint32_t>(std::max(per_cu, 1)));
     }
     return r;
   }
+
+  // The FP32 gray transport runs the 6-waves-per-SIMD build (k_transport_w6).
+  template <uint32_t F>
+  static constexpr bool kUseW6 = std::is_same<Real, float>::value && F == 0u;
 
   // One transport launch on the kernel variant compiled for this problem's
   // features (random walk / DDMC, multigroup).
@@ -555,9 +563,14 @@ struct Engine final : EngineBase {
       const uint32_t nf = (precision == STORM_GPU_MIXED && groups < 2) ? 1u : 2u;
       const uint32_t slots = pass_cache_slots<F>(block, nf);
       const uint32_t chunks = kChunksPerWave * static_cast<uint32_t>(block / 64);
-      k_transport<Real, F><<<blocks_for(n_in, 64u * chunks), block, cache_bytes(slots, nf), stream>>>(
-          static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
-          xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
+      if constexpr (kUseW6<F>)
+        k_transport_w6<Real><<<blocks_for(n_in, 64u * chunks), block, cache_bytes(slots, nf), stream>>>(
+            static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
+            xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
+      else
+        k_transport<Real, F><<<blocks_for(n_in, 64u * chunks), block, cache_bytes(slots, nf), stream>>>(
+            static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
+            xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
     }
   }
   void launch_transport(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshView<Real> &mv,

@@ -602,7 +602,7 @@ __device__ __forceinline__ uint32_t spread(uint32_t q, uint32_t nw) { return (q 
 // engine (deposit and path integral), 2 in the FP64 engine (deposit and
 // path integral as doubles).  The host sizes it to what a block's share of
 // the CU's 160 KB of LDS holds (cache_slots).
-constexpr uint32_t kCacheProbes = 8;
+constexpr uint32_t kCacheProbes = 4;
 inline __host__ __device__ uint32_t cache_bytes(uint32_t slots, uint32_t nf) {
   return slots * (8u * nf + 8u) + 16u;
 }
@@ -770,14 +770,13 @@ __device__ __forceinline__ void wave_add_u32(uint32_t v, unsigned long long *dst
 // region and its LDS tally table serves all of it; each cell it touched is
 // written to device memory once, when the block's last wavefront ends.
 // stats: [0] escapes, [1] random-walk steps, [2] DDMC steps, [4] longest history
+#define STORM_TRANSPORT_ARGS                                                                                  \
+  uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out, uint32_t *__restrict__ n_out, CensusSoA<Real> cen,  \
+      uint32_t *__restrict__ n_cen, MeshView<Real> mesh, const CellXS<Real> *__restrict__ xs,                 \
+      AccelView<Real> acc, StepConsts k, int32_t max_segments, Tallies t,                                     \
+      unsigned long long *__restrict__ stats, uint32_t cache_slots, uint32_t cache_fields, uint32_t chunks
 template <typename Real, uint32_t F>
-__global__ __launch_bounds__(512) void k_transport(uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out,
-                                                   uint32_t *__restrict__ n_out, CensusSoA<Real> cen,
-                                                   uint32_t *__restrict__ n_cen, MeshView<Real> mesh,
-                                                   const CellXS<Real> *__restrict__ xs, AccelView<Real> acc,
-                                                   StepConsts k, int32_t max_segments, Tallies t,
-                                                   unsigned long long *__restrict__ stats, uint32_t cache_slots,
-                                                   uint32_t cache_fields, uint32_t chunks) {
+__device__ __forceinline__ void transport_chunks(STORM_TRANSPORT_ARGS) {
   extern __shared__ __align__(16) unsigned char lds[];
   const TallyCache cache = cache_init(lds, cache_slots, cache_fields);
   __syncthreads();
@@ -817,6 +816,22 @@ __global__ __launch_bounds__(512) void k_transport(uint32_t n_in, PacketSoA<Real
     wave_add_u32(sink.ddmc, &stats[2]);
   }
 }
+
+template <typename Real, uint32_t F>
+__global__ __launch_bounds__(512) void k_transport(STORM_TRANSPORT_ARGS) {
+  transport_chunks<Real, F>(n_in, in, out, n_out, cen, n_cen, mesh, xs, acc, k, max_segments, t, stats, cache_slots,
+                            cache_fields, chunks);
+}
+// The FP32 gray kernel compiled for 6 waves per SIMD (80 VGPRs, a few bytes
+// of spill): measured 5% faster than at 5 (91 VGPRs); the variants with the
+// random walk or multigroup spill more and keep k_transport's allocation.
+template <typename Real>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_transport_w6(
+    STORM_TRANSPORT_ARGS) {
+  transport_chunks<Real, 0u>(n_in, in, out, n_out, cen, n_cen, mesh, xs, acc, k, max_segments, t, stats,
+                             cache_slots, cache_fields, chunks);
+}
+#undef STORM_TRANSPORT_ARGS
 
 // Persistent transport with lane refill.  Each wavefront advances its 64
 // packets one segment at a time; a lane whose packet ended (census, absorbed,
