@@ -602,7 +602,7 @@ __device__ __forceinline__ uint32_t spread(uint32_t q, uint32_t nw) { return (q 
 // engine (deposit and path integral), 2 in the FP64 engine (deposit and
 // path integral as doubles).  The host sizes it to what a block's share of
 // the CU's 160 KB of LDS holds (cache_slots).
-constexpr uint32_t kCacheProbes = 4;
+constexpr uint32_t kCacheProbes = 2;
 inline __host__ __device__ uint32_t cache_bytes(uint32_t slots, uint32_t nf) {
   return slots * (8u * nf + 8u) + 16u;
 }
