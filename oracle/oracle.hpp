@@ -956,6 +956,21 @@ struct EnergyLedger {  // diagnostics for the per-step balance (not in the refer
   double materialBefore = 0, materialAfter = 0;
 };
 
+// Photons per cell of generateParticles (radiation/RadiationIMC.hpp:2076-2087):
+// a share of N * n * 10 packets proportional to the cell's emission energy,
+// clamped to [n, 20 n], and none where the energy is 0 (n everywhere when the
+// total is 0).
+inline std::vector<size_t> photonAllocation(const std::vector<double> &E, double total, size_t n) {
+  const size_t N = E.size();
+  const size_t totalParticles = N * n * 10;
+  std::vector<size_t> out(N, 0);
+  for (size_t i = 0; i < N; ++i) {
+    size_t share = (total > 0) ? static_cast<size_t>(E[i] / total * totalParticles) : n;
+    out[i] = E[i] > 0.0 ? std::max(n, std::min(share, n * 20)) : 0;
+  }
+  return out;
+}
+
 class RadiationIMC {
  public:
   struct DDMCFaceLeak {
@@ -1008,6 +1023,8 @@ class RadiationIMC {
   const RandomWalk *randomWalk() const { return rw_.get(); }
   const std::vector<DDMCCellData> &ddmcData() const { return ddmc_; }
   const std::vector<char> &rwEligible() const { return rwEligible_; }
+  // one DDMC attempt on a packet (tryDDMCStep), for sampling its laws
+  bool ddmcStep(Particle &p, StepResult &fn) { return tryDDMCStep(p, fn); }
 
   // preStep (radiation/RadiationIMC.hpp:1618-1694), gray, no hydro
   std::vector<Particle> preStep(double fullDt) {
@@ -1058,12 +1075,7 @@ class RadiationIMC {
       Eemit_[i] = factorFleck_[i] * grid_.GetVolume(i) * units::arad * (T * T * T * T) * planck_[i] * fullDt * units::clight;
       total += Eemit_[i];
     }
-    size_t totalParticles = N * params_.newPhotonsPerCell * 10;
-    nPhotons_.assign(N, 0);
-    for (size_t i = 0; i < N; ++i) {
-      size_t share = (total > 0) ? static_cast<size_t>(Eemit_[i] / total * totalParticles) : params_.newPhotonsPerCell;
-      nPhotons_[i] = Eemit_[i] > 0.0 ? std::max(params_.newPhotonsPerCell, std::min(share, params_.newPhotonsPerCell * 20)) : 0;
-    }
+    nPhotons_ = photonAllocation(Eemit_, total, params_.newPhotonsPerCell);
     for (size_t i = 0; i < N; ++i) {
       size_t n = nPhotons_[i];
       if (n == 0) continue;

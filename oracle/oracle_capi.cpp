@@ -2,6 +2,7 @@
 // cpu_baseline).  TEST INFRASTRUCTURE ONLY -- never linked into the product.
 #include <chrono>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "oracle.hpp"
@@ -162,6 +163,117 @@ int oracle_prestep(const storm_problem *prob, double dt, double *sigma_out, doub
       if (fleck_out) fleck_out[i] = imc.factorFleck()[i];
       if (emit_out) emit_out[i] = imc.emissionEnergy()[i];
       if (nphot_out) nphot_out[i] = static_cast<double>(imc.photonsPerCell()[i]);
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Photon allocation of generateParticles for a given emission-energy vector.
+int oracle_alloc(int64_t n, const double *E, uint64_t new_photons, int64_t *out) {
+  try {
+    std::vector<double> e(E, E + n);
+    double total = 0.0;
+    for (double v : e) total += v;
+    std::vector<size_t> c = photonAllocation(e, total, static_cast<size_t>(new_photons));
+    for (int64_t i = 0; i < n; ++i) out[i] = static_cast<int64_t>(c[static_cast<size_t>(i)]);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+namespace {
+struct OracleState {
+  CartesianGrid grid;
+  Material mat;
+  std::vector<Cell> cells;
+  std::vector<Extensives> ext;
+  std::unique_ptr<RadiationIMC> imc;
+  OracleState(const storm_problem *prob, double dt)
+      : grid(makeGrid(prob->mesh)), mat(makeMaterial(prob->material, grid.GetPointNo(), &prob->groups)) {
+    const size_t N = grid.GetPointNo();
+    cells.resize(N);
+    ext.resize(N);
+    for (size_t i = 0; i < N; ++i) {
+      double rho = mat.density(i), V = grid.GetVolume(i);
+      cells[i].temperature = prob->T0[i];
+      cells[i].internalEnergy = mat.eFromT(prob->T0[i], rho) * V;
+      ext[i].mass = rho * V;
+      ext[i].internal_energy = cells[i].internalEnergy;
+    }
+    IMCParams ip;
+    ip.newPhotonsPerCell = prob->imc.new_photons_per_cell;
+    ip.withRandomWalk = false;
+    ip.withDDMC = true;
+    ip.ddmcMinCellOpticalDepth = prob->imc.ddmc_min_cell_optical_depth;
+    ip.ddmcMinParticleOpticalDepth = prob->imc.ddmc_min_particle_optical_depth;
+    ip.ddmcUseMultigroupPGRW = prob->imc.ddmc_multigroup_pgrw != 0;
+    imc = std::make_unique<RadiationIMC>(grid, nullptr, cells, ext, mat, ip, prob->seed);
+    imc->preStep(dt);
+  }
+};
+}  // namespace
+
+// DDMC cell data of preStep (precomputeDDMCData) from T0, DDMC on and the
+// random walk off: per cell eligible (0/1), sigma_A, sigma_T, D (cm^2/s),
+// total leak rate (1/s), the number of face leaks and up to 6 of them
+// (neighbour cell, rate in 1/s).
+int oracle_ddmc_data(const storm_problem *prob, double dt, double *eligible, double *sigma_a, double *sigma_t,
+                     double *D, double *total, int64_t *nleaks, int64_t *leak_next, double *leak_rate) {
+  try {
+    OracleState st(prob, dt);
+    const auto &dd = st.imc->ddmcData();
+    for (size_t i = 0; i < dd.size(); ++i) {
+      eligible[i] = dd[i].eligible ? 1.0 : 0.0;
+      sigma_a[i] = dd[i].sigmaA;
+      sigma_t[i] = dd[i].sigmaT;
+      D[i] = dd[i].diffusionCoefficient;
+      total[i] = dd[i].totalLeakRate;
+      nleaks[i] = static_cast<int64_t>(dd[i].faceLeaks.size());
+      for (size_t q = 0; q < 6; ++q) {
+        const bool has = q < dd[i].faceLeaks.size();
+        leak_next[6 * i + q] = has ? static_cast<int64_t>(dd[i].faceLeaks[q].nextCellIndex) : -1;
+        leak_rate[6 * i + q] = has ? dd[i].faceLeaks[q].rate : 0.0;
+      }
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// n independent DDMC attempts (tryDDMCStep) on a packet of weight 1 at
+// (x, y, z) in `cell` with `time_left` s: per attempt the outcome (-1: no
+// DDMC step, else the particle status), the next cell, the new direction
+// (3 per attempt), the time used and the weight after.
+int oracle_ddmc_samples(const storm_problem *prob, double dt, int64_t cell, double x, double y, double z,
+                        double time_left, int64_t n, int64_t *status, int64_t *next, double *dir, double *t_used,
+                        double *w_after) {
+  try {
+    OracleState st(prob, dt);
+    for (int64_t k = 0; k < n; ++k) {
+      Particle p;
+      p.cellIndex = static_cast<size_t>(cell);
+      p.location = Vec3(x, y, z);
+      p.velocity = Vec3(units::clight, 0.0, 0.0);
+      p.timeLeft = time_left;
+      p.weight = 1.0;
+      p.initialWeight = 1.0;
+      p.frequency = 0.0;
+      StepResult fn;
+      const bool took = st.imc->ddmcStep(p, fn);
+      status[k] = took ? static_cast<int64_t>(fn.change) : -1;
+      next[k] = took && fn.change == CELL_MOVE ? static_cast<int64_t>(fn.nextCellIndex) : -1;
+      dir[3 * k] = p.velocity.x / units::clight;
+      dir[3 * k + 1] = p.velocity.y / units::clight;
+      dir[3 * k + 2] = p.velocity.z / units::clight;
+      t_used[k] = time_left - p.timeLeft;
+      w_after[k] = p.weight;
     }
     return 0;
   } catch (const std::exception &e) {

@@ -38,6 +38,11 @@ def lib():
         L.oracle_rw_tables.argtypes = [dp, dp, dp]
         L.oracle_rw_sample.argtypes = [C.c_int, dp, dp, dp, dp]
         L.oracle_comb.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), dp, dp, dp]
+        i64p = C.POINTER(C.c_int64)
+        L.oracle_alloc.argtypes = [C.c_int64, dp, C.c_uint64, i64p]
+        L.oracle_ddmc_data.argtypes = [C.c_void_p, C.c_double, dp, dp, dp, dp, dp, i64p, i64p, dp]
+        L.oracle_ddmc_samples.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, C.c_int64, i64p, i64p, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -79,6 +84,54 @@ def prestep(spec, dt: float | None = None) -> dict:
                           _dp(out["emit"]), _dp(out["nphot"]))
     if rc != 0:
         raise RuntimeError("oracle: " + L.oracle_last_error().decode())
+    return out
+
+
+def _i64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def alloc(E, new_photons: int) -> np.ndarray:
+    """Photons per cell that generateParticles gives an emission-energy vector."""
+    L = lib()
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    out = np.empty(len(E), dtype=np.int64)
+    if L.oracle_alloc(len(E), _dp(E), int(new_photons), _i64(out)) != 0:
+        raise RuntimeError("oracle: " + L.oracle_last_error().decode())
+    return out
+
+
+def ddmc_data(spec, dt: float | None = None) -> dict:
+    """DDMC cell data of preStep from T0 (DDMC on, random walk off)."""
+    L = lib()
+    prob = spec.to_c()
+    n = spec.ncells
+    out = {k: np.empty(n) for k in ("eligible", "sigma_a", "sigma_t", "D", "total")}
+    out["nleaks"] = np.empty(n, dtype=np.int64)
+    out["next"] = np.empty(6 * n, dtype=np.int64)
+    out["rate"] = np.empty(6 * n)
+    rc = L.oracle_ddmc_data(C.byref(prob), spec.dt if dt is None else dt, _dp(out["eligible"]), _dp(out["sigma_a"]),
+                            _dp(out["sigma_t"]), _dp(out["D"]), _dp(out["total"]), _i64(out["nleaks"]),
+                            _i64(out["next"]), _dp(out["rate"]))
+    if rc != 0:
+        raise RuntimeError("oracle: " + L.oracle_last_error().decode())
+    out["next"] = out["next"].reshape(n, 6)
+    out["rate"] = out["rate"].reshape(n, 6)
+    return out
+
+
+def ddmc_samples(spec, cell: int, pos, time_left: float, n: int, dt: float | None = None) -> dict:
+    """n DDMC attempts on a weight-1 packet at `pos` in `cell` (tryDDMCStep)."""
+    L = lib()
+    prob = spec.to_c()
+    out = {"status": np.empty(n, dtype=np.int64), "next": np.empty(n, dtype=np.int64),
+           "dir": np.empty(3 * n), "t": np.empty(n), "w": np.empty(n)}
+    rc = L.oracle_ddmc_samples(C.byref(prob), spec.dt if dt is None else dt, int(cell), float(pos[0]), float(pos[1]),
+                               float(pos[2]), float(time_left), int(n), _i64(out["status"]), _i64(out["next"]),
+                               _dp(out["dir"]), _dp(out["t"]), _dp(out["w"]))
+    if rc != 0:
+        raise RuntimeError("oracle: " + L.oracle_last_error().decode())
+    out["dir"] = out["dir"].reshape(n, 3)
     return out
 
 
