@@ -3,7 +3,7 @@
 // and public methods as
 //   physics/MonteCarloPhysics.hpp:16-53            MonteCarloPhysics
 //   boundary/BoundaryCondition.hpp:30-51           BoundaryCondition
-//   boundary/SideTemperature.hpp:156-191           SideTemperature
+//   boundary/SideTemperature.hpp:20-167            SideTemperature
 //   boundary/TwoSidesTemperature.hpp:20-58         TwoSidesTemperature
 //   boundary/Vacuum.hpp:10-45                      VacuumBoundary
 //   boundary/RigidBoundary.hpp:9-24                RigidBoundary
@@ -72,7 +72,7 @@ class BoundaryCondition {
 
 // Planck surface source on the x- face; packets reaching x- leave, the other
 // faces reflect (boundary/SideTemperature.hpp:84-163).  As in the reference,
-// T^4 is taken at the first step (`static const double T4`, :252) unless
+// T^4 is taken at the first step (`static const double T4`, :117) unless
 // freezeT4 is false (the MarshakBoundary behaviour,
 // examples/marshak_wave/MarshakBoundary.hpp:150-156).
 template <typename T, typename Grid>
@@ -112,7 +112,7 @@ class SideTemperature : public BoundaryCondition<T, Grid> {
 
 // Planck sources on both x faces, which remove packets; the other faces
 // reflect.  T^4 of each side is taken at the first step
-// (`static const double T4_L/T4_R`, boundary/TwoSidesTemperature.hpp:158-159).
+// (`static const double T4_L/T4_R`, boundary/TwoSidesTemperature.hpp:128-129).
 template <typename T, typename Grid>
 class TwoSidesTemperature : public BoundaryCondition<T, Grid> {
  public:

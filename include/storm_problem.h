@@ -81,7 +81,7 @@ typedef struct storm_material_desc {
 typedef struct storm_boundary_desc {
   int32_t face_kind[6];             /* STORM_FACE_* per box face */
   int32_t source_face;              /* -1: none; STORM_XLO: Planck source entering along +x */
-  int32_t freeze_source_T4;         /* 1: T^4 taken once at the first step (SideTemperature.hpp:252) */
+  int32_t freeze_source_T4;         /* 1: T^4 taken once at the first step (SideTemperature.hpp:117) */
   double source_temperature;        /* K */
   uint64_t source_packets_per_face; /* Npercell of SideTemperature */
   int32_t source_xhi;               /* 1: a second source on x+ entering along -x, same Npercell

@@ -22,8 +22,8 @@
 // RNG: std::mt19937_64 + std::uniform_real_distribution<double> exactly like
 // the reference (radiation/RadiationIMC.hpp:519-520), but every stream is
 // seeded from the problem seed (the reference seeds the boundary, face and
-// comb streams with fixed constants, SideTemperature.hpp:254,
-// RandomOnFace.hpp:113, CombPopulationControl.hpp:94); results are compared
+// comb streams with fixed constants, SideTemperature.hpp:119,
+// RandomOnFace.hpp:17, CombPopulationControl.hpp:94); results are compared
 // statistically, never bit by bit.
 //
 // Parity status of each part is recorded in tests/test_oracle_*.py (what pins
@@ -560,7 +560,7 @@ class FaceTableBoundary {
   double escapedEnergy = 0.0;
   double sourceEnergy = 0.0;
 
-  // SetTemperature (boundary/SideTemperature.hpp:183); ignored once T^4 is frozen
+  // SetTemperature (boundary/SideTemperature.hpp:48); ignored once T^4 is frozen
   void setSourceTemperature(double T) { desc_.source_temperature = T; }
   // multigroup source: group boundaries in erg
   void setGroupBounds(const std::vector<double> &b) { bounds_ = b; }

@@ -57,9 +57,23 @@ const char *oracle_last_error(void) { return g_err.c_str(); }
 // last step (any may be NULL): T, internal energy, Erad time average,
 // census Erad (extensive), segment counts of the last step (as double), and a
 // per-step ledger (nsteps * ORACLE_LEDGER_STRIDE).
+int oracle_run_schedule(const storm_problem *prob, int nsteps, const double *dts, const double *source_T,
+                        double *T_out, double *e_out, double *erad_out, double *census_out,
+                        double *segments_out, double *ledger_out);
+
 int oracle_run(const storm_problem *prob, int nsteps, double dt, const double *source_T,
                double *T_out, double *e_out, double *erad_out, double *census_out,
                double *segments_out, double *ledger_out) {
+  std::vector<double> dts(static_cast<size_t>(nsteps > 0 ? nsteps : 0), dt);
+  return oracle_run_schedule(prob, nsteps, dts.data(), source_T, T_out, e_out, erad_out, census_out, segments_out,
+                             ledger_out);
+}
+
+// oracle_run with a time step per step (dts[s]), as the reference's example
+// drivers change dt between steps (examples/marshak_wave/MarshakCommon.hpp:343-357).
+int oracle_run_schedule(const storm_problem *prob, int nsteps, const double *dts, const double *source_T,
+                        double *T_out, double *e_out, double *erad_out, double *census_out,
+                        double *segments_out, double *ledger_out) {
   try {
     CartesianGrid grid = makeGrid(prob->mesh);
     size_t N = grid.GetPointNo();
@@ -99,7 +113,7 @@ int oracle_run(const storm_problem *prob, int nsteps, double dt, const double *s
       bc.escapedEnergy = 0;
       bc.sourceEnergy = 0;
       if (source_T) bc.setSourceTemperature(source_T[s]);
-      census = mgr.step(std::move(census), dt);
+      census = mgr.step(std::move(census), dts[s]);
       auto t1 = std::chrono::steady_clock::now();
       double censusOut = 0, censusComb = 0, matAfter = 0, segs = 0;
       for (const Particle &p : mgr.lastPreCombCensus()) censusOut += p.weight;

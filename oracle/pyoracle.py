@@ -34,6 +34,7 @@ def lib():
         dp = C.POINTER(C.c_double)
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_run.argtypes = [C.c_void_p, C.c_int, C.c_double, dp, dp, dp, dp, dp, dp, dp]
+        L.oracle_run_schedule.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
         L.oracle_prestep.argtypes = [C.c_void_p, C.c_double, dp, dp, dp, dp]
         L.oracle_rw_tables.argtypes = [dp, dp, dp]
         L.oracle_rw_sample.argtypes = [C.c_int, dp, dp, dp, dp]
@@ -68,6 +69,27 @@ def run(spec, steps: int | None = None, dt: float | None = None, source_T=None) 
     rc = L.oracle_run(C.byref(prob), steps, dt, _null() if sT is None else _dp(sT),
                       _dp(out["T"]), _dp(out["e"]), _dp(out["erad"]), _dp(out["census"]),
                       _dp(out["segments"]), _dp(ledger))
+    if rc != 0:
+        raise RuntimeError("oracle: " + L.oracle_last_error().decode())
+    led = ledger.reshape(steps, LEDGER_STRIDE)
+    out["ledger"] = {f: led[:, i].copy() for i, f in enumerate(LEDGER_FIELDS)}
+    return out
+
+
+def run_schedule(spec, dts, source_T) -> dict:
+    """Run the oracle with a time step and a boundary temperature per step;
+    per-cell results after the last step."""
+    L = lib()
+    prob = spec.to_c()
+    n = spec.ncells
+    dts = np.ascontiguousarray(dts, dtype=np.float64)
+    sT = np.ascontiguousarray(source_T, dtype=np.float64)
+    steps = len(dts)
+    assert len(sT) == steps
+    out = {k: np.empty(n) for k in ("T", "e", "erad", "census", "segments")}
+    ledger = np.empty(steps * LEDGER_STRIDE)
+    rc = L.oracle_run_schedule(C.byref(prob), steps, _dp(dts), _dp(sT), _dp(out["T"]), _dp(out["e"]),
+                               _dp(out["erad"]), _dp(out["census"]), _dp(out["segments"]), _dp(ledger))
     if rc != 0:
         raise RuntimeError("oracle: " + L.oracle_last_error().decode())
     led = ledger.reshape(steps, LEDGER_STRIDE)
