@@ -98,8 +98,9 @@ def test_packet_on_upper_face_of_thick_cell_reaches_census(oracle, sig_ev):
 
 def test_path_integral_rounding_is_unbiased(oracle):
     """fx::quantize_unbiased: a multiple of 2^e next to v, rounded up with
-    probability equal to the leftover fraction -- over the 2^24 equally spaced
-    draws the mean is v to within 2^-24 quanta, and exact multiples stay."""
+    probability equal to the leftover fraction -- over 4096 equally spaced
+    draws of the 24-bit rounding word the mean is v to within 1/4096 of a
+    quantum, and exact multiples stay."""
     q = oracle.rlib().replay_quantize_unbiased
     e = -40
     Q = 2.0 ** e
