@@ -19,6 +19,7 @@
 // once per cell per step on the host, never per segment) and the results.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstddef>
@@ -271,12 +272,26 @@ struct RadiationIMCParameters {
   bool noHydroFeedback = false;
   bool withEgTimeAvg = false;
   bool withCompton = false;
+  bool comptonUseInduced = true;
+  bool comptonAllowNZeroFallback = true;
+  bool comptonDebugParityCheck = false;
+  bool comptonCheckSignedTallies = false;
+  bool comptonDiagnostics = false;
+  bool comptonAngleDependent = true;
+  double comptonSignedTallyTolerance = 1e-10;
+  std::size_t comptonMatrixSamples = 200000;
   std::array<double, NumGroups + 1> energyBoundaries{};
   bool energyBoundariesProvided = false;
   struct PostProcessParameters {
     bool enabled = false;
+    double sourceDt = 0.0;
+    double transportTime = 0.0;
+    bool useCellVelocities = true;
     struct PolarizationParameters {
       bool enabled = false;
+      int manualScatteringsAfterAcceleration = 0;
+      double depolarizationScatterings = 0.0;
+      std::string acceleratedClosure;
     } polarization;
   } postProcess;
 };
@@ -297,6 +312,9 @@ struct SimpleExtensives {
 template <typename PointT, typename GridT, typename CellT, std::size_t NumGroups>
 class RadiationOpacityModel {
  public:
+  using Point = PointT;
+  using Grid = GridT;
+  using Cell = CellT;
   using GroupArray = std::array<double, NumGroups>;
   using GroupBoundaries = std::array<double, NumGroups + 1>;
   virtual ~RadiationOpacityModel() = default;
@@ -304,11 +322,111 @@ class RadiationOpacityModel {
   virtual double CalcAbsorptionOpacity(const CellT &cell, double) { return CalcPlanckOpacity(cell); }
   virtual double CalcScatteringOpacity(const CellT &) { return 0.0; }
   virtual double CalcScatteringOpacity(const CellT &cell, double) { return CalcScatteringOpacity(cell); }
+  // Emission and scattering directions.  The GPU samples both isotropically;
+  // RadiationIMC rejects at construction a model whose overrides behave
+  // otherwise (radiation/RadiationOpacityModel.hpp:50-68).
+  virtual PointT getRandomVelocity(const CellT &, std::mt19937_64 &rng, std::uniform_real_distribution<double> &dist) {
+    return isotropicVelocity(rng, dist);
+  }
+  virtual PointT getNewScatterVelocity(const CellT &cell, const PointT &, double, std::mt19937_64 &rng,
+                                       std::uniform_real_distribution<double> &dist) {
+    return getRandomVelocity(cell, rng, dist);
+  }
+  virtual bool ComptonIncludedInTransport() const { return false; }
   virtual void reseed(std::uint64_t) {}
   virtual std::size_t findGroup(double frequency, const GroupBoundaries &b) const {
     for (std::size_t g = 0; g < NumGroups; ++g)
       if (frequency < b[g + 1]) return g;
     return NumGroups - 1;
+  }
+  // Thermal photon energies: uniform over the whole range by default
+  // (:88-98), a group's share proportional to its width (:100-115), the
+  // cumulative opacity g+1 over G (:117-126), the group centres (:128-136).
+  virtual double GetThermalEnergy(const CellT &, double random, const GroupBoundaries &b) const {
+    return b[0] + random * (b[NumGroups] - b[0]);
+  }
+  virtual double SampleThermalEnergyInGroup(const CellT &, std::size_t group, double random,
+                                            const GroupBoundaries &b) const {
+    return b[group] + random * (b[group + 1] - b[group]);
+  }
+  virtual GroupArray GetThermalGroupPdf(const CellT &, const GroupBoundaries &b) const {
+    GroupArray pdf{};
+    const double span = b[NumGroups] - b[0];
+    if (!(span > 0.0)) return pdf;
+    for (std::size_t g = 0; g < NumGroups; ++g) pdf[g] = (b[g + 1] - b[g]) / span;
+    return pdf;
+  }
+  virtual GroupArray GetCumulativeOpacity(const CellT &, const GroupBoundaries &) const {
+    GroupArray c{};
+    for (std::size_t g = 0; g < NumGroups; ++g) c[g] = static_cast<double>(g + 1) / static_cast<double>(NumGroups);
+    return c;
+  }
+  virtual GroupArray getEnergyCenters(const GroupBoundaries &b) const {
+    GroupArray c{};
+    for (std::size_t g = 0; g < NumGroups; ++g) c[g] = 0.5 * (b[g] + b[g + 1]);
+    return c;
+  }
+
+ protected:
+  // phi = 2 pi U, mu = 2 U - 1, speed c (:139-150)
+  static PointT isotropicVelocity(std::mt19937_64 &rng, std::uniform_real_distribution<double> &dist) {
+    constexpr double kTwoPi = 6.283185307179586476925286766559;
+    constexpr double kC = 2.99792458e10;
+    const double phi = kTwoPi * dist(rng);
+    const double mu = 2.0 * dist(rng) - 1.0;
+    const double st = std::sqrt(std::max(0.0, 1.0 - mu * mu));
+    return PointT(st * std::cos(phi) * kC, st * std::sin(phi) * kC, mu * kC);
+  }
+};
+
+// Where a new packet is placed in its cell (utils/RandomInCell.hpp:10-67 and
+// radiation/RadiationIMC.hpp:317-329).  The GPU samples uniformly in the
+// cell itself, which is what RandomInCellPositionSampler does; these types
+// select that behaviour (RadiationIMC accepts no other sampler).
+template <typename PointT, typename GridT>
+struct RandomInCellPositionSampler {};
+template <typename PointT, typename GridT>
+struct CellCenterPositionSampler {};
+
+namespace detail {
+template <typename T, typename = void>
+struct has_tracers : std::false_type {};
+template <typename T>
+struct has_tracers<T, std::void_t<decltype(std::declval<const T &>().tracers)>> : std::true_type {};
+template <typename T, typename = void>
+struct has_static_tracer_names : std::false_type {};
+template <typename T>
+struct has_static_tracer_names<T, std::void_t<decltype(T::tracerNames)>> : std::true_type {};
+template <typename T, typename = void>
+struct has_member_tracer_names : std::false_type {};
+template <typename T>
+struct has_member_tracer_names<T, std::void_t<decltype(std::declval<const T &>().tracerNames)>> : std::true_type {};
+inline const std::vector<double> &no_tracers() {
+  static const std::vector<double> v;
+  return v;
+}
+inline const std::vector<std::string> &no_tracer_names() {
+  static const std::vector<std::string> v;
+  return v;
+}
+}  // namespace detail
+
+// The cell accessors RadiationIMC uses for the EOS calls
+// (radiation/RadiationIMCTraits.hpp:58-120): a cell's tracers and their
+// names if the cell type has them, empty vectors otherwise.
+template <typename PointT, typename CellT, typename ExtensivesT, std::size_t NumGroups>
+struct DirectRadiationIMCTraits {
+  using Point = PointT;
+  using Cell = CellT;
+  using Extensives = ExtensivesT;
+  const std::vector<double> &tracers(const CellT &cell) const {
+    if constexpr (detail::has_tracers<CellT>::value) return cell.tracers;
+    else return detail::no_tracers();
+  }
+  const std::vector<std::string> &tracerNames(const CellT &cell) const {
+    if constexpr (detail::has_static_tracer_names<CellT>::value) return CellT::tracerNames;
+    else if constexpr (detail::has_member_tracer_names<CellT>::value) return cell.tracerNames;
+    else return detail::no_tracer_names();
   }
 };
 
@@ -343,20 +461,32 @@ class GpuRadiationPhysics {
 };
 
 template <typename PointT, typename GridT, typename CellT, typename ExtensivesT, typename EOST,
-          std::size_t NumGroups>
+          std::size_t NumGroups,
+          typename TraitsT = DirectRadiationIMCTraits<PointT, CellT, ExtensivesT, NumGroups>,
+          typename PositionSamplerT = RandomInCellPositionSampler<PointT, GridT>>
 class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiationPhysics {
  public:
-  static_assert(NumGroups == 1, "the GPU engine is gray (NumGroups = 1)");
+  static_assert(NumGroups == 1, "the C++ drop-in manager runs gray problems (NumGroups = 1); multigroup "
+                                "problems go through the C ABI's storm_group_desc");
+  static_assert(std::is_same<PositionSamplerT, RandomInCellPositionSampler<PointT, GridT>>::value,
+                "the GPU places new packets uniformly in their cell (RandomInCellPositionSampler)");
   using Base = MonteCarloPhysics<PointT, GridT>;
   using MCParticle = Particle<PointT, GridT>;
   using Parameters = RadiationIMCParameters<NumGroups>;
   using OpacityModel = RadiationOpacityModel<PointT, GridT, CellT, NumGroups>;
+  using GroupArray = std::array<double, NumGroups>;
+  using GroupBoundaries = std::array<double, NumGroups + 1>;
+  using Traits = TraitsT;
+  using Sampler = PositionSamplerT;
 
+  // radiation/RadiationIMC.hpp:410-419, arguments in the same order
   RadiationIMC(const GridT &grid, const std::shared_ptr<BoundaryCondition<PointT, GridT>> &boundary,
                std::vector<CellT> &cells, std::vector<ExtensivesT> &extensives, std::shared_ptr<EOST> eos,
-               std::shared_ptr<OpacityModel> opacity, Parameters parameters, std::uint64_t seed = 42)
+               std::shared_ptr<OpacityModel> opacity, Parameters parameters, TraitsT traits = TraitsT(),
+               PositionSamplerT positionSampler = PositionSamplerT(), std::uint64_t seed = 42)
       : Base(grid, boundary), cells_(cells), extensives_(extensives), eos_(std::move(eos)), opacity_(std::move(opacity)),
-        parameters_(parameters), seed_(seed) {
+        parameters_(parameters), traits_(std::move(traits)), seed_(seed) {
+    (void)positionSampler;
     // constructor checks of radiation/RadiationIMC.hpp:585-602, 689-694
     if (parameters_.newPhotonsPerCell == 0) throw StormError("RadiationIMC requires newPhotonsPerCell > 0");
     if (!eos_) throw StormError("RadiationIMC requires a non-null EOS");
@@ -368,6 +498,7 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
     if (parameters_.postProcess.enabled) throw StormError("RadiationIMC option postProcess.enabled is not implemented");
     if (parameters_.withHydro || parameters_.MMC) throw StormError("RadiationIMC: hydro coupling is not supported on the GPU");
     if (parameters_.withMultigroupOpacity) throw StormError("RadiationIMC: the GPU engine is gray");
+    rejectDirectionOverrides();
     opacity_->reseed(seed + 1ULL);
     Erad_time_avg_.assign(grid.GetPointNo(), 0.0);
   }
@@ -444,14 +575,12 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
 
   void gpuTabulate(double *T, double *sa, double *ss, double *cv, double *e, double *mass) override {
     const std::size_t N = this->grid.GetPointNo();
-    const std::vector<double> tracers;
-    const std::vector<std::string> names;
     for (std::size_t i = 0; i < N; ++i) {
       const CellT &cell = cells_[i];
       T[i] = cell.temperature;
       sa[i] = opacity_->CalcPlanckOpacity(cell);
       ss[i] = opacity_->CalcScatteringOpacity(cell);
-      cv[i] = eos_->dT2cv(density(i), cell.temperature, tracers, names);
+      cv[i] = eos_->dT2cv(density(i), cell.temperature, traits_.tracers(cell), traits_.tracerNames(cell));
       e[i] = extensives_[i].internal_energy;
       mass[i] = extensives_[i].mass;
       if constexpr (detail::has_internal_energy_density<CellT>::value) {
@@ -464,8 +593,6 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
   void gpuAbsorb(const double *e, const double *eradAvg, const double *censusErad, const double *fleck,
                  const double *planck) override {
     const std::size_t N = this->grid.GetPointNo();
-    const std::vector<double> tracers;
-    const std::vector<std::string> names;
     Erad_time_avg_.assign(eradAvg, eradAvg + N);
     factorFleck_.assign(fleck, fleck + N);
     planck_.assign(planck, planck + N);
@@ -475,7 +602,7 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
       CellT &cell = cells_[i];
       const double specific = e[i] / extensives_[i].mass;
       if constexpr (detail::has_internal_energy_density<CellT>::value) cell.internalEnergy = e[i];
-      cell.temperature = eos_->de2T(density(i), specific, tracers, names);
+      cell.temperature = eos_->de2T(density(i), specific, traits_.tracers(cell), traits_.tracerNames(cell));
       // census radiation energy (RadiationIMC.hpp:1989-2028)
       if constexpr (detail::has_erad<ExtensivesT>::value) extensives_[i].Erad = censusErad[i];
       if constexpr (detail::has_erad<CellT>::value)
@@ -489,6 +616,41 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
 
  private:
   static constexpr const char *kGpuOnly = "RadiationIMC runs on the GPU: drive it with MonteCarloManagerGPU";
+
+  // The GPU draws emission and scattering directions isotropically, as the
+  // opacity model's defaults do (radiation/RadiationOpacityModel.hpp:50-68,
+  // 139-150).  A model that overrides getRandomVelocity or
+  // getNewScatterVelocity with another law, or that puts Compton scattering
+  // in the transport, cannot run here: each override is called beside the
+  // base class's own body (a qualified, non-virtual call) from equal random
+  // states on a few cells and directions, and any difference in the result
+  // or in the random numbers drawn is an error, not a silent change of
+  // physics.
+  void rejectDirectionOverrides() {
+    if (opacity_->ComptonIncludedInTransport())
+      throw StormError("RadiationIMC: opacity models with Compton scattering in the transport are not supported on the GPU");
+    const std::size_t n = std::min<std::size_t>(this->grid.GetPointNo(), 4);
+    auto same = [](const PointT &a, const PointT &b) { return a.x == b.x && a.y == b.y && a.z == b.z; };
+    for (std::size_t i = 0; i < n; ++i) {
+      for (std::uint64_t trial = 0; trial < 4; ++trial) {
+        std::mt19937_64 ra(0x5EEDull + 977 * i + trial), rb(0x5EEDull + 977 * i + trial);
+        std::uniform_real_distribution<double> da(0.0, 1.0), db(0.0, 1.0);
+        const PointT got = opacity_->getRandomVelocity(cells_[i], ra, da);
+        const PointT want = opacity_->OpacityModel::getRandomVelocity(cells_[i], rb, db);
+        if (!same(got, want) || ra() != rb())
+          throw StormError("RadiationIMC: the opacity model's getRandomVelocity is not isotropic; the GPU engine "
+                           "samples emission directions isotropically");
+        const PointT old(0.6 * 2.99792458e10, 0.0, 0.8 * 2.99792458e10);
+        const double nu = 1e-9 * static_cast<double>(trial + 1);
+        const PointT gs = opacity_->getNewScatterVelocity(cells_[i], old, nu, ra, da);
+        const PointT ws = opacity_->OpacityModel::getNewScatterVelocity(cells_[i], old, nu, rb, db);
+        if (!same(gs, ws) || ra() != rb())
+          throw StormError("RadiationIMC: the opacity model's getNewScatterVelocity is not the isotropic default; "
+                           "the GPU engine scatters isotropically");
+      }
+    }
+  }
+
   double density(std::size_t i) const {
     if constexpr (detail::has_density<CellT>::value) return cells_[i].density;
     else return extensives_[i].mass / this->grid.GetVolume(i);
@@ -499,6 +661,7 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
   std::shared_ptr<EOST> eos_;
   std::shared_ptr<OpacityModel> opacity_;
   Parameters parameters_;
+  TraitsT traits_;
   std::uint64_t seed_;
   std::vector<double> Erad_time_avg_, factorFleck_, planck_;
   std::size_t rw_ = 0, ddmc_ = 0;

@@ -100,8 +100,8 @@ int main(int argc, char **argv) {
       else
         boundary = std::make_shared<STORM::SideTemperature<Vec3, Grid>>(grid, kKevK, npc);
       auto physics = std::make_shared<IMC>(grid, boundary, cells, ext, std::make_shared<ConstantCvEOS>(),
-                                           std::make_shared<UnitOpacity>(), params,
-                                           static_cast<std::uint64_t>(seed0 + s));
+                                           std::make_shared<UnitOpacity>(), params, IMC::Traits(),
+                                           IMC::Sampler(), static_cast<std::uint64_t>(seed0 + s));
       auto comb = std::make_shared<STORM::CombPopulationControl<Vec3, Grid>>(grid, nmin, 2.0);
       STORM::MonteCarloManagerSerial<Vec3, Grid> manager(
           grid, physics, comb, boundary,
