@@ -10,7 +10,7 @@
  *   MonteCarloManagerSerial<T,Grid>      manager/MonteCarloManagerSerial.hpp:25-117
  *   RadiationIMC<...>                    radiation/RadiationIMC.hpp:332-556
  *   CombPopulationControl                population/CombPopulationControl.hpp:20-32
- *   SideTemperature / Vacuum / Rigid     boundary/SideTemperature.hpp:156-191,
+ *   SideTemperature / Vacuum / Rigid     boundary/SideTemperature.hpp:20-167,
  *                                        boundary/Vacuum.hpp:10-45, boundary/RigidBoundary.hpp:9-24
  * The C++ drop-in (include/storm_amd/MonteCarloManagerGPU.hpp) binds them.
  */
@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define STORM_GPU_ABI_VERSION 4
+#define STORM_GPU_ABI_VERSION 5
 
 typedef struct storm_gpu_engine storm_gpu_engine;
 
@@ -123,7 +123,24 @@ int storm_gpu_destroy(storm_gpu_engine *eng);
  * the boundary sources and population control. */
 int storm_gpu_step(storm_gpu_engine *eng, double dt);
 
-/* SideTemperature::SetTemperature (boundary/SideTemperature.hpp:183) and the
+/* storm_gpu_step without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * step, bit for bit, with every transient buffer of the step (packets in
+ * flight, the census before population control, sort keys, prefix sums,
+ * reduction partials, counters) carved out of a fixed 64 MiB device array of
+ * the library (one per GPU, held until the process ends); it calls no
+ * hipMalloc, hipFree or other allocating routine.  Limits, each a clean error
+ * (status 4) that leaves the census and material state unchanged: the step's
+ * transient buffers must fit the 64 MiB (about 2.5e5 packets per step in the
+ * mixed engine, fewer in FP64 or multigroup), and the census must fit the
+ * engine's census buffer, which only storm_gpu_step and
+ * storm_gpu_import_census grow (at least doubling).  All engines on one GPU
+ * share the array: calls that overlap on one GPU race; serialise them.
+ * Rejects an engine with the debug copy of transport inputs switched on. */
+int storm_gpu_step_noalloc(storm_gpu_engine *eng, double dt);
+
+/* SideTemperature::SetTemperature (boundary/SideTemperature.hpp:48) and the
  * MarshakBoundary equivalent. */
 int storm_gpu_set_source_temperature(storm_gpu_engine *eng, double T);
 

@@ -124,6 +124,7 @@ def load_library(path: str | None = None):
         "storm_gpu_create": ([C.POINTER(Problem), C.c_int32, C.c_int32, C.POINTER(vp)], C.c_int),
         "storm_gpu_destroy": ([vp], C.c_int),
         "storm_gpu_step": ([vp, C.c_double], C.c_int),
+        "storm_gpu_step_noalloc": ([vp, C.c_double], C.c_int),
         "storm_gpu_set_source_temperature": ([vp, C.c_double], C.c_int),
         "storm_gpu_set_cell_state": ([vp, _dp, _dp, _dp, _dp, _dp], C.c_int),
         "storm_gpu_get_cells": ([vp, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),

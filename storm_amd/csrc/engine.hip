@@ -12,6 +12,7 @@
 #include <initializer_list>
 #include <type_traits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -50,27 +51,106 @@ struct Retired {
 };
 thread_local std::vector<Retired> *t_retired = nullptr;
 
+// Scratch of storm_gpu_step_noalloc: the step's transient buffers (packets
+// in flight, the comb's keys and prefix sums, rocPRIM's temporary storage,
+// reduction partials, counters and read-back mailboxes) are carved out of the
+// file-scope device array `scratch` below instead of device allocations.
+// While a ScratchArena is installed on the thread, DevBuf::ensure of a
+// transient buffer takes memory from it, and a buffer of the engine's state
+// that would have to grow throws instead of allocating.
+struct ScratchArena {
+  unsigned char *base;
+  size_t size;
+  size_t used;
+  void *take(size_t bytes) {
+    const size_t at = (used + 255u) & ~size_t(255);
+    if (at > size || bytes > size - at)
+      throw EngineError(4, "storm_gpu_step_noalloc: this step needs more than the library's fixed scratch of " +
+                               std::to_string(size >> 20) + " MiB (too many packets in one step); use storm_gpu_step");
+    used = at + bytes;
+    return base + at;
+  }
+};
+thread_local ScratchArena *t_arena = nullptr;
+
+// The scratch array itself: one copy per GPU, held by every process that
+// loads this library until it ends.  Host code never passes `scratch` to a
+// kernel (on the host it only names the array); it uses the device address
+// that hipGetSymbolAddress returns for the current device, looked up once per
+// device when an engine is created there.  All engines on one GPU share the
+// array: storm_gpu_step_noalloc calls that overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxScratchDevices] = {};
+
+void *noalloc_scratch_address() {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices) throw EngineError(2, "device index beyond the scratch address table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw EngineError(2, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = ptr;
+  }
+  return g_scratch_dev[dev];
+}
+
+// A buffer's allocation while it is lent to a ScratchArena.
+struct SavedBuf {
+  void *p;
+  size_t n;
+};
+
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  bool view = false;  // points into a ScratchArena (never freed here)
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) {
+    if (p && !view) {
       if (t_retired) t_retired->push_back(Retired{p, n * sizeof(T)});
       else (void)hipFree(p);
     }
     p = nullptr;
     n = 0;
   }
+  // Hand the allocation to `saved` and become an empty view into the
+  // installed ScratchArena; give_back restores the allocation.
+  SavedBuf lend() {
+    SavedBuf sb{p, n};
+    p = nullptr;
+    n = 0;
+    view = true;
+    return sb;
+  }
+  void give_back(const SavedBuf &sb) {
+    p = static_cast<T *>(sb.p);
+    n = sb.n;
+    view = false;
+  }
   // Grow without keeping the contents, to a power of two at least twice the
   // old capacity: a census that grows step by step then reallocates a few
   // times only (hipFree waits for the device and costs about a millisecond).
   void ensure(size_t m) {
     if (m <= n && p) return;
+    if (t_arena) {
+      if (!view)
+        throw EngineError(4, "storm_gpu_step_noalloc: a buffer of the engine's state must grow to " +
+                                 std::to_string(m) + " elements (capacity " + std::to_string(n) +
+                                 "; the census outgrew what earlier steps allocated); use storm_gpu_step");
+      p = static_cast<T *>(t_arena->take(m * sizeof(T)));
+      n = m;
+      return;
+    }
     size_t cap = 1;
     while (cap < m || cap < 2 * n) cap <<= 1;
     release();
@@ -133,6 +213,7 @@ struct PairAdd {
 struct EngineBase {
   virtual ~EngineBase() = default;
   virtual void step(double dt) = 0;
+  virtual void step_noalloc(double dt) = 0;
   virtual void get_cells(double *, double *, double *, double *, double *, double *) = 0;
   virtual void get_segments(uint64_t *) = 0;
   virtual void get_begin(uint64_t *) = 0;
@@ -185,7 +266,7 @@ struct Engine final : EngineBase {
   uint64_t seed;
   int32_t precision, material_mode;
   bool frozen = false;
-  std::vector<double> frozenT4s;
+  double frozenT4s[2] = {0.0, 0.0};
   double source_T_hi = 0.0;
   uint32_t step_index = 0;
   int32_t eq_prev = 0;  // quantum exponent of the resident census weights
@@ -216,9 +297,11 @@ struct Engine final : EngineBase {
   // pinned memory, one synchronisation
   DevBuf<unsigned long long> mail;
   unsigned long long *mail_h = nullptr;
-  // pinned staging of the boundary face weights
+  // pinned staging of the boundary face weights (both x sides, sized at creation)
   double *wface_h = nullptr;
-  size_t wface_h_n = 0;
+  // host staging of the boundary sources' group CDFs (capacity reserved at creation)
+  std::vector<Real> bcdf_h;
+  std::vector<double> planck_acc;
   DevBuf<CellAccel<Real>> accel;
   DevBuf<double> Dtil;
   DevBuf<Real> rw_tau, rw_surv, rw_rad;
@@ -349,6 +432,26 @@ struct Engine final : EngineBase {
     qexp.ensure(2);     // energy quantum exponents of the step
     mail.ensure(8);
     HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&mail_h), 8 * sizeof(unsigned long long)));
+    // boundary source faces: at most both x sides of ny * nz faces
+    {
+      const size_t max_faces = 2 * static_cast<size_t>(ny) * nz;
+      wface.ensure(max_faces);
+      HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&wface_h), max_faces * sizeof(double)));
+    }
+    scratch_ptr = noalloc_scratch_address();
+    // The census buffer is the engine's state (the step's input and output);
+    // reserve it for twice the census the population control aims at (or
+    // four steps of sources without it), so that storm_gpu_step_noalloc,
+    // which never allocates, can run from the first step.
+    {
+      const double per_step = static_cast<double>(N) * static_cast<double>(imc.new_photons_per_cell) +
+                              2.0 * ny * nz * static_cast<double>(bnd.source_packets_per_face);
+      const double factor = imc.comb_factor >= 1.0 ? std::min(imc.comb_factor, 64.0) : 1.0;
+      const double want = imc.comb ? 2.0 * N * static_cast<double>(std::max<uint64_t>(imc.comb_nmin, 1)) * factor +
+                                         per_step
+                                   : 4.0 * per_step;
+      cnext.ensure(static_cast<size_t>(std::min(std::max(want, 1.0), 16777216.0)));
+    }
     counters_u32.ensure(4);
     if (groups >= 2) {
       up(bounds_erg, bounds_h.data(), bounds_h.size());
@@ -358,7 +461,9 @@ struct Engine final : EngineBase {
       HIPCHECK(hipMemcpy(bounds_kev.p, bk.data(), bk.size() * sizeof(Real), hipMemcpyHostToDevice));
       cdf.ensure(static_cast<size_t>(N) * static_cast<size_t>(groups + 1));
       sig_s_cell.ensure(N);
-      bcdf.ensure(static_cast<size_t>(groups + 1));
+      bcdf.ensure(2 * static_cast<size_t>(groups + 1));
+      bcdf_h.reserve(2 * static_cast<size_t>(groups + 1));
+      planck_acc.reserve(static_cast<size_t>(groups + 1));
       pa.with_nu = pb.with_nu = true;
     }
     if (accel_on) {
@@ -683,22 +788,26 @@ struct Engine final : EngineBase {
     // boundary sources: SideTemperature / TwoSidesTemperature energy per face
     // (T^4 frozen at the first call if asked), sides x- then x+
     const uint32_t per_side = static_cast<uint32_t>(ny) * nz;
-    std::vector<double> side_T;
-    if (bnd.source_face == STORM_XLO) side_T.push_back(source_T);
-    const uint32_t xhi_from = static_cast<uint32_t>(side_T.size());
-    if (bnd.source_xhi) side_T.push_back(source_T_hi);
-    const uint32_t npc = side_T.empty() ? 0u : static_cast<uint32_t>(bnd.source_packets_per_face);
-    const uint32_t nfaces = npc ? per_side * static_cast<uint32_t>(side_T.size()) : 0u;
-    std::vector<double> side_T4(side_T.size());
-    for (size_t s = 0; s < side_T.size(); ++s) side_T4[s] = side_T[s] * side_T[s] * side_T[s] * side_T[s];
+    double side_T[2];
+    uint32_t nsides = 0;
+    if (bnd.source_face == STORM_XLO) side_T[nsides++] = source_T;
+    const uint32_t xhi_from = nsides;
+    if (bnd.source_xhi) side_T[nsides++] = source_T_hi;
+    const uint32_t npc = nsides ? static_cast<uint32_t>(bnd.source_packets_per_face) : 0u;
+    const uint32_t nfaces = npc ? per_side * nsides : 0u;
+    double side_T4[2] = {0.0, 0.0};
+    for (uint32_t s = 0; s < nsides; ++s) side_T4[s] = side_T[s] * side_T[s] * side_T[s] * side_T[s];
     if (bnd.freeze_source_T4) {
       if (!frozen) {
-        frozenT4s = side_T4;
+        for (uint32_t s = 0; s < nsides; ++s) frozenT4s[s] = side_T4[s];
         frozen = true;
       }
-      side_T4 = frozenT4s;
+      for (uint32_t s = 0; s < nsides; ++s) side_T4[s] = frozenT4s[s];
     }
-    std::vector<double> wface_e(nfaces);
+    // face weights, computed straight into the pinned staging buffer (sized
+    // at creation; the stream is idle: the last step ended with a
+    // synchronisation, so its copy from this buffer is done)
+    double *wface_e = wface_h;
     double source_total = 0.0;
     for (uint32_t f = 0; f < nfaces; ++f) {
       const uint32_t s = f / per_side, jk = f % per_side;
@@ -734,33 +843,22 @@ struct Engine final : EngineBase {
     for (uint32_t f = 0; f < nfaces; ++f) wface_e[f] = fx::quantize_down(wface_e[f], eq);
     if (groups >= 2 && nfaces) {
       // Planck fractions of each source temperature per group
-      // (examples/densmore2012/DensmoreBoundary.hpp:96-104)
-      std::vector<Real> h;
-      for (double Ts : side_T) {
-        const double kT = units::k_boltz * Ts;
-        std::vector<double> acc(groups + 1, 0.0);
+      // (examples/densmore2012/DensmoreBoundary.hpp:96-104), into host
+      // vectors whose capacity was reserved at creation
+      bcdf_h.clear();
+      for (uint32_t s = 0; s < nsides; ++s) {
+        const double kT = units::k_boltz * side_T[s];
+        planck_acc.assign(static_cast<size_t>(groups) + 1, 0.0);
         for (int32_t g = 0; g < groups; ++g)
-          acc[g + 1] = acc[g] + planck::integral(bounds_h[g] / kT, bounds_h[g + 1] / kT);
+          planck_acc[g + 1] = planck_acc[g] + planck::integral(bounds_h[g] / kT, bounds_h[g + 1] / kT);
         for (int32_t g = 0; g <= groups; ++g)
-          h.push_back(acc[groups] > 0.0 ? static_cast<Real>(acc[g] / acc[groups]) : Real(0));
+          bcdf_h.push_back(planck_acc[groups] > 0.0 ? static_cast<Real>(planck_acc[g] / planck_acc[groups]) : Real(0));
       }
-      bcdf.ensure(h.size());
-      HIPCHECK(hipMemcpyAsync(bcdf.p, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice, stream));
+      bcdf.ensure(bcdf_h.size());
+      HIPCHECK(hipMemcpyAsync(bcdf.p, bcdf_h.data(), bcdf_h.size() * sizeof(Real), hipMemcpyHostToDevice, stream));
       HIPCHECK(hipStreamSynchronize(stream));
     }
-    wface.ensure(std::max<uint32_t>(nfaces, 1));
-    if (nfaces) {
-      // the stream is idle here (read_scalars above), so the staging buffer
-      // of the last step is free
-      if (wface_h_n < nfaces) {
-        if (wface_h) HIPCHECK(hipHostFree(wface_h));
-        wface_h = nullptr;
-        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&wface_h), nfaces * sizeof(double)));
-        wface_h_n = nfaces;
-      }
-      std::memcpy(wface_h, wface_e.data(), nfaces * sizeof(double));
-      HIPCHECK(hipMemcpyAsync(wface.p, wface_h, nfaces * sizeof(double), hipMemcpyHostToDevice, stream));
-    }
+    if (nfaces) HIPCHECK(hipMemcpyAsync(wface.p, wface_h, nfaces * sizeof(double), hipMemcpyHostToDevice, stream));
     const uint64_t n_bc = static_cast<uint64_t>(nfaces) * npc;
     const uint64_t n_total = n_census + n_emit + n_bc;
     // 32-bit packet indices; the refill queue head may run past the count by
@@ -896,6 +994,87 @@ struct Engine final : EngineBase {
 
   int32_t last_eq_ = 0, last_et_ = 0;
   double last_dt_ = 0.0;
+
+  // ---------------------------------------------------------- step_noalloc
+  // The same step with every transient buffer carved out of the library's
+  // fixed device scratch (storm_gpu_step_noalloc).  Transient buffers carry
+  // nothing from one step to the next: small fixed-size ones (reduction
+  // partials, counters, mailboxes, error flags) and ones sized by the step's
+  // packet count (packets in flight, the census before the comb, the comb's
+  // keys and prefix sums, rocPRIM's temporary storage).  The step clears
+  // (with fill kernels) every counter that must start at zero, so what an
+  // earlier call left in the scratch does not matter.  Buffers of the
+  // engine's state are never allocated here: the census buffer must already
+  // hold this step's census (EngineError 4 otherwise, raised before the
+  // comb writes the census and before the material update; the step counter
+  // is restored so that a following step repeats this one exactly).
+  void *scratch_ptr = nullptr;
+  template <typename F>
+  void for_fixed_transient(F &&f) {
+    f(partials);
+    f(counters_u32);
+    f(err);
+    f(qexp);
+    f(mail);
+    f(leaving);
+  }
+  template <typename F>
+  void for_sized_transient(F &&f) {
+    for (PacketBufs<Real> *b : {&pa, &pb}) {
+      f(b->pos);
+      f(b->dir);
+      f(b->w);
+      f(b->meta);
+      f(b->nu);
+    }
+    f(cpre.pos);
+    f(cpre.dir);
+    f(cpre.w);
+    f(cpre.meta);
+    f(keys);
+    f(keys_out);
+    f(vals);
+    f(vals_out);
+    f(limbs_sorted);
+    f(cum);
+    f(ncopy);
+    f(offs64);
+    f(temp);
+  }
+  static constexpr size_t kTransientBufs = 6 + 2 * 5 + 4 + 9;
+
+  void step_noalloc(double dt) override {
+    if (keep_inputs)
+      throw EngineError(1, "storm_gpu_step_noalloc: the debug copy of the transport inputs allocates host memory; "
+                           "switch it off first (storm_gpu_debug_keep_transport_inputs)");
+    if (!scratch_ptr) throw EngineError(2, "storm_gpu_step_noalloc: no scratch address for this device");
+    ScratchArena arena{static_cast<unsigned char *>(scratch_ptr), kNoAllocScratchBytes, 0};
+    SavedBuf saved[kTransientBufs];
+    size_t k = 0;
+    for_fixed_transient([&](auto &b) { saved[k++] = b.lend(); });
+    for_sized_transient([&](auto &b) { saved[k++] = b.lend(); });
+    auto restore = [&] {
+      t_arena = nullptr;
+      size_t q = 0;
+      for_fixed_transient([&](auto &b) { b.give_back(saved[q++]); });
+      for_sized_transient([&](auto &b) { b.give_back(saved[q++]); });
+    };
+    const uint32_t step_before = step_index;
+    t_arena = &arena;
+    try {
+      // the fixed ones are used without an ensure() in the step
+      k = 0;
+      for_fixed_transient([&](auto &b) { b.ensure(saved[k++].n); });
+      step(dt);
+    } catch (...) {
+      // nothing of this step may still run on the scratch
+      (void)hipStreamSynchronize(stream);
+      restore();
+      step_index = step_before;
+      throw;
+    }
+    restore();
+  }
 
   // Comb (or pass-through) from `cpre` into `cnext`; returns the census size.
   uint64_t comb(uint32_t n, int32_t eq) {
@@ -1362,6 +1541,10 @@ int storm_gpu_destroy(storm_gpu_engine *eng) {
 
 int storm_gpu_step(storm_gpu_engine *eng, double dt) {
   return guarded([&] { as_engine(eng)->step(dt); });
+}
+
+int storm_gpu_step_noalloc(storm_gpu_engine *eng, double dt) {
+  return guarded([&] { as_engine(eng)->step_noalloc(dt); });
 }
 
 int storm_gpu_set_source_temperature(storm_gpu_engine *eng, double T) {

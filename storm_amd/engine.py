@@ -64,6 +64,10 @@ class GpuEngine:
     def step(self, dt: float | None = None):
         self._check(self.lib.storm_gpu_step(self.h, float(self.spec.dt if dt is None else dt)))
 
+    def step_noalloc(self, dt: float | None = None):
+        """storm_gpu_step_noalloc: the same step without any device allocation."""
+        self._check(self.lib.storm_gpu_step_noalloc(self.h, float(self.spec.dt if dt is None else dt)))
+
     def set_source_temperature(self, T: float):
         self._check(self.lib.storm_gpu_set_source_temperature(self.h, float(T)))
 

@@ -44,7 +44,7 @@ def test_struct_layouts_match_header_sizes():
 
 def test_abi_version_without_gpu():
     lib = capi.load_library()
-    assert lib.storm_gpu_abi_version() == 4
+    assert lib.storm_gpu_abi_version() == 5
 
 
 def test_cpp_manager_program_fails_loudly_without_engine():
@@ -62,3 +62,4 @@ def test_cpp_manager_program_fails_loudly_without_engine():
         assert len(json.loads(out.stdout)["runs"]) == 1
     else:
         assert out.returncode == 1 and "gfx950 engine" in out.stderr, (out.returncode, out.stderr)
+
