@@ -1,23 +1,37 @@
-"""Benchmark: tracked segments/s and seconds per IMC step on one MI355X.
+"""Benchmark: tracked segments/s and seconds per IMC step on one MI355X,
+mixed precision against plain FP64 at matched per-cell tally error.
 
-Workload (N = 1): the 64^3 gray IMC cube (storm_amd.problems.c2_cube):
-sigma_a = sigma0 * max(T, 0.1 keV)^-3 with sigma0 log-uniform in [1, 100] per
-cell, cv proportional to T, T0 = 1e-3 keV, a 1 keV source on the x- face and
-escape on every face, ~1e7 packets per step, dt = 1e-11 s, census resident in
-HBM.  One "step" = one full IMC step (preStep, sourcing, transport, comb,
-postStep).  W untimed steps, then K timed steps; `value` is the whole-job
-segments/s of the mixed-precision engine.  The plain FP64 engine runs the
-same W + K steps for comparison, and the FP64 CPU oracle is timed on a
-bounded sample of the same workload (`cpu_baseline`).
+Workload (N = 1): the 64^3 gray IMC cube as stated (storm_amd.problems.c2_cube
+with literal=True): sigma_a = sigma0 * T^-3 with sigma0 log-uniform in
+[1, 100] per cell, cv proportional to T, T0 = 1e-3 keV, a 1 keV source on the
+x- face, escape on every face, >= 1e7 packets per step, dt = 1e-11 s, 50 IMC
+steps, the random walk on at the reference's default thresholds (the cold
+cube is ~1e7 mean free paths per cell), census resident in HBM.  One "step" =
+one full IMC step (preStep, sourcing, transport, comb, postStep).
+
+Timed region: seed 1 of the mixed engine, W untimed steps then K timed steps
+(`value` = segments/s over those K steps, wall clock around each host-
+synchronised step).  Every run then continues, untimed by `value`, to the
+problem's 50 steps: S seeds per form (mixed and FP64) give per-cell means and
+variances of the time-averaged radiation energy density at step 50.  The
+matched-error comparison:
+  rel_err = median over cells of the one-run relative standard deviation,
+  FOM     = 1 / (rel_err^2 * t_run), t_run = wall time of a 50-step run,
+and the per-cell agreement of the two forms (|z| <= 3, pooled variance).
 
 Roofline: no MFMA (nothing is a dense contraction), so the roofline is HBM
-bandwidth against the algorithmic bytes of the metric's model: 144 B per
-segment (a 48 B packet read and written, a 16 B cell record, two 16 B
-tallies).  achieved = segments * 144 B / transport time (HIP events on the
-engine's stream around each launch), over the timed steps.  The history
-kernel keeps a packet in registers for its whole history, so its measured
-HBM traffic per segment is far below the model (rocprof FETCH_SIZE /
-WRITE_SIZE).
+bandwidth against the metric's algorithmic bytes, 144 B per segment (a 48 B
+packet read and written, a 16 B cell record, two 16 B tallies).  achieved =
+segments * 144 B / transport time over every launch of the mixed transport
+kernel in this process (HIP events on the engine's stream), so its average
+launch duration is the one a profiler's kernel statistics of this command
+report.  The history kernel keeps a packet in registers for its whole
+history, so its measured HBM traffic (rocprof FETCH_SIZE / WRITE_SIZE,
+`traffic`) is far below the model.
+
+A second line of figures (`c3`) runs the two-material step slab (random walk
+and DDMC, 2e7 packets per step) for a few steps, and `cpu_baseline` times the
+FP64 CPU oracle on host cores on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -27,6 +41,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -34,10 +50,14 @@ METRIC = "tracked segments/s and s per IMC step, mixed vs FP64 GPU at matched ta
 B_SEG_MIXED = 144.0  # bytes per segment: 2 * 48 B packet + 16 B cell + 2 * 16 B tally
 B_SEG_FP64 = 224.0   # 2 * 80 B packet + 32 B cell + 2 * 16 B tally
 HBM_PEAK_GBS = 8000.0
+PROBLEM_STEPS = 50
+# C2 packet knobs: 2400 boundary packets per x- face (9.8e6), >= 2 emitted per
+# cell (5.2e5), comb floor 8 per cell: >= 1e7 packets in every step
+C2_KNOBS = dict(source_per_face=2400, comb_nmin=8)
 # HBM bytes per transport launch, measured by rocprofv3 (FETCH_SIZE and
 # WRITE_SIZE, scaled by the same counters on a plain copy kernel) on this
-# benchmark's own command; the summary holds the figures and the command.
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_transport_c2_cube_counters.md")
+# benchmark's own default command; the summary holds the figures and the command.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_transport_c2_literal_counters.md")
 
 
 def measured_traffic():
@@ -52,56 +72,184 @@ def measured_traffic():
     return None, None
 
 
-def run_engine(precision, steps, warmup, seed, log, schedule=None, copy_mib=0):
+def c2_spec(seed):
+    from storm_amd import problems as P
+    return P.c2_cube(seed=seed, literal=True, **C2_KNOBS)
+
+
+def run_seed(precision, seed, n_steps, snap_step, timed=None, log=None):
+    """One run of the C2 cube; per-step wall times and counters, the cells at
+    `snap_step`.  `timed` = (W, K): log the steps of that window."""
+    from storm_amd.engine import GpuEngine
+
+    spec = c2_spec(seed)
+    steps = []
+    snap = None
+    with GpuEngine(spec, precision) as e:
+        for s in range(n_steps):
+            t0 = time.perf_counter()
+            e.step()
+            wall = time.perf_counter() - t0
+            c = e.counters()
+            steps.append(dict(wall=wall, segments=c["segments"], packets=c["start_particles"],
+                              t_tr=c["seconds_transport"], launches=c["transport_launches"],
+                              rw=c["random_walk_steps"]))
+            if log and timed:
+                W, K = timed
+                tag = "warmup" if s < W else ("timed" if s < W + K else "untimed")
+                log(f"seed {seed} step {s + 1}/{n_steps} ({tag}): {c['segments']} segments, {c['start_particles']} "
+                    f"packets, {c['random_walk_steps']} walk steps, transport {c['seconds_transport'] * 1e3:.2f} ms, "
+                    f"step {wall * 1e3:.2f} ms")
+            if s + 1 == snap_step:
+                snap = e.cells()
+    return steps, snap
+
+
+def ensemble(precision, seeds, n_steps, W, K, log, label):
+    runs = []
+    for i, sd in enumerate(seeds):
+        t0 = time.perf_counter()
+        steps, snap = run_seed(precision, sd, n_steps, PROBLEM_STEPS, timed=(W, K) if i == 0 else None, log=log)
+        runs.append(dict(seed=sd, steps=steps, snap=snap))
+        log(f"{label} seed {sd}: {n_steps} steps in {time.perf_counter() - t0:.2f} s, "
+            f"{sum(s['segments'] for s in steps[:PROBLEM_STEPS]) / 1e9:.2f} G segments in the first {PROBLEM_STEPS}")
+    return runs
+
+
+def window(steps, lo, hi):
+    seg = sum(s["segments"] for s in steps[lo:hi])
+    wall = sum(s["wall"] for s in steps[lo:hi])
+    return seg, wall
+
+
+def matched_error(mixed, fp64):
+    """Per-cell relative error of one run (seed-ensemble standard deviation
+    over the mean) of the time-averaged radiation energy density at step 50,
+    the FOM of each form and the agreement of the two ensembles.
+
+    Cells: those the radiation has reached, mean above 1e-3 of the largest
+    (in the cold cube a cell's own emission is absorbed in place, so its
+    Erad is nearly deterministic and says nothing about Monte Carlo noise).
+    The z of the two forms uses the pooled variance plus (1e-6 mean)^2, the
+    representation error of FP32 opacities."""
+    A = np.array([r["snap"]["erad"] for r in mixed])
+    B = np.array([r["snap"]["erad"] for r in fp64]) if fp64 else None
+    ref = B if B is not None else A
+    mref = ref.mean(0)
+    cells = mref > 1e-3 * mref.max()
+    out = {}
+
+    def rel(X):
+        m = X.mean(0)[cells]
+        sd = X.std(0, ddof=1)[cells]
+        ok = m > 0
+        return float(np.median(sd[ok] / m[ok])), int(ok.sum())
+
+    t_m = float(np.mean([sum(s["wall"] for s in r["steps"][:PROBLEM_STEPS]) for r in mixed]))
+    ra, na = rel(A)
+    out["mixed"] = dict(rel_err=ra, cells=na, seconds_per_run=t_m, seconds_per_step=t_m / PROBLEM_STEPS,
+                        fom=1.0 / (ra * ra * t_m))
+    if B is not None:
+        t_f = float(np.mean([sum(s["wall"] for s in r["steps"][:PROBLEM_STEPS]) for r in fp64]))
+        rb, nb = rel(B)
+        out["fp64"] = dict(rel_err=rb, cells=nb, seconds_per_run=t_f, seconds_per_step=t_f / PROBLEM_STEPS,
+                           fom=1.0 / (rb * rb * t_f))
+        out["fom_ratio"] = out["mixed"]["fom"] / out["fp64"]["fom"]
+        from scipy import stats
+        ma, mb = A.mean(0)[cells], B.mean(0)[cells]
+        se2 = A.var(0, ddof=1)[cells] / len(A) + B.var(0, ddof=1)[cells] / len(B) + (1e-6 * np.maximum(ma, mb)) ** 2
+        d = ma - mb
+        z = np.where(se2 > 0, d / np.sqrt(np.where(se2 > 0, se2, 1.0)), np.where(d == 0, 0.0, np.inf))
+        nu = len(A) + len(B) - 2
+        out["cells_within_3sigma"] = float(np.mean(np.abs(z) <= 3.0))
+        out["expected_within_3sigma"] = float(1.0 - 2.0 * stats.t.sf(3.0, nu))
+        out["mean_z2"] = float(np.mean(z[np.isfinite(z)] ** 2))
+        out["expected_mean_z2"] = nu / (nu - 2.0)
+        out["max_abs_z"] = float(np.max(np.abs(z)))
+    out["tally"] = "time-averaged radiation energy density per cell at step 50 (getEradTimeAvg)"
+    out["cells"] = "cells with a mean above 1e-3 of the largest: " + str(int(cells.sum()))
+    out["seeds"] = len(A)
+    return out
+
+
+def c3_line(W, K, log):
+    """Two-material step slab (C3: random walk and DDMC, 2e7 packets per step)."""
+    from storm_amd import capi
     from storm_amd import problems as P
     from storm_amd.engine import GpuEngine
 
-    spec = P.c2_cube(seed=seed)
-    with GpuEngine(spec, precision) as e:
-        if schedule is not None:
-            e.set_tuning(schedule=schedule)
-        for s in range(warmup):
+    spec = P.c3_step(seed=1)
+    seg = wall = t_tr = 0.0
+    rw = ddmc = packets = launches = 0
+    seg_all = t_all = 0.0
+    n_all = 0
+    with GpuEngine(spec, capi.MIXED) as e:
+        for s in range(W + K):
+            t0 = time.perf_counter()
             e.step()
-            log(f"warmup step {s + 1}/{warmup}: {e.counters()['segments']} segments")
-        seg = 0
-        t_tr = 0.0
-        launches = 0
-        packets = 0
-        t0 = time.perf_counter()
-        for s in range(steps):
-            e.step()
+            dtw = time.perf_counter() - t0
             c = e.counters()
-            seg += c["segments"]
-            t_tr += c["seconds_transport"]
-            launches += c["transport_launches"]
-            packets += c["start_particles"]
-            log(f"step {s + 1}/{steps}: {c['segments']} segments, {c['start_particles']} packets, "
-                f"transport {c['seconds_transport'] * 1e3:.2f} ms, step {c['seconds_step'] * 1e3:.2f} ms")
-        wall = time.perf_counter() - t0
-    return dict(segments=seg, wall=wall, transport_s=t_tr, launches=launches, packets=packets,
-                name=spec.name, N=spec.ncells)
+            seg_all += c["segments"]
+            t_all += c["seconds_transport"]
+            n_all += c["transport_launches"]
+            log(f"c3 step {s + 1}/{W + K}: {c['segments']} segments, {c['start_particles']} packets, "
+                f"{c['random_walk_steps']} walk, {c['ddmc_steps']} DDMC, step {dtw * 1e3:.2f} ms")
+            if s >= W:
+                seg += c["segments"]
+                wall += dtw
+                t_tr += c["seconds_transport"]
+                rw += c["random_walk_steps"]
+                ddmc += c["ddmc_steps"]
+                packets += c["start_particles"]
+                launches += c["transport_launches"]
+    # every transport launch of the run (they are launches of the same kernel
+    # as the C2 cube's, counted in its roofline)
+    c3_all = (seg_all, t_all, n_all)
+    return dict(launch_totals=c3_all,
+                workload="c3_step: 256x16x16 two-material slab, sigma_a 1 | 1e4 (+ sigma_s 1e4), RW + DDMC, "
+                         "~2e7 packets/step, dt=1e-11 s",
+                value=seg / wall, unit="segments/s", steps=K, warmup=W, ms_per_step=wall / K * 1e3,
+                packets_per_step=packets / K, transport_segments_per_s=seg / t_tr,
+                random_walk_steps_per_step=rw / K, ddmc_steps_per_step=ddmc / K,
+                kernel="k_transport<float, 1u> (random walk + DDMC features)",
+                roofline_frac=seg * B_SEG_MIXED / t_tr / 1e9 / HBM_PEAK_GBS)
 
 
-def cpu_baseline(log):
-    """FP64 oracle on the host cores: one independent history batch per thread
-    (different seeds of the same workload, 3/20 of the packets, 2 steps),
-    aggregate segments over the wall time of the batch."""
+def cpu_baseline(W, log):
+    """FP64 oracle on the host: steps W+1, W+2 of the same workload at 1/100 of
+    the packets, timed from the oracle's own per-step clock; one run on one
+    thread, then one independent seed per thread on up to 16 threads."""
     import concurrent.futures as cf
     from oracle import pyoracle
     from storm_amd import problems as P
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    specs = [P.c2_cube(seed=1, packets_scale=0.15, steps=2).with_seed(1 + t) for t in range(threads)]
+    n_steps = W + 2
+
+    def spec(sd):
+        return P.c2_cube(seed=sd, literal=True, packets_scale=0.01, **C2_KNOBS)
+
+    def timed(out):
+        led = out["ledger"]
+        return float(led["segments"][W:].sum()), float(led["seconds"][W:].sum())
+
     pyoracle.lib()  # build / load outside the timed region
+    s1, t1 = timed(pyoracle.run(spec(1), steps=n_steps))
+    log(f"cpu baseline, 1 thread: {s1:.0f} segments in {t1:.2f} s")
+    threads = max(1, min(16, os.cpu_count() or 1))
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(max_workers=threads) as ex:
-        runs = list(ex.map(pyoracle.run, specs))
+        runs = list(ex.map(lambda sd: pyoracle.run(spec(sd), steps=n_steps), range(1, threads + 1)))
     wall = time.perf_counter() - t0
-    seg = float(sum(r["ledger"]["segments"].sum() for r in runs))
-    log(f"cpu baseline: {seg:.0f} segments on {threads} threads in {wall:.2f} s")
-    return dict(value=seg / wall, unit="segments/s", cores=threads, kind="port",
-                sample=f"64^3 cube, first 2 IMC steps at 3/20 of the packets (~1e6 per step), FP64 oracle, "
-                       f"{threads} independent seeds on {threads} threads")
+    seg = sum(timed(r)[0] for r in runs)
+    busy = max(sum(r["ledger"]["seconds"]) for r in runs)
+    log(f"cpu baseline, {threads} threads: {seg:.0f} segments of steps {W + 1}-{W + 2}; batch wall {wall:.2f} s")
+    # aggregate: timed steps' segments over the slowest thread's timed seconds
+    slowest = max(timed(r)[1] for r in runs)
+    return dict(value=seg / slowest, unit="segments/s", cores=threads, kind="port",
+                sample=f"64^3 cube as stated (sigma0 T^-3, random walk), steps {W + 1}-{W + 2} (the first timed GPU "
+                       f"steps) at 1/100 of the packets, FP64 oracle, {threads} independent seeds on {threads} threads, "
+                       f"timed by the oracle's per-step clock (slowest thread {busy:.1f} s for all steps)",
+                one_core=dict(value=s1 / t1, cores=1, sample="seed 1 of the same sample on one thread"))
 
 
 def main():
@@ -109,14 +257,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--seeds", type=int, default=16, help="seed ensemble per form for the matched-error comparison")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--quiet", action="store_true")
-    ap.add_argument("--schedule", choices=("pass", "refill"), default="pass",
-                    help="transport schedule of the mixed engine")
-    ap.add_argument("--copy-mib", type=int, default=0,
-                    help="after the run, one plain copy kernel over this many MiB (counter calibration)")
+    ap.add_argument("--copy-mib", type=int, default=256,
+                    help="after the run, one plain copy kernel over this many MiB (scales a profiler's HBM byte "
+                         "counters; 0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -132,31 +280,45 @@ def main():
     from storm_amd.engine import device_info
 
     log("device: " + device_info())
-    sched = capi.SCHEDULE_PASS if args.schedule == "pass" else capi.SCHEDULE_REFILL
-    log(f"mixed engine ({args.schedule} schedule)")
-    mixed = run_engine(capi.MIXED, args.steps, args.warmup, args.seed, log, schedule=sched)
-    value = mixed["segments"] / mixed["wall"]
-    ms_per_step = mixed["wall"] / args.steps * 1e3
-    achieved = mixed["segments"] * B_SEG_MIXED / mixed["transport_s"] / 1e9
-    traffic, traffic_src = measured_traffic() if args.schedule == "pass" else (None, None)
+    W, K = args.warmup, args.steps
+    n_steps = max(PROBLEM_STEPS, W + K)
+    seeds = list(range(1, max(args.seeds, 2) + 1))
+    log(f"mixed engine: {len(seeds)} seeds x {n_steps} steps (seed 1 timed at steps {W + 1}-{W + K})")
+    mixed = ensemble(capi.MIXED, seeds, n_steps, W, K, log, "mixed")
+    seg, wall = window(mixed[0]["steps"], W, W + K)
+    value = seg / wall
+    tr_seg = sum(s["segments"] for r in mixed for s in r["steps"])
+    tr_t = sum(s["t_tr"] for r in mixed for s in r["steps"])
+    tr_n = sum(s["launches"] for r in mixed for s in r["steps"])
+    c2_transport_rate = tr_seg / tr_t
+    c3 = None
+    if not args.no_c3:
+        c3 = c3_line(2, 3, log)
+        s3, t3, n3 = c3.pop("launch_totals")
+        tr_seg, tr_t, tr_n = tr_seg + s3, tr_t + t3, tr_n + n3
+    achieved = tr_seg * B_SEG_MIXED / tr_t / 1e9
+    traffic, traffic_src = measured_traffic()
+    packets = sum(s["packets"] for s in mixed[0]["steps"][W:W + K]) / K
+    seg50, wall50 = window(mixed[0]["steps"], 0, PROBLEM_STEPS)
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "segments/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": wall / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "c2_cube: 64^3 gray IMC, sigma_a=sigma0*max(T,0.1keV)^-3, ~1e7 packets/step, dt=1e-11 s",
-            "cells": mixed["N"],
-            "packets_per_step": mixed["packets"] / args.steps,
-            "segments_per_step": mixed["segments"] / args.steps,
+            "workload": "c2_cube: 64^3 gray IMC, sigma_a=sigma0*T^-3 (no floor), random walk, >=1e7 packets/step, "
+                        "dt=1e-11 s, 50 steps",
+            "cells": 64 ** 3,
+            "packets_per_step": packets,
+            "segments_per_step": seg / K,
             "arithmetic": "f32 cell-local tracking, 64-bit fixed-point tallies, f64 material update",
             "parallelism": "1 GPU",
         },
@@ -168,24 +330,40 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": "k_transport_w6<float>" if args.schedule == "pass" else "k_transport_refill<float, 0u>",
+            "kernel": "k_transport<float, 1u>",
             "bytes_per_segment": B_SEG_MIXED,
+            "launches": tr_n,
+            "avg_launch_us": tr_t / tr_n * 1e6,
+            "over": f"every launch of k_transport<float, 1u> in this process: the C2 cube's {len(seeds)} seeds x "
+                    f"{n_steps} steps" + ("" if c3 is None else " and the c3 line's 5 steps"),
+        },
+        "transport_segments_per_s": c2_transport_rate,
+        "full_problem": {
+            "steps": PROBLEM_STEPS,
+            "segments_per_s": seg50 / wall50,
+            "ms_per_step": wall50 / PROBLEM_STEPS * 1e3,
+            "packets_per_step": sum(s["packets"] for s in mixed[0]["steps"][:PROBLEM_STEPS]) / PROBLEM_STEPS,
+            "min_packets_per_step_after_first": min(s["packets"] for s in mixed[0]["steps"][1:PROBLEM_STEPS]),
         },
     }
+    fp64 = None
     if not args.no_fp64:
-        log("plain FP64 engine (one packet per thread)")
-        f64 = run_engine(capi.FP64, args.steps, args.warmup, args.seed, log)
+        log(f"plain FP64 engine: {len(seeds)} seeds x {n_steps} steps")
+        fp64 = ensemble(capi.FP64, seeds, n_steps, W, K, log, "fp64")
+        fseg, fwall = window(fp64[0]["steps"], W, W + K)
         out["fp64_gpu"] = {
-            "value": f64["segments"] / f64["wall"],
-            "ms_per_step": f64["wall"] / args.steps * 1e3,
-            "transport_segments_per_s": f64["segments"] / f64["transport_s"],
-            "form": "plain: FP64 tracking and tallies, one packet per thread",
+            "value": fseg / fwall,
+            "ms_per_step": fwall / K * 1e3,
+            "transport_segments_per_s": sum(s["segments"] for r in fp64 for s in r["steps"]) /
+                                        sum(s["t_tr"] for r in fp64 for s in r["steps"]),
+            "form": "plain: FP64 tracking and tallies, device libm",
         }
         out["speedup_vs_fp64"] = value / out["fp64_gpu"]["value"]
-
-    out["transport_segments_per_s"] = mixed["segments"] / mixed["transport_s"]
+    out["matched_error"] = matched_error(mixed, fp64)
+    if c3 is not None:
+        out["c3"] = c3
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(log)
+        out["cpu_baseline"] = cpu_baseline(W, log)
     if args.copy_mib > 0:
         copy_calibration(args.copy_mib, log)
     print(json.dumps(out))
