@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define STORM_GPU_ABI_VERSION 5
+#define STORM_GPU_ABI_VERSION 6
 
 typedef struct storm_gpu_engine storm_gpu_engine;
 

@@ -133,14 +133,60 @@ typedef struct storm_group_desc {
   double nu_ref;        /* erg */
 } storm_group_desc;
 
+/*
+ * A mesh of convex polyhedral cells flattened to per-cell face tables: a
+ * Voronoi tessellation (the reference's MadVoro meshes) or any other.  It
+ * carries what the reference's Grid concept answers
+ * (GetVolume, GetMeshPoint, GetCellFaces, GetFaceNeighbors, Normal, FaceCM,
+ * GetArea, GetPointsInFace / GetFacePoints; particle/Particle.hpp:188-279,
+ * radiation/RadiationIMC.hpp:1364-1404, utils/RandomInCell.hpp:10-67,
+ * utils/RandomOnFace.hpp:14-66).  Faces are stored per cell ("half faces"):
+ * a face between two cells appears in each, with opposite normals.  A face on
+ * the bounding box has neighbour -1 - side (sides in the STORM_XLO..STORM_ZHI
+ * order) and takes that side's storm_boundary_desc.face_kind; Planck sources
+ * enter through the faces of side x- (and x+ with source_xhi).
+ * Cells of a batch of independent problems (cell_instance set) never share a
+ * face; each instance then has its own sources, photon budget and
+ * population control (storm_instance_desc).
+ */
+typedef struct storm_unstructured_mesh {
+  uint32_t ncells;
+  uint32_t nfaces;                   /* half faces: sum over cells */
+  double box_lo[3], box_hi[3];
+  const double *volume;              /* ncells, cm^3 */
+  const double *mesh_point;          /* 3 ncells: the generator, strictly inside its cell */
+  const uint32_t *face_start;        /* ncells + 1: cell c owns faces [face_start[c], face_start[c + 1]) */
+  const int32_t *face_neighbor;      /* nfaces: the cell across the face, or -1 - box side */
+  const double *face_normal;         /* 3 nfaces: outward unit normal */
+  const double *face_centroid;       /* 3 nfaces */
+  const double *face_area;           /* nfaces, cm^2 */
+  const uint32_t *face_vertex_start; /* nfaces + 1 */
+  const double *vertices;            /* 3 per polygon vertex, counter-clockwise seen from outside */
+  const uint32_t *cell_instance;     /* ncells, or NULL: one problem */
+} storm_unstructured_mesh;
+
+/* Per instance of a batch (storm_unstructured_mesh.cell_instance): what
+ * differs between independent problems that share the material laws. */
+typedef struct storm_instance_desc {
+  double source_temperature;         /* K, on the instance's x- box faces */
+  uint64_t source_packets_per_face;
+  uint64_t new_photons_per_cell;
+  uint64_t comb_nmin;
+  double comb_factor;
+} storm_instance_desc;
+
 typedef struct storm_problem {
-  storm_mesh_desc mesh;
+  storm_mesh_desc mesh;              /* ignored when `unstructured` is set */
   storm_material_desc material;
   storm_boundary_desc boundary;
   storm_imc_params imc;
   const double *T0; /* initial material temperature per cell, K */
   uint64_t seed;
   storm_group_desc groups;
+  const storm_unstructured_mesh *unstructured; /* NULL: the Cartesian mesh */
+  uint32_t num_instances;            /* batches: instances 0 .. num_instances - 1 */
+  uint32_t reserved2;
+  const storm_instance_desc *instances;
 } storm_problem;
 
 #ifdef __cplusplus

@@ -40,6 +40,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -304,6 +305,117 @@ class CartesianGrid {
   std::vector<Face> boxFaces_;
 };
 
+// A mesh of convex polyhedral cells (storm_unstructured_mesh: a Voronoi
+// tessellation, say) behind the same Grid interface: global faces are built
+// from the per-cell half faces (a face shared by cells a < b takes a's
+// normal, centroid, area and polygon; its neighbours are (a, b)); a face on
+// the box is shared with the outside point N + face, whose GetMeshPoint is
+// the cell's mesh point mirrored across the face (MadVoro's outside points).
+class UnstructuredGrid {
+ public:
+  struct Face {
+    std::vector<Vec3> vertices;
+  };
+
+  explicit UnstructuredGrid(const storm_unstructured_mesh &m) : N_(m.ncells) {
+    if (!m.volume || !m.mesh_point || !m.face_start || !m.face_neighbor || !m.face_normal || !m.face_centroid ||
+        !m.face_area || !m.face_vertex_start || !m.vertices)
+      throw OracleError("UnstructuredGrid: null table");
+    ll_ = Vec3(m.box_lo[0], m.box_lo[1], m.box_lo[2]);
+    ur_ = Vec3(m.box_hi[0], m.box_hi[1], m.box_hi[2]);
+    vol_.assign(m.volume, m.volume + N_);
+    mp_.resize(N_);
+    for (size_t c = 0; c < N_; ++c) mp_[c] = Vec3(m.mesh_point[3 * c], m.mesh_point[3 * c + 1], m.mesh_point[3 * c + 2]);
+    cellFaces_.resize(N_);
+    cm_.assign(N_, Vec3(0.0));
+    std::unordered_map<uint64_t, size_t> shared;
+    for (size_t c = 0; c < N_; ++c) {
+      double vsum = 0.0;
+      for (uint32_t h = m.face_start[c]; h < m.face_start[c + 1]; ++h) {
+        const int32_t nb = m.face_neighbor[h];
+        // centroid of the cell: tetrahedra of the face fans about the mesh point
+        for (uint32_t v = m.face_vertex_start[h] + 1; v + 1 < m.face_vertex_start[h + 1]; ++v) {
+          const uint32_t v0 = m.face_vertex_start[h];
+          auto P = [&](uint32_t q) { return Vec3(m.vertices[3 * q], m.vertices[3 * q + 1], m.vertices[3 * q + 2]); };
+          const Vec3 a = P(v0) - mp_[c], b = P(v) - mp_[c], d = P(v + 1) - mp_[c];
+          const double tv = std::abs(dot(a, cross(b, d))) / 6.0;
+          vsum += tv;
+          cm_[c] += tv * (mp_[c] + 0.25 * (a + b + d));
+        }
+        size_t g;
+        if (nb >= 0 && static_cast<size_t>(nb) < c) {
+          auto it = shared.find((static_cast<uint64_t>(nb) << 32) | c);
+          if (it == shared.end()) throw OracleError("UnstructuredGrid: a face is not shared by its neighbour");
+          g = it->second;
+        } else {
+          if (nb >= 0 && static_cast<size_t>(nb) == c) throw OracleError("UnstructuredGrid: a face to its own cell");
+          g = nbr_.size();
+          const Vec3 n(m.face_normal[3 * h], m.face_normal[3 * h + 1], m.face_normal[3 * h + 2]);
+          nbr_.push_back({c, nb >= 0 ? static_cast<size_t>(nb) : N_ + g});
+          normal_.push_back(m.face_area[h] * n);
+          fc_.push_back(Vec3(m.face_centroid[3 * h], m.face_centroid[3 * h + 1], m.face_centroid[3 * h + 2]));
+          area_.push_back(m.face_area[h]);
+          std::vector<size_t> idx;
+          for (uint32_t v = m.face_vertex_start[h]; v < m.face_vertex_start[h + 1]; ++v) {
+            idx.push_back(verts_.size());
+            verts_.push_back(Vec3(m.vertices[3 * v], m.vertices[3 * v + 1], m.vertices[3 * v + 2]));
+          }
+          points_.push_back(std::move(idx));
+          if (nb >= 0) shared[(static_cast<uint64_t>(c) << 32) | static_cast<uint32_t>(nb)] = g;
+        }
+        cellFaces_[c].push_back(g);
+      }
+      cm_[c] *= 1.0 / vsum;
+    }
+    const Vec3 ll = ll_, ur = ur_;
+    boxFaces_.resize(6);
+    boxFaces_[STORM_XLO].vertices = {Vec3(ll.x, ll.y, ll.z), Vec3(ll.x, ur.y, ll.z), Vec3(ll.x, ll.y, ur.z)};
+    boxFaces_[STORM_XHI].vertices = {Vec3(ur.x, ll.y, ll.z), Vec3(ur.x, ur.y, ll.z), Vec3(ur.x, ll.y, ur.z)};
+    boxFaces_[STORM_YLO].vertices = {Vec3(ll.x, ll.y, ll.z), Vec3(ur.x, ll.y, ll.z), Vec3(ll.x, ll.y, ur.z)};
+    boxFaces_[STORM_YHI].vertices = {Vec3(ll.x, ur.y, ll.z), Vec3(ur.x, ur.y, ll.z), Vec3(ll.x, ur.y, ur.z)};
+    boxFaces_[STORM_ZLO].vertices = {Vec3(ll.x, ll.y, ll.z), Vec3(ur.x, ll.y, ll.z), Vec3(ll.x, ur.y, ll.z)};
+    boxFaces_[STORM_ZHI].vertices = {Vec3(ll.x, ll.y, ur.z), Vec3(ur.x, ll.y, ur.z), Vec3(ll.x, ur.y, ur.z)};
+  }
+
+  size_t GetPointNo() const { return N_; }
+  std::pair<Vec3, Vec3> GetBoxCoordinates() const { return {ll_, ur_}; }
+  Vec3 GetCellCM(size_t c) const { return cm_[c]; }
+  double GetVolume(size_t c) const { return vol_[c]; }
+  Vec3 GetMeshPoint(size_t p) const {
+    if (p < N_) return mp_[p];
+    const size_t g = p - N_;
+    const Vec3 n = normalize(normal_[g]);
+    const Vec3 &in = mp_[nbr_[g].first];
+    return in + (2.0 * dot(fc_[g] - in, n)) * n;
+  }
+  bool IsPointOutsideBox(size_t p) const { return p >= N_; }
+  bool IsPointOutsideBox(const Vec3 &p) const {
+    return p.x < ll_.x || p.x > ur_.x || p.y < ll_.y || p.y > ur_.y || p.z < ll_.z || p.z > ur_.z;
+  }
+  const std::vector<size_t> &GetCellFaces(size_t c) const { return cellFaces_[c]; }
+  std::pair<size_t, size_t> GetFaceNeighbors(size_t f) const { return nbr_[f]; }
+  Vec3 Normal(size_t f) const { return normal_[f]; }
+  Vec3 FaceCM(size_t f) const { return fc_[f]; }
+  double GetArea(size_t f) const { return area_[f]; }
+  const std::vector<size_t> &GetPointsInFace(size_t f) const { return points_[f]; }
+  const std::vector<Vec3> &GetFacePoints() const { return verts_; }
+  const std::vector<Face> &GetBoxFaces() const { return boxFaces_; }
+  size_t GetTotalFacesNumber() const { return nbr_.size(); }
+
+ private:
+  size_t N_;
+  Vec3 ll_, ur_;
+  std::vector<double> vol_;
+  std::vector<Vec3> mp_, cm_;
+  std::vector<std::vector<size_t>> cellFaces_;
+  std::vector<std::pair<size_t, size_t>> nbr_;
+  std::vector<Vec3> normal_, fc_;
+  std::vector<double> area_;
+  std::vector<std::vector<size_t>> points_;
+  std::vector<Vec3> verts_;
+  std::vector<Face> boxFaces_;
+};
+
 // ------------------------------------------------------------------ particle
 enum Status { NO_CELL_MOVE = 0, CELL_MOVE = 1, DONE = 2, REMOVE = 3, REFLECT = 4 };  // particle/ParticleStatus.hpp:8-15
 
@@ -364,9 +476,10 @@ inline Vec3 isotropicVelocity(Rng &rng) {
 // faces into tetrahedra around the cell centre, pick one by volume, then a
 // uniform point from sorted uniforms.  The per-cell triangulation is cached
 // (the reference rebuilds it each call; the draws and result are identical).
+template <typename Grid>
 class RandomInCellSampler {
  public:
-  explicit RandomInCellSampler(const CartesianGrid &g) : grid_(g), cache_(g.GetPointNo()) {}
+  explicit RandomInCellSampler(const Grid &g) : grid_(g), cache_(g.GetPointNo()) {}
   Vec3 operator()(size_t c, Rng &rng) {
     Entry &e = cache_[c];
     if (e.tris.empty()) build(c, e);
@@ -403,26 +516,28 @@ class RandomInCellSampler {
       }
     }
   }
-  const CartesianGrid &grid_;
+  const Grid &grid_;
   std::vector<Entry> cache_;
 };
 
 // RandomPointOnFace (utils/RandomOnFace.hpp:14-66): triangle fan, area-weighted
 // triangle choice, reflected (r1, r2) inside the triangle.
-inline Vec3 RandomPointOnFace(const CartesianGrid &grid, size_t f, Rng &rng) {
+template <typename Grid>
+Vec3 RandomPointOnFace(const Grid &grid, size_t f, Rng &rng) {
   auto fv = grid.GetPointsInFace(f);
   const auto &verts = grid.GetFacePoints();
-  double cumA[2], total = 0;
-  std::array<size_t, 3> tris[2];
+  std::vector<double> cumA;
+  std::vector<std::array<size_t, 3>> tris;
+  double total = 0;
   for (size_t i = 1; i + 1 < fv.size(); i++) {
     double area = norm(cross(verts[fv[i]] - verts[fv[0]], verts[fv[i + 1]] - verts[fv[0]]));
     total += area;
-    cumA[i - 1] = total;
-    tris[i - 1] = {fv[0], fv[i], fv[i + 1]};
+    cumA.push_back(total);
+    tris.push_back({fv[0], fv[i], fv[i + 1]});
   }
   double r = rng.unit() * total;
-  size_t idx = static_cast<size_t>(std::lower_bound(cumA, cumA + 2, r) - cumA);
-  if (idx >= 2) idx = 1;
+  size_t idx = static_cast<size_t>(std::lower_bound(cumA.begin(), cumA.end(), r) - cumA.begin());
+  if (idx >= tris.size()) idx = tris.size() - 1;
   double r1 = rng.unit(), r2 = rng.unit();
   if (r1 + r2 > 1) {
     r1 = 1 - r1;
@@ -552,9 +667,10 @@ struct Material {
 // getInwardBoxFaceNormalIfClose (boundary/BoundaryCondition.hpp:56-85); a
 // REMOVE face removes, every other face reflects
 // (reflectParticleOnBoxFace, boundary/BoundaryCondition.hpp:87-108).
+template <typename Grid>
 class FaceTableBoundary {
  public:
-  FaceTableBoundary(const CartesianGrid &g, const storm_boundary_desc &d, uint64_t seed)
+  FaceTableBoundary(const Grid &g, const storm_boundary_desc &d, uint64_t seed)
       : grid_(g), desc_(d), re_(mixSeed(seed, 11)), faceRe_(mixSeed(seed, 12)) {}
 
   double escapedEnergy = 0.0;
@@ -662,7 +778,7 @@ class FaceTableBoundary {
   }
 
  private:
-  bool getInwardBoxFaceNormalIfClose(const CartesianGrid::Face &face, const Vec3 &location,
+  bool getInwardBoxFaceNormalIfClose(const typename Grid::Face &face, const Vec3 &location,
                                      Vec3 &normal, double &faceScale) const {
     const Vec3 &onFace = face.vertices[0];
     Vec3 u = face.vertices[1] - face.vertices[0];
@@ -680,7 +796,7 @@ class FaceTableBoundary {
     if (dot(normal, boxCenter - onFace) < 0.0) normal *= -1.0;
     return true;
   }
-  bool reflectParticleOnBoxFace(Particle &p, const CartesianGrid::Face &face) const {
+  bool reflectParticleOnBoxFace(Particle &p, const typename Grid::Face &face) const {
     Vec3 normal;
     double faceScale = 0.0;
     if (!getInwardBoxFaceNormalIfClose(face, p.location, normal, faceScale)) return false;
@@ -694,7 +810,7 @@ class FaceTableBoundary {
     return true;
   }
 
-  const CartesianGrid &grid_;
+  const Grid &grid_;
   storm_boundary_desc desc_;
   std::vector<double> bounds_;
   Rng re_, faceRe_;
@@ -844,9 +960,10 @@ class RandomWalk {
 
 // ------------------------------------------------------------------ comb
 // CombPopulationControl::activate (population/CombPopulationControl.hpp:79-253)
+template <typename Grid>
 class Comb {
  public:
-  Comb(const CartesianGrid &g, size_t Nmin, double factor, uint64_t seed)
+  Comb(const Grid &g, size_t Nmin, double factor, uint64_t seed)
       : grid_(g), Nmin_(Nmin), factor_(factor), gen_(mixSeed(seed, 13)) {}
 
   std::vector<Particle> activate(const std::vector<Particle> &particles) {
@@ -926,7 +1043,7 @@ class Comb {
   }
 
  private:
-  const CartesianGrid &grid_;
+  const Grid &grid_;
   size_t Nmin_;
   double factor_;
   std::mt19937_64 gen_;
@@ -971,6 +1088,7 @@ inline std::vector<size_t> photonAllocation(const std::vector<double> &E, double
   return out;
 }
 
+template <typename Grid>
 class RadiationIMC {
  public:
   struct DDMCFaceLeak {
@@ -985,7 +1103,7 @@ class RadiationIMC {
     std::vector<DDMCFaceLeak> faceLeaks;
   };
 
-  RadiationIMC(const CartesianGrid &grid, FaceTableBoundary *boundary, std::vector<Cell> &cells,
+  RadiationIMC(const Grid &grid, FaceTableBoundary<Grid> *boundary, std::vector<Cell> &cells,
                std::vector<Extensives> &ext, const Material &mat, IMCParams params, uint64_t seed)
       : grid_(grid), boundary_(boundary), cells_(cells), ext_(ext), mat_(mat), params_(params),
         rng_(seed), sampler_(grid) {
@@ -1002,7 +1120,9 @@ class RadiationIMC {
     facePts_.assign(N, {});
     for (size_t i = 0; i < N; i++) {
       const auto &faces = grid_.GetCellFaces(i);
-      for (int q = 0; q < 6; ++q) {
+      normals_[i].resize(faces.size());
+      facePts_[i].resize(faces.size());
+      for (size_t q = 0; q < faces.size(); ++q) {
         size_t f = faces[q];
         Vec3 n = grid_.Normal(f);
         if (dot(n, grid_.GetMeshPoint(i) - grid_.FaceCM(f)) < 0) n *= -1;
@@ -1124,7 +1244,7 @@ class RadiationIMC {
     size_t minFace = std::numeric_limits<size_t>::max();
     double velAbs = kEpsilonParticle * norm(p.velocity);
     const auto &faces = grid_.GetCellFaces(p.cellIndex);
-    for (int q = 0; q < 6; ++q) {
+    for (size_t q = 0; q < normals_[p.cellIndex].size(); ++q) {
       const Vec3 &n = normals_[p.cellIndex][q];
       double nv = dot(n, p.velocity);
       if (nv >= -velAbs) continue;
@@ -1260,7 +1380,7 @@ class RadiationIMC {
 
   double minDistanceToFaces(size_t c, const Vec3 &x) const {  // RadiationIMC.hpp:956-969
     double Ro = std::numeric_limits<double>::max();
-    for (int q = 0; q < 6; ++q) Ro = std::min(Ro, dot(x - facePts_[c][q], normals_[c][q]));
+    for (size_t q = 0; q < normals_[c].size(); ++q) Ro = std::min(Ro, dot(x - facePts_[c][q], normals_[c][q]));
     return (Ro > 0.0) ? Ro : 0.0;
   }
   double surfaceArea(size_t c) const {
@@ -1391,12 +1511,12 @@ class RadiationIMC {
     p.location = center + disp * posDir;
     constexpr double nudge = 1e-6;
     p.location = p.location * (1.0 - nudge) + nudge * grid_.GetMeshPoint(c);
-    for (int q = 0; q < 6; ++q) {
+    for (size_t q = 0; q < normals_[c].size(); ++q) {
       double d = dot(p.location - facePts_[c][q], normals_[c][q]);
       if (d < 0.0) {
         disp *= 0.99;
         p.location = center + disp * posDir;
-        q = -1;
+        q = static_cast<size_t>(-1);
       }
     }
     p.velocity = isotropicVelocity(rng_);
@@ -1588,15 +1708,15 @@ class RadiationIMC {
     return true;
   }
 
-  const CartesianGrid &grid_;
-  FaceTableBoundary *boundary_;
+  const Grid &grid_;
+  FaceTableBoundary<Grid> *boundary_;
   std::vector<Cell> &cells_;
   std::vector<Extensives> &ext_;
   const Material &mat_;
   IMCParams params_;
   Rng rng_;
-  RandomInCellSampler sampler_;
-  std::vector<std::array<Vec3, 6>> normals_, facePts_;
+  RandomInCellSampler<Grid> sampler_;
+  std::vector<std::vector<Vec3>> normals_, facePts_;
   std::vector<double> factorFleck_, planck_, Erad_time_avg_, Eemit_;
   std::vector<size_t> nPhotons_;
   std::unique_ptr<RandomWalk> rw_;
@@ -1616,9 +1736,10 @@ class RadiationIMC {
 // MonteCarloManagerSerial (manager/MonteCarloManagerSerial.hpp:25-469).  The
 // free-list buffers (av/th) only decide memory reuse; the history loop is
 // restated over a plain vector in the same order.
+template <typename Grid>
 class Manager {
  public:
-  Manager(const CartesianGrid &g, RadiationIMC &phys, Comb *comb, FaceTableBoundary &bc)
+  Manager(const Grid &g, RadiationIMC<Grid> &phys, Comb<Grid> *comb, FaceTableBoundary<Grid> &bc)
       : grid_(g), phys_(phys), comb_(comb), bc_(bc) {}
 
   std::vector<size_t> cellsStepsCounters;
@@ -1683,10 +1804,10 @@ class Manager {
   const std::vector<Particle> &lastPreCombCensus() const { return lastRemaining_; }
 
  private:
-  const CartesianGrid &grid_;
-  RadiationIMC &phys_;
-  Comb *comb_;
-  FaceTableBoundary &bc_;
+  const Grid &grid_;
+  RadiationIMC<Grid> &phys_;
+  Comb<Grid> *comb_;
+  FaceTableBoundary<Grid> &bc_;
   size_t idCounter_ = 0;
   std::vector<Particle> lastRemaining_;
 };

@@ -43,6 +43,94 @@ Material makeMaterial(const storm_material_desc &d, size_t N, const storm_group_
 }
 }  // namespace
 
+namespace {
+// Per-step ledger layout (doubles): emitted, source, escaped, clamp,
+// censusIn, censusOut, censusComb, materialBefore, materialAfter, segments,
+// startParticles, endParticles, seconds
+constexpr size_t kLedgerStride = 13;
+
+template <typename Grid>
+int run_schedule_on(const Grid &grid, const storm_problem *prob, int nsteps, const double *dts, const double *source_T,
+                    double *T_out, double *e_out, double *erad_out, double *census_out, double *segments_out,
+                    double *ledger_out) {
+  try {
+    size_t N = grid.GetPointNo();
+    Material mat = makeMaterial(prob->material, N, &prob->groups);
+    std::vector<Cell> cells(N);
+    std::vector<Extensives> ext(N);
+    for (size_t i = 0; i < N; ++i) {
+      double rho = mat.density(i), V = grid.GetVolume(i);
+      cells[i].temperature = prob->T0[i];
+      cells[i].internalEnergy = mat.eFromT(prob->T0[i], rho) * V;
+      ext[i].mass = rho * V;
+      ext[i].internal_energy = cells[i].internalEnergy;
+    }
+    storm_boundary_desc bd = prob->boundary;
+    FaceTableBoundary<Grid> bc(grid, bd, prob->seed);
+    if (mat.multigroup()) bc.setGroupBounds(mat.bounds);
+    IMCParams ip;
+    ip.newPhotonsPerCell = prob->imc.new_photons_per_cell;
+    ip.withRandomWalk = prob->imc.with_random_walk != 0;
+    ip.withDDMC = prob->imc.with_ddmc != 0;
+    ip.rwMinCellOpticalDepth = prob->imc.rw_min_cell_optical_depth;
+    ip.rwMinParticleOpticalDepth = prob->imc.rw_min_particle_optical_depth;
+    ip.ddmcMinCellOpticalDepth = prob->imc.ddmc_min_cell_optical_depth;
+    ip.ddmcMinParticleOpticalDepth = prob->imc.ddmc_min_particle_optical_depth;
+    ip.ddmcUseMultigroupPGRW = prob->imc.ddmc_multigroup_pgrw != 0;
+    RadiationIMC<Grid> imc(grid, &bc, cells, ext, mat, ip, prob->seed);
+    std::unique_ptr<Comb<Grid>> comb;
+    if (prob->imc.comb) comb = std::make_unique<Comb<Grid>>(grid, prob->imc.comb_nmin, prob->imc.comb_factor, prob->seed);
+    Manager<Grid> mgr(grid, imc, comb.get(), bc);
+    std::vector<Particle> census;
+    for (int s = 0; s < nsteps; ++s) {
+      auto t0 = std::chrono::steady_clock::now();
+      double censusIn = 0, matBefore = 0;
+      for (const Particle &p : census) censusIn += p.weight;
+      for (size_t i = 0; i < N; ++i) matBefore += ext[i].internal_energy;
+      imc.ledger = EnergyLedger{};
+      bc.escapedEnergy = 0;
+      bc.sourceEnergy = 0;
+      if (source_T) bc.setSourceTemperature(source_T[s]);
+      census = mgr.step(std::move(census), dts[s]);
+      auto t1 = std::chrono::steady_clock::now();
+      double censusOut = 0, censusComb = 0, matAfter = 0, segs = 0;
+      for (const Particle &p : mgr.lastPreCombCensus()) censusOut += p.weight;
+      for (const Particle &p : census) censusComb += p.weight;
+      for (size_t i = 0; i < N; ++i) matAfter += ext[i].internal_energy;
+      for (size_t i = 0; i < N; ++i) segs += static_cast<double>(mgr.cellsStepsCounters[i]);
+      if (ledger_out) {
+        double *L = ledger_out + static_cast<size_t>(s) * kLedgerStride;
+        L[0] = imc.ledger.emitted;
+        L[1] = bc.sourceEnergy;
+        L[2] = bc.escapedEnergy;
+        L[3] = imc.ledger.clamp;
+        L[4] = censusIn;
+        L[5] = censusOut;
+        L[6] = censusComb;
+        L[7] = matBefore;
+        L[8] = matAfter;
+        L[9] = segs;
+        L[10] = static_cast<double>(mgr.startParticleCount);
+        L[11] = static_cast<double>(mgr.endParticleCount);
+        L[12] = std::chrono::duration<double>(t1 - t0).count();
+      }
+    }
+    for (size_t i = 0; i < N; ++i) {
+      if (T_out) T_out[i] = cells[i].temperature;
+      if (e_out) e_out[i] = ext[i].internal_energy;
+      if (erad_out) erad_out[i] = imc.eradTimeAvg()[i];
+      if (census_out) census_out[i] = ext[i].Erad;
+      if (segments_out) segments_out[i] = static_cast<double>(mgr.cellsStepsCounters[i]);
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 // Per-step ledger layout (doubles): emitted, source, escaped, clamp,
@@ -75,76 +163,14 @@ int oracle_run_schedule(const storm_problem *prob, int nsteps, const double *dts
                         double *T_out, double *e_out, double *erad_out, double *census_out,
                         double *segments_out, double *ledger_out) {
   try {
+    if (prob->unstructured) {
+      UnstructuredGrid grid(*prob->unstructured);
+      return run_schedule_on(grid, prob, nsteps, dts, source_T, T_out, e_out, erad_out, census_out, segments_out,
+                             ledger_out);
+    }
     CartesianGrid grid = makeGrid(prob->mesh);
-    size_t N = grid.GetPointNo();
-    Material mat = makeMaterial(prob->material, N, &prob->groups);
-    std::vector<Cell> cells(N);
-    std::vector<Extensives> ext(N);
-    for (size_t i = 0; i < N; ++i) {
-      double rho = mat.density(i), V = grid.GetVolume(i);
-      cells[i].temperature = prob->T0[i];
-      cells[i].internalEnergy = mat.eFromT(prob->T0[i], rho) * V;
-      ext[i].mass = rho * V;
-      ext[i].internal_energy = cells[i].internalEnergy;
-    }
-    storm_boundary_desc bd = prob->boundary;
-    FaceTableBoundary bc(grid, bd, prob->seed);
-    if (mat.multigroup()) bc.setGroupBounds(mat.bounds);
-    IMCParams ip;
-    ip.newPhotonsPerCell = prob->imc.new_photons_per_cell;
-    ip.withRandomWalk = prob->imc.with_random_walk != 0;
-    ip.withDDMC = prob->imc.with_ddmc != 0;
-    ip.rwMinCellOpticalDepth = prob->imc.rw_min_cell_optical_depth;
-    ip.rwMinParticleOpticalDepth = prob->imc.rw_min_particle_optical_depth;
-    ip.ddmcMinCellOpticalDepth = prob->imc.ddmc_min_cell_optical_depth;
-    ip.ddmcMinParticleOpticalDepth = prob->imc.ddmc_min_particle_optical_depth;
-    ip.ddmcUseMultigroupPGRW = prob->imc.ddmc_multigroup_pgrw != 0;
-    RadiationIMC imc(grid, &bc, cells, ext, mat, ip, prob->seed);
-    std::unique_ptr<Comb> comb;
-    if (prob->imc.comb) comb = std::make_unique<Comb>(grid, prob->imc.comb_nmin, prob->imc.comb_factor, prob->seed);
-    Manager mgr(grid, imc, comb.get(), bc);
-    std::vector<Particle> census;
-    for (int s = 0; s < nsteps; ++s) {
-      auto t0 = std::chrono::steady_clock::now();
-      double censusIn = 0, matBefore = 0;
-      for (const Particle &p : census) censusIn += p.weight;
-      for (size_t i = 0; i < N; ++i) matBefore += ext[i].internal_energy;
-      imc.ledger = EnergyLedger{};
-      bc.escapedEnergy = 0;
-      bc.sourceEnergy = 0;
-      if (source_T) bc.setSourceTemperature(source_T[s]);
-      census = mgr.step(std::move(census), dts[s]);
-      auto t1 = std::chrono::steady_clock::now();
-      double censusOut = 0, censusComb = 0, matAfter = 0, segs = 0;
-      for (const Particle &p : mgr.lastPreCombCensus()) censusOut += p.weight;
-      for (const Particle &p : census) censusComb += p.weight;
-      for (size_t i = 0; i < N; ++i) matAfter += ext[i].internal_energy;
-      for (size_t i = 0; i < N; ++i) segs += static_cast<double>(mgr.cellsStepsCounters[i]);
-      if (ledger_out) {
-        double *L = ledger_out + static_cast<size_t>(s) * ORACLE_LEDGER_STRIDE;
-        L[0] = imc.ledger.emitted;
-        L[1] = bc.sourceEnergy;
-        L[2] = bc.escapedEnergy;
-        L[3] = imc.ledger.clamp;
-        L[4] = censusIn;
-        L[5] = censusOut;
-        L[6] = censusComb;
-        L[7] = matBefore;
-        L[8] = matAfter;
-        L[9] = segs;
-        L[10] = static_cast<double>(mgr.startParticleCount);
-        L[11] = static_cast<double>(mgr.endParticleCount);
-        L[12] = std::chrono::duration<double>(t1 - t0).count();
-      }
-    }
-    for (size_t i = 0; i < N; ++i) {
-      if (T_out) T_out[i] = cells[i].temperature;
-      if (e_out) e_out[i] = ext[i].internal_energy;
-      if (erad_out) erad_out[i] = imc.eradTimeAvg()[i];
-      if (census_out) census_out[i] = ext[i].Erad;
-      if (segments_out) segments_out[i] = static_cast<double>(mgr.cellsStepsCounters[i]);
-    }
-    return 0;
+    return run_schedule_on(grid, prob, nsteps, dts, source_T, T_out, e_out, erad_out, census_out, segments_out,
+                           ledger_out);
   } catch (const std::exception &e) {
     g_err = e.what();
     return 1;
@@ -170,7 +196,7 @@ int oracle_prestep(const storm_problem *prob, double dt, double *sigma_out, doub
     }
     IMCParams ip;
     ip.newPhotonsPerCell = prob->imc.new_photons_per_cell;
-    RadiationIMC imc(grid, nullptr, cells, ext, mat, ip, prob->seed);
+    RadiationIMC<CartesianGrid> imc(grid, nullptr, cells, ext, mat, ip, prob->seed);
     imc.preStep(dt);
     for (size_t i = 0; i < N; ++i) {
       if (sigma_out) sigma_out[i] = imc.planckOpacities()[i];
@@ -206,7 +232,7 @@ struct OracleState {
   Material mat;
   std::vector<Cell> cells;
   std::vector<Extensives> ext;
-  std::unique_ptr<RadiationIMC> imc;
+  std::unique_ptr<RadiationIMC<CartesianGrid>> imc;
   OracleState(const storm_problem *prob, double dt)
       : grid(makeGrid(prob->mesh)), mat(makeMaterial(prob->material, grid.GetPointNo(), &prob->groups)) {
     const size_t N = grid.GetPointNo();
@@ -226,7 +252,7 @@ struct OracleState {
     ip.ddmcMinCellOpticalDepth = prob->imc.ddmc_min_cell_optical_depth;
     ip.ddmcMinParticleOpticalDepth = prob->imc.ddmc_min_particle_optical_depth;
     ip.ddmcUseMultigroupPGRW = prob->imc.ddmc_multigroup_pgrw != 0;
-    imc = std::make_unique<RadiationIMC>(grid, nullptr, cells, ext, mat, ip, prob->seed);
+    imc = std::make_unique<RadiationIMC<CartesianGrid>>(grid, nullptr, cells, ext, mat, ip, prob->seed);
     imc->preStep(dt);
   }
 };
@@ -339,7 +365,7 @@ int oracle_comb(const storm_problem *prob, int64_t n, const int64_t *cell, const
       ps[i].location = grid.GetCellCM(ps[i].cellIndex);
       ps[i].weight = weight[i];
     }
-    Comb comb(grid, prob->imc.comb_nmin, prob->imc.comb_factor, prob->seed);
+    Comb<CartesianGrid> comb(grid, prob->imc.comb_nmin, prob->imc.comb_factor, prob->seed);
     std::vector<Particle> out = comb.activate(ps);
     for (size_t i = 0; i < N; ++i) {
       count_out[i] = 0;

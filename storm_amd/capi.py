@@ -57,9 +57,27 @@ class GroupDesc(C.Structure):
                 ("nu_pow", C.c_double), ("nu_ref", C.c_double)]
 
 
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+
+
+class UnstructuredMesh(C.Structure):
+    _fields_ = [("ncells", C.c_uint32), ("nfaces", C.c_uint32), ("box_lo", C.c_double * 3),
+                ("box_hi", C.c_double * 3), ("volume", _dp), ("mesh_point", _dp), ("face_start", _u32p),
+                ("face_neighbor", _i32p), ("face_normal", _dp), ("face_centroid", _dp), ("face_area", _dp),
+                ("face_vertex_start", _u32p), ("vertices", _dp), ("cell_instance", _u32p)]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [("source_temperature", C.c_double), ("source_packets_per_face", C.c_uint64),
+                ("new_photons_per_cell", C.c_uint64), ("comb_nmin", C.c_uint64), ("comb_factor", C.c_double)]
+
+
 class Problem(C.Structure):
     _fields_ = [("mesh", MeshDesc), ("material", MaterialDesc), ("boundary", BoundaryDesc),
-                ("imc", ImcParams), ("T0", _dp), ("seed", C.c_uint64), ("groups", GroupDesc)]
+                ("imc", ImcParams), ("T0", _dp), ("seed", C.c_uint64), ("groups", GroupDesc),
+                ("unstructured", C.POINTER(UnstructuredMesh)), ("num_instances", C.c_uint32),
+                ("reserved2", C.c_uint32), ("instances", C.POINTER(InstanceDesc))]
 
 
 class PacketRecord(C.Structure):
@@ -143,6 +161,9 @@ def load_library(path: str | None = None):
         "storm_gpu_debug_accel": ([vp, vp, C.POINTER(C.c_uint64)], C.c_int),
         "storm_gpu_debug_spectrum": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_double), vp, vp, vp], C.c_int),
         "storm_gpu_calibration_copy": ([C.c_uint64, C.POINTER(C.c_double)], C.c_int),
+        "storm_voronoi_build": ([_dp, C.c_uint32, _dp, _dp, C.POINTER(vp)], C.c_int),
+        "storm_voronoi_mesh": ([vp, C.POINTER(UnstructuredMesh)], C.c_int),
+        "storm_voronoi_free": ([vp], C.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(lib, name)
@@ -154,9 +175,10 @@ def load_library(path: str | None = None):
 
 
 def exported_symbols_declared() -> list[str]:
-    """Function names declared in include/storm_gpu.h."""
+    """Function names declared in the library's C headers (include/storm_gpu.h, include/storm_mesh.h)."""
     import re
-    hdr = os.path.join(HERE, "..", "include", "storm_gpu.h")
-    with open(hdr) as f:
-        text = f.read()
-    return sorted(set(re.findall(r"\b(storm_gpu_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in ("storm_gpu.h", "storm_mesh.h"):
+        with open(os.path.join(HERE, "..", "include", h)) as f:
+            names |= set(re.findall(r"\b(storm_(?:gpu|voronoi)_[a-z0-9_]+)\s*\(", f.read()))
+    return sorted(names)

@@ -22,7 +22,11 @@
 #include <rocprim/rocprim.hpp>
 
 #include "../../include/storm_gpu.h"
+#include "../../include/storm_mesh.h"
 #include "kernels.h"
+#include "kernels_unstructured.h"
+#include "common/umesh_tables.h"
+#include "voronoi_build.h"
 
 namespace storm {
 namespace gpu {
@@ -318,38 +322,85 @@ struct Engine final : EngineBase {
   PacketBufs<Real> pa, pb;
   CensusBufs<Real> cpre, cnext;
 
+  // polyhedral meshes (storm_unstructured_mesh): host tables and their device copies
+  bool unstructured = false;
+  double box_lo[3] = {0, 0, 0}, box_hi[3] = {1, 1, 1};
+  umesh::Tables<Real> ut;
+  std::vector<double> gen_h;  // mesh points, 3 per cell
+  std::vector<double> chord_h;  // mean chord 4 V / S per cell
+  DevBuf<double> chord;
+  DevBuf<uint32_t> u_face_start, u_tet_start;
+  DevBuf<UFace<Real>> u_face;
+  DevBuf<Real> u_face_centroid, u_ddmc_rate;
+  DevBuf<double2> u_face_geo;
+  DevBuf<UTet<Real>> u_tets;
+  DevBuf<USource> u_src;
+  DevBuf<UTri<Real>> u_tris;
+  // batches of independent instances (cell_instance)
+  std::vector<storm_instance_desc> inst;
+  std::vector<uint32_t> cell_inst_h;
+  std::vector<double> inst_T4;  // sources' T^4 per instance (frozen at the first step if asked)
+  uint32_t last_nfaces_ = 0;    // source faces of the last step
+
   // kept transport inputs of the last step (debug / replay)
   std::vector<Packet<Real>> kept_packets;
   std::vector<CellXS<Real>> kept_xs;
   StepConsts kept_consts{};
 
   Engine(const storm_problem *p, int32_t prec, int32_t mmode) : precision(prec), material_mode(mmode) {
-    nx = p->mesh.nx;
-    ny = p->mesh.ny;
-    nz = p->mesh.nz;
-    if (nx < 1 || ny < 1 || nz < 1 || nx > 4096 || ny > 1024 || nz > 1024)
-      throw EngineError(1, "mesh dimensions must be 1..4096 x 1..1024 x 1..1024");
-    // cell indices stay below 2^31 in every tally offset (2 c + 1)
-    if (static_cast<uint64_t>(nx) * ny * nz > (1ull << 30)) throw EngineError(1, "more than 2^30 cells");
-    N = static_cast<uint32_t>(nx) * ny * nz;
-    {
-      const int32_t n3[3] = {nx, ny, nz};
-      morton.total = 0;
-      for (int a = 0; a < 3; ++a) {
-        uint32_t b = 0;
-        while ((1ll << b) < n3[a]) ++b;
-        morton.bits[a] = b;
-        morton.total += b;
-      }
-      // the comb's keys hold the code and one more value (dropped packets)
+    if (p->unstructured) {
+      // polyhedral cells, numbered in Z order already: the per-cell kernels
+      // run over a flat nx = N x 1 x 1 layout and packets carry cell indices
+      const storm_unstructured_mesh &um = *p->unstructured;
+      unstructured = true;
+      if (um.ncells < 1 || um.ncells > (1u << 30)) throw EngineError(1, "unstructured mesh: 1 .. 2^30 cells");
+      nx = static_cast<int32_t>(um.ncells);
+      ny = nz = 1;
+      N = um.ncells;
+      uint32_t b = 0;
+      while ((1ull << b) < N) ++b;
+      morton.bits[0] = b;
+      morton.bits[1] = morton.bits[2] = 0;
+      morton.total = b;
+      morton.flat = 1;
       if (morton.total > 31) throw EngineError(1, "mesh too large for 32-bit Z-order cell codes");
+      // axis "edges" only for the engine's Cartesian bookkeeping (unused)
+      xe = {0.0, 1.0};
+      ye = {0.0, 1.0};
+      ze = {0.0, 1.0};
+      for (int a = 0; a < 3; ++a) {
+        box_lo[a] = um.box_lo[a];
+        box_hi[a] = um.box_hi[a];
+      }
+    } else {
+      nx = p->mesh.nx;
+      ny = p->mesh.ny;
+      nz = p->mesh.nz;
+      if (nx < 1 || ny < 1 || nz < 1 || nx > 4096 || ny > 1024 || nz > 1024)
+        throw EngineError(1, "mesh dimensions must be 1..4096 x 1..1024 x 1..1024");
+      // cell indices stay below 2^31 in every tally offset (2 c + 1)
+      if (static_cast<uint64_t>(nx) * ny * nz > (1ull << 30)) throw EngineError(1, "more than 2^30 cells");
+      N = static_cast<uint32_t>(nx) * ny * nz;
+      {
+        const int32_t n3[3] = {nx, ny, nz};
+        morton.total = 0;
+        morton.flat = 0;
+        for (int a = 0; a < 3; ++a) {
+          uint32_t b = 0;
+          while ((1ll << b) < n3[a]) ++b;
+          morton.bits[a] = b;
+          morton.total += b;
+        }
+        // the comb's keys hold the code and one more value (dropped packets)
+        if (morton.total > 31) throw EngineError(1, "mesh too large for 32-bit Z-order cell codes");
+      }
+      xe.assign(p->mesh.x_edges, p->mesh.x_edges + nx + 1);
+      ye.assign(p->mesh.y_edges, p->mesh.y_edges + ny + 1);
+      ze.assign(p->mesh.z_edges, p->mesh.z_edges + nz + 1);
+      for (auto *e : {&xe, &ye, &ze})
+        for (size_t q = 1; q < e->size(); ++q)
+          if (!((*e)[q] > (*e)[q - 1])) throw EngineError(1, "mesh edges must be strictly increasing");
     }
-    xe.assign(p->mesh.x_edges, p->mesh.x_edges + nx + 1);
-    ye.assign(p->mesh.y_edges, p->mesh.y_edges + ny + 1);
-    ze.assign(p->mesh.z_edges, p->mesh.z_edges + nz + 1);
-    for (auto *e : {&xe, &ye, &ze})
-      for (size_t q = 1; q < e->size(); ++q)
-        if (!((*e)[q] > (*e)[q - 1])) throw EngineError(1, "mesh edges must be strictly increasing");
     matd = p->material;
     bnd = p->boundary;
     imc = p->imc;
@@ -378,11 +429,54 @@ struct Engine final : EngineBase {
     if (!p->material.sigma_a0 && material_mode == STORM_GPU_MATERIAL_DEVICE)
       throw EngineError(1, "material.sigma_a0 is required");
     source_T = bnd.source_temperature;
+    if (p->num_instances && !unstructured)
+      throw EngineError(1, "batched instances need an unstructured mesh with cell_instance");
     Vh.resize(N);
-    for (int i = 0; i < nx; ++i)
-      for (int j = 0; j < ny; ++j)
-        for (int k = 0; k < nz; ++k)
-          Vh[(static_cast<size_t>(i) * ny + j) * nz + k] = (xe[i + 1] - xe[i]) * (ye[j + 1] - ye[j]) * (ze[k + 1] - ze[k]);
+    chord_h.resize(N);
+    if (unstructured) {
+      const storm_unstructured_mesh &um = *p->unstructured;
+      if (p->num_instances) {
+        if (!um.cell_instance || !p->instances) throw EngineError(1, "instances need cell_instance and instances");
+        if (groups >= 2) throw EngineError(1, "batched instances are gray");
+        inst.assign(p->instances, p->instances + p->num_instances);
+        inst_T4.assign(inst.size(), 0.0);
+        cell_inst_h.assign(um.cell_instance, um.cell_instance + N);
+        for (uint32_t c = 0; c < N; ++c)
+          if (cell_inst_h[c] >= p->num_instances) throw EngineError(1, "cell_instance out of range");
+        for (const storm_instance_desc &d : inst)
+          if (d.new_photons_per_cell == 0) throw EngineError(1, "every instance needs new_photons_per_cell > 0");
+      } else if (um.cell_instance) {
+        throw EngineError(1, "cell_instance given without instances");
+      }
+      try {
+        ut = umesh::build<Real>(um, bnd.source_face == STORM_XLO, bnd.source_xhi != 0, [&](uint32_t c) -> uint64_t {
+          return inst.empty() ? bnd.source_packets_per_face : inst[cell_inst_h[c]].source_packets_per_face;
+        });
+      } catch (const std::exception &e) {
+        throw EngineError(1, e.what());
+      }
+      if (!inst.empty())
+        for (uint32_t c = 0; c < N; ++c)
+          for (uint32_t f = ut.face_start[c]; f < ut.face_start[c + 1]; ++f) {
+            const int32_t nb = ut.face[f].nbr;
+            if (nb >= 0 && cell_inst_h[static_cast<uint32_t>(nb)] != cell_inst_h[c])
+              throw EngineError(1, "cells of different instances share a face");
+          }
+      for (uint32_t c = 0; c < N; ++c) {
+        Vh[c] = um.volume[c];
+        chord_h[c] = ut.chord[c];
+      }
+      gen_h.assign(um.mesh_point, um.mesh_point + 3 * static_cast<size_t>(N));
+    } else {
+      for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+          for (int k = 0; k < nz; ++k) {
+            const size_t c = (static_cast<size_t>(i) * ny + j) * nz + k;
+            const double dx = xe[i + 1] - xe[i], dy = ye[j + 1] - ye[j], dz = ze[k + 1] - ze[k];
+            Vh[c] = dx * dy * dz;
+            chord_h[c] = 4.0 * Vh[c] / (2.0 * (dy * dz + dx * dz + dx * dy));
+          }
+    }
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&ev0));
     HIPCHECK(hipEventCreate(&ev1));
@@ -401,6 +495,24 @@ struct Engine final : EngineBase {
       HIPCHECK(hipMemcpy(d.p, src, n * sizeof(double), hipMemcpyHostToDevice));
     };
     up(V, Vh.data(), N);
+    up(chord, chord_h.data(), N);
+    if (unstructured) {
+      auto upv = [&](auto &d, const auto &h) {
+        d.ensure(std::max<size_t>(h.size(), 1));
+        if (!h.empty()) HIPCHECK(hipMemcpy(d.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
+      };
+      upv(u_face_start, ut.face_start);
+      upv(u_face, ut.face);
+      upv(u_face_centroid, ut.face_centroid);
+      std::vector<double2> geo(ut.face.size());
+      for (size_t f = 0; f < geo.size(); ++f) geo[f] = make_double2(ut.face_geo[2 * f], ut.face_geo[2 * f + 1]);
+      upv(u_face_geo, geo);
+      upv(u_tet_start, ut.tet_start);
+      upv(u_tets, ut.tets);
+      upv(u_src, ut.src);
+      upv(u_tris, ut.tris);
+      if (imc.with_ddmc) u_ddmc_rate.ensure(std::max<size_t>(ut.face.size(), 1));
+    }
     if (p->material.sigma_a0) up(sa0, p->material.sigma_a0, N);
     if (p->material.sigma_s0) up(ss0, p->material.sigma_s0, N);
     if (p->material.density) up(rho, p->material.density, N);
@@ -434,7 +546,7 @@ struct Engine final : EngineBase {
     HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&mail_h), 8 * sizeof(unsigned long long)));
     // boundary source faces: at most both x sides of ny * nz faces
     {
-      const size_t max_faces = 2 * static_cast<size_t>(ny) * nz;
+      const size_t max_faces = std::max<size_t>(unstructured ? ut.src.size() : 2 * static_cast<size_t>(ny) * nz, 1);
       wface.ensure(max_faces);
       HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&wface_h), max_faces * sizeof(double)));
     }
@@ -587,7 +699,10 @@ struct Engine final : EngineBase {
                                         : MathPolicy<Real>::inf();
     return a;
   }
-  MeshDims dims() const { return MeshDims{static_cast<uint32_t>(nx), static_cast<uint32_t>(ny), static_cast<uint32_t>(nz)}; }
+  MeshDims dims() const {
+    return MeshDims{static_cast<uint32_t>(nx), static_cast<uint32_t>(ny), static_cast<uint32_t>(nz),
+                    unstructured ? 1u : 0u};
+  }
   MeshView<Real> meshview() const {
     MeshView<Real> m;
     m.nx = nx;
@@ -655,9 +770,45 @@ struct Engine final : EngineBase {
 
   // One transport launch on the kernel variant compiled for this problem's
   // features (random walk / DDMC, multigroup).
+  UMeshView<Real> umeshview() const {
+    UMeshView<Real> m;
+    m.ncells = N;
+    m.escape_mask = 0u;
+    for (int f = 0; f < 6; ++f)
+      if (bnd.face_kind[f] == STORM_FACE_ESCAPE) m.escape_mask |= 1u << f;
+    m.face_start = u_face_start.p;
+    m.face = u_face.p;
+    m.face_centroid = u_face_centroid.p;
+    m.ddmc_rate = imc.with_ddmc ? u_ddmc_rate.p : nullptr;
+    m.spec = spectrum();
+    return m;
+  }
+  // Tally-cache slots of the polyhedral-mesh transport at this block size.
+  uint32_t pass_slots_cache_u[4][4] = {};
+  template <uint32_t F>
+  uint32_t pass_cache_slots_u(int32_t b, uint32_t nf) {
+    uint32_t &r = pass_slots_cache_u[F][b == 64 ? 0 : (b == 128 ? 1 : (b == 256 ? 2 : 3))];
+    if (!r) {
+      int per_cu = 0;
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<Real, F, UMeshView<Real>>, b, 0));
+      r = cache_slots(static_cast<uint32_t>(b) * kChunksPerWave, nf, static_cast<uint32_t>(std::max(per_cu, 1)));
+    }
+    return r;
+  }
+
   template <uint32_t F>
   void launch_transport_f(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshView<Real> &mv,
                           const AccelView<Real> &av, const StepConsts &kc) {
+    if (unstructured) {
+      // one packet per lane and pass (the refill schedule is Cartesian only)
+      const uint32_t nf = (precision == STORM_GPU_MIXED && groups < 2) ? 1u : 2u;
+      const uint32_t slots = pass_cache_slots_u<F>(block, nf);
+      const uint32_t chunks = kChunksPerWave * static_cast<uint32_t>(block / 64);
+      k_transport<Real, F, UMeshView<Real>><<<blocks_for(n_in, 64u * chunks), block, cache_bytes(slots, nf), stream>>>(
+          static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1,
+          umeshview(), xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
+      return;
+    }
     if (schedule == STORM_GPU_SCHEDULE_REFILL) {
       const int32_t rb = std::min<int32_t>(block, 256);
       const uint32_t grid = std::min<uint32_t>(blocks_for(n_in, rb), resident_blocks<F>(rb));
@@ -766,7 +917,7 @@ struct Engine final : EngineBase {
       k_prestep_mg<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
           N, T.p, md, have_host_sa ? host_sa.p : nullptr, have_host_ss ? host_ss.p : nullptr,
           have_host_cv ? host_cv.p : nullptr, V.p, dt, groups, bounds_erg.p, nu_pow, nu_ref, xs.p, cdf.p,
-          sig_s_cell.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p, d, meshview(),
+          sig_s_cell.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p, chord.p,
           imc.with_random_walk ? imc.rw_min_cell_optical_depth : HUGE_VAL, accel_on ? accel.p : nullptr,
           (imc.with_ddmc && imc.ddmc_multigroup_pgrw) ? imc.ddmc_min_cell_optical_depth : -1.0,
           accel_on ? Dtil.p : nullptr);
@@ -776,12 +927,17 @@ struct Engine final : EngineBase {
           have_host_cv ? host_cv.p : nullptr, V.p, dt, xs.p, fleck.p, sig_a.p, sig_s.p, Eemit.p, err.p);
     }
     if (accel_on && (groups < 2 || imc.with_ddmc)) {
-      const MeshView<Real> mv0 = meshview();
       k_accel_cells<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
-          N, d, mv0, sig_a.p, sig_s.p, fleck.p, imc.with_random_walk, imc.rw_min_cell_optical_depth, imc.with_ddmc,
+          N, chord.p, sig_a.p, sig_s.p, fleck.p, imc.with_random_walk, imc.rw_min_cell_optical_depth, imc.with_ddmc,
           imc.ddmc_min_cell_optical_depth, groups >= 2 ? 1 : 0, imc.ddmc_multigroup_pgrw ? 1 : 0, Dtil.p,
           accel.p);
-      k_accel_rates<Real><<<blocks_for(N, 256), 256, 0, stream>>>(N, d, mv0, Dtil.p, accel.p);
+      if (unstructured) {
+        if (imc.with_ddmc)
+          k_accel_rates_u<Real><<<blocks_for(N, 256), 256, 0, stream>>>(N, u_face_start.p, u_face.p, u_face_geo.p, V.p,
+                                                                         Dtil.p, accel.p, u_ddmc_rate.p);
+      } else {
+        k_accel_rates<Real><<<blocks_for(N, 256), 256, 0, stream>>>(N, d, meshview(), Dtil.p, accel.p);
+      }
     }
     sum_async(Eemit.p, N, dsc.p + 2);  // emission total, read by k_quantum and k_alloc
 
@@ -794,28 +950,52 @@ struct Engine final : EngineBase {
     const uint32_t xhi_from = nsides;
     if (bnd.source_xhi) side_T[nsides++] = source_T_hi;
     const uint32_t npc = nsides ? static_cast<uint32_t>(bnd.source_packets_per_face) : 0u;
-    const uint32_t nfaces = npc ? per_side * nsides : 0u;
+    const uint32_t nfaces = unstructured ? static_cast<uint32_t>(ut.src.size()) : (npc ? per_side * nsides : 0u);
     double side_T4[2] = {0.0, 0.0};
     for (uint32_t s = 0; s < nsides; ++s) side_T4[s] = side_T[s] * side_T[s] * side_T[s] * side_T[s];
     if (bnd.freeze_source_T4) {
       if (!frozen) {
         for (uint32_t s = 0; s < nsides; ++s) frozenT4s[s] = side_T4[s];
+        for (size_t q = 0; q < inst.size(); ++q) {
+          const double Ti = inst[q].source_temperature;
+          inst_T4[q] = Ti * Ti * Ti * Ti;
+        }
         frozen = true;
       }
       for (uint32_t s = 0; s < nsides; ++s) side_T4[s] = frozenT4s[s];
+    } else {
+      for (size_t q = 0; q < inst.size(); ++q) {
+        const double Ti = inst[q].source_temperature;
+        inst_T4[q] = Ti * Ti * Ti * Ti;
+      }
     }
     // face weights, computed straight into the pinned staging buffer (sized
     // at creation; the stream is idle: the last step ended with a
     // synchronisation, so its copy from this buffer is done)
     double *wface_e = wface_h;
     double source_total = 0.0;
-    for (uint32_t f = 0; f < nfaces; ++f) {
-      const uint32_t s = f / per_side, jk = f % per_side;
-      uint32_t j = jk / nz, k = jk % nz;
-      double A = (ye[j + 1] - ye[j]) * (ze[k + 1] - ze[k]);
-      wface_e[f] = units::sigma_sb * side_T4[s] * A * dt / npc;
-      source_total += wface_e[f] * npc;
+    uint64_t n_bc = 0;
+    if (unstructured) {
+      // each source face: its side's (or its instance's) T^4, its own area
+      // and packets per step
+      for (uint32_t f = 0; f < nfaces; ++f) {
+        const USource &S = ut.src[f];
+        const double T4 = inst.empty() ? side_T4[S.xhi ? xhi_from : 0u] : inst_T4[ut.src_instance[f]];
+        wface_e[f] = units::sigma_sb * T4 * ut.src_area[f] * dt / S.count;
+        source_total += wface_e[f] * S.count;
+        n_bc += S.count;
+      }
+    } else {
+      for (uint32_t f = 0; f < nfaces; ++f) {
+        const uint32_t s = f / per_side, jk = f % per_side;
+        uint32_t j = jk / nz, k = jk % nz;
+        double A = (ye[j + 1] - ye[j]) * (ze[k + 1] - ze[k]);
+        wface_e[f] = units::sigma_sb * side_T4[s] * A * dt / npc;
+        source_total += wface_e[f] * npc;
+      }
+      n_bc = static_cast<uint64_t>(nfaces) * npc;
     }
+    last_nfaces_ = nfaces;
 
     // energy quantum of this step (every energy below 2^62 quanta), photon
     // allocation and offsets; one read-back for the quantum and the count
@@ -859,7 +1039,6 @@ struct Engine final : EngineBase {
       HIPCHECK(hipStreamSynchronize(stream));
     }
     if (nfaces) HIPCHECK(hipMemcpyAsync(wface.p, wface_h, nfaces * sizeof(double), hipMemcpyHostToDevice, stream));
-    const uint64_t n_bc = static_cast<uint64_t>(nfaces) * npc;
     const uint64_t n_total = n_census + n_emit + n_bc;
     // 32-bit packet indices; the refill queue head may run past the count by
     // one wavefront of slots per resident wavefront
@@ -887,11 +1066,20 @@ struct Engine final : EngineBase {
     if (n_census)
       k_import<Real><<<blocks_for(n_census, 256), 256, 0, stream>>>(
           static_cast<uint32_t>(n_census), cnext.soa(), pa.soa(), s0, eq_prev, eq, d, tal, requant.p);
-    if (n_emit)
+    if (n_emit && unstructured)
+      k_emit_u<Real><<<blocks_for(n_emit, 256), 256, 0, stream>>>(
+          static_cast<uint32_t>(n_emit), M, offsets.p, wemit.p, pa.soa(), static_cast<uint32_t>(n_census), s0, seed,
+          step_index, tal.begin, spectrum(), u_tets.p, u_tet_start.p);
+    else if (n_emit)
       k_emit<Real><<<blocks_for(n_emit, 256), 256, 0, stream>>>(
           static_cast<uint32_t>(n_emit), M, offsets.p, wemit.p, pa.soa(), static_cast<uint32_t>(n_census), s0, d, morton,
           seed, step_index, tal.begin, spectrum());
-    if (n_bc)
+    if (n_bc && unstructured)
+      k_boundary_u<Real><<<blocks_for(n_bc, 256), 256, 0, stream>>>(
+          static_cast<uint32_t>(n_bc), nfaces, u_src.p, u_tris.p, wface.p, pa.soa(),
+          static_cast<uint32_t>(n_census + n_emit), s0, seed, step_index, tal.begin, spectrum(),
+          groups >= 2 ? bcdf.p : nullptr, xhi_from);
+    else if (n_bc)
       k_boundary<Real><<<blocks_for(n_bc, 256), 256, 0, stream>>>(
           static_cast<uint32_t>(n_bc), npc, wface.p, pa.soa(), static_cast<uint32_t>(n_census + n_emit), s0, d, seed,
           step_index, tal.begin, spectrum(), groups >= 2 ? bcdf.p : nullptr, xhi_from);
@@ -1286,13 +1474,12 @@ struct Engine final : EngineBase {
     L->comb_rem = u128(sum(b + 6 * n2));
     // boundary source: Npc * w_face per face, exact
     unsigned __int128 src = 0;
-    const uint32_t sides = (bnd.source_face == STORM_XLO ? 1u : 0u) + (bnd.source_xhi ? 1u : 0u);
-    const uint32_t npc = sides ? static_cast<uint32_t>(bnd.source_packets_per_face) : 0u;
-    if (npc) {
-      std::vector<double> wf(static_cast<size_t>(ny) * nz * sides);
+    if (last_nfaces_) {
+      std::vector<double> wf(last_nfaces_);
       HIPCHECK(hipMemcpy(wf.data(), wface.p, wf.size() * sizeof(double), hipMemcpyDeviceToHost));
-      for (double w : wf) {
-        fx::Limbs l = fx::limbs_of(w, last_eq_);
+      for (uint32_t f = 0; f < last_nfaces_; ++f) {
+        const uint64_t npc = unstructured ? ut.src[f].count : bnd.source_packets_per_face;
+        fx::Limbs l = fx::limbs_of(wf[f], last_eq_);
         src += ((static_cast<unsigned __int128>(l.hi) << fx::kLoBits) + l.lo) * npc;
       }
     }
@@ -1356,8 +1543,23 @@ struct Engine final : EngineBase {
       HIPCHECK(hipMemcpy(hm.data(), cnext.meta.p, n * sizeof(uint2), hipMemcpyDeviceToHost));
     }
     for (uint64_t q = 0; q < n; ++q) {
-      uint32_t i = ijk_i(hm[q].x), j = ijk_j(hm[q].x), k = ijk_k(hm[q].x);
       storm_packet_record &r = out[q];
+      if (unstructured) {
+        // position relative to the cell's mesh point
+        const uint32_t c = hm[q].x;
+        r.x = gen_h[3 * static_cast<size_t>(c)] + static_cast<double>(hp[q].x);
+        r.y = gen_h[3 * static_cast<size_t>(c) + 1] + static_cast<double>(hp[q].y);
+        r.z = gen_h[3 * static_cast<size_t>(c) + 2] + static_cast<double>(hp[q].z);
+        r.vx = static_cast<double>(hd[q].x) * units::clight;
+        r.vy = static_cast<double>(hd[q].y) * units::clight;
+        r.vz = static_cast<double>(hd[q].z) * units::clight;
+        r.weight = hw[q];
+        r.frequency = groups >= 2 ? static_cast<double>(hp[q].w) * units::kev : 0.0;
+        r.cell = c;
+        r.id = ~0ull;
+        continue;
+      }
+      uint32_t i = ijk_i(hm[q].x), j = ijk_j(hm[q].x), k = ijk_k(hm[q].x);
       r.x = xe[i] + static_cast<double>(hp[q].x) * (xe[i + 1] - xe[i]);
       r.y = ye[j] + static_cast<double>(hp[q].y) * (ye[j + 1] - ye[j]);
       r.z = ze[k] + static_cast<double>(hp[q].z) * (ze[k + 1] - ze[k]);
@@ -1386,6 +1588,30 @@ struct Engine final : EngineBase {
     double tot = 0.0;
     for (uint64_t q = 0; q < n; ++q) {
       const storm_packet_record &r = in[q];
+      if (unstructured) {
+        if (r.cell >= N) throw EngineError(1, "census packet on a polyhedral mesh needs its cell index");
+        const uint32_t c = static_cast<uint32_t>(r.cell);
+        hp[q].x = static_cast<Real>(r.x - gen_h[3 * static_cast<size_t>(c)]);
+        hp[q].y = static_cast<Real>(r.y - gen_h[3 * static_cast<size_t>(c) + 1]);
+        hp[q].z = static_cast<Real>(r.z - gen_h[3 * static_cast<size_t>(c) + 2]);
+        hp[q].w = Real(0);
+        if (groups >= 2) {
+          const double E = std::min(std::max(r.frequency, bounds_h.front()), bounds_h.back());
+          hp[q].w = static_cast<Real>(E / units::kev);
+        }
+        double vn = std::sqrt(r.vx * r.vx + r.vy * r.vy + r.vz * r.vz);
+        if (!(vn > 0.0)) throw EngineError(1, "census packet with zero velocity");
+        hd[q].x = static_cast<Real>(r.vx / vn);
+        hd[q].y = static_cast<Real>(r.vy / vn);
+        hd[q].z = static_cast<Real>(r.vz / vn);
+        hd[q].w = Real(0);
+        if (!std::isfinite(r.weight) || r.weight < 0.0) throw EngineError(1, "census packet weight is negative or not finite");
+        hw[q] = r.weight;
+        tot += r.weight;
+        hcell[q] = c;
+        hm[q] = make_uint2(c, static_cast<uint32_t>(q));
+        continue;
+      }
       int i = r.cell < N ? static_cast<int>(r.cell / (static_cast<uint64_t>(ny) * nz)) : locate(xe, r.x);
       int j = r.cell < N ? static_cast<int>((r.cell / nz) % ny) : locate(ye, r.y);
       int k = r.cell < N ? static_cast<int>(r.cell % nz) : locate(ze, r.z);
@@ -1633,6 +1859,48 @@ int storm_gpu_debug_spectrum(storm_gpu_engine *eng, int32_t *groups, double *nu_
 int storm_gpu_debug_tallies(storm_gpu_engine *eng, uint64_t *dep, uint64_t *rdep, uint64_t *pint, uint64_t *esc,
                             uint32_t *segs) {
   return guarded([&] { as_engine(eng)->debug_tallies(dep, rdep, pint, esc, segs); });
+}
+
+int storm_voronoi_build(const double *points, uint32_t n, const double lo[3], const double hi[3],
+                        storm_voronoi **out) {
+  return guarded([&] {
+    if (!points || !lo || !hi || !out) throw EngineError(1, "null argument");
+    try {
+      auto *m = new storm::voronoi::Mesh(storm::voronoi::build(points, n, lo, hi));
+      *out = reinterpret_cast<storm_voronoi *>(m);
+    } catch (const std::invalid_argument &e) {
+      throw EngineError(1, e.what());
+    }
+  });
+}
+
+int storm_voronoi_mesh(const storm_voronoi *v, storm_unstructured_mesh *out) {
+  return guarded([&] {
+    if (!v || !out) throw EngineError(1, "null argument");
+    const auto *m = reinterpret_cast<const storm::voronoi::Mesh *>(v);
+    storm_unstructured_mesh u{};
+    u.ncells = m->ncells;
+    u.nfaces = m->nfaces();
+    for (int a = 0; a < 3; ++a) {
+      u.box_lo[a] = m->lo[a];
+      u.box_hi[a] = m->hi[a];
+    }
+    u.volume = m->volume.data();
+    u.mesh_point = m->generator.data();
+    u.face_start = m->face_start.data();
+    u.face_neighbor = m->face_neighbor.data();
+    u.face_normal = m->face_normal.data();
+    u.face_centroid = m->face_centroid.data();
+    u.face_area = m->face_area.data();
+    u.face_vertex_start = m->face_vstart.data();
+    u.vertices = m->vertex.data();
+    u.cell_instance = nullptr;
+    *out = u;
+  });
+}
+
+int storm_voronoi_free(storm_voronoi *v) {
+  return guarded([&] { delete reinterpret_cast<storm::voronoi::Mesh *>(v); });
 }
 
 }  // extern "C"

@@ -191,9 +191,17 @@ struct MatDev {
 // Error flags raised by kernels (checked by the host after the step).
 enum : uint32_t { kErrFleck = 1u, kErrNegativeEnergy = 2u, kErrTallyOverflow = 4u, kErrCapacity = 8u };
 
+// Cell layout for the per-cell kernels: a Cartesian nx x ny x nz mesh whose
+// packets carry packed (i, j, k), or (flat = 1) a polyhedral mesh of nx cells
+// whose packets carry the cell index itself (ny = nz = 1).
 struct MeshDims {
   uint32_t nx, ny, nz;
+  uint32_t flat;
 };
+// flat cell index of a packet's cell code
+__device__ __forceinline__ uint32_t cell_of_code(uint32_t code, MeshDims d) {
+  return d.flat ? code : (ijk_i(code) * d.ny + ijk_j(code)) * d.nz + ijk_k(code);
+}
 
 __device__ __forceinline__ uint32_t ijk_of_flat(uint32_t c, MeshDims d) {
   uint32_t k = c % d.nz;
@@ -211,6 +219,7 @@ __device__ __forceinline__ uint32_t ijk_of_flat(uint32_t c, MeshDims d) {
 struct Morton {
   uint32_t bits[3];  // i, j, k
   uint32_t total;
+  uint32_t flat;     // cell codes are flat cell indices (polyhedral meshes, numbered in Z order already)
 };
 __host__ __device__ inline uint32_t morton_encode(uint32_t i, uint32_t j, uint32_t k, Morton m) {
   uint32_t code = 0, pos = 0;
@@ -292,7 +301,7 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
     int32_t G, const double *__restrict__ bounds, double nu_pow, double nu_ref, CellXS<Real> *__restrict__ xs,
     Real *__restrict__ cdf, Real *__restrict__ sig_s_cell, double *__restrict__ fleck,
     double *__restrict__ sig_a_out, double *__restrict__ sig_s_out, double *__restrict__ Eemit,
-    uint32_t *__restrict__ err, MeshDims dims, MeshView<Real> mv, double rw_cell_tau,
+    uint32_t *__restrict__ err, const double *__restrict__ chord_c, double rw_cell_tau,
     CellAccel<Real> *__restrict__ rw_out, double ddmc_pgrw_tau, double *__restrict__ Dtil) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
@@ -306,12 +315,7 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
   // partial-gray random walk (precomputeRandomWalkData, RadiationIMC.hpp:1008-1060):
   // the groups from the lowest up while each is optically thick over the
   // cell's mean chord; Planck-weighted sums over them
-  double chord = 0.0;
-  if (rw_out) {
-    const uint32_t ijk = ijk_of_flat(c, dims);
-    const double dx = mv.width[0][ijk_i(ijk)], dy = mv.width[1][ijk_j(ijk)], dz = mv.width[2][ijk_k(ijk)];
-    chord = 4.0 * dx * dy * dz / (2.0 * (dy * dz + dx * dz + dx * dy));
-  }
+  const double chord = rw_out ? chord_c[c] : 0.0;
   const double kTrw = units::k_boltz * Tc;
   double allAB = 0.0, dB = 0.0, dBA = 0.0, dBT = 0.0, dBoT = 0.0;
   uint32_t cut = 0;
@@ -483,7 +487,7 @@ __global__ __launch_bounds__(256) void k_import(uint32_t n, CensusSoA<Real> cen,
     p.o[1] = b.y;
     p.o[2] = b.z;
     double w = cen.w[i];
-    c = (ijk_i(m.x) * d.ny + ijk_j(m.x)) * d.nz + ijk_k(m.x);
+    c = cell_of_code(m.x, d);
     if (eq_new > eq_old) {
       double wq = fx::quantize_down(w, eq_new);
       if (wq != w) tally_limbs(requant, c, fx::limbs_of(w - wq, eq_old));  // exact in the old quantum
@@ -772,10 +776,10 @@ __device__ __forceinline__ void wave_add_u32(uint32_t v, unsigned long long *dst
 // stats: [0] escapes, [1] random-walk steps, [2] DDMC steps, [4] longest history
 #define STORM_TRANSPORT_ARGS                                                                                  \
   uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out, uint32_t *__restrict__ n_out, CensusSoA<Real> cen,  \
-      uint32_t *__restrict__ n_cen, MeshView<Real> mesh, const CellXS<Real> *__restrict__ xs,                 \
+      uint32_t *__restrict__ n_cen, MeshT mesh, const CellXS<Real> *__restrict__ xs,                          \
       AccelView<Real> acc, StepConsts k, int32_t max_segments, Tallies t,                                     \
       unsigned long long *__restrict__ stats, uint32_t cache_slots, uint32_t cache_fields, uint32_t chunks
-template <typename Real, uint32_t F>
+template <typename Real, uint32_t F, typename MeshT>
 __device__ __forceinline__ void transport_chunks(STORM_TRANSPORT_ARGS) {
   extern __shared__ __align__(16) unsigned char lds[];
   const TallyCache cache = cache_init(lds, cache_slots, cache_fields);
@@ -817,19 +821,19 @@ __device__ __forceinline__ void transport_chunks(STORM_TRANSPORT_ARGS) {
   }
 }
 
-template <typename Real, uint32_t F>
+template <typename Real, uint32_t F, typename MeshT = MeshView<Real>>
 __global__ __launch_bounds__(512) void k_transport(STORM_TRANSPORT_ARGS) {
-  transport_chunks<Real, F>(n_in, in, out, n_out, cen, n_cen, mesh, xs, acc, k, max_segments, t, stats, cache_slots,
-                            cache_fields, chunks);
+  transport_chunks<Real, F, MeshT>(n_in, in, out, n_out, cen, n_cen, mesh, xs, acc, k, max_segments, t, stats,
+                                   cache_slots, cache_fields, chunks);
 }
 // The FP32 gray kernel compiled for 6 waves per SIMD (80 VGPRs, a few bytes
 // of spill): measured 5% faster than at 5 (91 VGPRs); the variants with the
 // random walk or multigroup spill more and keep k_transport's allocation.
-template <typename Real>
+template <typename Real, typename MeshT = MeshView<Real>>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_transport_w6(
     STORM_TRANSPORT_ARGS) {
-  transport_chunks<Real, 0u>(n_in, in, out, n_out, cen, n_cen, mesh, xs, acc, k, max_segments, t, stats,
-                             cache_slots, cache_fields, chunks);
+  transport_chunks<Real, 0u, MeshT>(n_in, in, out, n_out, cen, n_cen, mesh, xs, acc, k, max_segments, t, stats,
+                                    cache_slots, cache_fields, chunks);
 }
 #undef STORM_TRANSPORT_ARGS
 
@@ -933,7 +937,7 @@ template <typename Real>
 // In a multigroup problem (`walk_set`) the random-walk fields come from
 // k_prestep_mg and only the DDMC ones are set here, from the Planck mean
 // sigma_P = sig_a (the reference's DDMC without ddmcUseMultigroupPGRW).
-__global__ __launch_bounds__(256) void k_accel_cells(uint32_t N, MeshDims d, MeshView<Real> mv,
+__global__ __launch_bounds__(256) void k_accel_cells(uint32_t N, const double *__restrict__ chord_c,
                                                      const double *__restrict__ sig_a,
                                                      const double *__restrict__ sig_s,
                                                      const double *__restrict__ fleck, int32_t rw_on,
@@ -942,11 +946,7 @@ __global__ __launch_bounds__(256) void k_accel_cells(uint32_t N, MeshDims d, Mes
                                                      double *__restrict__ Dtil, CellAccel<Real> *__restrict__ out) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
-  const uint32_t ijk = ijk_of_flat(c, d);
-  const double dx = mv.width[0][ijk_i(ijk)], dy = mv.width[1][ijk_j(ijk)], dz = mv.width[2][ijk_k(ijk)];
-  const double V = dx * dy * dz;
-  const double S = 2.0 * (dy * dz + dx * dz + dx * dy);
-  const double chord = 4.0 * V / S;
+  const double chord = chord_c[c];  // mean chord 4 V / S (RadiationIMC.hpp:1297-1298)
   const double st = sig_a[c] + sig_s[c];
   CellAccel<Real> a;
   uint32_t pre = 0u;
@@ -1052,7 +1052,8 @@ __global__ __launch_bounds__(256) void k_comb_keys(uint32_t n, CensusSoA<Real> c
   const uint32_t rank = perm32(m.y, salt);
   vals[i] = (static_cast<uint64_t>(rank) << 32) | i;
   // weight-0 packets are dropped (CombPopulationControl.hpp:136-137)
-  const uint32_t code = cen.w[i] == 0.0 ? (1u << mo.total) : morton_encode(ijk_i(m.x), ijk_j(m.x), ijk_k(m.x), mo);
+  const uint32_t code = cen.w[i] == 0.0 ? (1u << mo.total)
+                                         : (mo.flat ? m.x : morton_encode(ijk_i(m.x), ijk_j(m.x), ijk_k(m.x), mo));
   keys[i] = pbits ? (code << pbits) | (rank >> (32u - pbits)) : code;
 }
 

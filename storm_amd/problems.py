@@ -45,6 +45,135 @@ def _ptr(a):
 
 
 @dataclass
+class UMesh:
+    """A polyhedral mesh in the flattened face tables of storm_unstructured_mesh
+    (include/storm_problem.h): per cell its volume and mesh point, per half
+    face its neighbour (cell, or -1 - box side), outward unit normal,
+    centroid, area and polygon."""
+    volume: np.ndarray
+    mesh_point: np.ndarray         # (N, 3)
+    face_start: np.ndarray         # N + 1, uint32
+    face_neighbor: np.ndarray      # F, int32
+    face_normal: np.ndarray        # (F, 3)
+    face_centroid: np.ndarray      # (F, 3)
+    face_area: np.ndarray          # F
+    face_vertex_start: np.ndarray  # F + 1, uint32
+    vertices: np.ndarray           # (V, 3)
+    box_lo: tuple = (0.0, 0.0, 0.0)
+    box_hi: tuple = (1.0, 1.0, 1.0)
+    cell_instance: np.ndarray | None = None
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def ncells(self) -> int:
+        return len(self.volume)
+
+    @property
+    def nfaces(self) -> int:
+        return len(self.face_neighbor)
+
+    def cell_of_face(self) -> np.ndarray:
+        return np.repeat(np.arange(self.ncells), np.diff(self.face_start))
+
+    def to_c(self) -> capi.UnstructuredMesh:
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64).reshape(-1)  # noqa: E731
+        arrs = dict(volume=f64(self.volume), mesh_point=f64(self.mesh_point),
+                    face_start=np.ascontiguousarray(self.face_start, dtype=np.uint32),
+                    face_neighbor=np.ascontiguousarray(self.face_neighbor, dtype=np.int32),
+                    face_normal=f64(self.face_normal), face_centroid=f64(self.face_centroid),
+                    face_area=f64(self.face_area),
+                    face_vertex_start=np.ascontiguousarray(self.face_vertex_start, dtype=np.uint32),
+                    vertices=f64(self.vertices))
+        if self.cell_instance is not None:
+            arrs["cell_instance"] = np.ascontiguousarray(self.cell_instance, dtype=np.uint32)
+        self._keep = list(arrs.values())
+        m = capi.UnstructuredMesh()
+        m.ncells, m.nfaces = self.ncells, self.nfaces
+        for a in range(3):
+            m.box_lo[a], m.box_hi[a] = float(self.box_lo[a]), float(self.box_hi[a])
+        ptype = dict(face_start=C.POINTER(C.c_uint32), face_vertex_start=C.POINTER(C.c_uint32),
+                     face_neighbor=C.POINTER(C.c_int32), cell_instance=C.POINTER(C.c_uint32))
+        for k, v in arrs.items():
+            setattr(m, k, v.ctypes.data_as(ptype.get(k, C.POINTER(C.c_double))))
+        return m
+
+
+def voronoi_mesh(points, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)) -> UMesh:
+    """Voronoi tessellation of `points` (n x 3) in the box, built by the
+    engine library (storm_voronoi_build); cells in Z order of the generators."""
+    lib = capi.load_library()
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1)
+    lo_a = np.ascontiguousarray(lo, dtype=np.float64)
+    hi_a = np.ascontiguousarray(hi, dtype=np.float64)
+    h = C.c_void_p()
+    rc = lib.storm_voronoi_build(_ptr(pts), len(pts) // 3, _ptr(lo_a), _ptr(hi_a), C.byref(h))
+    if rc != 0:
+        raise RuntimeError(lib.storm_gpu_last_error().decode())
+    try:
+        v = capi.UnstructuredMesh()
+        if lib.storm_voronoi_mesh(h, C.byref(v)) != 0:
+            raise RuntimeError(lib.storm_gpu_last_error().decode())
+        N, F = v.ncells, v.nfaces
+        nv = int(np.ctypeslib.as_array(v.face_vertex_start, shape=(F + 1,))[-1])
+        cp = lambda ptr, n: np.ctypeslib.as_array(ptr, shape=(n,)).copy()  # noqa: E731
+        return UMesh(volume=cp(v.volume, N), mesh_point=cp(v.mesh_point, 3 * N).reshape(N, 3),
+                     face_start=cp(v.face_start, N + 1), face_neighbor=cp(v.face_neighbor, F),
+                     face_normal=cp(v.face_normal, 3 * F).reshape(F, 3),
+                     face_centroid=cp(v.face_centroid, 3 * F).reshape(F, 3), face_area=cp(v.face_area, F),
+                     face_vertex_start=cp(v.face_vertex_start, F + 1), vertices=cp(v.vertices, 3 * nv).reshape(nv, 3),
+                     box_lo=tuple(lo_a), box_hi=tuple(hi_a))
+    finally:
+        lib.storm_voronoi_free(h)
+
+
+def cartesian_umesh(xe, ye, ze) -> UMesh:
+    """A Cartesian mesh written as polyhedral cells: cell (i*ny + j)*nz + k,
+    faces x-, x+, y-, y+, z-, z+, the mesh point at the cell centre."""
+    xe, ye, ze = (np.asarray(a, dtype=np.float64) for a in (xe, ye, ze))
+    nx, ny, nz = len(xe) - 1, len(ye) - 1, len(ze) - 1
+    N = nx * ny * nz
+    vol, mp, nbr, nrm, fcs, area, vstart, verts = [], [], [], [], [], [], [0], []
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                lo = np.array([xe[i], ye[j], ze[k]])
+                hi = np.array([xe[i + 1], ye[j + 1], ze[k + 1]])
+                vol.append(float(np.prod(hi - lo)))
+                mp.append(0.5 * (lo + hi))
+                idx = (i, j, k)
+                n3 = (nx, ny, nz)
+                stride = (ny * nz, nz, 1)
+                c = (i * ny + j) * nz + k
+                for ax in range(3):
+                    for up in (0, 1):
+                        side = 2 * ax + up
+                        inside = (idx[ax] + 1 < n3[ax]) if up else (idx[ax] > 0)
+                        nbr.append(c + (stride[ax] if up else -stride[ax]) if inside else -1 - side)
+                        n = np.zeros(3)
+                        n[ax] = 1.0 if up else -1.0
+                        nrm.append(n)
+                        b, a2 = (ax + 1) % 3, (ax + 2) % 3
+                        x0 = hi[ax] if up else lo[ax]
+                        # counter-clockwise seen from outside: (b, a2) order for +n, reversed for -n
+                        corners = [(lo[b], lo[a2]), (hi[b], lo[a2]), (hi[b], hi[a2]), (lo[b], hi[a2])]
+                        if not up:
+                            corners = corners[::-1]
+                        for cb, ca in corners:
+                            v = np.zeros(3)
+                            v[ax], v[b], v[a2] = x0, cb, ca
+                            verts.append(v)
+                        vstart.append(len(verts))
+                        fc = 0.5 * (lo + hi)
+                        fc[ax] = x0
+                        fcs.append(fc)
+                        area.append(float((hi[b] - lo[b]) * (hi[a2] - lo[a2])))
+    return UMesh(volume=np.array(vol), mesh_point=np.array(mp), face_start=np.arange(0, 6 * N + 1, 6, dtype=np.uint32),
+                 face_neighbor=np.array(nbr, dtype=np.int32), face_normal=np.array(nrm), face_centroid=np.array(fcs),
+                 face_area=np.array(area), face_vertex_start=np.array(vstart, dtype=np.uint32),
+                 vertices=np.array(verts), box_lo=(xe[0], ye[0], ze[0]), box_hi=(xe[-1], ye[-1], ze[-1]))
+
+
+@dataclass
 class ProblemSpec:
     name: str
     x_edges: np.ndarray
@@ -88,10 +217,14 @@ class ProblemSpec:
     group_bounds: np.ndarray | None = None  # photon energies, erg (multigroup when >= 3 entries)
     nu_pow: float = 0.0
     nu_ref: float = 1.0
+    umesh: UMesh | None = None          # polyhedral mesh (the edges above are then unused)
+    instances: list | None = None       # batches: dicts of storm_instance_desc fields
     _keep: list = field(default_factory=list, repr=False)
 
     @property
     def shape(self):
+        if self.umesh is not None:
+            return (self.umesh.ncells, 1, 1)
         return (len(self.x_edges) - 1, len(self.y_edges) - 1, len(self.z_edges) - 1)
 
     @property
@@ -100,12 +233,17 @@ class ProblemSpec:
         return nx * ny * nz
 
     def volumes(self) -> np.ndarray:
+        if self.umesh is not None:
+            return np.asarray(self.umesh.volume, dtype=np.float64)
         dx = np.diff(self.x_edges)[:, None, None]
         dy = np.diff(self.y_edges)[None, :, None]
         dz = np.diff(self.z_edges)[None, None, :]
         return (dx * dy * dz).reshape(-1)
 
     def centers(self):
+        if self.umesh is not None:
+            mp = np.asarray(self.umesh.mesh_point)
+            return mp[:, 0].copy(), mp[:, 1].copy(), mp[:, 2].copy()
         xc = 0.5 * (self.x_edges[1:] + self.x_edges[:-1])
         yc = 0.5 * (self.y_edges[1:] + self.y_edges[:-1])
         zc = 0.5 * (self.z_edges[1:] + self.z_edges[:-1])
@@ -160,6 +298,21 @@ class ProblemSpec:
         g.bounds = _ptr(gb)
         g.nu_pow, g.nu_ref = float(self.nu_pow), float(self.nu_ref)
         p.groups = g
+        if self.umesh is not None:
+            um = self.umesh.to_c()
+            self._keep.append(um)
+            p.unstructured = C.pointer(um)
+        if self.instances:
+            arr = (capi.InstanceDesc * len(self.instances))()
+            for q, d in enumerate(self.instances):
+                arr[q].source_temperature = float(d["source_temperature"])
+                arr[q].source_packets_per_face = int(d["source_packets_per_face"])
+                arr[q].new_photons_per_cell = int(d["new_photons_per_cell"])
+                arr[q].comb_nmin = int(d["comb_nmin"])
+                arr[q].comb_factor = float(d["comb_factor"])
+            self._keep.append(arr)
+            p.num_instances = len(self.instances)
+            p.instances = arr
         return p
 
 
