@@ -340,6 +340,22 @@ struct Engine final : EngineBase {
   std::vector<storm_instance_desc> inst;
   std::vector<uint32_t> cell_inst_h;
   std::vector<double> inst_T4;  // sources' T^4 per instance (frozen at the first step if asked)
+  std::vector<uint32_t> inst_first_h;  // first cell of each instance, and N
+  DevBuf<uint32_t> d_cell_inst, d_inst_first, d_inst_ncells;
+  DevBuf<uint64_t> d_inst_nphot, d_inst_nmin;
+  DevBuf<double> d_inst_factor, d_inst_etot, d_inst_wtot;
+  InstanceView instview() const {
+    InstanceView v{};
+    if (inst.empty()) return v;
+    v.cell = d_cell_inst.p;
+    v.ncells = d_inst_ncells.p;
+    v.nphot = d_inst_nphot.p;
+    v.nmin = d_inst_nmin.p;
+    v.factor = d_inst_factor.p;
+    v.etot = d_inst_etot.p;
+    v.wtot = d_inst_wtot.p;
+    return v;
+  }
   uint32_t last_nfaces_ = 0;    // source faces of the last step
 
   // kept transport inputs of the last step (debug / replay)
@@ -441,8 +457,18 @@ struct Engine final : EngineBase {
         inst.assign(p->instances, p->instances + p->num_instances);
         inst_T4.assign(inst.size(), 0.0);
         cell_inst_h.assign(um.cell_instance, um.cell_instance + N);
-        for (uint32_t c = 0; c < N; ++c)
+        for (uint32_t c = 0; c < N; ++c) {
           if (cell_inst_h[c] >= p->num_instances) throw EngineError(1, "cell_instance out of range");
+          if (c && cell_inst_h[c] < cell_inst_h[c - 1])
+            throw EngineError(1, "the cells of each instance must be contiguous, instances in order");
+        }
+        inst_first_h.assign(p->num_instances + 1, 0);
+        for (uint32_t k = 0, c = 0; k <= p->num_instances; ++k) {
+          while (c < N && cell_inst_h[c] < k) ++c;
+          inst_first_h[k] = c;
+        }
+        for (uint32_t k = 0; k < p->num_instances; ++k)
+          if (inst_first_h[k + 1] == inst_first_h[k]) throw EngineError(1, "an instance has no cells");
         for (const storm_instance_desc &d : inst)
           if (d.new_photons_per_cell == 0) throw EngineError(1, "every instance needs new_photons_per_cell > 0");
       } else if (um.cell_instance) {
@@ -512,6 +538,26 @@ struct Engine final : EngineBase {
       upv(u_src, ut.src);
       upv(u_tris, ut.tris);
       if (imc.with_ddmc) u_ddmc_rate.ensure(std::max<size_t>(ut.face.size(), 1));
+      if (!inst.empty()) {
+        const size_t K = inst.size();
+        std::vector<uint32_t> nc(K);
+        std::vector<uint64_t> nph(K), nmn(K);
+        std::vector<double> fac(K);
+        for (size_t k = 0; k < K; ++k) {
+          nc[k] = inst_first_h[k + 1] - inst_first_h[k];
+          nph[k] = inst[k].new_photons_per_cell;
+          nmn[k] = std::max<uint64_t>(inst[k].comb_nmin, 1);
+          fac[k] = inst[k].comb_factor;
+        }
+        upv(d_cell_inst, cell_inst_h);
+        upv(d_inst_first, inst_first_h);
+        upv(d_inst_ncells, nc);
+        upv(d_inst_nphot, nph);
+        upv(d_inst_nmin, nmn);
+        upv(d_inst_factor, fac);
+        d_inst_etot.ensure(K);
+        d_inst_wtot.ensure(K);
+      }
     }
     if (p->material.sigma_a0) up(sa0, p->material.sigma_a0, N);
     if (p->material.sigma_s0) up(ss0, p->material.sigma_s0, N);
@@ -1000,8 +1046,10 @@ struct Engine final : EngineBase {
     // energy quantum of this step (every energy below 2^62 quanta), photon
     // allocation and offsets; one read-back for the quantum and the count
     k_quantum<<<1, 64, 0, stream>>>(dsc.p + 2, census_total_prev, source_total, units::clight, dt, qexp.p);
+    if (!inst.empty())
+      k_inst_sum<<<static_cast<uint32_t>(inst.size()), 256, 0, stream>>>(d_inst_first.p, Eemit.p, d_inst_etot.p);
     k_alloc<<<blocks_for(N, 256), 256, 0, stream>>>(N, Eemit.p, dsc.p + 2, imc.new_photons_per_cell, qexp.p,
-                                                    nphot.p, wemit.p, emit_limbs.p);
+                                                    nphot.p, wemit.p, emit_limbs.p, instview());
     // photon counts in the cells' Z order, so emission is laid out like the
     // combed census
     const uint32_t M = 1u << morton.total;
@@ -1302,9 +1350,12 @@ struct Engine final : EngineBase {
       ensure_temp(q);
     }
     // per-cell targets; W_c into the ledger's census_pre tally
+    if (!inst.empty())
+      k_inst_census<<<static_cast<uint32_t>(inst.size()), 256, 0, stream>>>(d_inst_first.p, run_start.p, run_end.p,
+                                                                           cum.p, eq, d_inst_wtot.p);
     k_comb_cells<<<blocks_for(N, 256), 256, 0, stream>>>(N, run_start.p, run_end.p, cum.p, n, imc.comb_nmin,
                                                          imc.comb_factor, eq, imc.comb ? 1 : 0, seed, step_index,
-                                                         comb_cells.p, tal.cpre, tal.rem);
+                                                         comb_cells.p, tal.cpre, tal.rem, instview());
     k_comb_count<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits, cpre.soa(), comb_cells.p,
                                                                run_start.p, run_end.p, cum.p, eq, morton, d, ncopy.p);
     {

@@ -429,17 +429,36 @@ __global__ void k_quantum(const double *__restrict__ Etot, double census_prev, d
 }
 
 // photons per cell and packet weight, quantised down to the step quantum
+// Batches (inst.cell non-null): each instance allocates over its own cells
+// with its own newPhotonsPerCell and emission total.
+struct InstanceView {
+  const uint32_t *cell;     // instance of each cell (null: one problem)
+  const uint32_t *ncells;   // per instance
+  const uint64_t *nphot;    // newPhotonsPerCell per instance
+  const uint64_t *nmin;     // comb Nmin per instance
+  const double *factor;     // comb totalParticlesFactor per instance
+  const double *etot;       // emission total per instance (k_inst_sum)
+  const double *wtot;       // census weight per instance, erg (k_inst_census)
+};
+
 __global__ __launch_bounds__(256) void k_alloc(uint32_t N, const double *__restrict__ Eemit,
-                                               const double *__restrict__ Etot_p, uint64_t n_min,
+                                               const double *__restrict__ Etot_p, uint64_t n_min_all,
                                                const int32_t *__restrict__ q, uint32_t *__restrict__ nphot,
                                                double *__restrict__ wemit,
-                                               unsigned long long *__restrict__ emit_limbs) {
+                                               unsigned long long *__restrict__ emit_limbs, InstanceView inst) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
-  const double Etot = *Etot_p;
+  double Etot = *Etot_p;
+  uint64_t Ncells = N, n_min = n_min_all;
+  if (inst.cell) {
+    const uint32_t k = inst.cell[c];
+    Etot = inst.etot[k];
+    Ncells = inst.ncells[k];
+    n_min = inst.nphot[k];
+  }
   const int32_t eq = q[0];
   const double E = Eemit[c];
-  const uint64_t total = static_cast<uint64_t>(N) * n_min * 10ull;
+  const uint64_t total = Ncells * n_min * 10ull;
   uint64_t share = Etot > 0.0 ? static_cast<uint64_t>(E / Etot * static_cast<double>(total)) : n_min;
   uint64_t n = 0;
   if (E > 0.0) {
@@ -1134,6 +1153,24 @@ struct CombCell {
   uint32_t mode;    // 0 empty, 1 split/pass-through, 2 comb
 };
 
+// Per-instance sum of a per-cell value over contiguous cell ranges
+// [first[k], first[k + 1]) (one block per instance, fixed tree: the result
+// does not depend on scheduling).
+__global__ __launch_bounds__(256) void k_inst_sum(const uint32_t *__restrict__ first, const double *__restrict__ v,
+                                                  double *__restrict__ out) {
+  __shared__ double sh[256];
+  const uint32_t k = blockIdx.x;
+  double s = 0.0;
+  for (uint32_t c = first[k] + threadIdx.x; c < first[k + 1]; c += 256u) s += v[c];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t w = 128u; w > 0u; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k] = sh[0];
+}
+
 // Exact value of an inclusive prefix sum of limb pairs.
 __device__ __forceinline__ unsigned __int128 cum_value(const ulonglong2 *cum, uint32_t i) {
   const ulonglong2 a = cum[i];
@@ -1170,19 +1207,67 @@ __device__ __forceinline__ double u128_to_double(unsigned __int128 v) {
          static_cast<double>(static_cast<uint64_t>(v));
 }
 
+// Census weight of each instance (erg): the sorted census is in cell order
+// and an instance's cells are contiguous, so its packets are the run from
+// the first packet of its first occupied cell to the last of its last one;
+// the weight is the difference of the exact prefix sums at the two ends.
+__global__ __launch_bounds__(256) void k_inst_census(const uint32_t *__restrict__ first,
+                                                     const uint32_t *__restrict__ start,
+                                                     const uint32_t *__restrict__ end,
+                                                     const ulonglong2 *__restrict__ cum, int32_t eq,
+                                                     double *__restrict__ wtot) {
+  __shared__ uint32_t lo_s[256], hi_s[256];
+  const uint32_t k = blockIdx.x;
+  uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+  for (uint32_t c = first[k] + threadIdx.x; c < first[k + 1]; c += 256u) {
+    const uint32_t e = end[c];
+    if (e) {
+      lo = start[c] < lo ? start[c] : lo;
+      hi = e > hi ? e : hi;
+    }
+  }
+  lo_s[threadIdx.x] = lo;
+  hi_s[threadIdx.x] = hi;
+  __syncthreads();
+  for (uint32_t w = 128u; w > 0u; w >>= 1) {
+    if (threadIdx.x < w) {
+      lo_s[threadIdx.x] = lo_s[threadIdx.x + w] < lo_s[threadIdx.x] ? lo_s[threadIdx.x + w] : lo_s[threadIdx.x];
+      hi_s[threadIdx.x] = hi_s[threadIdx.x + w] > hi_s[threadIdx.x] ? hi_s[threadIdx.x + w] : hi_s[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t a = lo_s[0], b = hi_s[0];
+    double W = 0.0;
+    if (b > 0u) {
+      const unsigned __int128 v = cum_value(cum, b - 1u) - (a ? cum_value(cum, a - 1u) : static_cast<unsigned __int128>(0));
+      W = u128_to_double(v) * fpm::pow2d(eq);
+    }
+    wtot[k] = W;
+  }
+}
+
 // Per-cell comb targets (CombPopulationControl.hpp:157-206).  The cell's
 // census weight W_c (CombPopulationControl.hpp:139-141) is the difference of
 // the prefix sums over its run of sorted packets; it is also stored, as limbs,
 // for the energy ledger.
 __global__ __launch_bounds__(256) void k_comb_cells(uint32_t N, const uint32_t *__restrict__ start,
                                                     const uint32_t *__restrict__ end,
-                                                    const ulonglong2 *__restrict__ cum, uint32_t n, uint64_t nmin,
-                                                    double factor, int32_t eq, int32_t comb_on, uint64_t seed,
+                                                    const ulonglong2 *__restrict__ cum, uint32_t n, uint64_t nmin_all,
+                                                    double factor_all, int32_t eq, int32_t comb_on, uint64_t seed,
                                                     uint32_t step, CombCell *__restrict__ cells,
                                                     unsigned long long *__restrict__ W,
-                                                    unsigned long long *__restrict__ rem) {
+                                                    unsigned long long *__restrict__ rem, InstanceView inst) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
+  uint64_t nmin = nmin_all, Ncells = N;
+  double factor = factor_all;
+  if (inst.cell) {
+    const uint32_t k = inst.cell[c];
+    nmin = inst.nmin[k];
+    factor = inst.factor[k];
+    Ncells = inst.ncells[k];
+  }
   CombCell cc;
   cc.w_out = 0.0;
   cc.r = 0.5;
@@ -1192,7 +1277,7 @@ __global__ __launch_bounds__(256) void k_comb_cells(uint32_t N, const uint32_t *
   const uint32_t s0 = e ? start[c] : 0u;
   const uint32_t count = e - s0;
   // total census weight (exact integer), as a double in erg
-  const double Wtot = u128_to_double(cum_value(cum, n - 1u)) * fpm::pow2d(eq);
+  const double Wtot = inst.cell ? inst.wtot[inst.cell[c]] : u128_to_double(cum_value(cum, n - 1u)) * fpm::pow2d(eq);
   const unsigned __int128 Wc =
       count ? cum_value(cum, e - 1u) - (s0 ? cum_value(cum, s0 - 1u) : static_cast<unsigned __int128>(0))
             : static_cast<unsigned __int128>(0);
@@ -1207,7 +1292,7 @@ __global__ __launch_bounds__(256) void k_comb_cells(uint32_t N, const uint32_t *
     cells[c] = cc;
     return;
   }
-  const uint64_t Ntotal = static_cast<uint64_t>(static_cast<double>(static_cast<uint64_t>(N) * nmin) * factor);
+  const uint64_t Ntotal = static_cast<uint64_t>(static_cast<double>(Ncells * nmin) * factor);
   uint64_t share = static_cast<uint64_t>(static_cast<double>(Ntotal) * cc.Wd / Wtot);
   uint64_t NinCell = share > nmin ? share : nmin;
   if (NinCell > 20ull * nmin) NinCell = 20ull * nmin;

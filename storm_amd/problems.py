@@ -549,3 +549,151 @@ def marshak_bath_T(problem: int, t: float) -> float:
     P = MARSHAK[problem]
     t_ns = max(t, 1e-20) * 1e9
     return P["Tc"] * t_ns ** P["Te"] * KEV_K
+
+
+def voronoi_points(n: int, seed: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)) -> np.ndarray:
+    """n generators uniform in the box, from a seeded generator."""
+    rng = np.random.default_rng(np.random.SeedSequence([int(seed), 0xC4]))
+    lo, hi = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+    return lo + (hi - lo) * rng.random((n, 3))
+
+
+def voronoi_slab(seed: int = 0, n: int = 200, mesh_seed: int = 4, steps: int = 3, thick: bool = False,
+                 packets_scale: float = 1.0, groups: int = 0) -> ProblemSpec:
+    """A Voronoi tessellation of n seeded points in [0,1] x [0,0.5] x [0,0.5]
+    cm: sigma_a = 3 cm^-1 (x < 0.5) and 30 cm^-1 beyond (thick=True: 2e3
+    cm^-1 with sigma_s = 2e3 and the random walk and DDMC on, cell depth 10,
+    particle depths 10 and 5), cv = 1e15 erg/(cm^3 keV), T0 = 1e-3 keV (0.05
+    keV when thick), a 1 keV x- source, escape on x+, reflecting elsewhere,
+    comb.  groups > 0: that many log-spaced groups over 0.01-20 keV with
+    sigma(E) proportional to E^-3 (1 keV reference)."""
+    lo, hi = (0.0, 0.0, 0.0), (1.0, 0.5, 0.5)
+    m = voronoi_mesh(voronoi_points(n, mesh_seed, lo, hi), lo, hi)
+    N = m.ncells
+    x = m.mesh_point[:, 0]
+    s = max(packets_scale, 1e-6)
+    sa = np.where(x < 0.5, 3.0, 2e3 if thick else 30.0)
+    ss = np.where(x < 0.5, 0.0, 2e3) if thick else None
+    spec = ProblemSpec(
+        name="voronoi_slab", x_edges=np.array([0.0, 1.0]), y_edges=np.array([0.0, 0.5]), z_edges=np.array([0.0, 0.5]),
+        sigma_a0=sa, sigma_s0=ss, T0=np.full(N, (0.05 if thick else 1e-3) * KEV_K), dt=1e-11, steps=steps,
+        eos_f=1e15 / KEV_K, eos_beta=1.0, eos_mu=0.0,
+        face_kind=(capi.FACE_REFLECT, capi.FACE_ESCAPE) + (capi.FACE_REFLECT,) * 4, source_face=capi.XLO,
+        source_T=KEV_K, freeze_T4=1, source_per_face=max(1, int(round(400 * s))),
+        new_photons_per_cell=max(1, int(round(8 * s))), comb=1, comb_nmin=max(1, int(round(12 * s))),
+        comb_factor=2.0, seed=seed, umesh=m,
+        with_rw=int(thick), with_ddmc=int(thick), rw_cell_tau=10.0, rw_particle_tau=10.0, ddmc_cell_tau=10.0,
+        ddmc_particle_tau=5.0)
+    if groups:
+        b = KEV * 0.01 * (2000.0 ** (np.arange(groups + 1) / groups))
+        b[-1] = KEV * 20.0
+        spec.group_bounds = b
+        spec.nu_pow, spec.nu_ref = 3.0, KEV
+        spec.T0 = np.full(N, 0.3 * KEV_K)
+    return spec
+
+
+def c4_voronoi(seed: int = 0, n: int = 200000, groups: int = 64, steps: int = 20, packets_scale: float = 1.0,
+               mesh: UMesh | None = None) -> ProblemSpec:
+    """Multigroup IMC on the Voronoi tessellation of n uniform seeded points
+    in the unit cube: 64 log-spaced groups over 0.01-20 keV, sigma(E, T) =
+    10 (E/keV)^-3 (T/keV)^-1/2 cm^-1 (the Densmore form,
+    examples/densmore2012/DensmoreOpacity.hpp:58-66), density
+    1 + 0.5 sin(2 pi x), e = 1e15 erg/(g keV) * rho * T, T0 = 0.1 keV, a 1 keV
+    source on the x- faces, escape on every face, comb; ~5e7 packets per step
+    at packets_scale = 1."""
+    m = mesh if mesh is not None else voronoi_mesh(voronoi_points(n, 0xC4))
+    N = m.ncells
+    rho = 1.0 + 0.5 * np.sin(2.0 * np.pi * m.mesh_point[:, 0])
+    b = KEV * 0.01 * (2000.0 ** (np.arange(groups + 1) / groups))
+    b[-1] = KEV * 20.0
+    s = max(packets_scale, 1e-6)
+    # the x- box faces: about n^(2/3) of them
+    nsrc = int(np.count_nonzero(m.face_neighbor == -1))
+    return ProblemSpec(
+        name="c4_voronoi", x_edges=np.array([0.0, 1.0]), y_edges=np.array([0.0, 1.0]), z_edges=np.array([0.0, 1.0]),
+        sigma_a0=np.full(N, 10.0), density=rho, T0=np.full(N, 0.1 * KEV_K), dt=1e-11, steps=steps,
+        T_ref=KEV_K, T_floor=1.0, a_pow=0.5, eos_f=1e15 / KEV_K, eos_beta=1.0, eos_mu=0.0,
+        face_kind=(capi.FACE_ESCAPE,) * 6, source_face=capi.XLO, source_T=KEV_K, freeze_T4=1,
+        source_per_face=max(1, int(round(1e7 * s / max(nsrc, 1)))),
+        new_photons_per_cell=max(1, int(round(100 * s))), comb=1, comb_nmin=max(1, int(round(100 * s))),
+        comb_factor=1.0, seed=seed, umesh=m, group_bounds=b, nu_pow=3.0, nu_ref=KEV)
+
+
+def c5_instances(count: int = 256, seed: int = 0, packets_scale: float = 1.0,
+                 cells_range=(50, 2000), packets_range=(1e4, 1e6)) -> list:
+    """Parameters of independent 1-D slab gray IMC instances: cells
+    log-uniform in cells_range, sigma_a log-uniform in [0.1, 1e3] cm^-1,
+    boundary temperature uniform in [0.3, 1.5] keV, packets per step
+    log-uniform in packets_range (half from the x- source, a quarter emitted,
+    a quarter the comb's floor)."""
+    rng = np.random.default_rng(np.random.SeedSequence([int(seed), 0xC5]))
+    out = []
+    lc = np.log10(np.asarray(cells_range, dtype=np.float64))
+    lp = np.log10(np.asarray(packets_range, dtype=np.float64))
+    for _ in range(count):
+        n = int(round(10.0 ** rng.uniform(lc[0], lc[1])))
+        sigma = 10.0 ** rng.uniform(-1.0, 3.0)
+        Tb = rng.uniform(0.3, 1.5)
+        P = 10.0 ** rng.uniform(lp[0], lp[1]) * packets_scale
+        out.append(dict(cells=n, sigma_a=sigma, T_keV=Tb, packets=P,
+                        source_packets_per_face=max(1, int(round(0.5 * P))),
+                        new_photons_per_cell=max(1, int(round(0.25 * P / n))),
+                        comb_nmin=max(1, int(round(0.25 * P / n))), comb_factor=1.0))
+    return out
+
+
+def c5_instance_spec(d: dict, seed: int = 0, steps: int = 10) -> ProblemSpec:
+    """One instance of the batch as a stand-alone Cartesian problem: nx x 1 x 1
+    cubic cells over [0, 1] cm, C1's material (cv = 1e15 erg/(cm^3 keV), T0 =
+    1e-3 keV), x- source at the instance's temperature (escape there, reflect
+    elsewhere), DDMC in cells of optical depth >= 10 (particle depth 5)."""
+    n = d["cells"]
+    h = 1.0 / n
+    return ProblemSpec(
+        name="c5_instance", x_edges=np.linspace(0.0, 1.0, n + 1), y_edges=np.array([0.0, h]),
+        z_edges=np.array([0.0, h]), sigma_a0=np.full(n, d["sigma_a"]), T0=np.full(n, 1e-3 * KEV_K), dt=1e-11,
+        steps=steps, eos_f=1e15 / KEV_K, eos_beta=1.0, eos_mu=0.0,
+        face_kind=(capi.FACE_ESCAPE,) + (capi.FACE_REFLECT,) * 5, source_face=capi.XLO, source_T=d["T_keV"] * KEV_K,
+        freeze_T4=1, source_per_face=d["source_packets_per_face"], new_photons_per_cell=d["new_photons_per_cell"],
+        comb=1, comb_nmin=d["comb_nmin"], comb_factor=d["comb_factor"], seed=seed,
+        with_ddmc=1, ddmc_cell_tau=10.0, ddmc_particle_tau=5.0)
+
+
+def c5_batch(instances: list, seed: int = 0, steps: int = 10) -> ProblemSpec:
+    """The instances as one batched problem: their slabs, written as polyhedral
+    cells (cartesian_umesh), concatenated with per-cell instance indices; no
+    face joins two instances."""
+    meshes = [cartesian_umesh(np.linspace(0.0, 1.0, d["cells"] + 1), np.array([0.0, 1.0 / d["cells"]]),
+                              np.array([0.0, 1.0 / d["cells"]])) for d in instances]
+    vol, mp, fs, nb, nrm, fc, fa, fvs, vs, ci = [], [], [0], [], [], [], [], [0], [], []
+    cell0 = 0
+    for k, m in enumerate(meshes):
+        vol.append(m.volume)
+        mp.append(m.mesh_point)
+        fs.extend((np.asarray(m.face_start[1:], np.int64) + fs[-1]).tolist())
+        nbk = np.asarray(m.face_neighbor, dtype=np.int64)
+        nb.append(np.where(nbk >= 0, nbk + cell0, nbk))
+        nrm.append(m.face_normal)
+        fc.append(m.face_centroid)
+        fa.append(m.face_area)
+        fvs.extend((np.asarray(m.face_vertex_start[1:], np.int64) + fvs[-1]).tolist())
+        vs.append(m.vertices)
+        ci.append(np.full(m.ncells, k, dtype=np.uint32))
+        cell0 += m.ncells
+    um = UMesh(volume=np.concatenate(vol), mesh_point=np.concatenate(mp), face_start=np.array(fs, dtype=np.uint32),
+               face_neighbor=np.concatenate(nb).astype(np.int32), face_normal=np.concatenate(nrm),
+               face_centroid=np.concatenate(fc), face_area=np.concatenate(fa),
+               face_vertex_start=np.array(fvs, dtype=np.uint32), vertices=np.concatenate(vs),
+               box_lo=(0.0, 0.0, 0.0), box_hi=(1.0, 1.0, 1.0), cell_instance=np.concatenate(ci))
+    N = um.ncells
+    sa = np.concatenate([np.full(d["cells"], d["sigma_a"]) for d in instances])
+    spec = c5_instance_spec(instances[0], seed=seed, steps=steps)
+    spec.name = "c5_batch"
+    spec.umesh = um
+    spec.sigma_a0 = sa
+    spec.T0 = np.full(N, 1e-3 * KEV_K)
+    spec.instances = [dict(source_temperature=d["T_keV"] * KEV_K, source_packets_per_face=d["source_packets_per_face"],
+                           new_photons_per_cell=d["new_photons_per_cell"], comb_nmin=d["comb_nmin"],
+                           comb_factor=d["comb_factor"]) for d in instances]
+    return spec
