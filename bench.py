@@ -29,9 +29,12 @@ report.  The history kernel keeps a packet in registers for its whole
 history, so its measured HBM traffic (rocprof FETCH_SIZE / WRITE_SIZE,
 `traffic`) is far below the model.
 
-A second line of figures (`c3`) runs the two-material step slab (random walk
-and DDMC, 2e7 packets per step) for a few steps, and `cpu_baseline` times the
-FP64 CPU oracle on host cores on a bounded sample of the same workload.
+Further lines of figures, a few steps each: `c3` runs the two-material step
+slab (random walk and DDMC, 2e7 packets per step); `c4` the 64-group problem
+on a Voronoi tessellation of 2e5 points (5e7 packets per step), mixed and
+FP64; `c5` 256 independent slab instances as one batched mixed engine against
+one FP64 engine per instance.  `cpu_baseline` times the FP64 CPU oracle on
+host cores on a bounded sample of the C2 workload.
 """
 from __future__ import annotations
 
@@ -215,6 +218,92 @@ def c3_line(W, K, log):
                 roofline_frac=seg * B_SEG_MIXED / t_tr / 1e9 / HBM_PEAK_GBS)
 
 
+def _run_steps(spec, precision, n, W, log, label):
+    """n steps of one engine; segments, wall, transport seconds and packets of
+    the steps after the first W."""
+    from storm_amd.engine import GpuEngine
+
+    tot = dict(segments=0, wall=0.0, transport=0.0, packets=0, rw=0, ddmc=0, steps=0)
+    with GpuEngine(spec, precision) as e:
+        for s in range(n):
+            t0 = time.perf_counter()
+            e.step()
+            dtw = time.perf_counter() - t0
+            c = e.counters()
+            if log:
+                log(f"{label} step {s + 1}/{n}: {c['segments']} segments, {c['start_particles']} packets, "
+                    f"{c['random_walk_steps']} walk, {c['ddmc_steps']} DDMC, transport "
+                    f"{c['seconds_transport'] * 1e3:.2f} ms, step {dtw * 1e3:.2f} ms")
+            if s >= W:
+                tot["segments"] += c["segments"]
+                tot["wall"] += dtw
+                tot["transport"] += c["seconds_transport"]
+                tot["packets"] += c["start_particles"]
+                tot["rw"] += c["random_walk_steps"]
+                tot["ddmc"] += c["ddmc_steps"]
+                tot["steps"] += 1
+    return tot
+
+
+def c4_line(W, K, log, fp64=True):
+    """Multigroup IMC on a Voronoi tessellation (C4: 2e5 cells, 64 groups,
+    ~5e7 packets per step), mixed engine and, beside it, the FP64 engine."""
+    from storm_amd import capi
+    from storm_amd import problems as P
+
+    t0 = time.perf_counter()
+    mesh = P.voronoi_mesh(P.voronoi_points(200000, 0xC4))
+    log(f"c4 mesh: {mesh.ncells} cells, {mesh.nfaces} faces in {time.perf_counter() - t0:.2f} s")
+    spec = P.c4_voronoi(seed=1, mesh=mesh)
+    m = _run_steps(spec, capi.MIXED, W + K, W, log, "c4 mixed")
+    out = dict(workload="c4_voronoi: Voronoi tessellation of 2e5 uniform points in the unit cube (flattened face "
+                        "tables), 64 log-spaced groups 0.01-20 keV, sigma = 10 nu^-3 T^-1/2, rho = 1 + 0.5 sin(2 pi "
+                        "x), ~5e7 packets/step, dt=1e-11 s",
+               value=m["segments"] / m["wall"], unit="segments/s", steps=K, warmup=W,
+               ms_per_step=m["wall"] / K * 1e3, packets_per_step=m["packets"] / K,
+               segments_per_step=m["segments"] / K, transport_segments_per_s=m["segments"] / m["transport"],
+               faces=int(mesh.nfaces), kernel="k_transport<float, 2u, UMeshView<float>> (multigroup)")
+    if fp64:
+        f = _run_steps(spec, capi.FP64, W + K, W, log, "c4 fp64")
+        out["fp64_gpu"] = dict(value=f["segments"] / f["wall"], ms_per_step=f["wall"] / K * 1e3,
+                               transport_segments_per_s=f["segments"] / f["transport"])
+        out["speedup_vs_fp64"] = out["value"] / out["fp64_gpu"]["value"]
+    return out
+
+
+def c5_line(W, K, log):
+    """Batched independent instances (C5: 256 1-D slabs of 50-2000 cells and
+    1e4-1e6 packets per step): the batch as one mixed engine against one FP64
+    engine per instance, stepped one after another."""
+    from storm_amd import capi
+    from storm_amd import problems as P
+
+    inst = P.c5_instances()
+    t0 = time.perf_counter()
+    batch = P.c5_batch(inst, seed=1, steps=W + K)
+    log(f"c5 batch: {len(inst)} instances, {batch.umesh.ncells} cells, built in {time.perf_counter() - t0:.2f} s")
+    b = _run_steps(batch, capi.MIXED, W + K, W, log, "c5 batched mixed")
+    per = dict(segments=0, wall=0.0, transport=0.0, packets=0, rw=0, ddmc=0)
+    t0 = time.perf_counter()
+    for i, d in enumerate(inst):
+        r = _run_steps(P.c5_instance_spec(d, seed=1 + i, steps=W + K), capi.FP64, W + K, W, None, "")
+        for k in per:
+            per[k] += r[k]
+    log(f"c5 per-instance fp64: {len(inst)} engines x {W + K} steps in {time.perf_counter() - t0:.2f} s "
+        f"(timed steps {per['wall']:.3f} s)")
+    return dict(workload=f"c5_batch: {len(inst)} independent 1-D slab gray IMC instances (cells log-uniform 50-2000, "
+                         "packets log-uniform 1e4-1e6 per step, sigma_a log-uniform 0.1-1e3, boundary 0.3-1.5 keV, "
+                         "DDMC in thick cells) as one engine",
+                value=b["segments"] / b["wall"], unit="segments/s", steps=K, warmup=W,
+                ms_per_step=b["wall"] / K * 1e3, cells=int(batch.umesh.ncells), packets_per_step=b["packets"] / K,
+                ddmc_steps_per_step=b["ddmc"] / K, transport_segments_per_s=b["segments"] / b["transport"],
+                per_instance_fp64=dict(value=per["segments"] / per["wall"], ms_per_step=per["wall"] / K * 1e3,
+                                       packets_per_step=per["packets"] / K,
+                                       form="one FP64 engine per instance, each stepped on its own"),
+                step_time_ratio=per["wall"] / b["wall"],
+                speedup_vs_per_instance_fp64=(b["segments"] / b["wall"]) / (per["segments"] / per["wall"]))
+
+
 def cpu_baseline(W, log):
     """FP64 oracle on the host: steps W+1, W+2 of the same workload at 1/100 of
     the packets, timed from the oracle's own per-step clock; one run on one
@@ -261,6 +350,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=256,
                     help="after the run, one plain copy kernel over this many MiB (scales a profiler's HBM byte "
@@ -362,6 +453,10 @@ def main():
     out["matched_error"] = matched_error(mixed, fp64)
     if c3 is not None:
         out["c3"] = c3
+    if not args.no_c4:
+        out["c4"] = c4_line(1, 2, log, fp64=not args.no_fp64)
+    if not args.no_c5:
+        out["c5"] = c5_line(1, 3, log)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(W, log)
     if args.copy_mib > 0:

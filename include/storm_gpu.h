@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define STORM_GPU_ABI_VERSION 6
+#define STORM_GPU_ABI_VERSION 7
 
 typedef struct storm_gpu_engine storm_gpu_engine;
 
@@ -172,8 +172,17 @@ int storm_gpu_get_beginning_counts(storm_gpu_engine *eng, uint64_t *per_cell);
 /* Counters of the last step. */
 int storm_gpu_get_counters(storm_gpu_engine *eng, storm_gpu_counters *out);
 
-/* Exact energy ledger of the last step. */
+/* Exact energy ledger of the last step.  In a batch (storm_problem
+ * instances) the integer fields count the step's quantum in each instance's
+ * own energy unit: the balances hold exactly, and the erg figures come from
+ * storm_gpu_get_instance_ledger. */
 int storm_gpu_get_ledger(storm_gpu_engine *eng, storm_gpu_ledger *out);
+
+/* The ledger of the last step restricted to instance k of a batch (its cells
+ * and source faces), with eq and eq_prev the instance's quantum exponents in
+ * erg.  The reference steps one problem per manager, whose ledger this is
+ * (MonteCarloManagerSerial.hpp:385-469). */
+int storm_gpu_get_instance_ledger(storm_gpu_engine *eng, uint32_t k, storm_gpu_ledger *out);
 
 /* Census exchange with the host -- the std::vector<Particle> moved into and
  * returned from MonteCarloManagerSerial::step (:44, :197-248, :464-468). */

@@ -571,7 +571,7 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
         n = share < 20ull * nmin ? share : 20ull * nmin;
         if (n < nmin) n = nmin;
       }
-      double w = n ? fx::quantize_down(E[c] / static_cast<double>(n), eq) : 0.0;
+      double w = n ? fx::emission_round(E[c] / static_cast<double>(n), eq, c, step) : 0.0;
       if (w == 0.0) n = 0;
       nph[c] = static_cast<uint32_t>(n);
       wem[c] = w;

@@ -52,6 +52,35 @@ STORM_HD double quantize_unbiased(double v, int e, uint32_t u24) {
   return (u < frac ? fl + 1.0 : fl) * fpm::pow2d(e);
 }
 
+// The weight a packet carries on after a cell visit (into the next cell, to
+// census or out of the problem), as a multiple of 2^e: w rounded without
+// bias, the draw a hash of the packet id, its random-stream position and the
+// cell (no stream draw, so the history is unchanged), and never above
+// w_enter, the weight the visit began with.  Rounding down every time would
+// take up to one quantum from a packet at each face it crosses, which ends
+// packets of a few quanta long before their physics does.
+STORM_HD double carry_round(double w, double w_enter, int e, uint32_t id, uint32_t ctr, uint32_t cell) {
+  uint32_t h = id * 0x2545F491u ^ (ctr + 0x3C6EF372u) * 0x9E3779B9u ^ cell * 0x27D4EB2Fu;
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  const double wq = quantize_unbiased(w, e, h >> 8);
+  return wq > w_enter ? quantize_down(w, e) : wq;
+}
+
+// Weight of each of a cell's emission packets, E / n rounded to a multiple of
+// 2^e without bias (the draw a hash of the cell and the step), so that the
+// energy a cell emits is E in expectation however few quanta a packet holds.
+STORM_HD double emission_round(double per, int e, uint32_t cell, uint32_t step) {
+  uint32_t h = cell * 0x9E3779B1u ^ (step + 0x165667B1u) * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  return quantize_unbiased(per, e, h >> 8);
+}
+
 // Limbs of w / 2^e for a w that is an integer multiple of 2^e below 2^62.
 STORM_HD Limbs limbs_of(double w, int e) {
   double y = w * fpm::pow2d(-e);            // exact integer

@@ -306,7 +306,7 @@ struct Tracker {
   template <typename Sink>
   STORM_HD void end(Packet<Real> &p, const StepConsts &k, Sink &sink) {
     if (outcome == kAlive) {
-      double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
+      double wq = fixed_point ? fx::carry_round(w, w_enter, k.eq, id, ctr, cell) : w;
       flush(wq, k, sink);
       w = wq;
     }
@@ -360,7 +360,7 @@ struct Tracker {
   template <typename Sink>
   STORM_HD void enter_neighbour(int fax, bool up, const MeshView<Real> &m, const CellXS<Real> *xs,
                                 const StepConsts &k, Sink &sink) {
-    double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
+    double wq = fixed_point ? fx::carry_round(w, w_enter, k.eq, id, ctr, cell) : w;
     flush(wq, k, sink);
     w = wq;
     w_enter = wq;
@@ -381,7 +381,7 @@ struct Tracker {
   STORM_HD void enter_loaded_neighbour(int fax, bool up, uint32_t nijk, uint32_t ncell, Real nw,
                                        const CellXS<Real> &nx, Real nsig_s, const MeshView<Real> &m,
                                        const StepConsts &k, Sink &sink) {
-    double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
+    double wq = fixed_point ? fx::carry_round(w, w_enter, k.eq, id, ctr, cell) : w;
     flush(wq, k, sink);
     w = wq;
     w_enter = wq;
@@ -470,7 +470,7 @@ struct Tracker {
   STORM_HD void census(const StepConsts &k, Sink &sink) {
     // the weight is kept as a multiple of the quantum in both engines so
     // that population control can work in exact integers
-    double wq = fx::quantize_down(w, k.eq);
+    double wq = fx::carry_round(w, w_enter, k.eq, id, ctr, cell);
     flush(wq, k, sink);
     w = wq;
     outcome = kCensus;
@@ -759,7 +759,7 @@ struct Tracker {
         put3<Real>(fax, u0, u1, u2, up ? Real(1) : Real(0));
         if ((m.escape_mask >> face) & 1u) {
           // escape (REMOVE); the weight leaves the problem
-          double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
+          double wq = fixed_point ? fx::carry_round(w, w_enter, k.eq, id, ctr, cell) : w;
           flush(wq, k, sink);
           sink.escape(cell, wq);
           w = 0.0;

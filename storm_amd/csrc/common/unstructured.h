@@ -136,7 +136,7 @@ struct UTracker {
   template <typename Sink>
   STORM_HD void end(Packet<Real> &p, const StepConsts &k, Sink &sink) {
     if (outcome == kAlive) {
-      double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
+      double wq = fixed_point ? fx::carry_round(w, w_enter, k.eq, id, ctr, cell) : w;
       flush(wq, k, sink);
       w = wq;
     }
@@ -232,7 +232,7 @@ struct UTracker {
 
   template <typename Sink>
   STORM_HD void census(const StepConsts &k, Sink &sink) {
-    double wq = fx::quantize_down(w, k.eq);
+    double wq = fx::carry_round(w, w_enter, k.eq, id, ctr, cell);
     flush(wq, k, sink);
     w = wq;
     outcome = kCensus;
@@ -242,7 +242,7 @@ struct UTracker {
   template <typename Sink>
   STORM_HD void cross(const UFace<Real> &fc, const UMeshView<Real> &m, const CellXS<Real> *xs, const StepConsts &k,
                       Sink &sink) {
-    double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
+    double wq = fixed_point ? fx::carry_round(w, w_enter, k.eq, id, ctr, cell) : w;
     flush(wq, k, sink);
     w = wq;
     w_enter = wq;
@@ -498,7 +498,7 @@ struct UTracker {
       } else {
         const uint32_t side = static_cast<uint32_t>(-1 - fcx.nbr);
         if ((m.escape_mask >> side) & 1u) {
-          double wq = fixed_point ? fx::quantize_down(w, k.eq) : w;
+          double wq = fixed_point ? fx::carry_round(w, w_enter, k.eq, id, ctr, cell) : w;
           flush(wq, k, sink);
           sink.escape(cell, wq);
           w = 0.0;

@@ -222,6 +222,7 @@ struct EngineBase {
   virtual void get_segments(uint64_t *) = 0;
   virtual void get_begin(uint64_t *) = 0;
   virtual void get_ledger(storm_gpu_ledger *) = 0;
+  virtual void get_instance_ledger(uint32_t, storm_gpu_ledger *) = 0;
   virtual void set_cell_state(const double *, const double *, const double *, const double *, const double *) = 0;
   virtual uint64_t census_size() = 0;
   virtual void export_census(storm_packet_record *, uint64_t, uint64_t *) = 0;
@@ -344,6 +345,15 @@ struct Engine final : EngineBase {
   DevBuf<uint32_t> d_cell_inst, d_inst_first, d_inst_ncells;
   DevBuf<uint64_t> d_inst_nphot, d_inst_nmin;
   DevBuf<double> d_inst_factor, d_inst_etot, d_inst_wtot;
+  // per instance: boundary source total of the step, census energy after the
+  // last step, material energy before this one (erg); unit exponents of this
+  // step and the last (device, and host copies for the source weights, the
+  // census export and the ledgers)
+  DevBuf<double> d_inst_src, d_inst_cen, d_inst_mat0;
+  DevBuf<int32_t> d_inst_shift, d_inst_shift_prev;
+  double *inst_src_h = nullptr;    // pinned staging, K
+  int32_t *inst_shift_h = nullptr; // pinned staging, K
+  std::vector<int32_t> inst_shift_now, inst_shift_last;
   InstanceView instview() const {
     InstanceView v{};
     if (inst.empty()) return v;
@@ -354,6 +364,8 @@ struct Engine final : EngineBase {
     v.factor = d_inst_factor.p;
     v.etot = d_inst_etot.p;
     v.wtot = d_inst_wtot.p;
+    v.shift = d_inst_shift.p;
+    v.shift_prev = d_inst_shift_prev.p;
     return v;
   }
   uint32_t last_nfaces_ = 0;    // source faces of the last step
@@ -557,6 +569,17 @@ struct Engine final : EngineBase {
         upv(d_inst_factor, fac);
         d_inst_etot.ensure(K);
         d_inst_wtot.ensure(K);
+        const std::vector<double> zd(K, 0.0);
+        const std::vector<int32_t> zi(K, 0);
+        upv(d_inst_src, zd);
+        upv(d_inst_cen, zd);
+        upv(d_inst_mat0, zd);
+        upv(d_inst_shift, zi);
+        upv(d_inst_shift_prev, zi);
+        inst_shift_now.assign(K, 0);
+        inst_shift_last.assign(K, 0);
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&inst_src_h), K * sizeof(double)));
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&inst_shift_h), K * sizeof(int32_t)));
       }
     }
     if (p->material.sigma_a0) up(sa0, p->material.sigma_a0, N);
@@ -696,6 +719,8 @@ struct Engine final : EngineBase {
     if (stream) (void)hipStreamSynchronize(stream);
     if (mail_h) (void)hipHostFree(mail_h);
     if (wface_h) (void)hipHostFree(wface_h);
+    if (inst_src_h) (void)hipHostFree(inst_src_h);
+    if (inst_shift_h) (void)hipHostFree(inst_shift_h);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -801,18 +826,11 @@ struct Engine final : EngineBase {
     uint32_t &r = pass_slots_cache[F][b == 64 ? 0 : (b == 128 ? 1 : (b == 256 ? 2 : 3))];
     if (!r) {
       int per_cu = 0;
-      if constexpr (kUseW6<F>)
-        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport_w6<Real>, b, 0));
-      else
-        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<Real, F>, b, 0));
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<Real, F>, b, 0));
       r = cache_slots(static_cast<uint32_t>(b) * kChunksPerWave, nf, static_cast<uint32_t>(std::max(per_cu, 1)));
     }
     return r;
   }
-
-  // The FP32 gray transport runs the 6-waves-per-SIMD build (k_transport_w6).
-  template <uint32_t F>
-  static constexpr bool kUseW6 = std::is_same<Real, float>::value && F == 0u;
 
   // One transport launch on the kernel variant compiled for this problem's
   // features (random walk / DDMC, multigroup).
@@ -865,14 +883,9 @@ struct Engine final : EngineBase {
       const uint32_t nf = (precision == STORM_GPU_MIXED && groups < 2) ? 1u : 2u;
       const uint32_t slots = pass_cache_slots<F>(block, nf);
       const uint32_t chunks = kChunksPerWave * static_cast<uint32_t>(block / 64);
-      if constexpr (kUseW6<F>)
-        k_transport_w6<Real><<<blocks_for(n_in, 64u * chunks), block, cache_bytes(slots, nf), stream>>>(
-            static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
-            xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
-      else
-        k_transport<Real, F><<<blocks_for(n_in, 64u * chunks), block, cache_bytes(slots, nf), stream>>>(
-            static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
-            xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
+      k_transport<Real, F><<<blocks_for(n_in, 64u * chunks), block, cache_bytes(slots, nf), stream>>>(
+          static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
+          xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
     }
   }
   void launch_transport(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshView<Real> &mv,
@@ -957,6 +970,8 @@ struct Engine final : EngineBase {
     zero_u32(err.p, 1);
     eq_prev_ledger_ = eq_prev;
     sum_async(e_mat.p, N, dsc.p);  // material energy before the step, read by the ledger
+    if (!inst.empty())
+      k_inst_sum<<<static_cast<uint32_t>(inst.size()), 256, 0, stream>>>(d_inst_first.p, e_mat.p, d_inst_mat0.p);
 
     // preStep: opacities, Fleck factor, emission energies
     if (groups >= 2) {
@@ -1043,13 +1058,22 @@ struct Engine final : EngineBase {
     }
     last_nfaces_ = nfaces;
 
-    // energy quantum of this step (every energy below 2^62 quanta), photon
-    // allocation and offsets; one read-back for the quantum and the count
-    k_quantum<<<1, 64, 0, stream>>>(dsc.p + 2, census_total_prev, source_total, units::clight, dt, qexp.p);
-    if (!inst.empty())
-      k_inst_sum<<<static_cast<uint32_t>(inst.size()), 256, 0, stream>>>(d_inst_first.p, Eemit.p, d_inst_etot.p);
+    // energy quantum of this step (every energy below 2^62 quanta; in a batch
+    // every instance's, in its own unit), photon allocation and offsets; one
+    // read-back for the quantum and the count
+    if (inst.empty()) {
+      k_quantum<<<1, 64, 0, stream>>>(dsc.p + 2, census_total_prev, source_total, units::clight, dt, qexp.p);
+    } else {
+      const size_t K = inst.size();
+      std::fill(inst_src_h, inst_src_h + K, 0.0);
+      for (uint32_t f = 0; f < nfaces; ++f) inst_src_h[ut.src_instance[f]] += wface_e[f] * ut.src[f].count;
+      HIPCHECK(hipMemcpyAsync(d_inst_src.p, inst_src_h, K * sizeof(double), hipMemcpyHostToDevice, stream));
+      k_inst_sum<<<static_cast<uint32_t>(K), 256, 0, stream>>>(d_inst_first.p, Eemit.p, d_inst_etot.p);
+      k_inst_quantum<<<1, 256, 0, stream>>>(static_cast<uint32_t>(K), d_inst_cen.p, d_inst_etot.p, d_inst_src.p,
+                                             units::clight * dt, d_inst_shift.p, d_inst_shift_prev.p, qexp.p);
+    }
     k_alloc<<<blocks_for(N, 256), 256, 0, stream>>>(N, Eemit.p, dsc.p + 2, imc.new_photons_per_cell, qexp.p,
-                                                    nphot.p, wemit.p, emit_limbs.p, instview());
+                                                    nphot.p, wemit.p, emit_limbs.p, instview(), step_index);
     // photon counts in the cells' Z order, so emission is laid out like the
     // combed census
     const uint32_t M = 1u << morton.total;
@@ -1061,6 +1085,9 @@ struct Engine final : EngineBase {
       };
       ensure_temp(q);
     }
+    if (!inst.empty())
+      HIPCHECK(hipMemcpyAsync(inst_shift_h, d_inst_shift.p, inst.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              stream));
     const unsigned long long *hs =
         read_scalars({{offsets.p + (M - 1), 8u}, {nphot_z.p + (M - 1), 4u}, {qexp.p, 4u}, {qexp.p + 1, 4u}});
     const uint64_t n_emit = hs[0] + hs[1];
@@ -1068,7 +1095,15 @@ struct Engine final : EngineBase {
     const int32_t et = static_cast<int32_t>(static_cast<uint32_t>(hs[3]));
     k_fill_u64<<<1, 64, 0, stream>>>(reinterpret_cast<unsigned long long *>(offsets.p + M), 1, n_emit);
     counters.energy_quantum_exp = eq;
-    for (uint32_t f = 0; f < nfaces; ++f) wface_e[f] = fx::quantize_down(wface_e[f], eq);
+    if (inst.empty()) {
+      for (uint32_t f = 0; f < nfaces; ++f) wface_e[f] = fx::quantize_down(wface_e[f], eq);
+    } else {
+      // read_scalars synchronised the stream: the shifts are on the host
+      inst_shift_last = inst_shift_now;
+      inst_shift_now.assign(inst_shift_h, inst_shift_h + inst.size());
+      for (uint32_t f = 0; f < nfaces; ++f)
+        wface_e[f] = fx::quantize_down(std::ldexp(wface_e[f], inst_shift_now[ut.src_instance[f]]), eq);
+    }
     if (groups >= 2 && nfaces) {
       // Planck fractions of each source temperature per group
       // (examples/densmore2012/DensmoreBoundary.hpp:96-104), into host
@@ -1113,7 +1148,7 @@ struct Engine final : EngineBase {
     cpre.ensure(n_total);
     if (n_census)
       k_import<Real><<<blocks_for(n_census, 256), 256, 0, stream>>>(
-          static_cast<uint32_t>(n_census), cnext.soa(), pa.soa(), s0, eq_prev, eq, d, tal, requant.p);
+          static_cast<uint32_t>(n_census), cnext.soa(), pa.soa(), s0, eq_prev, eq, d, tal, requant.p, instview());
     if (n_emit && unstructured)
       k_emit_u<Real><<<blocks_for(n_emit, 256), 256, 0, stream>>>(
           static_cast<uint32_t>(n_emit), M, offsets.p, wemit.p, pa.soa(), static_cast<uint32_t>(n_census), s0, seed,
@@ -1199,10 +1234,12 @@ struct Engine final : EngineBase {
     k_post<Real><<<blocks_for(N, 256), 256, 0, stream>>>(
         N, tal, eq, et, eq_prev, emit_limbs.p, requant.p, dt, V.p, md,
         material_mode == STORM_GPU_MATERIAL_DEVICE ? 1 : 0, T.p, e_mat.p, e_sync.p, erad_avg.p, census_erad.p,
-        clamp.p, err.p, precision == STORM_GPU_FP64 ? 1 : 0, xs.p, groups >= 2 ? 0 : 1);
+        clamp.p, err.p, precision == STORM_GPU_FP64 ? 1 : 0, xs.p, groups >= 2 ? 0 : 1, instview());
     // census energy for the next step's quantum, total segments, error flags
     // and counters: one synchronisation
     sum_async(census_erad.p, N, dsc.p + 1);
+    if (!inst.empty())
+      k_inst_sum<<<static_cast<uint32_t>(inst.size()), 256, 0, stream>>>(d_inst_first.p, census_erad.p, d_inst_cen.p);
     k_total_u32<<<std::min<uint32_t>(blocks_for(N, 256), 1024u), 256, 0, stream>>>(N, tal.segs, leaving.p + 3);
     const unsigned long long *hp = read_scalars({{dsc.p + 1, 8u}, {err.p, 4u}, {leaving.p, 8u}, {leaving.p + 1, 8u},
                                                  {leaving.p + 2, 8u}, {leaving.p + 3, 8u}, {leaving.p + 4, 8u}});
@@ -1482,7 +1519,15 @@ struct Engine final : EngineBase {
     return r;
   }
 
-  void get_ledger(storm_gpu_ledger *L) override {
+  void get_ledger(storm_gpu_ledger *L) override { ledger_range(L, -1); }
+
+  // The ledger over all cells (k < 0) or over the cells and source faces of
+  // instance k of a batch.  Integer fields count quanta of 2^eq erg; over a
+  // whole batch they count the step's quantum in the instances' units (each
+  // balance is exact, the erg figures are per instance).
+  void ledger_range(storm_gpu_ledger *L, int64_t k) {
+    const uint32_t c0 = k < 0 ? 0u : inst_first_h[k], c1 = k < 0 ? N : inst_first_h[k + 1];
+    const int32_t sh = k < 0 ? 0 : inst_shift_now[k], shp = k < 0 ? 0 : inst_shift_last[k];
     const size_t n2 = 2 * static_cast<size_t>(N);
     std::vector<unsigned long long> h(tally_u64.n), em(n2), rq(n2);
     HIPCHECK(hipMemcpy(h.data(), tally_u64.p, h.size() * 8, hipMemcpyDeviceToHost));
@@ -1490,7 +1535,7 @@ struct Engine final : EngineBase {
     HIPCHECK(hipMemcpy(rq.data(), requant.p, n2 * 8, hipMemcpyDeviceToHost));
     auto sum = [&](const unsigned long long *a) {
       unsigned __int128 s = 0;
-      for (uint32_t c = 0; c < N; ++c) s += (static_cast<unsigned __int128>(a[2 * c + 1]) << fx::kLoBits) + a[2 * c];
+      for (uint32_t c = c0; c < c1; ++c) s += (static_cast<unsigned __int128>(a[2 * c + 1]) << fx::kLoBits) + a[2 * c];
       return s;
     };
     const unsigned long long *b = h.data();
@@ -1504,7 +1549,7 @@ struct Engine final : EngineBase {
       std::vector<double> f(tally_f64.n);
       HIPCHECK(hipMemcpy(f.data(), tally_f64.p, f.size() * sizeof(double), hipMemcpyDeviceToHost));
       long double dsum = 0.0L, esum = 0.0L;
-      for (uint32_t c = 0; c < N; ++c) {
+      for (uint32_t c = c0; c < c1; ++c) {
         dsum += f[c];
         esum += f[2 * static_cast<size_t>(N) + c];
       }
@@ -1529,28 +1574,34 @@ struct Engine final : EngineBase {
       std::vector<double> wf(last_nfaces_);
       HIPCHECK(hipMemcpy(wf.data(), wface.p, wf.size() * sizeof(double), hipMemcpyDeviceToHost));
       for (uint32_t f = 0; f < last_nfaces_; ++f) {
+        if (k >= 0 && ut.src_instance[f] != static_cast<uint32_t>(k)) continue;
         const uint64_t npc = unstructured ? ut.src[f].count : bnd.source_packets_per_face;
         fx::Limbs l = fx::limbs_of(wf[f], last_eq_);
         src += ((static_cast<unsigned __int128>(l.hi) << fx::kLoBits) + l.lo) * npc;
       }
     }
     L->source = u128(src);
-    L->eq = last_eq_;
-    L->eq_prev = eq_prev_ledger_;
+    L->eq = last_eq_ - sh;
+    L->eq_prev = eq_prev_ledger_ - shp;
     double rq_sum = 0.0;
-    for (uint32_t c = 0; c < N; ++c) rq_sum += fx::to_double(rq[2 * c], rq[2 * c + 1], eq_prev_ledger_);
+    for (uint32_t c = c0; c < c1; ++c) rq_sum += fx::to_double(rq[2 * c], rq[2 * c + 1], L->eq_prev);
     L->requant_rem = rq_sum;
     std::vector<double> cl(N), em2(N);
     HIPCHECK(hipMemcpy(cl.data(), clamp.p, N * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(em2.data(), e_mat.p, N * sizeof(double), hipMemcpyDeviceToHost));
     double cls = 0.0, ms = 0.0;
-    for (uint32_t c = 0; c < N; ++c) {
+    for (uint32_t c = c0; c < c1; ++c) {
       cls += cl[c];
       ms += em2[c];
     }
     L->clamp = cls;
     L->material_after = ms;
-    HIPCHECK(hipMemcpy(&L->material_before, dsc.p, sizeof(double), hipMemcpyDeviceToHost));
+    if (k < 0) HIPCHECK(hipMemcpy(&L->material_before, dsc.p, sizeof(double), hipMemcpyDeviceToHost));
+    else HIPCHECK(hipMemcpy(&L->material_before, d_inst_mat0.p + k, sizeof(double), hipMemcpyDeviceToHost));
+  }
+  void get_instance_ledger(uint32_t k, storm_gpu_ledger *L) override {
+    if (k >= inst.size()) throw EngineError(1, "no such instance");
+    ledger_range(L, static_cast<int64_t>(k));
   }
   int32_t eq_prev_ledger_ = 0;
 
@@ -1604,7 +1655,8 @@ struct Engine final : EngineBase {
         r.vx = static_cast<double>(hd[q].x) * units::clight;
         r.vy = static_cast<double>(hd[q].y) * units::clight;
         r.vz = static_cast<double>(hd[q].z) * units::clight;
-        r.weight = hw[q];
+        // batches: from the instance's unit to erg
+        r.weight = inst.empty() ? hw[q] : std::ldexp(hw[q], -inst_shift_now[cell_inst_h[c]]);
         r.frequency = groups >= 2 ? static_cast<double>(hp[q].w) * units::kev : 0.0;
         r.cell = c;
         r.id = ~0ull;
@@ -1722,6 +1774,16 @@ struct Engine final : EngineBase {
     for (double w : hw) totq += w;
     census_total_prev = totq;
     eq_prev = eq_imp;
+    if (!inst.empty()) {
+      // imported weights are in erg: every instance's unit is 1 erg until the
+      // next step picks the new ones
+      const size_t K = inst.size();
+      std::vector<double> ck(K, 0.0);
+      for (uint64_t q = 0; q < n; ++q) ck[cell_inst_h[hcell[q]]] += hw[q];
+      inst_shift_now.assign(K, 0);
+      HIPCHECK(hipMemcpy(d_inst_shift.p, inst_shift_now.data(), K * sizeof(int32_t), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(d_inst_cen.p, ck.data(), K * sizeof(double), hipMemcpyHostToDevice));
+    }
   }
 };
 
@@ -1852,6 +1914,10 @@ int storm_gpu_get_counters(storm_gpu_engine *eng, storm_gpu_counters *out) {
 
 int storm_gpu_get_ledger(storm_gpu_engine *eng, storm_gpu_ledger *out) {
   return guarded([&] { as_engine(eng)->get_ledger(out); });
+}
+
+int storm_gpu_get_instance_ledger(storm_gpu_engine *eng, uint32_t k, storm_gpu_ledger *out) {
+  return guarded([&] { as_engine(eng)->get_instance_ledger(k, out); });
 }
 
 int storm_gpu_census_size(storm_gpu_engine *eng, uint64_t *n) {

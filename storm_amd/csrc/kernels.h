@@ -428,24 +428,72 @@ __global__ void k_quantum(const double *__restrict__ Etot, double census_prev, d
   q[1] = Ebound > 0.0 ? fx::ceil_log2(Ebound * c * dt) - fx::kHeadroomBits : -1000;
 }
 
-// photons per cell and packet weight, quantised down to the step quantum
-// Batches (inst.cell non-null): each instance allocates over its own cells
-// with its own newPhotonsPerCell and emission total.
+// Batches of independent instances (inst.cell non-null).  Each instance counts energy in its own unit, 2^-shift erg (shift >= 0, set
+// every step by k_inst_quantum): packet weights and integer tallies of its
+// cells are in that unit, so the step's one quantum 2^eq is 2^(eq - shift)
+// erg for it, the quantum a stand-alone run of the instance would use.  The
+// transport is linear in the weights and never mixes instances, so it runs
+// unchanged on the scaled weights.
 struct InstanceView {
-  const uint32_t *cell;     // instance of each cell (null: one problem)
-  const uint32_t *ncells;   // per instance
-  const uint64_t *nphot;    // newPhotonsPerCell per instance
-  const uint64_t *nmin;     // comb Nmin per instance
-  const double *factor;     // comb totalParticlesFactor per instance
-  const double *etot;       // emission total per instance (k_inst_sum)
-  const double *wtot;       // census weight per instance, erg (k_inst_census)
+  const uint32_t *cell;      // instance of each cell (null: one problem)
+  const uint32_t *ncells;    // per instance
+  const uint64_t *nphot;     // newPhotonsPerCell per instance
+  const uint64_t *nmin;      // comb Nmin per instance
+  const double *factor;      // comb totalParticlesFactor per instance
+  const double *etot;        // emission total per instance, erg (k_inst_sum)
+  const double *wtot;        // census weight per instance, in its unit (k_inst_census)
+  const int32_t *shift;      // unit exponent of this step, per instance
+  const int32_t *shift_prev; // unit exponent of the last step (the census' unit)
 };
 
+// Quantum and unit exponents of a step of a batch: instance k's energy bound
+// E_k (last census + emission + boundary source, erg) gives its own exponent
+// e_k = ceil(log2 E_k) - 62, the one k_quantum gives a stand-alone problem;
+// the step's quantum is eq = max_k e_k and instance k's unit 2^-(eq - e_k)
+// erg, so every instance has E_k below 2^62 quanta.  The path-integral
+// quantum is eq + ceil(log2(c dt)).  The last step's shifts move to
+// shift_prev first.  One block.
+__global__ __launch_bounds__(256) void k_inst_quantum(uint32_t K, const double *__restrict__ census,
+                                                      const double *__restrict__ etot,
+                                                      const double *__restrict__ src, double cdt,
+                                                      int32_t *__restrict__ shift, int32_t *__restrict__ shift_prev,
+                                                      int32_t *__restrict__ q) {
+  __shared__ int32_t sh[256];
+  int32_t m = -1000;
+  for (uint32_t k = threadIdx.x; k < K; k += 256u) {
+    shift_prev[k] = shift[k];
+    const double E = census[k] + etot[k] + src[k];
+    if (E > 0.0) {
+      const int32_t e = fx::ceil_log2(E) - fx::kHeadroomBits;
+      m = e > m ? e : m;
+    }
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (uint32_t w = 128u; w > 0u; w >>= 1) {
+    if (threadIdx.x < w && sh[threadIdx.x + w] > sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  const int32_t eq = sh[0];
+  for (uint32_t k = threadIdx.x; k < K; k += 256u) {
+    const double E = census[k] + etot[k] + src[k];
+    shift[k] = E > 0.0 ? eq - (fx::ceil_log2(E) - fx::kHeadroomBits) : 0;
+  }
+  if (threadIdx.x == 0) {
+    q[0] = eq;
+    q[1] = eq == -1000 ? -1000 : eq + fx::ceil_log2(cdt);
+  }
+}
+
+// photons per cell and packet weight, rounded to the step quantum without bias.
+// Batches: each instance allocates over its own cells with its own
+// newPhotonsPerCell and emission total, weights in its unit.
 __global__ __launch_bounds__(256) void k_alloc(uint32_t N, const double *__restrict__ Eemit,
                                                const double *__restrict__ Etot_p, uint64_t n_min_all,
                                                const int32_t *__restrict__ q, uint32_t *__restrict__ nphot,
                                                double *__restrict__ wemit,
-                                               unsigned long long *__restrict__ emit_limbs, InstanceView inst) {
+                                               unsigned long long *__restrict__ emit_limbs, InstanceView inst,
+                                               uint32_t step) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
   double Etot = *Etot_p;
@@ -465,7 +513,11 @@ __global__ __launch_bounds__(256) void k_alloc(uint32_t N, const double *__restr
     n = share < 20ull * n_min ? share : 20ull * n_min;
     if (n < n_min) n = n_min;
   }
-  double w = n ? fx::quantize_down(E / static_cast<double>(n), eq) : 0.0;
+  double w = 0.0;
+  if (n) {
+    const double per = E / static_cast<double>(n);
+    w = fx::emission_round(inst.cell ? ldexp(per, inst.shift[inst.cell[c]]) : per, eq, c, step);
+  }
   if (w == 0.0) n = 0;  // below one quantum: nothing to emit
   nphot[c] = static_cast<uint32_t>(n);
   wemit[c] = w;
@@ -484,10 +536,14 @@ __device__ __forceinline__ Real uni(uint32_t x) {
 // Census re-entry: timeLeft = dt, initialWeight = |w| (MonteCarloManagerSerial.hpp:393-405).
 // Weights are re-quantised if the quantum grew; the remainder (in the old
 // quantum) goes to the material.
+// Batches: the census is in its instances' units of the last step; a weight
+// moves to this step's unit (x 2^(shift - shift_prev), exact) after the
+// re-quantisation, which runs in the old unit (new quantum 2^(eq_new - delta)).
 template <typename Real>
 __global__ __launch_bounds__(256) void k_import(uint32_t n, CensusSoA<Real> cen, PacketSoA<Real> out,
                                                 Real s0, int32_t eq_old, int32_t eq_new, MeshDims d,
-                                                Tallies t, unsigned long long *__restrict__ requant) {
+                                                Tallies t, unsigned long long *__restrict__ requant,
+                                                InstanceView inst) {
   // no early return: the per-cell sums below need every lane of the wavefront
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n;
@@ -507,11 +563,17 @@ __global__ __launch_bounds__(256) void k_import(uint32_t n, CensusSoA<Real> cen,
     p.o[2] = b.z;
     double w = cen.w[i];
     c = cell_of_code(m.x, d);
-    if (eq_new > eq_old) {
-      double wq = fx::quantize_down(w, eq_new);
+    int32_t delta = 0;
+    if (inst.cell) {
+      const uint32_t k = inst.cell[c];
+      delta = inst.shift[k] - inst.shift_prev[k];
+    }
+    if (eq_new - delta > eq_old) {
+      double wq = fx::quantize_down(w, eq_new - delta);
       if (wq != w) tally_limbs(requant, c, fx::limbs_of(w - wq, eq_old));  // exact in the old quantum
       w = wq;
     }
+    if (delta) w = ldexp(w, delta);
     p.w = w;
     p.wmin = static_cast<Real>(w * 1e-3);
     p.ijk = m.x;
@@ -840,19 +902,25 @@ __device__ __forceinline__ void transport_chunks(STORM_TRANSPORT_ARGS) {
   }
 }
 
+// Waves per SIMD the history kernel is compiled for.  FP32: 6 (80 VGPRs, a
+// few dozen bytes of spill), measured 17% faster than the compiler's own
+// choice of 4 (98-118 VGPRs) on the 64^3 cube with the random walk and 5%
+// faster than 5 on the gray cube.  FP64: the compiler's choice (2-3 waves);
+// at 6 it spills 200-500 bytes per lane.
+template <typename Real>
+struct TransportWaves {
+  static constexpr int kMin = 1;
+};
+template <>
+struct TransportWaves<float> {
+  static constexpr int kMin = 6;
+};
+
 template <typename Real, uint32_t F, typename MeshT = MeshView<Real>>
-__global__ __launch_bounds__(512) void k_transport(STORM_TRANSPORT_ARGS) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(TransportWaves<Real>::kMin, 8))) void
+k_transport(STORM_TRANSPORT_ARGS) {
   transport_chunks<Real, F, MeshT>(n_in, in, out, n_out, cen, n_cen, mesh, xs, acc, k, max_segments, t, stats,
                                    cache_slots, cache_fields, chunks);
-}
-// The FP32 gray kernel compiled for 6 waves per SIMD (80 VGPRs, a few bytes
-// of spill): measured 5% faster than at 5 (91 VGPRs); the variants with the
-// random walk or multigroup spill more and keep k_transport's allocation.
-template <typename Real, typename MeshT = MeshView<Real>>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_transport_w6(
-    STORM_TRANSPORT_ARGS) {
-  transport_chunks<Real, 0u, MeshT>(n_in, in, out, n_out, cen, n_cen, mesh, xs, acc, k, max_segments, t, stats,
-                                    cache_slots, cache_fields, chunks);
 }
 #undef STORM_TRANSPORT_ARGS
 
@@ -1430,14 +1498,23 @@ __global__ __launch_bounds__(256) void k_post(uint32_t N, Tallies t, int32_t eq,
                                               double *__restrict__ e_sync, double *__restrict__ erad_avg,
                                               double *__restrict__ census_erad, double *__restrict__ clamp,
                                               uint32_t *__restrict__ err, int32_t fp64,
-                                              const CellXS<Real> *__restrict__ xs, int32_t gray) {
+                                              const CellXS<Real> *__restrict__ xs, int32_t gray,
+                                              InstanceView inst) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
+  int32_t sh = 0;  // batches: the cell's instance unit is 2^-sh erg (2^-sh_prev for the census' requant)
+  if (inst.cell) {
+    const uint32_t k = inst.cell[c];
+    sh = inst.shift[k];
+    eq_old -= inst.shift_prev[k];
+    eq -= sh;
+    et -= sh;
+  }
   const double emit = fx::to_double(emit_limbs[2 * c], emit_limbs[2 * c + 1], eq);
   double dep, pint;
   if (fp64) {
-    dep = t.dep_fp[c];
-    pint = t.pint_fp[c];
+    dep = ldexp(t.dep_fp[c], -sh);
+    pint = ldexp(t.pint_fp[c], -sh);
   } else {
     dep = fx::to_double(t.dep[2 * c], t.dep[2 * c + 1], eq);
     // Each segment absorbs w (1 - e^{-x}) and adds w (1 - e^{-x}) / sig_abs to

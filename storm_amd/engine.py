@@ -105,9 +105,13 @@ class GpuEngine:
         self._check(self.lib.storm_gpu_get_counters(self.h, C.byref(c)))
         return {name: getattr(c, name) for name, _ in c._fields_}
 
-    def ledger(self) -> dict:
+    def ledger(self, instance: int | None = None) -> dict:
+        """Energy ledger of the last step; `instance`: that instance of a batch."""
         L = capi.Ledger()
-        self._check(self.lib.storm_gpu_get_ledger(self.h, C.byref(L)))
+        if instance is None:
+            self._check(self.lib.storm_gpu_get_ledger(self.h, C.byref(L)))
+        else:
+            self._check(self.lib.storm_gpu_get_instance_ledger(self.h, int(instance), C.byref(L)))
         out = {}
         for name, typ in L._fields_:
             v = getattr(L, name)

@@ -601,7 +601,9 @@ def c4_voronoi(seed: int = 0, n: int = 200000, groups: int = 64, steps: int = 20
     examples/densmore2012/DensmoreOpacity.hpp:58-66), density
     1 + 0.5 sin(2 pi x), e = 1e15 erg/(g keV) * rho * T, T0 = 0.1 keV, a 1 keV
     source on the x- faces, escape on every face, comb; ~5e7 packets per step
-    at packets_scale = 1."""
+    at packets_scale = 1: 1e7 from the source, 2e7 emitted (the emission
+    budget is 10 NewPhotonsPerCell per cell), a census combed to 100-200 per
+    cell."""
     m = mesh if mesh is not None else voronoi_mesh(voronoi_points(n, 0xC4))
     N = m.ncells
     rho = 1.0 + 0.5 * np.sin(2.0 * np.pi * m.mesh_point[:, 0])
@@ -616,7 +618,7 @@ def c4_voronoi(seed: int = 0, n: int = 200000, groups: int = 64, steps: int = 20
         T_ref=KEV_K, T_floor=1.0, a_pow=0.5, eos_f=1e15 / KEV_K, eos_beta=1.0, eos_mu=0.0,
         face_kind=(capi.FACE_ESCAPE,) * 6, source_face=capi.XLO, source_T=KEV_K, freeze_T4=1,
         source_per_face=max(1, int(round(1e7 * s / max(nsrc, 1)))),
-        new_photons_per_cell=max(1, int(round(100 * s))), comb=1, comb_nmin=max(1, int(round(100 * s))),
+        new_photons_per_cell=max(1, int(round(10 * s))), comb=1, comb_nmin=max(1, int(round(100 * s))),
         comb_factor=1.0, seed=seed, umesh=m, group_bounds=b, nu_pow=3.0, nu_ref=KEV)
 
 

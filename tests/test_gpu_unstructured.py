@@ -62,7 +62,8 @@ def test_voronoi_energy_ledger_exact(make):
     with GpuEngine(spec, capi.MIXED) as e:
         for _ in range(spec.steps):
             e.step()
-            assert _ledger_ok(e.ledger())
+            ok, detail = _ledger_ok(e.ledger())
+            assert ok, detail
 
 
 @pytest.mark.parametrize("tau", [0.5, 2.0])
