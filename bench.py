@@ -214,7 +214,7 @@ def c3_line(W, K, log):
                 value=seg / wall, unit="segments/s", steps=K, warmup=W, ms_per_step=wall / K * 1e3,
                 packets_per_step=packets / K, transport_segments_per_s=seg / t_tr,
                 random_walk_steps_per_step=rw / K, ddmc_steps_per_step=ddmc / K,
-                kernel="k_transport<float, 1u> (random walk + DDMC features)",
+                kernel="k_transport<float, 1u, MeshView<float>> (random walk + DDMC features)",
                 roofline_frac=seg * B_SEG_MIXED / t_tr / 1e9 / HBM_PEAK_GBS)
 
 
@@ -421,11 +421,12 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": "k_transport<float, 1u>",
+            "kernel": "k_transport<float, 1u, MeshView<float>>",
             "bytes_per_segment": B_SEG_MIXED,
+            "segments": tr_seg,
             "launches": tr_n,
             "avg_launch_us": tr_t / tr_n * 1e6,
-            "over": f"every launch of k_transport<float, 1u> in this process: the C2 cube's {len(seeds)} seeds x "
+            "over": f"every launch of k_transport<float, 1u, MeshView<float>> in this process: the C2 cube's {len(seeds)} seeds x "
                     f"{n_steps} steps" + ("" if c3 is None else " and the c3 line's 5 steps"),
         },
         "transport_segments_per_s": c2_transport_rate,
