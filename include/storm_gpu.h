@@ -163,6 +163,13 @@ int storm_gpu_get_cells(storm_gpu_engine *eng, double *T, double *internal_energ
                         double *erad_time_avg, double *census_erad, double *fleck,
                         double *planck_opacity);
 
+/* Time-averaged radiation energy density of the last step per cell and
+ * photon group, out[cell * G + g] (erg/cm^3): getEgTimeAvg
+ * (RadiationIMC.hpp:433-434, tallied at :1166-1173, :1485-1492, :1802-1809,
+ * normalised at :1896-1903).  Needs a multigroup problem created with
+ * storm_group_desc.with_eg_time_avg (withEgTimeAvg); status 1 otherwise. */
+int storm_gpu_get_group_erad(storm_gpu_engine *eng, double *out);
+
 /* GetCellsStepsCounters (MonteCarloManagerSerial.hpp:74-75). */
 int storm_gpu_get_segments(storm_gpu_engine *eng, uint64_t *per_cell);
 

@@ -127,7 +127,9 @@ typedef struct storm_imc_params {
  */
 typedef struct storm_group_desc {
   int32_t num_groups;
-  int32_t reserved;
+  int32_t with_eg_time_avg; /* RadiationIMCParameters::withEgTimeAvg: tally the
+                               time-averaged radiation energy per cell and group
+                               (storm_gpu_get_group_erad) */
   const double *bounds; /* num_groups + 1 ascending photon energies, erg */
   double nu_pow;
   double nu_ref;        /* erg */

@@ -1060,6 +1060,7 @@ struct IMCParams {  // radiation/RadiationIMCParameters.hpp:12-56 (gray subset)
   double ddmcMinParticleOpticalDepth = 5.0;
   bool ddmcUseMultigroupPGRW = false;
   bool noHydroFeedback = false;
+  bool withEgTimeAvg = false;  // per-group Erad_time_avg (multigroup only)
 };
 
 struct EnergyLedger {  // diagnostics for the per-step balance (not in the reference)
@@ -1138,6 +1139,9 @@ class RadiationIMC {
   const std::vector<double> &factorFleck() const { return factorFleck_; }
   const std::vector<double> &planckOpacities() const { return planck_; }
   const std::vector<double> &eradTimeAvg() const { return Erad_time_avg_; }
+  // getEgTimeAvg (RadiationIMC.hpp:433-434): [cell * G + g], empty unless
+  // withEgTimeAvg and multigroup
+  const std::vector<double> &egTimeAvg() const { return Eg_time_avg_; }
   const std::vector<size_t> &photonsPerCell() const { return nPhotons_; }
   const std::vector<double> &emissionEnergy() const { return Eemit_; }
   const RandomWalk *randomWalk() const { return rw_.get(); }
@@ -1152,6 +1156,7 @@ class RadiationIMC {
     planck_.assign(N, 0.0);
     factorFleck_.assign(N, 1.0);
     Erad_time_avg_.assign(N, 0.0);
+    if (egOn()) Eg_time_avg_.assign(N * (mat_.bounds.size() - 1), 0.0);  // :1626-1630
     for (size_t i = 0; i < N; ++i) {
       Cell &cell = cells_[i];
       planck_[i] = mat_.CalcPlanckOpacity(i, cell.temperature);
@@ -1308,6 +1313,7 @@ class RadiationIMC {
     p.location += p.velocity * dt;
     if (!params_.noHydroFeedback) ext_[c].internal_energy += -e2 * p.weight;
     Erad_time_avg_[c] += integrated;
+    egTally(c, p.frequency, integrated);  // :1802-1809
     p.weight *= 1.0 + e1;
     if (std::abs(p.weight) < p.initialWeight * 1e-3) {
       fn.change = REMOVE;
@@ -1338,6 +1344,11 @@ class RadiationIMC {
   void postStep(const std::vector<Particle> &census, double fullDt) {
     size_t N = grid_.GetPointNo();
     for (size_t i = 0; i < N; ++i) Erad_time_avg_[i] /= (fullDt * grid_.GetVolume(i));
+    if (egOn()) {  // :1896-1903
+      const size_t G = mat_.bounds.size() - 1;
+      for (size_t i = 0; i < N; ++i)
+        for (size_t g = 0; g < G; ++g) Eg_time_avg_[i * G + g] /= fullDt * grid_.GetVolume(i);
+    }
     if (!params_.noHydroFeedback) {
       for (size_t i = 0; i < N; ++i) {
         throwIfNegativeInternalEnergy(i);
@@ -1486,7 +1497,10 @@ class RadiationIMC {
     double rate = sa * f * units::clight;
     double ex = std::expm1(-dt * rate);
     if (!params_.noHydroFeedback) ext_[c].internal_energy += -ex * p.weight;
-    if (rate > 0.0) Erad_time_avg_[c] += p.weight * ex * (-1.0 / rate);
+    if (rate > 0.0) {
+      Erad_time_avg_[c] += p.weight * ex * (-1.0 / rate);
+      egTally(c, p.frequency, p.weight * ex * (-1.0 / rate));  // :1166-1173
+    }
     p.weight *= 1.0 + ex;
     p.timeLeft -= dt;
     if (std::abs(p.weight) < p.initialWeight * 1e-4) {
@@ -1653,6 +1667,7 @@ class RadiationIMC {
     if (!params_.noHydroFeedback) ext_[c].internal_energy += -ex * oldW;
     double integ = (absRate > 0.0) ? oldW * ex * (-1.0 / absRate) : oldW * dt;
     Erad_time_avg_[c] += integ;
+    egTally(c, p.frequency, integ);  // :1485-1492
     p.weight *= 1.0 + ex;
     p.timeLeft -= dt;
     if (std::abs(p.weight) < p.initialWeight * 1e-3) {
@@ -1717,7 +1732,19 @@ class RadiationIMC {
   Rng rng_;
   RandomInCellSampler<Grid> sampler_;
   std::vector<std::vector<Vec3>> normals_, facePts_;
-  std::vector<double> factorFleck_, planck_, Erad_time_avg_, Eemit_;
+  std::vector<double> factorFleck_, planck_, Erad_time_avg_, Eemit_, Eg_time_avg_;
+  bool egOn() const { return params_.withEgTimeAvg && mat_.multigroup(); }
+  // RadiationOpacityModel::findGroup (RadiationOpacityModel.hpp:76-86): the
+  // first group whose upper bound lies above the energy, else the last
+  size_t findGroup(double E) const {
+    const size_t G = mat_.bounds.size() - 1;
+    for (size_t g = 0; g < G; ++g)
+      if (E < mat_.bounds[g + 1]) return g;
+    return G - 1;
+  }
+  void egTally(size_t c, double E, double v) {
+    if (egOn()) Eg_time_avg_[c * (mat_.bounds.size() - 1) + findGroup(E)] += v;
+  }
   std::vector<size_t> nPhotons_;
   std::unique_ptr<RandomWalk> rw_;
   std::vector<char> rwEligible_;

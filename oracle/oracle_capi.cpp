@@ -52,7 +52,7 @@ constexpr size_t kLedgerStride = 13;
 template <typename Grid>
 int run_schedule_on(const Grid &grid, const storm_problem *prob, int nsteps, const double *dts, const double *source_T,
                     double *T_out, double *e_out, double *erad_out, double *census_out, double *segments_out,
-                    double *ledger_out) {
+                    double *ledger_out, double *eg_out) {
   try {
     size_t N = grid.GetPointNo();
     Material mat = makeMaterial(prob->material, N, &prob->groups);
@@ -77,6 +77,7 @@ int run_schedule_on(const Grid &grid, const storm_problem *prob, int nsteps, con
     ip.ddmcMinCellOpticalDepth = prob->imc.ddmc_min_cell_optical_depth;
     ip.ddmcMinParticleOpticalDepth = prob->imc.ddmc_min_particle_optical_depth;
     ip.ddmcUseMultigroupPGRW = prob->imc.ddmc_multigroup_pgrw != 0;
+    ip.withEgTimeAvg = prob->groups.with_eg_time_avg != 0;
     RadiationIMC<Grid> imc(grid, &bc, cells, ext, mat, ip, prob->seed);
     std::unique_ptr<Comb<Grid>> comb;
     if (prob->imc.comb) comb = std::make_unique<Comb<Grid>>(grid, prob->imc.comb_nmin, prob->imc.comb_factor, prob->seed);
@@ -122,6 +123,10 @@ int run_schedule_on(const Grid &grid, const storm_problem *prob, int nsteps, con
       if (census_out) census_out[i] = ext[i].Erad;
       if (segments_out) segments_out[i] = static_cast<double>(mgr.cellsStepsCounters[i]);
     }
+    if (eg_out) {
+      const std::vector<double> &eg = imc.egTimeAvg();
+      std::copy(eg.begin(), eg.end(), eg_out);
+    }
     return 0;
   } catch (const std::exception &e) {
     g_err = e.what();
@@ -149,6 +154,13 @@ int oracle_run_schedule(const storm_problem *prob, int nsteps, const double *dts
                         double *T_out, double *e_out, double *erad_out, double *census_out,
                         double *segments_out, double *ledger_out);
 
+// oracle_run_schedule with the per-group time-averaged radiation energy of
+// the last step as well (eg_out: ncells * num_groups, [cell * G + g]; filled
+// when the problem asks for it, storm_group_desc.with_eg_time_avg).
+int oracle_run_groups(const storm_problem *prob, int nsteps, const double *dts, const double *source_T,
+                      double *T_out, double *e_out, double *erad_out, double *census_out, double *segments_out,
+                      double *ledger_out, double *eg_out);
+
 int oracle_run(const storm_problem *prob, int nsteps, double dt, const double *source_T,
                double *T_out, double *e_out, double *erad_out, double *census_out,
                double *segments_out, double *ledger_out) {
@@ -162,15 +174,22 @@ int oracle_run(const storm_problem *prob, int nsteps, double dt, const double *s
 int oracle_run_schedule(const storm_problem *prob, int nsteps, const double *dts, const double *source_T,
                         double *T_out, double *e_out, double *erad_out, double *census_out,
                         double *segments_out, double *ledger_out) {
+  return oracle_run_groups(prob, nsteps, dts, source_T, T_out, e_out, erad_out, census_out, segments_out, ledger_out,
+                           nullptr);
+}
+
+int oracle_run_groups(const storm_problem *prob, int nsteps, const double *dts, const double *source_T,
+                      double *T_out, double *e_out, double *erad_out, double *census_out, double *segments_out,
+                      double *ledger_out, double *eg_out) {
   try {
     if (prob->unstructured) {
       UnstructuredGrid grid(*prob->unstructured);
       return run_schedule_on(grid, prob, nsteps, dts, source_T, T_out, e_out, erad_out, census_out, segments_out,
-                             ledger_out);
+                             ledger_out, eg_out);
     }
     CartesianGrid grid = makeGrid(prob->mesh);
     return run_schedule_on(grid, prob, nsteps, dts, source_T, T_out, e_out, erad_out, census_out, segments_out,
-                           ledger_out);
+                           ledger_out, eg_out);
   } catch (const std::exception &e) {
     g_err = e.what();
     return 1;

@@ -35,6 +35,7 @@ def lib():
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_run.argtypes = [C.c_void_p, C.c_int, C.c_double, dp, dp, dp, dp, dp, dp, dp]
         L.oracle_run_schedule.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.oracle_run_groups.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.oracle_prestep.argtypes = [C.c_void_p, C.c_double, dp, dp, dp, dp]
         L.oracle_rw_tables.argtypes = [dp, dp, dp]
         L.oracle_rw_sample.argtypes = [C.c_int, dp, dp, dp, dp]
@@ -57,7 +58,9 @@ def _null():
 
 
 def run(spec, steps: int | None = None, dt: float | None = None, source_T=None) -> dict:
-    """Run the oracle on a storm_amd.problems.ProblemSpec; per-cell results of the last step."""
+    """Run the oracle on a storm_amd.problems.ProblemSpec; per-cell results of
+    the last step ("eg": per cell and group, [cell, group], when the problem
+    asks for the per-group tally)."""
     L = lib()
     prob = spec.to_c()
     n = spec.ncells
@@ -66,9 +69,14 @@ def run(spec, steps: int | None = None, dt: float | None = None, source_T=None) 
     out = {k: np.empty(n) for k in ("T", "e", "erad", "census", "segments")}
     ledger = np.empty(steps * LEDGER_STRIDE)
     sT = None if source_T is None else np.ascontiguousarray(source_T, dtype=np.float64)
-    rc = L.oracle_run(C.byref(prob), steps, dt, _null() if sT is None else _dp(sT),
-                      _dp(out["T"]), _dp(out["e"]), _dp(out["erad"]), _dp(out["census"]),
-                      _dp(out["segments"]), _dp(ledger))
+    G = prob.groups.num_groups if prob.groups.num_groups >= 2 and prob.groups.with_eg_time_avg else 0
+    eg = np.zeros(max(n * G, 1))
+    dts = np.full(steps, float(dt))
+    rc = L.oracle_run_groups(C.byref(prob), steps, _dp(dts), _null() if sT is None else _dp(sT),
+                             _dp(out["T"]), _dp(out["e"]), _dp(out["erad"]), _dp(out["census"]),
+                             _dp(out["segments"]), _dp(ledger), _dp(eg) if G else _null())
+    if G:
+        out["eg"] = eg.reshape(n, G)
     if rc != 0:
         raise RuntimeError("oracle: " + L.oracle_last_error().decode())
     led = ledger.reshape(steps, LEDGER_STRIDE)

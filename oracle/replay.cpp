@@ -59,6 +59,9 @@ struct HostSink {
   }
   void rw_step() { ++rw; }
   void ddmc_step() { ++ddmc; }
+  // the replay does not check the per-group tally (the problems it replays
+  // leave it off)
+  void group_path(uint32_t, int32_t, double, const StepConsts &, uint32_t) {}
 };
 
 // Random-walk tables in the engine's precision (same host build as the engine).

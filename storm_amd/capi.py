@@ -53,7 +53,7 @@ class ImcParams(C.Structure):
 
 
 class GroupDesc(C.Structure):
-    _fields_ = [("num_groups", C.c_int32), ("reserved", C.c_int32), ("bounds", _dp),
+    _fields_ = [("num_groups", C.c_int32), ("with_eg_time_avg", C.c_int32), ("bounds", _dp),
                 ("nu_pow", C.c_double), ("nu_ref", C.c_double)]
 
 
@@ -151,6 +151,7 @@ def load_library(path: str | None = None):
         "storm_gpu_get_counters": ([vp, C.POINTER(Counters)], C.c_int),
         "storm_gpu_get_ledger": ([vp, C.POINTER(Ledger)], C.c_int),
         "storm_gpu_get_instance_ledger": ([vp, C.c_uint32, C.POINTER(Ledger)], C.c_int),
+        "storm_gpu_get_group_erad": ([vp, _dp], C.c_int),
         "storm_gpu_census_size": ([vp, C.POINTER(C.c_uint64)], C.c_int),
         "storm_gpu_export_census": ([vp, C.POINTER(PacketRecord), C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
         "storm_gpu_import_census": ([vp, C.POINTER(PacketRecord), C.c_uint64], C.c_int),

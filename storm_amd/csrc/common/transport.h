@@ -57,7 +57,32 @@ struct SpectrumView {
   const Real *bounds;  // groups + 1
   const Real *cdf;     // per cell, groups + 1, cdf[0] = 0 and cdf[groups] = 1
   const Real *sig_s;   // per cell scattering opacity (gray)
+  int32_t eg;          // 1: tally the path integral per cell and group (withEgTimeAvg)
 };
+
+// RadiationOpacityModel::findGroup (radiation/RadiationOpacityModel.hpp:76-86):
+// the first group whose upper bound lies above E, else the last.
+template <typename Real>
+STORM_HD int32_t group_of(Real e, const Real *bounds, int32_t groups) {
+  int32_t lo = 0, hi = groups - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (e < bounds[mid + 1]) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// The draw for the unbiased rounding of a per-group path integral: a hash of
+// the packet, its stream position, the cell and the group.
+STORM_HD uint32_t group_tally_draw(uint32_t id, uint32_t ctr, uint32_t cell, int32_t g) {
+  uint32_t h = id * 0x85EBCA6Bu ^ (ctr + 0x2F1B9A3Du) * 0xC2B2AE35u ^ cell * 0x27D4EB2Du ^
+               static_cast<uint32_t>(g) * 0x165667B1u;
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  return h >> 8;
+}
 
 // Energy (keV) at the quantile `u` of a normalised group CDF, linear within
 // the group (LinearInterpolation, utils/LinearInterpolation.hpp:17-42, of
@@ -280,6 +305,11 @@ struct Tracker {
   uint32_t ijk, ctr, cell, visit_segs, id;
   int outcome;
   bool derive_pint;
+  // per-group tally (multigroup with withEgTimeAvg): the packet's group and
+  // the part of the visit's path integral already handed to it
+  bool eg_track;
+  int32_t grp;
+  double eg_mark;
 
   STORM_HD void begin(const Packet<Real> &p, const MeshView<Real> &m, const CellXS<Real> *xs, const StepConsts &k) {
     rs = PacketStream::make(k.seed, p.id, k.step, kTagTransport);
@@ -294,6 +324,9 @@ struct Tracker {
     wmin = p.wmin;
     id = p.id;
     nu = p.nu;
+    eg_track = (F & kFeatMultigroup) != 0u && m.spec.groups >= 2 && m.spec.eg != 0;
+    grp = 0;
+    eg_mark = 0.0;
     load_cell(m, xs);
     w_enter = w;
     path_int = 0.0;  // sum of w * (1 - e^{-x}) / sig_abs over the visit, erg*cm
@@ -334,8 +367,24 @@ struct Tracker {
     return fixed_point && cell_sig_abs > Real(0);
   }
 
+  // Hands the path integral since the last call to the packet's group
+  // (RadiationIMC.hpp:1166-1173, 1485-1492, 1802-1809): at the end of each
+  // visit and before the photon energy changes.
+  template <typename Sink>
+  STORM_HD void group_split(const StepConsts &k, Sink &sink) {
+    if constexpr ((F & kFeatMultigroup) != 0u) {
+      if (eg_track) {
+        const double v = path_int - eg_mark;
+        if (v > 0.0) sink.group_path(cell, grp, v, k, group_tally_draw(id, ctr, cell, grp));
+        eg_mark = path_int;
+      }
+    }
+  }
+
   template <typename Sink>
   STORM_HD void flush(double w_exit, const StepConsts &k, Sink &sink) {
+    group_split(k, sink);
+    eg_mark = 0.0;
     if constexpr (fixed_point) {
       fx::Limbs d = fx::sub(fx::limbs_of(w_enter, k.eq), fx::limbs_of(w_exit, k.eq));
       fx::Limbs e{0u, 0u};
@@ -431,6 +480,7 @@ struct Tracker {
       cell_sig_s = sig_s;
       cell_sig_abs = x.sig_abs * fac;
       cell_sig_ev = cell_sig_s + x.sig_ev * fac;
+      if (eg_track) grp = group_of<Real>(e, m.spec.bounds, m.spec.groups);
     }
     inv_sig_ev = cell_sig_ev > Real(0) ? Real(1) / cell_sig_ev : Real(0);
     derive_pint = derives_path_integral(m);
@@ -570,6 +620,7 @@ struct Tracker {
             const int32_t G = m.spec.groups;
             const Real *cdfc = m.spec.cdf + static_cast<size_t>(cell) * static_cast<size_t>(G + 1);
             const Real lo = cdfc[ca.gcut];
+            group_split(k, sink);
             if (lo < Real(1)) {
               nu = spectrum_quantile<Real>(cdfc, m.spec.bounds, G, lo + M::uniform(r2.v[3]) * (Real(1) - lo));
             } else {
@@ -780,6 +831,7 @@ struct Tracker {
         // (probability sigma_s / event opacity) keeps it
         const bool elastic = M::uniform(r.v[1]) * cell_sig_ev < cell_sig_s;
         if (!elastic) {
+          group_split(k, sink);
           const Philox4 r2 = rs.block(ctr++);
           const int32_t G = m.spec.groups;
           nu = spectrum_quantile<Real>(m.spec.cdf + static_cast<size_t>(cell) * static_cast<size_t>(G + 1),

@@ -110,6 +110,10 @@ struct UTracker {
   uint32_t cell, ctr, visit_segs, id, fbeg, fend;
   int outcome;
   bool derive_pint;
+  // per-group tally (multigroup with withEgTimeAvg), as in Tracker
+  bool eg_track;
+  int32_t grp;
+  double eg_mark;
 
   STORM_HD void begin(const Packet<Real> &p, const UMeshView<Real> &m, const CellXS<Real> *xs, const StepConsts &k) {
     rs = PacketStream::make(k.seed, p.id, k.step, kTagTransport);
@@ -126,6 +130,9 @@ struct UTracker {
     wmin = p.wmin;
     id = p.id;
     nu = p.nu;
+    eg_track = (F & kFeatMultigroup) != 0u && m.spec.groups >= 2 && m.spec.eg != 0;
+    grp = 0;
+    eg_mark = 0.0;
     load_cell(m, xs);
     w_enter = w;
     path_int = 0.0;
@@ -162,8 +169,23 @@ struct UTracker {
     return fixed_point && cell_sig_abs > Real(0);
   }
 
+  // Hands the path integral since the last call to the packet's group: at
+  // the end of each visit and before the photon energy changes.
+  template <typename Sink>
+  STORM_HD void group_split(const StepConsts &k, Sink &sink) {
+    if constexpr ((F & kFeatMultigroup) != 0u) {
+      if (eg_track) {
+        const double v = path_int - eg_mark;
+        if (v > 0.0) sink.group_path(cell, grp, v, k, group_tally_draw(id, ctr, cell, grp));
+        eg_mark = path_int;
+      }
+    }
+  }
+
   template <typename Sink>
   STORM_HD void flush(double w_exit, const StepConsts &k, Sink &sink) {
+    group_split(k, sink);
+    eg_mark = 0.0;
     if constexpr (fixed_point) {
       fx::Limbs d = fx::sub(fx::limbs_of(w_enter, k.eq), fx::limbs_of(w_exit, k.eq));
       fx::Limbs e{0u, 0u};
@@ -201,6 +223,7 @@ struct UTracker {
       cell_sig_s = sig_s;
       cell_sig_abs = xc.sig_abs * fac;
       cell_sig_ev = cell_sig_s + xc.sig_ev * fac;
+      if (eg_track) grp = group_of<Real>(e, m.spec.bounds, m.spec.groups);
     }
     inv_sig_ev = cell_sig_ev > Real(0) ? Real(1) / cell_sig_ev : Real(0);
     derive_pint = derives_path_integral(m);
@@ -334,6 +357,7 @@ struct UTracker {
             const int32_t G = m.spec.groups;
             const Real *cdfc = m.spec.cdf + static_cast<size_t>(cell) * static_cast<size_t>(G + 1);
             const Real lo = cdfc[ca.gcut];
+            group_split(k, sink);
             if (lo < Real(1)) {
               nu = spectrum_quantile<Real>(cdfc, m.spec.bounds, G, lo + M::uniform(r2.v[3]) * (Real(1) - lo));
             } else {
@@ -520,6 +544,7 @@ struct UTracker {
       if ((F & kFeatMultigroup) != 0u && m.spec.groups >= 2) {
         const bool elastic = M::uniform(r.v[1]) * cell_sig_ev < cell_sig_s;
         if (!elastic) {
+          group_split(k, sink);
           const Philox4 r2 = rs.block(ctr++);
           const int32_t G = m.spec.groups;
           nu = spectrum_quantile<Real>(m.spec.cdf + static_cast<size_t>(cell) * static_cast<size_t>(G + 1),

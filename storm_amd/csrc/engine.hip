@@ -219,6 +219,7 @@ struct EngineBase {
   virtual void step(double dt) = 0;
   virtual void step_noalloc(double dt) = 0;
   virtual void get_cells(double *, double *, double *, double *, double *, double *) = 0;
+  virtual void get_group_erad(double *) = 0;
   virtual void get_segments(uint64_t *) = 0;
   virtual void get_begin(uint64_t *) = 0;
   virtual void get_ledger(storm_gpu_ledger *) = 0;
@@ -313,6 +314,9 @@ struct Engine final : EngineBase {
   bool accel_on = false;
   // multigroup (storm_group_desc)
   int32_t groups = 0;
+  bool eg_on = false;  // withEgTimeAvg in a multigroup problem
+  DevBuf<unsigned long long> eg_u64;  // per-group path integral, mixed engine
+  DevBuf<double> eg_f64;              // FP64 engine
   double nu_pow = 0.0, nu_ref = 1.0;
   std::vector<double> bounds_h;  // erg
   DevBuf<double> bounds_erg;
@@ -707,6 +711,23 @@ struct Engine final : EngineBase {
         tal.dep_fp_rep = tal.pint_fp_rep = nullptr;
       }
     }
+    // per-group time-averaged radiation energy (withEgTimeAvg, multigroup)
+    eg_on = groups >= 2 && p->groups.with_eg_time_avg != 0;
+    tal.eg = nullptr;
+    tal.eg_fp = nullptr;
+    tal.eg_groups = eg_on ? static_cast<uint32_t>(groups) : 0u;
+    if (eg_on) {
+      const size_t n = static_cast<size_t>(N) * static_cast<size_t>(groups);
+      if (precision == STORM_GPU_FP64) {
+        eg_f64.ensure(n);
+        zero_f64(eg_f64.p, n);
+        tal.eg_fp = eg_f64.p;
+      } else {
+        eg_u64.ensure(n);
+        zero_u64(eg_u64.p, n);
+        tal.eg = eg_u64.p;
+      }
+    }
     zero_u64(tally_u64.p, tally_u64.n);
     zero_f64(tally_f64.p, tally_f64.n);
     zero_u32(segs_begin->p, segs_begin->n);
@@ -798,6 +819,7 @@ struct Engine final : EngineBase {
       s.bounds = bounds_kev.p;
       s.cdf = cdf.p;
       s.sig_s = sig_s_cell.p;
+      s.eg = eg_on ? 1 : 0;
     }
     return s;
   }
@@ -1136,6 +1158,8 @@ struct Engine final : EngineBase {
     zero_u64(tally_u64.p, tally_u64.n);
     if (precision == STORM_GPU_FP64) zero_f64(tally_f64.p, tally_f64.n);
     zero_u32(tal.segs, 2 * static_cast<size_t>(N));
+    if (tal.eg) zero_u64(tal.eg, static_cast<size_t>(N) * tal.eg_groups);
+    if (tal.eg_fp) zero_f64(tal.eg_fp, static_cast<size_t>(N) * tal.eg_groups);
     zero_u32(rep_u32.p, rep_u32.n);
     if (precision == STORM_GPU_FP64) zero_f64(rep_f64.p, rep_f64.n);
     else zero_u64(rep_u64.p, rep_u64.n);
@@ -1499,6 +1523,24 @@ struct Engine final : EngineBase {
     dl(cen, census_erad);
     dl(fl, fleck);
     dl(pl, sig_a);
+  }
+
+  // getEgTimeAvg (RadiationIMC.hpp:433-434, normalised as :1896-1903): the
+  // per-group path integral of the last step over c dt V, [cell * G + g]
+  void get_group_erad(double *out) override {
+    if (!eg_on) throw EngineError(1, "the per-group tally is off (storm_group_desc.with_eg_time_avg)");
+    HIPCHECK(hipStreamSynchronize(stream));
+    const size_t G = static_cast<size_t>(groups), n = static_cast<size_t>(N) * G;
+    std::vector<double> v(n);
+    if (precision == STORM_GPU_FP64) {
+      HIPCHECK(hipMemcpy(v.data(), eg_f64.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      std::vector<unsigned long long> q(n);
+      HIPCHECK(hipMemcpy(q.data(), eg_u64.p, n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n; ++i) v[i] = std::ldexp(static_cast<double>(q[i]), last_et_);
+    }
+    for (uint32_t c = 0; c < N; ++c)
+      for (size_t g = 0; g < G; ++g) out[c * G + g] = v[c * G + g] / (units::clight * last_dt_ * Vh[c]);
   }
 
   void get_segments(uint64_t *out) override {
@@ -1893,6 +1935,10 @@ int storm_gpu_set_source_temperature(storm_gpu_engine *eng, double T) {
 int storm_gpu_set_cell_state(storm_gpu_engine *eng, const double *T, const double *sa, const double *ss,
                              const double *cv, const double *e) {
   return guarded([&] { as_engine(eng)->set_cell_state(T, sa, ss, cv, e); });
+}
+
+int storm_gpu_get_group_erad(storm_gpu_engine *eng, double *out) {
+  return guarded([&] { as_engine(eng)->get_group_erad(out); });
 }
 
 int storm_gpu_get_cells(storm_gpu_engine *eng, double *T, double *e, double *erad, double *cen, double *fl,

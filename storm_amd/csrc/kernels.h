@@ -138,6 +138,11 @@ struct Tallies {
   double *dep_fp_rep, *pint_fp_rep;        // [(R-1) * N]
   uint32_t *segs_rep;                      // [(R-1) * N]
   uint32_t ncell;
+  // per-group path integral (withEgTimeAvg), [cell * eg_groups + g]: quanta
+  // of 2^et in the mixed engine, FP64 in the FP64 engine (null: off)
+  unsigned long long *eg;
+  double *eg_fp;
+  uint32_t eg_groups;
 };
 
 // Tally replicas: one per XCD of an MI355X (8 XCDs of 32 CUs).
@@ -839,6 +844,17 @@ struct DeviceSink {
   }
   __device__ void rw_step() { ++rw; }
   __device__ void ddmc_step() { ++ddmc; }
+  // path integral v (erg cm) of cell c in group g: rounded without bias to
+  // the path-integral quantum in the mixed engine (u24: the draw)
+  __device__ void group_path(uint32_t c, int32_t g, double v, const StepConsts &k, uint32_t u24) {
+    const size_t i = static_cast<size_t>(c) * t.eg_groups + static_cast<uint32_t>(g);
+    if (t.eg_fp) {
+      atomicAdd(&t.eg_fp[i], v);
+    } else if (t.eg) {
+      const double q = fx::quantize_unbiased(v, k.et, u24);
+      if (q > 0.0) atomicAdd(&t.eg[i], static_cast<unsigned long long>(fx::value_of(fx::limbs_of(q, k.et))));
+    }
+  }
 };
 
 // Sum over the active lanes of a wavefront, added once to *dst.

@@ -90,6 +90,13 @@ class GpuEngine:
                                                             ("T", "e", "erad", "census", "fleck", "sigma_p"))))
         return out
 
+    def group_erad(self, groups: int) -> np.ndarray:
+        """getEgTimeAvg: time-averaged radiation energy density per cell and
+        group of the last step, [cell, group] (problems with eg_time_avg)."""
+        out = np.empty(self.N * groups)
+        self._check(self.lib.storm_gpu_get_group_erad(self.h, _dp(out)))
+        return out.reshape(self.N, groups)
+
     def segments(self) -> np.ndarray:
         a = np.empty(self.N, dtype=np.uint64)
         self._check(self.lib.storm_gpu_get_segments(self.h, a.ctypes.data_as(C.POINTER(C.c_uint64))))

@@ -217,6 +217,7 @@ class ProblemSpec:
     group_bounds: np.ndarray | None = None  # photon energies, erg (multigroup when >= 3 entries)
     nu_pow: float = 0.0
     nu_ref: float = 1.0
+    eg_time_avg: bool = False           # withEgTimeAvg: per-group time-averaged radiation energy
     umesh: UMesh | None = None          # polyhedral mesh (the edges above are then unused)
     instances: list | None = None       # batches: dicts of storm_instance_desc fields
     _keep: list = field(default_factory=list, repr=False)
@@ -297,6 +298,7 @@ class ProblemSpec:
         g.num_groups = 0 if gb is None else len(gb) - 1
         g.bounds = _ptr(gb)
         g.nu_pow, g.nu_ref = float(self.nu_pow), float(self.nu_ref)
+        g.with_eg_time_avg = int(bool(self.eg_time_avg))
         p.groups = g
         if self.umesh is not None:
             um = self.umesh.to_c()
