@@ -95,6 +95,8 @@ typedef struct storm_gpu_ledger {
   storm_gpu_u128 census_pre;  /* census weight before population control   */
   storm_gpu_u128 census_out;  /* census weight after population control    */
   storm_gpu_u128 comb_rem;    /* comb rounding remainder (to the material) */
+  storm_gpu_u128 comb_dropped; /* stratified comb: weight of the bins it drops (left the problem,
+                                  as in CombPopulationControl.hpp:348-349, 409-413) */
   int32_t eq;                 /* quantum exponent                          */
   int32_t eq_prev;
   double requant_rem;         /* census requantisation remainder, erg      */

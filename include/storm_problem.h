@@ -98,10 +98,13 @@ typedef struct storm_imc_params {
   double rw_min_particle_optical_depth; /* default 5 */
   double ddmc_min_cell_optical_depth; /* default 15 */
   double ddmc_min_particle_optical_depth; /* default 5 */
-  int32_t comb;                       /* 0: NoPopulationControl, 1: CombPopulationControl */
+  int32_t comb;                       /* 0: NoPopulationControl, 1: CombPopulationControl,
+                                         2: StratifiedCombPopulationControl classified by
+                                         photon group (multigroup; one group when gray) */
   int32_t ddmc_multigroup_pgrw;       /* ddmcUseMultigroupPGRW (default 0) */
   uint64_t comb_nmin;                 /* Nmin (default 20) */
   double comb_factor;                 /* totalParticlesFactor (default 2.0) */
+  uint64_t comb_min_per_group;        /* stratified comb: minParticlesPerGroup (default 2) */
 } storm_imc_params;
 
 /*

@@ -959,14 +959,20 @@ class RandomWalk {
 };
 
 // ------------------------------------------------------------------ comb
+// PopulationControl (population/PopulationControl.hpp): the manager's hook.
+struct PopulationControl {
+  virtual ~PopulationControl() = default;
+  virtual std::vector<Particle> activate(const std::vector<Particle> &particles) = 0;
+};
+
 // CombPopulationControl::activate (population/CombPopulationControl.hpp:79-253)
 template <typename Grid>
-class Comb {
+class Comb : public PopulationControl {
  public:
   Comb(const Grid &g, size_t Nmin, double factor, uint64_t seed)
       : grid_(g), Nmin_(Nmin), factor_(factor), gen_(mixSeed(seed, 13)) {}
 
-  std::vector<Particle> activate(const std::vector<Particle> &particles) {
+  std::vector<Particle> activate(const std::vector<Particle> &particles) override {
     std::vector<Particle> result;
     size_t Ncells = grid_.GetPointNo();
     size_t Ntotal = Ncells;
@@ -1046,6 +1052,171 @@ class Comb {
   const Grid &grid_;
   size_t Nmin_;
   double factor_;
+  std::mt19937_64 gen_;
+};
+
+// StratifiedCombPopulationControl::activate
+// (population/CombPopulationControl.hpp:256-511): the census binned by cell
+// and class (here the photon group of the packet's energy, findGroup over
+// `bounds`); each cell's target (as in Comb) is shared among its occupied
+// bins -- minPerGroup each when they all fit, else one each for the heaviest
+// `target` bins and none for the rest, whose packets are dropped (the
+// reference discards them, :409-413 with :348-349) -- then the remainder in
+// proportion to the bin weights with the largest fractional parts rounded up
+// (:421-451); each bin is then combed or split like a Comb cell (:346-404).
+// `dropped` sums the weight of the dropped bins.
+template <typename Grid>
+class StratifiedComb : public PopulationControl {
+ public:
+  StratifiedComb(const Grid &g, std::vector<double> bounds, size_t Nmin, double factor, size_t minPerGroup,
+                 uint64_t seed)
+      : grid_(g), bounds_(std::move(bounds)), G_(std::max<size_t>(1, bounds_.size() > 1 ? bounds_.size() - 1 : 1)),
+        Nmin_(Nmin), factor_(factor), minPer_(minPerGroup), gen_(mixSeed(seed, 17)) {}
+
+  double dropped = 0.0;
+
+  size_t classify(const Particle &p) const {
+    if (bounds_.size() < 2) return 0;
+    for (size_t g = 0; g < G_; ++g)
+      if (p.frequency < bounds_[g + 1]) return g;
+    return G_ - 1;
+  }
+
+  std::vector<Particle> activate(const std::vector<Particle> &particles) override {
+    std::vector<Particle> result;
+    dropped = 0.0;
+    const size_t N = grid_.GetPointNo();
+    std::vector<double> cellW(N, 0.0), binW(N * G_, 0.0);
+    std::vector<std::vector<const Particle *>> bins(N * G_);
+    double total = 0.0;
+    for (const Particle &p : particles) {
+      if (grid_.IsPointOutsideBox(p.location)) throw OracleError("Stratified comb: input particle is outside the box");
+      if (!std::isfinite(p.weight)) throw OracleError("Stratified comb: particle weight is not finite");
+      if (p.weight < 0.0) throw OracleError("Stratified comb: particle weight is negative");
+      if (p.weight == 0.0) continue;
+      if (p.cellIndex >= N) throw OracleError("Stratified comb: invalid cell index");
+      const size_t b = p.cellIndex * G_ + std::min(classify(p), G_ - 1);
+      cellW[p.cellIndex] += p.weight;
+      binW[b] += p.weight;
+      total += p.weight;
+      bins[b].push_back(&p);
+    }
+    if (!std::isfinite(total)) throw OracleError("Stratified comb: total particle weight is not finite");
+    if (total == 0.0) return result;
+    const size_t Ntotal = static_cast<size_t>(N * Nmin_ * factor_);
+    std::uniform_real_distribution<double> dist(0, 1);
+    for (size_t c = 0; c < N; ++c) {
+      if (cellW[c] <= 0.0) continue;
+      const size_t target = std::min(Nmin_ * 20, std::max(Nmin_, static_cast<size_t>(Ntotal * cellW[c] / total)));
+      std::vector<size_t> active;
+      for (size_t g = 0; g < G_; ++g)
+        if (!bins[c * G_ + g].empty()) active.push_back(g);
+      if (active.empty()) continue;
+      std::vector<size_t> tg(G_, 0);
+      size_t given = 0;
+      if (active.size() * minPer_ <= target) {
+        for (size_t g : active) {
+          tg[g] = minPer_;
+          given += minPer_;
+        }
+      } else {
+        std::sort(active.begin(), active.end(), [&](size_t a, size_t b) { return binW[c * G_ + a] > binW[c * G_ + b]; });
+        const size_t keep = std::min(active.size(), target);
+        for (size_t j = 0; j < keep; ++j) {
+          tg[active[j]] = 1;
+          ++given;
+        }
+      }
+      const size_t rest = target > given ? target - given : 0;
+      size_t extraSum = 0;
+      std::vector<double> frac(G_, 0.0);
+      for (size_t g : active) {
+        if (tg[g] == 0) continue;
+        const double x = static_cast<double>(rest) * binW[c * G_ + g] / cellW[c];
+        const size_t e = static_cast<size_t>(std::floor(x));
+        tg[g] += e;
+        extraSum += e;
+        frac[g] = x - static_cast<double>(e);
+      }
+      while (extraSum < rest) {
+        size_t best = G_;
+        double bf = -1.0;
+        for (size_t g : active)
+          if (tg[g] > 0 && frac[g] > bf) {
+            best = g;
+            bf = frac[g];
+          }
+        if (best == G_) break;
+        ++tg[best];
+        frac[best] = 0.0;
+        ++extraSum;
+      }
+      for (size_t g : active) combBin(bins[c * G_ + g], binW[c * G_ + g], tg[g], c, result, dist);
+    }
+    // packets outside the box move toward their cell's mesh point (:487-506)
+    for (Particle &p : result) {
+      if (!grid_.IsPointOutsideBox(p.location)) continue;
+      const Vec3 original = p.location, direction = grid_.GetMeshPoint(p.cellIndex) - original;
+      double t = 1e-6;
+      while (grid_.IsPointOutsideBox(p.location) && t < 1.0) {
+        p.location = original + t * direction;
+        t *= 2;
+      }
+      if (grid_.IsPointOutsideBox(p.location)) p.location = grid_.GetMeshPoint(p.cellIndex);
+    }
+    return result;
+  }
+
+ private:
+  void combBin(std::vector<const Particle *> &bin, double W, size_t target, size_t c, std::vector<Particle> &out,
+               std::uniform_real_distribution<double> &dist) {
+    if (bin.empty() || W <= 0.0) return;
+    if (target == 0) {
+      dropped += W;
+      return;
+    }
+    if (bin.size() <= target) {
+      const double ideal = W / static_cast<double>(target);
+      for (const Particle *p : bin) {
+        if (p->weight > 2.0 * ideal) {
+          const size_t k = static_cast<size_t>(std::ceil(p->weight / ideal));
+          Particle cp = *p;
+          cp.weight = cp.initialWeight = p->weight / static_cast<double>(k);
+          cp.id = std::numeric_limits<size_t>::max();
+          cp.cellIndex = c;
+          cp.timeLeft = 0;
+          cp.steps = 0;
+          for (size_t j = 0; j < k; ++j) out.push_back(cp);
+        } else {
+          out.push_back(*p);
+        }
+      }
+      return;
+    }
+    std::sort(bin.begin(), bin.end(), [](const Particle *a, const Particle *b) { return a->id < b->id; });
+    std::shuffle(bin.begin(), bin.end(), gen_);
+    const double w = W / static_cast<double>(target);
+    const double r = dist(gen_);
+    size_t k = 0;
+    double cum = 0.0;
+    for (const Particle *p : bin) {
+      while (cum + p->weight > (static_cast<double>(k) + r) * w && k < target) {
+        ++k;
+        out.push_back(*p);
+        out.back().id = std::numeric_limits<size_t>::max();
+        out.back().cellIndex = c;
+        out.back().timeLeft = 0;
+        out.back().weight = out.back().initialWeight = w;
+      }
+      cum += p->weight;
+    }
+  }
+
+  const Grid &grid_;
+  std::vector<double> bounds_;
+  size_t G_, Nmin_;
+  double factor_;
+  size_t minPer_;
   std::mt19937_64 gen_;
 };
 
@@ -1766,7 +1937,7 @@ class RadiationIMC {
 template <typename Grid>
 class Manager {
  public:
-  Manager(const Grid &g, RadiationIMC<Grid> &phys, Comb<Grid> *comb, FaceTableBoundary<Grid> &bc)
+  Manager(const Grid &g, RadiationIMC<Grid> &phys, PopulationControl *comb, FaceTableBoundary<Grid> &bc)
       : grid_(g), phys_(phys), comb_(comb), bc_(bc) {}
 
   std::vector<size_t> cellsStepsCounters;
@@ -1833,7 +2004,7 @@ class Manager {
  private:
   const Grid &grid_;
   RadiationIMC<Grid> &phys_;
-  Comb<Grid> *comb_;
+  PopulationControl *comb_;
   FaceTableBoundary<Grid> &bc_;
   size_t idCounter_ = 0;
   std::vector<Particle> lastRemaining_;

@@ -46,8 +46,8 @@ Material makeMaterial(const storm_material_desc &d, size_t N, const storm_group_
 namespace {
 // Per-step ledger layout (doubles): emitted, source, escaped, clamp,
 // censusIn, censusOut, censusComb, materialBefore, materialAfter, segments,
-// startParticles, endParticles, seconds
-constexpr size_t kLedgerStride = 13;
+// startParticles, endParticles, seconds, combDropped
+constexpr size_t kLedgerStride = 14;
 
 template <typename Grid>
 int run_schedule_on(const Grid &grid, const storm_problem *prob, int nsteps, const double *dts, const double *source_T,
@@ -79,8 +79,16 @@ int run_schedule_on(const Grid &grid, const storm_problem *prob, int nsteps, con
     ip.ddmcUseMultigroupPGRW = prob->imc.ddmc_multigroup_pgrw != 0;
     ip.withEgTimeAvg = prob->groups.with_eg_time_avg != 0;
     RadiationIMC<Grid> imc(grid, &bc, cells, ext, mat, ip, prob->seed);
-    std::unique_ptr<Comb<Grid>> comb;
-    if (prob->imc.comb) comb = std::make_unique<Comb<Grid>>(grid, prob->imc.comb_nmin, prob->imc.comb_factor, prob->seed);
+    std::unique_ptr<PopulationControl> comb;
+    StratifiedComb<Grid> *strat = nullptr;
+    if (prob->imc.comb == 2) {
+      auto sc = std::make_unique<StratifiedComb<Grid>>(grid, mat.bounds, prob->imc.comb_nmin, prob->imc.comb_factor,
+                                                        prob->imc.comb_min_per_group, prob->seed);
+      strat = sc.get();
+      comb = std::move(sc);
+    } else if (prob->imc.comb) {
+      comb = std::make_unique<Comb<Grid>>(grid, prob->imc.comb_nmin, prob->imc.comb_factor, prob->seed);
+    }
     Manager<Grid> mgr(grid, imc, comb.get(), bc);
     std::vector<Particle> census;
     for (int s = 0; s < nsteps; ++s) {
@@ -114,6 +122,7 @@ int run_schedule_on(const Grid &grid, const storm_problem *prob, int nsteps, con
         L[10] = static_cast<double>(mgr.startParticleCount);
         L[11] = static_cast<double>(mgr.endParticleCount);
         L[12] = std::chrono::duration<double>(t1 - t0).count();
+        L[13] = strat ? strat->dropped : 0.0;
       }
     }
     for (size_t i = 0; i < N; ++i) {
@@ -140,8 +149,9 @@ extern "C" {
 
 // Per-step ledger layout (doubles): emitted, source, escaped, clamp,
 // censusIn, censusOut, censusComb, materialBefore, materialAfter, segments,
-// startParticles, endParticles, seconds
-#define ORACLE_LEDGER_STRIDE 13
+// startParticles, endParticles, seconds, combDropped (the stratified comb's
+// dropped bins)
+#define ORACLE_LEDGER_STRIDE 14
 
 const char *oracle_last_error(void) { return g_err.c_str(); }
 
@@ -394,6 +404,47 @@ int oracle_comb(const storm_problem *prob, int64_t n, const int64_t *cell, const
       count_out[p.cellIndex] += 1;
       wsum_out[p.cellIndex] += p.weight;
     }
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// StratifiedCombPopulationControl::activate on a hand-built census of the
+// problem's Cartesian mesh: packet i in cell[i] with photon energy freq[i]
+// (erg) and weight[i]; per cell and group (cell * G + g, G the problem's
+// groups, 1 when gray) the count and weight of the packets out, and the
+// weight of the dropped bins.
+int oracle_strat_comb(const storm_problem *prob, int64_t n, const int64_t *cell, const double *freq,
+                      const double *weight, double *count_out, double *wsum_out, double *dropped_out) {
+  try {
+    CartesianGrid grid = makeGrid(prob->mesh);
+    const size_t N = grid.GetPointNo();
+    std::vector<double> bounds;
+    if (prob->groups.num_groups >= 2) bounds.assign(prob->groups.bounds, prob->groups.bounds + prob->groups.num_groups + 1);
+    const size_t G = bounds.size() > 1 ? bounds.size() - 1 : 1;
+    std::vector<Particle> ps(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) {
+      ps[i].id = static_cast<size_t>(i);
+      ps[i].cellIndex = static_cast<size_t>(cell[i]);
+      ps[i].location = grid.GetCellCM(ps[i].cellIndex);
+      ps[i].weight = weight[i];
+      ps[i].frequency = freq[i];
+    }
+    StratifiedComb<CartesianGrid> comb(grid, bounds, prob->imc.comb_nmin, prob->imc.comb_factor,
+                                       prob->imc.comb_min_per_group, prob->seed);
+    std::vector<Particle> out = comb.activate(ps);
+    for (size_t i = 0; i < N * G; ++i) {
+      count_out[i] = 0;
+      wsum_out[i] = 0;
+    }
+    for (const Particle &p : out) {
+      const size_t b = p.cellIndex * G + comb.classify(p);
+      count_out[b] += 1;
+      wsum_out[b] += p.weight;
+    }
+    *dropped_out = comb.dropped;
     return 0;
   } catch (const std::exception &e) {
     g_err = e.what();

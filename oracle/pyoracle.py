@@ -13,10 +13,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
-LEDGER_STRIDE = 13
+LEDGER_STRIDE = 14
 LEDGER_FIELDS = ["emitted", "source", "escaped", "clamp", "census_in", "census_out", "census_comb",
                  "material_before", "material_after", "segments", "start_particles", "end_particles",
-                 "seconds"]
+                 "seconds", "comb_dropped"]
 
 _lib = None
 
@@ -40,6 +40,7 @@ def lib():
         L.oracle_rw_tables.argtypes = [dp, dp, dp]
         L.oracle_rw_sample.argtypes = [C.c_int, dp, dp, dp, dp]
         L.oracle_comb.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), dp, dp, dp]
+        L.oracle_strat_comb.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), dp, dp, dp, dp, dp]
         i64p = C.POINTER(C.c_int64)
         L.oracle_alloc.argtypes = [C.c_int64, dp, C.c_uint64, i64p]
         L.oracle_ddmc_data.argtypes = [C.c_void_p, C.c_double, dp, dp, dp, dp, dp, i64p, i64p, dp]
@@ -193,6 +194,23 @@ def comb(spec, cells, weights):
     if rc != 0:
         raise RuntimeError("oracle: " + L.oracle_last_error().decode())
     return cnt, ws
+
+
+def strat_comb(spec, cells, freqs, weights):
+    """StratifiedCombPopulationControl on a hand-built census: per cell and
+    group ([cell, group]) the count and weight out, and the dropped weight."""
+    L = lib()
+    prob = spec.to_c()
+    G = max(prob.groups.num_groups, 1) if prob.groups.num_groups >= 2 else 1
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    cnt, ws, dropped = np.empty(spec.ncells * G), np.empty(spec.ncells * G), np.zeros(1)
+    rc = L.oracle_strat_comb(C.byref(prob), len(w), cells.ctypes.data_as(C.POINTER(C.c_int64)), _dp(f), _dp(w),
+                             _dp(cnt), _dp(ws), _dp(dropped))
+    if rc != 0:
+        raise RuntimeError("oracle: " + L.oracle_last_error().decode())
+    return cnt.reshape(spec.ncells, G), ws.reshape(spec.ncells, G), float(dropped[0])
 
 
 # ---------------------------------------------------------------- replay

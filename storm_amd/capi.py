@@ -49,7 +49,7 @@ class ImcParams(C.Structure):
                 ("ddmc_min_cell_optical_depth", C.c_double),
                 ("ddmc_min_particle_optical_depth", C.c_double),
                 ("comb", C.c_int32), ("ddmc_multigroup_pgrw", C.c_int32),
-                ("comb_nmin", C.c_uint64), ("comb_factor", C.c_double)]
+                ("comb_nmin", C.c_uint64), ("comb_factor", C.c_double), ("comb_min_per_group", C.c_uint64)]
 
 
 class GroupDesc(C.Structure):
@@ -110,6 +110,7 @@ class U128(C.Structure):
 class Ledger(C.Structure):
     _fields_ = [("census_in", U128), ("emitted", U128), ("source", U128), ("deposited", U128),
                 ("escaped", U128), ("census_pre", U128), ("census_out", U128), ("comb_rem", U128),
+                ("comb_dropped", U128),
                 ("eq", C.c_int32), ("eq_prev", C.c_int32), ("requant_rem", C.c_double),
                 ("clamp", C.c_double), ("material_before", C.c_double),
                 ("material_after", C.c_double)]

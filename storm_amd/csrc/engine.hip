@@ -314,6 +314,7 @@ struct Engine final : EngineBase {
   bool accel_on = false;
   // multigroup (storm_group_desc)
   int32_t groups = 0;
+  uint32_t strat_groups = 1, strat_gbits = 0;  // stratified comb: groups per cell and their key bits
   bool eg_on = false;  // withEgTimeAvg in a multigroup problem
   DevBuf<unsigned long long> eg_u64;  // per-group path integral, mixed engine
   DevBuf<double> eg_f64;              // FP64 engine
@@ -438,6 +439,7 @@ struct Engine final : EngineBase {
     imc = p->imc;
     seed = p->seed;
     if (imc.new_photons_per_cell == 0) throw EngineError(1, "RadiationIMC requires newPhotonsPerCell > 0");
+    if (imc.comb < 0 || imc.comb > 2) throw EngineError(1, "imc.comb must be 0, 1 or 2");
     accel_on = imc.with_random_walk || imc.with_ddmc;
     if (p->groups.num_groups >= 2) {
       groups = p->groups.num_groups;
@@ -607,9 +609,18 @@ struct Engine final : EngineBase {
     nphot_z.ensure(size_t(1) << morton.total);
     offsets.ensure((size_t(1) << morton.total) + 1);
     err.ensure(1);
-    run_start.ensure(N);
-    run_end.ensure(N);
-    comb_cells.ensure(N);
+    // the stratified comb (a multigroup problem with imc.comb = 2) works on
+    // (cell, group) bins; with one group it is the plain comb
+    strat_groups = (imc.comb == 2 && groups >= 2) ? static_cast<uint32_t>(groups) : 1u;
+    if (strat_groups > 1) {
+      if (strat_groups > kMaxStratGroups)
+        throw EngineError(1, "the stratified comb takes at most 64 groups");
+      strat_gbits = static_cast<uint32_t>(fx::ceil_log2(static_cast<double>(strat_groups)));
+      if (morton.total + strat_gbits > 31u) throw EngineError(1, "too many cells and groups for the stratified comb");
+    }
+    run_start.ensure(static_cast<size_t>(N) * strat_groups);
+    run_end.ensure(static_cast<size_t>(N) * strat_groups);
+    comb_cells.ensure(static_cast<size_t>(N) * strat_groups);
     emit_limbs.ensure(2 * static_cast<size_t>(N));
     requant.ensure(2 * static_cast<size_t>(N));
     leaving.ensure(5);  // escapes, random-walk steps, DDMC steps, segments, longest history
@@ -669,7 +680,7 @@ struct Engine final : EngineBase {
         up_r(rw_rad, r);
       }
     }
-    tally_u64.ensure(8 * 2 * static_cast<size_t>(N));
+    tally_u64.ensure(9 * 2 * static_cast<size_t>(N));
     tally_f64.ensure(3 * static_cast<size_t>(N));
     DevBuf<uint32_t> *segs_begin = new DevBuf<uint32_t>();  // owned below
     segs_begin_.reset(segs_begin);
@@ -684,6 +695,7 @@ struct Engine final : EngineBase {
     tal.cout = tu + 5 * n2;
     tal.rem = tu + 6 * n2;
     tal.rdep = tu + 7 * n2;
+    tal.cdrop = tu + 8 * n2;
     tal.dep_fp = tally_f64.p;
     tal.pint_fp = tally_f64.p + N;
     tal.esc_fp = tally_f64.p + 2 * static_cast<size_t>(N);
@@ -1387,10 +1399,14 @@ struct Engine final : EngineBase {
     offs64.ensure(n);
     const uint32_t salt = static_cast<uint32_t>(seed) * 0x2545F491u + step_index * 0x9E3779B9u;
     // 32-bit keys: the cell's Z-order code (morton.total bits, and 2^total
-    // for dropped packets) above the top pbits bits of the rank; ties by
-    // k_comb_ties
-    const uint32_t pbits = 31u - morton.total;
-    k_comb_keys<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, cpre.soa(), morton, salt, pbits, keys.p, vals.p);
+    // for dropped packets), the photon group for the stratified comb
+    // (gbits), then the top pbits bits of the rank; ties by k_comb_ties
+    const uint32_t G = strat_groups, gbits = strat_gbits;
+    const uint32_t pbits = 31u - morton.total - gbits;
+    const CombUnits cu{pbits, gbits, G};
+    k_comb_keys<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, cpre.soa(), morton, salt, pbits, keys.p, vals.p,
+                                                              gbits, G > 1 ? bounds_kev.p : nullptr,
+                                                              static_cast<int32_t>(G));
     {
       auto q = [&](void *tp, size_t &bytes) {
         HIPCHECK(rocprim::radix_sort_pairs(tp, bytes, keys.p, keys_out.p, vals.p, vals_out.p, static_cast<size_t>(n),
@@ -1398,10 +1414,12 @@ struct Engine final : EngineBase {
       };
       ensure_temp(q);
     }
-    k_comb_ties<<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, (1u << morton.total) << pbits);
-    // runs of cells over the sorted census, weight limbs and their prefix sums
-    zero_u32(run_end.p, N);
-    k_comb_sorted<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits, cpre.soa(), eq, morton, d,
+    k_comb_ties<<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p,
+                                                        ((1u << morton.total) << gbits) << pbits);
+    // runs of cells (bins) over the sorted census, weight limbs and their
+    // prefix sums
+    zero_u32(run_end.p, static_cast<size_t>(N) * G);
+    k_comb_sorted<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, cu, cpre.soa(), eq, morton, d,
                                                                 limbs_sorted.p, run_start.p, run_end.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
@@ -1414,10 +1432,15 @@ struct Engine final : EngineBase {
     if (!inst.empty())
       k_inst_census<<<static_cast<uint32_t>(inst.size()), 256, 0, stream>>>(d_inst_first.p, run_start.p, run_end.p,
                                                                            cum.p, eq, d_inst_wtot.p);
-    k_comb_cells<<<blocks_for(N, 256), 256, 0, stream>>>(N, run_start.p, run_end.p, cum.p, n, imc.comb_nmin,
-                                                         imc.comb_factor, eq, imc.comb ? 1 : 0, seed, step_index,
-                                                         comb_cells.p, tal.cpre, tal.rem, instview());
-    k_comb_count<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits, cpre.soa(), comb_cells.p,
+    if (G > 1)
+      k_comb_strat<<<blocks_for(N, 256), 256, 0, stream>>>(N, G, run_start.p, run_end.p, cum.p, n, imc.comb_nmin,
+                                                           imc.comb_factor, imc.comb_min_per_group, eq, seed,
+                                                           step_index, comb_cells.p, tal.cpre, tal.rem, tal.cdrop);
+    else
+      k_comb_cells<<<blocks_for(N, 256), 256, 0, stream>>>(N, run_start.p, run_end.p, cum.p, n, imc.comb_nmin,
+                                                           imc.comb_factor, eq, imc.comb ? 1 : 0, seed, step_index,
+                                                           comb_cells.p, tal.cpre, tal.rem, instview());
+    k_comb_count<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, cu, cpre.soa(), comb_cells.p,
                                                                run_start.p, run_end.p, cum.p, eq, morton, d, ncopy.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
@@ -1431,7 +1454,7 @@ struct Engine final : EngineBase {
     if (n_out >= (1ull << 32) - 1) throw EngineError(4, "census larger than 2^32 packets");
     if (n_out == 0) return 0;
     cnext.ensure(n_out);
-    k_comb_write<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, pbits, cpre.soa(),
+    k_comb_write<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, cu, cpre.soa(),
                                                                comb_cells.p, ncopy.p, offs64.p, cnext.soa(), eq, morton,
                                                                d, tal);
     return n_out;
@@ -1610,6 +1633,7 @@ struct Engine final : EngineBase {
     L->census_pre = u128(sum(b + 4 * n2));
     L->census_out = u128(sum(b + 5 * n2));
     L->comb_rem = u128(sum(b + 6 * n2));
+    L->comb_dropped = u128(sum(b + 8 * n2));
     // boundary source: Npc * w_face per face, exact
     unsigned __int128 src = 0;
     if (last_nfaces_) {

@@ -124,6 +124,7 @@ struct Tallies {
   unsigned long long *cout;  // [2N] census weight after population control
   unsigned long long *rem;   // [2N] comb remainders (go to the material)
   unsigned long long *rdep;  // [2N] rouletted weight (goes to the material)
+  unsigned long long *cdrop; // [2N] stratified comb: weight of the dropped bins (leaves the problem)
   double *dep_fp;            // [N]  FP64 engine: deposit
   double *pint_fp;           // [N]  FP64 engine: path integral
   double *esc_fp;            // [N]  FP64 engine: escaped
@@ -1146,18 +1147,28 @@ __device__ __forceinline__ uint32_t perm32(uint32_t x, uint32_t salt) {
 // the packet's index, for k_comb_ties.  Weight-0 packets get code
 // 2^total and sort last.
 template <typename Real>
+// Stratified comb (gbits > 0): the packet's photon group (group_of over
+// `bounds`, `groups` of them) sits between the cell code and the rank bits,
+// so each cell's run splits into runs of its groups.
 __global__ __launch_bounds__(256) void k_comb_keys(uint32_t n, CensusSoA<Real> cen, Morton mo, uint32_t salt,
                                                    uint32_t pbits, uint32_t *__restrict__ keys,
-                                                   uint64_t *__restrict__ vals) {
+                                                   uint64_t *__restrict__ vals, uint32_t gbits,
+                                                   const Real *__restrict__ bounds, int32_t groups) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint2 m = cen.meta[i];
   const uint32_t rank = perm32(m.y, salt);
   vals[i] = (static_cast<uint64_t>(rank) << 32) | i;
   // weight-0 packets are dropped (CombPopulationControl.hpp:136-137)
-  const uint32_t code = cen.w[i] == 0.0 ? (1u << mo.total)
-                                         : (mo.flat ? m.x : morton_encode(ijk_i(m.x), ijk_j(m.x), ijk_k(m.x), mo));
-  keys[i] = pbits ? (code << pbits) | (rank >> (32u - pbits)) : code;
+  const bool drop = cen.w[i] == 0.0;
+  const uint32_t code = drop ? (1u << mo.total)
+                             : (mo.flat ? m.x : morton_encode(ijk_i(m.x), ijk_j(m.x), ijk_k(m.x), mo));
+  uint32_t key = code;
+  if (gbits) {
+    const Real e = cen.pos[i].w;
+    key = (key << gbits) | (drop ? 0u : static_cast<uint32_t>(group_of<Real>(e, bounds, groups)));
+  }
+  keys[i] = pbits ? (key << pbits) | (rank >> (32u - pbits)) : key;
 }
 
 // Orders each run of equal sort keys by the full rank (the value's high
@@ -1212,21 +1223,35 @@ __global__ __launch_bounds__(256) void k_comb_ties(uint32_t n, const uint32_t *_
 // Over the sorted census: the (lo, hi) limbs of each weight, for the prefix
 // sums, and each cell's run [start, end) of sorted indices (end stays 0 for a
 // cell with no packet).
+// The comb's unit of a sorted key: the cell, or with gbits > 0 (the
+// stratified comb) the (cell, group) bin, numbered cell * groups + group.
+struct CombUnits {
+  uint32_t pbits, gbits, groups;
+  __device__ uint32_t unit(uint32_t key, const Morton &mo, const MeshDims &d, uint32_t &cell) const {
+    const uint32_t code = key >> (pbits + gbits);
+    cell = morton_cell(code, mo, d);
+    if (!gbits) return cell;
+    return cell * groups + ((key >> pbits) & ((1u << gbits) - 1u));
+  }
+  __device__ bool dropped(uint32_t key, const Morton &mo) const { return (key >> (pbits + gbits)) >> mo.total; }
+};
+
 template <typename Real>
 __global__ __launch_bounds__(256) void k_comb_sorted(uint32_t n, const uint32_t *__restrict__ keys,
-                                                     const uint64_t *__restrict__ vals, uint32_t pbits,
+                                                     const uint64_t *__restrict__ vals, CombUnits cu,
                                                      CensusSoA<Real> cen, int32_t eq, Morton mo, MeshDims d,
                                                      ulonglong2 *__restrict__ limbs, uint32_t *__restrict__ start,
                                                      uint32_t *__restrict__ end) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t code = keys[i] >> pbits;
+  const uint32_t ukey = keys[i] >> cu.pbits;
   fx::Limbs l = fx::limbs_of(cen.w[static_cast<uint32_t>(vals[i])], eq);
   limbs[i] = make_ulonglong2(l.lo, l.hi);
-  if (code >> mo.total) return;  // dropped
-  const uint32_t c = morton_cell(code, mo, d);
-  if (i == 0u || (keys[i - 1u] >> pbits) != code) start[c] = i;
-  if (i + 1u == n || (keys[i + 1u] >> pbits) != code) end[c] = i + 1u;
+  if (cu.dropped(keys[i], mo)) return;
+  uint32_t c;
+  const uint32_t u = cu.unit(keys[i], mo, d, c);
+  if (i == 0u || (keys[i - 1u] >> cu.pbits) != ukey) start[u] = i;
+  if (i + 1u == n || (keys[i + 1u] >> cu.pbits) != ukey) end[u] = i + 1u;
 }
 
 struct CombCell {
@@ -1234,7 +1259,7 @@ struct CombCell {
   double r;         // comb offset in (0,1)
   double Wd;        // W_c as double
   uint32_t target;  // N_c
-  uint32_t mode;    // 0 empty, 1 split/pass-through, 2 comb
+  uint32_t mode;    // 0 empty, 1 split/pass-through, 2 comb, 3 dropped (stratified comb)
 };
 
 // Per-instance sum of a per-cell value over contiguous cell ranges
@@ -1289,6 +1314,32 @@ __device__ __forceinline__ fx::Limbs remainder_limbs(unsigned __int128 W, unsign
 __device__ __forceinline__ double u128_to_double(unsigned __int128 v) {
   return static_cast<double>(static_cast<uint64_t>(v >> 64)) * 18446744073709551616.0 +
          static_cast<double>(static_cast<uint64_t>(v));
+}
+
+// One unit of the comb (a cell, or a bin of the stratified comb) holding
+// `count` packets of total weight W (quanta) combed to `target` packets:
+// kept and split (count <= target) or combed with teeth of floor(W/target)
+// quanta, the remainder into the material of `cell`, the offset drawn from
+// the stream of (seed, unit, step).
+__device__ void comb_unit_setup(CombCell &cc, unsigned __int128 W, uint32_t count, uint64_t target, int32_t eq,
+                                uint64_t seed, uint32_t unit, uint32_t step, unsigned long long *__restrict__ rem,
+                                uint32_t cell) {
+  cc.Wd = u128_to_double(W) * fpm::pow2d(eq);
+  cc.target = static_cast<uint32_t>(target);
+  if (count <= target) {
+    cc.mode = 1u;
+    cc.w_out = cc.Wd / static_cast<double>(target);  // weight_ideal
+  } else {
+    cc.mode = 2u;
+    // tooth weight: floor(W / N) quanta, cut to 53 bits so it is a double;
+    // the remainder W - N * tooth goes to the material
+    const unsigned __int128 q = floor53(W / static_cast<unsigned __int128>(target));
+    cc.w_out = u128_exact_double(q) * fpm::pow2d(eq);
+    tally_limbs(rem, cell, remainder_limbs(W, q, target));
+    PacketStream rs = PacketStream::make(seed, unit, step, kTagComb);
+    Philox4 r = rs.block(0);
+    cc.r = u01_f64(r.v[0], r.v[1]);
+  }
 }
 
 // Census weight of each instance (erg): the sorted census is in cell order
@@ -1380,22 +1431,133 @@ __global__ __launch_bounds__(256) void k_comb_cells(uint32_t N, const uint32_t *
   uint64_t share = static_cast<uint64_t>(static_cast<double>(Ntotal) * cc.Wd / Wtot);
   uint64_t NinCell = share > nmin ? share : nmin;
   if (NinCell > 20ull * nmin) NinCell = 20ull * nmin;
-  cc.target = static_cast<uint32_t>(NinCell);
-  if (count <= NinCell) {
-    cc.mode = 1u;
-    cc.w_out = cc.Wd / static_cast<double>(NinCell);  // weight_ideal
-  } else {
-    cc.mode = 2u;
-    // tooth weight: floor(W_c / N_c) quanta, cut to 53 bits so it is a
-    // double; the remainder W_c - N_c * tooth goes to the material
-    const unsigned __int128 q = floor53(Wc / static_cast<unsigned __int128>(NinCell));
-    cc.w_out = u128_exact_double(q) * fpm::pow2d(eq);
-    tally_limbs(rem, c, remainder_limbs(Wc, q, NinCell));
-    PacketStream rs = PacketStream::make(seed, c, step, kTagComb);
-    Philox4 r = rs.block(0);
-    cc.r = u01_f64(r.v[0], r.v[1]);
-  }
+  comb_unit_setup(cc, Wc, count, NinCell, eq, seed, c, step, rem, c);
   cells[c] = cc;
+}
+
+// StratifiedCombPopulationControl::activate, per cell
+// (population/CombPopulationControl.hpp:415-466): the cell's target as in
+// k_comb_cells, shared among its occupied (cell, group) bins -- minPerGroup
+// each when they all fit, else one each for the `target` heaviest (ties:
+// lower group first) and none for the rest -- then the remainder in
+// proportion to the bin weights, the largest fractional parts rounded up
+// (ties: the first in that order); each bin is then split or combed as a
+// cell of the plain comb (:346-404).  A bin given no packets is dropped: its
+// weight leaves the problem (tallied in `drop`), as the reference discards
+// those packets (:348-349, :409-413).  `bins` has `groups` entries per cell.
+constexpr uint32_t kMaxStratGroups = 64;
+
+__global__ __launch_bounds__(256) void k_comb_strat(uint32_t N, uint32_t groups, const uint32_t *__restrict__ start,
+                                                    const uint32_t *__restrict__ end,
+                                                    const ulonglong2 *__restrict__ cum, uint32_t n, uint64_t nmin,
+                                                    double factor, uint64_t min_per, int32_t eq, uint64_t seed,
+                                                    uint32_t step, CombCell *__restrict__ bins,
+                                                    unsigned long long *__restrict__ W,
+                                                    unsigned long long *__restrict__ rem,
+                                                    unsigned long long *__restrict__ drop) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const uint32_t G = groups;
+  auto bin_weight = [&](uint32_t g) {
+    const uint32_t u = c * G + g, e = end[u];
+    if (!e) return static_cast<unsigned __int128>(0);
+    const uint32_t s0 = start[u];
+    return cum_value(cum, e - 1u) - (s0 ? cum_value(cum, s0 - 1u) : static_cast<unsigned __int128>(0));
+  };
+  double wb[kMaxStratGroups];
+  uint32_t tg[kMaxStratGroups];
+  double frac[kMaxStratGroups];
+  uint8_t order[kMaxStratGroups];
+  unsigned __int128 Wc = 0;
+  uint32_t nact = 0;
+  for (uint32_t g = 0; g < G; ++g) {
+    const unsigned __int128 b = bin_weight(g);
+    wb[g] = end[c * G + g] ? u128_to_double(b) : -1.0;  // -1: no packets
+    tg[g] = 0u;
+    frac[g] = 0.0;
+    Wc += b;
+    if (end[c * G + g]) ++nact;
+  }
+  W[2 * c] = static_cast<unsigned long long>(static_cast<uint64_t>(Wc) & fx::kLoMask);
+  W[2 * c + 1] = static_cast<unsigned long long>(static_cast<uint64_t>(Wc >> fx::kLoBits));
+  uint64_t target = 0;
+  if (nact) {
+    const double Wcd = u128_to_double(Wc);
+    const double Wtot = u128_to_double(cum_value(cum, n - 1u));
+    const uint64_t Ntotal = static_cast<uint64_t>(static_cast<double>(static_cast<uint64_t>(N) * nmin) * factor);
+    const uint64_t share = static_cast<uint64_t>(static_cast<double>(Ntotal) * Wcd / Wtot);
+    target = share > nmin ? share : nmin;
+    if (target > 20ull * nmin) target = 20ull * nmin;
+    uint64_t given = 0;
+    const bool all_fit = static_cast<uint64_t>(nact) * min_per <= target;
+    // the order the reference walks its active groups in: by group, or by
+    // weight (heaviest first) when they do not all fit
+    uint32_t k = 0;
+    for (uint32_t g = 0; g < G; ++g) {
+      if (wb[g] < 0.0) continue;
+      uint32_t pos = k++;
+      if (!all_fit) {
+        pos = 0;
+        for (uint32_t h = 0; h < G; ++h)
+          if (wb[h] >= 0.0 && (wb[h] > wb[g] || (wb[h] == wb[g] && h < g))) ++pos;
+      }
+      order[pos] = static_cast<uint8_t>(g);
+    }
+    for (uint32_t j = 0; j < nact; ++j) {
+      const uint32_t g = order[j];
+      const uint64_t t = all_fit ? min_per : (j < target ? 1u : 0u);
+      tg[g] = static_cast<uint32_t>(t);
+      given += t;
+    }
+    const uint64_t rest = target > given ? target - given : 0;
+    uint64_t extra = 0;
+    for (uint32_t j = 0; j < nact; ++j) {
+      const uint32_t g = order[j];
+      if (!tg[g]) continue;
+      const double x = static_cast<double>(rest) * wb[g] / Wcd;
+      const double e = floor(x);
+      tg[g] += static_cast<uint32_t>(e);
+      extra += static_cast<uint64_t>(e);
+      frac[g] = x - e;
+    }
+    while (extra < rest) {
+      int best = -1;
+      double bf = -1.0;
+      for (uint32_t j = 0; j < nact; ++j) {
+        const uint32_t g = order[j];
+        if (tg[g] && frac[g] > bf) {
+          best = static_cast<int>(g);
+          bf = frac[g];
+        }
+      }
+      if (best < 0) break;
+      ++tg[best];
+      frac[best] = 0.0;
+      ++extra;
+    }
+  }
+  for (uint32_t g = 0; g < G; ++g) {
+    const uint32_t u = c * G + g;
+    CombCell cc;
+    cc.w_out = 0.0;
+    cc.r = 0.5;
+    cc.target = 0;
+    cc.mode = 0u;
+    cc.Wd = 0.0;
+    const uint32_t e = end[u];
+    if (e) {
+      const uint32_t count = e - start[u];
+      const unsigned __int128 b = bin_weight(g);
+      if (!tg[g]) {
+        cc.mode = 3u;
+        cc.Wd = u128_to_double(b) * fpm::pow2d(eq);
+        tally_limbs(drop, c, fx::Limbs{static_cast<uint64_t>(b) & fx::kLoMask, static_cast<uint64_t>(b >> fx::kLoBits)});
+      } else {
+        comb_unit_setup(cc, b, count, tg[g], eq, seed, u, step, rem, c);
+      }
+    }
+    bins[u] = cc;
+  }
 }
 
 __device__ __forceinline__ uint64_t comb_teeth(double C, double Wc, double target, double r) {
@@ -1411,7 +1573,7 @@ __device__ __forceinline__ uint64_t comb_teeth(double C, double Wc, double targe
 // sorted order; [start, end) is each cell's run.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *__restrict__ keys,
-                                                    const uint64_t *__restrict__ vals, uint32_t pbits,
+                                                    const uint64_t *__restrict__ vals, CombUnits cu,
                                                     CensusSoA<Real> cen,
                                                     const CombCell *__restrict__ cells,
                                                     const uint32_t *__restrict__ start,
@@ -1420,20 +1582,22 @@ __global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *
                                                     MeshDims d, uint32_t *__restrict__ ncopy) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t code = keys[i] >> pbits;
-  if (code >> mo.total) {  // dropped
+  if (cu.dropped(keys[i], mo)) {
     ncopy[i] = 0u;
     return;
   }
-  const uint32_t c = morton_cell(code, mo, d);
-  const CombCell cc = cells[c];
+  uint32_t c;
+  const uint32_t u = cu.unit(keys[i], mo, d, c);
+  const CombCell cc = cells[u];
   const double w = cen.w[static_cast<uint32_t>(vals[i])];
   uint32_t k = 1;
-  if (cc.mode == 1u) {
+  if (cc.mode == 3u) {
+    k = 0;  // a bin the stratified comb drops
+  } else if (cc.mode == 1u) {
     if (w > 2.0 * cc.w_out) k = static_cast<uint32_t>(ceil(w / cc.w_out));
   } else if (cc.mode == 2u) {
-    const uint32_t s = start[c];
-    const bool last = (i + 1u == end[c]);
+    const uint32_t s = start[u];
+    const bool last = (i + 1u == end[u]);
     const unsigned __int128 Cbase = s > 0u ? cum_value(cum, s - 1u) : static_cast<unsigned __int128>(0);
     fx::Limbs wl = fx::limbs_of(w, eq);
     unsigned __int128 wv = (static_cast<unsigned __int128>(wl.hi) << fx::kLoBits) + wl.lo;
@@ -1454,7 +1618,7 @@ __global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *
 // packet into the same two words.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_comb_write(uint32_t n, const uint32_t *__restrict__ keys,
-                                                    const uint64_t *__restrict__ vals, uint32_t pbits,
+                                                    const uint64_t *__restrict__ vals, CombUnits cu,
                                                     CensusSoA<Real> cen,
                                                     const CombCell *__restrict__ cells,
                                                     const uint32_t *__restrict__ ncopy,
@@ -1464,11 +1628,11 @@ __global__ __launch_bounds__(256) void k_comb_write(uint32_t n, const uint32_t *
   uint32_t c = 0xFFFFFFFFu;  // lanes past the end and dropped packets: no cell
   unsigned long long lo = 0, hi = 0;
   if (i < n) {
-    const uint32_t code = keys[i] >> pbits;
-    if (!(code >> mo.total)) c = morton_cell(code, mo, d);
+    uint32_t u = 0;
+    if (!cu.dropped(keys[i], mo)) u = cu.unit(keys[i], mo, d, c);
     const uint32_t k = ncopy[i];
     if (k) {
-      const CombCell cc = cells[c];
+      const CombCell cc = cells[u];
       const uint32_t src = static_cast<uint32_t>(vals[i]);
       const double w = cen.w[src];
       double wout = w;

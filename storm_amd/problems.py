@@ -206,6 +206,7 @@ class ProblemSpec:
     comb: int = 1
     comb_nmin: int = 20
     comb_factor: float = 2.0
+    comb_min_per_group: int = 2        # stratified comb (comb=2): minParticlesPerGroup
     with_rw: int = 0
     with_ddmc: int = 0
     ddmc_pgrw: int = 0           # ddmcUseMultigroupPGRW (multigroup DDMC on the diffusive groups)
@@ -290,6 +291,7 @@ class ProblemSpec:
         q.rw_min_cell_optical_depth, q.rw_min_particle_optical_depth = self.rw_cell_tau, self.rw_particle_tau
         q.ddmc_min_cell_optical_depth, q.ddmc_min_particle_optical_depth = self.ddmc_cell_tau, self.ddmc_particle_tau
         q.comb, q.comb_nmin, q.comb_factor = int(self.comb), int(self.comb_nmin), float(self.comb_factor)
+        q.comb_min_per_group = int(self.comb_min_per_group)
         q.ddmc_multigroup_pgrw = int(self.ddmc_pgrw)
         p.imc = q
         p.T0 = _ptr(t0)

@@ -199,7 +199,7 @@ def test_c3_random_walk_ddmc_matches_oracle(c3_ref, precision):
 def _ledger_ok(L):
     lhs = L["census_in"] + L["emitted"] + L["source"]
     rhs = L["deposited"] + L["escaped"] + L["census_pre"]
-    return lhs == rhs and L["census_pre"] == L["census_out"] + L["comb_rem"], (lhs, rhs)
+    return lhs == rhs and L["census_pre"] == L["census_out"] + L["comb_rem"] + L["comb_dropped"], (lhs, rhs)
 
 
 @pytest.mark.parametrize("make", [C1, C2], ids=["c1", "c2"])
