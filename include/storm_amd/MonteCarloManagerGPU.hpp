@@ -17,7 +17,10 @@
 //   * the boundary: SideTemperature, VacuumBoundary or RigidBoundary describe
 //     their faces themselves; any other BoundaryCondition is probed once
 //     through apply() with a packet leaving each box face (REMOVE = escape,
-//     REFLECT = specular) and must not generate particles;
+//     REFLECT = specular), and the packets its generateNewBoundaryParticles
+//     makes (a MarshakBoundary, say) are made on the host every step and
+//     joined to the step's sources on the GPU, as RadiationIMC::preStep joins
+//     them (RadiationIMC.hpp:1683-1692);
 //   * population control: CombPopulationControl (Nmin, factor) runs on the
 //     GPU, NoPopulationControl passes the census through; other controls are
 //     rejected.
@@ -108,6 +111,7 @@ class MonteCarloManagerGPU {
     if (sourceT_) check(storm_gpu_set_source_temperature(eng_, sourceT_()));
     if (!resident_ || !particleList.empty()) importCensus(particleList);
     particleList.clear();
+    if (hostSource_) queueBoundaryParticles(boundary_->generateNewBoundaryParticles(fullDt));
     check(storm_gpu_step(eng_, fullDt));
     check(storm_gpu_get_counters(eng_, &counters_));
     std::vector<double> erad(Ncells_), cen(Ncells_), fleck(Ncells_), planck(Ncells_);
@@ -285,8 +289,8 @@ class MonteCarloManagerGPU {
       else if (s == ParticleStatus::REFLECT) info.face_kind[f] = STORM_FACE_REFLECT;
       else throw StormError("MonteCarloManagerGPU: boundary apply() must return REMOVE or REFLECT");
     }
-    if (!boundary_->generateNewBoundaryParticles(1e-30).empty())
-      throw StormError("MonteCarloManagerGPU: only SideTemperature sources are generated on the GPU");
+    // a boundary that makes its own packets: made on the host every step
+    hostSource_ = !boundary_->generateNewBoundaryParticles(1e-30).empty();
     return info;
   }
 
@@ -332,6 +336,27 @@ class MonteCarloManagerGPU {
     check(storm_gpu_create(&prob, static_cast<int32_t>(precision_), STORM_GPU_MATERIAL_HOST, &eng_));
   }
 
+  void queueBoundaryParticles(const std::vector<MCParticle> &particles) {
+    std::vector<storm_packet_record> recs(particles.size());
+    std::vector<double> tl(particles.size());
+    for (size_t i = 0; i < particles.size(); ++i) {
+      const MCParticle &p = particles[i];
+      storm_packet_record &r = recs[i];
+      r.x = p.location.x;
+      r.y = p.location.y;
+      r.z = p.location.z;
+      r.vx = p.velocity.x;
+      r.vy = p.velocity.y;
+      r.vz = p.velocity.z;
+      r.weight = p.weight;
+      r.frequency = p.frequency;
+      r.cell = p.cellIndex < Ncells_ ? p.cellIndex : grid_.GetContainingCell(p.location);
+      r.id = p.id;
+      tl[i] = p.timeLeft;
+    }
+    check(storm_gpu_queue_source_packets(eng_, recs.data(), tl.data(), recs.size()));
+  }
+
   void importCensus(const std::vector<MCParticle> &particles) {
     std::vector<storm_packet_record> recs(particles.size());
     for (size_t i = 0; i < particles.size(); ++i) {
@@ -363,6 +388,7 @@ class MonteCarloManagerGPU {
   storm_imc_params imc_params_{};
   storm_gpu_engine *eng_ = nullptr;
   bool resident_ = false;
+  bool hostSource_ = false;  // the boundary makes its packets on the host
   std::function<double()> sourceT_;
   std::vector<double> T_, sa_, ss_, cv_, e_, mass_;
   std::vector<size_t> steps_, begin_;

@@ -200,6 +200,17 @@ int storm_gpu_export_census(storm_gpu_engine *eng, storm_packet_record *out, uin
                             uint64_t *n_written);
 int storm_gpu_import_census(storm_gpu_engine *eng, const storm_packet_record *in, uint64_t n);
 
+/* New packets made on the host for the next step: the packets a user
+ * boundary's generateNewBoundaryParticles returns, which RadiationIMC::preStep
+ * adds to the step's sources (RadiationIMC.hpp:1683-1692), each with its own
+ * timeLeft (time_left[i], s; MarshakBoundary draws dt * U,
+ * examples/marshak_wave/MarshakBoundary.hpp:87-130).  Weights in erg; the
+ * next storm_gpu_step puts them on its energy quantum, counts them as
+ * boundary source in the ledger and transports them with the step's other
+ * packets.  Calls accumulate until that step. */
+int storm_gpu_queue_source_packets(storm_gpu_engine *eng, const storm_packet_record *in, const double *time_left,
+                                   uint64_t n);
+
 /* Launch parameters of the transport: block size (64, 128, 256, 512; the
  * default 256; REFILL uses at most 256; 0: chosen on line, below), segments
  * per packet and launch, and the schedule: PASS (the default) runs one packet

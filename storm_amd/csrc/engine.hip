@@ -228,6 +228,7 @@ struct EngineBase {
   virtual uint64_t census_size() = 0;
   virtual void export_census(storm_packet_record *, uint64_t, uint64_t *) = 0;
   virtual void import_census(const storm_packet_record *, uint64_t) = 0;
+  virtual void queue_source_packets(const storm_packet_record *, const double *, uint64_t) = 0;
   virtual void debug_inputs(uint64_t *, void *, void *, int64_t *) = 0;
   virtual void debug_tallies(uint64_t *, uint64_t *, uint64_t *, uint64_t *, uint32_t *) = 0;
   virtual void debug_accel(void *, uint64_t *) = 0;
@@ -374,6 +375,11 @@ struct Engine final : EngineBase {
     return v;
   }
   uint32_t last_nfaces_ = 0;    // source faces of the last step
+  // host-made source packets queued for the next step (weights in erg) and,
+  // for the ledger, the last step's ones on the step's quantum
+  std::vector<HostPacket> host_queue_;
+  std::vector<std::pair<uint32_t, double>> host_last_;  // (cell, weight on the quantum)
+  DevBuf<HostPacket> host_pk;
 
   // kept transport inputs of the last step (debug / replay)
   std::vector<Packet<Real>> kept_packets;
@@ -1091,6 +1097,9 @@ struct Engine final : EngineBase {
       n_bc = static_cast<uint64_t>(nfaces) * npc;
     }
     last_nfaces_ = nfaces;
+    // host-made source packets join the step's boundary source
+    const uint64_t n_host = host_queue_.size();
+    for (const HostPacket &h : host_queue_) source_total += h.w;
 
     // energy quantum of this step (every energy below 2^62 quanta; in a batch
     // every instance's, in its own unit), photon allocation and offsets; one
@@ -1101,6 +1110,7 @@ struct Engine final : EngineBase {
       const size_t K = inst.size();
       std::fill(inst_src_h, inst_src_h + K, 0.0);
       for (uint32_t f = 0; f < nfaces; ++f) inst_src_h[ut.src_instance[f]] += wface_e[f] * ut.src[f].count;
+      for (const HostPacket &h : host_queue_) inst_src_h[cell_inst_h[h.cell]] += h.w;
       HIPCHECK(hipMemcpyAsync(d_inst_src.p, inst_src_h, K * sizeof(double), hipMemcpyHostToDevice, stream));
       k_inst_sum<<<static_cast<uint32_t>(K), 256, 0, stream>>>(d_inst_first.p, Eemit.p, d_inst_etot.p);
       k_inst_quantum<<<1, 256, 0, stream>>>(static_cast<uint32_t>(K), d_inst_cen.p, d_inst_etot.p, d_inst_src.p,
@@ -1156,14 +1166,23 @@ struct Engine final : EngineBase {
       HIPCHECK(hipStreamSynchronize(stream));
     }
     if (nfaces) HIPCHECK(hipMemcpyAsync(wface.p, wface_h, nfaces * sizeof(double), hipMemcpyHostToDevice, stream));
-    const uint64_t n_total = n_census + n_emit + n_bc;
+    const uint64_t n_total = n_census + n_emit + n_bc + n_host;
     // 32-bit packet indices; the refill queue head may run past the count by
     // one wavefront of slots per resident wavefront
     if (n_total >= (1ull << 32) - (1ull << 24)) throw EngineError(4, "more than 2^32 - 2^24 packets in one step");
     counters.initial_particles = n_census;
     counters.emitted_particles = n_emit;
-    counters.boundary_particles = n_bc;
-    counters.prestep_particles = n_emit + n_bc;
+    counters.boundary_particles = n_bc + n_host;
+    counters.prestep_particles = n_emit + n_bc + n_host;
+    // host-made packets on the step's quantum (in a batch, in their
+    // instance's unit), their path left within the step
+    host_last_.clear();
+    for (HostPacket &h : host_queue_) {
+      const double w = inst.empty() ? h.w : std::ldexp(h.w, inst_shift_now[cell_inst_h[h.cell]]);
+      h.w = fx::quantize_down(w, eq);
+      h.s = std::min(h.s, static_cast<double>(s0));
+      host_last_.emplace_back(h.cell, h.w);
+    }
     counters.start_particles = n_total;
 
     // reset tallies
@@ -1202,6 +1221,18 @@ struct Engine final : EngineBase {
       k_boundary<Real><<<blocks_for(n_bc, 256), 256, 0, stream>>>(
           static_cast<uint32_t>(n_bc), npc, wface.p, pa.soa(), static_cast<uint32_t>(n_census + n_emit), s0, d, seed,
           step_index, tal.begin, spectrum(), groups >= 2 ? bcdf.p : nullptr, xhi_from);
+
+    if (n_host) {
+      host_pk.ensure(n_host);
+      HIPCHECK(hipMemcpyAsync(host_pk.p, host_queue_.data(), n_host * sizeof(HostPacket), hipMemcpyHostToDevice,
+                              stream));
+      k_host_packets<Real><<<blocks_for(n_host, 256), 256, 0, stream>>>(
+          static_cast<uint32_t>(n_host), host_pk.p, pa.soa(), static_cast<uint32_t>(n_census + n_emit + n_bc),
+          tal.begin);
+      // the copy reads host_queue_: wait for it before the queue is cleared
+      HIPCHECK(hipStreamSynchronize(stream));
+      host_queue_.clear();
+    }
 
     StepConsts kc;
     kc.seed = seed;
@@ -1357,6 +1388,9 @@ struct Engine final : EngineBase {
       throw EngineError(1, "storm_gpu_step_noalloc: the debug copy of the transport inputs allocates host memory; "
                            "switch it off first (storm_gpu_debug_keep_transport_inputs)");
     if (!scratch_ptr) throw EngineError(2, "storm_gpu_step_noalloc: no scratch address for this device");
+    if (!host_queue_.empty())
+      throw EngineError(1, "storm_gpu_step_noalloc: host-made source packets are queued (their upload allocates); "
+                           "use storm_gpu_step");
     ScratchArena arena{static_cast<unsigned char *>(scratch_ptr), kNoAllocScratchBytes, 0};
     SavedBuf saved[kTransientBufs];
     size_t k = 0;
@@ -1646,6 +1680,11 @@ struct Engine final : EngineBase {
         src += ((static_cast<unsigned __int128>(l.hi) << fx::kLoBits) + l.lo) * npc;
       }
     }
+    for (const auto &hp : host_last_) {
+      if (hp.first < c0 || hp.first >= c1) continue;
+      fx::Limbs l = fx::limbs_of(hp.second, last_eq_);
+      src += (static_cast<unsigned __int128>(l.hi) << fx::kLoBits) + l.lo;
+    }
     L->source = u128(src);
     L->eq = last_eq_ - sh;
     L->eq_prev = eq_prev_ledger_ - shp;
@@ -1741,6 +1780,57 @@ struct Engine final : EngineBase {
       r.id = ~0ull;
     }
     *nw = n;
+  }
+
+  // Host-made new packets for the next step (BoundaryCondition::
+  // generateNewBoundaryParticles, RadiationIMC.hpp:1683-1692): each with its
+  // own time left; weights in erg, put on the step's quantum then.
+  void queue_source_packets(const storm_packet_record *in, const double *time_left, uint64_t n) override {
+    if (host_queue_.size() + n >= (1ull << 31)) throw EngineError(4, "more than 2^31 queued source packets");
+    auto locate = [](const std::vector<double> &e, double v) {
+      auto it = std::upper_bound(e.begin(), e.end(), v);
+      int idx = static_cast<int>(it - e.begin()) - 1;
+      return std::clamp(idx, 0, static_cast<int>(e.size()) - 2);
+    };
+    host_queue_.reserve(host_queue_.size() + n);
+    for (uint64_t q = 0; q < n; ++q) {
+      const storm_packet_record &r = in[q];
+      if (!std::isfinite(r.weight) || r.weight < 0.0) throw EngineError(1, "source packet weight is negative or not finite");
+      if (!(time_left[q] >= 0.0) || !std::isfinite(time_left[q])) throw EngineError(1, "source packet time left is invalid");
+      const double vn = std::sqrt(r.vx * r.vx + r.vy * r.vy + r.vz * r.vz);
+      if (!(vn > 0.0)) throw EngineError(1, "source packet with zero velocity");
+      HostPacket h{};
+      h.o[0] = r.vx / vn;
+      h.o[1] = r.vy / vn;
+      h.o[2] = r.vz / vn;
+      h.w = r.weight;
+      h.s = time_left[q] * units::clight;
+      h.nu = 0.0f;
+      if (groups >= 2) h.nu = static_cast<float>(std::min(std::max(r.frequency, bounds_h.front()), bounds_h.back()) / units::kev);
+      if (unstructured) {
+        if (r.cell >= N) throw EngineError(1, "source packet on a polyhedral mesh needs its cell index");
+        const uint32_t c = static_cast<uint32_t>(r.cell);
+        h.u[0] = r.x - gen_h[3 * static_cast<size_t>(c)];
+        h.u[1] = r.y - gen_h[3 * static_cast<size_t>(c) + 1];
+        h.u[2] = r.z - gen_h[3 * static_cast<size_t>(c) + 2];
+        h.cell = c;
+        h.code = c;
+      } else {
+        const int i = r.cell < N ? static_cast<int>(r.cell / (static_cast<uint64_t>(ny) * nz)) : locate(xe, r.x);
+        const int j = r.cell < N ? static_cast<int>((r.cell / nz) % ny) : locate(ye, r.y);
+        const int k = r.cell < N ? static_cast<int>(r.cell % nz) : locate(ze, r.z);
+        auto frac = [](double v, double lo, double hi) {
+          const double f = (v - lo) / (hi - lo);
+          return f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+        };
+        h.u[0] = frac(r.x, xe[i], xe[i + 1]);
+        h.u[1] = frac(r.y, ye[j], ye[j + 1]);
+        h.u[2] = frac(r.z, ze[k], ze[k + 1]);
+        h.cell = static_cast<uint32_t>((static_cast<uint64_t>(i) * ny + j) * nz + k);
+        h.code = pack_ijk(i, j, k);
+      }
+      host_queue_.push_back(h);
+    }
   }
 
   void import_census(const storm_packet_record *in, uint64_t n) override {
@@ -2000,6 +2090,11 @@ int storm_gpu_export_census(storm_gpu_engine *eng, storm_packet_record *out, uin
 
 int storm_gpu_import_census(storm_gpu_engine *eng, const storm_packet_record *in, uint64_t n) {
   return guarded([&] { as_engine(eng)->import_census(in, n); });
+}
+
+int storm_gpu_queue_source_packets(storm_gpu_engine *eng, const storm_packet_record *in, const double *time_left,
+                                   uint64_t n) {
+  return guarded([&] { as_engine(eng)->queue_source_packets(in, time_left, n); });
 }
 
 int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segments_per_pass, int32_t schedule) {

@@ -602,6 +602,51 @@ __global__ __launch_bounds__(256) void k_import(uint32_t n, CensusSoA<Real> cen,
   }
 }
 
+// Host-made new packets (BoundaryCondition::generateNewBoundaryParticles,
+// radiation/RadiationIMC.hpp:1683-1692): one record each, position in the
+// cell's frame (Cartesian: fractions of the cell; polyhedral: relative to its
+// mesh point), unit direction, weight already on the step's quantum, path
+// length left (c * timeLeft), photon energy (keV).
+struct HostPacket {
+  double u[3];
+  double o[3];
+  double w;
+  double s;
+  float nu;
+  uint32_t cell;  // flat cell index
+  uint32_t code;  // the packet's cell code (packed ijk, or the flat index)
+  uint32_t pad;
+};
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_host_packets(uint32_t n, const HostPacket *__restrict__ in,
+                                                      PacketSoA<Real> out, uint32_t base,
+                                                      uint32_t *__restrict__ begin) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = q < n;
+  uint32_t cell = 0xFFFFFFFFu;
+  if (valid) {
+    const HostPacket h = in[q];
+    Packet<Real> p;
+    for (int a = 0; a < 3; ++a) {
+      p.u[a] = static_cast<Real>(h.u[a]);
+      p.o[a] = static_cast<Real>(h.o[a]);
+    }
+    p.s = static_cast<Real>(h.s);
+    p.w = h.w;
+    p.wmin = static_cast<Real>(h.w * 1e-3);
+    p.ijk = h.code;
+    p.id = base + q;
+    p.ctr = 0u;
+    p.flags = 0u;
+    p.nu = static_cast<Real>(h.nu);
+    store_packet(out, base + q, p);
+    cell = h.cell;
+  }
+  uint32_t one = valid ? 1u : 0u;
+  if (wave_run_sum(cell, one) && valid) atomicAdd(&begin[cell], one);
+}
+
 // Emission packets: uniform position in the cell (RandomInCellPositionSampler
 // is uniform in the box), isotropic direction, timeLeft = dt * U.
 template <typename Real>

@@ -138,6 +138,15 @@ class GpuEngine:
         self._check(self.lib.storm_gpu_export_census(self.h, buf, n, C.byref(w)))
         return np.ctypeslib.as_array(buf)[: w.value].copy() if n else np.zeros(0, dtype=np.ctypeslib.as_array(buf).dtype)
 
+    def queue_source_packets(self, records, time_left):
+        """New packets made on the host for the next step (a user boundary's
+        generateNewBoundaryParticles), each with its own time left (s)."""
+        recs = np.ascontiguousarray(records)
+        tl = np.ascontiguousarray(time_left, dtype=np.float64)
+        assert len(tl) == len(recs)
+        ptr = recs.ctypes.data_as(C.POINTER(capi.PacketRecord))
+        self._check(self.lib.storm_gpu_queue_source_packets(self.h, ptr, _dp(tl), len(recs)))
+
     def import_census(self, records):
         recs = np.ascontiguousarray(records)
         n = len(recs)

@@ -6,11 +6,16 @@
 //
 // usage: manager_slab <precision 0|1> <resident 0|1> <steps> <seed0> <nseeds>
 //                     <nx> <boundary packets> <photons per cell> <comb Nmin>
-//                     [<x+ temperature, keV>: TwoSidesTemperature instead]
+//                     [<x+ temperature, keV>: TwoSidesTemperature instead |
+//                      "user": a boundary of this program that makes its own
+//                      packets, the way the reference's Marshak example does]
 // Prints one JSON object: per seed the cell temperatures (K), time-averaged
 // Erad (erg/cm^3) and census Erad (erg), plus the manager's counters.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <random>
 #include <iostream>
 #include <memory>
 #include <string>
@@ -53,6 +58,71 @@ struct UnitOpacity : STORM::RadiationOpacityModel<Vec3, Grid, STORM::RadiationCe
 
 using IMC = STORM::RadiationIMC<Vec3, Grid, STORM::RadiationCell, STORM::SimpleExtensives, ConstantCvEOS, 1>;
 
+// A user boundary that makes its own packets, as the reference's Marshak
+// example does (examples/marshak_wave/MarshakBoundary.hpp:87-130): on every
+// cell face of the x- side of the box, Npercell packets of energy
+// sigma_SB T^4 A dt / Npercell each, at a uniform point of the face, mu =
+// sqrt(U) into the box, azimuth 2 pi U, timeLeft = dt U.  Packets reaching
+// x- leave; the other faces reflect.
+class FaceSourceBoundary : public STORM::BoundaryCondition<Vec3, Grid> {
+ public:
+  FaceSourceBoundary(const Grid &grid, double T, size_t Npercell, std::uint64_t seed)
+      : STORM::BoundaryCondition<Vec3, Grid>(grid), T_(T), n_(Npercell), re_(seed) {}
+
+  STORM::ParticleStatus apply(STORM::Particle<Vec3, Grid> &p) override {
+    const auto box = this->grid.GetBoxCoordinates();
+    if (p.location.x <= box.first.x) return STORM::ParticleStatus::REMOVE;
+    return STORM::ParticleStatus::REFLECT;
+  }
+
+  std::vector<STORM::Particle<Vec3, Grid>> generateNewBoundaryParticles(double fullDt) override {
+    constexpr double kSigmaSB = 5.670374419e-5, kC = 2.99792458e10, kPi = 3.14159265358979323846;
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::vector<STORM::Particle<Vec3, Grid>> out;
+    const double T4 = T_ * T_ * T_ * T_;
+    const size_t N = this->grid.GetPointNo();
+    for (size_t i = 0; i < N; ++i) {
+      const Vec3 c = this->grid.GetMeshPoint(i);
+      for (size_t f : this->grid.GetCellFaces(i)) {
+        const auto nb = this->grid.GetFaceNeighbors(f);
+        const size_t other = nb.first == i ? nb.second : nb.first;
+        if (!this->grid.IsPointOutsideBox(other)) continue;
+        const Vec3 g = this->grid.GetMeshPoint(other);
+        const double dx = g.x - c.x, dy = g.y - c.y, dz = g.z - c.z;
+        if (dx / std::sqrt(dx * dx + dy * dy + dz * dz) > -0.99) continue;  // not the x- side
+        const auto corners = this->grid.GetPointsInFace(f);
+        const auto &v = this->grid.GetFacePoints();
+        double ylo = v[corners[0]].y, yhi = ylo, zlo = v[corners[0]].z, zhi = zlo;
+        for (size_t q : corners) {
+          ylo = std::min(ylo, v[q].y);
+          yhi = std::max(yhi, v[q].y);
+          zlo = std::min(zlo, v[q].z);
+          zhi = std::max(zhi, v[q].z);
+        }
+        const double E = kSigmaSB * T4 * this->grid.GetArea(f) * fullDt / static_cast<double>(n_);
+        for (size_t k = 0; k < n_; ++k) {
+          STORM::Particle<Vec3, Grid> p;
+          p.location = Vec3(v[corners[0]].x, ylo + U(re_) * (yhi - ylo), zlo + U(re_) * (zhi - zlo));
+          const double mu = std::sqrt(U(re_)), st = std::sqrt(1.0 - mu * mu), th = 2.0 * kPi * U(re_);
+          p.velocity = Vec3(kC * mu, kC * st * std::cos(th), kC * st * std::sin(th));
+          p.frequency = 0.0;
+          p.weight = E;
+          p.initialWeight = E;
+          p.timeLeft = fullDt * U(re_);
+          p.cellIndex = i;
+          out.push_back(p);
+        }
+      }
+    }
+    return out;
+  }
+
+ private:
+  double T_;
+  size_t n_;
+  std::mt19937_64 re_;
+};
+
 void print_array(const char *name, const std::vector<double> &a, bool last = false) {
   std::printf("\"%s\": [", name);
   for (size_t i = 0; i < a.size(); ++i) std::printf(i ? ", %.17g" : "%.17g", a[i]);
@@ -66,7 +136,8 @@ int main(int argc, char **argv) {
     std::fprintf(stderr, "usage: %s precision resident steps seed0 nseeds nx npc nphot nmin [T_right_keV]\n", argv[0]);
     return 2;
   }
-  const bool two_sides = argc == 11;
+  const bool user_boundary = argc == 11 && std::strcmp(argv[10], "user") == 0;
+  const bool two_sides = argc == 11 && !user_boundary;
   const double T_right = two_sides ? std::atof(argv[10]) * kKevK : 0.0;
   const int precision = std::atoi(argv[1]);
   const bool resident = std::atoi(argv[2]) != 0;
@@ -95,7 +166,9 @@ int main(int argc, char **argv) {
       STORM::RadiationIMCParameters<1> params;
       params.newPhotonsPerCell = nphot;
       std::shared_ptr<STORM::BoundaryCondition<Vec3, Grid>> boundary;
-      if (two_sides)
+      if (user_boundary)
+        boundary = std::make_shared<FaceSourceBoundary>(grid, kKevK, npc, static_cast<std::uint64_t>(seed0 + s) * 7919u);
+      else if (two_sides)
         boundary = std::make_shared<STORM::TwoSidesTemperature<Vec3, Grid>>(grid, kKevK, T_right, npc);
       else
         boundary = std::make_shared<STORM::SideTemperature<Vec3, Grid>>(grid, kKevK, npc);

@@ -466,13 +466,17 @@ MANAGER_EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__fil
                            "manager_slab")
 
 
-def _run_manager(spec, precision, resident, seeds):
+def _run_manager(spec, precision, resident, seeds, user_boundary=False):
     """Runs tests/cpp/manager_slab.cpp (reference-style user code over
-    MonteCarloManagerGPU.hpp) on the slab `spec` for each seed."""
+    MonteCarloManagerGPU.hpp) on the slab `spec` for each seed
+    (`user_boundary`: its own boundary class that makes the x- packets on the
+    host instead of SideTemperature)."""
     args = [MANAGER_EXE, str(precision), str(int(resident)), str(spec.steps), str(seeds[0]), str(len(seeds)),
             str(spec.shape[0]), str(spec.source_per_face), str(spec.new_photons_per_cell), str(spec.comb_nmin)]
     if spec.source_xhi:
         args.append(repr(spec.source_T_hi / P.KEV_K))
+    elif user_boundary:
+        args.append("user")
     print("manager:", " ".join(args), flush=True)
     out = subprocess.run(args, capture_output=True, text=True, timeout=240)
     assert out.returncode == 0, out.stderr
@@ -487,6 +491,16 @@ def test_cpp_manager_matches_oracle(c1_ref, precision):
     with the FP64 oracle like the engine does."""
     got, _ = _run_manager(C1(0), precision, False, list(range(NSEEDS)))
     _assert_parity(c1_ref, got, f"c1-manager/{precision}")
+
+
+@pytest.mark.parametrize("precision", [capi.MIXED, capi.FP64], ids=["mixed", "fp64"])
+def test_cpp_manager_with_a_boundary_that_makes_its_packets(c1_ref, precision):
+    """A user boundary whose generateNewBoundaryParticles makes the x- source
+    packets on the host (as the reference's MarshakBoundary does): the
+    manager joins them to every step's sources on the GPU, and the physics is
+    the SideTemperature slab's, so it agrees with the same oracle ensemble."""
+    got, _ = _run_manager(C1(0), precision, False, list(range(NSEEDS)), user_boundary=True)
+    _assert_parity(c1_ref, got, f"c1-manager-user-boundary/{precision}")
 
 
 def test_cpp_manager_resident_is_the_engine():
