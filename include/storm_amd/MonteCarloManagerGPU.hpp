@@ -21,9 +21,13 @@
 //     makes (a MarshakBoundary, say) are made on the host every step and
 //     joined to the step's sources on the GPU, as RadiationIMC::preStep joins
 //     them (RadiationIMC.hpp:1683-1692);
-//   * population control: CombPopulationControl (Nmin, factor) runs on the
+//   * population control: CombPopulationControl (Nmin, factor) and
+//     StratifiedCombPopulationControl (classified by photon group) run on the
 //     GPU, NoPopulationControl passes the census through; other controls are
-//     rejected.
+//     rejected;
+//   * multigroup physics (NumGroups >= 2): the groups, the opacity's energy
+//     law (checked by RadiationIMC) and, with withEgTimeAvg, the per-group
+//     time-averaged radiation energy back after each step.
 // The packets themselves never leave HBM during a step.  By default step()
 // returns the census as std::vector<Particle> like the reference and takes it
 // back on the next call; setResidentCensus(true) keeps it in HBM instead
@@ -117,6 +121,11 @@ class MonteCarloManagerGPU {
     std::vector<double> erad(Ncells_), cen(Ncells_), fleck(Ncells_), planck(Ncells_);
     check(storm_gpu_get_cells(eng_, nullptr, e_.data(), erad.data(), cen.data(), fleck.data(), planck.data()));
     imc_->gpuAbsorb(e_.data(), erad.data(), cen.data(), fleck.data(), planck.data());
+    if (egOn_) {
+      std::vector<double> eg(Ncells_ * (bounds_.size() - 1));
+      check(storm_gpu_get_group_erad(eng_, eg.data()));
+      imc_->gpuGroupErad(eg.data());
+    }
     imc_->gpuCounters(counters_.random_walk_steps, counters_.ddmc_steps);
     if (auto *vac = dynamic_cast<VacuumBoundary<T, Grid> *>(boundary_.get())) {
       storm_gpu_ledger L;
@@ -297,14 +306,32 @@ class MonteCarloManagerGPU {
   void describePopulation() {
     imc_params_ = storm_imc_params{};
     imc_->gpuParameters(imc_params_);
-    if (auto *comb = dynamic_cast<CombPopulationControl<T, Grid> *>(population_.get())) {
+    groups_ = imc_->gpuGroups(bounds_, nuPow_, nuRef_, egOn_);
+    if (auto *sc = dynamic_cast<StratifiedCombPopulationControl<T, Grid> *>(population_.get())) {
+      // the GPU classifies by photon group: the classifier must agree on a
+      // packet at the centre of every group
+      const size_t G = groups_ ? bounds_.size() - 1 : 1;
+      if (sc->getGroupCount() != G)
+        throw StormError("MonteCarloManagerGPU: the stratified comb's group count must be the problem's groups");
+      for (size_t g = 0; groups_ && g < G; ++g) {
+        MCParticle p;
+        p.frequency = 0.5 * (bounds_[g] + bounds_[g + 1]);
+        p.cellIndex = 0;
+        if (sc->classify(p) != g)
+          throw StormError("MonteCarloManagerGPU: the stratified comb runs on the GPU classified by photon group");
+      }
+      imc_params_.comb = 2;
+      imc_params_.comb_nmin = sc->getNmin();
+      imc_params_.comb_factor = sc->getTotalParticlesFactor();
+      imc_params_.comb_min_per_group = sc->getMinParticlesPerGroup();
+    } else if (auto *comb = dynamic_cast<CombPopulationControl<T, Grid> *>(population_.get())) {
       imc_params_.comb = 1;
       imc_params_.comb_nmin = comb->getNmin();
       imc_params_.comb_factor = comb->getTotalParticlesFactor();
     } else if (dynamic_cast<NoPopulationControl<T, Grid> *>(population_.get())) {
       imc_params_.comb = 0;
     } else {
-      throw StormError("MonteCarloManagerGPU: population control must be Comb or NoPopulationControl");
+      throw StormError("MonteCarloManagerGPU: population control must be Comb, StratifiedComb or NoPopulationControl");
     }
   }
 
@@ -331,6 +358,17 @@ class MonteCarloManagerGPU {
     prob.material.eos_beta = 1.0;
     prob.boundary = bnd_;
     prob.imc = imc_params_;
+    if (groups_) {
+      prob.groups.num_groups = static_cast<int32_t>(bounds_.size() - 1);
+      prob.groups.with_eg_time_avg = egOn_ ? 1 : 0;
+      prob.groups.bounds = bounds_.data();
+      prob.groups.nu_pow = nuPow_;
+      prob.groups.nu_ref = nuRef_;
+      // photon temperatures of the Planck fractions floored at 1 K (the
+      // reference's multigroup models take max(T, 1 K), e.g.
+      // examples/densmore2012/DensmoreOpacity.hpp:40)
+      prob.material.opacity_T_floor = 1.0;
+    }
     prob.T0 = T_.data();
     prob.seed = imc_->gpuSeed();
     check(storm_gpu_create(&prob, static_cast<int32_t>(precision_), STORM_GPU_MATERIAL_HOST, &eng_));
@@ -389,6 +427,9 @@ class MonteCarloManagerGPU {
   storm_gpu_engine *eng_ = nullptr;
   bool resident_ = false;
   bool hostSource_ = false;  // the boundary makes its packets on the host
+  bool groups_ = false, egOn_ = false;  // multigroup; with the per-group tally
+  std::vector<double> bounds_;
+  double nuPow_ = 0.0, nuRef_ = 1.0;
   std::function<double()> sourceT_;
   std::vector<double> T_, sa_, ss_, cv_, e_, mass_;
   std::vector<size_t> steps_, begin_;

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <functional>
 #include <cstddef>
 #include <cstdint>
 #include <memory>
@@ -82,7 +83,10 @@ class SideTemperature : public BoundaryCondition<T, Grid> {
   SideTemperature(const Grid &grid, double temperature, size_t Npercell,
                   const std::vector<double> &energyBoundaries = {}, bool freezeT4 = true)
       : BoundaryCondition<T, Grid>(grid), temperature(temperature), Npercell(Npercell), freezeT4(freezeT4) {
-    if (!energyBoundaries.empty()) throw StormError("SideTemperature: multigroup sources are not supported on the GPU");
+    // multigroup: the GPU draws the photon energy from the source's Planck
+    // fractions over the problem's groups, linear within a group, as the
+    // reference does with these boundaries (SideTemperature.hpp:148-152)
+    (void)energyBoundaries;
   }
   ParticleStatus apply(Particle<T, Grid> &) override {
     throw StormError("SideTemperature::apply is evaluated on the GPU (MonteCarloManagerGPU)");
@@ -252,6 +256,37 @@ class NoPopulationControl : public PopulationControl<T, Grid> {
   using MCParticle = Particle<T, Grid>;
   explicit NoPopulationControl(const Grid &grid) : PopulationControl<T, Grid>(grid) {}
   std::vector<MCParticle> activate(const std::vector<MCParticle> &particles) override { return particles; }
+};
+
+// StratifiedCombPopulationControl (population/CombPopulationControl.hpp:35-72,
+// 256-511): the comb by cell and class.  The GPU classifies a packet by its
+// photon group; the manager checks the classifier against that on packets
+// at the groups' centres and rejects any other.
+template <typename T, typename Grid>
+class StratifiedCombPopulationControl : public PopulationControl<T, Grid> {
+ public:
+  using MCParticle = Particle<T, Grid>;
+  using Classifier = std::function<size_t(const MCParticle &)>;
+  StratifiedCombPopulationControl(const Grid &grid, size_t groupCount, Classifier classifier, size_t Nmin = 20,
+                                  double totalParticlesFactor = 2.0, size_t minParticlesPerGroup = 2)
+      : PopulationControl<T, Grid>(grid), groupCount(std::max<size_t>(1, groupCount)),
+        classifier(std::move(classifier)), Nmin(Nmin), totalParticlesFactor(totalParticlesFactor),
+        minParticlesPerGroup(minParticlesPerGroup) {}
+  std::vector<MCParticle> activate(const std::vector<MCParticle> &) override {
+    throw StormError("StratifiedCombPopulationControl runs on the GPU (MonteCarloManagerGPU)");
+  }
+  size_t getGroupCount() const { return groupCount; }
+  size_t getNmin() const { return Nmin; }
+  double getTotalParticlesFactor() const { return totalParticlesFactor; }
+  size_t getMinParticlesPerGroup() const { return minParticlesPerGroup; }
+  size_t classify(const MCParticle &p) const { return classifier ? classifier(p) : 0; }
+
+ private:
+  size_t groupCount;
+  Classifier classifier;
+  size_t Nmin;
+  double totalParticlesFactor;
+  size_t minParticlesPerGroup;
 };
 
 // ------------------------------------------------------------------ radiation
@@ -458,6 +493,10 @@ class GpuRadiationPhysics {
   virtual void gpuAbsorb(const double *e, const double *eradAvg, const double *censusErad, const double *fleck,
                          const double *planck) = 0;
   virtual void gpuCounters(std::size_t rw, std::size_t ddmc) = 0;
+  // multigroup problems: the groups and the opacity's energy law (false: gray)
+  virtual bool gpuGroups(std::vector<double> &, double &, double &, bool &) const { return false; }
+  // the per-group time-averaged radiation energy of the step, [cell * G + g]
+  virtual void gpuGroupErad(const double *) {}
 };
 
 template <typename PointT, typename GridT, typename CellT, typename ExtensivesT, typename EOST,
@@ -466,8 +505,6 @@ template <typename PointT, typename GridT, typename CellT, typename ExtensivesT,
           typename PositionSamplerT = RandomInCellPositionSampler<PointT, GridT>>
 class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiationPhysics {
  public:
-  static_assert(NumGroups == 1, "the C++ drop-in manager runs gray problems (NumGroups = 1); multigroup "
-                                "problems go through the C ABI's storm_group_desc");
   static_assert(std::is_same<PositionSamplerT, RandomInCellPositionSampler<PointT, GridT>>::value,
                 "the GPU places new packets uniformly in their cell (RandomInCellPositionSampler)");
   using Base = MonteCarloPhysics<PointT, GridT>;
@@ -497,10 +534,14 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
     if (parameters_.withCompton) throw StormError("RadiationIMC option withCompton is not implemented");
     if (parameters_.postProcess.enabled) throw StormError("RadiationIMC option postProcess.enabled is not implemented");
     if (parameters_.withHydro || parameters_.MMC) throw StormError("RadiationIMC: hydro coupling is not supported on the GPU");
-    if (parameters_.withMultigroupOpacity) throw StormError("RadiationIMC: the GPU engine is gray");
+    if (NumGroups >= 2 && !parameters_.withMultigroupOpacity)
+      throw StormError("RadiationIMC: NumGroups >= 2 needs withMultigroupOpacity");
+    if (NumGroups < 2 && parameters_.withMultigroupOpacity)
+      throw StormError("RadiationIMC: withMultigroupOpacity needs NumGroups >= 2");
     rejectDirectionOverrides();
     opacity_->reseed(seed + 1ULL);
     Erad_time_avg_.assign(grid.GetPointNo(), 0.0);
+    if constexpr (NumGroups >= 2) probeGroupModel();
   }
 
   std::vector<MCParticle> preStep(double) override { throw StormError(kGpuOnly); }
@@ -536,6 +577,8 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
         MCParticle p;
         p.cellIndex = i;
         p.frequency = 0.0;
+        // multigroup: an energy from the cell's thermal spectrum (:2604-2605)
+        if constexpr (NumGroups >= 2) p.frequency = opacity_->GetThermalEnergy(cells_[i], U(rng), parameters_.energyBoundaries);
         // uniform in the cell (RandomInCellPositionSampler on a box cell)
         p.location = PointT(lo[0] + U(rng) * (hi[0] - lo[0]), lo[1] + U(rng) * (hi[1] - lo[1]),
                             lo[2] + U(rng) * (hi[2] - lo[2]));
@@ -556,6 +599,9 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
   const std::vector<double> &getPlanckOpacities() const { return planck_; }
   const std::vector<double> &getEradTimeAvg() const { return Erad_time_avg_; }
   std::vector<double> &getEradTimeAvg() { return Erad_time_avg_; }
+  // RadiationIMC.hpp:433-434 (withEgTimeAvg in a multigroup problem)
+  const std::vector<GroupArray> &getEgTimeAvg() const { return Eg_time_avg_; }
+  std::vector<GroupArray> &getEgTimeAvg() { return Eg_time_avg_; }
   const Parameters &getParameters() const { return parameters_; }
   std::size_t getRandomWalkStepCount() const override { return rw_; }
   std::size_t getDDMCStepCount() const override { return ddmc_; }
@@ -570,6 +616,7 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
     p.rw_min_particle_optical_depth = parameters_.rwMinParticleOpticalDepth;
     p.ddmc_min_cell_optical_depth = parameters_.ddmcMinCellOpticalDepth;
     p.ddmc_min_particle_optical_depth = parameters_.ddmcMinParticleOpticalDepth;
+    p.ddmc_multigroup_pgrw = parameters_.ddmcUseMultigroupPGRW ? 1 : 0;
   }
   std::uint64_t gpuSeed() const override { return seed_; }
 
@@ -578,7 +625,10 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
     for (std::size_t i = 0; i < N; ++i) {
       const CellT &cell = cells_[i];
       T[i] = cell.temperature;
-      sa[i] = opacity_->CalcPlanckOpacity(cell);
+      // gray: the Planck opacity; multigroup: sigma_a(cell, T) at nu_ref, the
+      // factor of the energy law the GPU applies per packet
+      if constexpr (NumGroups >= 2) sa[i] = opacity_->CalcAbsorptionOpacity(cell, nu_ref_);
+      else sa[i] = opacity_->CalcPlanckOpacity(cell);
       ss[i] = opacity_->CalcScatteringOpacity(cell);
       cv[i] = eos_->dT2cv(density(i), cell.temperature, traits_.tracers(cell), traits_.tracerNames(cell));
       e[i] = extensives_[i].internal_energy;
@@ -612,6 +662,23 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
   void gpuCounters(std::size_t rw, std::size_t ddmc) override {
     rw_ = rw;
     ddmc_ = ddmc;
+  }
+  bool gpuGroups(std::vector<double> &bounds, double &nu_pow, double &nu_ref, bool &eg) const override {
+    if constexpr (NumGroups < 2) {
+      return false;
+    } else {
+      bounds.assign(parameters_.energyBoundaries.begin(), parameters_.energyBoundaries.end());
+      nu_pow = nu_pow_;
+      nu_ref = nu_ref_;
+      eg = parameters_.withEgTimeAvg;
+      return true;
+    }
+  }
+  void gpuGroupErad(const double *eg) override {
+    const std::size_t N = this->grid.GetPointNo();
+    Eg_time_avg_.assign(N, GroupArray{});
+    for (std::size_t i = 0; i < N; ++i)
+      for (std::size_t g = 0; g < NumGroups; ++g) Eg_time_avg_[i][g] = eg[i * NumGroups + g];
   }
 
  private:
@@ -651,6 +718,38 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
     }
   }
 
+  // The GPU's multigroup opacity is sigma_a(cell, T) (max(E, b_0) / nu_ref)^-p
+  // (storm_group_desc, storm_problem.h), scattering gray.  The model is
+  // checked to be of that form on its first cells -- three energies across
+  // the groups give p, the middle one must follow it to 1e-9, an energy
+  // below b_0 must give b_0's opacity, scattering must not depend on the
+  // energy -- and sigma_a(cell, T) is tabulated at nu_ref every step.
+  void probeGroupModel() {
+    const auto &b = parameters_.energyBoundaries;
+    for (std::size_t g = 0; g < NumGroups; ++g)
+      if (!(b[g + 1] > b[g]) || !(b[0] > 0.0))
+        throw StormError("RadiationIMC: energyBoundaries must be positive and increasing");
+    const double E1 = 0.5 * (b[0] + b[1]), E3 = 0.5 * (b[NumGroups - 1] + b[NumGroups]), E2 = std::sqrt(E1 * E3);
+    nu_ref_ = E2;
+    const std::size_t n = std::min<std::size_t>(this->grid.GetPointNo(), 8);
+    for (std::size_t i = 0; i < n; ++i) {
+      const CellT &cell = cells_[i];
+      const double s1 = opacity_->CalcAbsorptionOpacity(cell, E1), s2 = opacity_->CalcAbsorptionOpacity(cell, E2),
+                   s3 = opacity_->CalcAbsorptionOpacity(cell, E3);
+      if (!(s1 > 0.0 && s2 > 0.0 && s3 > 0.0) || !std::isfinite(s1) || !std::isfinite(s3))
+        throw StormError("RadiationIMC: the multigroup opacity must be positive and finite");
+      const double p = -std::log(s3 / s1) / std::log(E3 / E1);
+      if (i == 0) nu_pow_ = p;
+      const bool law = std::abs(p - nu_pow_) <= 1e-9 * std::max(1.0, std::abs(p)) &&
+                       std::abs(s2 - s1 * std::pow(E2 / E1, -p)) <= 1e-9 * s2;
+      const double below = opacity_->CalcAbsorptionOpacity(cell, 0.5 * b[0]), at = opacity_->CalcAbsorptionOpacity(cell, b[0]);
+      const double sc1 = opacity_->CalcScatteringOpacity(cell, E1), sc3 = opacity_->CalcScatteringOpacity(cell, E3);
+      if (!law || std::abs(below - at) > 1e-9 * at || std::abs(sc1 - sc3) > 1e-12 * std::max(std::abs(sc1), 1e-300))
+        throw StormError("RadiationIMC: the GPU engine's multigroup opacity is sigma_a(cell, T) * E^-p for E above the "
+                         "lowest group bound (constant below it) with gray scattering; this opacity model is not");
+    }
+  }
+
   double density(std::size_t i) const {
     if constexpr (detail::has_density<CellT>::value) return cells_[i].density;
     else return extensives_[i].mass / this->grid.GetVolume(i);
@@ -664,6 +763,8 @@ class RadiationIMC : public MonteCarloPhysics<PointT, GridT>, public GpuRadiatio
   TraitsT traits_;
   std::uint64_t seed_;
   std::vector<double> Erad_time_avg_, factorFleck_, planck_;
+  std::vector<GroupArray> Eg_time_avg_;
+  double nu_pow_ = 0.0, nu_ref_ = 1.0;
   std::size_t rw_ = 0, ddmc_ = 0;
 };
 

@@ -12,3 +12,6 @@ mv "$here/_build/manager_slab.tmp" "$here/_build/manager_slab"
 g++ -std=c++17 -O2 -Wall -Wextra -Wno-overloaded-virtual -I"$root/include" -o "$here/_build/plugin_checks.tmp" \
   "$here/plugin_checks.cpp" -L"$root/storm_amd/_lib" -lstorm_amd -Wl,-rpath,'$ORIGIN/../../../storm_amd/_lib'
 mv "$here/_build/plugin_checks.tmp" "$here/_build/plugin_checks"
+g++ -std=c++17 -O2 -Wall -Wextra -I"$root/include" -o "$here/_build/manager_groups.tmp" "$here/manager_groups.cpp" \
+  -L"$root/storm_amd/_lib" -lstorm_amd -Wl,-rpath,'$ORIGIN/../../../storm_amd/_lib'
+mv "$here/_build/manager_groups.tmp" "$here/_build/manager_groups"

@@ -3,9 +3,12 @@
 // parameters, traits, position sampler, seed), calls the EOS with the
 // traits' tracers, and rejects at construction an opacity model whose
 // direction overrides the GPU cannot honour
-// (radiation/RadiationOpacityModel.hpp:50-68).
+// (radiation/RadiationOpacityModel.hpp:50-68); a multigroup RadiationIMC
+// accepts an opacity model of the engine's family sigma(T) E^-p (p found
+// from the model) and rejects others.
 // Prints one line per case, "<case> ok" or "<case> FAIL: ...", and exits 0
 // only if every case behaves as expected.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <memory>
@@ -93,6 +96,64 @@ std::string construct() {
   }
 }
 
+// Multigroup: 4 groups over 1e-3..10 keV (erg), two cells.
+constexpr std::size_t kG = 4;
+constexpr double kKev = 1.602176634e-9;
+struct PowerLaw : STORM::RadiationOpacityModel<Vec3, Grid, Cell, kG> {
+  double p = 3.0;
+  double CalcPlanckOpacity(const Cell &) override { return 1.0; }
+  double CalcAbsorptionOpacity(const Cell &c, double E) override {
+    const double e = std::max(E, 1e-3 * kKev) / kKev;
+    return (1.0 + c.temperature) * std::pow(e, -p);
+  }
+};
+struct PowerLaw2 : PowerLaw {
+  PowerLaw2() { p = 2.0; }
+};
+// E^-3 below 1 keV, E^-2 above: not one power law
+struct BrokenLaw : PowerLaw {
+  double CalcAbsorptionOpacity(const Cell &, double E) override {
+    const double e = std::max(E, 1e-3 * kKev) / kKev;
+    return e < 1.0 ? std::pow(e, -3.0) : std::pow(e, -2.0);
+  }
+};
+// energy-dependent scattering
+struct ColouredScatter : PowerLaw {
+  double CalcScatteringOpacity(const Cell &, double E) override { return E / kKev; }
+};
+using IMCG = STORM::RadiationIMC<Vec3, Grid, Cell, STORM::SimpleExtensives, TracerEOS, kG>;
+
+template <typename Opacity>
+std::string construct_groups(double want_p) {
+  std::vector<double> xe = {0.0, 0.5, 1.0};
+  Grid grid(xe, 0.0, 0.5, 1, 0.0, 0.5, 1);
+  std::vector<Cell> cells(2);
+  cells[0].temperature = 1.0;
+  cells[1].temperature = 3.0;
+  std::vector<STORM::SimpleExtensives> ext(2);
+  for (auto &e : ext) e.mass = 1.0;
+  STORM::RadiationIMCParameters<kG> p;
+  p.newPhotonsPerCell = 1;
+  p.withMultigroupOpacity = true;
+  for (std::size_t g = 0; g <= kG; ++g) p.energyBoundaries[g] = kKev * 1e-3 * std::pow(1e4, static_cast<double>(g) / kG);
+  auto bc = std::make_shared<STORM::VacuumBoundary<Vec3, Grid>>(grid);
+  try {
+    IMCG imc(grid, bc, cells, ext, std::make_shared<TracerEOS>(), std::make_shared<Opacity>(), p);
+    std::vector<double> b;
+    double pow_ = 0.0, ref = 0.0;
+    bool eg = true;
+    if (!imc.gpuGroups(b, pow_, ref, eg) || b.size() != kG + 1 || eg) return "groups not handed to the manager";
+    if (std::abs(pow_ - want_p) > 1e-9) return "wrong exponent " + std::to_string(pow_);
+    double T[2], sa[2], ss[2], cv[2], e[2], m[2];
+    imc.gpuTabulate(T, sa, ss, cv, e, m);
+    // sigma(T) at nu_ref: (1 + T) (nu_ref / keV)^-p
+    if (std::abs(sa[1] - 4.0 * std::pow(ref / kKev, -want_p)) > 1e-12 * sa[1]) return "sigma(T) not tabulated at nu_ref";
+    return "";
+  } catch (const STORM::StormError &err) {
+    return std::string("StormError: ") + err.what();
+  }
+}
+
 }  // namespace
 
 int main() {
@@ -108,5 +169,9 @@ int main() {
   expect("forward_peaked_scatter", construct<ForwardPeaked>(), true);
   expect("beamed_emission", construct<Beamed>(), true);
   expect("compton_in_transport", construct<WithCompton>(), true);
+  expect("groups_power_law_3", construct_groups<PowerLaw>(3.0), false);
+  expect("groups_power_law_2", construct_groups<PowerLaw2>(2.0), false);
+  expect("groups_broken_law", construct_groups<BrokenLaw>(0.0), true);
+  expect("groups_coloured_scatter", construct_groups<ColouredScatter>(3.0), true);
   return bad ? 1 : 0;
 }

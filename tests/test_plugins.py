@@ -12,11 +12,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_cpp_plugin_surface_checks():
     """tests/cpp/plugin_checks: the reference's RadiationIMC constructor order
     (parameters, traits, position sampler, seed), EOS calls with the traits'
-    tracers, and rejection of opacity models whose direction overrides or
-    Compton flag the GPU cannot honour (no GPU needed: construction only)."""
+    tracers, rejection of opacity models whose direction overrides or
+    Compton flag the GPU cannot honour, and, in multigroup, acceptance of
+    sigma(T) E^-p models with p found from the model and rejection of other
+    energy laws (no GPU needed: construction only)."""
     exe = os.path.join(ROOT, "tests", "cpp", "_build", "plugin_checks")
     if not os.path.exists(exe):
         pytest.fail("tests/cpp/_build/plugin_checks is not built (run __graft_entry__.build())")
     out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stdout + out.stderr
-    assert out.stdout.count(" ok") == 5, out.stdout
+    assert out.stdout.count(" ok") == 9, out.stdout
