@@ -215,16 +215,30 @@ int storm_gpu_queue_source_packets(storm_gpu_engine *eng, const storm_packet_rec
  * default 256; REFILL uses at most 256; 0: chosen on line, below), segments
  * per packet and launch, and the schedule: PASS (the default) runs one packet
  * per thread for a launch, REFILL keeps every lane busy by handing a lane
- * whose packet ended the next queued packet.  Results per packet do not depend
- * on the schedule or the block size, only on segments_per_pass.  With block
- * size 0 the next six steps run the transport with blocks of 128, 256, 512,
- * 512, 256 and 128 threads (an order that cancels a steady trend in the
- * workload) and the engine then keeps the size with the highest mean rate of
- * transport segments per second (counters.transport_block shows it).  The
- * results are those of any fixed size. */
+ * whose packet ended the next queued packet (Cartesian meshes).  Results per
+ * packet do not depend on the schedule, the block size or the chunks per
+ * wavefront, only on segments_per_pass.  With block size 0 the engine tunes
+ * the launch on line, one parameter at a time: the block size (128, 256,
+ * 512), then the chunks per wavefront (1, 2, 4), then the schedule; each
+ * candidate runs one step forwards and one backwards (an order that cancels a
+ * steady trend of the workload) and the one with the highest mean rate of
+ * transport segments per second is kept (16 steps; storm_gpu_get_launch shows
+ * the choice).  The results are those of any fixed launch.  No counterpart in
+ * the reference, whose manager has no launch to tune. */
 #define STORM_GPU_SCHEDULE_PASS 0
 #define STORM_GPU_SCHEDULE_REFILL 1
 int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segments_per_pass, int32_t schedule);
+
+/* 64-packet chunks of the cell-ordered input per wavefront that a transport
+ * block owns and hands out (1, 2 or 4; default 2): the block's share of the
+ * input and of its LDS tally table.  No counterpart in the reference. */
+int storm_gpu_set_chunks_per_wave(storm_gpu_engine *eng, int32_t chunks);
+
+/* The transport launch in use: block size, chunks per wavefront, schedule,
+ * and whether on-line tuning is still running (1) or done (0).  No
+ * counterpart in the reference. */
+int storm_gpu_get_launch(storm_gpu_engine *eng, int32_t *block_size, int32_t *chunks_per_wave, int32_t *schedule,
+                         int32_t *tuning);
 
 /* Keep a copy of the next step's transport inputs (initial packets, cell
  * opacities, step constants) and its integer tallies, for an exact CPU replay

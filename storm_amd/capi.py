@@ -153,6 +153,9 @@ def load_library(path: str | None = None):
         "storm_gpu_get_ledger": ([vp, C.POINTER(Ledger)], C.c_int),
         "storm_gpu_get_instance_ledger": ([vp, C.c_uint32, C.POINTER(Ledger)], C.c_int),
         "storm_gpu_get_group_erad": ([vp, _dp], C.c_int),
+        "storm_gpu_set_chunks_per_wave": ([vp, C.c_int32], C.c_int),
+        "storm_gpu_get_launch": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32)], C.c_int),
         "storm_gpu_queue_source_packets": ([vp, C.POINTER(PacketRecord), _dp, C.c_uint64], C.c_int),
         "storm_gpu_census_size": ([vp, C.POINTER(C.c_uint64)], C.c_int),
         "storm_gpu_export_census": ([vp, C.POINTER(PacketRecord), C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),

@@ -166,9 +166,10 @@ struct DevBuf {
 inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t>((n + b - 1) / b); }
 
 constexpr uint64_t kMaxSegmentsPerStep = uint64_t(1) << 24;
-// k_transport: 64-packet chunks per wavefront of a block (a block owns
-// kChunksPerWave * block packets of the cell-ordered input and one LDS
-// tally table for all of them).
+// k_transport: 64-packet chunks per wavefront of a block by default (a block
+// owns chunks * block packets of the cell-ordered input and one LDS tally
+// table for all of them); 1, 2 or 4, storm_gpu_set_chunks_per_wave or the
+// on-line tuner.
 constexpr uint32_t kChunksPerWave = 2;
 
 template <typename Real>
@@ -236,27 +237,73 @@ struct EngineBase {
   storm_gpu_counters counters{};
   double source_T = 0.0;
   int32_t block = 256;  // k_transport; k_transport_refill uses at most 256
-  // On-line choice of the block size (storm_gpu_set_tuning with block 0):
-  // steps tried so far (-1: off) and the summed rates of each candidate.
-  static constexpr int32_t kTuneOrder[6] = {128, 256, 512, 512, 256, 128};
-  int32_t tune_step = -1;
-  double tune_rate[3] = {0.0, 0.0, 0.0};
-  void tune_before_step() {
-    if (tune_step >= 0) block = kTuneOrder[tune_step];
-  }
-  void tune_after_step(double segments, double seconds) {
-    if (tune_step < 0) return;
-    const int32_t b = kTuneOrder[tune_step];
-    tune_rate[b == 128 ? 0 : (b == 256 ? 1 : 2)] += seconds > 0.0 ? segments / seconds : 0.0;
-    if (++tune_step < 6) return;
-    int best = 1;
-    for (int k = 0; k < 3; ++k)
-      if (tune_rate[k] > tune_rate[best]) best = k;
-    block = best == 0 ? 128 : (best == 1 ? 256 : 512);
-    tune_step = -1;
-  }
+  int32_t chunks_per_wave = static_cast<int32_t>(kChunksPerWave);
   int32_t seg_per_pass = 1 << 20;
   int32_t schedule = STORM_GPU_SCHEDULE_PASS;
+  bool refill_ok = true;  // the refill schedule exists for Cartesian meshes
+
+  // On-line tuning of the transport's launch (storm_gpu_set_tuning with
+  // block 0).  A packet's history does not depend on the block size, the
+  // chunks a block hands out or the schedule, so trying them changes no
+  // result.  Three phases, one parameter each with the others at their best
+  // so far: the block size (128, 256, 512), the chunks per wavefront (1, 2,
+  // 4), the schedule (pass, refill).  A phase runs its candidates forwards
+  // then backwards, one step each (an order that cancels a steady trend of
+  // the workload), and keeps the one with the highest mean rate of transport
+  // segments per second.
+  struct LaunchShape {
+    int32_t block, chunks, schedule;
+  };
+  int32_t tune_phase = -1;  // -1: off
+  std::vector<LaunchShape> tune_seq;
+  std::vector<double> tune_sum;
+  size_t tune_pos = 0, tune_ncand = 0;
+  void tune_start() {
+    tune_phase = 0;
+    tune_plan();
+  }
+  void tune_plan() {
+    std::vector<LaunchShape> c;
+    const LaunchShape cur{block, chunks_per_wave, schedule};
+    if (tune_phase == 0) {
+      for (int32_t b : {128, 256, 512}) c.push_back({b, cur.chunks, STORM_GPU_SCHEDULE_PASS});
+    } else if (tune_phase == 1) {
+      for (int32_t k : {1, 2, 4}) c.push_back({cur.block, k, STORM_GPU_SCHEDULE_PASS});
+    } else {
+      c.push_back({cur.block, cur.chunks, STORM_GPU_SCHEDULE_PASS});
+      c.push_back({cur.block, cur.chunks, STORM_GPU_SCHEDULE_REFILL});
+    }
+    tune_ncand = c.size();
+    tune_seq = c;
+    for (size_t k = c.size(); k-- > 0;) tune_seq.push_back(c[k]);
+    tune_sum.assign(tune_ncand, 0.0);
+    tune_pos = 0;
+  }
+  void tune_before_step() {
+    if (tune_phase < 0) return;
+    const LaunchShape &t = tune_seq[tune_pos];
+    block = t.block;
+    chunks_per_wave = t.chunks;
+    schedule = t.schedule;
+  }
+  void tune_after_step(double segments, double seconds) {
+    if (tune_phase < 0) return;
+    const size_t k = tune_pos < tune_ncand ? tune_pos : 2 * tune_ncand - 1 - tune_pos;
+    tune_sum[k] += seconds > 0.0 ? segments / seconds : 0.0;
+    if (++tune_pos < tune_seq.size()) return;
+    size_t best = 0;
+    for (size_t q = 1; q < tune_ncand; ++q)
+      if (tune_sum[q] > tune_sum[best]) best = q;
+    block = tune_seq[best].block;
+    chunks_per_wave = tune_seq[best].chunks;
+    schedule = tune_seq[best].schedule;
+    ++tune_phase;
+    if (tune_phase > 2 || (tune_phase == 2 && !refill_ok)) {
+      tune_phase = -1;
+      return;
+    }
+    tune_plan();
+  }
   bool keep_inputs = false;
 };
 
@@ -392,6 +439,7 @@ struct Engine final : EngineBase {
       // run over a flat nx = N x 1 x 1 layout and packets carry cell indices
       const storm_unstructured_mesh &um = *p->unstructured;
       unstructured = true;
+      refill_ok = false;
       if (um.ncells < 1 || um.ncells > (1u << 30)) throw EngineError(1, "unstructured mesh: 1 .. 2^30 cells");
       nx = static_cast<int32_t>(um.ncells);
       ny = nz = 1;
@@ -858,16 +906,20 @@ struct Engine final : EngineBase {
     return r;
   }
 
-  // Tally-cache slots of k_transport<Real, F> at this block size: the share
-  // of the CU's LDS of each block its registers let run there.
-  uint32_t pass_slots_cache[4][4] = {};
+  // Tally-cache slots of k_transport<Real, F> at this block size and
+  // chunking: the share of the CU's LDS of each block its registers let run
+  // there.
+  static int chunk_index(int32_t k) { return k == 1 ? 0 : (k == 2 ? 1 : 2); }
+  static int block_index(int32_t b) { return b == 64 ? 0 : (b == 128 ? 1 : (b == 256 ? 2 : 3)); }
+  uint32_t pass_slots_cache[4][4][3] = {};
   template <uint32_t F>
   uint32_t pass_cache_slots(int32_t b, uint32_t nf) {
-    uint32_t &r = pass_slots_cache[F][b == 64 ? 0 : (b == 128 ? 1 : (b == 256 ? 2 : 3))];
+    uint32_t &r = pass_slots_cache[F][block_index(b)][chunk_index(chunks_per_wave)];
     if (!r) {
       int per_cu = 0;
       HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<Real, F>, b, 0));
-      r = cache_slots(static_cast<uint32_t>(b) * kChunksPerWave, nf, static_cast<uint32_t>(std::max(per_cu, 1)));
+      r = cache_slots(static_cast<uint32_t>(b) * static_cast<uint32_t>(chunks_per_wave), nf,
+                      static_cast<uint32_t>(std::max(per_cu, 1)));
     }
     return r;
   }
@@ -888,14 +940,15 @@ struct Engine final : EngineBase {
     return m;
   }
   // Tally-cache slots of the polyhedral-mesh transport at this block size.
-  uint32_t pass_slots_cache_u[4][4] = {};
+  uint32_t pass_slots_cache_u[4][4][3] = {};
   template <uint32_t F>
   uint32_t pass_cache_slots_u(int32_t b, uint32_t nf) {
-    uint32_t &r = pass_slots_cache_u[F][b == 64 ? 0 : (b == 128 ? 1 : (b == 256 ? 2 : 3))];
+    uint32_t &r = pass_slots_cache_u[F][block_index(b)][chunk_index(chunks_per_wave)];
     if (!r) {
       int per_cu = 0;
       HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<Real, F, UMeshView<Real>>, b, 0));
-      r = cache_slots(static_cast<uint32_t>(b) * kChunksPerWave, nf, static_cast<uint32_t>(std::max(per_cu, 1)));
+      r = cache_slots(static_cast<uint32_t>(b) * static_cast<uint32_t>(chunks_per_wave), nf,
+                      static_cast<uint32_t>(std::max(per_cu, 1)));
     }
     return r;
   }
@@ -907,7 +960,7 @@ struct Engine final : EngineBase {
       // one packet per lane and pass (the refill schedule is Cartesian only)
       const uint32_t nf = (precision == STORM_GPU_MIXED && groups < 2) ? 1u : 2u;
       const uint32_t slots = pass_cache_slots_u<F>(block, nf);
-      const uint32_t chunks = kChunksPerWave * static_cast<uint32_t>(block / 64);
+      const uint32_t chunks = static_cast<uint32_t>(chunks_per_wave) * static_cast<uint32_t>(block / 64);
       k_transport<Real, F, UMeshView<Real>><<<blocks_for(n_in, 64u * chunks), block, cache_bytes(slots, nf), stream>>>(
           static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1,
           umeshview(), xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
@@ -922,7 +975,7 @@ struct Engine final : EngineBase {
     } else {
       const uint32_t nf = (precision == STORM_GPU_MIXED && groups < 2) ? 1u : 2u;
       const uint32_t slots = pass_cache_slots<F>(block, nf);
-      const uint32_t chunks = kChunksPerWave * static_cast<uint32_t>(block / 64);
+      const uint32_t chunks = static_cast<uint32_t>(chunks_per_wave) * static_cast<uint32_t>(block / 64);
       k_transport<Real, F><<<blocks_for(n_in, 64u * chunks), block, cache_bytes(slots, nf), stream>>>(
           static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
           xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
@@ -2105,15 +2158,32 @@ int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segm
     if (schedule != STORM_GPU_SCHEDULE_PASS && schedule != STORM_GPU_SCHEDULE_REFILL)
       throw EngineError(1, "schedule must be STORM_GPU_SCHEDULE_PASS or STORM_GPU_SCHEDULE_REFILL");
     EngineBase *e = as_engine(eng);
+    e->seg_per_pass = segments_per_pass;
+    e->schedule = schedule;
     if (block_size == 0) {
-      e->tune_step = 0;
-      for (double &r : e->tune_rate) r = 0.0;
+      e->tune_start();
     } else {
-      e->tune_step = -1;
+      e->tune_phase = -1;
       e->block = block_size;
     }
-    as_engine(eng)->seg_per_pass = segments_per_pass;
-    as_engine(eng)->schedule = schedule;
+  });
+}
+
+int storm_gpu_set_chunks_per_wave(storm_gpu_engine *eng, int32_t chunks) {
+  return guarded([&] {
+    if (chunks != 1 && chunks != 2 && chunks != 4) throw EngineError(1, "chunks per wavefront must be 1, 2 or 4");
+    as_engine(eng)->chunks_per_wave = chunks;
+  });
+}
+
+int storm_gpu_get_launch(storm_gpu_engine *eng, int32_t *block_size, int32_t *chunks_per_wave, int32_t *schedule,
+                         int32_t *tuning) {
+  return guarded([&] {
+    EngineBase *e = as_engine(eng);
+    *block_size = e->block;
+    *chunks_per_wave = e->chunks_per_wave;
+    *schedule = e->schedule;
+    *tuning = e->tune_phase >= 0 ? 1 : 0;
   });
 }
 

@@ -97,6 +97,16 @@ class GpuEngine:
         self._check(self.lib.storm_gpu_get_group_erad(self.h, _dp(out)))
         return out.reshape(self.N, groups)
 
+    def set_chunks_per_wave(self, chunks: int):
+        self._check(self.lib.storm_gpu_set_chunks_per_wave(self.h, int(chunks)))
+
+    def launch(self) -> dict:
+        """The transport launch in use (block, chunks per wavefront, schedule)
+        and whether on-line tuning is still running."""
+        v = [C.c_int32() for _ in range(4)]
+        self._check(self.lib.storm_gpu_get_launch(self.h, *(C.byref(x) for x in v)))
+        return dict(block=v[0].value, chunks=v[1].value, schedule=v[2].value, tuning=bool(v[3].value))
+
     def segments(self) -> np.ndarray:
         a = np.empty(self.N, dtype=np.uint64)
         self._check(self.lib.storm_gpu_get_segments(self.h, a.ctypes.data_as(C.POINTER(C.c_uint64))))

@@ -392,29 +392,47 @@ def test_refill_schedule_matches_pass_schedule(make):
     assert a[2]["segments"] == b[2]["segments"]
 
 
-def test_online_block_size_choice_keeps_results():
-    """Block size 0: the first six steps try 128, 256, 512, 512, 256, 128
-    threads per block, then one is kept; the cell states and tallies are
-    those of a fixed block size, step by step."""
-    spec = C2(13)
-    spec.steps = 8
-    fixed, tuned, blocks = [], [], []
+def _tuned_against_fixed(spec, steps):
+    fixed, tuned, shapes = [], [], []
     with GpuEngine(spec, capi.MIXED) as e:
         e.set_tuning(block=256)
-        for _ in range(spec.steps):
+        for _ in range(steps):
             e.step()
             fixed.append((e.cells(), e.tallies()))
     with GpuEngine(spec, capi.MIXED) as e:
         e.set_tuning(block=0)
-        for _ in range(spec.steps):
+        for _ in range(steps):
             e.step()
             tuned.append((e.cells(), e.tallies()))
-            blocks.append(e.counters()["transport_block"])
-    assert blocks[:6] == [128, 256, 512, 512, 256, 128], blocks
-    assert blocks[6] == blocks[7] and blocks[6] in (128, 256, 512), blocks
+            L = e.launch()
+            shapes.append((e.counters()["transport_block"], L["chunks"], L["schedule"], L["tuning"]))
+    return fixed, tuned, shapes
+
+
+@pytest.mark.parametrize("mesh", ["cartesian", "voronoi"])
+def test_online_launch_tuning_keeps_results(mesh):
+    """Block size 0: the transport's launch is tuned on line -- blocks of
+    128, 256, 512, 512, 256, 128 threads, then 1, 2, 4, 4, 2, 1 chunks per
+    wavefront, then (Cartesian meshes) the pass and refill schedules -- and
+    the best is kept; the cell states and tallies are those of a fixed
+    launch, step by step."""
+    if mesh == "cartesian":
+        spec, steps, nphase = C2(13), 18, 16
+    else:
+        spec, steps, nphase = P.voronoi_slab(seed=13, steps=14), 14, 12
+    fixed, tuned, shapes = _tuned_against_fixed(spec, steps)
+    assert [b for b, _, _, _ in shapes[:6]] == [128, 256, 512, 512, 256, 128], shapes
+    assert [k for _, k, _, _ in shapes[6:12]] == [1, 2, 4, 4, 2, 1], shapes
+    if mesh == "cartesian":
+        assert [s for _, _, s, _ in shapes[12:16]] == [0, 1, 1, 0], shapes
+    # the choice after tuning is one of the candidates and stays
+    assert all(t for _, _, _, t in shapes[:nphase - 1]) and not any(t for _, _, _, t in shapes[nphase - 1:]), shapes
+    assert len(set(x[:3] for x in shapes[nphase:])) == 1, shapes
+    b, k, s, _ = shapes[-1]
+    assert b in (128, 256, 512) and k in (1, 2, 4) and s in (0, 1)
     for (ca, ta), (cb, tb) in zip(fixed, tuned):
-        for k in ("T", "e", "erad", "census"):
-            assert np.array_equal(ca[k], cb[k]), k
+        for key in ("T", "e", "erad", "census"):
+            assert np.array_equal(ca[key], cb[key]), key
         for k in ("dep", "rdep", "pint", "esc", "segs"):
             assert np.array_equal(ta[k], tb[k]), k
 
