@@ -4,6 +4,12 @@
  * Plain C: opaque handle, POD structs, pointers and sizes, int status codes
  * (0 = success) with the message of the last failure on this thread in
  * storm_gpu_last_error().  No torch, HIP or C++ types cross this boundary.
+ * Status: 1 invalid argument or problem, 2 device or HIP failure, 4 a size
+ * or capacity limit, 5 a physics error the reference also stops on
+ * (RadiationIMC.hpp:771-817, Fleck factor) or the tally bound (a cell with
+ * 2^28 segments in a step; the material state is then left as it was) --
+ * the step stopped part-way, and the engine refuses further steps -- and 6 a
+ * packet beyond 2^24 segments in one step.
  *
  * The entry points replace the serial manager and its IMC physics plugin of
  * the reference for one GPU:

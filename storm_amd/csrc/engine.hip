@@ -1051,7 +1051,11 @@ struct Engine final : EngineBase {
     }
   };
 
+  bool failed_ = false;  // a step stopped part-way with status 5
   void step(double dt) override {
+    if (failed_)
+      throw EngineError(5, "an earlier step stopped part-way (status 5); the engine's state is not consistent: "
+                           "destroy it");
     if (!(dt > 0.0)) throw EngineError(1, "dt must be positive");
     auto t_start = std::chrono::steady_clock::now();
     StepRetire retire(*this);
@@ -1340,7 +1344,7 @@ struct Engine final : EngineBase {
       n_cen_total = h[1];
       ping = !ping;
     }
-    k_fold_replicas<<<blocks_for(N, 256), 256, 0, stream>>>(tal, kTallyReplicas);
+    k_fold_replicas<<<blocks_for(N, 256), 256, 0, stream>>>(tal, kTallyReplicas, err.p);
     counters.seconds_transport = t_transport;
     counters.seconds_transport_max_launch = t_max;
     counters.census_before_comb = n_cen_total;
@@ -1377,9 +1381,13 @@ struct Engine final : EngineBase {
     last_eq_ = eq;
     last_et_ = et;
     last_dt_ = dt;
+    // a step that stopped here has replaced the census: the engine takes no
+    // further step (status 5 again) and is to be destroyed
+    if (herr & (kErrFleck | kErrNegativeEnergy | kErrTallyOverflow)) failed_ = true;
     if (herr & kErrFleck) throw EngineError(5, "Invalid factor fleck in RadiationIMC::preStep");
     if (herr & kErrNegativeEnergy) throw EngineError(5, "Negative material internal energy in RadiationIMC");
-    if (herr & kErrTallyOverflow) throw EngineError(5, "a cell received 2^28 tally additions in one step");
+    if (herr & kErrTallyOverflow)
+      throw EngineError(5, "a cell received 2^28 tally additions in one step; the material state was not updated");
     tune_after_step(static_cast<double>(counters.segments), counters.seconds_transport);
     auto t_end = std::chrono::steady_clock::now();
     counters.seconds_step = std::chrono::duration<double>(t_end - t_start).count();

@@ -1727,6 +1727,7 @@ __global__ __launch_bounds__(256) void k_post(uint32_t N, Tallies t, int32_t eq,
                                               InstanceView inst) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
+  if (*err & kErrTallyOverflow) return;  // the step's tallies are not trusted: commit nothing
   int32_t sh = 0;  // batches: the cell's instance unit is 2^-sh erg (2^-sh_prev for the census' requant)
   if (inst.cell) {
     const uint32_t k = inst.cell[c];
@@ -1775,7 +1776,6 @@ __global__ __launch_bounds__(256) void k_post(uint32_t N, Tallies t, int32_t eq,
     const double rho = mat.density(c);
     T[c] = mat.de2T(rho, E / (rho * Vc));  // synchronizeMaterialCell (:830-866)
   }
-  if (t.segs[c] >= (1u << 28)) atomicOr(err, kErrTallyOverflow);
 }
 
 // out[0] += sum of v[0..N) (integer, order independent)
@@ -1820,13 +1820,19 @@ __global__ void k_fill_f64(double *p, uint64_t n, double v) {
 
 // Adds tally replicas 1..R-1 into replica 0, cell by cell in replica order
 // (integer sums are exact; the FP64 engine's sums are in a fixed order).
-__global__ __launch_bounds__(256) void k_fold_replicas(Tallies t, uint32_t reps) {
+// Also the check of the two-limb tallies' bound: a cell's low limbs take up
+// to 2^28 additions of values below 2^36 within 64 bits, so a cell with 2^28
+// or more segments in the step sets kErrTallyOverflow here, before the comb
+// and postStep, and postStep then commits nothing.  The segment count
+// saturates at 2^32 - 1.
+__global__ __launch_bounds__(256) void k_fold_replicas(Tallies t, uint32_t reps, uint32_t *__restrict__ err) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= t.ncell) return;
   const size_t n = t.ncell;
-  uint32_t sg = t.segs[c];
+  uint64_t sg = t.segs[c];
   for (uint32_t r = 0; r + 1 < reps; ++r) sg += t.segs_rep[r * n + c];
-  t.segs[c] = sg;
+  t.segs[c] = sg > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(sg);
+  if (sg >= (1ull << 28)) atomicOr(err, kErrTallyOverflow);
   if (t.dep_rep) {
     unsigned long long d0 = t.dep[2 * c], d1 = t.dep[2 * c + 1];
     unsigned long long p0 = t.pint[2 * c], p1 = t.pint[2 * c + 1];
