@@ -132,8 +132,9 @@ int storm_gpu_destroy(storm_gpu_engine *eng);
 int storm_gpu_step(storm_gpu_engine *eng, double dt);
 
 /* storm_gpu_step without any allocation.  No counterpart in the reference:
- * for callers that cannot allocate device memory (code under HIP graph
- * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * for callers that cannot allocate device memory (code that runs beside HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc; the
+ * step itself synchronises with the host, so it is not captured).  The same
  * step, bit for bit, with every transient buffer of the step (packets in
  * flight, the census before population control, sort keys, prefix sums,
  * reduction partials, counters) carved out of a fixed 64 MiB device array of
