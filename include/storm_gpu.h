@@ -219,12 +219,16 @@ int storm_gpu_queue_source_packets(storm_gpu_engine *eng, const storm_packet_rec
                                    uint64_t n);
 
 /* Launch parameters of the transport: block size (64, 128, 256, 512; the
- * default 256; REFILL uses at most 256; 0: chosen on line, below), segments
- * per packet and launch, and the schedule: PASS (the default) runs one packet
- * per thread for a launch, REFILL keeps every lane busy by handing a lane
- * whose packet ended the next queued packet (Cartesian meshes).  Results per
- * packet do not depend on the schedule, the block size or the chunks per
- * wavefront, only on segments_per_pass.  With block size 0 the engine tunes
+ * default 256; 0: chosen on line, below), segments per packet and launch,
+ * and the schedule.  PASS runs each lane's packet to its end (or the segment
+ * budget) before the wavefront takes the next 64; REFILL advances the
+ * wavefront's packets one segment at a time and hands lanes whose packet
+ * ended the next packets of the block's range, four or more lanes at a time.
+ * Default: PASS in the mixed engine, REFILL in the FP64 engine.  A packet's
+ * history does not depend on the schedule, the block size or the chunks per
+ * wavefront, only on segments_per_pass: the mixed engine's results are
+ * bit-identical across them, the FP64 engine's tallies differ only in the
+ * order of their floating-point sums.  With block size 0 the engine tunes
  * the launch on line, one parameter at a time: the block size (128, 256,
  * 512), then the chunks per wavefront (1, 2, 4), then the schedule; each
  * candidate runs one step forwards and one backwards (an order that cancels a

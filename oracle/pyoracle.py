@@ -238,6 +238,7 @@ def rlib():
                                            C.POINTER(C.c_int64)]
         L.cpu_first_step.argtypes = [C.c_void_p, C.c_double, C.c_int32, C.c_uint32, dp, dp, dp, dp]
         L.replay_rw_tables.argtypes = [dp, dp, dp]
+        L.replay_rw_leak_table.argtypes = [dp]
         L.replay_planck.argtypes = [C.c_int64, dp, dp, dp]
         L.replay_rw_sample.argtypes = [C.c_int32, C.c_int64, dp, dp, dp, dp]
         _rlib = L
@@ -328,6 +329,14 @@ def engine_rw_tables():
     tau, surv, rad = np.empty(1024), np.empty(1024), np.empty(1024 * 256)
     L.replay_rw_tables(_dp(tau), _dp(surv), _dp(rad))
     return tau, surv, rad.reshape(1024, 256)
+
+
+def engine_rw_leak_table():
+    """The engine's inverse leak-time table T(y_i), y_i = exp(-25 + 26.5 i / 2048)."""
+    L = rlib()
+    leak = np.empty(2049)
+    L.replay_rw_leak_table(_dp(leak))
+    return leak
 
 
 def engine_rw_sample(precision, xi, tau):

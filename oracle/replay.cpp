@@ -67,12 +67,13 @@ struct HostSink {
 // Random-walk tables in the engine's precision (same host build as the engine).
 template <typename Real>
 struct HostTables {
-  std::vector<Real> tau, surv, rad;
+  std::vector<Real> tau, leak, rad;
   HostTables() {
-    std::vector<double> t(rw::kTableSize), s(rw::kTableSize), r(rw::kTableSize * rw::kRadiusSize);
+    std::vector<double> t(rw::kTableSize), s(rw::kTableSize), r(rw::kTableSize * rw::kRadiusSize), l(rw::kLeakSize);
     rw::build_tables(t.data(), s.data(), r.data());
+    rw::build_leak_table(l.data());
     tau.assign(t.begin(), t.end());
-    surv.assign(s.begin(), s.end());
+    leak.assign(l.begin(), l.end());
     rad.assign(r.begin(), r.end());
   }
 };
@@ -83,7 +84,7 @@ AccelView<Real> make_accel_view(const CellAccel<Real> *cells, const HostTables<R
   AccelView<Real> a;
   a.cell = cells;
   a.rw_tau = tb ? tb->tau.data() : nullptr;
-  a.rw_survival = tb ? tb->surv.data() : nullptr;
+  a.rw_leak = tb ? tb->leak.data() : nullptr;
   a.rw_radius = tb ? tb->rad.data() : nullptr;
   a.rw_log_tau_min = static_cast<Real>(std::log(rw::kTauMin));
   a.rw_log_tau_max = static_cast<Real>(std::log(rw::kTauMax));
@@ -336,6 +337,9 @@ void replay_fpmath(int32_t which, int64_t n, const float *x, float *y) {
 // (0 float, 1 double) on a batch.
 void replay_rw_tables(double *tau, double *survival, double *radius) { rw::build_tables(tau, survival, radius); }
 
+// The engine's inverse leak-time table T(y), rw::kLeakSize doubles.
+void replay_rw_leak_table(double *leak) { rw::build_leak_table(leak); }
+
 // The engine's normalised Planck integral over [a_i, b_i] (in units of kT).
 void replay_planck(int64_t n, const double *a, const double *b, double *out) {
   for (int64_t i = 0; i < n; ++i) out[i] = planck::integral(a[i], b[i]);
@@ -346,7 +350,7 @@ void replay_rw_sample(int32_t precision, int64_t n, const double *xi, const doub
   auto go = [&](auto tag) {
     using Real = decltype(tag);
     HostTables<Real> tb;
-    rw::Tables<Real> t{tb.tau.data(), tb.surv.data(), tb.rad.data(), static_cast<Real>(std::log(rw::kTauMin)),
+    rw::Tables<Real> t{tb.tau.data(), tb.leak.data(), tb.rad.data(), static_cast<Real>(std::log(rw::kTauMin)),
                        static_cast<Real>(std::log(rw::kTauMax))};
     for (int64_t i = 0; i < n; ++i) {
       leak[i] = static_cast<double>(rw::sample_leak_time<Real>(t, static_cast<Real>(xi[i])));

@@ -13,8 +13,12 @@
 // The series are truncated after 500 terms or once n^2 pi^2 tau > 700, as
 // the reference does.
 //
-// Samplers (RandomWalk.hpp:81-137): leak time by inverse interpolation of S
-// in log tau, radius by bilinear interpolation in (log tau, xi).
+// Samplers (RandomWalk.hpp:81-137): radius by bilinear interpolation in
+// (log tau, xi), as the reference does.  The leak time inverts the same
+// S(tau) through a table of the inverse itself (build_leak_table), where the
+// reference searches its S table: the same distribution (the two agree to
+// ~1e-5 in tau), with one log and two neighbouring loads in place of a chain
+// of ten dependent loads per random-walk step.
 #pragma once
 
 #include <stdint.h>
@@ -29,33 +33,48 @@ constexpr int kTableSize = 1024;
 constexpr int kRadiusSize = 256;
 constexpr double kTauMin = 1e-8;
 constexpr double kTauMax = 64.0;
+// Inverse leak-time table: T(y), y = -ln S(T), on kLeakSize points uniform
+// in ln y over [kLeakLogYMin, kLeakLogYMax] (y from 1.4e-11 to 4.48).
+constexpr int kLeakSize = 2049;
+constexpr double kLeakLogYMin = -25.0;
+constexpr double kLeakLogYMax = 1.5;
 
 template <typename Real>
 struct Tables {
   const Real *tau;       // [kTableSize]
-  const Real *survival;  // [kTableSize], decreasing
+  const Real *leak;      // [kLeakSize], increasing (build_leak_table)
   const Real *radius;    // [kTableSize * kRadiusSize]
   Real log_tau_min, log_tau_max;
 };
 
-// Leak time for a uniform xi: smallest table index with S <= 1 - xi, then
-// log-linear interpolation between its neighbours.
+// Leak time for a uniform xi: the T with S(T) = 1 - xi.  With
+// y = -ln(1 - xi), T is interpolated linearly in ln y between the two
+// neighbouring table points.  Past the table the first mode alone,
+// S = 2 exp(-pi^2 T), gives T = (y + ln 2) / pi^2; the next mode is below
+// exp(-3 pi^2 T) < 2e-7 of it there.  Below the table (xi < 1.4e-11) T is
+// the first entry; xi = 1 gives kTauMax, as the reference's table end does.
 template <typename Real>
 STORM_HD Real sample_leak_time(const Tables<Real> &t, Real xi) {
   using M = MathPolicy<Real>;
-  const Real target = Real(1) - xi;
-  if (target >= t.survival[0]) return t.tau[0];
-  if (target <= t.survival[kTableSize - 1]) return t.tau[kTableSize - 1];
-  int lo = 0, hi = kTableSize - 1;  // survival[lo] > target >= survival[hi]
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (t.survival[mid] > target) lo = mid;
-    else hi = mid;
+  const Real s = Real(1) - xi;
+  if (!(s > Real(0))) return Real(kTauMax);
+  // small xi: the series of -ln(1 - xi) (relative error below xi^5 / 6),
+  // since 1 - xi keeps too few of xi's bits in float
+  const Real y = xi < Real(0.015625)
+                     ? xi * (Real(1) + xi * (Real(0.5) + xi * (Real(1) / Real(3) + xi * (Real(0.25) + xi * Real(0.2)))))
+                     : -M::log(s);
+  const Real ly = y > Real(0) ? M::log(y) : Real(kLeakLogYMin);
+  if (ly >= Real(kLeakLogYMax)) {
+    const Real T = (y + Real(0.69314718055994530942)) * Real(0.10132118364233777144);  // 1 / pi^2
+    return T < Real(kTauMax) ? T : Real(kTauMax);
   }
-  const Real S0 = t.survival[hi - 1], S1 = t.survival[hi];
-  const Real frac = (S0 - target) / (S0 - S1);
-  const Real l0 = M::log(t.tau[hi - 1]), l1 = M::log(t.tau[hi]);
-  return M::exp(l0 + frac * (l1 - l0));
+  Real pos = (ly - Real(kLeakLogYMin)) * Real((kLeakSize - 1) / (kLeakLogYMax - kLeakLogYMin));
+  if (pos < Real(0)) pos = Real(0);
+  int i = static_cast<int>(pos);
+  if (i > kLeakSize - 2) i = kLeakSize - 2;
+  const Real f = pos - static_cast<Real>(i);
+  const Real t0 = t.leak[i], t1 = t.leak[i + 1];
+  return t0 + f * (t1 - t0);
 }
 
 template <typename Real>
@@ -155,6 +174,27 @@ inline void build_tables(double *tau, double *survival, double *radius) {
         out = x0 + fr * (x1 - x0);
       }
     }
+  }
+}
+
+// T(y_i) for the leak sampler: the root of survival_series(T) = exp(-y_i)
+// at y_i = exp(kLeakLogYMin + i h), by bisection in ln T (T grows with y, so
+// each root brackets the next from below; S(1e-3) is 1 to double precision
+// and S(kTauMax) ~ 1e-274, so [1e-3, kTauMax] holds every root).
+inline void build_leak_table(double *leak) {
+  double lo0 = log(1e-3);
+  const double hi0 = log(kTauMax);
+  for (int i = 0; i < kLeakSize; ++i) {
+    const double ly = kLeakLogYMin + (kLeakLogYMax - kLeakLogYMin) * static_cast<double>(i) / (kLeakSize - 1);
+    const double target = exp(-exp(ly));
+    double lo = lo0, hi = hi0;
+    for (int it = 0; it < 64 && hi - lo > 1e-15; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      if (survival_series(exp(mid)) > target) lo = mid;
+      else hi = mid;
+    }
+    leak[i] = exp(0.5 * (lo + hi));
+    lo0 = lo;
   }
 }
 

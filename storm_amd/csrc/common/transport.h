@@ -174,7 +174,7 @@ template <typename Real>
 struct AccelView {
   const CellAccel<Real> *cell;  // null: no random walk and no DDMC in this problem
   const Real *rw_tau;
-  const Real *rw_survival;
+  const Real *rw_leak;      // [rw::kLeakSize]
   const Real *rw_radius;
   Real rw_log_tau_min, rw_log_tau_max;
   Real rw_particle_tau;    // rwMinParticleOpticalDepth
@@ -561,7 +561,7 @@ struct Tracker {
           // ---------------- random-walk step (tryRandomWalkStep)
           const Philox4 r = rs.block(ctr++);
           const Philox4 r2 = rs.block(ctr++);
-          rw::Tables<Real> tb{acc.rw_tau, acc.rw_survival, acc.rw_radius, acc.rw_log_tau_min, acc.rw_log_tau_max};
+          rw::Tables<Real> tb{acc.rw_tau, acc.rw_leak, acc.rw_radius, acc.rw_log_tau_min, acc.rw_log_tau_max};
           const Real tauLeak = rw::sample_leak_time<Real>(tb, M::uniform(r.v[0]));
           // tLeak = tau Ro^2 / D with D = c / (3 sig_d): as a path, tau Ro^2 3 sig_d
           const Real sig_d = pgrw ? ca.sig_d : ca.sig_t;
@@ -844,6 +844,12 @@ struct Tracker {
     }
     return outcome;
   }
+};
+
+// The tracker of a mesh type (the polyhedral mesh's is in unstructured.h).
+template <typename Real, uint32_t F, typename MeshT>
+struct TrackerOf {
+  using type = Tracker<Real, F>;
 };
 
 // One transport pass of a packet: at most `max_segments` segments, then the

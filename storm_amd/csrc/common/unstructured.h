@@ -299,7 +299,7 @@ struct UTracker {
           // ---------------- random-walk step (tryRandomWalkStep)
           const Philox4 r = rs.block(ctr++);
           const Philox4 r2 = rs.block(ctr++);
-          rw::Tables<Real> tb{acc.rw_tau, acc.rw_survival, acc.rw_radius, acc.rw_log_tau_min, acc.rw_log_tau_max};
+          rw::Tables<Real> tb{acc.rw_tau, acc.rw_leak, acc.rw_radius, acc.rw_log_tau_min, acc.rw_log_tau_max};
           const Real tauLeak = rw::sample_leak_time<Real>(tb, M::uniform(r.v[0]));
           const Real sig_d = pgrw ? ca.sig_d : ca.sig_t;
           const Real sLeak = tauLeak * Ro * Ro * (Real(3) * sig_d);
@@ -557,6 +557,11 @@ struct UTracker {
     }
     return outcome;
   }
+};
+
+template <typename Real, uint32_t F>
+struct TrackerOf<Real, F, UMeshView<Real>> {
+  using type = UTracker<Real, F>;
 };
 
 template <typename Real, uint32_t F = kFeatAll, typename Sink>

@@ -172,6 +172,26 @@ constexpr uint64_t kMaxSegmentsPerStep = uint64_t(1) << 24;
 // on-line tuner.
 constexpr uint32_t kChunksPerWave = 2;
 
+// The random-walk tables in double, built once per process (the engine
+// converts them to its precision on upload).
+struct RwHostTables {
+  std::vector<double> tau, survival, radius, leak;
+};
+
+static const RwHostTables &rw_host_tables() {
+  static const RwHostTables tb = [] {
+    RwHostTables t;
+    t.tau.resize(rw::kTableSize);
+    t.survival.resize(rw::kTableSize);
+    t.radius.resize(static_cast<size_t>(rw::kTableSize) * rw::kRadiusSize);
+    t.leak.resize(rw::kLeakSize);
+    rw::build_tables(t.tau.data(), t.survival.data(), t.radius.data());
+    rw::build_leak_table(t.leak.data());
+    return t;
+  }();
+  return tb;
+}
+
 template <typename Real>
 struct PacketBufs {
   DevBuf<typename V4<Real>::T> pos, dir;
@@ -240,7 +260,6 @@ struct EngineBase {
   int32_t chunks_per_wave = static_cast<int32_t>(kChunksPerWave);
   int32_t seg_per_pass = 1 << 20;
   int32_t schedule = STORM_GPU_SCHEDULE_PASS;
-  bool refill_ok = true;  // the refill schedule exists for Cartesian meshes
 
   // On-line tuning of the transport's launch (storm_gpu_set_tuning with
   // block 0).  A packet's history does not depend on the block size, the
@@ -298,7 +317,7 @@ struct EngineBase {
     chunks_per_wave = tune_seq[best].chunks;
     schedule = tune_seq[best].schedule;
     ++tune_phase;
-    if (tune_phase > 2 || (tune_phase == 2 && !refill_ok)) {
+    if (tune_phase > 2) {
       tune_phase = -1;
       return;
     }
@@ -358,7 +377,7 @@ struct Engine final : EngineBase {
   std::vector<double> planck_acc;
   DevBuf<CellAccel<Real>> accel;
   DevBuf<double> Dtil;
-  DevBuf<Real> rw_tau, rw_surv, rw_rad;
+  DevBuf<Real> rw_tau, rw_leak, rw_rad;
   bool accel_on = false;
   // multigroup (storm_group_desc)
   int32_t groups = 0;
@@ -375,6 +394,7 @@ struct Engine final : EngineBase {
 
   PacketBufs<Real> pa, pb;
   CensusBufs<Real> cpre, cnext;
+  CensusBufs<Real> cstage;  // k_transport_refill: census staged per block (its input range)
 
   // polyhedral meshes (storm_unstructured_mesh): host tables and their device copies
   bool unstructured = false;
@@ -439,7 +459,6 @@ struct Engine final : EngineBase {
       // run over a flat nx = N x 1 x 1 layout and packets carry cell indices
       const storm_unstructured_mesh &um = *p->unstructured;
       unstructured = true;
-      refill_ok = false;
       if (um.ncells < 1 || um.ncells > (1u << 30)) throw EngineError(1, "unstructured mesh: 1 .. 2^30 cells");
       nx = static_cast<int32_t>(um.ncells);
       ny = nz = 1;
@@ -507,9 +526,13 @@ struct Engine final : EngineBase {
       nu_ref = p->groups.nu_ref;
       if (!(nu_ref > 0.0)) throw EngineError(1, "groups.nu_ref must be positive");
     }
-    // one packet per thread per pass: measured faster than the refill
-    // schedule on the 64^3 cube for both engines
-    schedule = STORM_GPU_SCHEDULE_PASS;
+    // The mixed engine (6 waves per SIMD) runs one packet per lane and pass:
+    // within 4% of lane refill on the cube and the step slab, 27% faster on
+    // the Voronoi mesh.  The FP64 engine (2-3 waves per SIMD, where a lane
+    // idling until its chunk's longest history ends costs more) refills
+    // lanes: 18% faster on the cube, 12% on the step slab, even on the
+    // Voronoi mesh (20% slower on the batch of slabs, where the tuner picks).
+    schedule = precision == STORM_GPU_FP64 ? STORM_GPU_SCHEDULE_REFILL : STORM_GPU_SCHEDULE_PASS;
     if (bnd.source_face != -1 && bnd.source_face != STORM_XLO)
       throw EngineError(1, "source_face must be -1 or STORM_XLO (an x+ source is source_xhi)");
     if (bnd.source_xhi && !(bnd.source_temperature_xhi >= 0.0)) throw EngineError(1, "invalid x+ source temperature");
@@ -720,18 +743,18 @@ struct Engine final : EngineBase {
       accel.ensure(N);
       Dtil.ensure(N);
       if (imc.with_random_walk) {
-        // random-walk tables built once on the host, like the RandomWalk
-        // constructor the reference calls from RadiationIMC's constructor
-        std::vector<double> t(rw::kTableSize), s(rw::kTableSize), r(rw::kTableSize * rw::kRadiusSize);
-        rw::build_tables(t.data(), s.data(), r.data());
+        // random-walk tables built on the host (once per process), like the
+        // RandomWalk constructor the reference calls from RadiationIMC's
+        // constructor
+        const RwHostTables &tb = rw_host_tables();
         auto up_r = [&](DevBuf<Real> &d, const std::vector<double> &v) {
           std::vector<Real> h(v.begin(), v.end());
           d.ensure(h.size());
           HIPCHECK(hipMemcpy(d.p, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice));
         };
-        up_r(rw_tau, t);
-        up_r(rw_surv, s);
-        up_r(rw_rad, r);
+        up_r(rw_tau, tb.tau);
+        up_r(rw_leak, tb.leak);
+        up_r(rw_rad, tb.radius);
       }
     }
     tally_u64.ensure(9 * 2 * static_cast<size_t>(N));
@@ -846,7 +869,7 @@ struct Engine final : EngineBase {
     AccelView<Real> a;
     a.cell = accel_on ? accel.p : nullptr;
     a.rw_tau = rw_tau.p;
-    a.rw_survival = rw_surv.p;
+    a.rw_leak = rw_leak.p;
     a.rw_radius = rw_rad.p;
     a.rw_log_tau_min = static_cast<Real>(std::log(rw::kTauMin));
     a.rw_log_tau_max = static_cast<Real>(std::log(rw::kTauMax));
@@ -890,42 +913,6 @@ struct Engine final : EngineBase {
     return s;
   }
 
-  // Workgroups of a refill transport kernel that fit on the device at once.
-  uint32_t resident_cache[4][3] = {};
-  template <uint32_t F>
-  uint32_t resident_blocks(int32_t b) {
-    uint32_t &r = resident_cache[F][b == 64 ? 0 : (b == 128 ? 1 : 2)];
-    if (!r) {
-      int dev = 0, per_cu = 0;
-      HIPCHECK(hipGetDevice(&dev));
-      hipDeviceProp_t prop;
-      HIPCHECK(hipGetDeviceProperties(&prop, dev));
-      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport_refill<Real, F>, b, 0));
-      r = static_cast<uint32_t>(std::max(per_cu, 1) * std::max(prop.multiProcessorCount, 1));
-    }
-    return r;
-  }
-
-  // Tally-cache slots of k_transport<Real, F> at this block size and
-  // chunking: the share of the CU's LDS of each block its registers let run
-  // there.
-  static int chunk_index(int32_t k) { return k == 1 ? 0 : (k == 2 ? 1 : 2); }
-  static int block_index(int32_t b) { return b == 64 ? 0 : (b == 128 ? 1 : (b == 256 ? 2 : 3)); }
-  uint32_t pass_slots_cache[4][4][3] = {};
-  template <uint32_t F>
-  uint32_t pass_cache_slots(int32_t b, uint32_t nf) {
-    uint32_t &r = pass_slots_cache[F][block_index(b)][chunk_index(chunks_per_wave)];
-    if (!r) {
-      int per_cu = 0;
-      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<Real, F>, b, 0));
-      r = cache_slots(static_cast<uint32_t>(b) * static_cast<uint32_t>(chunks_per_wave), nf,
-                      static_cast<uint32_t>(std::max(per_cu, 1)));
-    }
-    return r;
-  }
-
-  // One transport launch on the kernel variant compiled for this problem's
-  // features (random walk / DDMC, multigroup).
   UMeshView<Real> umeshview() const {
     UMeshView<Real> m;
     m.ncells = N;
@@ -939,47 +926,48 @@ struct Engine final : EngineBase {
     m.spec = spectrum();
     return m;
   }
-  // Tally-cache slots of the polyhedral-mesh transport at this block size.
-  uint32_t pass_slots_cache_u[4][4][3] = {};
-  template <uint32_t F>
-  uint32_t pass_cache_slots_u(int32_t b, uint32_t nf) {
-    uint32_t &r = pass_slots_cache_u[F][block_index(b)][chunk_index(chunks_per_wave)];
+  // Tally-cache slots of a transport kernel at this block size and chunking:
+  // the share of the CU's LDS of each block its registers let run there.
+  static int chunk_index(int32_t k) { return k == 1 ? 0 : (k == 2 ? 1 : 2); }
+  static int block_index(int32_t b) { return b == 64 ? 0 : (b == 128 ? 1 : (b == 256 ? 2 : 3)); }
+  uint32_t slots_cache[2][2][4][4][3] = {};  // [refill][polyhedral][F][block][chunks]
+  template <typename K>
+  uint32_t cache_slots_for(K kernel, bool refill, uint32_t F, int32_t b, uint32_t nf) {
+    uint32_t &r = slots_cache[refill ? 1 : 0][unstructured ? 1 : 0][F][block_index(b)][chunk_index(chunks_per_wave)];
     if (!r) {
       int per_cu = 0;
-      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<Real, F, UMeshView<Real>>, b, 0));
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, b, 0));
       r = cache_slots(static_cast<uint32_t>(b) * static_cast<uint32_t>(chunks_per_wave), nf,
                       static_cast<uint32_t>(std::max(per_cu, 1)));
     }
     return r;
   }
 
+  // One transport launch on the kernel variant compiled for this problem's
+  // features (random walk / DDMC, multigroup) and mesh, in the schedule in use.
+  template <uint32_t F, typename MeshT>
+  void launch_transport_m(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshT &mesh,
+                          const AccelView<Real> &av, const StepConsts &kc) {
+    const uint32_t nf = (precision == STORM_GPU_MIXED && groups < 2) ? 1u : 2u;
+    const uint32_t chunks = static_cast<uint32_t>(chunks_per_wave) * static_cast<uint32_t>(block / 64);
+    const uint32_t grid = blocks_for(n_in, 64u * chunks);
+    if (schedule == STORM_GPU_SCHEDULE_REFILL) {
+      const uint32_t slots = cache_slots_for(k_transport_refill<Real, F, MeshT>, true, F, block, nf);
+      k_transport_refill<Real, F, MeshT><<<grid, block, cache_bytes(slots, nf), stream>>>(
+          static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mesh,
+          xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks, cstage.soa());
+    } else {
+      const uint32_t slots = cache_slots_for(k_transport<Real, F, MeshT>, false, F, block, nf);
+      k_transport<Real, F, MeshT><<<grid, block, cache_bytes(slots, nf), stream>>>(
+          static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mesh,
+          xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
+    }
+  }
   template <uint32_t F>
   void launch_transport_f(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshView<Real> &mv,
                           const AccelView<Real> &av, const StepConsts &kc) {
-    if (unstructured) {
-      // one packet per lane and pass (the refill schedule is Cartesian only)
-      const uint32_t nf = (precision == STORM_GPU_MIXED && groups < 2) ? 1u : 2u;
-      const uint32_t slots = pass_cache_slots_u<F>(block, nf);
-      const uint32_t chunks = static_cast<uint32_t>(chunks_per_wave) * static_cast<uint32_t>(block / 64);
-      k_transport<Real, F, UMeshView<Real>><<<blocks_for(n_in, 64u * chunks), block, cache_bytes(slots, nf), stream>>>(
-          static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1,
-          umeshview(), xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
-      return;
-    }
-    if (schedule == STORM_GPU_SCHEDULE_REFILL) {
-      const int32_t rb = std::min<int32_t>(block, 256);
-      const uint32_t grid = std::min<uint32_t>(blocks_for(n_in, rb), resident_blocks<F>(rb));
-      k_transport_refill<Real, F><<<grid, rb, 0, stream>>>(
-          static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
-          xs.p, av, kc, seg_per_pass, tal, leaving.p, counters_u32.p + 2);
-    } else {
-      const uint32_t nf = (precision == STORM_GPU_MIXED && groups < 2) ? 1u : 2u;
-      const uint32_t slots = pass_cache_slots<F>(block, nf);
-      const uint32_t chunks = static_cast<uint32_t>(chunks_per_wave) * static_cast<uint32_t>(block / 64);
-      k_transport<Real, F><<<blocks_for(n_in, 64u * chunks), block, cache_bytes(slots, nf), stream>>>(
-          static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mv,
-          xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
-    }
+    if (unstructured) launch_transport_m<F>(n_in, src, dst, umeshview(), av, kc);
+    else launch_transport_m<F>(n_in, src, dst, mv, av, kc);
   }
   void launch_transport(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshView<Real> &mv,
                         const AccelView<Real> &av, const StepConsts &kc) {
@@ -1303,6 +1291,7 @@ struct Engine final : EngineBase {
 
     // transport passes until every packet is in census, absorbed or gone
     tune_before_step();
+    if (schedule == STORM_GPU_SCHEDULE_REFILL) cstage.ensure(n_total);
     counters.transport_block = block;
     const MeshView<Real> mv = meshview();
     const AccelView<Real> av = accelview();
@@ -1326,7 +1315,6 @@ struct Engine final : EngineBase {
       PacketBufs<Real> &src = ping ? pa : pb;
       PacketBufs<Real> &dst = ping ? pb : pa;
       zero_u32(counters_u32.p, 1);  // n_out
-      if (schedule == STORM_GPU_SCHEDULE_REFILL) zero_u32(counters_u32.p + 2, 1);  // queue head
       HIPCHECK(hipEventRecord(ev0, stream));
       launch_transport(n_in, src, dst, mv, av, kc);
       HIPCHECK(hipEventRecord(ev1, stream));
@@ -1432,6 +1420,10 @@ struct Engine final : EngineBase {
     f(cpre.dir);
     f(cpre.w);
     f(cpre.meta);
+    f(cstage.pos);
+    f(cstage.dir);
+    f(cstage.w);
+    f(cstage.meta);
     f(keys);
     f(keys_out);
     f(vals);
@@ -1442,7 +1434,7 @@ struct Engine final : EngineBase {
     f(offs64);
     f(temp);
   }
-  static constexpr size_t kTransientBufs = 6 + 2 * 5 + 4 + 9;
+  static constexpr size_t kTransientBufs = 6 + 2 * 5 + 4 + 4 + 9;
 
   void step_noalloc(double dt) override {
     if (keep_inputs)

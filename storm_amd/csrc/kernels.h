@@ -114,6 +114,15 @@ __device__ __forceinline__ void store_census(const CensusSoA<Real> &c, uint32_t 
   c.meta[i] = make_uint2(p.ijk, p.id);
 }
 
+template <typename Real>
+__device__ __forceinline__ void copy_census(const CensusSoA<Real> &from, uint32_t i, const CensusSoA<Real> &to,
+                                            uint32_t j) {
+  to.pos[j] = from.pos[i];
+  to.dir[j] = from.dir[i];
+  to.w[j] = from.w[i];
+  to.meta[j] = from.meta[i];
+}
+
 // Per-cell integer tallies: pairs (lo sum, hi sum) of fixed-point limbs.
 struct Tallies {
   unsigned long long *dep;   // [2N] material deposit by continuous absorption
@@ -709,16 +718,9 @@ __global__ __launch_bounds__(256) void k_boundary(uint32_t n_bc, uint32_t npc, c
 
 // ------------------------------------------------------------------ transport
 // Packets enter the transport ordered by cell (the census comes sorted from
-// population control, emission is generated cell by cell).  k_transport
-// keeps that order (a block's visits then fall in few cells, which its LDS
-// tally cache exploits).  k_transport_refill has no block-local cache and
-// spreads instead, so that the 64 lanes of a wavefront do not send their
-// device-memory atomics to the same addresses together: queue slot q holds
-// packet spread(q), lane l of wavefront w taking packet l * nw + w
-// (nw = ceil(n / 64)).  Slots with spread(q) >= n hold no packet; there are
-// fewer than 64.
-__device__ __forceinline__ uint32_t spread_waves(uint32_t n) { return (n + 63u) / 64u; }
-__device__ __forceinline__ uint32_t spread(uint32_t q, uint32_t nw) { return (q & 63u) * nw + (q >> 6); }
+// population control, emission is generated cell by cell).  Both transport
+// kernels keep that order: a block takes a contiguous range of the input, so
+// its visits fall in few cells, which its LDS tally cache exploits.
 
 // Block-local cache of the per-visit tallies in LDS.  Global atomics on
 // device memory are performed at the memory controller (every one a DRAM
@@ -757,6 +759,7 @@ struct TallyCache {
   uint32_t *segs;
   uint32_t *waves_done;
   uint32_t *next_chunk;   // k_transport's chunks handed out so far
+  uint32_t *staged;       // k_transport_refill: census packets staged by the block
   __device__ __forceinline__ unsigned long long &at(uint32_t f, uint32_t s) const { return v[f * (mask + 1u) + s]; }
 };
 
@@ -772,6 +775,7 @@ __device__ __forceinline__ TallyCache cache_init(unsigned char *lds, uint32_t sl
   L.segs = L.tag + slots;
   L.waves_done = L.segs + slots;
   L.next_chunk = L.waves_done + 1;
+  L.staged = L.waves_done + 2;
   for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
     L.tag[s] = 0u;
     L.segs[s] = 0u;
@@ -780,6 +784,7 @@ __device__ __forceinline__ TallyCache cache_init(unsigned char *lds, uint32_t sl
   if (threadIdx.x == 0) {
     *L.waves_done = 0u;
     *L.next_chunk = 0u;
+    *L.staged = 0u;
   }
   return L;
 }
@@ -857,14 +862,14 @@ struct DeviceSink {
     atomicAdd(&segs[c], n);
   }
   // Called by every wavefront of the block once its packets are done: the
-  // last one writes the cache to device memory.
-  __device__ void finish_block() {
-    if (!cache.v) return;
+  // last one writes the cache to device memory (and returns true).
+  __device__ bool finish_block() {
+    if (!cache.v) return false;
     uint32_t done = 0u;
     if (lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u)
       done = __hip_atomic_fetch_add(cache.waves_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     done = __builtin_amdgcn_readfirstlane(done);
-    if (done + 1u != (blockDim.x + 63u) / 64u) return;
+    if (done + 1u != (blockDim.x + 63u) / 64u) return false;
     for (uint32_t s = __lane_id(); s <= cache.mask; s += 64u) {
       const uint32_t key = __hip_atomic_load(&cache.tag[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (!key) continue;
@@ -880,6 +885,7 @@ struct DeviceSink {
       }
       if (cache.segs[s]) atomicAdd(&segs[c], cache.segs[s]);
     }
+    return true;
   }
   __device__ void escape(uint32_t c, double w) {
     if (t.esc_fp) {
@@ -984,72 +990,79 @@ k_transport(STORM_TRANSPORT_ARGS) {
   transport_chunks<Real, F, MeshT>(n_in, in, out, n_out, cen, n_cen, mesh, xs, acc, k, max_segments, t, stats,
                                    cache_slots, cache_fields, chunks);
 }
-#undef STORM_TRANSPORT_ARGS
 
-// Persistent transport with lane refill.  Each wavefront advances its 64
-// packets one segment at a time; a lane whose packet ended (census, absorbed,
-// escaped, or `max_segments` segments in this launch) takes the next packet
-// from its wavefront's pool of queued slots.  The pool is refilled with
-// kRefillChunk consecutive slots (packets spread(q), as in k_transport) by
-// one atomic on `*queue` (the count of handed-out slots) when it runs dry,
-// so the shared counter sees one atomic per chunk rather than one per lane
-// refill.  Lanes stay busy until the
-// queue is drained, instead of idling until the longest history of their
-// wavefront ends.  Per packet the result is that of k_transport with the
-// same `max_segments`.
-constexpr uint32_t kRefillChunk = 64;
-// stats: [0] escapes, [1] random-walk steps, [2] DDMC steps
-template <typename Real, uint32_t F>
-__global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketSoA<Real> in, PacketSoA<Real> out,
-                                                          uint32_t *__restrict__ n_out, CensusSoA<Real> cen,
-                                                          uint32_t *__restrict__ n_cen, MeshView<Real> mesh,
-                                                          const CellXS<Real> *__restrict__ xs, AccelView<Real> acc,
-                                                          StepConsts k, int32_t max_segments, Tallies t,
-                                                          unsigned long long *__restrict__ stats,
-                                                          uint32_t *__restrict__ queue) {
+// Block-local lane refill (the refill schedule).  The block owns the same
+// range of the cell-ordered input as k_transport and sums its visits in the
+// same LDS table, but each wavefront advances its 64 packets one segment at
+// a time: once at least kRefillIdle of its lanes are idle (their packet
+// ended or took `max_segments` segments), they take the next packets of the
+// block's range, 64 at a time from the block's LDS counter.  A lane then idles only until
+// enough others have finished, not until the longest history of its chunk
+// ends; a refill of several lanes costs one memory round trip for all of
+// them.  Per packet the result is that of k_transport with the same
+// `max_segments` (a history does not depend on the lane that runs it).
+// Census packets are staged in the block's own part of `stage` (its input
+// range: a packet ends in census at most once) through an LDS counter, and
+// the block's last wavefront reserves their census slots with one atomic and
+// copies them there.  Appending them one or two at a time as lanes finish
+// would put an atomic per census packet on the one census counter that every
+// wavefront of the launch shares.
+// Idle lanes that take new packets together: 1 to 8 measured within 1% of
+// each other, 16 and 32 up to 2% slower (cube, step slab, batch of slabs).
+constexpr uint32_t kRefillIdle = 4;
+template <typename Real, uint32_t F, typename MeshT>
+__device__ __forceinline__ void transport_refill(STORM_TRANSPORT_ARGS, CensusSoA<Real> stage) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const TallyCache cache = cache_init(lds, cache_slots, cache_fields);
+  __syncthreads();
   DeviceSink sink(t, k.eq);
-  Tracker<Real, F> tr;
+  sink.cache = cache;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * chunks * 64u;
+  const uint64_t lim = base + static_cast<uint64_t>(chunks) * 64u < n_in ? base + static_cast<uint64_t>(chunks) * 64u
+                                                                          : static_cast<uint64_t>(n_in);
+  const uint32_t span = static_cast<uint32_t>(lim - base);  // packets of this block
+  typename TrackerOf<Real, F, MeshT>::type tr;
   bool busy = false;
-  // the queue hands out slots q in [0, 64 nw); slot q holds packet spread(q)
-  const uint32_t nw = spread_waves(n_in);
-  const uint32_t nq = 64u * nw;
-  // uniform across the wavefront: the pool [pool_next, pool_end) of queued
-  // slots this wavefront owns, and whether the queue is exhausted
-  bool drained = false;
-  uint32_t pool_next = 0, pool_end = 0;
   int32_t nseg = 0;
-  uint32_t escapes = 0, longest = 0;
+  // wave-uniform: the chunk being handed out, [pool_next, pool_end) of the
+  // block's range, and whether the block's range is exhausted
+  uint32_t pool_next = 0, pool_end = 0;
+  bool drained = false;
+  uint32_t escaped = 0, longest = 0;
   for (;;) {
     if (!drained) {
       const uint64_t idle = __builtin_amdgcn_ballot_w64(!busy);
-      if (idle) {
-        if (pool_next >= pool_end) {
-          uint32_t base = 0u;
-          if (lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u) base = atomicAdd(queue, kRefillChunk);
-          base = __builtin_amdgcn_readfirstlane(base);
-          if (base >= nq) {
-            drained = true;
-          } else {
-            pool_next = base;
-            pool_end = base + kRefillChunk < nq ? base + kRefillChunk : nq;
+      const uint32_t nidle = static_cast<uint32_t>(__popcll(idle));
+      if (nidle >= kRefillIdle) {
+        const uint32_t r = lane_rank(idle);
+        uint32_t given = 0;
+        while (given < nidle) {
+          if (pool_next >= pool_end) {
+            uint32_t ch = 0;
+            if (lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u)
+              ch = __hip_atomic_fetch_add(cache.next_chunk, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            ch = __builtin_amdgcn_readfirstlane(ch);
+            if (ch >= chunks || ch * 64u >= span) {
+              drained = true;
+              break;
+            }
+            pool_next = ch * 64u;
+            pool_end = pool_next + 64u < span ? pool_next + 64u : span;
           }
-        }
-        if (!drained) {
           const uint32_t avail = pool_end - pool_next;
-          const uint32_t want = static_cast<uint32_t>(__popcll(idle));
-          const uint32_t r = lane_rank(idle);
-          const uint32_t pk = spread(pool_next + r, nw);
-          if (!busy && r < avail && pk < n_in) {
-            const Packet<Real> p = load_packet(in, pk);
+          const uint32_t take = nidle - given < avail ? nidle - given : avail;
+          if (!busy && r >= given && r < given + take) {
+            const Packet<Real> p = load_packet(in, static_cast<uint32_t>(base + pool_next + (r - given)));
             tr.begin(p, mesh, xs, k);
             busy = true;
             nseg = 0;
           }
-          pool_next += want < avail ? want : avail;
+          pool_next += take;
+          given += take;
         }
       }
     }
-    // every lane idle: the queue was drained on this or an earlier round
+    // every lane idle: the block's range was exhausted on this or an earlier round
     if (__builtin_amdgcn_ballot_w64(busy) == 0ull) break;
     int outcome = kAlive;
     if (busy) {
@@ -1064,20 +1077,45 @@ __global__ __launch_bounds__(256) void k_transport_refill(uint32_t n_in, PacketS
     }
     const bool census = fin && outcome == kCensus;
     const bool survivor = fin && outcome == kAlive;
-    escapes += (fin && outcome == kEscaped) ? 1u : 0u;
+    escaped += (fin && outcome == kEscaped) ? 1u : 0u;
     if (fin && p.ctr > longest) longest = p.ctr;
-    const uint32_t cslot = wave_append(census, n_cen);
-    if (census) store_census(cen, cslot, p);
     const uint32_t oslot = wave_append(survivor, n_out);
     if (survivor) store_packet(out, oslot, p);
+    const uint64_t want = __builtin_amdgcn_ballot_w64(census);
+    if (want) {
+      uint32_t first = 0u;
+      if (lane_rank(__builtin_amdgcn_ballot_w64(true)) == 0u)
+        first = __hip_atomic_fetch_add(cache.staged, static_cast<uint32_t>(__popcll(want)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+      first = __builtin_amdgcn_readfirstlane(first);
+      if (census) store_census(stage, static_cast<uint32_t>(base) + first + lane_rank(want), p);
+    }
   }
-  wave_add_u32(escapes, &stats[0]);
+  // the staged census becomes visible to the block's last wavefront
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  if (sink.finish_block()) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const uint32_t nst = __hip_atomic_load(cache.staged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    uint32_t g = 0u;
+    if (nst && __lane_id() == 0u) g = atomicAdd(n_cen, nst);
+    g = __builtin_amdgcn_readfirstlane(g);
+    for (uint32_t q = __lane_id(); q < nst; q += 64u) copy_census(stage, static_cast<uint32_t>(base) + q, cen, g + q);
+  }
+  wave_add_u32(escaped, &stats[0]);
   wave_max_u32(longest, &stats[4]);
   if (acc.cell) {
     wave_add_u32(sink.rw, &stats[1]);
     wave_add_u32(sink.ddmc, &stats[2]);
   }
 }
+
+template <typename Real, uint32_t F, typename MeshT = MeshView<Real>>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(TransportWaves<Real>::kMin, 8))) void
+k_transport_refill(STORM_TRANSPORT_ARGS, CensusSoA<Real> stage) {
+  transport_refill<Real, F, MeshT>(n_in, in, out, n_out, cen, n_cen, mesh, xs, acc, k, max_segments, t, stats,
+                                   cache_slots, cache_fields, chunks, stage);
+}
+#undef STORM_TRANSPORT_ARGS
 
 // ------------------------------------------------------------------ accelerations
 // Per-cell random-walk / DDMC data (precomputeRandomWalkData, RadiationIMC.hpp:982-1063;

@@ -72,10 +72,11 @@ class GpuEngine:
         self._check(self.lib.storm_gpu_set_source_temperature(self.h, float(T)))
 
     def set_tuning(self, block: int = 256, segments_per_pass: int = 1 << 20, schedule: int | None = None):
-        """Transport launch shape; `schedule` is capi.SCHEDULE_PASS (default) or capi.SCHEDULE_REFILL;
-        block 0 lets the engine pick the block size over the next six steps."""
+        """Transport launch shape; `schedule` is capi.SCHEDULE_PASS or capi.SCHEDULE_REFILL (None: the
+        engine's default, pass in the mixed engine and refill in the FP64 one); block 0 tunes the block
+        size, the chunks per wavefront and the schedule on line over the next 16 steps."""
         if schedule is None:
-            schedule = capi.SCHEDULE_PASS
+            schedule = capi.SCHEDULE_REFILL if self.precision == capi.FP64 else capi.SCHEDULE_PASS
         self._check(self.lib.storm_gpu_set_tuning(self.h, int(block), int(segments_per_pass), int(schedule)))
 
     def set_cell_state(self, T=None, sigma_a=None, sigma_s=None, cv=None, e=None):

@@ -377,11 +377,16 @@ def _run_tallies(spec, precision, **tuning):
         return e.cells(), e.tallies(), e.counters()
 
 
-@pytest.mark.parametrize("make", [C2, C3], ids=["c2", "c3"])
+def _voronoi_mg(s):
+    return P.voronoi_slab(seed=s, groups=8)
+
+
+@pytest.mark.parametrize("make", [C2, C3, P.voronoi_slab, _voronoi_mg], ids=["c2", "c3", "voronoi", "voronoi-8g"])
 def test_refill_schedule_matches_pass_schedule(make):
-    """The persistent refill transport and the one-packet-per-thread pass
-    give bit-identical tallies and cell states: a packet's history depends
-    only on its id, the step and the segment budget, never on the lane."""
+    """The lane-refill transport and the one-packet-per-lane pass give
+    bit-identical tallies and cell states in the mixed engine (Cartesian and
+    polyhedral meshes, gray and multigroup): a packet's history depends only
+    on its id, the step and the segment budget, never on the lane."""
     spec = make(12)
     a = _run_tallies(spec, capi.MIXED, schedule=capi.SCHEDULE_PASS)
     b = _run_tallies(spec, capi.MIXED, schedule=capi.SCHEDULE_REFILL, block=128)
@@ -390,6 +395,25 @@ def test_refill_schedule_matches_pass_schedule(make):
     for k in ("dep", "rdep", "pint", "esc", "segs"):
         assert np.array_equal(a[1][k], b[1][k]), k
     assert a[2]["segments"] == b[2]["segments"]
+    assert a[2]["end_particles"] == b[2]["end_particles"]
+
+
+@pytest.mark.parametrize("make", [C2, P.voronoi_slab], ids=["c2", "voronoi"])
+def test_fp64_refill_default_matches_pass(make):
+    """The FP64 engine refills lanes by default; the pass schedule gives the
+    same histories, so the same segment counts and census, and tallies equal
+    up to the order of their floating-point sums (one step: over several, a
+    last-bit difference of the material state may send a packet another way)."""
+    spec = make(12)
+    spec.steps = 1
+    with GpuEngine(spec, capi.FP64) as e:
+        assert e.launch()["schedule"] == capi.SCHEDULE_REFILL
+    a = _run_tallies(spec, capi.FP64, schedule=capi.SCHEDULE_PASS)
+    b = _run_tallies(spec, capi.FP64)
+    assert np.array_equal(a[1]["segs"], b[1]["segs"])
+    assert a[2]["segments"] == b[2]["segments"] and a[2]["end_particles"] == b[2]["end_particles"]
+    for k in ("T", "e", "erad", "census"):
+        assert np.allclose(a[0][k], b[0][k], rtol=1e-9, atol=0.0), k
 
 
 def _tuned_against_fixed(spec, steps):
@@ -413,18 +437,17 @@ def _tuned_against_fixed(spec, steps):
 def test_online_launch_tuning_keeps_results(mesh):
     """Block size 0: the transport's launch is tuned on line -- blocks of
     128, 256, 512, 512, 256, 128 threads, then 1, 2, 4, 4, 2, 1 chunks per
-    wavefront, then (Cartesian meshes) the pass and refill schedules -- and
-    the best is kept; the cell states and tallies are those of a fixed
-    launch, step by step."""
+    wavefront, then the pass and refill schedules -- and the best is kept;
+    the cell states and tallies are those of a fixed launch, step by step."""
     if mesh == "cartesian":
-        spec, steps, nphase = C2(13), 18, 16
+        spec, steps = C2(13), 18
     else:
-        spec, steps, nphase = P.voronoi_slab(seed=13, steps=14), 14, 12
+        spec, steps = P.voronoi_slab(seed=13, steps=18), 18
+    nphase = 16
     fixed, tuned, shapes = _tuned_against_fixed(spec, steps)
     assert [b for b, _, _, _ in shapes[:6]] == [128, 256, 512, 512, 256, 128], shapes
     assert [k for _, k, _, _ in shapes[6:12]] == [1, 2, 4, 4, 2, 1], shapes
-    if mesh == "cartesian":
-        assert [s for _, _, s, _ in shapes[12:16]] == [0, 1, 1, 0], shapes
+    assert [s for _, _, s, _ in shapes[12:16]] == [0, 1, 1, 0], shapes
     # the choice after tuning is one of the candidates and stays
     assert all(t for _, _, _, t in shapes[:nphase - 1]) and not any(t for _, _, _, t in shapes[nphase - 1:]), shapes
     assert len(set(x[:3] for x in shapes[nphase:])) == 1, shapes

@@ -12,7 +12,12 @@ with facts the mathematics fixes, independent of the eigen series:
     variance 2 tau per axis, at large tau the first eigenmode's
     (sin(pi x) - pi x cos(pi x)) / pi, and the two formulas the reference
     switches between at tau = 0.01 agree there;
-  * the engine's FP32 tables equal the oracle's to FP32 rounding.
+  * the engine's FP32 tables equal the oracle's to FP32 rounding;
+  * the engine's inverse leak-time table and sampler (both precisions) against
+    the dual (image) series of the same survival, obtained by Poisson
+    summation and free of its cancellation at small tau:
+        1 - S(tau) = 2 / sqrt(pi tau) sum_{k>=0} exp(-(2k+1)^2 / (4 tau)),
+    over every float uniform the engine draws near both ends of (0, 1].
 A dropped alternating sign, a wrong pi^2 factor or a transposed table fails
 one of these.
 """
@@ -115,3 +120,49 @@ def test_engine_tables_equal_oracle_tables(tables):
     assert np.allclose(et, tau, rtol=2e-7)
     assert np.allclose(es, S, rtol=2e-7, atol=1e-37)
     assert np.allclose(er, rad, atol=2e-7)
+
+
+def _one_minus_survival(T):
+    T = np.asarray(T, dtype=float)[..., None]
+    k = np.arange(40)
+    return (2.0 / np.sqrt(np.pi * T[..., 0])) * np.exp(-(2 * k + 1) ** 2 / (4.0 * T)).sum(-1)
+
+
+def _exact_leak_time(xi):
+    # -ln S(T) = y, solved by bisection in ln T on the image series
+    y = -np.log1p(-np.asarray(xi, dtype=float))
+    lo, hi = np.full_like(y, np.log(1e-3)), np.full_like(y, np.log(8.0))
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        above = -np.log1p(-_one_minus_survival(np.exp(mid))) < y
+        lo, hi = np.where(above, mid, lo), np.where(above, hi, mid)
+    return np.exp(0.5 * (lo + hi))
+
+
+def test_engine_leak_table_is_the_inverse_of_survival():
+    leak = pyoracle.engine_rw_leak_table()
+    y = np.exp(-25.0 + 26.5 * np.arange(2049) / 2048)
+    got = -np.log1p(-_one_minus_survival(leak))
+    assert np.all(np.diff(leak) > 0)
+    # below y ~ 1e-9 the eigen series' rounding (1e-16 absolute in S) is
+    # the limit; above it the roots are exact
+    big = y > 1e-9
+    assert np.allclose(got[big], y[big], rtol=1e-8)
+    assert np.allclose(got, y, rtol=1e-4)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["float", "double"])
+def test_engine_leak_sampler_inverts_survival(precision):
+    # the engine's float uniforms (k + 1/2) 2^-24 rounded to float, at both
+    # ends of (0, 1) and spread between; (0, 1] includes 1 by the rounding
+    k = np.concatenate([np.arange(2000), np.geomspace(1, 2**24 - 1, 3000).astype(np.int64), 2**24 - 1 - np.arange(2000)])
+    xi = ((k.astype(np.float32) + np.float32(0.5)) * np.float32(2.0**-24)).astype(np.float64)
+    top = xi >= 1.0
+    leak, _ = pyoracle.engine_rw_sample(precision, xi, np.full_like(xi, 0.1))
+    assert np.all(leak[top] == 64.0)
+    exact = _exact_leak_time(xi[~top])
+    assert np.allclose(leak[~top], exact, rtol=3e-5)
+    # past the table: the first eigenmode, T = (y + ln 2) / pi^2
+    far = xi[~top] > 1.0 - np.exp(-5.0)
+    assert far.sum() > 100
+    assert np.allclose(leak[~top][far], (-np.log1p(-xi[~top][far]) + np.log(2.0)) / np.pi**2, rtol=3e-6)
