@@ -253,7 +253,9 @@ int storm_gpu_get_launch(storm_gpu_engine *eng, int32_t *block_size, int32_t *ch
 
 /* Keep a copy of the next step's transport inputs (initial packets, cell
  * opacities, step constants) and its integer tallies, for an exact CPU replay
- * of the packet random streams.  Sizes first with NULL buffers.  After a step
+ * of the packet random streams.  Sizes first with NULL buffers.  cell_xs
+ * receives two numbers per cell in the tracking type (f sigma_a and the
+ * effective-scatter opacity).  After a step
  * fails because a packet exceeded the segment limit (error 6), the kept
  * packets are the (first 4096) packets still in flight instead. */
 int storm_gpu_debug_keep_transport_inputs(storm_gpu_engine *eng, int32_t enable);

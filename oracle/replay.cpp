@@ -78,6 +78,26 @@ struct HostTables {
   }
 };
 
+// The engine's device layouts from the replay's plain inputs: widths
+// (nx + ny + nz) into interleaved (w, 1/w) pairs per axis, and per-cell
+// (sig_abs, sig_ev) pairs into CellXS with the derived reciprocal.
+struct HostMesh {
+  std::vector<float> wp;
+  std::vector<CellXS<float>> xs;
+  HostMesh(int32_t nx, int32_t ny, int32_t nz, const float *widths, const float *xs2) {
+    wp.resize(2 * static_cast<size_t>(nx + ny + nz));
+    width_pairs<float>(widths, nx + ny + nz, wp.data());
+    const size_t N = static_cast<size_t>(nx) * ny * nz;
+    xs.resize(N);
+    for (size_t c = 0; c < N; ++c) xs[c] = make_xs<float>(xs2[2 * c], xs2[2 * c + 1]);
+  }
+  void view(MeshView<float> &mv) const {
+    mv.width[0] = wp.data();
+    mv.width[1] = wp.data() + 2 * mv.nx;
+    mv.width[2] = wp.data() + 2 * (mv.nx + mv.ny);
+  }
+};
+
 template <typename Real>
 AccelView<Real> make_accel_view(const CellAccel<Real> *cells, const HostTables<Real> *tb, int rw_on, int ddmc_on,
                                 double rw_ptau, double ddmc_ptau) {
@@ -382,9 +402,8 @@ int replay_transport_f32(int64_t n, const void *packets, int32_t nx, int32_t ny,
     mv.escape_mask = 0u;
     for (int f = 0; f < 6; ++f)
       if (face_kind[f] == STORM_FACE_ESCAPE) mv.escape_mask |= 1u << f;
-    mv.width[0] = widths;
-    mv.width[1] = widths + nx;
-    mv.width[2] = widths + nx + ny;
+    const HostMesh hm(nx, ny, nz, widths, xs);
+    hm.view(mv);
     mv.spec = SpectrumView<float>{};
     mv.spec.groups = groups;
     if (groups >= 2) {
@@ -412,7 +431,7 @@ int replay_transport_f32(int64_t n, const void *packets, int32_t nx, int32_t ny,
     for (auto &p : pk) {
       int o = kAlive;
       while (o == kAlive)
-        o = transport_packet<float>(p, mv, reinterpret_cast<const CellXS<float> *>(xs), av, kc, K, sink);
+        o = transport_packet<float>(p, mv, hm.xs.data(), av, kc, K, sink);
       counts[o] += 1;
     }
     counts[4] += static_cast<int64_t>(sink.rw);
@@ -439,9 +458,8 @@ int64_t replay_trace_f32(const void *packet, int32_t nx, int32_t ny, int32_t nz,
     mv.escape_mask = 0u;
     for (int f = 0; f < 6; ++f)
       if (face_kind[f] == STORM_FACE_ESCAPE) mv.escape_mask |= 1u << f;
-    mv.width[0] = widths;
-    mv.width[1] = widths + nx;
-    mv.width[2] = widths + nx + ny;
+    const HostMesh hm(nx, ny, nz, widths, xs);
+    hm.view(mv);
     mv.spec = SpectrumView<float>{};
     StepConsts kc;
     kc.seed = static_cast<uint64_t>(consts[0]);
@@ -461,10 +479,10 @@ int64_t replay_trace_f32(const void *packet, int32_t nx, int32_t ny, int32_t nz,
     std::vector<uint32_t> sg(N);
     HostSink sink{d.data(), r.data(), q.data(), e.data(), nullptr, nullptr, nullptr, sg.data(), kc.eq};
     Tracker<float> t;
-    t.begin(p, mv, reinterpret_cast<const CellXS<float> *>(xs), kc);
+    t.begin(p, mv, hm.xs.data(), kc);
     int64_t k = 0;
     while (k < max_seg && t.outcome == kAlive) {
-      t.segment(mv, reinterpret_cast<const CellXS<float> *>(xs), av, kc, sink);
+      t.segment(mv, hm.xs.data(), av, kc, sink);
       double *o = trace + 12 * k;
       o[0] = t.u0; o[1] = t.u1; o[2] = t.u2;
       o[3] = t.o0; o[4] = t.o1; o[5] = t.o2;
@@ -606,14 +624,17 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       mv.escape_mask = 0u;
       for (int q = 0; q < 6; ++q)
         if (p->boundary.face_kind[q] == STORM_FACE_ESCAPE) mv.escape_mask |= 1u << q;
-      mv.width[0] = w0.data();
-      mv.width[1] = w1.data();
-      mv.width[2] = w2.data();
+      std::vector<Real> p0(2 * nx), p1(2 * ny), p2(2 * nz);
+      width_pairs<Real>(w0.data(), nx, p0.data());
+      width_pairs<Real>(w1.data(), ny, p1.data());
+      width_pairs<Real>(w2.data(), nz, p2.data());
+      mv.width[0] = p0.data();
+      mv.width[1] = p1.data();
+      mv.width[2] = p2.data();
       std::vector<CellXS<Real>> xs(N);
-      for (uint32_t c = 0; c < N; ++c) {
-        xs[c].sig_abs = static_cast<Real>(f[c] * s1[c]);
-        xs[c].sig_ev = G ? static_cast<Real>((1.0 - f[c]) * s1[c]) : static_cast<Real>(ss[c] + (1.0 - f[c]) * sa[c]);
-      }
+      for (uint32_t c = 0; c < N; ++c)
+        xs[c] = make_xs<Real>(static_cast<Real>(f[c] * s1[c]),
+                              G ? static_cast<Real>((1.0 - f[c]) * s1[c]) : static_cast<Real>(ss[c] + (1.0 - f[c]) * sa[c]));
       std::vector<Real> cdfr(cdfd.begin(), cdfd.end()), ssr(ss.begin(), ss.end()), bk(G ? G + 1 : 0), bcdf;
       for (int32_t g = 0; G && g <= G; ++g) bk[g] = static_cast<Real>(gb[g] / units::kev);
       mv.spec = SpectrumView<Real>{};

@@ -109,9 +109,22 @@ template <typename Real>
 struct MeshView {
   int32_t nx, ny, nz;
   uint32_t escape_mask;  // bit f set: box face f removes the packet (else it reflects)
-  const Real *width[3];  // width[a][idx] for a = x, y, z
+  // per axis a = x, y, z the cell widths and their reciprocals, interleaved:
+  // width[a][2 i] = w_i, width[a][2 i + 1] = 1 / w_i (width_pairs)
+  const Real *width[3];
   SpectrumView<Real> spec;
 };
+
+// Host: widths w[0..n) and their reciprocals, interleaved into out[0..2n).
+// The reciprocal is the one IEEE division a segment would otherwise do on
+// entering the cell, so the tracking is unchanged bit for bit.
+template <typename Real>
+inline void width_pairs(const Real *w, int n, Real *out) {
+  for (int i = 0; i < n; ++i) {
+    out[2 * i] = w[i];
+    out[2 * i + 1] = Real(1) / w[i];
+  }
+}
 
 template <typename Real>
 STORM_HD uint32_t flat_cell(const MeshView<Real> &m, uint32_t ijk) {
@@ -121,10 +134,20 @@ STORM_HD uint32_t flat_cell(const MeshView<Real> &m, uint32_t ijk) {
 // Per-cell opacities used by a segment (cm^-1):
 //   sig_abs = f * sigma_a              continuous absorption (weight attenuation)
 //   sig_ev  = sigma_s + (1-f)*sigma_a  effective-scatter event opacity
+//   inv_ev  = 1 / sig_ev (0 where sig_ev is 0), for the event distance
 template <typename Real>
 struct CellXS {
-  Real sig_abs, sig_ev;
+  Real sig_abs, sig_ev, inv_ev;
 };
+
+template <typename Real>
+STORM_HD CellXS<Real> make_xs(Real sig_abs, Real sig_ev) {
+  CellXS<Real> x;
+  x.sig_abs = sig_abs;
+  x.sig_ev = sig_ev;
+  x.inv_ev = sig_ev > Real(0) ? Real(1) / sig_ev : Real(0);
+  return x;
+}
 
 // Per-cell data of the diffusion accelerations:
 //   flags   bit 0 random-walk cell, bit 1 DDMC cell
@@ -427,7 +450,7 @@ struct Tracker {
   // `nijk`/`ncell` its index, `nw` its width along `fax`, `nx`/`nsig_s` its
   // opacities.
   template <typename Sink>
-  STORM_HD void enter_loaded_neighbour(int fax, bool up, uint32_t nijk, uint32_t ncell, Real nw,
+  STORM_HD void enter_loaded_neighbour(int fax, bool up, uint32_t nijk, uint32_t ncell, Real nw, Real niw,
                                        const CellXS<Real> &nx, Real nsig_s, const MeshView<Real> &m,
                                        const StepConsts &k, Sink &sink) {
     double wq = fixed_point ? fx::carry_round(w, w_enter, k.eq, id, ctr, cell) : w;
@@ -438,18 +461,21 @@ struct Tracker {
     cell = ncell;
     put3<Real>(fax, u0, u1, u2, up ? Real(0) : Real(1));
     put3<Real>(fax, cw0, cw1, cw2, nw);
-    put3<Real>(fax, icw0, icw1, icw2, Real(1) / nw);
+    put3<Real>(fax, icw0, icw1, icw2, niw);
     set_opacity(m, nx, nsig_s);
   }
 
   STORM_HD void load_cell(const MeshView<Real> &m, const CellXS<Real> *xs) {
     cell = flat_cell(m, ijk);
-    cw0 = m.width[0][ijk_i(ijk)];
-    cw1 = m.width[1][ijk_j(ijk)];
-    cw2 = m.width[2][ijk_k(ijk)];
-    icw0 = Real(1) / cw0;
-    icw1 = Real(1) / cw1;
-    icw2 = Real(1) / cw2;
+    const Real *p0 = m.width[0] + 2u * ijk_i(ijk);
+    const Real *p1 = m.width[1] + 2u * ijk_j(ijk);
+    const Real *p2 = m.width[2] + 2u * ijk_k(ijk);
+    cw0 = p0[0];
+    icw0 = p0[1];
+    cw1 = p1[0];
+    icw1 = p1[1];
+    cw2 = p2[0];
+    icw2 = p2[1];
     load_opacity(m, xs);
   }
 
@@ -473,6 +499,7 @@ struct Tracker {
     cell_sig_abs = x.sig_abs;
     cell_sig_ev = x.sig_ev;
     cell_sig_s = Real(0);
+    inv_sig_ev = x.inv_ev;
     if ((F & kFeatMultigroup) != 0u && m.spec.groups >= 2) {
       // clampFrequencyToBounds (RadiationIMC.hpp:868-872) and E^-nu_pow
       Real e = nu < m.spec.e_lo ? m.spec.e_lo : (nu > m.spec.e_hi ? m.spec.e_hi : nu);
@@ -481,8 +508,8 @@ struct Tracker {
       cell_sig_abs = x.sig_abs * fac;
       cell_sig_ev = cell_sig_s + x.sig_ev * fac;
       if (eg_track) grp = group_of<Real>(e, m.spec.bounds, m.spec.groups);
+      inv_sig_ev = cell_sig_ev > Real(0) ? Real(1) / cell_sig_ev : Real(0);
     }
-    inv_sig_ev = cell_sig_ev > Real(0) ? Real(1) / cell_sig_ev : Real(0);
     derive_pint = derives_path_integral(m);
   }
 
@@ -753,7 +780,8 @@ struct Tracker {
     const uint32_t ncell = outside ? cell : (up ? cell + cstride : cell - cstride);
     const uint32_t nijk = outside ? ijk : (up ? ijk + istride : ijk - istride);
     const CellXS<Real> nxs = xs[ncell];
-    const Real nw = sel3<const Real *>(fax, m.width[0], m.width[1], m.width[2])[nidx];
+    const Real *nwp = sel3<const Real *>(fax, m.width[0], m.width[1], m.width[2]) + 2u * nidx;
+    const Real nw = nwp[0], niw = nwp[1];
     Real nsig_s = Real(0);
     if ((F & kFeatMultigroup) != 0u && m.spec.groups >= 2) nsig_s = m.spec.sig_s[ncell];
 
@@ -804,7 +832,7 @@ struct Tracker {
     if (ev == 0) {
       if (!outside) {
         // interior face: the packet enters the neighbour on the opposite face
-        enter_loaded_neighbour(fax, up, nijk, ncell, nw, nxs, nsig_s, m, k, sink);
+        enter_loaded_neighbour(fax, up, nijk, ncell, nw, niw, nxs, nsig_s, m, k, sink);
       } else {
         const int face = 2 * fax + (up ? 1 : 0);
         put3<Real>(fax, u0, u1, u2, up ? Real(1) : Real(0));

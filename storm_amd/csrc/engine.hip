@@ -601,12 +601,13 @@ struct Engine final : EngineBase {
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&ev0));
     HIPCHECK(hipEventCreate(&ev1));
-    // mesh widths
+    // mesh widths and their reciprocals, interleaved (MeshView::width)
     auto up_w = [&](DevBuf<Real> &d, const std::vector<double> &e) {
-      std::vector<Real> w(e.size() - 1);
+      std::vector<Real> w(e.size() - 1), wp(2 * (e.size() - 1));
       for (size_t q = 0; q + 1 < e.size(); ++q) w[q] = static_cast<Real>(e[q + 1] - e[q]);
-      d.ensure(w.size());
-      HIPCHECK(hipMemcpy(d.p, w.data(), w.size() * sizeof(Real), hipMemcpyHostToDevice));
+      width_pairs<Real>(w.data(), static_cast<int>(w.size()), wp.data());
+      d.ensure(wp.size());
+      HIPCHECK(hipMemcpy(d.p, wp.data(), wp.size() * sizeof(Real), hipMemcpyHostToDevice));
     };
     up_w(wx, xe);
     up_w(wy, ye);
@@ -1588,7 +1589,14 @@ struct Engine final : EngineBase {
   void debug_inputs(uint64_t *n, void *packets, void *cell_xs, int64_t *consts4) override {
     *n = kept_packets.size();
     if (packets) std::memcpy(packets, kept_packets.data(), kept_packets.size() * sizeof(Packet<Real>));
-    if (cell_xs) std::memcpy(cell_xs, kept_xs.data(), kept_xs.size() * sizeof(CellXS<Real>));
+    if (cell_xs) {
+      // two numbers per cell (sig_abs, sig_ev); the reciprocal is derived
+      Real *o = static_cast<Real *>(cell_xs);
+      for (size_t c = 0; c < kept_xs.size(); ++c) {
+        o[2 * c] = kept_xs[c].sig_abs;
+        o[2 * c + 1] = kept_xs[c].sig_ev;
+      }
+    }
     if (consts4) {
       consts4[0] = static_cast<int64_t>(kept_consts.seed);
       consts4[1] = kept_consts.step;

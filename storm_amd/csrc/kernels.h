@@ -297,10 +297,7 @@ __global__ __launch_bounds__(256) void k_prestep(uint32_t N, const double *__res
   sig_a_out[c] = sa;
   sig_s_out[c] = ss;
   Eemit[c] = f * V[c] * units::arad * (T3 * Tc) * sa * dt * units::clight;
-  CellXS<Real> x;
-  x.sig_abs = static_cast<Real>(f * sa);
-  x.sig_ev = static_cast<Real>(ss + (1.0 - f) * sa);
-  xs[c] = x;
+  xs[c] = make_xs<Real>(static_cast<Real>(f * sa), static_cast<Real>(ss + (1.0 - f) * sa));
 }
 
 // Multigroup preStep per cell (storm_group_desc): the Planck opacity
@@ -391,10 +388,7 @@ __global__ __launch_bounds__(256) void k_prestep_mg(
   sig_s_out[c] = ss;
   Eemit[c] = f * V[c] * units::arad * (T3 * Tc) * sP * dt * units::clight;
   const double s1 = sa_gray * pow(units::kev / nu_ref, -nu_pow);  // sigma_a at 1 keV
-  CellXS<Real> x;
-  x.sig_abs = static_cast<Real>(f * s1);
-  x.sig_ev = static_cast<Real>((1.0 - f) * s1);
-  xs[c] = x;
+  xs[c] = make_xs<Real>(static_cast<Real>(f * s1), static_cast<Real>((1.0 - f) * s1));
   sig_s_cell[c] = static_cast<Real>(ss);
   if (rw_out) {
     CellAccel<Real> r;
@@ -1193,7 +1187,7 @@ __global__ __launch_bounds__(256) void k_accel_rates(uint32_t N, MeshDims d, Mes
     const double D2 = Dtil[nb];
     double Df = D1;
     if (D2 > 0.0) Df = (D1 > 0.0 && D2 > 0.0) ? 2.0 * D1 * D2 / (D1 + D2) : fmax(D1, D2);
-    const double w = mv.width[ax][idx[ax]];
+    const double w = mv.width[ax][2u * idx[ax]];
     const double rate = Df / (w * (0.5 * w));  // A / V = 1 / w, face distance w / 2
     if (rate > 0.0 && isfinite(rate)) {
       a.rate[f] = static_cast<Real>(rate);
