@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define STORM_GPU_ABI_VERSION 7
+#define STORM_GPU_ABI_VERSION 8
 
 typedef struct storm_gpu_engine storm_gpu_engine;
 
@@ -83,6 +83,8 @@ typedef struct storm_gpu_counters {
   double seconds_transport_max_launch; /* longest single transport launch              */
   uint64_t max_packet_blocks;  /* most random-number blocks one packet used this step
                                   (about its segments): the longest history */
+  int32_t transport_chunks;    /* chunks per wavefront of the last step's transport */
+  int32_t transport_schedule;  /* its schedule (STORM_GPU_SCHEDULE_*)               */
 } storm_gpu_counters;
 
 /* Exact energy ledger of the last step, in units of its quantum 2^eq erg
@@ -218,31 +220,40 @@ int storm_gpu_import_census(storm_gpu_engine *eng, const storm_packet_record *in
 int storm_gpu_queue_source_packets(storm_gpu_engine *eng, const storm_packet_record *in, const double *time_left,
                                    uint64_t n);
 
-/* Launch parameters of the transport: block size (64, 128, 256, 512; the
- * default 256; 0: chosen on line, below), segments per packet and launch,
- * and the schedule.  PASS runs each lane's packet to its end (or the segment
- * budget) before the wavefront takes the next 64; REFILL advances the
- * wavefront's packets one segment at a time and hands lanes whose packet
- * ended the next packets of the block's range, four or more lanes at a time.
- * Default: PASS in the mixed engine, REFILL in the FP64 engine.  A packet's
- * history does not depend on the schedule, the block size or the chunks per
- * wavefront, only on segments_per_pass: the mixed engine's results are
- * bit-identical across them, the FP64 engine's tallies differ only in the
- * order of their floating-point sums.  With block size 0 the engine tunes
- * the launch on line, one parameter at a time: the block size (128, 256,
- * 512), then the chunks per wavefront (1, 2, 4), then the schedule; each
- * candidate runs one step forwards and one backwards (an order that cancels a
- * steady trend of the workload) and the one with the highest mean rate of
- * transport segments per second is kept (16 steps; storm_gpu_get_launch shows
- * the choice).  The results are those of any fixed launch.  No counterpart in
- * the reference, whose manager has no launch to tune. */
+/* Launch parameters of the transport: block size (64, 128, 256, 512; 0:
+ * tuned on line, below), segments per packet and launch, and the schedule
+ * (-1: keep the engine's).  PASS runs each lane's packet to its end (or the
+ * segment budget) before the wavefront takes the next 64 of the block's
+ * range; REFILL advances the wavefront's packets one segment at a time and
+ * hands lanes whose packet ended the next packets of the block's range, four
+ * or more lanes at a time.  A block's range is block size x chunks per
+ * wavefront (storm_gpu_set_chunks_per_wave; each schedule keeps its own,
+ * 2 for PASS and 8 for REFILL by default, 4 for REFILL in the mixed engine on
+ * polyhedral meshes).  The engine starts in REFILL, except the mixed engine
+ * on polyhedral meshes (PASS), and watches: every 8 steps one step runs in
+ * the other schedule and the engine switches when that one moved more
+ * transport segments per second than the mean of the steps either side of
+ * it, by more than 3%.  A block size other than 0 fixes the launch and stops
+ * the watch.  A packet's history does not depend on the schedule, the block
+ * size or the chunks per wavefront, only on segments_per_pass: the mixed
+ * engine's results are bit-identical across them, the FP64 engine's tallies
+ * differ only in the order of their floating-point sums.  With block size 0
+ * the engine tunes the launch on line, one parameter at a time: the schedule
+ * (each with its chunks), the block size (128, 256, 512), then the chunks per
+ * wavefront (1, 2, 4, 8, 16); each candidate runs one step forwards and one
+ * backwards (an order that cancels a steady trend of the workload) and the
+ * one with the highest mean rate of transport segments per second is kept
+ * (20 steps; storm_gpu_get_launch shows the choice), after which the watch
+ * runs.  The results are those of any fixed launch.  No counterpart in the
+ * reference, whose manager has no launch to tune. */
 #define STORM_GPU_SCHEDULE_PASS 0
 #define STORM_GPU_SCHEDULE_REFILL 1
 int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segments_per_pass, int32_t schedule);
 
 /* 64-packet chunks of the cell-ordered input per wavefront that a transport
- * block owns and hands out (1, 2 or 4; default 2): the block's share of the
- * input and of its LDS tally table.  No counterpart in the reference. */
+ * block owns and hands out in the current schedule (1, 2, 4, 8 or 16): the
+ * block's share of the input and of its LDS tally table.  No counterpart in
+ * the reference. */
 int storm_gpu_set_chunks_per_wave(storm_gpu_engine *eng, int32_t chunks);
 
 /* The transport launch in use: block size, chunks per wavefront, schedule,

@@ -48,15 +48,12 @@ struct HostSink {
     pint_fp[c] += e;
     segs[c] += s;
   }
-  void escape(uint32_t c, double w) {
-    if (esc_fp) {
-      esc_fp[c] += w;
-    } else {
-      fx::Limbs l = fx::limbs_of(w, eq);
-      esc[2 * c] += l.lo;
-      esc[2 * c + 1] += l.hi;
-    }
+  void escape_fx(uint32_t c, double w) {
+    fx::Limbs l = fx::limbs_of(w, eq);
+    esc[2 * c] += l.lo;
+    esc[2 * c + 1] += l.hi;
   }
+  void escape_fp(uint32_t c, double w) { esc_fp[c] += w; }
   void rw_step() { ++rw; }
   void ddmc_step() { ++ddmc; }
   // the replay does not check the per-group tally (the problems it replays
@@ -67,14 +64,11 @@ struct HostSink {
 // Random-walk tables in the engine's precision (same host build as the engine).
 template <typename Real>
 struct HostTables {
-  std::vector<Real> tau, leak, rad;
+  std::vector<Real> all;  // rw::tables_at's layout
   HostTables() {
-    std::vector<double> t(rw::kTableSize), s(rw::kTableSize), r(rw::kTableSize * rw::kRadiusSize), l(rw::kLeakSize);
-    rw::build_tables(t.data(), s.data(), r.data());
-    rw::build_leak_table(l.data());
-    tau.assign(t.begin(), t.end());
-    leak.assign(l.begin(), l.end());
-    rad.assign(r.begin(), r.end());
+    std::vector<double> a(rw::kAllSize);
+    rw::build_all_tables(a.data());
+    all.assign(a.begin(), a.end());
   }
 };
 
@@ -91,11 +85,7 @@ struct HostMesh {
     xs.resize(N);
     for (size_t c = 0; c < N; ++c) xs[c] = make_xs<float>(xs2[2 * c], xs2[2 * c + 1]);
   }
-  void view(MeshView<float> &mv) const {
-    mv.width[0] = wp.data();
-    mv.width[1] = wp.data() + 2 * mv.nx;
-    mv.width[2] = wp.data() + 2 * (mv.nx + mv.ny);
-  }
+  void view(MeshView<float> &mv) const { mv.width = wp.data(); }
 };
 
 template <typename Real>
@@ -103,11 +93,7 @@ AccelView<Real> make_accel_view(const CellAccel<Real> *cells, const HostTables<R
                                 double rw_ptau, double ddmc_ptau) {
   AccelView<Real> a;
   a.cell = cells;
-  a.rw_tau = tb ? tb->tau.data() : nullptr;
-  a.rw_leak = tb ? tb->leak.data() : nullptr;
-  a.rw_radius = tb ? tb->rad.data() : nullptr;
-  a.rw_log_tau_min = static_cast<Real>(std::log(rw::kTauMin));
-  a.rw_log_tau_max = static_cast<Real>(std::log(rw::kTauMax));
+  a.rw = tb ? tb->all.data() : nullptr;
   a.rw_particle_tau = rw_on ? static_cast<Real>(rw_ptau) : MathPolicy<Real>::inf();
   a.ddmc_particle_tau = ddmc_on ? static_cast<Real>(ddmc_ptau) : MathPolicy<Real>::inf();
   return a;
@@ -370,8 +356,7 @@ void replay_rw_sample(int32_t precision, int64_t n, const double *xi, const doub
   auto go = [&](auto tag) {
     using Real = decltype(tag);
     HostTables<Real> tb;
-    rw::Tables<Real> t{tb.tau.data(), tb.leak.data(), tb.rad.data(), static_cast<Real>(std::log(rw::kTauMin)),
-                       static_cast<Real>(std::log(rw::kTauMax))};
+    const rw::Tables<Real> t = rw::tables_at(tb.all.data());
     for (int64_t i = 0; i < n; ++i) {
       leak[i] = static_cast<double>(rw::sample_leak_time<Real>(t, static_cast<Real>(xi[i])));
       radius[i] = static_cast<double>(rw::sample_radius<Real>(t, static_cast<Real>(tau[i]), static_cast<Real>(xi[i])));
@@ -624,13 +609,11 @@ int cpu_first_step(const storm_problem *p, double dt, int32_t precision, uint32_
       mv.escape_mask = 0u;
       for (int q = 0; q < 6; ++q)
         if (p->boundary.face_kind[q] == STORM_FACE_ESCAPE) mv.escape_mask |= 1u << q;
-      std::vector<Real> p0(2 * nx), p1(2 * ny), p2(2 * nz);
-      width_pairs<Real>(w0.data(), nx, p0.data());
-      width_pairs<Real>(w1.data(), ny, p1.data());
-      width_pairs<Real>(w2.data(), nz, p2.data());
-      mv.width[0] = p0.data();
-      mv.width[1] = p1.data();
-      mv.width[2] = p2.data();
+      std::vector<Real> wp(2 * static_cast<size_t>(nx + ny + nz));
+      width_pairs<Real>(w0.data(), nx, wp.data());
+      width_pairs<Real>(w1.data(), ny, wp.data() + 2 * nx);
+      width_pairs<Real>(w2.data(), nz, wp.data() + 2 * (nx + ny));
+      mv.width = wp.data();
       std::vector<CellXS<Real>> xs(N);
       for (uint32_t c = 0; c < N; ++c)
         xs[c] = make_xs<Real>(static_cast<Real>(f[c] * s1[c]),

@@ -97,7 +97,9 @@ class Counters(C.Structure):
                 ("ddmc_steps", C.c_uint64), ("energy_quantum_exp", C.c_int32),
                 ("transport_block", C.c_int32), ("seconds_step", C.c_double),
                 ("seconds_transport", C.c_double), ("seconds_transport_max_launch", C.c_double),
-                ("max_packet_blocks", C.c_uint64)]
+                ("max_packet_blocks", C.c_uint64),
+                ("transport_chunks", C.c_int32),
+                ("transport_schedule", C.c_int32)]
 
 
 class U128(C.Structure):

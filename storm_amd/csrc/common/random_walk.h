@@ -16,8 +16,9 @@
 // Samplers (RandomWalk.hpp:81-137): radius by bilinear interpolation in
 // (log tau, xi), as the reference does.  The leak time inverts the same
 // S(tau) through a table of the inverse itself (build_leak_table), where the
-// reference searches its S table: the same distribution (the two agree to
-// ~1e-5 in tau), with one log and two neighbouring loads in place of a chain
+// reference searches its S table: the same distribution (the table is within
+// 2e-5 in tau of the exact inverse, the search of the S table within 1.3e-3),
+// with one log and two neighbouring loads in place of a chain
 // of ten dependent loads per random-walk step.
 #pragma once
 
@@ -39,13 +40,25 @@ constexpr int kLeakSize = 2049;
 constexpr double kLeakLogYMin = -25.0;
 constexpr double kLeakLogYMax = 1.5;
 
+// ln kTauMin and ln kTauMax
+constexpr double kLogTauMin = -18.420680743952367;
+constexpr double kLogTauMax = 4.1588830833596715;
+
 template <typename Real>
 struct Tables {
   const Real *tau;       // [kTableSize]
   const Real *leak;      // [kLeakSize], increasing (build_leak_table)
   const Real *radius;    // [kTableSize * kRadiusSize]
-  Real log_tau_min, log_tau_max;
 };
+
+// The three tables in one buffer of kAllSize numbers: leak, tau, radius.
+constexpr int kTauOffset = kLeakSize;
+constexpr int kRadiusOffset = kLeakSize + kTableSize;
+constexpr int kAllSize = kRadiusOffset + kTableSize * kRadiusSize;
+template <typename Real>
+STORM_HD Tables<Real> tables_at(const Real *all) {
+  return Tables<Real>{all + kTauOffset, all, all + kRadiusOffset};
+}
 
 // Leak time for a uniform xi: the T with S(T) = 1 - xi.  With
 // y = -ln(1 - xi), T is interpolated linearly in ln y between the two
@@ -82,7 +95,7 @@ STORM_HD Real sample_radius(const Tables<Real> &t, Real tau, Real xi) {
   using M = MathPolicy<Real>;
   const Real tmin = t.tau[0], tmax = t.tau[kTableSize - 1];
   const Real tc = tau < tmin ? tmin : (tau > tmax ? tmax : tau);
-  const Real pos = (M::log(tc) - t.log_tau_min) / (t.log_tau_max - t.log_tau_min) * Real(kTableSize - 1);
+  const Real pos = (M::log(tc) - Real(kLogTauMin)) / (Real(kLogTauMax) - Real(kLogTauMin)) * Real(kTableSize - 1);
   int ti = static_cast<int>(pos);
   if (ti > kTableSize - 2) ti = kTableSize - 2;
   if (ti < 0) ti = 0;
@@ -196,6 +209,13 @@ inline void build_leak_table(double *leak) {
     leak[i] = exp(0.5 * (lo + hi));
     lo0 = lo;
   }
+}
+
+// Fills `all` [kAllSize] with the three tables (tables_at's layout).
+inline void build_all_tables(double *all) {
+  double survival[kTableSize];
+  build_leak_table(all);
+  build_tables(all + kTauOffset, survival, all + kRadiusOffset);
 }
 
 }  // namespace rw

@@ -109,11 +109,18 @@ template <typename Real>
 struct MeshView {
   int32_t nx, ny, nz;
   uint32_t escape_mask;  // bit f set: box face f removes the packet (else it reflects)
-  // per axis a = x, y, z the cell widths and their reciprocals, interleaved:
-  // width[a][2 i] = w_i, width[a][2 i + 1] = 1 / w_i (width_pairs)
-  const Real *width[3];
+  // the cell widths and their reciprocals, interleaved (width_pairs), axis
+  // x then y then z: index i along axis a at width_at(m, a, i)
+  const Real *width;
   SpectrumView<Real> spec;
 };
+
+// (width, 1 / width) of index i along axis a
+template <typename Real>
+STORM_HD const Real *width_at(const MeshView<Real> &m, int a, uint32_t i) {
+  const uint32_t off = a == 0 ? 0u : (a == 1 ? static_cast<uint32_t>(m.nx) : static_cast<uint32_t>(m.nx + m.ny));
+  return m.width + 2u * (off + i);
+}
 
 // Host: widths w[0..n) and their reciprocals, interleaved into out[0..2n).
 // The reciprocal is the one IEEE division a segment would otherwise do on
@@ -196,10 +203,7 @@ struct CellAccel {
 template <typename Real>
 struct AccelView {
   const CellAccel<Real> *cell;  // null: no random walk and no DDMC in this problem
-  const Real *rw_tau;
-  const Real *rw_leak;      // [rw::kLeakSize]
-  const Real *rw_radius;
-  Real rw_log_tau_min, rw_log_tau_max;
+  const Real *rw;           // [rw::kAllSize]: the tables (rw::tables_at)
   Real rw_particle_tau;    // rwMinParticleOpticalDepth
   Real ddmc_particle_tau;  // ddmcMinParticleOpticalDepth
 };
@@ -297,7 +301,7 @@ STORM_HD double next_above(double x) {
 //   visit(cell, dep_limbs, path_limbs, segments)  (Real = float) or
 //   visit_fp(cell, dep, path, segments)           (Real = double): a cell visit ends
 //   roulette(cell, weight)                          Real = float: weight rouletted into the material
-//   escape(cell, weight)                            the packet leaves the problem
+//   escape_fx(cell, weight) / escape_fp(...)        the packet leaves the problem (mixed / FP64)
 //   rw_step() / ddmc_step()                         acceleration counters
 // The mixed engine tallies the path integral only in cells without
 // absorption: elsewhere it equals the absorbed energy divided by f sigma_a
@@ -467,9 +471,9 @@ struct Tracker {
 
   STORM_HD void load_cell(const MeshView<Real> &m, const CellXS<Real> *xs) {
     cell = flat_cell(m, ijk);
-    const Real *p0 = m.width[0] + 2u * ijk_i(ijk);
-    const Real *p1 = m.width[1] + 2u * ijk_j(ijk);
-    const Real *p2 = m.width[2] + 2u * ijk_k(ijk);
+    const Real *p0 = width_at(m, 0, ijk_i(ijk));
+    const Real *p1 = width_at(m, 1, ijk_j(ijk));
+    const Real *p2 = width_at(m, 2, ijk_k(ijk));
     cw0 = p0[0];
     icw0 = p0[1];
     cw1 = p1[0];
@@ -588,7 +592,7 @@ struct Tracker {
           // ---------------- random-walk step (tryRandomWalkStep)
           const Philox4 r = rs.block(ctr++);
           const Philox4 r2 = rs.block(ctr++);
-          rw::Tables<Real> tb{acc.rw_tau, acc.rw_leak, acc.rw_radius, acc.rw_log_tau_min, acc.rw_log_tau_max};
+          const rw::Tables<Real> tb = rw::tables_at(acc.rw);
           const Real tauLeak = rw::sample_leak_time<Real>(tb, M::uniform(r.v[0]));
           // tLeak = tau Ro^2 / D with D = c / (3 sig_d): as a path, tau Ro^2 3 sig_d
           const Real sig_d = pgrw ? ca.sig_d : ca.sig_t;
@@ -780,7 +784,7 @@ struct Tracker {
     const uint32_t ncell = outside ? cell : (up ? cell + cstride : cell - cstride);
     const uint32_t nijk = outside ? ijk : (up ? ijk + istride : ijk - istride);
     const CellXS<Real> nxs = xs[ncell];
-    const Real *nwp = sel3<const Real *>(fax, m.width[0], m.width[1], m.width[2]) + 2u * nidx;
+    const Real *nwp = width_at(m, fax, nidx);
     const Real nw = nwp[0], niw = nwp[1];
     Real nsig_s = Real(0);
     if ((F & kFeatMultigroup) != 0u && m.spec.groups >= 2) nsig_s = m.spec.sig_s[ncell];
@@ -840,7 +844,8 @@ struct Tracker {
           // escape (REMOVE); the weight leaves the problem
           double wq = fixed_point ? fx::carry_round(w, w_enter, k.eq, id, ctr, cell) : w;
           flush(wq, k, sink);
-          sink.escape(cell, wq);
+          if constexpr (fixed_point) sink.escape_fx(cell, wq);
+          else sink.escape_fp(cell, wq);
           w = 0.0;
           outcome = kEscaped;
         } else {

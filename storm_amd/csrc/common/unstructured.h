@@ -305,7 +305,7 @@ struct UTracker {
           // ---------------- random-walk step (tryRandomWalkStep)
           const Philox4 r = rs.block(ctr++);
           const Philox4 r2 = rs.block(ctr++);
-          rw::Tables<Real> tb{acc.rw_tau, acc.rw_leak, acc.rw_radius, acc.rw_log_tau_min, acc.rw_log_tau_max};
+          const rw::Tables<Real> tb = rw::tables_at(acc.rw);
           const Real tauLeak = rw::sample_leak_time<Real>(tb, M::uniform(r.v[0]));
           const Real sig_d = pgrw ? ca.sig_d : ca.sig_t;
           const Real sLeak = tauLeak * Ro * Ro * (Real(3) * sig_d);
@@ -530,7 +530,8 @@ struct UTracker {
         if ((m.escape_mask >> side) & 1u) {
           double wq = fixed_point ? fx::carry_round(w, w_enter, k.eq, id, ctr, cell) : w;
           flush(wq, k, sink);
-          sink.escape(cell, wq);
+          if constexpr (fixed_point) sink.escape_fx(cell, wq);
+          else sink.escape_fp(cell, wq);
           w = 0.0;
           outcome = kEscaped;
         } else {

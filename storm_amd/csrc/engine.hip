@@ -166,27 +166,18 @@ struct DevBuf {
 inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t>((n + b - 1) / b); }
 
 constexpr uint64_t kMaxSegmentsPerStep = uint64_t(1) << 24;
-// k_transport: 64-packet chunks per wavefront of a block by default (a block
-// owns chunks * block packets of the cell-ordered input and one LDS tally
-// table for all of them); 1, 2 or 4, storm_gpu_set_chunks_per_wave or the
-// on-line tuner.
+// The pass schedule's 64-packet chunks per wavefront of a block by default
+// (a block owns chunks * block packets of the cell-ordered input and one LDS
+// tally table for all of them); 1, 2, 4, 8 or 16 by
+// storm_gpu_set_chunks_per_wave or the on-line tuner.
 constexpr uint32_t kChunksPerWave = 2;
 
 // The random-walk tables in double, built once per process (the engine
 // converts them to its precision on upload).
-struct RwHostTables {
-  std::vector<double> tau, survival, radius, leak;
-};
-
-static const RwHostTables &rw_host_tables() {
-  static const RwHostTables tb = [] {
-    RwHostTables t;
-    t.tau.resize(rw::kTableSize);
-    t.survival.resize(rw::kTableSize);
-    t.radius.resize(static_cast<size_t>(rw::kTableSize) * rw::kRadiusSize);
-    t.leak.resize(rw::kLeakSize);
-    rw::build_tables(t.tau.data(), t.survival.data(), t.radius.data());
-    rw::build_leak_table(t.leak.data());
+static const std::vector<double> &rw_host_tables() {
+  static const std::vector<double> tb = [] {
+    std::vector<double> t(rw::kAllSize);
+    rw::build_all_tables(t.data());
     return t;
   }();
   return tb;
@@ -256,20 +247,27 @@ struct EngineBase {
   virtual void debug_spectrum(int32_t *, double *, void *, void *, void *) = 0;
   storm_gpu_counters counters{};
   double source_T = 0.0;
-  int32_t block = 256;  // k_transport; k_transport_refill uses at most 256
-  int32_t chunks_per_wave = static_cast<int32_t>(kChunksPerWave);
+  int32_t block = 256;
   int32_t seg_per_pass = 1 << 20;
   int32_t schedule = STORM_GPU_SCHEDULE_PASS;
+  // 64-packet chunks per wavefront of a block's range, per schedule (pass,
+  // refill): refill gains from a larger range (its lanes idle only once the
+  // range is drained), pass does not (its wavefront waits for its longest
+  // history whatever the range)
+  int32_t chunks_of[2] = {static_cast<int32_t>(kChunksPerWave), 8};
+  int32_t chunks_per_wave() const { return chunks_of[schedule == STORM_GPU_SCHEDULE_REFILL ? 1 : 0]; }
+  int32_t &chunks_ref() { return chunks_of[schedule == STORM_GPU_SCHEDULE_REFILL ? 1 : 0]; }
 
   // On-line tuning of the transport's launch (storm_gpu_set_tuning with
   // block 0).  A packet's history does not depend on the block size, the
   // chunks a block hands out or the schedule, so trying them changes no
   // result.  Three phases, one parameter each with the others at their best
-  // so far: the block size (128, 256, 512), the chunks per wavefront (1, 2,
-  // 4), the schedule (pass, refill).  A phase runs its candidates forwards
-  // then backwards, one step each (an order that cancels a steady trend of
-  // the workload), and keeps the one with the highest mean rate of transport
-  // segments per second.
+  // so far: the schedule (pass, refill, each with its own chunks), the block
+  // size (128, 256, 512), the chunks per wavefront of the chosen schedule (1,
+  // 2, 4, 8, 16).  A phase runs its candidates forwards then backwards, one
+  // step each (an order that cancels a steady trend of the workload), and
+  // keeps the one with the highest mean rate of transport segments per
+  // second; 20 steps in all.
   struct LaunchShape {
     int32_t block, chunks, schedule;
   };
@@ -283,14 +281,13 @@ struct EngineBase {
   }
   void tune_plan() {
     std::vector<LaunchShape> c;
-    const LaunchShape cur{block, chunks_per_wave, schedule};
     if (tune_phase == 0) {
-      for (int32_t b : {128, 256, 512}) c.push_back({b, cur.chunks, STORM_GPU_SCHEDULE_PASS});
+      c.push_back({block, chunks_of[0], STORM_GPU_SCHEDULE_PASS});
+      c.push_back({block, chunks_of[1], STORM_GPU_SCHEDULE_REFILL});
     } else if (tune_phase == 1) {
-      for (int32_t k : {1, 2, 4}) c.push_back({cur.block, k, STORM_GPU_SCHEDULE_PASS});
+      for (int32_t b : {128, 256, 512}) c.push_back({b, chunks_per_wave(), schedule});
     } else {
-      c.push_back({cur.block, cur.chunks, STORM_GPU_SCHEDULE_PASS});
-      c.push_back({cur.block, cur.chunks, STORM_GPU_SCHEDULE_REFILL});
+      for (int32_t k : {1, 2, 4, 8, 16}) c.push_back({block, k, schedule});
     }
     tune_ncand = c.size();
     tune_seq = c;
@@ -298,30 +295,82 @@ struct EngineBase {
     tune_sum.assign(tune_ncand, 0.0);
     tune_pos = 0;
   }
-  void tune_before_step() {
-    if (tune_phase < 0) return;
-    const LaunchShape &t = tune_seq[tune_pos];
+  void apply(const LaunchShape &t) {
     block = t.block;
-    chunks_per_wave = t.chunks;
     schedule = t.schedule;
+    chunks_ref() = t.chunks;
   }
-  void tune_after_step(double segments, double seconds) {
-    if (tune_phase < 0) return;
-    const size_t k = tune_pos < tune_ncand ? tune_pos : 2 * tune_ncand - 1 - tune_pos;
-    tune_sum[k] += seconds > 0.0 ? segments / seconds : 0.0;
-    if (++tune_pos < tune_seq.size()) return;
-    size_t best = 0;
-    for (size_t q = 1; q < tune_ncand; ++q)
-      if (tune_sum[q] > tune_sum[best]) best = q;
-    block = tune_seq[best].block;
-    chunks_per_wave = tune_seq[best].chunks;
-    schedule = tune_seq[best].schedule;
-    ++tune_phase;
-    if (tune_phase > 2) {
-      tune_phase = -1;
+
+  // The schedule watch.  The workload of a problem changes as it runs (a
+  // cold start that the random walk dominates, a heated region that then
+  // spreads), and so does the faster schedule: on the 64^3 cube as stated
+  // refill is 24% faster over steps 3-12 and pass 12% faster over steps
+  // 13-50.  Every kWatchEvery steps one step runs in the other schedule; its
+  // rate is compared with the mean of the steps either side of it (which
+  // cancels a steady trend) and the engine switches when the other is more
+  // than 3% faster.  On unless the launch was fixed (storm_gpu_set_tuning
+  // with a block size); results do not depend on it.
+  static constexpr uint32_t kWatchEvery = 8;
+  bool watch = true;
+  int32_t watch_state = 0;  // 0: normal, 1: this step probes, 2: the step after the probe
+  uint32_t watch_n = 0;
+  double watch_prev = 0.0, watch_alt = 0.0;
+  void watch_reset() {
+    watch_state = 0;
+    watch_n = 0;
+    watch_prev = 0.0;
+  }
+  static int32_t other(int32_t s) {
+    return s == STORM_GPU_SCHEDULE_REFILL ? STORM_GPU_SCHEDULE_PASS : STORM_GPU_SCHEDULE_REFILL;
+  }
+
+  void tune_before_step() {
+    if (tune_phase >= 0) {
+      apply(tune_seq[tune_pos]);
       return;
     }
-    tune_plan();
+    if (watch && watch_state == 0 && watch_n >= kWatchEvery && watch_prev > 0.0) {
+      watch_state = 1;
+      schedule = other(schedule);
+    }
+  }
+  void tune_after_step(double segments, double seconds) {
+    const double rate = seconds > 0.0 ? segments / seconds : 0.0;
+    if (tune_phase >= 0) {
+      const size_t k = tune_pos < tune_ncand ? tune_pos : 2 * tune_ncand - 1 - tune_pos;
+      tune_sum[k] += rate;
+      if (++tune_pos < tune_seq.size()) return;
+      size_t best = 0;
+      for (size_t q = 1; q < tune_ncand; ++q)
+        if (tune_sum[q] > tune_sum[best]) best = q;
+      apply(tune_seq[best]);
+      if (++tune_phase > 2) {
+        tune_phase = -1;
+        watch_reset();
+        return;
+      }
+      tune_plan();
+      return;
+    }
+    if (!watch) return;
+    if (watch_state == 0) {
+      watch_prev = rate;
+      ++watch_n;
+    } else if (watch_state == 1) {
+      watch_alt = rate;
+      schedule = other(schedule);
+      watch_state = 2;
+    } else {
+      const double around = 0.5 * (watch_prev + rate);
+      if (watch_alt > 1.03 * around) {
+        schedule = other(schedule);
+        watch_prev = watch_alt;
+      } else {
+        watch_prev = rate;
+      }
+      watch_state = 0;
+      watch_n = 0;
+    }
   }
   bool keep_inputs = false;
 };
@@ -350,7 +399,7 @@ struct Engine final : EngineBase {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   // device: mesh and cells
-  DevBuf<Real> wx, wy, wz;
+  DevBuf<Real> widths;  // MeshView::width
   DevBuf<double> V, sa0, ss0, rho, T, e_mat, e_sync, fleck, sig_a, sig_s, Eemit, wemit, erad_avg, census_erad,
       clamp, partials, host_sa, host_ss, host_cv, wface, dsc;
   bool have_host_sa = false, have_host_ss = false, have_host_cv = false;
@@ -377,7 +426,7 @@ struct Engine final : EngineBase {
   std::vector<double> planck_acc;
   DevBuf<CellAccel<Real>> accel;
   DevBuf<double> Dtil;
-  DevBuf<Real> rw_tau, rw_leak, rw_rad;
+  DevBuf<Real> rw_tables;  // rw::tables_at's layout
   bool accel_on = false;
   // multigroup (storm_group_desc)
   int32_t groups = 0;
@@ -526,13 +575,16 @@ struct Engine final : EngineBase {
       nu_ref = p->groups.nu_ref;
       if (!(nu_ref > 0.0)) throw EngineError(1, "groups.nu_ref must be positive");
     }
-    // The mixed engine (6 waves per SIMD) runs one packet per lane and pass:
-    // within 4% of lane refill on the cube and the step slab, 27% faster on
-    // the Voronoi mesh.  The FP64 engine (2-3 waves per SIMD, where a lane
-    // idling until its chunk's longest history ends costs more) refills
-    // lanes: 18% faster on the cube, 12% on the step slab, even on the
-    // Voronoi mesh (20% slower on the batch of slabs, where the tuner picks).
-    schedule = precision == STORM_GPU_FP64 ? STORM_GPU_SCHEDULE_REFILL : STORM_GPU_SCHEDULE_PASS;
+    // The starting schedule (the watch corrects it as the problem evolves),
+    // measured over steps 3-12 at 256 threads: Cartesian meshes refill lanes
+    // from ranges of 8 chunks per wavefront (the 64^3 cube: 22.4 against 18.0
+    // G segments/s mixed, 13.5 against 8.2 FP64; the step slab: 18.3 against
+    // 16.5 mixed); the mixed engine on polyhedral meshes runs passes (the
+    // Voronoi problem: 10.3 against 8.5), the FP64 engine refills there too.
+    schedule = (!unstructured || precision == STORM_GPU_FP64) ? STORM_GPU_SCHEDULE_REFILL : STORM_GPU_SCHEDULE_PASS;
+    // refill on polyhedral meshes from 4 chunks (the batch of slabs: 28.5
+    // against 26.3 at 8 chunks)
+    if (unstructured && precision != STORM_GPU_FP64) chunks_of[1] = 4;
     if (bnd.source_face != -1 && bnd.source_face != STORM_XLO)
       throw EngineError(1, "source_face must be -1 or STORM_XLO (an x+ source is source_xhi)");
     if (bnd.source_xhi && !(bnd.source_temperature_xhi >= 0.0)) throw EngineError(1, "invalid x+ source temperature");
@@ -602,16 +654,18 @@ struct Engine final : EngineBase {
     HIPCHECK(hipEventCreate(&ev0));
     HIPCHECK(hipEventCreate(&ev1));
     // mesh widths and their reciprocals, interleaved (MeshView::width)
-    auto up_w = [&](DevBuf<Real> &d, const std::vector<double> &e) {
-      std::vector<Real> w(e.size() - 1), wp(2 * (e.size() - 1));
-      for (size_t q = 0; q + 1 < e.size(); ++q) w[q] = static_cast<Real>(e[q + 1] - e[q]);
-      width_pairs<Real>(w.data(), static_cast<int>(w.size()), wp.data());
-      d.ensure(wp.size());
-      HIPCHECK(hipMemcpy(d.p, wp.data(), wp.size() * sizeof(Real), hipMemcpyHostToDevice));
-    };
-    up_w(wx, xe);
-    up_w(wy, ye);
-    up_w(wz, ze);
+    {
+      std::vector<Real> wp;
+      for (const std::vector<double> *e : {&xe, &ye, &ze}) {
+        std::vector<Real> w(e->size() - 1);
+        for (size_t q = 0; q + 1 < e->size(); ++q) w[q] = static_cast<Real>((*e)[q + 1] - (*e)[q]);
+        const size_t at = wp.size();
+        wp.resize(at + 2 * w.size());
+        width_pairs<Real>(w.data(), static_cast<int>(w.size()), wp.data() + at);
+      }
+      widths.ensure(wp.size());
+      HIPCHECK(hipMemcpy(widths.p, wp.data(), wp.size() * sizeof(Real), hipMemcpyHostToDevice));
+    }
     auto up = [&](DevBuf<double> &d, const double *src, size_t n) {
       d.ensure(n);
       HIPCHECK(hipMemcpy(d.p, src, n * sizeof(double), hipMemcpyHostToDevice));
@@ -747,15 +801,10 @@ struct Engine final : EngineBase {
         // random-walk tables built on the host (once per process), like the
         // RandomWalk constructor the reference calls from RadiationIMC's
         // constructor
-        const RwHostTables &tb = rw_host_tables();
-        auto up_r = [&](DevBuf<Real> &d, const std::vector<double> &v) {
-          std::vector<Real> h(v.begin(), v.end());
-          d.ensure(h.size());
-          HIPCHECK(hipMemcpy(d.p, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice));
-        };
-        up_r(rw_tau, tb.tau);
-        up_r(rw_leak, tb.leak);
-        up_r(rw_rad, tb.radius);
+        const std::vector<double> &tb = rw_host_tables();
+        std::vector<Real> h(tb.begin(), tb.end());
+        rw_tables.ensure(h.size());
+        HIPCHECK(hipMemcpy(rw_tables.p, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice));
       }
     }
     tally_u64.ensure(9 * 2 * static_cast<size_t>(N));
@@ -869,11 +918,7 @@ struct Engine final : EngineBase {
   AccelView<Real> accelview() const {
     AccelView<Real> a;
     a.cell = accel_on ? accel.p : nullptr;
-    a.rw_tau = rw_tau.p;
-    a.rw_leak = rw_leak.p;
-    a.rw_radius = rw_rad.p;
-    a.rw_log_tau_min = static_cast<Real>(std::log(rw::kTauMin));
-    a.rw_log_tau_max = static_cast<Real>(std::log(rw::kTauMax));
+    a.rw = rw_tables.p;
     // a particle threshold of +inf switches a method off
     a.rw_particle_tau = imc.with_random_walk ? static_cast<Real>(imc.rw_min_particle_optical_depth)
                                              : MathPolicy<Real>::inf();
@@ -893,9 +938,7 @@ struct Engine final : EngineBase {
     m.escape_mask = 0u;
     for (int f = 0; f < 6; ++f)
       if (bnd.face_kind[f] == STORM_FACE_ESCAPE) m.escape_mask |= 1u << f;
-    m.width[0] = wx.p;
-    m.width[1] = wy.p;
-    m.width[2] = wz.p;
+    m.width = widths.p;
     m.spec = spectrum();
     return m;
   }
@@ -929,16 +972,16 @@ struct Engine final : EngineBase {
   }
   // Tally-cache slots of a transport kernel at this block size and chunking:
   // the share of the CU's LDS of each block its registers let run there.
-  static int chunk_index(int32_t k) { return k == 1 ? 0 : (k == 2 ? 1 : 2); }
+  static int chunk_index(int32_t k) { return k == 1 ? 0 : (k == 2 ? 1 : (k == 4 ? 2 : (k == 8 ? 3 : 4))); }
   static int block_index(int32_t b) { return b == 64 ? 0 : (b == 128 ? 1 : (b == 256 ? 2 : 3)); }
-  uint32_t slots_cache[2][2][4][4][3] = {};  // [refill][polyhedral][F][block][chunks]
+  uint32_t slots_cache[2][2][4][4][5] = {};  // [refill][polyhedral][F][block][chunks]
   template <typename K>
   uint32_t cache_slots_for(K kernel, bool refill, uint32_t F, int32_t b, uint32_t nf) {
-    uint32_t &r = slots_cache[refill ? 1 : 0][unstructured ? 1 : 0][F][block_index(b)][chunk_index(chunks_per_wave)];
+    uint32_t &r = slots_cache[refill ? 1 : 0][unstructured ? 1 : 0][F][block_index(b)][chunk_index(chunks_per_wave())];
     if (!r) {
       int per_cu = 0;
       HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, b, 0));
-      r = cache_slots(static_cast<uint32_t>(b) * static_cast<uint32_t>(chunks_per_wave), nf,
+      r = cache_slots(static_cast<uint32_t>(b) * static_cast<uint32_t>(chunks_per_wave()), nf,
                       static_cast<uint32_t>(std::max(per_cu, 1)));
     }
     return r;
@@ -950,7 +993,7 @@ struct Engine final : EngineBase {
   void launch_transport_m(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshT &mesh,
                           const AccelView<Real> &av, const StepConsts &kc) {
     const uint32_t nf = (precision == STORM_GPU_MIXED && groups < 2) ? 1u : 2u;
-    const uint32_t chunks = static_cast<uint32_t>(chunks_per_wave) * static_cast<uint32_t>(block / 64);
+    const uint32_t chunks = static_cast<uint32_t>(chunks_per_wave()) * static_cast<uint32_t>(block / 64);
     const uint32_t grid = blocks_for(n_in, 64u * chunks);
     if (schedule == STORM_GPU_SCHEDULE_REFILL) {
       const uint32_t slots = cache_slots_for(k_transport_refill<Real, F, MeshT>, true, F, block, nf);
@@ -1294,6 +1337,8 @@ struct Engine final : EngineBase {
     tune_before_step();
     if (schedule == STORM_GPU_SCHEDULE_REFILL) cstage.ensure(n_total);
     counters.transport_block = block;
+    counters.transport_chunks = chunks_per_wave();
+    counters.transport_schedule = schedule;
     const MeshView<Real> mv = meshview();
     const AccelView<Real> av = accelview();
     uint64_t n_in = n_total;
@@ -2163,24 +2208,28 @@ int storm_gpu_set_tuning(storm_gpu_engine *eng, int32_t block_size, int32_t segm
     if (block_size != 0 && block_size != 64 && block_size != 128 && block_size != 256 && block_size != 512)
       throw EngineError(1, "block size must be 0 (chosen on line), 64, 128, 256 or 512");
     if (segments_per_pass < 1) throw EngineError(1, "segments per pass must be >= 1");
-    if (schedule != STORM_GPU_SCHEDULE_PASS && schedule != STORM_GPU_SCHEDULE_REFILL)
-      throw EngineError(1, "schedule must be STORM_GPU_SCHEDULE_PASS or STORM_GPU_SCHEDULE_REFILL");
+    if (schedule != STORM_GPU_SCHEDULE_PASS && schedule != STORM_GPU_SCHEDULE_REFILL && schedule != -1)
+      throw EngineError(1, "schedule must be STORM_GPU_SCHEDULE_PASS, STORM_GPU_SCHEDULE_REFILL or -1");
     EngineBase *e = as_engine(eng);
     e->seg_per_pass = segments_per_pass;
-    e->schedule = schedule;
+    if (schedule != -1) e->schedule = schedule;
+    e->watch_reset();
     if (block_size == 0) {
       e->tune_start();
+      e->watch = true;
     } else {
       e->tune_phase = -1;
       e->block = block_size;
+      e->watch = false;
     }
   });
 }
 
 int storm_gpu_set_chunks_per_wave(storm_gpu_engine *eng, int32_t chunks) {
   return guarded([&] {
-    if (chunks != 1 && chunks != 2 && chunks != 4) throw EngineError(1, "chunks per wavefront must be 1, 2 or 4");
-    as_engine(eng)->chunks_per_wave = chunks;
+    if (chunks != 1 && chunks != 2 && chunks != 4 && chunks != 8 && chunks != 16)
+      throw EngineError(1, "chunks per wavefront must be 1, 2, 4, 8 or 16");
+    as_engine(eng)->chunks_ref() = chunks;
   });
 }
 
@@ -2189,7 +2238,7 @@ int storm_gpu_get_launch(storm_gpu_engine *eng, int32_t *block_size, int32_t *ch
   return guarded([&] {
     EngineBase *e = as_engine(eng);
     *block_size = e->block;
-    *chunks_per_wave = e->chunks_per_wave;
+    *chunks_per_wave = e->chunks_per_wave();
     *schedule = e->schedule;
     *tuning = e->tune_phase >= 0 ? 1 : 0;
   });

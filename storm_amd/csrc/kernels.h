@@ -881,13 +881,9 @@ struct DeviceSink {
     }
     return true;
   }
-  __device__ void escape(uint32_t c, double w) {
-    if (t.esc_fp) {
-      atomicAdd(&t.esc_fp[c], w);
-    } else {
-      tally_limbs(t.esc, c, fx::limbs_of(w, eq));
-    }
-  }
+  // escaped weight: in quanta (mixed engine) or as a double (FP64 engine)
+  __device__ void escape_fx(uint32_t c, double w) { tally_limbs(t.esc, c, fx::limbs_of(w, eq)); }
+  __device__ void escape_fp(uint32_t c, double w) { atomicAdd(&t.esc_fp[c], w); }
   __device__ void rw_step() { ++rw; }
   __device__ void ddmc_step() { ++ddmc; }
   // path integral v (erg cm) of cell c in group g: rounded without bias to
@@ -1187,7 +1183,7 @@ __global__ __launch_bounds__(256) void k_accel_rates(uint32_t N, MeshDims d, Mes
     const double D2 = Dtil[nb];
     double Df = D1;
     if (D2 > 0.0) Df = (D1 > 0.0 && D2 > 0.0) ? 2.0 * D1 * D2 / (D1 + D2) : fmax(D1, D2);
-    const double w = mv.width[ax][2u * idx[ax]];
+    const double w = width_at(mv, ax, idx[ax])[0];
     const double rate = Df / (w * (0.5 * w));  // A / V = 1 / w, face distance w / 2
     if (rate > 0.0 && isfinite(rate)) {
       a.rate[f] = static_cast<Real>(rate);

@@ -72,11 +72,11 @@ class GpuEngine:
         self._check(self.lib.storm_gpu_set_source_temperature(self.h, float(T)))
 
     def set_tuning(self, block: int = 256, segments_per_pass: int = 1 << 20, schedule: int | None = None):
-        """Transport launch shape; `schedule` is capi.SCHEDULE_PASS or capi.SCHEDULE_REFILL (None: the
-        engine's default, pass in the mixed engine and refill in the FP64 one); block 0 tunes the block
-        size, the chunks per wavefront and the schedule on line over the next 16 steps."""
+        """Transport launch shape; `schedule` is capi.SCHEDULE_PASS or capi.SCHEDULE_REFILL (None: keep
+        the engine's); a block size fixes the launch (no schedule watch), block 0 tunes the schedule,
+        the block size and the chunks per wavefront on line over the next 20 steps, then watches."""
         if schedule is None:
-            schedule = capi.SCHEDULE_REFILL if self.precision == capi.FP64 else capi.SCHEDULE_PASS
+            schedule = -1
         self._check(self.lib.storm_gpu_set_tuning(self.h, int(block), int(segments_per_pass), int(schedule)))
 
     def set_cell_state(self, T=None, sigma_a=None, sigma_s=None, cv=None, e=None):

@@ -44,7 +44,7 @@ def test_struct_layouts_match_header_sizes():
 
 def test_abi_version_without_gpu():
     lib = capi.load_library()
-    assert lib.storm_gpu_abi_version() == 7
+    assert lib.storm_gpu_abi_version() == 8
 
 
 def test_cpp_manager_program_fails_loudly_without_engine():

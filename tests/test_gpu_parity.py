@@ -428,31 +428,34 @@ def _tuned_against_fixed(spec, steps):
         for _ in range(steps):
             e.step()
             tuned.append((e.cells(), e.tallies()))
-            L = e.launch()
-            shapes.append((e.counters()["transport_block"], L["chunks"], L["schedule"], L["tuning"]))
+            c = e.counters()
+            shapes.append((c["transport_block"], c["transport_chunks"], c["transport_schedule"], e.launch()["tuning"]))
     return fixed, tuned, shapes
 
 
 @pytest.mark.parametrize("mesh", ["cartesian", "voronoi"])
 def test_online_launch_tuning_keeps_results(mesh):
-    """Block size 0: the transport's launch is tuned on line -- blocks of
-    128, 256, 512, 512, 256, 128 threads, then 1, 2, 4, 4, 2, 1 chunks per
-    wavefront, then the pass and refill schedules -- and the best is kept;
-    the cell states and tallies are those of a fixed launch, step by step."""
-    if mesh == "cartesian":
-        spec, steps = C2(13), 18
-    else:
-        spec, steps = P.voronoi_slab(seed=13, steps=18), 18
-    nphase = 16
+    """Block size 0: the transport's launch is tuned on line -- the pass and
+    refill schedules, then blocks of 128, 256, 512, 512, 256, 128 threads,
+    then 1, 2, 4, 8, 16, 16, 8, 4, 2, 1 chunks per wavefront -- and the best
+    is kept, after which the schedule watch probes the other schedule every
+    8 steps; the cell states and tallies are those of a fixed launch, step by
+    step."""
+    spec = C2(13) if mesh == "cartesian" else P.voronoi_slab(seed=13)
+    steps = 32
+    spec.steps = steps
+    nphase = 20
     fixed, tuned, shapes = _tuned_against_fixed(spec, steps)
-    assert [b for b, _, _, _ in shapes[:6]] == [128, 256, 512, 512, 256, 128], shapes
-    assert [k for _, k, _, _ in shapes[6:12]] == [1, 2, 4, 4, 2, 1], shapes
-    assert [s for _, _, s, _ in shapes[12:16]] == [0, 1, 1, 0], shapes
-    # the choice after tuning is one of the candidates and stays
+    assert [s for _, _, s, _ in shapes[:4]] == [0, 1, 1, 0], shapes
+    assert [b for b, _, _, _ in shapes[4:10]] == [128, 256, 512, 512, 256, 128], shapes
+    assert [k for _, k, _, _ in shapes[10:20]] == [1, 2, 4, 8, 16, 16, 8, 4, 2, 1], shapes
     assert all(t for _, _, _, t in shapes[:nphase - 1]) and not any(t for _, _, _, t in shapes[nphase - 1:]), shapes
-    assert len(set(x[:3] for x in shapes[nphase:])) == 1, shapes
-    b, k, s, _ = shapes[-1]
-    assert b in (128, 256, 512) and k in (1, 2, 4) and s in (0, 1)
+    # after tuning: the chosen block stays; the watch runs one step in the
+    # other schedule 9 steps after tuning ends
+    b = shapes[nphase][0]
+    assert b in (128, 256, 512) and all(x[0] == b for x in shapes[nphase:]), shapes
+    sch = [s for _, _, s, _ in shapes[nphase:]]
+    assert sch[8] != sch[7], shapes
     for (ca, ta), (cb, tb) in zip(fixed, tuned):
         for key in ("T", "e", "erad", "census"):
             assert np.array_equal(ca[key], cb[key]), key
