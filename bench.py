@@ -21,11 +21,13 @@ and the per-cell agreement of the two forms (|z| <= 3, pooled variance).
 
 Roofline: no MFMA (nothing is a dense contraction), so the roofline is HBM
 bandwidth against the metric's algorithmic bytes, 144 B per segment (a 48 B
-packet read and written, a 16 B cell record, two 16 B tallies).  achieved =
-segments * 144 B / transport time over every launch of the mixed transport
-kernel in this process (HIP events on the engine's stream), so its average
-launch duration is the one a profiler's kernel statistics of this command
-report.  The history kernel keeps a packet in registers for its whole
+packet read and written, a 16 B cell record, two 16 B tallies).  The mixed
+engine runs two transport kernels (the pass and lane-refill schedules,
+switched as the problem evolves); the roofline is that of the one that ran
+the timed window: achieved = segments * 144 B / transport time over its
+launches in this process (HIP events on the engine's stream, attributed by
+the schedule each step ran), so its average launch duration is the one a
+profiler's kernel statistics of this command report; `by_kernel` gives both.  The history kernel keeps a packet in registers for its whole
 history, so its measured HBM traffic (rocprof FETCH_SIZE / WRITE_SIZE,
 `traffic`) is far below the model.
 
@@ -63,13 +65,51 @@ C2_KNOBS = dict(source_per_face=2400, comb_nmin=8)
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_transport_c2_literal_counters.md")
 
 
-def measured_traffic():
-    """(bytes per launch, source) from the committed counter summary, or (None, None)."""
+# the mixed engine's Cartesian transport kernels with the random-walk / DDMC
+# features, by schedule (storm_gpu_counters.transport_schedule)
+KERNEL_NAME = {0: "k_transport<float, 1u, MeshView<float>>", 1: "k_transport_refill<float, 1u, MeshView<float>>"}
+SCHEDULE_NAME = {0: "pass", 1: "refill"}
+
+
+class KernelTotals:
+    """Segments, transport seconds and launches per transport kernel."""
+
+    def __init__(self):
+        self.tot = {}
+
+    def add(self, schedule, segments, seconds, launches):
+        a = self.tot.setdefault(schedule, [0.0, 0.0, 0])
+        a[0] += segments
+        a[1] += seconds
+        a[2] += launches
+
+    def merge(self, other):
+        for k, (sg, t, n) in other.tot.items():
+            self.add(k, sg, t, n)
+
+    def segments(self):
+        return sum(a[0] for a in self.tot.values())
+
+    def seconds(self):
+        return sum(a[1] for a in self.tot.values())
+
+    def names(self):
+        return " + ".join(KERNEL_NAME[k] for k in sorted(self.tot))
+
+    def report(self):
+        return {KERNEL_NAME[k]: {"launches": n, "segments": sg, "seconds": t, "avg_launch_us": t / n * 1e6,
+                                 "achieved_GBps": sg * B_SEG_MIXED / t / 1e9}
+                for k, (sg, t, n) in sorted(self.tot.items())}
+
+
+def measured_traffic(kernel):
+    """(bytes per launch of `kernel`, source) from the committed counter
+    summary (a line "traffic_bytes_per_launch <kernel>: <bytes>"), or (None, None)."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1]), os.path.relpath(TRAFFIC_PROFILE, ROOT)
+                if line.startswith("traffic_bytes_per_launch") and kernel in line:
+                    return float(line.rsplit(":", 1)[1]), os.path.relpath(TRAFFIC_PROFILE, ROOT)
     except OSError:
         pass
     return None, None
@@ -96,13 +136,13 @@ def run_seed(precision, seed, n_steps, snap_step, timed=None, log=None):
             c = e.counters()
             steps.append(dict(wall=wall, segments=c["segments"], packets=c["start_particles"],
                               t_tr=c["seconds_transport"], launches=c["transport_launches"],
-                              rw=c["random_walk_steps"]))
+                              rw=c["random_walk_steps"], schedule=c["transport_schedule"]))
             if log and timed:
                 W, K = timed
                 tag = "warmup" if s < W else ("timed" if s < W + K else "untimed")
                 log(f"seed {seed} step {s + 1}/{n_steps} ({tag}): {c['segments']} segments, {c['start_particles']} "
-                    f"packets, {c['random_walk_steps']} walk steps, transport {c['seconds_transport'] * 1e3:.2f} ms, "
-                    f"step {wall * 1e3:.2f} ms")
+                    f"packets, {c['random_walk_steps']} walk steps, transport {c['seconds_transport'] * 1e3:.2f} ms "
+                    f"({SCHEDULE_NAME[c['transport_schedule']]}), step {wall * 1e3:.2f} ms")
             if s + 1 == snap_step:
                 snap = e.cells()
     return steps, snap
@@ -184,17 +224,14 @@ def c3_line(W, K, log):
     spec = P.c3_step(seed=1)
     seg = wall = t_tr = 0.0
     rw = ddmc = packets = launches = 0
-    seg_all = t_all = 0.0
-    n_all = 0
+    by_kernel = KernelTotals()
     with GpuEngine(spec, capi.MIXED) as e:
         for s in range(W + K):
             t0 = time.perf_counter()
             e.step()
             dtw = time.perf_counter() - t0
             c = e.counters()
-            seg_all += c["segments"]
-            t_all += c["seconds_transport"]
-            n_all += c["transport_launches"]
+            by_kernel.add(c["transport_schedule"], c["segments"], c["seconds_transport"], c["transport_launches"])
             log(f"c3 step {s + 1}/{W + K}: {c['segments']} segments, {c['start_particles']} packets, "
                 f"{c['random_walk_steps']} walk, {c['ddmc_steps']} DDMC, step {dtw * 1e3:.2f} ms")
             if s >= W:
@@ -205,16 +242,15 @@ def c3_line(W, K, log):
                 ddmc += c["ddmc_steps"]
                 packets += c["start_particles"]
                 launches += c["transport_launches"]
-    # every transport launch of the run (they are launches of the same kernel
-    # as the C2 cube's, counted in its roofline)
-    c3_all = (seg_all, t_all, n_all)
-    return dict(launch_totals=c3_all,
+    # every transport launch of the run (launches of the C2 cube's kernels,
+    # counted in its roofline)
+    return dict(launch_totals=by_kernel,
                 workload="c3_step: 256x16x16 two-material slab, sigma_a 1 | 1e4 (+ sigma_s 1e4), RW + DDMC, "
                          "~2e7 packets/step, dt=1e-11 s",
                 value=seg / wall, unit="segments/s", steps=K, warmup=W, ms_per_step=wall / K * 1e3,
                 packets_per_step=packets / K, transport_segments_per_s=seg / t_tr,
                 random_walk_steps_per_step=rw / K, ddmc_steps_per_step=ddmc / K,
-                kernel="k_transport<float, 1u, MeshView<float>> (random walk + DDMC features)",
+                kernels=by_kernel.names() + " (random walk + DDMC features)",
                 roofline_frac=seg * B_SEG_MIXED / t_tr / 1e9 / HBM_PEAK_GBS)
 
 
@@ -384,17 +420,24 @@ def main():
     mixed = ensemble(capi.MIXED, seeds, n_steps, W, K, log, "mixed")
     seg, wall = window(mixed[0]["steps"], W, W + K)
     value = seg / wall
-    tr_seg = sum(s["segments"] for r in mixed for s in r["steps"])
-    tr_t = sum(s["t_tr"] for r in mixed for s in r["steps"])
-    tr_n = sum(s["launches"] for r in mixed for s in r["steps"])
-    c2_transport_rate = tr_seg / tr_t
+    by_kernel = KernelTotals()
+    for r in mixed:
+        for st in r["steps"]:
+            by_kernel.add(st["schedule"], st["segments"], st["t_tr"], st["launches"])
+    c2_transport_rate = by_kernel.segments() / by_kernel.seconds()
     c3 = None
     if not args.no_c3:
         c3 = c3_line(2, 3, log)
-        s3, t3, n3 = c3.pop("launch_totals")
-        tr_seg, tr_t, tr_n = tr_seg + s3, tr_t + t3, tr_n + n3
+        by_kernel.merge(c3.pop("launch_totals"))
+    # the roofline of the kernel that ran the timed window (the schedule
+    # watch may switch kernels later in a run, at steps that depend on timing)
+    win = {}
+    for st in mixed[0]["steps"][W:W + K]:
+        win[st["schedule"]] = win.get(st["schedule"], 0.0) + st["t_tr"]
+    dom = max(win, key=win.get)
+    tr_seg, tr_t, tr_n = by_kernel.tot[dom]
     achieved = tr_seg * B_SEG_MIXED / tr_t / 1e9
-    traffic, traffic_src = measured_traffic()
+    traffic, traffic_src = measured_traffic(KERNEL_NAME[dom])
     packets = sum(s["packets"] for s in mixed[0]["steps"][W:W + K]) / K
     seg50, wall50 = window(mixed[0]["steps"], 0, PROBLEM_STEPS)
     out = {
@@ -427,13 +470,15 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": "k_transport<float, 1u, MeshView<float>>",
+            "kernel": KERNEL_NAME[dom],
             "bytes_per_segment": B_SEG_MIXED,
             "segments": tr_seg,
             "launches": tr_n,
             "avg_launch_us": tr_t / tr_n * 1e6,
-            "over": f"every launch of k_transport<float, 1u, MeshView<float>> in this process: the C2 cube's {len(seeds)} seeds x "
-                    f"{n_steps} steps" + ("" if c3 is None else " and the c3 line's 5 steps"),
+            "over": f"every launch of {KERNEL_NAME[dom]} in this process (the C2 cube's {len(seeds)} seeds x "
+                    f"{n_steps} steps" + ("" if c3 is None else " and the c3 line's 5 steps") + "), the kernel that "
+                    "ran the timed window",
+            "by_kernel": by_kernel.report(),
         },
         "transport_segments_per_s": c2_transport_rate,
         "full_problem": {

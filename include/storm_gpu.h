@@ -230,11 +230,12 @@ int storm_gpu_queue_source_packets(storm_gpu_engine *eng, const storm_packet_rec
  * wavefront (storm_gpu_set_chunks_per_wave; each schedule keeps its own,
  * 2 for PASS and 8 for REFILL by default, 4 for REFILL in the mixed engine on
  * polyhedral meshes).  The engine starts in REFILL, except the mixed engine
- * on polyhedral meshes (PASS), and watches: every 8 steps one step runs in
- * the other schedule and the engine switches when that one moved more
- * transport segments per second than the mean of the steps either side of
- * it, by more than 3%.  A block size other than 0 fixes the launch and stops
- * the watch.  A packet's history does not depend on the schedule, the block
+ * on multigroup polyhedral problems (PASS), and watches: after 16 steps one
+ * step runs in the other schedule, and the engine switches when that one
+ * moved more transport segments per second than the mean of the steps
+ * either side of it, by more than 3%; the next probe follows 16 steps later,
+ * or twice the last interval (at most 128) when the probe lost by more than
+ * 10%.  A block size other than 0 fixes the launch and stops the watch.  A packet's history does not depend on the schedule, the block
  * size or the chunks per wavefront, only on segments_per_pass: the mixed
  * engine's results are bit-identical across them, the FP64 engine's tallies
  * differ only in the order of their floating-point sums.  With block size 0

@@ -303,21 +303,27 @@ struct EngineBase {
 
   // The schedule watch.  The workload of a problem changes as it runs (a
   // cold start that the random walk dominates, a heated region that then
-  // spreads), and so does the faster schedule: on the 64^3 cube as stated
-  // refill is 24% faster over steps 3-12 and pass 12% faster over steps
-  // 13-50.  Every kWatchEvery steps one step runs in the other schedule; its
-  // rate is compared with the mean of the steps either side of it (which
-  // cancels a steady trend) and the engine switches when the other is more
-  // than 3% faster.  On unless the launch was fixed (storm_gpu_set_tuning
-  // with a block size); results do not depend on it.
-  static constexpr uint32_t kWatchEvery = 8;
+  // spreads), and so may the faster schedule (with a refill kernel compiled
+  // for 6 waves per SIMD, refill was 24% faster on the 64^3 cube over steps
+  // 3-12 and pass 12% faster over steps 13-50; at 5 waves refill leads
+  // throughout).  Every watch_every steps one step runs in the other
+  // schedule; its rate is compared with the mean of the steps either side of
+  // it (which cancels a steady trend) and the engine switches when the other
+  // is more than 3% faster.  A probe that loses by more than 10% doubles the
+  // interval (16 steps at first, at most 128), a switch resets it: a problem
+  // runs hundreds to thousands of steps and its workload drifts over tens,
+  // while each probe of a clearly slower schedule costs its margin.  On
+  // unless the launch was fixed (storm_gpu_set_tuning with a block size);
+  // results do not depend on it.
+  static constexpr uint32_t kWatchFirst = 16, kWatchMax = 128;
   bool watch = true;
   int32_t watch_state = 0;  // 0: normal, 1: this step probes, 2: the step after the probe
-  uint32_t watch_n = 0;
+  uint32_t watch_n = 0, watch_every = kWatchFirst;
   double watch_prev = 0.0, watch_alt = 0.0;
   void watch_reset() {
     watch_state = 0;
     watch_n = 0;
+    watch_every = kWatchFirst;
     watch_prev = 0.0;
   }
   static int32_t other(int32_t s) {
@@ -329,7 +335,7 @@ struct EngineBase {
       apply(tune_seq[tune_pos]);
       return;
     }
-    if (watch && watch_state == 0 && watch_n >= kWatchEvery && watch_prev > 0.0) {
+    if (watch && watch_state == 0 && watch_n >= watch_every && watch_prev > 0.0) {
       watch_state = 1;
       schedule = other(schedule);
     }
@@ -365,8 +371,10 @@ struct EngineBase {
       if (watch_alt > 1.03 * around) {
         schedule = other(schedule);
         watch_prev = watch_alt;
+        watch_every = kWatchFirst;
       } else {
         watch_prev = rate;
+        if (watch_alt < 0.9 * around) watch_every = std::min(2 * watch_every, kWatchMax);
       }
       watch_state = 0;
       watch_n = 0;
@@ -576,14 +584,14 @@ struct Engine final : EngineBase {
       if (!(nu_ref > 0.0)) throw EngineError(1, "groups.nu_ref must be positive");
     }
     // The starting schedule (the watch corrects it as the problem evolves),
-    // measured over steps 3-12 at 256 threads: Cartesian meshes refill lanes
-    // from ranges of 8 chunks per wavefront (the 64^3 cube: 22.4 against 18.0
-    // G segments/s mixed, 13.5 against 8.2 FP64; the step slab: 18.3 against
-    // 16.5 mixed); the mixed engine on polyhedral meshes runs passes (the
-    // Voronoi problem: 10.3 against 8.5), the FP64 engine refills there too.
-    schedule = (!unstructured || precision == STORM_GPU_FP64) ? STORM_GPU_SCHEDULE_REFILL : STORM_GPU_SCHEDULE_PASS;
-    // refill on polyhedral meshes from 4 chunks (the batch of slabs: 28.5
-    // against 26.3 at 8 chunks)
+    // measured over the first steps at 256 threads: lanes refill from ranges
+    // of 8 chunks per wavefront (the 64^3 cube, steps 3-12: 25.8 against 18.0
+    // G segments/s for pass in the mixed engine, 13.5 against 8.2 in FP64;
+    // the step slab 19.2 against 16.4), 4 for the mixed engine on polyhedral
+    // meshes (the batch of slabs: 36.0 against 26.5); the mixed multigroup
+    // problem on a Voronoi mesh runs passes (10.9 against 9.8).
+    const bool mg_poly = unstructured && precision != STORM_GPU_FP64 && p->groups.num_groups >= 2;
+    schedule = mg_poly ? STORM_GPU_SCHEDULE_PASS : STORM_GPU_SCHEDULE_REFILL;
     if (unstructured && precision != STORM_GPU_FP64) chunks_of[1] = 4;
     if (bnd.source_face != -1 && bnd.source_face != STORM_XLO)
       throw EngineError(1, "source_face must be -1 or STORM_XLO (an x+ source is source_xhi)");

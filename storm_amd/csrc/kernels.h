@@ -1099,8 +1099,22 @@ __device__ __forceinline__ void transport_refill(STORM_TRANSPORT_ARGS, CensusSoA
   }
 }
 
+// Waves per SIMD the refill kernel is compiled for.  FP32: 5 (96 VGPRs),
+// measured 17% faster than 6 on the cube (whose spills it reloads in its
+// per-segment loop: 77 B of HBM traffic per segment at 6) and 9% faster than
+// 4; on the step slab 4 and 5 are 8% and 5% faster than 6.  FP64: the
+// compiler's choice.
+template <typename Real>
+struct RefillWaves {
+  static constexpr int kMin = 1;
+};
+template <>
+struct RefillWaves<float> {
+  static constexpr int kMin = 5;
+};
+
 template <typename Real, uint32_t F, typename MeshT = MeshView<Real>>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(TransportWaves<Real>::kMin, 8))) void
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(RefillWaves<Real>::kMin, 8))) void
 k_transport_refill(STORM_TRANSPORT_ARGS, CensusSoA<Real> stage) {
   transport_refill<Real, F, MeshT>(n_in, in, out, n_out, cen, n_cen, mesh, xs, acc, k, max_segments, t, stats,
                                    cache_slots, cache_fields, chunks, stage);
