@@ -62,7 +62,7 @@ C2_KNOBS = dict(source_per_face=2400, comb_nmin=8)
 # HBM bytes per transport launch, measured by rocprofv3 (FETCH_SIZE and
 # WRITE_SIZE, scaled by the same counters on a plain copy kernel) on this
 # benchmark's own default command; the summary holds the figures and the command.
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_transport_c2_literal_counters.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_transport_refill_c2_literal_counters.md")
 
 
 # the mixed engine's Cartesian transport kernels with the random-walk / DDMC
