@@ -978,20 +978,47 @@ struct Engine final : EngineBase {
     m.spec = spectrum();
     return m;
   }
-  // Tally-cache slots of a transport kernel at this block size and chunking:
-  // the share of the CU's LDS of each block its registers let run there.
+  // Per transport kernel and block size: the blocks its registers let run
+  // on one CU, and the tally-cache slots (the share of the CU's LDS of each
+  // such block) for each chunking.
   static int chunk_index(int32_t k) { return k == 1 ? 0 : (k == 2 ? 1 : (k == 4 ? 2 : (k == 8 ? 3 : 4))); }
   static int block_index(int32_t b) { return b == 64 ? 0 : (b == 128 ? 1 : (b == 256 ? 2 : 3)); }
-  uint32_t slots_cache[2][2][4][4][5] = {};  // [refill][polyhedral][F][block][chunks]
+  struct KernelFit {
+    uint32_t per_cu = 0;
+    uint32_t slots[5] = {};
+  };
+  KernelFit fit_cache[2][2][4][4];  // [refill][polyhedral][F][block]
+  int n_cu = 0;
   template <typename K>
-  uint32_t cache_slots_for(K kernel, bool refill, uint32_t F, int32_t b, uint32_t nf) {
-    uint32_t &r = slots_cache[refill ? 1 : 0][unstructured ? 1 : 0][F][block_index(b)][chunk_index(chunks_per_wave())];
-    if (!r) {
+  KernelFit &fit_for(K kernel, bool refill, uint32_t F, int32_t b) {
+    KernelFit &f = fit_cache[refill ? 1 : 0][unstructured ? 1 : 0][F][block_index(b)];
+    if (!f.per_cu) {
       int per_cu = 0;
       HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, b, 0));
-      r = cache_slots(static_cast<uint32_t>(b) * static_cast<uint32_t>(chunks_per_wave()), nf,
-                      static_cast<uint32_t>(std::max(per_cu, 1)));
+      f.per_cu = static_cast<uint32_t>(std::max(per_cu, 1));
+      if (!n_cu) {
+        int dev = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        HIPCHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        n_cu = std::max(n_cu, 1);
+      }
     }
+    return f;
+  }
+  // The chunks per wavefront of a launch over n_in packets: the schedule's,
+  // halved while the launch would have fewer blocks than the device holds at
+  // once (a small step in large block ranges leaves CUs idle: the 100-cell
+  // slab ran 5 times slower in ranges of 8 chunks).
+  int32_t launch_chunks(const KernelFit &f, uint64_t n_in) const {
+    int32_t ch = chunks_per_wave();
+    const uint64_t resident = static_cast<uint64_t>(f.per_cu) * static_cast<uint64_t>(n_cu);
+    while (ch > 1 && (n_in + static_cast<uint64_t>(block) * ch - 1) / (static_cast<uint64_t>(block) * ch) < resident)
+      ch >>= 1;
+    return ch;
+  }
+  uint32_t slots_of(KernelFit &f, int32_t ch, uint32_t nf) {
+    uint32_t &r = f.slots[chunk_index(ch)];
+    if (!r) r = cache_slots(static_cast<uint32_t>(block) * static_cast<uint32_t>(ch), nf, f.per_cu);
     return r;
   }
 
@@ -1001,15 +1028,18 @@ struct Engine final : EngineBase {
   void launch_transport_m(uint64_t n_in, PacketBufs<Real> &src, PacketBufs<Real> &dst, const MeshT &mesh,
                           const AccelView<Real> &av, const StepConsts &kc) {
     const uint32_t nf = (precision == STORM_GPU_MIXED && groups < 2) ? 1u : 2u;
-    const uint32_t chunks = static_cast<uint32_t>(chunks_per_wave()) * static_cast<uint32_t>(block / 64);
+    const bool refill = schedule == STORM_GPU_SCHEDULE_REFILL;
+    KernelFit &f = refill ? fit_for(k_transport_refill<Real, F, MeshT>, true, F, block)
+                          : fit_for(k_transport<Real, F, MeshT>, false, F, block);
+    const int32_t ch = launch_chunks(f, n_in);
+    const uint32_t slots = slots_of(f, ch, nf);
+    const uint32_t chunks = static_cast<uint32_t>(ch) * static_cast<uint32_t>(block / 64);
     const uint32_t grid = blocks_for(n_in, 64u * chunks);
-    if (schedule == STORM_GPU_SCHEDULE_REFILL) {
-      const uint32_t slots = cache_slots_for(k_transport_refill<Real, F, MeshT>, true, F, block, nf);
+    if (refill) {
       k_transport_refill<Real, F, MeshT><<<grid, block, cache_bytes(slots, nf), stream>>>(
           static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mesh,
           xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks, cstage.soa());
     } else {
-      const uint32_t slots = cache_slots_for(k_transport<Real, F, MeshT>, false, F, block, nf);
       k_transport<Real, F, MeshT><<<grid, block, cache_bytes(slots, nf), stream>>>(
           static_cast<uint32_t>(n_in), src.soa(), dst.soa(), counters_u32.p, cpre.soa(), counters_u32.p + 1, mesh,
           xs.p, av, kc, seg_per_pass, tal, leaving.p, slots, nf, chunks);
