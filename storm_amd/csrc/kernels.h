@@ -997,8 +997,8 @@ k_transport(STORM_TRANSPORT_ARGS) {
 // copies them there.  Appending them one or two at a time as lanes finish
 // would put an atomic per census packet on the one census counter that every
 // wavefront of the launch shares.
-// Idle lanes that take new packets together: 1 to 8 measured within 1% of
-// each other, 16 and 32 up to 2% slower (cube, step slab, batch of slabs).
+// Idle lanes that take new packets together: 2, 4 and 16 measured within 1%
+// of each other (cube, step slab, batch of slabs).
 constexpr uint32_t kRefillIdle = 4;
 template <typename Real, uint32_t F, typename MeshT>
 __device__ __forceinline__ void transport_refill(STORM_TRANSPORT_ARGS, CensusSoA<Real> stage) {
