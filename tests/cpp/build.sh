@@ -15,3 +15,7 @@ mv "$here/_build/plugin_checks.tmp" "$here/_build/plugin_checks"
 g++ -std=c++17 -O2 -Wall -Wextra -I"$root/include" -o "$here/_build/manager_groups.tmp" "$here/manager_groups.cpp" \
   -L"$root/storm_amd/_lib" -lstorm_amd -Wl,-rpath,'$ORIGIN/../../../storm_amd/_lib'
 mv "$here/_build/manager_groups.tmp" "$here/_build/manager_groups"
+# the launch policy alone (host code, no library)
+g++ -std=c++17 -O2 -Wall -Wextra -I"$root/include" -I"$root/storm_amd/csrc" -o "$here/_build/launch_policy.tmp" \
+  "$here/launch_policy.cpp"
+mv "$here/_build/launch_policy.tmp" "$here/_build/launch_policy"
