@@ -143,8 +143,9 @@ int storm_gpu_step(storm_gpu_engine *eng, double dt);
  * the library (one per GPU, held until the process ends); it calls no
  * hipMalloc, hipFree or other allocating routine.  Limits, each a clean error
  * (status 4) that leaves the census and material state unchanged: the step's
- * transient buffers must fit the 64 MiB (about 2.5e5 packets per step in the
- * mixed engine, fewer in FP64 or multigroup), and the census must fit the
+ * transient buffers must fit the 64 MiB (about 2e5 packets per step in the
+ * mixed engine's refill schedule, which stages the census, 2.5e5 in its pass
+ * schedule, fewer in FP64 or multigroup), and the census must fit the
  * engine's census buffer, which only storm_gpu_step and
  * storm_gpu_import_census grow (at least doubling).  All engines on one GPU
  * share the array: calls that overlap on one GPU race; serialise them.
