@@ -457,6 +457,8 @@ struct Engine final : EngineBase {
     const bool mg_poly = unstructured && precision != STORM_GPU_FP64 && p->groups.num_groups >= 2;
     schedule = mg_poly ? STORM_GPU_SCHEDULE_PASS : STORM_GPU_SCHEDULE_REFILL;
     if (unstructured && precision != STORM_GPU_FP64) chunks_of[1] = 4;
+    // its passes from 1 chunk per wavefront (11.1 against 10.9 at 2)
+    if (mg_poly) chunks_of[0] = 1;
     if (bnd.source_face != -1 && bnd.source_face != STORM_XLO)
       throw EngineError(1, "source_face must be -1 or STORM_XLO (an x+ source is source_xhi)");
     if (bnd.source_xhi && !(bnd.source_temperature_xhi >= 0.0)) throw EngineError(1, "invalid x+ source temperature");
