@@ -212,15 +212,6 @@ struct CensusBufs {
   size_t cap() const { return meta.n; }
 };
 
-// Sum of two (lo, hi) limb pairs with lo < 2^36, normalised the same way (the
-// representation is unique, so the operator is associative).
-struct PairAdd {
-  __host__ __device__ ulonglong2 operator()(const ulonglong2 &a, const ulonglong2 &b) const {
-    const unsigned long long lo = a.x + b.x;
-    return make_ulonglong2(lo & fx::kLoMask, a.y + b.y + (lo >> fx::kLoBits));
-  }
-};
-
 // ---------------------------------------------------------------------------
 struct EngineBase : LaunchPolicy {
   virtual ~EngineBase() = default;
@@ -284,7 +275,7 @@ struct Engine final : EngineBase {
   DevBuf<unsigned long long> rep_u64;
   DevBuf<double> rep_f64;
   DevBuf<uint32_t> rep_u32;
-  DevBuf<ulonglong2> limbs_sorted, cum;
+  DevBuf<uint64_t> limbs_sorted, cum;  // sorted census weights in quanta, their prefix sums
   DevBuf<CombCell> comb_cells;
   DevBuf<int32_t> qexp;
   // scalars read back by the host: gathered on the device, one copy into
@@ -1460,8 +1451,8 @@ struct Engine final : EngineBase {
                                                                 limbs_sorted.p, run_start.p, run_end.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
-        HIPCHECK(rocprim::inclusive_scan(tp, bytes, limbs_sorted.p, cum.p, static_cast<size_t>(n), PairAdd(),
-                                         stream));
+        HIPCHECK(rocprim::inclusive_scan(tp, bytes, limbs_sorted.p, cum.p, static_cast<size_t>(n),
+                                         rocprim::plus<uint64_t>(), stream));
       };
       ensure_temp(q);
     }

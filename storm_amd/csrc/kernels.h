@@ -1327,13 +1327,12 @@ template <typename Real>
 __global__ __launch_bounds__(256) void k_comb_sorted(uint32_t n, const uint32_t *__restrict__ keys,
                                                      const uint64_t *__restrict__ vals, CombUnits cu,
                                                      CensusSoA<Real> cen, int32_t eq, Morton mo, MeshDims d,
-                                                     ulonglong2 *__restrict__ limbs, uint32_t *__restrict__ start,
+                                                     uint64_t *__restrict__ wq, uint32_t *__restrict__ start,
                                                      uint32_t *__restrict__ end) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t ukey = keys[i] >> cu.pbits;
-  fx::Limbs l = fx::limbs_of(cen.w[static_cast<uint32_t>(vals[i])], eq);
-  limbs[i] = make_ulonglong2(l.lo, l.hi);
+  wq[i] = fx::value_of(fx::limbs_of(cen.w[static_cast<uint32_t>(vals[i])], eq));
   if (cu.dropped(keys[i], mo)) return;
   uint32_t c;
   const uint32_t u = cu.unit(keys[i], mo, d, c);
@@ -1367,10 +1366,11 @@ __global__ __launch_bounds__(256) void k_inst_sum(const uint32_t *__restrict__ f
   if (threadIdx.x == 0) out[k] = sh[0];
 }
 
-// Exact value of an inclusive prefix sum of limb pairs.
-__device__ __forceinline__ unsigned __int128 cum_value(const ulonglong2 *cum, uint32_t i) {
-  const ulonglong2 a = cum[i];
-  return (static_cast<unsigned __int128>(a.y) << fx::kLoBits) + static_cast<unsigned __int128>(a.x);
+// Inclusive prefix sum of the sorted census weights in quanta (below 2^62,
+// as is every sum of weights on the step's quantum), widened for the
+// 128-bit arithmetic of the comb.
+__device__ __forceinline__ unsigned __int128 cum_value(const uint64_t *cum, uint32_t i) {
+  return static_cast<unsigned __int128>(cum[i]);
 }
 
 // Largest integer <= v with at most 53 significant bits (exactly representable
@@ -1436,7 +1436,7 @@ __device__ void comb_unit_setup(CombCell &cc, unsigned __int128 W, uint32_t coun
 __global__ __launch_bounds__(256) void k_inst_census(const uint32_t *__restrict__ first,
                                                      const uint32_t *__restrict__ start,
                                                      const uint32_t *__restrict__ end,
-                                                     const ulonglong2 *__restrict__ cum, int32_t eq,
+                                                     const uint64_t *__restrict__ cum, int32_t eq,
                                                      double *__restrict__ wtot) {
   __shared__ uint32_t lo_s[256], hi_s[256];
   const uint32_t k = blockIdx.x;
@@ -1475,7 +1475,7 @@ __global__ __launch_bounds__(256) void k_inst_census(const uint32_t *__restrict_
 // for the energy ledger.
 __global__ __launch_bounds__(256) void k_comb_cells(uint32_t N, const uint32_t *__restrict__ start,
                                                     const uint32_t *__restrict__ end,
-                                                    const ulonglong2 *__restrict__ cum, uint32_t n, uint64_t nmin_all,
+                                                    const uint64_t *__restrict__ cum, uint32_t n, uint64_t nmin_all,
                                                     double factor_all, int32_t eq, int32_t comb_on, uint64_t seed,
                                                     uint32_t step, CombCell *__restrict__ cells,
                                                     unsigned long long *__restrict__ W,
@@ -1536,7 +1536,7 @@ constexpr uint32_t kMaxStratGroups = 64;
 
 __global__ __launch_bounds__(256) void k_comb_strat(uint32_t N, uint32_t groups, const uint32_t *__restrict__ start,
                                                     const uint32_t *__restrict__ end,
-                                                    const ulonglong2 *__restrict__ cum, uint32_t n, uint64_t nmin,
+                                                    const uint64_t *__restrict__ cum, uint32_t n, uint64_t nmin,
                                                     double factor, uint64_t min_per, int32_t eq, uint64_t seed,
                                                     uint32_t step, CombCell *__restrict__ bins,
                                                     unsigned long long *__restrict__ W,
@@ -1665,7 +1665,7 @@ __global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *
                                                     const CombCell *__restrict__ cells,
                                                     const uint32_t *__restrict__ start,
                                                     const uint32_t *__restrict__ end,
-                                                    const ulonglong2 *__restrict__ cum, int32_t eq, Morton mo,
+                                                    const uint64_t *__restrict__ cum, int32_t eq, Morton mo,
                                                     MeshDims d, uint32_t *__restrict__ ncopy) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
