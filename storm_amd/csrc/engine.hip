@@ -268,7 +268,8 @@ struct Engine final : EngineBase {
   bool have_host_sa = false, have_host_ss = false, have_host_cv = false;
   DevBuf<CellXS<Real>> xs;
   DevBuf<uint32_t> nphot, nphot_z, err, run_start, run_end, ncopy, keys, keys_out, counters_u32;
-  DevBuf<uint64_t> offsets, vals, vals_out, offs64, nphot64;
+  DevBuf<uint64_t> offsets, offs64, nphot64;
+  DevBuf<uint32_t> vals, vals_out;  // comb: census indices sorted with the keys
   DevBuf<unsigned long long> tally_u64, emit_limbs, requant, leaving;
   DevBuf<double> tally_f64;
   // per-XCD replicas 1..kTallyReplicas-1 of the per-visit tallies
@@ -1428,7 +1429,9 @@ struct Engine final : EngineBase {
     const uint32_t salt = static_cast<uint32_t>(seed) * 0x2545F491u + step_index * 0x9E3779B9u;
     // 32-bit keys: the cell's Z-order code (morton.total bits, and 2^total
     // for dropped packets), the photon group for the stratified comb
-    // (gbits), then the top pbits bits of the rank; ties by k_comb_ties
+    // (gbits), then the top pbits bits of the rank; ties by k_comb_ties.
+    // (24-bit keys save one radix pass but leave so few rank bits that the
+    // ties in the crowded cells of a cold start took 0.8 ms to order.)
     const uint32_t G = strat_groups, gbits = strat_gbits;
     const uint32_t pbits = 31u - morton.total - gbits;
     const CombUnits cu{pbits, gbits, G};
@@ -1443,7 +1446,7 @@ struct Engine final : EngineBase {
       ensure_temp(q);
     }
     k_comb_ties<<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p,
-                                                        ((1u << morton.total) << gbits) << pbits);
+                                                        ((1u << morton.total) << gbits) << pbits, cpre.meta.p, salt);
     // runs of cells (bins) over the sorted census, weight limbs and their
     // prefix sums
     zero_u32(run_end.p, static_cast<size_t>(N) * G);

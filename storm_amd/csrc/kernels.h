@@ -1239,13 +1239,13 @@ template <typename Real>
 // so each cell's run splits into runs of its groups.
 __global__ __launch_bounds__(256) void k_comb_keys(uint32_t n, CensusSoA<Real> cen, Morton mo, uint32_t salt,
                                                    uint32_t pbits, uint32_t *__restrict__ keys,
-                                                   uint64_t *__restrict__ vals, uint32_t gbits,
+                                                   uint32_t *__restrict__ vals, uint32_t gbits,
                                                    const Real *__restrict__ bounds, int32_t groups) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint2 m = cen.meta[i];
   const uint32_t rank = perm32(m.y, salt);
-  vals[i] = (static_cast<uint64_t>(rank) << 32) | i;
+  vals[i] = i;
   // weight-0 packets are dropped (CombPopulationControl.hpp:136-137)
   const bool drop = cen.w[i] == 0.0;
   const uint32_t code = drop ? (1u << mo.total)
@@ -1258,26 +1258,31 @@ __global__ __launch_bounds__(256) void k_comb_keys(uint32_t n, CensusSoA<Real> c
   keys[i] = pbits ? (key << pbits) | (rank >> (32u - pbits)) : key;
 }
 
-// Orders each run of equal sort keys by the full rank (the value's high
-// word; ranks are distinct), so that the order inside a cell is the order of
-// the full 32-bit ranks whatever order the census was written in.  Runs are
-// short (a few packets share the top rank bits of a cell); long ones, from a
-// cell holding far more packets than 2^pbits, are heap-sorted.
+// Orders each run of equal sort keys by the full rank of its packets
+// (perm32 of the packet id; ranks are distinct), so that the order inside a
+// cell is the order of the full 32-bit ranks whatever order the census was
+// written in.  The values are census indices; a packet's rank is recomputed
+// from its id where the sort compares it.  Runs are short (a few packets
+// share the top rank bits of a cell); long ones, from a cell holding far more
+// packets than 2^pbits, are heap-sorted.
 __global__ __launch_bounds__(256) void k_comb_ties(uint32_t n, const uint32_t *__restrict__ keys,
-                                                   uint64_t *__restrict__ vals, uint32_t dropped_key) {
+                                                   uint32_t *__restrict__ vals, uint32_t dropped_key,
+                                                   const uint2 *__restrict__ meta, uint32_t salt) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i + 1u >= n) return;
   const uint32_t k = keys[i];
   if (k >= dropped_key || keys[i + 1u] != k || (i > 0u && keys[i - 1u] == k)) return;
   uint32_t j = i + 2u;
   while (j < n && keys[j] == k) ++j;
-  uint64_t *v = vals + i;
+  uint32_t *v = vals + i;
+  auto rank = [&](uint32_t idx) { return perm32(meta[idx].y, salt); };
   const uint32_t L = j - i;
   if (L <= 32u) {
     for (uint32_t a = 1; a < L; ++a) {
-      const uint64_t x = v[a];
+      const uint32_t x = v[a];
+      const uint32_t rx = rank(x);
       uint32_t b = a;
-      while (b > 0u && v[b - 1u] > x) {
+      while (b > 0u && rank(v[b - 1u]) > rx) {
         v[b] = v[b - 1u];
         --b;
       }
@@ -1285,14 +1290,22 @@ __global__ __launch_bounds__(256) void k_comb_ties(uint32_t n, const uint32_t *_
     }
     return;
   }
-  // heap sort (ascending)
+  // heap sort by rank (ascending)
   auto sift = [&](uint32_t root, uint32_t len) {
-    const uint64_t x = v[root];
+    const uint32_t x = v[root];
+    const uint32_t rx = rank(x);
     for (;;) {
       uint32_t ch = 2u * root + 1u;
       if (ch >= len) break;
-      if (ch + 1u < len && v[ch + 1u] > v[ch]) ++ch;
-      if (v[ch] <= x) break;
+      uint32_t rc = rank(v[ch]);
+      if (ch + 1u < len) {
+        const uint32_t r1 = rank(v[ch + 1u]);
+        if (r1 > rc) {
+          ++ch;
+          rc = r1;
+        }
+      }
+      if (rc <= rx) break;
       v[root] = v[ch];
       root = ch;
     }
@@ -1300,16 +1313,16 @@ __global__ __launch_bounds__(256) void k_comb_ties(uint32_t n, const uint32_t *_
   };
   for (uint32_t r = L / 2u; r-- > 0u;) sift(r, L);
   for (uint32_t e = L - 1u; e > 0u; --e) {
-    const uint64_t t = v[0];
+    const uint32_t t = v[0];
     v[0] = v[e];
     v[e] = t;
     sift(0u, e);
   }
 }
 
-// Over the sorted census: the (lo, hi) limbs of each weight, for the prefix
-// sums, and each cell's run [start, end) of sorted indices (end stays 0 for a
-// cell with no packet).
+// Over the sorted census: each weight in quanta, for the prefix sums, and
+// each cell's run [start, end) of sorted indices (end stays 0 for a cell with
+// no packet).
 // The comb's unit of a sorted key: the cell, or with gbits > 0 (the
 // stratified comb) the (cell, group) bin, numbered cell * groups + group.
 struct CombUnits {
@@ -1325,14 +1338,14 @@ struct CombUnits {
 
 template <typename Real>
 __global__ __launch_bounds__(256) void k_comb_sorted(uint32_t n, const uint32_t *__restrict__ keys,
-                                                     const uint64_t *__restrict__ vals, CombUnits cu,
+                                                     const uint32_t *__restrict__ vals, CombUnits cu,
                                                      CensusSoA<Real> cen, int32_t eq, Morton mo, MeshDims d,
                                                      uint64_t *__restrict__ wq, uint32_t *__restrict__ start,
                                                      uint32_t *__restrict__ end) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t ukey = keys[i] >> cu.pbits;
-  wq[i] = fx::value_of(fx::limbs_of(cen.w[static_cast<uint32_t>(vals[i])], eq));
+  wq[i] = fx::value_of(fx::limbs_of(cen.w[vals[i]], eq));
   if (cu.dropped(keys[i], mo)) return;
   uint32_t c;
   const uint32_t u = cu.unit(keys[i], mo, d, c);
@@ -1660,7 +1673,7 @@ __device__ __forceinline__ uint64_t comb_teeth(double C, double Wc, double targe
 // sorted order; [start, end) is each cell's run.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *__restrict__ keys,
-                                                    const uint64_t *__restrict__ vals, CombUnits cu,
+                                                    const uint32_t *__restrict__ vals, CombUnits cu,
                                                     CensusSoA<Real> cen,
                                                     const CombCell *__restrict__ cells,
                                                     const uint32_t *__restrict__ start,
@@ -1676,7 +1689,7 @@ __global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *
   uint32_t c;
   const uint32_t u = cu.unit(keys[i], mo, d, c);
   const CombCell cc = cells[u];
-  const double w = cen.w[static_cast<uint32_t>(vals[i])];
+  const double w = cen.w[vals[i]];
   uint32_t k = 1;
   if (cc.mode == 3u) {
     k = 0;  // a bin the stratified comb drops
@@ -1705,7 +1718,7 @@ __global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *
 // packet into the same two words.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_comb_write(uint32_t n, const uint32_t *__restrict__ keys,
-                                                    const uint64_t *__restrict__ vals, CombUnits cu,
+                                                    const uint32_t *__restrict__ vals, CombUnits cu,
                                                     CensusSoA<Real> cen,
                                                     const CombCell *__restrict__ cells,
                                                     const uint32_t *__restrict__ ncopy,
@@ -1720,7 +1733,7 @@ __global__ __launch_bounds__(256) void k_comb_write(uint32_t n, const uint32_t *
     const uint32_t k = ncopy[i];
     if (k) {
       const CombCell cc = cells[u];
-      const uint32_t src = static_cast<uint32_t>(vals[i]);
+      const uint32_t src = vals[i];
       const double w = cen.w[src];
       double wout = w;
       if (cc.mode == 2u) {
