@@ -61,8 +61,8 @@ def test_marshak_wave_matches_reference_profile(problem):
     x = spec.centers()[0]
     l1 = P.marshak_l1(x, c["T"], ref)
     l1_hot, shift, h = P.marshak_shape(x, c["T"], ref)
-    verdict = "PASS" if 0.0 <= l1 < 0.10 else "WARN"
-    print(f"marshak {problem}: Tgas relative L1 = {l1:.4f} ({verdict}, reference bar 0.10); "
+    status = "PASS" if 0.0 <= l1 < 0.10 else "WARN"
+    print(f"marshak {problem}: Tgas relative L1 = {l1:.4f} ({status}, reference bar 0.10); "
           f"heated-region L1 = {l1_hot:.4f}; front shift = {shift / h:+.2f} cells", flush=True)
     assert l1 >= 0.0
     if problem != 2:
