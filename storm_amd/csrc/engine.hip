@@ -277,6 +277,7 @@ struct Engine final : EngineBase {
   DevBuf<double> rep_f64;
   DevBuf<uint32_t> rep_u32;
   DevBuf<uint64_t> limbs_sorted, cum;  // sorted census weights in quanta, their prefix sums
+  DevBuf<double> wsorted;              // sorted census weights
   DevBuf<CombCell> comb_cells;
   DevBuf<int32_t> qexp;
   // scalars read back by the host: gathered on the device, one copy into
@@ -767,6 +768,27 @@ struct Engine final : EngineBase {
   void zero_f64(double *p, size_t n) {
     if (n) k_fill_f64<<<blocks_for(n, 256), 256, 0, stream>>>(p, n, 0.0);
   }
+  // Buffers to zero with one k_zero_ranges launch (a full batch is launched
+  // as the next one is added; zero_flush launches the rest).
+  struct ZeroBatch {
+    ZeroRanges r{};
+    uint32_t k = 0;
+    uint64_t most = 0;  // largest range, in words
+  };
+  void zero_add(ZeroBatch &z, void *p, size_t bytes) {
+    if (!p || bytes < 4) return;
+    if (z.k == static_cast<uint32_t>(kZeroRanges)) zero_flush(z);
+    z.r.p[z.k] = static_cast<uint32_t *>(p);
+    z.r.n[z.k] = bytes / 4;  // every buffer here holds 4- or 8-byte words
+    z.most = std::max<uint64_t>(z.most, bytes / 4);
+    ++z.k;
+  }
+  void zero_flush(ZeroBatch &z) {
+    if (!z.k) return;
+    const uint32_t bx = static_cast<uint32_t>(std::min<uint64_t>((z.most + 255) / 256, 2048));
+    k_zero_ranges<<<dim3(bx, z.k), 256, 0, stream>>>(z.r);
+    z = ZeroBatch{};
+  }
 
   MatDev matdev() const {
     MatDev m;
@@ -1174,17 +1196,22 @@ struct Engine final : EngineBase {
     }
     counters.start_particles = n_total;
 
-    // reset tallies
-    zero_u64(tally_u64.p, tally_u64.n);
-    if (precision == STORM_GPU_FP64) zero_f64(tally_f64.p, tally_f64.n);
-    zero_u32(tal.segs, 2 * static_cast<size_t>(N));
-    if (tal.eg) zero_u64(tal.eg, static_cast<size_t>(N) * tal.eg_groups);
-    if (tal.eg_fp) zero_f64(tal.eg_fp, static_cast<size_t>(N) * tal.eg_groups);
-    zero_u32(rep_u32.p, rep_u32.n);
-    if (precision == STORM_GPU_FP64) zero_f64(rep_f64.p, rep_f64.n);
-    else zero_u64(rep_u64.p, rep_u64.n);
-    zero_u64(requant.p, requant.n);
-    zero_u64(leaving.p, 5);
+    // reset tallies (one launch)
+    {
+      ZeroBatch z;
+      zero_add(z, tally_u64.p, tally_u64.n * sizeof(unsigned long long));
+      if (precision == STORM_GPU_FP64) zero_add(z, tally_f64.p, tally_f64.n * sizeof(double));
+      zero_add(z, tal.segs, 2 * static_cast<size_t>(N) * sizeof(uint32_t));
+      zero_add(z, tal.eg, static_cast<size_t>(N) * tal.eg_groups * sizeof(unsigned long long));
+      zero_add(z, tal.eg_fp, static_cast<size_t>(N) * tal.eg_groups * sizeof(double));
+      zero_add(z, rep_u32.p, rep_u32.n * sizeof(uint32_t));
+      if (precision == STORM_GPU_FP64) zero_add(z, rep_f64.p, rep_f64.n * sizeof(double));
+      else zero_add(z, rep_u64.p, rep_u64.n * sizeof(unsigned long long));
+      zero_add(z, requant.p, requant.n * sizeof(unsigned long long));
+      zero_add(z, leaving.p, 5 * sizeof(unsigned long long));
+      zero_add(z, counters_u32.p, 2 * sizeof(uint32_t));
+      zero_flush(z);
+    }
 
     // packets: [census | emission | boundary]
     pa.ensure(n_total);
@@ -1244,8 +1271,8 @@ struct Engine final : EngineBase {
     uint64_t n_in = n_total;
     bool ping = true;
     double t_transport = 0.0, t_max = 0.0;
-    uint32_t n_cen_total = 0;
-    zero_u32(counters_u32.p, 2);
+    uint32_t n_cen_total = 0;  // both counters were zeroed with the tallies
+    bool first_pass = true;
     // A packet may take at most kMaxSegmentsPerStep segments in one step (a
     // history that long is a problem set-up the random walk should cover, and
     // one lane running it would keep the GPU busy for minutes).
@@ -1260,7 +1287,8 @@ struct Engine final : EngineBase {
       }
       PacketBufs<Real> &src = ping ? pa : pb;
       PacketBufs<Real> &dst = ping ? pb : pa;
-      zero_u32(counters_u32.p, 1);  // n_out
+      if (!first_pass) zero_u32(counters_u32.p, 1);  // n_out
+      first_pass = false;
       HIPCHECK(hipEventRecord(ev0, stream));
       launch_transport(n_in, src, dst, mv, av, kc);
       HIPCHECK(hipEventRecord(ev1, stream));
@@ -1375,12 +1403,13 @@ struct Engine final : EngineBase {
     f(vals);
     f(vals_out);
     f(limbs_sorted);
+    f(wsorted);
     f(cum);
     f(ncopy);
     f(offs64);
     f(temp);
   }
-  static constexpr size_t kTransientBufs = 6 + 2 * 5 + 4 + 4 + 9;
+  static constexpr size_t kTransientBufs = 6 + 2 * 5 + 4 + 4 + 10;
 
   void step_noalloc(double dt) override {
     if (keep_inputs)
@@ -1427,6 +1456,7 @@ struct Engine final : EngineBase {
     vals.ensure(n);
     vals_out.ensure(n);
     limbs_sorted.ensure(n);
+    wsorted.ensure(n);
     cum.ensure(n);
     ncopy.ensure(n);
     offs64.ensure(n);
@@ -1455,7 +1485,7 @@ struct Engine final : EngineBase {
     // prefix sums
     zero_u32(run_end.p, static_cast<size_t>(N) * G);
     k_comb_sorted<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, cu, cpre.soa(), eq, morton, d,
-                                                                limbs_sorted.p, run_start.p, run_end.p);
+                                                                wsorted.p, limbs_sorted.p, run_start.p, run_end.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
         HIPCHECK(rocprim::inclusive_scan(tp, bytes, limbs_sorted.p, cum.p, static_cast<size_t>(n),
@@ -1475,8 +1505,8 @@ struct Engine final : EngineBase {
       k_comb_cells<<<blocks_for(N, 256), 256, 0, stream>>>(N, run_start.p, run_end.p, cum.p, n, imc.comb_nmin,
                                                            imc.comb_factor, eq, imc.comb ? 1 : 0, seed, step_index,
                                                            comb_cells.p, tal.cpre, tal.rem, instview());
-    k_comb_count<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, cu, cpre.soa(), comb_cells.p,
-                                                               run_start.p, run_end.p, cum.p, eq, morton, d, ncopy.p);
+    k_comb_count<<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, cu, wsorted.p, comb_cells.p, run_start.p,
+                                                         run_end.p, cum.p, eq, morton, d, ncopy.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
         HIPCHECK(rocprim::exclusive_scan(tp, bytes, ncopy.p, offs64.p, uint64_t(0), static_cast<size_t>(n),
@@ -1489,7 +1519,7 @@ struct Engine final : EngineBase {
     if (n_out >= (1ull << 32) - 1) throw EngineError(4, "census larger than 2^32 packets");
     if (n_out == 0) return 0;
     cnext.ensure(n_out);
-    k_comb_write<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, cu, cpre.soa(),
+    k_comb_write<Real><<<blocks_for(n, 256), 256, 0, stream>>>(n, keys_out.p, vals_out.p, cu, cpre.soa(), wsorted.p,
                                                                comb_cells.p, ncopy.p, offs64.p, cnext.soa(), eq, morton,
                                                                d, tal);
     return n_out;

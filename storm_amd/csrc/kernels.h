@@ -1320,9 +1320,10 @@ __global__ __launch_bounds__(256) void k_comb_ties(uint32_t n, const uint32_t *_
   }
 }
 
-// Over the sorted census: each weight in quanta, for the prefix sums, and
-// each cell's run [start, end) of sorted indices (end stays 0 for a cell with
-// no packet).
+// Over the sorted census: each weight in sorted order (`ws`, read by the
+// later comb kernels in place of a gather through the sort's values) and in
+// quanta (`wq`, for the prefix sums), and each cell's run [start, end) of
+// sorted indices (end stays 0 for a cell with no packet).
 // The comb's unit of a sorted key: the cell, or with gbits > 0 (the
 // stratified comb) the (cell, group) bin, numbered cell * groups + group.
 struct CombUnits {
@@ -1340,12 +1341,14 @@ template <typename Real>
 __global__ __launch_bounds__(256) void k_comb_sorted(uint32_t n, const uint32_t *__restrict__ keys,
                                                      const uint32_t *__restrict__ vals, CombUnits cu,
                                                      CensusSoA<Real> cen, int32_t eq, Morton mo, MeshDims d,
-                                                     uint64_t *__restrict__ wq, uint32_t *__restrict__ start,
-                                                     uint32_t *__restrict__ end) {
+                                                     double *__restrict__ ws, uint64_t *__restrict__ wq,
+                                                     uint32_t *__restrict__ start, uint32_t *__restrict__ end) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t ukey = keys[i] >> cu.pbits;
-  wq[i] = fx::value_of(fx::limbs_of(cen.w[vals[i]], eq));
+  const double w = cen.w[vals[i]];
+  ws[i] = w;
+  wq[i] = fx::value_of(fx::limbs_of(w, eq));
   if (cu.dropped(keys[i], mo)) return;
   uint32_t c;
   const uint32_t u = cu.unit(keys[i], mo, d, c);
@@ -1669,12 +1672,10 @@ __device__ __forceinline__ uint64_t comb_teeth(double C, double Wc, double targe
 }
 
 // Output copies per sorted census packet (0 for the dropped weight-0 packets
-// at the end).  `cum` holds inclusive prefix sums of the weight limbs over the
-// sorted order; [start, end) is each cell's run.
-template <typename Real>
-__global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *__restrict__ keys,
-                                                    const uint32_t *__restrict__ vals, CombUnits cu,
-                                                    CensusSoA<Real> cen,
+// at the end).  `ws` holds the weights in sorted order, `cum` the inclusive
+// prefix sums of their quanta; [start, end) is each cell's run.
+__global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *__restrict__ keys, CombUnits cu,
+                                                    const double *__restrict__ ws,
                                                     const CombCell *__restrict__ cells,
                                                     const uint32_t *__restrict__ start,
                                                     const uint32_t *__restrict__ end,
@@ -1689,7 +1690,7 @@ __global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *
   uint32_t c;
   const uint32_t u = cu.unit(keys[i], mo, d, c);
   const CombCell cc = cells[u];
-  const double w = cen.w[vals[i]];
+  const double w = ws[i];
   uint32_t k = 1;
   if (cc.mode == 3u) {
     k = 0;  // a bin the stratified comb drops
@@ -1719,7 +1720,7 @@ __global__ __launch_bounds__(256) void k_comb_count(uint32_t n, const uint32_t *
 template <typename Real>
 __global__ __launch_bounds__(256) void k_comb_write(uint32_t n, const uint32_t *__restrict__ keys,
                                                     const uint32_t *__restrict__ vals, CombUnits cu,
-                                                    CensusSoA<Real> cen,
+                                                    CensusSoA<Real> cen, const double *__restrict__ ws,
                                                     const CombCell *__restrict__ cells,
                                                     const uint32_t *__restrict__ ncopy,
                                                     const uint64_t *__restrict__ offs, CensusSoA<Real> out,
@@ -1734,7 +1735,7 @@ __global__ __launch_bounds__(256) void k_comb_write(uint32_t n, const uint32_t *
     if (k) {
       const CombCell cc = cells[u];
       const uint32_t src = vals[i];
-      const double w = cen.w[src];
+      const double w = ws[i];
       double wout = w;
       if (cc.mode == 2u) {
         wout = cc.w_out;
@@ -1871,6 +1872,21 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
 __global__ void k_fill_f64(double *p, uint64_t n, double v) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// Zeroes up to kZeroRanges ranges of 32-bit words in one launch (blockIdx.y
+// picks the range, the x blocks stride over it): the step's tallies and
+// counters are cleared by one kernel instead of one per buffer.
+constexpr int kZeroRanges = 8;
+struct ZeroRanges {
+  uint32_t *p[kZeroRanges];
+  uint64_t n[kZeroRanges];  // words
+};
+__global__ __launch_bounds__(256) void k_zero_ranges(ZeroRanges r) {
+  uint32_t *__restrict__ p = r.p[blockIdx.y];
+  const uint64_t n = r.n[blockIdx.y];
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256u;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256u + threadIdx.x; i < n; i += stride) p[i] = 0u;
 }
 
 // Adds tally replicas 1..R-1 into replica 0, cell by cell in replica order
