@@ -231,7 +231,8 @@ int storm_gpu_queue_source_packets(storm_gpu_engine *eng, const storm_packet_rec
  * wavefront (storm_gpu_set_chunks_per_wave; each schedule keeps its own,
  * 2 for PASS and 8 for REFILL by default, 16 for REFILL in the FP64 engine;
  * in the mixed engine on polyhedral meshes 4 for REFILL, and 1 for PASS on
- * multigroup problems; a launch halves
+ * multigroup problems; 16 for REFILL, in 512-thread blocks, on Cartesian
+ * problems with DDMC; blocks are otherwise 256 threads; a launch halves
  * it while it would have fewer blocks than the device holds at once).  The engine starts in REFILL, except the mixed engine
  * on multigroup polyhedral problems (PASS), and watches: after 16 steps one
  * step runs in the other schedule, and the engine switches when that one
