@@ -398,6 +398,25 @@ def test_refill_schedule_matches_pass_schedule(make):
     assert a[2]["end_particles"] == b[2]["end_particles"]
 
 
+def test_ddmc_default_launch_matches_fixed_launch():
+    """A Cartesian problem with DDMC starts in 512-thread blocks refilling
+    from 16 chunks per wavefront; its results are those of a fixed 256-thread
+    pass launch, bit for bit."""
+    spec = C3(12)
+    assert spec.with_ddmc
+    with GpuEngine(spec, capi.MIXED) as e:
+        L = e.launch()
+        assert (L["block"], L["chunks"], L["schedule"]) == (512, 16, capi.SCHEDULE_REFILL), L
+    a = _run_tallies(spec, capi.MIXED, schedule=capi.SCHEDULE_PASS)
+    b = _run_tallies(spec, capi.MIXED)
+    assert b[2]["transport_block"] == 512
+    for k in ("T", "e", "erad", "census"):
+        assert np.array_equal(a[0][k], b[0][k]), k
+    for k in ("dep", "rdep", "pint", "esc", "segs"):
+        assert np.array_equal(a[1][k], b[1][k]), k
+    assert a[2]["segments"] == b[2]["segments"]
+
+
 @pytest.mark.parametrize("make", [C2, P.voronoi_slab], ids=["c2", "voronoi"])
 def test_fp64_refill_default_matches_pass(make):
     """The FP64 engine refills lanes by default; the pass schedule gives the
