@@ -617,7 +617,7 @@ struct Engine final : EngineBase {
     xs.ensure(N);
     nphot.ensure(N);
     nphot64.ensure(N + 1);
-    nphot_z.ensure(size_t(1) << morton.total);
+    nphot_z.ensure((size_t(1) << morton.total) + 1);
     offsets.ensure((size_t(1) << morton.total) + 1);
     err.ensure(1);
     // the stratified comb (a multigroup problem with imc.comb = 2) works on
@@ -1026,7 +1026,23 @@ struct Engine final : EngineBase {
     const MatDev md = matdev();
     const MeshDims d = dims();
     const Real s0 = static_cast<Real>(units::clight * dt);
-    zero_u32(err.p, 1);
+    // the step's tallies, counters and error flags start at zero (one launch)
+    {
+      ZeroBatch z;
+      zero_add(z, tally_u64.p, tally_u64.n * sizeof(unsigned long long));
+      if (precision == STORM_GPU_FP64) zero_add(z, tally_f64.p, tally_f64.n * sizeof(double));
+      zero_add(z, tal.segs, 2 * static_cast<size_t>(N) * sizeof(uint32_t));
+      zero_add(z, tal.eg, static_cast<size_t>(N) * tal.eg_groups * sizeof(unsigned long long));
+      zero_add(z, tal.eg_fp, static_cast<size_t>(N) * tal.eg_groups * sizeof(double));
+      zero_add(z, rep_u32.p, rep_u32.n * sizeof(uint32_t));
+      if (precision == STORM_GPU_FP64) zero_add(z, rep_f64.p, rep_f64.n * sizeof(double));
+      else zero_add(z, rep_u64.p, rep_u64.n * sizeof(unsigned long long));
+      zero_add(z, requant.p, requant.n * sizeof(unsigned long long));
+      zero_add(z, leaving.p, 5 * sizeof(unsigned long long));
+      zero_add(z, counters_u32.p, 2 * sizeof(uint32_t));
+      zero_add(z, err.p, sizeof(uint32_t));
+      zero_flush(z);
+    }
     eq_prev_ledger_ = eq_prev;
     sum_async(e_mat.p, N, dsc.p);  // material energy before the step, read by the ledger
     if (!inst.empty())
@@ -1140,10 +1156,10 @@ struct Engine final : EngineBase {
     // photon counts in the cells' Z order, so emission is laid out like the
     // combed census
     const uint32_t M = 1u << morton.total;
-    k_morton_counts<<<blocks_for(M, 256), 256, 0, stream>>>(M, morton, d, nphot.p, nphot_z.p);
+    k_morton_counts<<<blocks_for(M + 1, 256), 256, 0, stream>>>(M, morton, d, nphot.p, nphot_z.p);
     {
       auto q = [&](void *tp, size_t &bytes) {
-        HIPCHECK(rocprim::exclusive_scan(tp, bytes, nphot_z.p, offsets.p, uint64_t(0), static_cast<size_t>(M),
+        HIPCHECK(rocprim::exclusive_scan(tp, bytes, nphot_z.p, offsets.p, uint64_t(0), static_cast<size_t>(M) + 1,
                                          rocprim::plus<uint64_t>(), stream));
       };
       ensure_temp(q);
@@ -1152,11 +1168,10 @@ struct Engine final : EngineBase {
       HIPCHECK(hipMemcpyAsync(inst_shift_h, d_inst_shift.p, inst.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
                               stream));
     const unsigned long long *hs =
-        read_scalars({{offsets.p + (M - 1), 8u}, {nphot_z.p + (M - 1), 4u}, {qexp.p, 4u}, {qexp.p + 1, 4u}});
-    const uint64_t n_emit = hs[0] + hs[1];
-    const int32_t eq = static_cast<int32_t>(static_cast<uint32_t>(hs[2]));
-    const int32_t et = static_cast<int32_t>(static_cast<uint32_t>(hs[3]));
-    k_fill_u64<<<1, 64, 0, stream>>>(reinterpret_cast<unsigned long long *>(offsets.p + M), 1, n_emit);
+        read_scalars({{offsets.p + M, 8u}, {qexp.p, 4u}, {qexp.p + 1, 4u}});
+    const uint64_t n_emit = hs[0];
+    const int32_t eq = static_cast<int32_t>(static_cast<uint32_t>(hs[1]));
+    const int32_t et = static_cast<int32_t>(static_cast<uint32_t>(hs[2]));
     counters.energy_quantum_exp = eq;
     if (inst.empty()) {
       for (uint32_t f = 0; f < nfaces; ++f) wface_e[f] = fx::quantize_down(wface_e[f], eq);
@@ -1204,22 +1219,6 @@ struct Engine final : EngineBase {
     }
     counters.start_particles = n_total;
 
-    // reset tallies (one launch)
-    {
-      ZeroBatch z;
-      zero_add(z, tally_u64.p, tally_u64.n * sizeof(unsigned long long));
-      if (precision == STORM_GPU_FP64) zero_add(z, tally_f64.p, tally_f64.n * sizeof(double));
-      zero_add(z, tal.segs, 2 * static_cast<size_t>(N) * sizeof(uint32_t));
-      zero_add(z, tal.eg, static_cast<size_t>(N) * tal.eg_groups * sizeof(unsigned long long));
-      zero_add(z, tal.eg_fp, static_cast<size_t>(N) * tal.eg_groups * sizeof(double));
-      zero_add(z, rep_u32.p, rep_u32.n * sizeof(uint32_t));
-      if (precision == STORM_GPU_FP64) zero_add(z, rep_f64.p, rep_f64.n * sizeof(double));
-      else zero_add(z, rep_u64.p, rep_u64.n * sizeof(unsigned long long));
-      zero_add(z, requant.p, requant.n * sizeof(unsigned long long));
-      zero_add(z, leaving.p, 5 * sizeof(unsigned long long));
-      zero_add(z, counters_u32.p, 2 * sizeof(uint32_t));
-      zero_flush(z);
-    }
 
     // packets: [census | emission | boundary]
     pa.ensure(n_total);

@@ -267,7 +267,11 @@ __global__ __launch_bounds__(256) void k_morton_counts(uint32_t M, Morton mo, Me
                                                        const uint32_t *__restrict__ nphot,
                                                        uint32_t *__restrict__ out) {
   const uint32_t code = blockIdx.x * blockDim.x + threadIdx.x;
-  if (code >= M) return;
+  if (code > M) return;
+  if (code == M) {  // a zero past the last code: the exclusive scan of M + 1 counts ends in the total
+    out[M] = 0u;
+    return;
+  }
   uint32_t i, j, k;
   morton_ijk(code, mo, i, j, k);
   out[code] = (i < d.nx && j < d.ny && k < d.nz) ? nphot[(i * d.ny + j) * d.nz + k] : 0u;
